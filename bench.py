@@ -1,0 +1,185 @@
+"""Benchmark of the field engine on one MI355X.
+
+Workload (default): the plain FP64 FMM on a water-like box -- 66,667 rigid
+3-site waters (200,001 sites) on a jittered grid at liquid density, charges
+plus random dipoles (dim 4), self + 2 intramolecular exclusions per site,
+THOLE damping, theta 0.5, ncrit 64, expansion order 6, field order 2.
+Inputs are synthetic (seeded) and resident in HBM before the timed region.
+
+A step is one field_fmm call on the device-resident inputs (tree, lists,
+P2M, M2M, P2P, M2L, L2L, L2P).  Prints one JSON line with throughput in
+site-field evaluations per second, the roofline of the dominant kernel (P2P)
+and the reference CPU path timed on this host on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--no-cpu] [--no-error]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector = FP64 matrix peak
+HBM_PEAK_GBS = 8000.0
+
+
+def c_T(K):
+    """Algorithmic FLOPs of the order-K tensor recursion (FMA = 2)."""
+    return 9 + 2 * (K + 1) + 3 * sum((k + 1) * (k + 2) // 2 * (K - k + 1) for k in range(1, K + 1))
+
+
+def p2p_flops_per_pair(m, F, damp, sets=1):
+    K = m + F
+    ncomp = lambda k: (k + 1) * (k + 2) * (k + 3) // 6  # noqa: E731
+    d = 0 if damp == "" else (2 + 4 * (K + 1) if damp == "ERF" else 1 + 4 * (K + 1))
+    return c_T(K) + d + sets * 2 * ncomp(m) * ncomp(F)
+
+
+def col(a, dev, dtype=None):
+    import torch
+    t = torch.as_tensor(np.ascontiguousarray(np.asarray(a).T), device=dev)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.T
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_baseline(seconds_hint=True):
+    """The reference Fortran (compiled from its own sources into oracle/_ref)
+    on one host core, on a 1/10-scale sample of the same workload."""
+    import oracle
+    from fmm_amd import synth
+    if not oracle.have_ref():
+        return None
+    c = synth.config(1, scale=0.1)
+    n = c["coordinates"].shape[0]
+    t0 = time.perf_counter()
+    oracle.field_fmm(c["coordinates"], c["multipoles"], c["exclusions"], c["theta"], c["ncrit"],
+                     c["expansion_order"], c["field_order"], c["damp_type"], c["damping_factors"], impl="ref")
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "site-field evals/s", "cores": 1, "kind": "reference",
+            "sample": f"reference field_fmm (flang -O1 -g, serial) on the same recipe at 1/10 scale: {n} sites, "
+                      f"THOLE, theta 0.5, ncrit 64, p 6, field order 2; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fmm_amd import synth
+    from fmm_amd.octree import Context
+
+    dev = torch.device("cuda:0")
+    c = synth.config(args.config)
+    n = c["coordinates"].shape[0]
+    F = c["field_order"]
+    nf = synth.ncomp(F)
+    dim = c["multipoles"].shape[1]
+    m = {1: 0, 4: 1, 10: 2, 20: 3}[dim]
+    X = col(c["coordinates"], dev)
+    M = col(c["multipoles"], dev)
+    E = col(c["exclusions"], dev, torch.int32)
+    df = torch.as_tensor(c["damping_factors"], device=dev) if c["damp_type"] else None
+    out = col(np.zeros((n, nf)), dev)
+    ctx = Context(0)
+    ctx.set_timing(True)
+
+    def step():
+        out.zero_()
+        torch.cuda.current_stream().synchronize()
+        ctx.field_fmm_device(X, M, E, c["theta"], c["ncrit"], c["expansion_order"], F, out, c["damp_type"], df)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        phases.append(ctx.phase_times())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    stats = ctx.stats()
+    phase_avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    value = n * 1 / (ms_per_step / 1e3)
+
+    pairs = stats["p2p_site_pairs"]
+    fpp = p2p_flops_per_pair(m, F, c["damp_type"])
+    p2p_ms = phase_avg["p2p"]
+    achieved = pairs * fpp / (p2p_ms / 1e3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "kernel": "k_near (P2P, FP64 VALU)", "kernel_ms": p2p_ms, "flop_per_pair": fpp,
+                "site_pairs": pairs}
+
+    errors = None
+    if not args.no_error:
+        field = out.cpu().numpy()
+        d = col(np.zeros((n, nf)), dev)
+        ctx.field_direct_device(X, M, E, F, d, c["damp_type"], df)
+        direct = d.cpu().numpy()
+        d.zero_()
+        torch.cuda.synchronize()
+        ctx.field_direct_device(X, M, E, F, d, "", None)
+        direct_u = d.cpu().numpy()
+        d.zero_()
+        torch.cuda.synchronize()
+        ctx.field_fmm_device(X, M, E, c["theta"], c["ncrit"], c["expansion_order"], F, d, "", None)
+        fmm_u = d.cpu().numpy()
+        errors = {
+            "field_block_vs_fp64_direct": rel(field[:, 1:4], direct[:, 1:4]),
+            "potential_block_vs_fp64_direct": rel(field[:, :1], direct[:, :1]),
+            "field_block_truncation": rel(fmm_u[:, 1:4], direct_u[:, 1:4]),
+            "note": "truncation = undamped FMM vs undamped direct (damping applies to near-field pairs only, "
+                    "as in the reference, so the damped FMM differs from the damped direct sum by the damping "
+                    "of far pairs)",
+        }
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "site-field evals/s at <=1e-6 rel err vs FP64 direct; % of FP64/FP32 peak",
+        "value": value,
+        "unit": "site-field evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": c["name"], "sites": n, "sets": 1, "multipole_dim": dim, "field_order": F,
+                   "theta": c["theta"], "ncrit": c["ncrit"], "expansion_order": c["expansion_order"],
+                   "damp_type": c["damp_type"], "exclusions_per_site": int(c["exclusions"].shape[1])},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "phases_ms": phase_avg,
+        "tree": stats,
+        "errors": errors,
+    }
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,891 @@
+// Host runtime and C ABI of the field engine (include/fmm_hip.h).
+//
+// Pipeline of one field_fmm call (src/fmm.F90:1354-1400), all on the
+// context's stream:
+//   tree     centroid -> keys -> sort -> levels -> scan -> nodes -> links
+//            -> leaves -> level lists -> leaf member order      (2 host syncs)
+//   lists    level-synchronous dual traversal, sorted into CSR  (1 sync/level)
+//   gather   sorted SoA site arrays, Taylor-scaled multipoles
+//   P2M, M2M (level by level), P2P, M2L, L2L (level by level), L2P + scatter
+// Device buffers are pooled per context and only grow.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fmm_hip.h"
+#include "common.h"
+#include "kernels.h"
+
+using namespace fmmh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct Tables {
+    int p = -1, m = -1;
+    int* ibuf = nullptr;
+    double* dbuf = nullptr;
+    ShiftTab p2m{}, m2m{}, l2l{};
+    const int* mono_parent = nullptr;
+    const int* mono_axis = nullptr;
+    const int* slot_st = nullptr;
+    const int* xyz = nullptr;
+    const int* level_base = nullptr;
+    const double* cs = nullptr;       // Taylor factors for the p expansion
+    const double* scale = nullptr;    // Taylor factors for the dim source components
+    int nsrc = 0;
+};
+
+}  // namespace
+
+struct fmmh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::string, DevBuf> bufs;
+    int* h_pinned = nullptr;          // small pinned readback area
+    std::map<long long, Tables> tables;
+    bool timing = false;
+    std::vector<std::pair<std::string, hipEvent_t>> marks;
+    std::vector<hipEvent_t> event_pool;
+    size_t event_used = 0;
+    std::vector<std::pair<std::string, double>> phase_ms;
+    int64_t stats[6] = {0, 0, 0, 0, 0, 0};
+    // last tree (for export)
+    int num_nodes = 0, num_leaves = 0, max_depth = 0, n_m2l = 0, n_p2p = 0, n_sites = 0;
+    std::vector<int> level_off;       // host copy, size KEY_LEVELS + 3
+
+    template <class T>
+    T* buf(const std::string& name, size_t count) {
+        DevBuf& b = bufs[name];
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (bytes > b.cap) {
+            if (b.p) FMMH_HIP(hipFree(b.p));
+            b.p = nullptr;
+            size_t nb = std::max(bytes, b.cap + b.cap / 2);
+            FMMH_HIP(hipMalloc(&b.p, nb));
+            b.cap = nb;
+        }
+        return (T*)b.p;
+    }
+
+    void mark(const char* name) {
+        if (!timing) return;
+        if (event_used == event_pool.size()) {
+            hipEvent_t e;
+            FMMH_HIP(hipEventCreate(&e));
+            event_pool.push_back(e);
+        }
+        hipEvent_t e = event_pool[event_used++];
+        FMMH_HIP(hipEventRecord(e, stream));
+        marks.emplace_back(name, e);
+    }
+    void begin_timing() {
+        marks.clear();
+        event_used = 0;
+        phase_ms.clear();
+        mark("start");
+    }
+    void end_timing() {
+        if (!timing) return;
+        FMMH_HIP(hipStreamSynchronize(stream));
+        for (size_t i = 1; i < marks.size(); ++i) {
+            float ms = 0.f;
+            FMMH_HIP(hipEventElapsedTime(&ms, marks[i - 1].second, marks[i].second));
+            phase_ms.emplace_back(marks[i].first, (double)ms);
+        }
+    }
+    void sync() { FMMH_HIP(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+
+std::mutex g_default_mutex;
+fmmh_context* g_default = nullptr;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        f();
+        return FMMH_OK;
+    } catch (const fmmh::Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::exception& e) {
+        return fail(FMMH_ERR_INTERNAL, e.what());
+    }
+}
+
+fmmh_context* make_context(int device) {
+    fmmh_context* c = new fmmh_context;
+    if (device < 0) FMMH_HIP(hipGetDevice(&device));
+    c->device = device;
+    FMMH_HIP(hipSetDevice(device));
+    FMMH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    FMMH_HIP(hipHostMalloc((void**)&c->h_pinned, 4096, hipHostMallocDefault));
+    return c;
+}
+
+fmmh_context* resolve(fmmh_context* ctx) {
+    if (ctx) {
+        FMMH_HIP(hipSetDevice(ctx->device));
+        return ctx;
+    }
+    std::lock_guard<std::mutex> lk(g_default_mutex);
+    if (!g_default) g_default = make_context(-1);
+    FMMH_HIP(hipSetDevice(g_default->device));
+    return g_default;
+}
+
+int order_from_dim(int dim) { return (int)std::lround(std::pow((double)dim * 6.0, 1.0 / 3.0)) - 2; }
+
+double factorial(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= i;
+    return f;
+}
+
+struct XYZ { int x, y, z; };
+std::vector<XYZ> packed_components(int K) {
+    std::vector<XYZ> v;
+    for (int k = 0; k <= K; ++k)
+        for (int x = k; x >= 0; --x)
+            for (int y = k - x; y >= 0; --y) v.push_back({x, y, k - x - y});
+    return v;
+}
+
+double taylor_factor(const XYZ& s) {
+    int k = s.x + s.y + s.z;
+    return ((k + 1) % 2 ? -1.0 : 1.0) / (factorial(s.x) * factorial(s.y) * factorial(s.z));
+}
+
+// Host-built operator tables for expansion order p and source multipole order m.
+Tables& get_tables(fmmh_context* c, int p, int m) {
+    long long key = (long long)p * 64 + m;
+    auto it = c->tables.find(key);
+    if (it != c->tables.end()) return it->second;
+    const int nm = ncomp(p);
+    const int K = 2 * p;
+    std::vector<XYZ> comp = packed_components(std::max(K, m));
+    auto idx = [](int x, int y, int z) { return xyz2idx(x, y, z); };
+    std::vector<int> ints;
+    std::vector<double> dbl;
+    struct Seg { size_t i0, d0; };
+    auto shift_table = [&](bool up, int tmax_order, int smax_order, std::vector<int>& off, std::vector<int>& s,
+                           std::vector<int>& e, std::vector<double>& coef) {
+        int nt = ncomp(tmax_order);
+        off.assign(nt + 1, 0);
+        for (int t = 0; t < nt; ++t) {
+            off[t] = (int)s.size();
+            const XYZ& T = comp[t];
+            for (int si = 0; si < ncomp(smax_order); ++si) {
+                const XYZ& S = comp[si];
+                if (up) {
+                    if (S.x > T.x || S.y > T.y || S.z > T.z) continue;
+                    s.push_back(si);
+                    e.push_back(idx(T.x - S.x, T.y - S.y, T.z - S.z));
+                    auto bin = [](int n, int k) { return factorial(n) / (factorial(k) * factorial(n - k)); };
+                    coef.push_back(bin(T.x, S.x) * bin(T.y, S.y) * bin(T.z, S.z));
+                } else {
+                    if (S.x < T.x || S.y < T.y || S.z < T.z) continue;
+                    s.push_back(si);
+                    e.push_back(idx(S.x - T.x, S.y - T.y, S.z - T.z));
+                    coef.push_back(1.0 / (factorial(S.x - T.x) * factorial(S.y - T.y) * factorial(S.z - T.z)));
+                }
+            }
+        }
+        off[nt] = (int)s.size();
+    };
+    std::vector<int> p2m_off, p2m_s, p2m_e, m2m_off, m2m_s, m2m_e, l2l_off, l2l_s, l2l_e;
+    std::vector<double> p2m_c, m2m_c, l2l_c;
+    int msrc = std::min(m, p);
+    shift_table(true, p, msrc, p2m_off, p2m_s, p2m_e, p2m_c);
+    shift_table(true, p, p, m2m_off, m2m_s, m2m_e, m2m_c);
+    shift_table(false, p, p, l2l_off, l2l_s, l2l_e, l2l_c);
+    std::vector<int> mono_parent(nm), mono_axis(nm);
+    for (int e = 0; e < nm; ++e) {
+        const XYZ& E = comp[e];
+        if (e == 0) { mono_parent[e] = 0; mono_axis[e] = 0; }
+        else if (E.x > 0) { mono_parent[e] = idx(E.x - 1, E.y, E.z); mono_axis[e] = 0; }
+        else if (E.y > 0) { mono_parent[e] = idx(E.x, E.y - 1, E.z); mono_axis[e] = 1; }
+        else { mono_parent[e] = idx(E.x, E.y, E.z - 1); mono_axis[e] = 2; }
+    }
+    std::vector<int> level_base(K + 2, 0);
+    for (int k = 0; k <= K; ++k) level_base[k + 1] = level_base[k] + ndeg(k) * (K - k + 1);
+    std::vector<int> xyz(ncomp(K));
+    for (int i = 0; i < ncomp(K); ++i) xyz[i] = comp[i].x | (comp[i].y << 8) | (comp[i].z << 16);
+    std::vector<int> slot_st((size_t)nm * nm);
+    for (int s = 0; s < nm; ++s)
+        for (int t = 0; t < nm; ++t) {
+            const XYZ& S = comp[s];
+            const XYZ& T = comp[t];
+            int k = S.x + S.y + S.z + T.x + T.y + T.z;
+            int i = idx(S.x + T.x, S.y + T.y, S.z + T.z);
+            slot_st[(size_t)s * nm + t] = level_base[k] + (i - ncomp(k - 1));
+        }
+    std::vector<double> cs(nm), scale(ncomp(m));
+    for (int s = 0; s < nm; ++s) cs[s] = taylor_factor(comp[s]);
+    for (int s = 0; s < ncomp(m); ++s) scale[s] = taylor_factor(comp[s]);
+
+    // pack into one int and one double buffer
+    std::vector<std::pair<std::vector<int>*, const int**>> iparts;
+    Tables t;
+    t.p = p; t.m = m; t.nsrc = ncomp(msrc);
+    std::vector<std::vector<int>*> ilist = {&p2m_off, &p2m_s, &p2m_e, &m2m_off, &m2m_s, &m2m_e, &l2l_off, &l2l_s,
+                                            &l2l_e, &mono_parent, &mono_axis, &slot_st, &xyz, &level_base};
+    std::vector<std::vector<double>*> dlist = {&p2m_c, &m2m_c, &l2l_c, &cs, &scale};
+    std::vector<size_t> ioff, doff;
+    size_t ni = 0, nd = 0;
+    for (auto* v : ilist) { ioff.push_back(ni); ni += v->size(); }
+    for (auto* v : dlist) { doff.push_back(nd); nd += v->size(); }
+    std::vector<int> hi(ni);
+    std::vector<double> hd(nd);
+    for (size_t k = 0; k < ilist.size(); ++k) std::copy(ilist[k]->begin(), ilist[k]->end(), hi.begin() + ioff[k]);
+    for (size_t k = 0; k < dlist.size(); ++k) std::copy(dlist[k]->begin(), dlist[k]->end(), hd.begin() + doff[k]);
+    FMMH_HIP(hipMalloc((void**)&t.ibuf, std::max<size_t>(ni, 1) * sizeof(int)));
+    FMMH_HIP(hipMalloc((void**)&t.dbuf, std::max<size_t>(nd, 1) * sizeof(double)));
+    FMMH_HIP(hipMemcpy(t.ibuf, hi.data(), ni * sizeof(int), hipMemcpyHostToDevice));
+    FMMH_HIP(hipMemcpy(t.dbuf, hd.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+    const int* I = t.ibuf;
+    const double* D = t.dbuf;
+    t.p2m = ShiftTab{I + ioff[0], I + ioff[1], I + ioff[2], D + doff[0]};
+    t.m2m = ShiftTab{I + ioff[3], I + ioff[4], I + ioff[5], D + doff[1]};
+    t.l2l = ShiftTab{I + ioff[6], I + ioff[7], I + ioff[8], D + doff[2]};
+    t.mono_parent = I + ioff[9];
+    t.mono_axis = I + ioff[10];
+    t.slot_st = I + ioff[11];
+    t.xyz = I + ioff[12];
+    t.level_base = I + ioff[13];
+    t.cs = D + doff[3];
+    t.scale = D + doff[4];
+    return c->tables.emplace(key, t).first->second;
+}
+
+__global__ void k_iota(int* a, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+__global__ void k_direct_blocks(int* start, int* count, int* list_off, int* list_src, int nb, int n) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) {
+        start[b] = 64 * b;
+        count[b] = min(64, n - 64 * b);
+        list_off[b] = b;
+        list_src[b] = 0;
+    }
+    if (b == 0) list_off[nb] = nb;
+}
+
+__global__ void k_pair_count(const int* off, const int* src, const int* leaf_node, const int* count, int nl,
+                             unsigned long long* total) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nl) return;
+    int node = leaf_node[l];
+    unsigned long long s = 0;
+    for (int e = off[node]; e < off[node + 1]; ++e) s += (unsigned long long)count[src[e]];
+    atomicAdd(total, s * (unsigned long long)count[node]);
+}
+
+__global__ void k_fill_i32(int* a, int v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+struct Call {
+    const double* X; int64_t ldx;
+    const double* M; int64_t ldm;
+    int n, dim;
+    const int* E; int64_t lde; int nexcl;
+    int F; double* field; int64_t ldf;
+    int damp; const double* df;
+};
+
+void check_common(const Call& c, int64_t n_damping) {
+    if (c.n < 1) throw Error(FMMH_ERR_ARG, "n must be >= 1");
+    if (c.ldx < c.n || c.ldm < c.n || c.ldf < c.n || (c.nexcl > 0 && c.lde < c.n))
+        throw Error(FMMH_ERR_ARG, "leading dimension smaller than n");
+    if (c.nexcl < 0) throw Error(FMMH_ERR_ARG, "nexcl must be >= 0");
+    if (c.damp < 0 || c.damp > 3) throw Error(FMMH_ERR_DAMP, "Unknown damp type code");
+    if (c.damp != FMMH_DAMP_NONE && (n_damping < c.n || c.df == nullptr))
+        throw Error(FMMH_ERR_ARG, "damping_factors must hold n values when damping is on");
+    int m = order_from_dim(c.dim);
+    if (m < 0 || ncomp(m) != c.dim) throw Error(FMMH_ERR_ARG, "dim is not (m+1)(m+2)(m+3)/6 for any order m");
+    if (!near_supported(m, c.F))
+        throw Error(FMMH_ERR_UNSUPPORTED, "multipole order " + std::to_string(m) + " / field order " +
+                                              std::to_string(c.F) + " outside the compiled range (<= " +
+                                              std::to_string(NEAR_MMAX) + " / <= " + std::to_string(NEAR_FMAX) + ")");
+}
+
+void check_exclusions_host(const Call& c) {
+    for (int q = 0; q < c.nexcl; ++q)
+        for (int i = 0; i < c.n; ++i) {
+            int v = c.E[(int64_t)q * c.lde + i];
+            if (v < 0 || v > c.n)
+                throw Error(FMMH_ERR_ARG, "exclusions(" + std::to_string(i + 1) + "," + std::to_string(q + 1) +
+                                              ") = " + std::to_string(v) + " is outside 0..n");
+        }
+}
+
+// ---------------------------------------------------------------- direct
+void run_direct(fmmh_context* c, const Call& k) {
+    hipStream_t s = c->stream;
+    const int n = k.n;
+    const int m = order_from_dim(k.dim);
+    Tables& tb = get_tables(c, 0, m);
+    const int nb = nblk(n, 64);
+    int* start = c->buf<int>("d_start", nb);
+    int* count = c->buf<int>("d_count", nb);
+    int* loff = c->buf<int>("d_loff", nb + 1);
+    int* lsrc = c->buf<int>("d_lsrc", nb);
+    int* s_start = c->buf<int>("d_sstart", 1);
+    int* s_count = c->buf<int>("d_scount", 1);
+    int* iota = c->buf<int>("iota", std::max(n, nb));
+    double* ms = c->buf<double>("d_ms", (size_t)n * k.dim);
+    double* ms_raw = c->buf<double>("d_ms_raw", (size_t)n * k.dim);
+    double* xs = c->buf<double>("d_xs", n);
+    double* ys = c->buf<double>("d_ys", n);
+    double* zs = c->buf<double>("d_zs", n);
+    double* dfs = c->buf<double>("d_dfs", n);
+    int* excl = c->buf<int>("d_excl", (size_t)n * std::max(k.nexcl, 1));
+    hipLaunchKernelGGL(k_direct_blocks, dim3(nblk(nb, 256)), dim3(256), 0, s, start, count, loff, lsrc, nb, n);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(std::max(n, nb), 256)), dim3(256), 0, s, iota, std::max(n, nb));
+    int zero_n[2] = {0, n};
+    FMMH_HIP(hipMemcpyAsync(s_start, &zero_n[0], sizeof(int), hipMemcpyHostToDevice, s));
+    FMMH_HIP(hipMemcpyAsync(s_count, &zero_n[1], sizeof(int), hipMemcpyHostToDevice, s));
+    SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, iota, n,
+               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale};
+    launch_gather(g, s);
+    c->mark("gather");
+    NearArgs a{iota, start, count, loff, lsrc, s_start, s_count, xs, ys, zs, ms, n, dfs, iota, excl, n, k.nexcl,
+               k.field, k.ldf, 1};
+    launch_near(a, nb, m, k.F, k.damp, s);
+    c->mark("direct");
+    FMMH_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- tree
+struct TreeOut {
+    TreeDev t{};
+    int num_nodes = 0, num_leaves = 0, max_depth = 0;
+    std::vector<int> level_off;   // KEY_LEVELS + 3 entries; level L nodes at [off[L], off[L+1])
+};
+
+void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit, TreeOut& out) {
+    hipStream_t s = c->stream;
+    TreeDev& t = out.t;
+    unsigned long long* keys_in = c->buf<unsigned long long>("t_keys_in", n);
+    int* idx_in = c->buf<int>("t_idx_in", n);
+    t.keys = c->buf<unsigned long long>("t_keys", n);
+    t.perm = c->buf<int>("t_perm", n);
+    t.lmin = c->buf<int>("t_lmin", n);
+    t.lam = c->buf<int>("t_lam", n);
+    t.cnt = c->buf<int>("t_cnt", n);
+    t.off = c->buf<int>("t_off", n);
+    t.w = c->buf<int>("t_w", n);
+    t.root = c->buf<double>("t_root", 8);
+    t.maxabs = c->buf<unsigned long long>("t_maxabs", 3);
+    t.info = c->buf<int>("t_info", 64);
+    t.level_fill = c->buf<int>("t_level_fill", KEY_LEVELS + 3);
+    FMMH_HIP(hipMemsetAsync(t.info, 0, 64 * sizeof(int), s));
+    launch_centroid(X, ldx, n, t, s);
+    launch_keys(X, ldx, n, t, keys_in, idx_in, s);
+    size_t tb = std::max(sort_pairs_u64_temp(n), exscan_i32_temp(n));
+    void* temp = c->buf<char>("t_temp", tb);
+    sort_pairs_u64(temp, tb, keys_in, t.keys, idx_in, t.perm, n, 63, s);
+    launch_site_levels(n, ncrit, t, s);
+    exscan_i32(temp, tb, t.cnt, t.off, n, s);
+    int* hp = c->h_pinned;
+    FMMH_HIP(hipMemcpyAsync(hp, t.off + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(hp + 1, t.cnt + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(hp + 2, t.info + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->sync();
+    if (hp[2]) throw Error(FMMH_ERR_TREE, "octree deeper than 21 levels: >= ncrit sites share one cell");
+    const int nn = hp[0] + hp[1];
+    out.num_nodes = nn;
+    t.level = c->buf<int>("n_level", nn);
+    t.start = c->buf<int>("n_start", nn);
+    t.count = c->buf<int>("n_count", nn);
+    t.parent = c->buf<int>("n_parent", nn);
+    t.is_leaf = c->buf<int>("n_is_leaf", nn);
+    t.octant = c->buf<int>("n_octant", nn);
+    t.children = c->buf<int>("n_children", (size_t)nn * 8);
+    t.cx = c->buf<double>("n_cx", nn);
+    t.cy = c->buf<double>("n_cy", nn);
+    t.cz = c->buf<double>("n_cz", nn);
+    t.r = c->buf<double>("n_r", nn);
+    t.rmax = c->buf<double>("n_rmax", nn);
+    t.leaf_id = c->buf<int>("n_leaf_id", nn);
+    t.leaf_node = c->buf<int>("n_leaf_node", nn);
+    t.level_nodes = c->buf<int>("n_level_nodes", nn);
+    int* leaf_scan = c->buf<int>("n_leaf_scan", nn);
+    FMMH_HIP(hipMemsetAsync(t.children, 0xff, (size_t)nn * 8 * sizeof(int), s));
+    launch_emit_nodes(n, t, s);
+    launch_node_links(nn, t, s);
+    size_t tb2 = exscan_i32_temp(nn);
+    void* temp2 = c->buf<char>("t_temp2", tb2);
+    exscan_i32(temp2, tb2, t.is_leaf, leaf_scan, nn, s);
+    FMMH_HIP(hipMemcpyAsync(hp, t.info, (3 + KEY_LEVELS + 2) * sizeof(int), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(hp + 40, leaf_scan + (nn - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(hp + 41, t.is_leaf + (nn - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    c->sync();
+    out.max_depth = hp[2];
+    out.num_leaves = hp[40] + hp[41];
+    out.level_off.assign(KEY_LEVELS + 3, 0);
+    for (int L = 1; L <= KEY_LEVELS + 1; ++L) out.level_off[L + 1] = out.level_off[L] + hp[3 + L];
+    out.level_off[0] = 0;
+    int* level_off_d = c->buf<int>("t_level_off", KEY_LEVELS + 3);
+    std::memcpy(hp + 64, out.level_off.data(), (KEY_LEVELS + 3) * sizeof(int));
+    FMMH_HIP(hipMemcpyAsync(level_off_d, hp + 64, (KEY_LEVELS + 3) * sizeof(int), hipMemcpyHostToDevice, s));
+    FMMH_HIP(hipMemsetAsync(t.level_fill, 0, (KEY_LEVELS + 3) * sizeof(int), s));
+    launch_leaf_index(nn, leaf_scan, t, s);
+    launch_level_scatter(nn, level_off_d, t, s);
+    launch_leaf_sort(out.num_leaves, ncrit, t, s);
+    // the pinned area is reused by the next readback; the level offsets copy must land first
+    c->sync();
+}
+
+struct ListsOut {
+    int* m2l_off = nullptr; int* m2l_src = nullptr; int n_m2l = 0;
+    int* p2p_off = nullptr; int* p2p_src = nullptr; int n_p2p = 0;
+};
+
+void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
+    hipStream_t s = c->stream;
+    const int nn = tr.num_nodes;
+    int cap_f = std::max(4096, 16 * nn);
+    int cap_m = std::max(4096, 64 * nn);
+    int cap_p = std::max(4096, 32 * nn);
+    int* hp = c->h_pinned;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        int2* fa = c->buf<int2>("l_front_a", cap_f);
+        int2* fb = c->buf<int2>("l_front_b", cap_f);
+        unsigned long long* m2l = c->buf<unsigned long long>("l_m2l", cap_m);
+        unsigned long long* p2p = c->buf<unsigned long long>("l_p2p", cap_p);
+        int* counters = c->buf<int>("l_counters", 8);   // n_a, n_b, n_m2l, n_p2p, overflow
+        int2 root = make_int2(0, 0);
+        int init[5] = {1, 0, 0, 0, 0};
+        std::memcpy(hp + 100, init, sizeof(init));
+        FMMH_HIP(hipMemcpyAsync(counters, hp + 100, 5 * sizeof(int), hipMemcpyHostToDevice, s));
+        std::memcpy(hp + 110, &root, sizeof(root));
+        FMMH_HIP(hipMemcpyAsync(fa, hp + 110, sizeof(int2), hipMemcpyHostToDevice, s));
+        int n_in = 1;
+        int cur = 0;
+        bool overflow = false;
+        while (n_in > 0) {
+            int2* fin = cur == 0 ? fa : fb;
+            int2* fout = cur == 0 ? fb : fa;
+            int* cin = counters + cur;
+            int* cout = counters + (1 - cur);
+            FMMH_HIP(hipMemsetAsync(cout, 0, sizeof(int), s));
+            TraverseArgs a{fin, cin, fout, cout, cap_f, m2l, counters + 2, cap_m, p2p, counters + 3, cap_p,
+                           counters + 4, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf, tr.t.children, theta};
+            launch_traverse(a, std::min(nblk(n_in, 256), 4096), s);
+            FMMH_HIP(hipMemcpyAsync(hp, counters, 5 * sizeof(int), hipMemcpyDeviceToHost, s));
+            c->sync();
+            if (hp[4]) { overflow = true; break; }
+            n_in = hp[1 - cur];
+            cur = 1 - cur;
+        }
+        if (overflow) {
+            if (hp[4] & 1) cap_m *= 2;
+            if (hp[4] & 2) cap_p *= 2;
+            if (hp[4] & 4) cap_f *= 2;
+            continue;
+        }
+        L.n_m2l = hp[2];
+        L.n_p2p = hp[3];
+        int bits = 32;
+        while ((1ll << (bits - 32)) < nn) ++bits;
+        size_t tb = std::max(sort_keys_u64_temp(std::max(L.n_m2l, 1)), sort_keys_u64_temp(std::max(L.n_p2p, 1)));
+        void* temp = c->buf<char>("l_temp", tb);
+        unsigned long long* sorted = c->buf<unsigned long long>("l_sorted", std::max(L.n_m2l, L.n_p2p));
+        L.m2l_off = c->buf<int>("l_m2l_off", nn + 1);
+        L.m2l_src = c->buf<int>("l_m2l_src", L.n_m2l);
+        L.p2p_off = c->buf<int>("l_p2p_off", nn + 1);
+        L.p2p_src = c->buf<int>("l_p2p_src", L.n_p2p);
+        if (L.n_m2l > 0) sort_keys_u64(temp, tb, m2l, sorted, L.n_m2l, bits, s);
+        launch_csr(sorted, L.n_m2l, nn, L.m2l_off, L.m2l_src, s);
+        if (L.n_p2p > 0) sort_keys_u64(temp, tb, p2p, sorted, L.n_p2p, bits, s);
+        launch_csr(sorted, L.n_p2p, nn, L.p2p_off, L.p2p_src, s);
+        return;
+    }
+    throw Error(FMMH_ERR_INTERNAL, "interaction list capacity did not converge");
+}
+
+// ---------------------------------------------------------------- field_fmm
+void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
+    hipStream_t s = c->stream;
+    const int n = k.n;
+    const int m = order_from_dim(k.dim);
+    const int nf = ncomp(k.F);
+    const int nm = ncomp(p);
+    TreeOut tr;
+    build_tree(c, k.X, k.ldx, n, ncrit, tr);
+    c->mark("tree");
+    ListsOut L;
+    build_lists(c, tr, theta, L);
+    c->mark("lists");
+    const int nn = tr.num_nodes, nl = tr.num_leaves;
+    c->num_nodes = nn; c->num_leaves = nl; c->max_depth = tr.max_depth;
+    c->n_m2l = L.n_m2l; c->n_p2p = L.n_p2p; c->n_sites = n;
+    c->level_off = tr.level_off;
+    Tables& tb = get_tables(c, p, m);
+    double* xs = c->buf<double>("s_xs", n);
+    double* ys = c->buf<double>("s_ys", n);
+    double* zs = c->buf<double>("s_zs", n);
+    double* ms_raw = c->buf<double>("s_ms_raw", (size_t)n * k.dim);
+    double* ms = c->buf<double>("s_ms", (size_t)n * k.dim);
+    double* dfs = c->buf<double>("s_dfs", n);
+    int* excl = c->buf<int>("s_excl", (size_t)n * std::max(k.nexcl, 1));
+    SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, tr.t.perm, n,
+               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale};
+    launch_gather(g, s);
+    c->mark("gather");
+    double* CM = c->buf<double>("e_cm", (size_t)nn * nm);
+    double* LE = c->buf<double>("e_le", (size_t)nn * nm);
+    UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
+              tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis};
+    launch_p2m(up, nl, s);
+    for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
+        int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
+        launch_m2m(up, tr.t.level_nodes + o0, o1 - o0, s);
+    }
+    c->mark("upward");
+    double* near = c->buf<double>("e_near", (size_t)n * nf);
+    NearArgs na{tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
+                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0};
+    launch_near(na, nl, m, k.F, k.damp, s);
+    c->mark("p2p");
+    int* iota = c->buf<int>("iota", nn);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(nn, 256)), dim3(256), 0, s, iota, nn);
+    M2LArgs ma{iota, L.m2l_off, L.m2l_src, tr.t.cx, tr.t.cy, tr.t.cz, CM, LE, nm, p, tb.slot_st, tb.cs, tb.xyz,
+               tb.level_base};
+    launch_m2l(ma, nn, s);
+    c->mark("m2l");
+    DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
+                tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf};
+    for (int lev = 2; lev <= tr.max_depth; ++lev) {
+        int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
+        launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
+    }
+    c->mark("l2l");
+    launch_l2p(da, nl, s);
+    c->mark("l2p");
+    unsigned long long* pc = c->buf<unsigned long long>("e_paircount", 1);
+    FMMH_HIP(hipMemsetAsync(pc, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_pair_count, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src, tr.t.leaf_node,
+                       tr.t.count, nl, pc);
+    unsigned long long pairs = 0;
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 200, pc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    c->sync();
+    std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
+    c->stats[0] = nn; c->stats[1] = nl; c->stats[2] = tr.max_depth;
+    c->stats[3] = L.n_m2l; c->stats[4] = L.n_p2p; c->stats[5] = (int64_t)pairs;
+}
+
+// ---------------------------------------------------------------- host <-> device staging
+struct Staged {
+    Call k;
+    int64_t nf;
+};
+
+Staged stage_inputs(fmmh_context* c, const Call& h, int64_t n_damping) {
+    hipStream_t s = c->stream;
+    const int n = h.n;
+    Staged st;
+    st.k = h;
+    st.nf = ncomp(h.F);
+    double* X = c->buf<double>("h_X", (size_t)n * 3);
+    double* M = c->buf<double>("h_M", (size_t)n * h.dim);
+    int* E = c->buf<int>("h_E", (size_t)n * std::max(h.nexcl, 1));
+    double* F = c->buf<double>("h_F", (size_t)n * st.nf);
+    double* df = c->buf<double>("h_df", n);
+    FMMH_HIP(hipMemcpy2DAsync(X, n * sizeof(double), h.X, h.ldx * sizeof(double), n * sizeof(double), 3,
+                              hipMemcpyHostToDevice, s));
+    FMMH_HIP(hipMemcpy2DAsync(M, n * sizeof(double), h.M, h.ldm * sizeof(double), n * sizeof(double), h.dim,
+                              hipMemcpyHostToDevice, s));
+    if (h.nexcl > 0)
+        FMMH_HIP(hipMemcpy2DAsync(E, n * sizeof(int), h.E, h.lde * sizeof(int), n * sizeof(int), h.nexcl,
+                                  hipMemcpyHostToDevice, s));
+    FMMH_HIP(hipMemcpy2DAsync(F, n * sizeof(double), h.field, h.ldf * sizeof(double), n * sizeof(double), st.nf,
+                              hipMemcpyHostToDevice, s));
+    if (h.damp != FMMH_DAMP_NONE)
+        FMMH_HIP(hipMemcpyAsync(df, h.df, n * sizeof(double), hipMemcpyHostToDevice, s));
+    (void)n_damping;
+    st.k.X = X; st.k.ldx = n;
+    st.k.M = M; st.k.ldm = n;
+    st.k.E = E; st.k.lde = n;
+    st.k.field = F; st.k.ldf = n;
+    st.k.df = h.damp != FMMH_DAMP_NONE ? df : nullptr;
+    return st;
+}
+
+void unstage_field(fmmh_context* c, const Staged& st, double* field, int64_t ldf) {
+    FMMH_HIP(hipMemcpy2DAsync(field, ldf * sizeof(double), st.k.field, st.k.n * sizeof(double),
+                              st.k.n * sizeof(double), st.nf, hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+}
+
+Call make_call(const double* X, int64_t ldx, const double* M, int64_t ldm, int n, int dim, const int* E, int64_t lde,
+               int nexcl, int F, double* field, int64_t ldf, int damp, const double* df) {
+    return Call{X, ldx, M, ldm, n, dim, E, lde, nexcl, F, field, ldf, damp, df};
+}
+
+void check_fmm_args(double theta, int ncrit, int p, int is_periodic) {
+    if (is_periodic) throw Error(FMMH_ERR_UNSUPPORTED, "periodic FMM is not supported by this engine");
+    if (!(theta > 0.0)) throw Error(FMMH_ERR_ARG, "theta must be > 0");
+    if (ncrit < 2 || ncrit > 4096) throw Error(FMMH_ERR_ARG, "ncrit must be in 2..4096");
+    if (p < 0 || p > PMAX) throw Error(FMMH_ERR_UNSUPPORTED, "expansion_order must be in 0.." + std::to_string(PMAX));
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* fmmh_last_error(void) { return g_last_error.c_str(); }
+
+int fmmh_context_create(fmmh_context** ctx, int32_t device) {
+    return guarded([&] { *ctx = make_context(device); });
+}
+
+int fmmh_context_destroy(fmmh_context* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        FMMH_HIP(hipSetDevice(ctx->device));
+        FMMH_HIP(hipStreamSynchronize(ctx->stream));
+        for (auto& kv : ctx->bufs)
+            if (kv.second.p) FMMH_HIP(hipFree(kv.second.p));
+        for (auto& kv : ctx->tables) {
+            FMMH_HIP(hipFree(kv.second.ibuf));
+            FMMH_HIP(hipFree(kv.second.dbuf));
+        }
+        for (hipEvent_t e : ctx->event_pool) FMMH_HIP(hipEventDestroy(e));
+        FMMH_HIP(hipHostFree(ctx->h_pinned));
+        FMMH_HIP(hipStreamDestroy(ctx->stream));
+        delete ctx;
+    });
+}
+
+int fmmh_parse_damp_type(const char* s, int64_t len, int32_t* code) {
+    return guarded([&] {
+        int64_t l = len;
+        while (l > 0 && (s[l - 1] == ' ' || s[l - 1] == '\0')) --l;
+        std::string t(s, (size_t)l);
+        if (t.empty()) *code = FMMH_DAMP_NONE;
+        else if (t == "THOLE") *code = FMMH_DAMP_THOLE;
+        else if (t == "AMOEBA") *code = FMMH_DAMP_AMOEBA;
+        else if (t == "ERF") *code = FMMH_DAMP_ERF;
+        else throw Error(FMMH_ERR_DAMP, "Unknown damp type: " + std::string(s, (size_t)len));
+    });
+}
+
+int fmmh_field_direct(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                      const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                      const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
+                      double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                      int64_t n_damping) {
+    (void)comm;
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        Call h = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
+                           nexcl, field_order, field, ld_field, damp_type, damping_factors);
+        check_common(h, n_damping);
+        check_exclusions_host(h);
+        c->begin_timing();
+        Staged st = stage_inputs(c, h, n_damping);
+        c->mark("h2d");
+        run_direct(c, st.k);
+        unstage_field(c, st, field, ld_field);
+        c->mark("d2h");
+        c->end_timing();
+    });
+}
+
+int fmmh_field_direct_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                          const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                          const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
+                          double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                          int64_t n_damping) {
+    (void)comm;
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        Call k = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
+                           nexcl, field_order, field, ld_field, damp_type, damping_factors);
+        check_common(k, n_damping);
+        c->begin_timing();
+        run_direct(c, k);
+        c->end_timing();
+    });
+}
+
+static int field_fmm_impl(fmmh_context* ctx, bool host, const double* coordinates, int64_t ld_coordinates,
+                          const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                          const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, double theta,
+                          int32_t ncrit, int32_t expansion_order, int32_t field_order, double* field,
+                          int64_t ld_field, int32_t damp_type, const double* damping_factors, int64_t n_damping,
+                          int32_t is_periodic) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        Call h = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
+                           nexcl, field_order, field, ld_field, damp_type, damping_factors);
+        // src/fmm.F90:1372-1382: damp_type first, then the periodic checks
+        if (damp_type < 0 || damp_type > 3) throw Error(FMMH_ERR_DAMP, "Unknown damp type code");
+        if (is_periodic && n <= ncrit)
+            throw Error(FMMH_ERR_UNSUPPORTED, "Not enough coordinates for periodic FMM, please decrease ncrit");
+        check_fmm_args(theta, ncrit, expansion_order, is_periodic);
+        check_common(h, n_damping);
+        if (host) check_exclusions_host(h);
+        c->begin_timing();
+        Call k = h;
+        Staged st;
+        if (host) {
+            st = stage_inputs(c, h, n_damping);
+            k = st.k;
+            c->mark("h2d");
+        }
+        if (n <= ncrit) run_direct(c, k);
+        else run_fmm(c, k, theta, ncrit, expansion_order);
+        if (host) {
+            unstage_field(c, st, field, ld_field);
+            c->mark("d2h");
+        }
+        c->end_timing();
+    });
+}
+
+int fmmh_field_fmm(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                   const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, const int32_t* exclusions,
+                   int64_t ld_exclusions, int32_t nexcl, double theta, int32_t ncrit, int32_t expansion_order,
+                   int32_t field_order, double* field, int64_t ld_field, int32_t damp_type,
+                   const double* damping_factors, int64_t n_damping, int32_t is_periodic, const double* box_matrix) {
+    (void)comm; (void)box_matrix;
+    return field_fmm_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions,
+                          ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field, ld_field,
+                          damp_type, damping_factors, n_damping, is_periodic);
+}
+
+int fmmh_field_fmm_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                       const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                       const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, double theta, int32_t ncrit,
+                       int32_t expansion_order, int32_t field_order, double* field, int64_t ld_field,
+                       int32_t damp_type, const double* damping_factors, int64_t n_damping, int32_t is_periodic,
+                       const double* box_matrix) {
+    (void)comm; (void)box_matrix;
+    return field_fmm_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions,
+                          ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field, ld_field,
+                          damp_type, damping_factors, n_damping, is_periodic);
+}
+
+int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates, int32_t n, int32_t ncrit,
+                    double theta, int64_t* info) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (n < 1 || ld_coordinates < n) throw Error(FMMH_ERR_ARG, "bad n / leading dimension");
+        if (n < ncrit) throw Error(FMMH_ERR_ARG, "the tree needs n >= ncrit");
+        check_fmm_args(theta, ncrit, 0, 0);
+        double* X = c->buf<double>("h_X", (size_t)n * 3);
+        FMMH_HIP(hipMemcpy2DAsync(X, n * sizeof(double), coordinates, ld_coordinates * sizeof(double),
+                                  n * sizeof(double), 3, hipMemcpyHostToDevice, c->stream));
+        TreeOut tr;
+        build_tree(c, X, n, n, ncrit, tr);
+        ListsOut L;
+        build_lists(c, tr, theta, L);
+        c->sync();
+        c->num_nodes = tr.num_nodes; c->num_leaves = tr.num_leaves; c->max_depth = tr.max_depth;
+        c->n_m2l = L.n_m2l; c->n_p2p = L.n_p2p; c->n_sites = n;
+        info[0] = tr.num_nodes; info[1] = tr.num_leaves; info[2] = tr.max_depth; info[3] = L.n_m2l;
+        info[4] = L.n_p2p;
+    });
+}
+
+int fmmh_tree_export(fmmh_context* ctx, double* centers, double* r, double* rmax, int32_t* depth, int32_t* parent,
+                     int32_t* is_leaf, int32_t* nleaf, int32_t* mem_off, int32_t* members, int32_t* m2l_off,
+                     int32_t* m2l_src, int32_t* p2p_off, int32_t* p2p_src) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        const int nn = c->num_nodes, n = c->n_sites;
+        if (nn <= 0) throw Error(FMMH_ERR_ARG, "no tree has been built on this context");
+        auto get = [&](void* dst, const char* name, size_t bytes) {
+            auto it = c->bufs.find(name);
+            if (it == c->bufs.end()) throw Error(FMMH_ERR_INTERNAL, std::string("missing buffer ") + name);
+            FMMH_HIP(hipMemcpy(dst, it->second.p, bytes, hipMemcpyDeviceToHost));
+        };
+        get(centers, "n_cx", nn * sizeof(double));
+        get(centers + nn, "n_cy", nn * sizeof(double));
+        get(centers + 2 * (size_t)nn, "n_cz", nn * sizeof(double));
+        get(r, "n_r", nn * sizeof(double));
+        get(rmax, "n_rmax", nn * sizeof(double));
+        get(depth, "n_level", nn * sizeof(int));
+        get(parent, "n_parent", nn * sizeof(int));
+        get(is_leaf, "n_is_leaf", nn * sizeof(int));
+        get(nleaf, "n_count", nn * sizeof(int));
+        std::vector<int> start(nn), perm(n);
+        get(start.data(), "n_start", nn * sizeof(int));
+        get(perm.data(), "t_perm", n * sizeof(int));
+        int mo = 0;
+        for (int id = 0; id < nn; ++id) {
+            mem_off[id] = mo;
+            if (is_leaf[id])
+                for (int q = 0; q < nleaf[id]; ++q) members[mo++] = perm[start[id] + q];
+        }
+        mem_off[nn] = mo;
+        get(m2l_off, "l_m2l_off", (nn + 1) * sizeof(int));
+        get(p2p_off, "l_p2p_off", (nn + 1) * sizeof(int));
+        if (c->n_m2l) get(m2l_src, "l_m2l_src", c->n_m2l * sizeof(int));
+        if (c->n_p2p) get(p2p_src, "l_p2p_src", c->n_p2p * sizeof(int));
+    });
+}
+
+int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (key == FMMH_OPT_TIMING) c->timing = value != 0.0;
+        else throw Error(FMMH_ERR_ARG, "unknown option key " + std::to_string(key));
+    });
+}
+
+int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap) {
+    int count = 0;
+    int rc = guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        std::string all;
+        for (auto& kv : c->phase_ms) {
+            if (count < cap) ms[count] = kv.second;
+            ++count;
+            if (!all.empty()) all += ";";
+            all += kv.first;
+        }
+        if (names && name_cap > 0) {
+            std::strncpy(names, all.c_str(), (size_t)name_cap - 1);
+            names[name_cap - 1] = '\0';
+        }
+    });
+    return rc == FMMH_OK ? std::min(count, (int)cap) : -rc;
+}
+
+int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        for (int i = 0; i < std::min(cap, 6); ++i) out[i] = c->stats[i];
+    });
+}
+
+}  // extern "C"

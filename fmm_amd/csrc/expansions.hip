@@ -1,0 +1,312 @@
+// Far-field operators of the field engine, FP64:
+//   P2M  src/fmm.F90:437-472, 520-534   leaf moments about the leaf centre
+//   M2M  src/fmm.F90:387-435, 536-540   (reference: leaf -> every ancestor;
+//        here child -> parent level by level, identical by the binomial
+//        theorem, different summation order)
+//   M2L  src/fmm.F90:826-882            undamped T of order 2p per pair
+//   L2L  src/fmm.F90:981-1044           parent -> child, level by level
+//   L2P  src/fmm.F90:1046-1132          evaluation at leaf sites, added to
+//        the near field and scattered to the caller's site order
+// Shift operators run one workgroup per node with one lane per target
+// coefficient, driven by host-built (target, source, monomial, coefficient)
+// tables.  M2L builds the order-2p tensor of each pair cooperatively in LDS
+// with the Hermite recursion (one barrier per tensor degree) and contracts it
+// against the LDS-staged source moments.
+#include "common.h"
+#include "kernels.h"
+#include "tensors.h"
+
+namespace fmmh {
+
+constexpr int MAX_NM = ncomp(PMAX);          // expansion coefficients per node
+constexpr int UP_CHUNK = 16;                 // sites staged per P2M pass
+constexpr int UP_NSRC = ncomp(NEAR_MMAX);    // source multipole components staged per site
+
+// monomials d^e for every packed e with |e| <= p, built from one-step parents.
+__device__ __forceinline__ void build_mono(double* mono, int nm, double dx, double dy, double dz,
+                                           const int* __restrict__ mono_parent, const int* __restrict__ mono_axis) {
+    mono[0] = 1.0;
+    for (int e = 1; e < nm; ++e) {
+        const int ax = mono_axis[e];
+        const double d = ax == 0 ? dx : (ax == 1 ? dy : dz);
+        mono[e] = mono[mono_parent[e]] * d;
+    }
+}
+
+__global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
+    __shared__ double mono[UP_CHUNK][MAX_NM];
+    __shared__ double mch[UP_CHUNK][UP_NSRC];
+    const int node = a.leaf_node[blockIdx.x];
+    const int s0 = a.start[node], c = a.count[node];
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    const int nm = a.nm;
+    const int nsrc = a.nsrc;
+    double acc[(MAX_NM + 127) / 128];
+#pragma unroll
+    for (int q = 0; q < (MAX_NM + 127) / 128; ++q) acc[q] = 0.0;
+    for (int j0 = 0; j0 < c; j0 += UP_CHUNK) {
+        const int cc = min(UP_CHUNK, c - j0);
+        if ((int)threadIdx.x < cc) {
+            const int j = s0 + j0 + threadIdx.x;
+            build_mono(mono[threadIdx.x], nm, a.xs[j] - cx, a.ys[j] - cy, a.zs[j] - cz, a.mono_parent, a.mono_axis);
+        }
+        for (int x = threadIdx.x; x < cc * nsrc; x += blockDim.x) {
+            const int jj = x / nsrc, sc = x % nsrc;
+            mch[jj][sc] = a.ms_raw[(int64_t)sc * a.ldm + s0 + j0 + jj];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
+            const int t = threadIdx.x + 128 * q;
+            if (t < nm) {
+                const int b0 = a.p2m.off[t], b1 = a.p2m.off[t + 1];
+                for (int b = b0; b < b1; ++b) {
+                    const int s = a.p2m.s[b], e = a.p2m.e[b];
+                    double sum = 0.0;
+                    for (int jj = 0; jj < cc; ++jj) sum = __builtin_fma(mono[jj][e], mch[jj][s], sum);
+                    acc[q] = __builtin_fma(a.p2m.coef[b], sum, acc[q]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
+        const int t = threadIdx.x + 128 * q;
+        if (t < nm) a.CM[(int64_t)node * nm + t] = acc[q];
+    }
+}
+
+__global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ nodes) {
+    __shared__ double mono[MAX_NM];
+    const int node = nodes[blockIdx.x];
+    const int nm = a.nm;
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    double acc[(MAX_NM + 127) / 128];
+#pragma unroll
+    for (int q = 0; q < (MAX_NM + 127) / 128; ++q) acc[q] = 0.0;
+    bool any = false;
+    for (int o = 0; o < 8; ++o) {
+        const int ch = a.children[node * 8 + o];
+        if (ch < 0) continue;
+        any = true;
+        if (threadIdx.x == 0)
+            build_mono(mono, nm, a.cx[ch] - cx, a.cy[ch] - cy, a.cz[ch] - cz, a.mono_parent, a.mono_axis);
+        __syncthreads();
+        const double* src = a.CM + (int64_t)ch * nm;
+#pragma unroll
+        for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
+            const int t = threadIdx.x + 128 * q;
+            if (t < nm) {
+                const int b0 = a.m2m.off[t], b1 = a.m2m.off[t + 1];
+                double sum = 0.0;
+                for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.m2m.coef[b] * mono[a.m2m.e[b]], src[a.m2m.s[b]], sum);
+                acc[q] += sum;
+            }
+        }
+        __syncthreads();
+    }
+    if (!any) return;   // leaves keep their P2M moments
+#pragma unroll
+    for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
+        const int t = threadIdx.x + 128 * q;
+        if (t < nm) a.CM[(int64_t)node * nm + t] = acc[q];
+    }
+}
+
+// ---------------------------------------------------------------- M2L
+constexpr int M2L_KMAX = 2 * 10;                       // p <= 10 for the LDS tensor
+constexpr int M2L_AUX = (M2L_KMAX + 1) * (M2L_KMAX + 2) * (M2L_KMAX + 3) * (M2L_KMAX + 4) / 24;
+constexpr int M2L_NMMAX = ncomp(10);
+
+__global__ __launch_bounds__(64) void k_m2l(M2LArgs a) {
+    __shared__ double A[M2L_AUX];
+    __shared__ double cm[M2L_NMMAX];
+    const int i = a.targets[blockIdx.x];
+    const int nm = a.nm;
+    const int K = 2 * a.p;
+    const int e0 = a.off[i], e1 = a.off[i + 1];
+    const double cxi = a.cx[i], cyi = a.cy[i], czi = a.cz[i];
+    constexpr int NQ = (M2L_NMMAX + 63) / 64;
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int e = e0; e < e1; ++e) {
+        const int j = a.src[e];
+        const double dx = cxi - a.cx[j], dy = cyi - a.cy[j], dz = czi - a.cz[j];
+        if ((int)threadIdx.x <= K) {
+            const double R2 = dx * dx + dy * dy + dz * dz;
+            const double inv = rsqrt(R2);
+            const double inv2 = inv * inv;
+            double g = inv;
+            for (int n = 1; n <= (int)threadIdx.x; ++n) g *= -(2.0 * n - 1.0) * inv2;
+            A[threadIdx.x] = g;               // degree 0: slot n
+        }
+        for (int s = threadIdx.x; s < nm; s += 64) cm[s] = a.cs[s] * a.CM[(int64_t)j * nm + s];
+        __syncthreads();
+        for (int k = 1; k <= K; ++k) {
+            const int nd = (k + 1) * (k + 2) / 2;
+            const int ndm1 = k * (k + 1) / 2;
+            const int ndm2 = (k - 1) * k / 2;
+            const int nent = nd * (K - k + 1);
+            const int base = a.level_base[k], bm1 = a.level_base[k - 1], bm2 = k >= 2 ? a.level_base[k - 2] : 0;
+            const int c0 = ncomp(k - 1), c1 = k >= 1 ? ncomp(k - 2) : 0, c2 = k >= 2 ? ncomp(k - 3) : 0;
+            for (int x = threadIdx.x; x < nent; x += 64) {
+                const int n = x / nd, l = x - n * nd;
+                const int pk = a.xyz[c0 + l];
+                const int t = pk & 255, u = (pk >> 8) & 255, v = pk >> 16;
+                double val;
+                if (t > 0) {
+                    val = dx * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t - 1, u, v) - c1)];
+                    if (t > 1) val = __builtin_fma((double)(t - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t - 2, u, v) - c2)], val);
+                } else if (u > 0) {
+                    val = dy * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u - 1, v) - c1)];
+                    if (u > 1) val = __builtin_fma((double)(u - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u - 2, v) - c2)], val);
+                } else {
+                    val = dz * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u, v - 1) - c1)];
+                    if (v > 1) val = __builtin_fma((double)(v - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u, v - 2) - c2)], val);
+                }
+                A[base + n * nd + l] = val;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int t = threadIdx.x + 64 * q;
+            if (t < nm) {
+                double sum = 0.0;
+                for (int s = 0; s < nm; ++s) sum = __builtin_fma(cm[s], A[a.slot_st[s * nm + t]], sum);
+                acc[q] += sum;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int t = threadIdx.x + 64 * q;
+        if (t < nm) a.LE[(int64_t)i * nm + t] = acc[q];
+    }
+}
+
+// ---------------------------------------------------------------- L2L / L2P
+__global__ __launch_bounds__(128) void k_l2l(DownArgs a, const int* __restrict__ nodes) {
+    __shared__ double mono[MAX_NM];
+    __shared__ double lp[MAX_NM];
+    const int node = nodes[blockIdx.x];
+    const int par = a.parent[node];
+    const int nm = a.nm;
+    if (threadIdx.x == 0)
+        build_mono(mono, nm, a.cx[node] - a.cx[par], a.cy[node] - a.cy[par], a.cz[node] - a.cz[par], a.mono_parent,
+                   a.mono_axis);
+    for (int s = threadIdx.x; s < nm; s += blockDim.x) lp[s] = a.LE[(int64_t)par * nm + s];
+    __syncthreads();
+    for (int t = threadIdx.x; t < nm; t += blockDim.x) {
+        const int b0 = a.l2l.off[t], b1 = a.l2l.off[t + 1];
+        double sum = 0.0;
+        for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.l2l.coef[b] * mono[a.l2l.e[b]], lp[a.l2l.s[b]], sum);
+        a.LE[(int64_t)node * nm + t] += sum;
+    }
+}
+
+// f_t = sum_e L_{t+e} d^e / (ex! ey! ez!), |t| <= F, |t + e| <= p, one lane per site.
+template <int F>
+__global__ __launch_bounds__(64) void k_l2p(DownArgs a) {
+    constexpr int NF = ncomp(F);
+    __shared__ double L[MAX_NM];
+    __shared__ double fact[PMAX + 1];
+    const int node = a.leaf_node[blockIdx.x];
+    const int s0 = a.start[node], c = a.count[node];
+    const int nm = a.nm, p = a.p;
+    for (int s = threadIdx.x; s < nm; s += 64) L[s] = a.LE[(int64_t)node * nm + s];
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        fact[0] = 1.0;
+        for (int n = 1; n <= PMAX; ++n) { f *= (double)n; fact[n] = 1.0 / f; }
+    }
+    __syncthreads();
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    for (int base = 0; base < c; base += 64) {
+        const int jj = base + threadIdx.x;
+        if (jj >= c) break;
+        const int k = s0 + jj;
+        const double dx = a.xs[k] - cx, dy = a.ys[k] - cy, dz = a.zs[k] - cz;
+        double f[NF];
+#pragma unroll
+        for (int t = 0; t < NF; ++t) f[t] = 0.0;
+        double px = 1.0;
+        for (int ex = 0; ex <= p; ++ex) {
+            double pxy = px;
+            for (int ey = 0; ey + ex <= p; ++ey) {
+                double m = pxy * fact[ex] * fact[ey];
+                for (int ez = 0; ez + ey + ex <= p; ++ez) {
+                    const double mz = m * fact[ez];
+                    const int ne = ex + ey + ez;
+#pragma unroll
+                    for (int t = 0; t < NF; ++t) {
+                        const Exps et = idx2xyz(t);
+                        if (et.x + et.y + et.z + ne <= p)
+                            f[t] = __builtin_fma(mz, L[xyz2idx(et.x + ex, et.y + ey, et.z + ez)], f[t]);
+                    }
+                    m *= dz;
+                }
+                pxy *= dy;
+            }
+            px *= dx;
+        }
+        const int o = a.perm[k];
+#pragma unroll
+        for (int t = 0; t < NF; ++t) {
+            double* fp = a.field + (int64_t)t * a.ld_field + o;
+            *fp = (*fp + a.near[(int64_t)t * a.ld_near + k]) + f[t];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(const double* __restrict__ near, int64_t ldn,
+                                                    double* __restrict__ field, int64_t ldf, int n, int nf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int t = 0; t < nf; ++t) field[(int64_t)t * ldf + i] += near[(int64_t)t * ldn + i];
+}
+
+void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
+    if (num_leaves <= 0) return;
+    hipLaunchKernelGGL(k_p2m, dim3(num_leaves), dim3(128), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, a, nodes);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2l(const M2LArgs& a, int num_targets, hipStream_t s) {
+    if (num_targets <= 0) return;
+    hipLaunchKernelGGL(k_m2l, dim3(num_targets), dim3(64), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_l2l, dim3(count), dim3(128), 0, s, a, nodes);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
+    if (num_leaves <= 0) return;
+    switch (a.nf) {
+        case 1: hipLaunchKernelGGL(k_l2p<0>, dim3(num_leaves), dim3(64), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_l2p<1>, dim3(num_leaves), dim3(64), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_l2p<2>, dim3(num_leaves), dim3(64), 0, s, a); break;
+    }
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_accumulate(const double* near, int64_t ld_near, double* field, int64_t ld_field, int n, int nf,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, near, ld_near, field, ld_field, n, nf);
+    FMMH_HIP(hipGetLastError());
+}
+
+}  // namespace fmmh

@@ -1,0 +1,179 @@
+// Kernel argument blocks and launch wrappers shared by the translation units
+// of the field engine.  Everything here is device memory.
+#pragma once
+#include "common.h"
+
+namespace fmmh {
+
+constexpr int NEAR_MMAX = 2;   // multipole order compiled into the near-field kernel
+constexpr int NEAR_FMAX = 2;   // field order compiled into the near-field kernel
+
+struct NearArgs {
+    const int* blk;            // workgroup -> target block row (a leaf's node id, or a chunk id)
+    const int* tgt_start;      // per target block: first sorted site
+    const int* tgt_count;      //   and number of sites
+    const int* list_off;       // CSR over target blocks -> source blocks
+    const int* list_src;
+    const int* src_start;      // per source block: first sorted site
+    const int* src_count;
+    const double* xs;          // sorted site coordinates (SoA)
+    const double* ys;
+    const double* zs;
+    const double* ms;          // sorted multipoles scaled by (-1)^{|s|+1}/(sx!sy!sz!), component-major
+    int64_t ldm;
+    const double* dfs;         // sorted damping factors
+    const int* orig;           // sorted -> original (0-based) site index
+    const int* excl;           // sorted exclusion rows (1-based original indices), component-major
+    int64_t lde;
+    int nexcl;
+    double* out;               // per sorted site, ncomp(F) components
+    int64_t ldo;
+    int accumulate;            // 0: out = result, 1: out += result
+};
+
+bool near_supported(int m, int F);
+void launch_near(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s);
+
+// ---------------------------------------------------------------- tree
+struct TreeDev {
+    // per site (sorted order after the build)
+    unsigned long long* keys;  // 63-bit octant-digit keys, sorted
+    int* perm;                 // sorted position -> original site
+    int* lmin;                 // shallowest level whose node starts at this site
+    int* lam;                  // leaf level of the site
+    int* cnt;                  // nodes that start at this site
+    int* off;                  // exclusive scan of cnt = pre-order id of the first such node
+    int* w;                    // window common-prefix lengths
+    // per node (pre-order id)
+    int* level;                // root = 1
+    int* start;
+    int* count;
+    int* parent;
+    int* is_leaf;
+    int* octant;
+    int* children;             // 8 per node, -1 = empty
+    double* cx;
+    double* cy;
+    double* cz;
+    double* r;
+    double* rmax;
+    int* leaf_id;              // node -> leaf index or -1
+    int* leaf_node;            // leaf index -> node
+    int* level_nodes;          // nodes grouped by level
+    // scalars
+    double* root;              // cx, cy, cz, r0, rmax0
+    unsigned long long* maxabs;  // 3 bit patterns of max |x - c|
+    int* info;                 // [0] num_nodes, [1] error flag, [2] max level, [3..3+KEY_LEVELS+1] level counts
+    int* level_fill;
+};
+
+void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, hipStream_t s);
+void launch_keys(const double* X, int64_t ldx, int n, TreeDev& t, unsigned long long* keys_out, int* idx_out,
+                 hipStream_t s);
+void launch_site_levels(int n, int ncrit, TreeDev& t, hipStream_t s);
+void launch_emit_nodes(int n, TreeDev& t, hipStream_t s);
+void launch_node_links(int num_nodes, TreeDev& t, hipStream_t s);
+void launch_leaf_index(int num_nodes, const int* leaf_scan, TreeDev& t, hipStream_t s);
+void launch_level_scatter(int num_nodes, const int* level_off, TreeDev& t, hipStream_t s);
+void launch_leaf_sort(int num_leaves, int max_leaf, TreeDev& t, hipStream_t s);
+
+// sorted site arrays
+struct SiteArgs {
+    const double* X; int64_t ldx;
+    const double* M; int64_t ldm; int dim;
+    const int* E; int64_t lde; int nexcl;
+    const double* df;
+    const int* perm;
+    int n;
+    double* xs; double* ys; double* zs;
+    double* ms_raw; double* ms_scaled;   // component-major, leading dimension n
+    double* dfs;
+    int* excl_s;                         // component-major, leading dimension n
+    const double* scale;                 // per multipole component factor (device, dim entries)
+};
+void launch_gather(const SiteArgs& a, hipStream_t s);
+
+// ---------------------------------------------------------------- lists
+struct TraverseArgs {
+    const int2* in; const int* n_in;
+    int2* out; int* n_out; int cap_out;
+    unsigned long long* m2l; int* n_m2l; int cap_m2l;
+    unsigned long long* p2p; int* n_p2p; int cap_p2p;
+    int* overflow;
+    const double* cx; const double* cy; const double* cz; const double* rmax;
+    const int* is_leaf; const int* children;
+    double theta;
+};
+void launch_traverse(const TraverseArgs& a, int grid, hipStream_t s);
+void launch_csr(const unsigned long long* keys, int n_keys, int num_targets, int* off, int* src, hipStream_t s);
+
+// ---------------------------------------------------------------- expansions
+// Shift tables: entries grouped by target component t (CSR by t); each entry
+// carries the source component s, the exponent triple of the monomial, and
+// the combinatorial coefficient.
+struct ShiftTab {
+    const int* off;      // ncomp(order_t) + 1
+    const int* s;        // source packed index
+    const int* e;        // packed index of the monomial exponent triple
+    const double* coef;
+};
+
+struct UpArgs {
+    const double* xs; const double* ys; const double* zs;
+    const double* ms_raw; int64_t ldm;
+    const int* leaf_node; const int* start; const int* count;
+    const double* cx; const double* cy; const double* cz;
+    const int* children;
+    double* CM; int nm; int p;
+    int nsrc;                 // multipole components entering P2M (|s| <= min(m, p))
+    ShiftTab p2m; ShiftTab m2m;
+    const int* mono_parent;   // packed index -> packed index with one exponent less
+    const int* mono_axis;     //   and which axis was removed
+};
+void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s);
+void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s);
+
+struct M2LArgs {
+    const int* targets;       // nodes with non-empty lists
+    const int* off; const int* src;
+    const double* cx; const double* cy; const double* cz;
+    const double* CM; double* LE; int nm; int p;
+    const int* slot_st;       // [s * nm + t] -> aux slot of T_{s+t}
+    const double* cs;         // (-1)^{|s|+1} / (sx! sy! sz!)
+    const int* xyz;           // packed index -> t | u << 8 | v << 16
+    const int* level_base;    // aux slot base per degree (K + 2 entries)
+};
+void launch_m2l(const M2LArgs& a, int num_targets, hipStream_t s);
+
+struct DownArgs {
+    const double* cx; const double* cy; const double* cz;
+    const int* parent;
+    double* LE; int nm; int p;
+    ShiftTab l2l; ShiftTab l2p;
+    const int* mono_parent; const int* mono_axis;
+    // L2P
+    const int* leaf_node; const int* start; const int* count;
+    const double* xs; const double* ys; const double* zs;
+    const int* perm;
+    const double* near; int64_t ld_near;   // P2P result per sorted site
+    double* field; int64_t ld_field;       // accumulated at original indices
+    int nf;
+};
+void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s);
+void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s);
+
+// field[i, t] += near[i, t] for i < n (direct sum, original order)
+void launch_accumulate(const double* near, int64_t ld_near, double* field, int64_t ld_field, int n, int nf,
+                       hipStream_t s);
+
+// ---------------------------------------------------------------- primitives (rocPRIM)
+size_t sort_pairs_u64_temp(int n);
+void sort_pairs_u64(void* temp, size_t temp_bytes, const unsigned long long* kin, unsigned long long* kout,
+                    const int* vin, int* vout, int n, int end_bit, hipStream_t s);
+size_t sort_keys_u64_temp(int n);
+void sort_keys_u64(void* temp, size_t temp_bytes, const unsigned long long* kin, unsigned long long* kout, int n,
+                   int end_bit, hipStream_t s);
+size_t exscan_i32_temp(int n);
+void exscan_i32(void* temp, size_t temp_bytes, const int* in, int* out, int n, hipStream_t s);
+
+}  // namespace fmmh

@@ -1,0 +1,422 @@
+// Octree construction and dual-tree interaction lists, bit-exact with
+// peter-reinholdt/fmm src/fmm.F90:137-385.
+//
+// The reference inserts sites one by one into a pointer octree.  Its result
+// depends only on (a) the FP64 chain of node centres, root = centroid,
+// child = parent +- r/2 per axis (src/fmm.F90:184-189, 229-240), and (b) the
+// rule "a node is internal iff its subtree holds >= ncrit sites".  So every
+// site can compute its own octant digits down to 21 levels independently
+// (a 63-bit key, digit 4z + 2y + x per level, strict '>' tests as
+// src/fmm.F90:197-206); after a stable sort by key,
+//   * the sites of any node form a contiguous run;
+//   * a run of length >= ncrit is internal, and a site's leaf level is
+//     2 + max over the ncrit-windows that contain it of the common prefix
+//     length of the window's first and last key;
+//   * pre-order DFS ids (src/fmm.F90:286-299) are the order of
+//     (first site, level) pairs, so one exclusive scan numbers all nodes.
+// Leaf members are then re-sorted by site index, which is the reference's
+// insertion order within a leaf.
+//
+// All centre and MAC arithmetic is FP64 with contraction disabled, using the
+// same operation order as the reference (compiled without FMA), and the MAC
+// distance reproduces flang's NORM2 (running maximum, scaled sum of squares).
+#include "common.h"
+#include "kernels.h"
+
+namespace fmmh {
+
+__device__ __forceinline__ int common_digits(unsigned long long a, unsigned long long b) {
+    unsigned long long x = a ^ b;
+    if (x == 0ull) return KEY_LEVELS;
+    return (__clzll((long long)x) - 1) / 3;
+}
+
+__device__ __forceinline__ int key_digit(unsigned long long key, int d) {   // d = 1..21
+    return (int)((key >> (63 - 3 * d)) & 7ull);
+}
+
+// Sequential centroid, one workgroup per axis: sum(coordinates, 1) / N.
+// flang evaluates SUM left to right; a parallel reduction would round
+// differently and move node centres.  All lanes run the same uniform loop
+// so the loads are scalar.
+__global__ __launch_bounds__(64) void k_centroid(const double* __restrict__ X, int64_t ldx, int n,
+                                                 double* __restrict__ root) {
+#pragma clang fp contract(off)
+    const double* col = X + (int64_t)blockIdx.x * ldx;
+    double s = 0.0;
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        double v0 = col[i], v1 = col[i + 1], v2 = col[i + 2], v3 = col[i + 3];
+        double v4 = col[i + 4], v5 = col[i + 5], v6 = col[i + 6], v7 = col[i + 7];
+        s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
+    }
+    for (; i < n; ++i) s += col[i];
+    if (threadIdx.x == 0) root[blockIdx.x] = s / (double)n;
+}
+
+// max |x - c| per axis (exact, so any reduction order is fine).
+__global__ __launch_bounds__(256) void k_maxabs(const double* __restrict__ X, int64_t ldx, int n,
+                                                const double* __restrict__ root,
+                                                unsigned long long* __restrict__ maxabs) {
+    __shared__ double red[3][256];
+    double m[3] = {0.0, 0.0, 0.0};
+    const double c[3] = {root[0], root[1], root[2]};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 3; ++k) m[k] = fmax(m[k], fabs(X[(int64_t)k * ldx + i] - c[k]));
+    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = m[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) atomicMax(&maxabs[threadIdx.x], (unsigned long long)__double_as_longlong(red[threadIdx.x][0]));
+}
+
+__device__ __forceinline__ double root_halfwidth(const unsigned long long* maxabs) {
+#pragma clang fp contract(off)
+    double mx = __longlong_as_double((long long)maxabs[0]);
+    double my = __longlong_as_double((long long)maxabs[1]);
+    double mz = __longlong_as_double((long long)maxabs[2]);
+    return fmax(mx, fmax(my, mz)) * 1.0001;
+}
+
+__device__ __forceinline__ double rmax_of(double r) {
+#pragma clang fp contract(off)
+    double t = 1.5 * r;
+    return sqrt(t * r);
+}
+
+__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ X, int64_t ldx, int n,
+                                              double* __restrict__ root, const unsigned long long* __restrict__ maxabs,
+                                              unsigned long long* __restrict__ keys, int* __restrict__ idx) {
+#pragma clang fp contract(off)
+    const double r0 = root_halfwidth(maxabs);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { root[3] = r0; root[4] = rmax_of(r0); }
+    if (i >= n) return;
+    const double x = X[i], y = X[ldx + i], z = X[2 * ldx + i];
+    double cx = root[0], cy = root[1], cz = root[2], r = r0;
+    unsigned long long key = 0ull;
+    for (int d = 0; d < KEY_LEVELS; ++d) {
+        const int o = (x > cx ? 1 : 0) | (y > cy ? 2 : 0) | (z > cz ? 4 : 0);
+        key = (key << 3) | (unsigned long long)o;
+        r = r * 0.5;
+        cx = (o & 1) ? cx + r : cx - r;
+        cy = (o & 2) ? cy + r : cy - r;
+        cz = (o & 4) ? cz + r : cz - r;
+    }
+    keys[i] = key;
+    idx[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_cpl(const unsigned long long* __restrict__ keys, int n, int ncrit,
+                                             int* __restrict__ lmin, int* __restrict__ w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    lmin[k] = (k == 0) ? 1 : common_digits(keys[k - 1], keys[k]) + 2;
+    if (k + ncrit - 1 < n) w[k] = common_digits(keys[k], keys[k + ncrit - 1]);
+}
+
+__global__ __launch_bounds__(256) void k_lambda(int n, int ncrit, const int* __restrict__ lmin,
+                                                const int* __restrict__ w, int* __restrict__ lam,
+                                                int* __restrict__ cnt, int* __restrict__ info) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j0 = max(0, k - ncrit + 1), j1 = min(k, n - ncrit);
+    int W = 0;
+    for (int j = j0; j <= j1; ++j) W = max(W, w[j]);
+    if (W >= KEY_LEVELS) atomicOr(&info[1], 1);   // >= ncrit sites share a 21-level cell
+    const int l = W + 2;
+    lam[k] = l;
+    cnt[k] = max(0, l - lmin[k] + 1);
+}
+
+// first index in [lo, hi) whose prefix (key >> shift) exceeds p; keys sorted.
+__device__ __forceinline__ int upper_prefix(const unsigned long long* keys, int lo, int hi, int shift,
+                                            unsigned long long p) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((keys[mid] >> shift) > p) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+// first index in [lo, hi) whose prefix is >= p.
+__device__ __forceinline__ int lower_prefix(const unsigned long long* keys, int lo, int hi, int shift,
+                                            unsigned long long p) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((keys[mid] >> shift) >= p) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_emit_nodes(int n, TreeDev t) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = t.cnt[k];
+    if (c <= 0) return;
+    const unsigned long long key = t.keys[k];
+    const int l0 = t.lmin[k], l1 = t.lam[k];
+    const int id0 = t.off[k];
+    double cx = t.root[0], cy = t.root[1], cz = t.root[2], r = t.root[3];
+    int lev = 1;
+    for (int L = l0; L <= l1; ++L) {
+        while (lev < L) {            // walk the centre chain down to level L
+            const int o = key_digit(key, lev);
+            r = r * 0.5;
+            cx = (o & 1) ? cx + r : cx - r;
+            cy = (o & 2) ? cy + r : cy - r;
+            cz = (o & 4) ? cz + r : cz - r;
+            ++lev;
+        }
+        const int id = id0 + (L - l0);
+        t.level[id] = L;
+        t.start[id] = k;
+        const int q = L - 1;                    // prefix digits shared by the node's sites
+        int end;
+        if (q == 0) end = n;
+        else {
+            const int sh = 63 - 3 * q;
+            end = upper_prefix(t.keys, k + 1, n, sh, key >> sh);
+        }
+        t.count[id] = end - k;
+        t.is_leaf[id] = (L == l1) ? 1 : 0;
+        t.cx[id] = cx; t.cy[id] = cy; t.cz[id] = cz;
+        t.r[id] = r;
+        t.rmax[id] = rmax_of(r);
+        if (L == 1) {
+            t.parent[id] = -1;
+            t.octant[id] = -1;
+        } else {
+            t.octant[id] = key_digit(key, L - 1);
+            if (L - 1 >= l0) t.parent[id] = id - 1;
+            else {
+                const int qp = L - 2;
+                int ps = 0;
+                if (qp > 0) {
+                    const int sh = 63 - 3 * qp;
+                    ps = lower_prefix(t.keys, 0, k, sh, key >> sh);
+                }
+                t.parent[id] = t.off[ps] + (L - 1 - t.lmin[ps]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_node_links(int num_nodes, TreeDev t) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= num_nodes) return;
+    const int L = t.level[id];
+    if (id > 0) t.children[t.parent[id] * 8 + t.octant[id]] = id;
+    atomicAdd(&t.info[3 + L], 1);
+    atomicMax(&t.info[2], L);
+}
+
+__global__ __launch_bounds__(256) void k_leaf_index(int num_nodes, const int* __restrict__ leaf_scan, TreeDev t) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= num_nodes) return;
+    if (t.is_leaf[id]) {
+        const int li = leaf_scan[id];
+        t.leaf_id[id] = li;
+        t.leaf_node[li] = id;
+    } else {
+        t.leaf_id[id] = -1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_level_scatter(int num_nodes, const int* __restrict__ level_off, TreeDev t) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= num_nodes) return;
+    const int L = t.level[id];
+    const int pos = atomicAdd(&t.level_fill[L], 1);
+    t.level_nodes[level_off[L] + pos] = id;
+}
+
+// Leaf members in ascending site index (the reference's insertion order).
+__global__ __launch_bounds__(64) void k_leaf_sort(TreeDev t) {
+    extern __shared__ int buf[];
+    const int node = t.leaf_node[blockIdx.x];
+    const int s0 = t.start[node], c = t.count[node];
+    for (int i = threadIdx.x; i < c; i += 64) buf[i] = t.perm[s0 + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += 64) {
+        const int v = buf[i];
+        int rank = 0;
+        for (int j = 0; j < c; ++j) rank += (buf[j] < v) ? 1 : 0;
+        t.perm[s0 + rank] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather(SiteArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const int o = a.perm[k];
+    a.xs[k] = a.X[o];
+    a.ys[k] = a.X[a.ldx + o];
+    a.zs[k] = a.X[2 * a.ldx + o];
+    for (int c = 0; c < a.dim; ++c) {
+        const double v = a.M[(int64_t)c * a.ldm + o];
+        a.ms_raw[(int64_t)c * a.n + k] = v;
+        a.ms_scaled[(int64_t)c * a.n + k] = a.scale[c] * v;
+    }
+    a.dfs[k] = a.df ? a.df[o] : 0.0;
+    for (int q = 0; q < a.nexcl; ++q) a.excl_s[(int64_t)q * a.n + k] = a.E[(int64_t)q * a.lde + o];
+}
+
+// flang NORM2 of a 3-vector (see the header comment).
+__device__ __forceinline__ double norm2_3(double v0, double v1, double v2) {
+#pragma clang fp contract(off)
+    const double v[3] = {v0, v1, v2};
+    double mx = 0.0, s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double a = fabs(v[k]);
+        if (mx == 0.0) {
+            mx = a;
+        } else if (a > mx) {
+            const double t = mx / a;
+            const double tsq = t * t;
+            s = s * tsq;
+            s = s + tsq;
+            mx = a;
+        } else {
+            const double t = a / mx;
+            const double tt = t * t;
+            s = s + tt;
+        }
+    }
+    const double one_s = 1.0 + s;
+    return mx * sqrt(one_s);
+}
+
+// One level of the dual traversal of src/fmm.F90:352-385 over a frontier of
+// (target, source) node pairs.  Outputs are appended in arbitrary order and
+// sorted afterwards; for a fixed target the reference's DFS appends sources
+// in ascending pre-order id, which the sort reproduces.
+__global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
+#pragma clang fp contract(off)
+    const int n = *a.n_in;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int2 pr = a.in[e];
+        const int i = pr.x, j = pr.y;
+        const double d0 = a.cx[i] - (a.cx[j] + 0.0);
+        const double d1 = a.cy[i] - (a.cy[j] + 0.0);
+        const double d2 = a.cz[i] - (a.cz[j] + 0.0);
+        const double r = norm2_3(d0, d1, d2);
+        const double lhs = r * a.theta;
+        const double rhs = a.rmax[i] + a.rmax[j];
+        const bool li = a.is_leaf[i] != 0, lj = a.is_leaf[j] != 0;
+        if (lhs > rhs) {
+            const int pos = atomicAdd(a.n_m2l, 1);
+            if (pos < a.cap_m2l) a.m2l[pos] = ((unsigned long long)i << 32) | (unsigned)j;
+            else atomicOr(a.overflow, 1);
+        } else if (li && lj) {
+            const int pos = atomicAdd(a.n_p2p, 1);
+            if (pos < a.cap_p2p) a.p2p[pos] = ((unsigned long long)i << 32) | (unsigned)j;
+            else atomicOr(a.overflow, 2);
+        } else if (lj || ((a.rmax[i] > a.rmax[j]) && !li)) {
+            for (int o = 0; o < 8; ++o) {
+                const int c = a.children[i * 8 + o];
+                if (c < 0) continue;
+                const int pos = atomicAdd(a.n_out, 1);
+                if (pos < a.cap_out) a.out[pos] = make_int2(c, j);
+                else atomicOr(a.overflow, 4);
+            }
+        } else {
+            for (int o = 0; o < 8; ++o) {
+                const int c = a.children[j * 8 + o];
+                if (c < 0) continue;
+                const int pos = atomicAdd(a.n_out, 1);
+                if (pos < a.cap_out) a.out[pos] = make_int2(i, c);
+                else atomicOr(a.overflow, 4);
+            }
+        }
+    }
+}
+
+// CSR offsets by target from sorted (target << 32 | source) keys.
+__global__ __launch_bounds__(256) void k_csr(const unsigned long long* __restrict__ keys, int nk, int nt,
+                                             int* __restrict__ off, int* __restrict__ src) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nk == 0) {
+        for (int x = e; x <= nt; x += gridDim.x * blockDim.x) off[x] = 0;
+        return;
+    }
+    if (e >= nk) return;
+    const int t = (int)(keys[e] >> 32);
+    const int tp = e == 0 ? -1 : (int)(keys[e - 1] >> 32);
+    for (int x = tp + 1; x <= t; ++x) off[x] = e;
+    src[e] = (int)(keys[e] & 0xffffffffull);
+    if (e == nk - 1)
+        for (int x = t + 1; x <= nt; ++x) off[x] = nk;
+}
+
+static inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
+
+void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, hipStream_t s) {
+    hipLaunchKernelGGL(k_centroid, dim3(3), dim3(64), 0, s, X, ldx, n, t.root);
+    FMMH_HIP(hipMemsetAsync(t.maxabs, 0, 3 * sizeof(unsigned long long), s));
+    int g = std::min(blocks_for(n, 256), 1024);
+    hipLaunchKernelGGL(k_maxabs, dim3(g), dim3(256), 0, s, X, ldx, n, t.root, t.maxabs);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_keys(const double* X, int64_t ldx, int n, TreeDev& t, unsigned long long* keys_out, int* idx_out,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, X, ldx, n, t.root, t.maxabs, keys_out,
+                       idx_out);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_site_levels(int n, int ncrit, TreeDev& t, hipStream_t s) {
+    hipLaunchKernelGGL(k_cpl, dim3(blocks_for(n, 256)), dim3(256), 0, s, t.keys, n, ncrit, t.lmin, t.w);
+    hipLaunchKernelGGL(k_lambda, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, ncrit, t.lmin, t.w, t.lam, t.cnt,
+                       t.info);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_emit_nodes(int n, TreeDev& t, hipStream_t s) {
+    hipLaunchKernelGGL(k_emit_nodes, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, t);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_node_links(int num_nodes, TreeDev& t, hipStream_t s) {
+    hipLaunchKernelGGL(k_node_links, dim3(blocks_for(num_nodes, 256)), dim3(256), 0, s, num_nodes, t);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_leaf_index(int num_nodes, const int* leaf_scan, TreeDev& t, hipStream_t s) {
+    hipLaunchKernelGGL(k_leaf_index, dim3(blocks_for(num_nodes, 256)), dim3(256), 0, s, num_nodes, leaf_scan, t);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_level_scatter(int num_nodes, const int* level_off, TreeDev& t, hipStream_t s) {
+    hipLaunchKernelGGL(k_level_scatter, dim3(blocks_for(num_nodes, 256)), dim3(256), 0, s, num_nodes, level_off, t);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_leaf_sort(int num_leaves, int max_leaf, TreeDev& t, hipStream_t s) {
+    if (num_leaves <= 0) return;
+    hipLaunchKernelGGL(k_leaf_sort, dim3(num_leaves), dim3(64), max_leaf * sizeof(int), s, t);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_gather(const SiteArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_traverse(const TraverseArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(256), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_csr(const unsigned long long* keys, int n_keys, int num_targets, int* off, int* src, hipStream_t s) {
+    int g = n_keys == 0 ? blocks_for(num_targets + 1, 256) : blocks_for(n_keys, 256);
+    hipLaunchKernelGGL(k_csr, dim3(g), dim3(256), 0, s, keys, n_keys, num_targets, off, src);
+    FMMH_HIP(hipGetLastError());
+}
+
+}  // namespace fmmh

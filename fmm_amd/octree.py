@@ -1,0 +1,213 @@
+"""Python face of the field engine, mirroring the f2py module `octree` that
+peter-reinholdt/fmm builds (CMakeLists.txt:21-39, `only: field_fmm
+field_direct`).  As with f2py, the intent(out) `field` is returned rather
+than passed, and arrays may be any numpy layout (they are copied to
+Fortran order).
+
+    field = field_direct(comm, coordinates, multipoles, exclusions, field_order,
+                         damp_type, damping_factors)
+    field = field_fmm(comm, coordinates, multipoles, exclusions, theta, ncrit,
+                      expansion_order, field_order, damp_type, damping_factors,
+                      is_periodic, box_matrix)
+
+Errors raise FmmError carrying the reference's messages ("Unknown damp
+type: ...", src/fmm.F90:594, 1376).  All compute runs in libfmm_hip.so on the
+GPU; there is no CPU path.
+
+`Context` adds what f2py cannot express: a dedicated stream with pooled
+device memory, device-resident calls on torch tensors (`*_device`), phase
+timers and tree export.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FmmError, check
+
+__all__ = ["field_direct", "field_fmm", "Context", "FmmError", "ncomp"]
+
+DAMP = {"": 0, "THOLE": 1, "AMOEBA": 2, "ERF": 3}
+PHASE_CAP = 64
+
+
+def ncomp(k):
+    return (k + 1) * (k + 2) * (k + 3) // 6
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _damp_code(damp_type):
+    lib = _lib.load()
+    raw = damp_type.encode() if isinstance(damp_type, str) else bytes(damp_type)
+    code = ctypes.c_int32(0)
+    check(lib.fmmh_parse_damp_type(raw, len(raw), ctypes.byref(code)))
+    return code.value
+
+
+def _prep(coordinates, multipoles, exclusions, damping_factors):
+    X = np.asfortranarray(coordinates, dtype=np.float64)
+    M = np.asfortranarray(multipoles, dtype=np.float64)
+    E = np.asfortranarray(exclusions, dtype=np.int32)
+    if E.ndim == 1:
+        E = E.reshape(-1, 1, order="F")
+    if X.ndim != 2 or X.shape[1] != 3:
+        raise ValueError("coordinates must be N x 3")
+    if M.ndim != 2 or M.shape[0] != X.shape[0]:
+        raise ValueError("multipoles must be N x dim")
+    if E.shape[0] != X.shape[0]:
+        raise ValueError("exclusions must be N x nexcl")
+    df = np.ascontiguousarray(damping_factors, dtype=np.float64).reshape(-1)
+    return X, M, E, df
+
+
+class Context:
+    """One HIP stream plus pooled device buffers on one device."""
+
+    def __init__(self, device=-1):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._lib.fmmh_context_create(ctypes.byref(h), device))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            check(self._lib.fmmh_context_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- host arrays ---------------------------------------------------------
+    def field_direct(self, comm, coordinates, multipoles, exclusions, field_order, damp_type="",
+                     damping_factors=(), field=None):
+        X, M, E, df = _prep(coordinates, multipoles, exclusions, damping_factors)
+        n, dim = M.shape
+        code = _damp_code(damp_type)
+        F = np.zeros((n, ncomp(field_order)), order="F") if field is None else field
+        check(self._lib.fmmh_field_direct(self._h, int(comm), _p(X), n, _p(M), n, n, dim, _p(E), n, E.shape[1],
+                                          int(field_order), _p(F), F.shape[0], code,
+                                          _p(df) if df.size else None, df.size))
+        return F
+
+    def field_fmm(self, comm, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order,
+                  damp_type="", damping_factors=(), is_periodic=False, box_matrix=None, field=None):
+        X, M, E, df = _prep(coordinates, multipoles, exclusions, damping_factors)
+        n, dim = M.shape
+        code = _damp_code(damp_type)
+        F = np.zeros((n, ncomp(field_order)), order="F") if field is None else field
+        box = np.zeros((3, 3), order="F") if box_matrix is None else np.asfortranarray(box_matrix, np.float64)
+        check(self._lib.fmmh_field_fmm(self._h, int(comm), _p(X), n, _p(M), n, n, dim, _p(E), n, E.shape[1],
+                                       float(theta), int(ncrit), int(expansion_order), int(field_order), _p(F),
+                                       F.shape[0], code, _p(df) if df.size else None, df.size,
+                                       1 if is_periodic else 0, _p(box)))
+        return F
+
+    # -- device arrays (torch tensors already in HBM) --------------------------
+    @staticmethod
+    def _dptr(t):
+        return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else None
+
+    @staticmethod
+    def _ld(t):
+        """Leading dimension of a column-major (Fortran-order) 2-D tensor."""
+        if t.dim() == 1:
+            return t.shape[0]
+        if t.stride(0) != 1:
+            raise ValueError("device arrays must be column-major: use t.T.contiguous().T")
+        return t.stride(1) if t.shape[1] > 1 else t.shape[0]
+
+    def field_direct_device(self, coordinates, multipoles, exclusions, field_order, field, damp_type="",
+                            damping_factors=None):
+        n, dim = multipoles.shape
+        nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
+        ndf = damping_factors.numel() if damping_factors is not None else 0
+        check(self._lib.fmmh_field_direct_dev(
+            self._h, 0, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles),
+            self._ld(multipoles), n, dim, self._dptr(exclusions), self._ld(exclusions), nexcl, int(field_order),
+            self._dptr(field), self._ld(field), _damp_code(damp_type), self._dptr(damping_factors), ndf))
+        return field
+
+    def field_fmm_device(self, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order,
+                         field, damp_type="", damping_factors=None):
+        n, dim = multipoles.shape
+        nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
+        ndf = damping_factors.numel() if damping_factors is not None else 0
+        check(self._lib.fmmh_field_fmm_dev(
+            self._h, 0, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles),
+            self._ld(multipoles), n, dim, self._dptr(exclusions), self._ld(exclusions), nexcl, float(theta),
+            int(ncrit), int(expansion_order), int(field_order), self._dptr(field), self._ld(field),
+            _damp_code(damp_type), self._dptr(damping_factors), ndf, 0, None))
+        return field
+
+    # -- diagnostics -----------------------------------------------------------
+    def set_timing(self, on=True):
+        check(self._lib.fmmh_set_option(self._h, 1, 1.0 if on else 0.0))
+
+    def phase_times(self):
+        ms = np.zeros(PHASE_CAP)
+        names = ctypes.create_string_buffer(4096)
+        cnt = self._lib.fmmh_phase_times(self._h, _p(ms), PHASE_CAP, names, 4096)
+        if cnt < 0:
+            check(-cnt)
+        keys = names.value.decode().split(";") if names.value else []
+        return dict(zip(keys, ms[:cnt].tolist()))
+
+    def stats(self):
+        out = np.zeros(6, dtype=np.int64)
+        check(self._lib.fmmh_stats(self._h, _p(out), 6))
+        keys = ("num_nodes", "num_leaves", "max_depth", "n_m2l", "n_p2p", "p2p_site_pairs")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def tree(self, coordinates, ncrit, theta):
+        """Octree + interaction lists (pre-order ids, 0-based) as numpy arrays."""
+        X = np.asfortranarray(coordinates, dtype=np.float64)
+        n = X.shape[0]
+        info = np.zeros(5, dtype=np.int64)
+        check(self._lib.fmmh_tree_build(self._h, _p(X), n, n, int(ncrit), float(theta), _p(info)))
+        nn, nl, md, nm2l, np2p = (int(v) for v in info)
+        out = dict(centers=np.zeros((nn, 3), order="F"), r=np.zeros(nn), rmax=np.zeros(nn),
+                   depth=np.zeros(nn, np.int32), parent=np.zeros(nn, np.int32), is_leaf=np.zeros(nn, np.int32),
+                   nleaf=np.zeros(nn, np.int32), mem_off=np.zeros(nn + 1, np.int32),
+                   members=np.zeros(n, np.int32), m2l_off=np.zeros(nn + 1, np.int32),
+                   m2l_src=np.zeros(max(nm2l, 1), np.int32), p2p_off=np.zeros(nn + 1, np.int32),
+                   p2p_src=np.zeros(max(np2p, 1), np.int32))
+        check(self._lib.fmmh_tree_export(self._h, *[_p(out[k]) for k in (
+            "centers", "r", "rmax", "depth", "parent", "is_leaf", "nleaf", "mem_off", "members", "m2l_off",
+            "m2l_src", "p2p_off", "p2p_src")]))
+        out["m2l_src"] = out["m2l_src"][:nm2l]
+        out["p2p_src"] = out["p2p_src"][:np2p]
+        out["max_depth"] = md
+        out["num_leaves"] = nl
+        return out
+
+
+_default = None
+
+
+def _ctx():
+    global _default
+    if _default is None:
+        _default = Context()
+    return _default
+
+
+def field_direct(comm, coordinates, multipoles, exclusions, field_order, damp_type="", damping_factors=()):
+    """Drop-in for octree.field_direct (src/fmm.F90:1338-1352)."""
+    return _ctx().field_direct(comm, coordinates, multipoles, exclusions, field_order, damp_type, damping_factors)
+
+
+def field_fmm(comm, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order, damp_type="",
+              damping_factors=(), is_periodic=False, box_matrix=None):
+    """Drop-in for octree.field_fmm (src/fmm.F90:1354-1400)."""
+    return _ctx().field_fmm(comm, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order,
+                            damp_type, damping_factors, is_periodic, box_matrix)

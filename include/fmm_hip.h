@@ -1,0 +1,147 @@
+/*
+ * fmm_hip.h -- C ABI of the MI355X (gfx950) field engine.
+ *
+ * Drop-in for the field engine of peter-reinholdt/fmm: the Fortran module
+ * procedures `field_direct` (src/fmm.F90:1338-1352) and `field_fmm`
+ * (src/fmm.F90:1354-1400).  A Fortran caller uses the module `fmm` of
+ * fmm_amd/fortran/fmm_hip.F90, which keeps those signatures and forwards
+ * here through iso_c_binding; Python callers use fmm_amd.octree, which keeps
+ * the f2py `octree` signatures (CMakeLists.txt:21-39).
+ *
+ * Conventions shared by every entry point
+ *   - Arrays are column-major, as in Fortran.  coordinates is n x 3,
+ *     multipoles n x dim (dim = (m+1)(m+2)(m+3)/6 packed q,x,y,z,xx,xy,...),
+ *     exclusions n x nexcl (1-based site indices, 0 = padding), field
+ *     n x (F+1)(F+2)(F+3)/6.  ld_* is the leading dimension (>= n) of each.
+ *   - field is ACCUMULATED into (field += result), like the reference, whose
+ *     intent(out) dummy is never zeroed (src/fmm.F90:731, 813, 1113).
+ *   - Exclusions are honoured the reference's way: an excluded pair is
+ *     evaluated at displacement (1e10, 1e10, 1e10) (src/fmm.F90:714-721,
+ *     802-806).  Every site must list itself or its field is not finite,
+ *     exactly as in the reference.
+ *   - Damping (damp_type) applies to near-field pairs only; far-field
+ *     translations are undamped (src/fmm.F90:872-873).
+ *   - comm is accepted and ignored: one rank, one GPU.
+ *   - Calls are synchronous.  A context owns one HIP stream (default
+ *     priority) and pooled device memory; ctx = NULL uses a per-process
+ *     default context on the current device.  Calls on one context must not
+ *     overlap.
+ *   - Return value: FMMH_OK or an error code; fmmh_last_error() gives the
+ *     message of the calling thread's last failure.
+ */
+#ifndef FMM_HIP_H
+#define FMM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fmmh_status {
+    FMMH_OK = 0,
+    FMMH_ERR_ARG = 1,          /* invalid sizes, leading dimensions, exclusion index out of range */
+    FMMH_ERR_DAMP = 2,         /* "Unknown damp type: " -- src/fmm.F90:594, 1372-1377 */
+    FMMH_ERR_UNSUPPORTED = 3,  /* periodic boxes, orders outside the compiled range */
+    FMMH_ERR_HIP = 4,          /* HIP runtime failure (allocation, launch) */
+    FMMH_ERR_TREE = 5,         /* >= ncrit sites share one cell 21 levels down */
+    FMMH_ERR_INTERNAL = 6
+};
+
+enum fmmh_damp {
+    FMMH_DAMP_NONE = 0,        /* ''       */
+    FMMH_DAMP_THOLE = 1,       /* 'THOLE'  */
+    FMMH_DAMP_AMOEBA = 2,      /* 'AMOEBA' */
+    FMMH_DAMP_ERF = 3          /* 'ERF'    */
+};
+
+typedef struct fmmh_context fmmh_context;
+
+/* Context on HIP device `device` (-1: current device). */
+int fmmh_context_create(fmmh_context** ctx, int32_t device);
+int fmmh_context_destroy(fmmh_context* ctx);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* fmmh_last_error(void);
+
+/* damp_type as the reference compares it: trim(damp_type) against 'THOLE',
+ * 'AMOEBA', 'ERF', '' (case-sensitive, trailing blanks ignored;
+ * src/fmm.F90:585-595).  s need not be NUL-terminated.  Returns
+ * FMMH_ERR_DAMP with "Unknown damp type: <s>" for anything else. */
+int fmmh_parse_damp_type(const char* s, int64_t len, int32_t* code);
+
+/* Replaces field_direct (src/fmm.F90:1338-1352; all-pairs sum of
+ * particle_fields_direct, src/fmm.F90:749-824).  Host arrays.
+ * damping_factors needs n_damping >= n when damp_type != FMMH_DAMP_NONE. */
+int fmmh_field_direct(fmmh_context* ctx, int32_t comm,
+                      const double* coordinates, int64_t ld_coordinates,
+                      const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                      const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                      int32_t field_order, double* field, int64_t ld_field,
+                      int32_t damp_type, const double* damping_factors, int64_t n_damping);
+
+/* Replaces field_fmm (src/fmm.F90:1354-1400).  Host arrays.  n <= ncrit
+ * falls back to the direct sum (src/fmm.F90:1381-1383).  is_periodic != 0
+ * returns FMMH_ERR_UNSUPPORTED; box_matrix (3 x 3) may be NULL otherwise. */
+int fmmh_field_fmm(fmmh_context* ctx, int32_t comm,
+                   const double* coordinates, int64_t ld_coordinates,
+                   const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                   const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                   double theta, int32_t ncrit, int32_t expansion_order,
+                   int32_t field_order, double* field, int64_t ld_field,
+                   int32_t damp_type, const double* damping_factors, int64_t n_damping,
+                   int32_t is_periodic, const double* box_matrix);
+
+/* Device-resident forms of the two calls above: identical arguments, but
+ * every array pointer is device memory of the context's device.  No host
+ * copies; exclusion indices are not range-checked.  Used when the inputs are
+ * already in HBM (benchmarks, GPU-side callers). */
+int fmmh_field_direct_dev(fmmh_context* ctx, int32_t comm,
+                          const double* coordinates, int64_t ld_coordinates,
+                          const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                          const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                          int32_t field_order, double* field, int64_t ld_field,
+                          int32_t damp_type, const double* damping_factors, int64_t n_damping);
+
+int fmmh_field_fmm_dev(fmmh_context* ctx, int32_t comm,
+                       const double* coordinates, int64_t ld_coordinates,
+                       const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                       const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                       double theta, int32_t ncrit, int32_t expansion_order,
+                       int32_t field_order, double* field, int64_t ld_field,
+                       int32_t damp_type, const double* damping_factors, int64_t n_damping,
+                       int32_t is_periodic, const double* box_matrix);
+
+/* ---- diagnostics (no counterpart in the reference) -------------------------
+ * Octree and interaction lists of field_fmm for the given sites (host
+ * coordinates), exported for bit-exact comparison with the reference's
+ * tree (src/fmm.F90:137-385).  fmmh_tree_build keeps the result in the
+ * context and returns sizes in info[5] = {num_nodes, num_leaves, max_depth,
+ * n_m2l, n_p2p}; fmmh_tree_export then fills caller arrays (node ids are
+ * pre-order, 0-based; centers num_nodes x 3 column-major; parent -1 for the
+ * root; nleaf = sites in the subtree; members/mem_off list each leaf's sites
+ * (0-based) in the reference's order, offsets num_nodes + 1 with empty ranges
+ * for internal nodes; m2l/p2p lists as CSR by target node, ascending source). */
+int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates, int32_t n,
+                    int32_t ncrit, double theta, int64_t* info);
+int fmmh_tree_export(fmmh_context* ctx, double* centers, double* r, double* rmax, int32_t* depth,
+                     int32_t* parent, int32_t* is_leaf, int32_t* nleaf, int32_t* mem_off, int32_t* members,
+                     int32_t* m2l_off, int32_t* m2l_src, int32_t* p2p_off, int32_t* p2p_src);
+
+/* Per-phase device times (ms) of the context's last call, measured with HIP
+ * events on the context stream when timing is enabled.  names receives a
+ * ';'-separated list of phase names (capacity name_cap bytes).  Returns the
+ * number of phases written. */
+enum fmmh_option { FMMH_OPT_TIMING = 1 };
+int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
+int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
+
+/* Counters of the last field_fmm call: {num_nodes, num_leaves, max_depth,
+ * n_m2l, n_p2p, p2p_site_pairs}. */
+int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMM_HIP_H */

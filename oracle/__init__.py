@@ -1,0 +1,124 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes front end for
+  * liboracle.so: the C++ restatement of peter-reinholdt/fmm's field engine
+    (fmm_oracle.cpp), and
+  * _ref/libfmmref.so (+ _gnu variant): the reference Fortran sources compiled
+    unmodified with the bind(C) wrapper ref_bindc.F90.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker.  Build with `make -C oracle`.
+"""
+import ctypes
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DAMP = {"": 0, "THOLE": 1, "AMOEBA": 2, "ERF": 3}
+_libs = {}
+
+
+def _lib(name):
+    if name not in _libs:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        _libs[name] = ctypes.CDLL(path)
+    return _libs[name]
+
+
+def have_ref(variant=""):
+    return os.path.exists(os.path.join(_HERE, "_ref", f"libfmmref{variant}.so"))
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def ncomp(k):
+    return (k + 1) * (k + 2) * (k + 3) // 6
+
+
+def _prep(coordinates, multipoles, exclusions, damping_factors):
+    X = np.asfortranarray(coordinates, dtype=np.float64)
+    M = np.asfortranarray(multipoles, dtype=np.float64)
+    E = np.asfortranarray(exclusions, dtype=np.int32)
+    if E.ndim == 1:
+        E = E.reshape(-1, 1, order="F")
+    df = np.ascontiguousarray(damping_factors, dtype=np.float64)
+    if df.size == 0:
+        df = np.zeros(1)
+    return X, M, E, df
+
+
+def field_direct(coordinates, multipoles, exclusions, field_order, damp_type="", damping_factors=(),
+                 impl="oracle"):
+    """impl: 'oracle' (C++ restatement), 'ref' or 'ref_gnu' (compiled reference)."""
+    X, M, E, df = _prep(coordinates, multipoles, exclusions, damping_factors)
+    n, dim = M.shape
+    F = np.zeros((n, ncomp(field_order)), order="F")
+    d = DAMP[damp_type.strip()]
+    if impl == "oracle":
+        _lib("liboracle.so").orc_field_direct(n, dim, _p(X), _p(M), _p(E), E.shape[1], field_order, d, _p(df), _p(F))
+    else:
+        lib = _lib("_ref/libfmmref.so" if impl == "ref" else "_ref/libfmmref_gnu.so")
+        lib.ref_field_direct(n, dim, E.shape[1], _p(X), _p(M), _p(E), field_order, d, _p(df), _p(F))
+    return F
+
+
+def field_fmm(coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order, damp_type="",
+              damping_factors=(), impl="oracle"):
+    X, M, E, df = _prep(coordinates, multipoles, exclusions, damping_factors)
+    n, dim = M.shape
+    F = np.zeros((n, ncomp(field_order)), order="F")
+    d = DAMP[damp_type.strip()]
+    if impl == "oracle":
+        _lib("liboracle.so").orc_field_fmm(n, dim, _p(X), _p(M), _p(E), E.shape[1], ctypes.c_double(theta), ncrit,
+                                           expansion_order, field_order, d, _p(df), _p(F))
+    else:
+        lib = _lib("_ref/libfmmref.so" if impl == "ref" else "_ref/libfmmref_gnu.so")
+        lib.ref_field_fmm(n, dim, E.shape[1], _p(X), _p(M), _p(E), ctypes.c_double(theta), ncrit, expansion_order,
+                          field_order, d, _p(df), _p(F))
+    return F
+
+
+def tensor(K, damp_type, x, y, z, a=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    n = x.size
+    T = np.zeros((n, ncomp(K)), order="F")
+    aa = np.ascontiguousarray(a if a is not None else np.zeros(n), dtype=np.float64)
+    _lib("liboracle.so").orc_tensor(K, DAMP[damp_type], n, _p(x), _p(y), _p(z), _p(aa), _p(T))
+    return T
+
+
+def norm2(v):
+    lib = _lib("liboracle.so")
+    lib.orc_norm2.restype = ctypes.c_double
+    return lib.orc_norm2(_p(np.ascontiguousarray(v, dtype=np.float64)))
+
+
+def tree(coordinates, ncrit, theta):
+    """Tree and interaction lists of the restatement, pre-order ids, 0-based."""
+    lib = _lib("liboracle.so")
+    lib.orc_tree_build.restype = ctypes.c_void_p
+    X = np.asfortranarray(coordinates, dtype=np.float64)
+    n = X.shape[0]
+    h = ctypes.c_void_p(lib.orc_tree_build(n, _p(X), ncrit, ctypes.c_double(theta)))
+    info = np.zeros(5, dtype=np.int64)
+    lib.orc_tree_info(h, _p(info))
+    nn, nl, md, nm2l, np2p = (int(v) for v in info)
+    out = dict(centers=np.zeros((nn, 3), order="F"), r=np.zeros(nn), rmax=np.zeros(nn),
+               depth=np.zeros(nn, np.int32), parent=np.zeros(nn, np.int32), is_leaf=np.zeros(nn, np.int32),
+               nleaf=np.zeros(nn, np.int32), mem_off=np.zeros(nn + 1, np.int32), members=np.zeros(n, np.int32),
+               m2l_off=np.zeros(nn + 1, np.int32), m2l_src=np.zeros(max(nm2l, 1), np.int32),
+               p2p_off=np.zeros(nn + 1, np.int32), p2p_src=np.zeros(max(np2p, 1), np.int32))
+    lib.orc_tree_export(h, *[_p(out[k]) for k in ("centers", "r", "rmax", "depth", "parent", "is_leaf", "nleaf",
+                                                   "mem_off", "members", "m2l_off", "m2l_src", "p2p_off",
+                                                   "p2p_src")])
+    lib.orc_tree_free(h)
+    out["m2l_src"] = out["m2l_src"][:nm2l]
+    out["p2p_src"] = out["p2p_src"][:np2p]
+    out["max_depth"] = md
+    out["num_leaves"] = nl
+    return out

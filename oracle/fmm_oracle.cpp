@@ -1,0 +1,765 @@
+// ORACLE -- test infrastructure only.
+//
+// Serial FP64 CPU restatement of the field engine of peter-reinholdt/fmm
+// (`field_direct`, `field_fmm` in src/fmm.F90).  It exists to check the HIP
+// library; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it, and never as the thing measured or shipped.
+//
+// Each function cites the reference lines it follows.  Loop and summation
+// orders follow the reference wherever that is cheap, so the remaining
+// differences against the compiled reference are confined to the T-tensor
+// evaluation (closed forms there, a McMurchie-Davidson recursion over radial
+// derivatives here) and to libm's exp/erf.
+//
+// How each part is pinned (see tests/test_oracle_*.py):
+//   xyz2idx / packing ........ exhaustive enumeration vs the closed form
+//   T tensors (all kinds) .... compiled reference field_direct golden vectors,
+//                              finite differences, Laplace trace identity,
+//                              large-R limits, AMOEBA T0 == 0, orders > 6 == 0
+//   tree / lists ............. brute-force invariants (partition, leaf size,
+//                              MAC, pair completeness) and field_fmm parity
+//                              against the compiled reference (a flipped MAC
+//                              tie changes the field far above 1e-12)
+//   P2M/M2M/M2L/L2L/L2P/P2P .. field_fmm golden vectors from the compiled
+//                              reference, polynomial shift identities
+//   direct ................... golden vectors + closed-form 1e10 artefact
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <string>
+
+namespace orc {
+
+// ---------------------------------------------------------------- packing
+// src/fmm.F90:80-89 (xyz2idx), returned 0-based here.
+static inline int xyz2idx(int x, int y, int z) {
+    int k = x + y + z;
+    return k * (k + 1) * (k + 2) / 6 + (y * y + 2 * y * z + y + z * z + 3 * z) / 2;
+}
+static inline int ncomp(int k) { return (k + 1) * (k + 2) * (k + 3) / 6; }
+
+// src/fmm.F90:91-111, FP64 products as in the reference.
+static double factorial(int n) {
+    double f = 1.0;
+    for (int i = n; i >= 1; --i) f *= (double)i;
+    return f;
+}
+static double trinom(int i, int j, int k) {
+    return factorial(i + j + k) / (factorial(i) * factorial(j) * factorial(k));
+}
+static double binom(int n, int k) { return factorial(n) / (factorial(k) * factorial(n - k)); }
+
+// Components of total degree k in the reference's loop order
+// (sx descending, then sy descending; src/fmm.F90:599-608).
+static void degree_components(int k, std::vector<int>& xs, std::vector<int>& ys, std::vector<int>& zs) {
+    xs.clear(); ys.clear(); zs.clear();
+    for (int sx = k; sx >= 0; --sx)
+        for (int sy = k - sx; sy >= 0; --sy) {
+            xs.push_back(sx); ys.push_back(sy); zs.push_back(k - sx - sy);
+        }
+}
+
+// multipole order inferred from the number of columns, src/fmm.F90:567.
+static int order_from_dim(int dim) {
+    return (int)std::lround(std::pow((double)dim * 6.0, 1.0 / 3.0)) - 2;
+}
+
+// ---------------------------------------------------------------- tensors
+enum { DAMP_NONE = 0, DAMP_THOLE = 1, DAMP_AMOEBA = 2, DAMP_ERF = 3 };
+
+// Radial derivative sequence g_n(R) = ((1/R) d/dR)^n f(R), n = 0..K, for
+//   none   f = 1/R                              (src/tensors.F90:128-139)
+//   THOLE  f = (1 - (1 + aR/2) e^{-aR}) / R     (src/tensors_damped_thole.F90:77-78)
+//   AMOEBA f' = -(1 - e^{-a^3 R^3}) R / R^3, f := 0 (src/tensors_damped_amoeba.F90:77, 93-96)
+//   ERF    f = erf(sqrt(a) R) / R               (src/tensors_damped_erf.F90:77)
+// Written as g_n = (-1)^n (2n-1)!! R^{-(2n+1)} lambda_n with lambda_n from
+// polynomial recursions derived from those radial forms:
+//   THOLE  lambda_n = 1 - e^{-u} P_n(u), u = aR,     P_0 = 1 + u/2,
+//          P_{n+1} = P_n + u (P_n - P_n') / (2n+1)
+//   AMOEBA lambda_n = 1 - e^{-v} Q_n(v), v = a^3 R^3, Q_1 = 1,
+//          Q_{n+1} = Q_n + 3v (Q_n - Q_n') / (2n+1)
+//   ERF    lambda_n = erf(w) - (2/sqrt(pi)) e^{-w^2} S_n(w), w = sqrt(a) R, S_0 = 0,
+//          S_{n+1} = S_n + w (1 - S_n' + 2 w S_n) / (2n+1)
+static void poly_next(const std::vector<double>& p, std::vector<double>& q, int n, int kind) {
+    // returns q = next polynomial, coefficient vectors in ascending powers
+    int deg = (int)p.size();
+    std::vector<double> dp(std::max(deg - 1, 1), 0.0);
+    for (int i = 1; i < deg; ++i) dp[i - 1] = p[i] * i;
+    double inv = 1.0 / (2.0 * n + 1.0);
+    if (kind == DAMP_THOLE) {            // P + u (P - P')/(2n+1)
+        q.assign(deg + 1, 0.0);
+        for (int i = 0; i < deg; ++i) q[i] += p[i];
+        for (int i = 0; i < deg; ++i) q[i + 1] += p[i] * inv;
+        for (int i = 0; i < (int)dp.size(); ++i) q[i + 1] -= dp[i] * inv;
+    } else if (kind == DAMP_AMOEBA) {    // Q + 3v (Q - Q')/(2n+1)
+        q.assign(deg + 1, 0.0);
+        for (int i = 0; i < deg; ++i) q[i] += p[i];
+        for (int i = 0; i < deg; ++i) q[i + 1] += 3.0 * p[i] * inv;
+        for (int i = 0; i < (int)dp.size(); ++i) q[i + 1] -= 3.0 * dp[i] * inv;
+    } else {                             // S + w (1 - S' + 2 w S)/(2n+1)
+        q.assign(deg + 2, 0.0);
+        for (int i = 0; i < deg; ++i) q[i] += p[i];
+        q[1] += inv;
+        for (int i = 0; i < (int)dp.size(); ++i) q[i + 1] -= dp[i] * inv;
+        for (int i = 0; i < deg; ++i) q[i + 2] += 2.0 * p[i] * inv;
+    }
+}
+static double poly_eval(const std::vector<double>& p, double x) {
+    double s = 0.0;
+    for (int i = (int)p.size() - 1; i >= 0; --i) s = s * x + p[i];
+    return s;
+}
+
+static void radial(int K, int damp, double R2, double a, double* g) {
+    double R = std::sqrt(R2);
+    double inv = 1.0 / R;
+    double inv2 = inv * inv;
+    double b = inv;                       // (-1)^n (2n-1)!! R^{-(2n+1)}
+    std::vector<double> base(K + 1);
+    for (int n = 0; n <= K; ++n) {
+        if (n > 0) b *= -(2.0 * n - 1.0) * inv2;
+        base[n] = b;
+    }
+    if (damp == DAMP_NONE) {
+        for (int n = 0; n <= K; ++n) g[n] = base[n];
+        return;
+    }
+    if (damp == DAMP_THOLE) {
+        double u = a * R, e = std::exp(-u);
+        std::vector<double> P = {1.0, 0.5}, Q;
+        for (int n = 0; n <= K; ++n) {
+            g[n] = base[n] * (1.0 - e * poly_eval(P, u));
+            poly_next(P, Q, n, DAMP_THOLE); P.swap(Q);
+        }
+        return;
+    }
+    if (damp == DAMP_AMOEBA) {
+        double v = a * a * a * std::pow(R2, 1.5), e = std::exp(-v);
+        std::vector<double> P = {1.0}, Q;
+        g[0] = 0.0;
+        for (int n = 1; n <= K; ++n) {
+            g[n] = base[n] * (1.0 - e * poly_eval(P, v));
+            poly_next(P, Q, n, DAMP_AMOEBA); P.swap(Q);
+        }
+        return;
+    }
+    // ERF
+    double w = std::sqrt(a) * R, e = std::exp(-w * w), er = std::erf(w);
+    const double two_over_sqrtpi = 1.1283791670955126;
+    std::vector<double> P = {0.0}, Q;
+    for (int n = 0; n <= K; ++n) {
+        g[n] = base[n] * (er - two_over_sqrtpi * e * poly_eval(P, w));
+        poly_next(P, Q, n, DAMP_ERF); P.swap(Q);
+    }
+}
+
+// Cartesian derivatives T_{tuv} = d^t/dx^t d^u/dy^u d^v/dz^v f(R) for all
+// t+u+v <= K, packed as in src/fmm.F90:568-575, from the Hermite-type
+// recursion A^{(n)}_{t+1,u,v} = t A^{(n+1)}_{t-1,u,v} + x A^{(n+1)}_{t,u,v},
+// A^{(n)}_{000} = g_n.  Damped kinds stop at order 6 like
+// src/tensors_damped_*.F90:55-62 (higher components stay 0, src/fmm.F90:577).
+static void tensor(int K, int damp, double x, double y, double z, double a, double* T) {
+    int nT = ncomp(K);
+    std::vector<double> g(K + 1);
+    radial(K, damp, x * x + y * y + z * z, a, g.data());
+    std::vector<double> A((size_t)(K + 1) * nT, 0.0);
+    auto at = [&](int n, int i) -> double& { return A[(size_t)n * nT + i]; };
+    for (int n = 0; n <= K; ++n) at(n, 0) = g[n];
+    for (int k = 1; k <= K; ++k)
+        for (int t = k; t >= 0; --t)
+            for (int u = k - t; u >= 0; --u) {
+                int v = k - t - u;
+                int i = xyz2idx(t, u, v);
+                for (int n = 0; n <= K - k; ++n) {
+                    double val;
+                    if (t > 0) {
+                        val = x * at(n + 1, xyz2idx(t - 1, u, v));
+                        if (t > 1) val += (t - 1) * at(n + 1, xyz2idx(t - 2, u, v));
+                    } else if (u > 0) {
+                        val = y * at(n + 1, xyz2idx(t, u - 1, v));
+                        if (u > 1) val += (u - 1) * at(n + 1, xyz2idx(t, u - 2, v));
+                    } else {
+                        val = z * at(n + 1, xyz2idx(t, u, v - 1));
+                        if (v > 1) val += (v - 1) * at(n + 1, xyz2idx(t, u, v - 2));
+                    }
+                    at(n, i) = val;
+                }
+            }
+    for (int i = 0; i < nT; ++i) T[i] = at(0, i);
+    if (damp != DAMP_NONE && K > 6)
+        for (int i = ncomp(6); i < nT; ++i) T[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- multipole_field
+// src/fmm.F90:548-621.  Field up to max_field_order at one point from ns
+// sources.  M is ns x dim (column-major, leading dimension ldm); dx,dy,dz,a
+// have ns entries.  Returns F (ncomp(max_field_order)).
+static void multipole_field(const double* M, int ldm, int dim, int ns, const double* dx, const double* dy,
+                            const double* dz, int max_field_order, int damp, const double* a, double* F) {
+    int m = order_from_dim(dim);
+    int K = max_field_order + m;
+    int nT = ncomp(K);
+    int nf = ncomp(max_field_order);
+    for (int i = 0; i < nf; ++i) F[i] = 0.0;
+    std::vector<double> T((size_t)ns * nT);
+    for (int j = 0; j < ns; ++j) tensor(K, damp, dx[j], dy[j], dz[j], damp ? a[j] : 0.0, &T[(size_t)j * nT]);
+    std::vector<int> sxv, syv, szv, txv, tyv, tzv;
+    for (int fo = 0; fo <= max_field_order; ++fo) {
+        degree_components(fo, txv, tyv, tzv);
+        for (int mo = 0; mo <= m; ++mo) {
+            double taylor = ((mo + 1) % 2 ? -1.0 : 1.0) / factorial(mo);
+            degree_components(mo, sxv, syv, szv);
+            for (size_t si = 0; si < sxv.size(); ++si) {
+                int sx = sxv[si], sy = syv[si], sz = szv[si];
+                int midx = xyz2idx(sx, sy, sz);
+                for (size_t ti = 0; ti < txv.size(); ++ti) {
+                    int tx = txv[ti], ty = tyv[ti], tz = tzv[ti];
+                    int fidx = xyz2idx(tx, ty, tz);
+                    int tidx = xyz2idx(sx + tx, sy + ty, sz + tz);
+                    double symfac = trinom(sx, sy, sz);
+                    double s = 0.0;
+                    for (int j = 0; j < ns; ++j) s += M[(size_t)midx * ldm + j] * T[(size_t)j * nT + tidx];
+                    F[fidx] += taylor * symfac * s;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- tree
+struct Node {
+    double c[3];
+    double r, rmax;
+    int depth;
+    int parent;              // node index (pre-order id) or -1
+    int child[8];            // child node index or -1
+    bool terminal;
+    std::vector<int> leaf;   // member sites (0-based), insertion order
+    int nleaf;
+    int idx;                 // pre-order id (0-based)
+};
+
+struct Tree {
+    std::vector<Node> nodes;      // creation order
+    std::vector<int> order;       // pre-order id -> creation index
+    int max_depth = 0;
+    int ncrit = 0;
+    double theta = 0.0;
+    std::vector<std::vector<int>> m2l, p2p;   // by pre-order id, source pre-order ids
+};
+
+// flang's NORM2 on a 3-vector, matched bit-for-bit by a black-box probe of
+// programs compiled here: running maximum with a scaled sum of squares,
+// result max * sqrt(1 + sum).  Used by the MAC (src/fmm.F90:363).
+static double norm2_3(const double v[3]) {
+    double mx = 0.0, s = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double a = std::fabs(v[k]);
+        if (mx == 0.0) {
+            mx = a;
+        } else if (a > mx) {
+            double t = mx / a;
+            double tsq = t * t;
+            s *= tsq;
+            s += tsq;
+            mx = a;
+        } else {
+            double t = a / mx;
+            s += t * t;
+        }
+    }
+    return mx * std::sqrt(1.0 + s);
+}
+
+// src/fmm.F90:221-245
+static int add_child(Tree& tr, int ni, int octant) {
+    Node ch;
+    Node& nd = tr.nodes[ni];
+    double r = nd.r * 0.5;
+    ch.c[0] = nd.c[0] + r * (double)((octant & 1) * 2 - 1);
+    ch.c[1] = nd.c[1] + r * (double)((octant & 2) - 1);
+    ch.c[2] = nd.c[2] + r * (double)((octant & 4) / 2 - 1);
+    ch.r = r;
+    ch.rmax = std::sqrt(1.5 * r * r);
+    ch.depth = nd.depth + 1;
+    ch.parent = ni;               // creation index until renumbered
+    for (int o = 0; o < 8; ++o) ch.child[o] = -1;
+    ch.terminal = true;
+    ch.nleaf = 0;
+    ch.idx = -1;
+    tr.nodes[ni].terminal = false;
+    tr.nodes.push_back(ch);
+    int ci = (int)tr.nodes.size() - 1;
+    tr.nodes[ni].child[octant] = ci;
+    return ci;
+}
+
+static int octant_of(const double* X, int ldx, int i, const double c[3]) {
+    int o = 0;
+    if (X[i] > c[0]) o += 1;
+    if (X[ldx + i] > c[1]) o += 2;
+    if (X[2 * (size_t)ldx + i] > c[2]) o += 4;
+    return o;
+}
+
+// src/fmm.F90:247-273
+static void split_node(Tree& tr, int ni, const double* X, int ldx) {
+    for (int k = 0; k < tr.nodes[ni].nleaf; ++k) {
+        int site = tr.nodes[ni].leaf[k];
+        int o = octant_of(X, ldx, site, tr.nodes[ni].c);
+        if (tr.nodes[ni].child[o] < 0) add_child(tr, ni, o);
+        int ci = tr.nodes[ni].child[o];
+        tr.nodes[ci].nleaf += 1;
+        tr.nodes[ci].leaf.push_back(site);
+        if (tr.nodes[ci].nleaf >= tr.ncrit) split_node(tr, ci, X, ldx);
+    }
+}
+
+// src/fmm.F90:286-299 (pre-order ids, max_depth)
+static void downward_pass(Tree& tr, int ni) {
+    tr.max_depth = std::max(tr.max_depth, tr.nodes[ni].depth);
+    tr.nodes[ni].idx = (int)tr.order.size();
+    tr.order.push_back(ni);
+    for (int o = 0; o < 8; ++o)
+        if (tr.nodes[ni].child[o] >= 0) downward_pass(tr, tr.nodes[ni].child[o]);
+}
+
+// src/fmm.F90:137-219
+static void octree_build(Tree& tr, const double* X, int ldx, int n, int ncrit, double theta) {
+    tr.ncrit = ncrit;
+    tr.theta = theta;
+    tr.nodes.clear();
+    tr.nodes.reserve(2 * (size_t)n / std::max(ncrit, 1) * 8 + 16);
+    Node root;
+    for (int o = 0; o < 8; ++o) root.child[o] = -1;
+    root.terminal = true;
+    root.nleaf = 0;
+    root.depth = 1;
+    root.parent = -1;
+    root.idx = -1;
+    for (int k = 0; k < 3; ++k) {            // sum(coordinates, 1) / N, sequential
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += X[(size_t)k * ldx + i];
+        root.c[k] = s / (double)n;
+    }
+    double mx[3];
+    for (int k = 0; k < 3; ++k) {
+        double m = 0.0;
+        bool first = true;
+        for (int i = 0; i < n; ++i) {
+            double d = std::fabs(X[(size_t)k * ldx + i] - root.c[k]);
+            if (first || d > m) { m = d; first = false; }
+        }
+        mx[k] = m;
+    }
+    root.r = std::max(mx[0], std::max(mx[1], mx[2])) * 1.0001;
+    root.rmax = std::sqrt(1.5 * root.r * root.r);
+    tr.nodes.push_back(root);
+    for (int i = 0; i < n; ++i) {
+        int ni = 0;
+        while (tr.nodes[ni].nleaf >= ncrit) {
+            tr.nodes[ni].nleaf += 1;
+            int o = octant_of(X, ldx, i, tr.nodes[ni].c);
+            if (tr.nodes[ni].child[o] < 0) add_child(tr, ni, o);
+            ni = tr.nodes[ni].child[o];
+        }
+        tr.nodes[ni].nleaf += 1;
+        tr.nodes[ni].leaf.push_back(i);
+        if (tr.nodes[ni].nleaf >= ncrit) split_node(tr, ni, X, ldx);
+    }
+    tr.order.clear();
+    tr.max_depth = 0;
+    downward_pass(tr, 0);
+}
+
+static Node& node_by_id(Tree& tr, int id) { return tr.nodes[tr.order[id]]; }
+
+// src/fmm.F90:352-385 (non-periodic: displacement 0)
+static void interact(Tree& tr, int ci, int cj) {
+    Node& a = tr.nodes[ci];
+    Node& b = tr.nodes[cj];
+    double delta[3];
+    for (int k = 0; k < 3; ++k) delta[k] = a.c[k] - (b.c[k] + 0.0);
+    double r = norm2_3(delta);
+    if (r * tr.theta > a.rmax + b.rmax) {
+        tr.m2l[a.idx].push_back(b.idx);
+    } else if (a.terminal && b.terminal) {
+        tr.p2p[a.idx].push_back(b.idx);
+    } else if (b.terminal || ((a.rmax > b.rmax) && !a.terminal)) {
+        for (int o = 0; o < 8; ++o)
+            if (tr.nodes[ci].child[o] >= 0) interact(tr, tr.nodes[ci].child[o], cj);
+    } else {
+        for (int o = 0; o < 8; ++o)
+            if (tr.nodes[cj].child[o] >= 0) interact(tr, ci, tr.nodes[cj].child[o]);
+    }
+}
+
+// src/fmm.F90:325-350
+static void build_lists(Tree& tr) {
+    size_t nn = tr.order.size();
+    tr.m2l.assign(nn, {});
+    tr.p2p.assign(nn, {});
+    interact(tr, 0, 0);
+}
+
+// ---------------------------------------------------------------- upward pass
+// src/fmm.F90:437-472 (P2M), monomials over the leaf members.
+static void shift_multipole_vec(const std::vector<const double*>& cols, int nsrc_order_dim, int ns,
+                                double* target, const double* dx, const double* dy, const double* dz, int p) {
+    int m = order_from_dim(nsrc_order_dim);
+    std::vector<int> sxv, syv, szv, txv, tyv, tzv;
+    std::vector<double> mono(ns);
+    for (int so = 0; so <= m; ++so) {
+        degree_components(so, sxv, syv, szv);
+        for (int to = so; to <= p; ++to) {
+            for (size_t si = 0; si < sxv.size(); ++si) {
+                int sx = sxv[si], sy = syv[si], sz = szv[si];
+                int sidx = xyz2idx(sx, sy, sz);
+                degree_components(to, txv, tyv, tzv);
+                for (size_t ti = 0; ti < txv.size(); ++ti) {
+                    int tx = txv[ti], ty = tyv[ti], tz = tzv[ti];
+                    if (tx < sx || ty < sy || tz < sz) continue;
+                    int tidx = xyz2idx(tx, ty, tz);
+                    double symfac = binom(tx, sx) * binom(ty, sy) * binom(tz, sz);
+                    double s = 0.0;
+                    for (int j = 0; j < ns; ++j) {
+                        mono[j] = std::pow(dx[j], tx - sx) * std::pow(dy[j], ty - sy) * std::pow(dz[j], tz - sz);
+                        s += mono[j] * cols[sidx][j];
+                    }
+                    target[tidx] += symfac * s;
+                }
+            }
+        }
+    }
+}
+
+// src/fmm.F90:387-423 (M2M kernel, one source expansion into one target)
+static void multipole_accumulate_kernel(const double d[3], const double* src, double* tgt, int p) {
+    std::vector<int> sxv, syv, szv, txv, tyv, tzv;
+    for (int so = 0; so <= p; ++so) {
+        degree_components(so, sxv, syv, szv);
+        for (int to = so; to <= p; ++to) {
+            for (size_t si = 0; si < sxv.size(); ++si) {
+                int sx = sxv[si], sy = syv[si], sz = szv[si];
+                int sidx = xyz2idx(sx, sy, sz);
+                degree_components(to, txv, tyv, tzv);
+                for (size_t ti = 0; ti < txv.size(); ++ti) {
+                    int tx = txv[ti], ty = tyv[ti], tz = tzv[ti];
+                    if (tx < sx || ty < sy || tz < sz) continue;
+                    int tidx = xyz2idx(tx, ty, tz);
+                    double symfac = binom(tx, sx) * binom(ty, sy) * binom(tz, sz);
+                    double mono = std::pow(d[0], tx - sx) * std::pow(d[1], ty - sy) * std::pow(d[2], tz - sz);
+                    tgt[tidx] += symfac * mono * src[sidx];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- engine
+struct Inputs {
+    int n, dim, nexcl, field_order, damp;
+    const double* X; int ldx;
+    const double* M; int ldm;
+    const int* E; int lde;
+    const double* df;
+    double* F; int ldf;
+};
+
+static bool excluded(const Inputs& in, int i, int j) {
+    for (int k = 0; k < in.nexcl; ++k)
+        if (in.E[(size_t)k * in.lde + i] == j + 1) return true;
+    return false;
+}
+
+// src/fmm.F90:749-824 (particle_fields_direct) via src/fmm.F90:1338-1352
+static void field_direct(const Inputs& in) {
+    int n = in.n;
+    int nf = ncomp(in.field_order);
+    std::vector<double> dx(n), dy(n), dz(n), a(n), F(nf);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            dx[j] = in.X[i] - in.X[j];
+            dy[j] = in.X[in.ldx + i] - in.X[in.ldx + j];
+            dz[j] = in.X[2 * (size_t)in.ldx + i] - in.X[2 * (size_t)in.ldx + j];
+        }
+        for (int k = 0; k < in.nexcl; ++k) {
+            int e = in.E[(size_t)k * in.lde + i];
+            if (e == 0) continue;
+            dx[e - 1] = 1e10; dy[e - 1] = 1e10; dz[e - 1] = 1e10;
+        }
+        if (in.damp) for (int j = 0; j < n; ++j) a[j] = in.df[i] * in.df[j];
+        multipole_field(in.M, in.ldm, in.dim, n, dx.data(), dy.data(), dz.data(), in.field_order, in.damp, a.data(), F.data());
+        for (int t = 0; t < nf; ++t) in.F[(size_t)t * in.ldf + i] += F[t];
+    }
+}
+
+// src/fmm.F90:474-546 (P2M + leaf->every-ancestor M2M)
+static void multipole_expansion(Tree& tr, const Inputs& in, int p, std::vector<double>& CM) {
+    int nm = ncomp(p);
+    size_t nn = tr.order.size();
+    CM.assign(nn * nm, 0.0);
+    std::vector<const double*> cols(in.dim);
+    for (size_t id = 0; id < nn; ++id) {
+        Node& nd = node_by_id(tr, (int)id);
+        if (!nd.terminal) continue;
+        int ns = nd.nleaf;
+        std::vector<double> dx(ns), dy(ns), dz(ns);
+        std::vector<std::vector<double>> g(in.dim, std::vector<double>(ns));
+        for (int j = 0; j < ns; ++j) {
+            int s = nd.leaf[j];
+            dx[j] = in.X[s] - nd.c[0];
+            dy[j] = in.X[in.ldx + s] - nd.c[1];
+            dz[j] = in.X[2 * (size_t)in.ldx + s] - nd.c[2];
+            for (int c = 0; c < in.dim; ++c) g[c][j] = in.M[(size_t)c * in.ldm + s];
+        }
+        for (int c = 0; c < in.dim; ++c) cols[c] = g[c].data();
+        shift_multipole_vec(cols, in.dim, ns, &CM[id * nm], dx.data(), dy.data(), dz.data(), p);
+    }
+    for (size_t id = 0; id < nn; ++id) {
+        Node& nd = node_by_id(tr, (int)id);
+        if (!nd.terminal) continue;
+        // multipole_accumulate(leaf, parent): recursion reaches the root first
+        std::vector<int> anc;
+        for (int a = nd.parent; a >= 0; a = tr.nodes[a].parent) anc.push_back(a);
+        for (int k = (int)anc.size() - 1; k >= 0; --k) {
+            Node& an = tr.nodes[anc[k]];
+            double d[3] = {nd.c[0] - an.c[0], nd.c[1] - an.c[1], nd.c[2] - an.c[2]};
+            multipole_accumulate_kernel(d, &CM[id * nm], &CM[(size_t)an.idx * nm], p);
+        }
+    }
+}
+
+// src/fmm.F90:623-747 (particle_fields), serial, non-periodic
+static void particle_fields(Tree& tr, const Inputs& in) {
+    int nf = ncomp(in.field_order);
+    std::vector<double> F(nf);
+    size_t nn = tr.order.size();
+    for (size_t id = 0; id < nn; ++id) {
+        Node& ni = node_by_id(tr, (int)id);
+        if (!ni.terminal) continue;
+        for (int li = 0; li < ni.nleaf; ++li) {
+            int i = ni.leaf[li];
+            for (int jid : tr.p2p[id]) {
+                Node& nj = node_by_id(tr, jid);
+                int ns = nj.nleaf;
+                std::vector<double> dx(ns), dy(ns), dz(ns), a(ns), Mj((size_t)ns * in.dim);
+                for (int jj = 0; jj < ns; ++jj) {
+                    int j = nj.leaf[jj];
+                    dx[jj] = in.X[i] - (in.X[j] + 0.0);
+                    dy[jj] = in.X[in.ldx + i] - (in.X[in.ldx + j] + 0.0);
+                    dz[jj] = in.X[2 * (size_t)in.ldx + i] - (in.X[2 * (size_t)in.ldx + j] + 0.0);
+                    if (excluded(in, i, j)) { dx[jj] = 1e10; dy[jj] = 1e10; dz[jj] = 1e10; }
+                    if (in.damp) a[jj] = in.df[i] * in.df[j];
+                    for (int c = 0; c < in.dim; ++c) Mj[(size_t)c * ns + jj] = in.M[(size_t)c * in.ldm + j];
+                }
+                multipole_field(Mj.data(), ns, in.dim, ns, dx.data(), dy.data(), dz.data(), in.field_order, in.damp,
+                                a.data(), F.data());
+                for (int t = 0; t < nf; ++t) in.F[(size_t)t * in.ldf + i] += F[t];
+            }
+        }
+    }
+}
+
+// src/fmm.F90:826-882 (multipole_to_local), undamped, field order p
+static void multipole_to_local(Tree& tr, int p, const std::vector<double>& CM, std::vector<double>& LE) {
+    int nm = ncomp(p);
+    size_t nn = tr.order.size();
+    LE.assign(nn * nm, 0.0);
+    std::vector<double> F(nm);
+    for (size_t id = 0; id < nn; ++id) {
+        Node& ni = node_by_id(tr, (int)id);
+        const std::vector<int>& lst = tr.m2l[id];
+        int ns = (int)lst.size();
+        std::vector<double> dx(ns), dy(ns), dz(ns), Ms((size_t)ns * nm);
+        for (int k = 0; k < ns; ++k) {
+            Node& nj = node_by_id(tr, lst[k]);
+            dx[k] = ni.c[0] - (nj.c[0] + 0.0);
+            dy[k] = ni.c[1] - (nj.c[1] + 0.0);
+            dz[k] = ni.c[2] - (nj.c[2] + 0.0);
+            for (int c = 0; c < nm; ++c) Ms[(size_t)c * ns + k] = CM[(size_t)lst[k] * nm + c];
+        }
+        multipole_field(Ms.data(), ns, nm, ns, dx.data(), dy.data(), dz.data(), p, DAMP_NONE, nullptr, F.data());
+        for (int c = 0; c < nm; ++c) LE[id * nm + c] += F[c];
+    }
+}
+
+// src/fmm.F90:981-1044 (local_to_local), depth by depth
+static void local_to_local(Tree& tr, int p, std::vector<double>& LE) {
+    int nm = ncomp(p);
+    size_t nn = tr.order.size();
+    std::vector<int> sxv, syv, szv, txv, tyv, tzv;
+    for (int depth = 2; depth <= tr.max_depth; ++depth) {
+        std::vector<int> tgt, src;
+        for (size_t id = 0; id < nn; ++id) {
+            Node& nd = node_by_id(tr, (int)id);
+            if (nd.depth == depth) { tgt.push_back((int)id); src.push_back(tr.nodes[nd.parent].idx); }
+        }
+        size_t cnt = tgt.size();
+        std::vector<double> dx(cnt), dy(cnt), dz(cnt);
+        for (size_t k = 0; k < cnt; ++k) {
+            Node& a = node_by_id(tr, tgt[k]);
+            Node& b = node_by_id(tr, src[k]);
+            dx[k] = a.c[0] - b.c[0]; dy[k] = a.c[1] - b.c[1]; dz[k] = a.c[2] - b.c[2];
+        }
+        for (int to = 0; to <= p; ++to) {
+            degree_components(to, txv, tyv, tzv);
+            for (size_t ti = 0; ti < txv.size(); ++ti) {
+                int tx = txv[ti], ty = tyv[ti], tz = tzv[ti];
+                int tidx = xyz2idx(tx, ty, tz);
+                for (int so = to; so <= p; ++so) {
+                    degree_components(so, sxv, syv, szv);
+                    for (size_t si = 0; si < sxv.size(); ++si) {
+                        int sx = sxv[si], sy = syv[si], sz = szv[si];
+                        if (sx < tx || sy < ty || sz < tz) continue;
+                        int sidx = xyz2idx(sx, sy, sz);
+                        double pref = 1.0 / (factorial(sx - tx) * factorial(sy - ty) * factorial(sz - tz));
+                        for (size_t k = 0; k < cnt; ++k)
+                            LE[(size_t)tgt[k] * nm + tidx] += pref * LE[(size_t)src[k] * nm + sidx] *
+                                (std::pow(dx[k], sx - tx) * std::pow(dy[k], sy - ty) * std::pow(dz[k], sz - tz));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// src/fmm.F90:1046-1132 (local_to_particle)
+static void local_to_particle(Tree& tr, const Inputs& in, int p, const std::vector<double>& LE) {
+    int nm = ncomp(p);
+    size_t nn = tr.order.size();
+    std::vector<int> sxv, syv, szv, txv, tyv, tzv;
+    for (size_t id = 0; id < nn; ++id) {
+        Node& nd = node_by_id(tr, (int)id);
+        if (!nd.terminal) continue;
+        int ns = nd.nleaf;
+        std::vector<double> dx(ns), dy(ns), dz(ns);
+        for (int j = 0; j < ns; ++j) {
+            int s = nd.leaf[j];
+            dx[j] = in.X[s] - nd.c[0];
+            dy[j] = in.X[in.ldx + s] - nd.c[1];
+            dz[j] = in.X[2 * (size_t)in.ldx + s] - nd.c[2];
+        }
+        for (int to = 0; to <= in.field_order; ++to) {
+            degree_components(to, txv, tyv, tzv);
+            for (size_t ti = 0; ti < txv.size(); ++ti) {
+                int tx = txv[ti], ty = tyv[ti], tz = tzv[ti];
+                int tidx = xyz2idx(tx, ty, tz);
+                for (int so = to; so <= p; ++so) {
+                    degree_components(so, sxv, syv, szv);
+                    for (size_t si = 0; si < sxv.size(); ++si) {
+                        int sx = sxv[si], sy = syv[si], sz = szv[si];
+                        if (sx < tx || sy < ty || sz < tz) continue;
+                        int sidx = xyz2idx(sx, sy, sz);
+                        double pref = 1.0 / (factorial(sx - tx) * factorial(sy - ty) * factorial(sz - tz));
+                        for (int j = 0; j < ns; ++j)
+                            in.F[(size_t)tidx * in.ldf + nd.leaf[j]] += pref * LE[id * nm + sidx] *
+                                std::pow(dx[j], sx - tx) * std::pow(dy[j], sy - ty) * std::pow(dz[j], sz - tz);
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace orc
+
+// ================================================================ C ABI (ctypes)
+extern "C" {
+
+// Column-major arrays as the Fortran interface: X n x 3, M n x dim,
+// E n x nexcl (1-based, 0-padded), F n x ncomp(field_order), accumulated.
+int orc_field_direct(int n, int dim, const double* X, const double* M, const int* E, int nexcl, int field_order,
+                     int damp, const double* df, double* F) {
+    orc::Inputs in{n, dim, nexcl, field_order, damp, X, n, M, n, E, n, df, F, n};
+    orc::field_direct(in);
+    return 0;
+}
+
+// src/fmm.F90:1354-1400, non-periodic.
+int orc_field_fmm(int n, int dim, const double* X, const double* M, const int* E, int nexcl, double theta, int ncrit,
+                  int p, int field_order, int damp, const double* df, double* F) {
+    orc::Inputs in{n, dim, nexcl, field_order, damp, X, n, M, n, E, n, df, F, n};
+    if (n <= ncrit) { orc::field_direct(in); return 0; }
+    orc::Tree tr;
+    orc::octree_build(tr, X, n, n, ncrit, theta);
+    std::vector<double> CM, LE;
+    orc::multipole_expansion(tr, in, p, CM);
+    orc::build_lists(tr);
+    orc::particle_fields(tr, in);
+    orc::multipole_to_local(tr, p, CM, LE);
+    orc::local_to_local(tr, p, LE);
+    orc::local_to_particle(tr, in, p, LE);
+    return 0;
+}
+
+// T tensor of one kind at n points, T is n x ncomp(K) column-major.
+int orc_tensor(int K, int damp, int n, const double* x, const double* y, const double* z, const double* a, double* T) {
+    int nT = orc::ncomp(K);
+    std::vector<double> t(nT);
+    for (int i = 0; i < n; ++i) {
+        orc::tensor(K, damp, x[i], y[i], z[i], a ? a[i] : 0.0, t.data());
+        for (int c = 0; c < nT; ++c) T[(size_t)c * n + i] = t[c];
+    }
+    return 0;
+}
+
+double orc_norm2(const double* v) { return orc::norm2_3(v); }
+
+// Tree + interaction lists, exported in pre-order with 0-based ids.
+struct OrcTree { orc::Tree tr; };
+
+void* orc_tree_build(int n, const double* X, int ncrit, double theta) {
+    OrcTree* h = new OrcTree;
+    orc::octree_build(h->tr, X, n, n, ncrit, theta);
+    orc::build_lists(h->tr);
+    return h;
+}
+
+// info: num_nodes, num_leaves, max_depth, n_m2l, n_p2p
+void orc_tree_info(void* hp, long long* info) {
+    OrcTree* h = (OrcTree*)hp;
+    long long nl = 0, nm = 0, np = 0;
+    size_t nn = h->tr.order.size();
+    for (size_t id = 0; id < nn; ++id) {
+        if (orc::node_by_id(h->tr, (int)id).terminal) nl++;
+        nm += h->tr.m2l[id].size();
+        np += h->tr.p2p[id].size();
+    }
+    info[0] = (long long)nn; info[1] = nl; info[2] = h->tr.max_depth; info[3] = nm; info[4] = np;
+}
+
+// centers nn x 3 (column-major), r, rmax, depth, parent id (-1 root),
+// is_leaf, nleaf (subtree count), member offsets (nn+1, leaves only
+// non-empty) and members (sites, 0-based, in leaf order), list CSR by target.
+void orc_tree_export(void* hp, double* centers, double* r, double* rmax, int* depth, int* parent, int* is_leaf,
+                     int* nleaf, int* mem_off, int* members, int* m2l_off, int* m2l_src, int* p2p_off, int* p2p_src) {
+    OrcTree* h = (OrcTree*)hp;
+    orc::Tree& tr = h->tr;
+    size_t nn = tr.order.size();
+    int mo = 0, lo = 0, po = 0;
+    for (size_t id = 0; id < nn; ++id) {
+        orc::Node& nd = orc::node_by_id(tr, (int)id);
+        for (int k = 0; k < 3; ++k) centers[k * nn + id] = nd.c[k];
+        r[id] = nd.r; rmax[id] = nd.rmax; depth[id] = nd.depth;
+        parent[id] = nd.parent >= 0 ? tr.nodes[nd.parent].idx : -1;
+        is_leaf[id] = nd.terminal ? 1 : 0;
+        nleaf[id] = nd.nleaf;
+        mem_off[id] = mo;
+        if (nd.terminal) for (int s : nd.leaf) members[mo++] = s;
+        m2l_off[id] = lo;
+        for (int s : tr.m2l[id]) m2l_src[lo++] = s;
+        p2p_off[id] = po;
+        for (int s : tr.p2p[id]) p2p_src[po++] = s;
+    }
+    mem_off[nn] = mo; m2l_off[nn] = lo; p2p_off[nn] = po;
+}
+
+void orc_tree_free(void* hp) { delete (OrcTree*)hp; }
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+The reference Fortran (peter-reinholdt/fmm src/*.F90) is compiled unmodified
+by oracle/Makefile into oracle/_ref/ (flang, the shipped -O1 -g flags); this
+script calls its field_direct / field_fmm through the bind(C) wrapper
+oracle/ref_bindc.F90 and stores inputs and outputs only.
+
+Run in the build container (it needs the reference sources):
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+from fmm_amd import synth  # noqa: E402
+
+
+def water_inputs(n_mol, dim, seed):
+    x, q, excl, df, rng = synth.water_box(n_mol, seed)
+    n = x.shape[0]
+    m = np.zeros((n, dim))
+    m[:, 0] = q
+    if dim > 1:
+        m[:, 1:4] = rng.normal(0.0, 0.05, (n, 3))
+    if dim > 4:
+        m[:, 4:] = rng.normal(0.0, 0.05, (n, dim - 4))
+    synth.traceless(m)
+    erf_df = rng.uniform(0.8, 1.2, n)
+    return x, np.asfortranarray(m), excl, df, erf_df
+
+
+def main():
+    assert oracle.have_ref() and oracle.have_ref("_gnu"), "build oracle/_ref first (make -C oracle ref)"
+    # 1. the CPU-runnable benchmark workload: 4096 uniform sites, quadrupoles
+    c = synth.config(0)
+    X, M, E = c["coordinates"], c["multipoles"], c["exclusions"]
+    fd = oracle.field_direct(X, M, E, 1, impl="ref")
+    ff = oracle.field_fmm(X, M, E, 0.5, 64, 5, 1, impl="ref")
+    np.savez_compressed(os.path.join(HERE, "uniform4096_p5.npz"), coordinates=X, multipoles=M, exclusions=E,
+                        theta=0.5, ncrit=64, expansion_order=5, field_order=1,
+                        field_direct=fd, field_fmm=ff)
+    # 2. 999-site water box, quadrupoles, every damping kind, field order 2
+    x, m, excl, df, erf_df = water_inputs(333, 10, 7)
+    out = dict(coordinates=x, multipoles=m, exclusions=excl, thole_df=df, erf_df=erf_df, theta=0.5, ncrit=64,
+               expansion_order=6, field_order=2)
+    for damp in ("", "THOLE", "AMOEBA", "ERF"):
+        d = erf_df if damp == "ERF" else df
+        key = damp.lower() or "none"
+        out[f"direct_{key}"] = oracle.field_direct(x, m, excl, 2, damp, d, impl="ref")
+        out[f"fmm_{key}"] = oracle.field_fmm(x, m, excl, 0.5, 64, 6, 2, damp, d, impl="ref")
+    np.savez_compressed(os.path.join(HERE, "water999_q.npz"), **out)
+    # 3. expansion-order sweep (undamped dipoles) incl. the orders where the
+    #    reference's int32 T-tensor coefficients overflow (p >= 7 with flang,
+    #    p >= 6 on the gfortran branch), and theta 0.6 / ncrit 16 variants
+    x, m, excl, df, _ = water_inputs(333, 4, 11)
+    out = dict(coordinates=x, multipoles=m, exclusions=excl)
+    out["direct"] = oracle.field_direct(x, m, excl, 1, impl="ref")
+    for p in (2, 3, 4, 5, 6, 7, 8):
+        out[f"fmm_p{p}"] = oracle.field_fmm(x, m, excl, 0.5, 64, p, 1, impl="ref")
+        out[f"fmm_gnu_p{p}"] = oracle.field_fmm(x, m, excl, 0.5, 64, p, 1, impl="ref_gnu")
+    out["fmm_theta06_ncrit16_p5"] = oracle.field_fmm(x, m, excl, 0.6, 16, 5, 1, impl="ref")
+    np.savez_compressed(os.path.join(HERE, "water999_dipole_orders.npz"), **out)
+    # 4. charges only, N <= ncrit fallback and a tiny tree
+    rng = np.random.default_rng(5)
+    for n, name in ((40, "tiny40"), (300, "small300")):
+        x = np.asfortranarray(rng.uniform(0, 10, (n, 3)))
+        q = rng.normal(0, 1, (n, 1))
+        q -= q.mean()
+        q = np.asfortranarray(q)
+        e = synth.self_exclusions(n)
+        np.savez_compressed(os.path.join(HERE, f"charges_{name}.npz"), coordinates=x, multipoles=q, exclusions=e,
+                            field_direct=oracle.field_direct(x, q, e, 2, impl="ref"),
+                            field_fmm=oracle.field_fmm(x, q, e, 0.5, 64, 4, 2, impl="ref"))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

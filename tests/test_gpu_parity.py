@@ -1,0 +1,163 @@
+"""HIP engine vs the reference's golden vectors and the oracle.  Every call
+goes through the C ABI of libfmm_hip.so (fmm_amd.octree -> ctypes).
+
+Bars: <= 1e-12 relative 2-norm error per output-order block for FP64 fields;
+exact equality for the tree and interaction lists (integer work and the FP64
+node-centre chain)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import block_rel, load_golden, rel
+from fmm_amd import synth
+
+pytestmark = pytest.mark.gpu
+PARITY = 1e-12
+
+
+def test_uniform4096_direct(gpu_ctx):
+    g = load_golden("uniform4096_p5.npz")
+    fd = gpu_ctx.field_direct(0, g["coordinates"], g["multipoles"], g["exclusions"], 1)
+    assert max(block_rel(fd, g["field_direct"], 1)) < PARITY
+
+
+def test_uniform4096_fmm(gpu_ctx):
+    g = load_golden("uniform4096_p5.npz")
+    ff = gpu_ctx.field_fmm(0, g["coordinates"], g["multipoles"], g["exclusions"], 0.5, 64, 5, 1)
+    assert max(block_rel(ff, g["field_fmm"], 1)) < PARITY
+
+
+@pytest.mark.parametrize("damp", ["", "THOLE", "AMOEBA", "ERF"])
+def test_water_direct_damped(gpu_ctx, damp):
+    g = load_golden("water999_q.npz")
+    df = g["erf_df"] if damp == "ERF" else g["thole_df"]
+    key = damp.lower() or "none"
+    fd = gpu_ctx.field_direct(0, g["coordinates"], g["multipoles"], g["exclusions"], 2, damp, df)
+    assert max(block_rel(fd, g[f"direct_{key}"], 2, damp == "AMOEBA")) < PARITY
+
+
+@pytest.mark.parametrize("damp", ["", "THOLE", "AMOEBA", "ERF"])
+def test_water_fmm_damped(gpu_ctx, damp):
+    g = load_golden("water999_q.npz")
+    df = g["erf_df"] if damp == "ERF" else g["thole_df"]
+    key = damp.lower() or "none"
+    ff = gpu_ctx.field_fmm(0, g["coordinates"], g["multipoles"], g["exclusions"], 0.5, 64, 6, 2, damp, df)
+    assert max(block_rel(ff, g[f"fmm_{key}"], 2)) < PARITY
+
+
+@pytest.mark.parametrize("F", [0, 1])
+def test_lower_field_orders_are_leading_blocks(gpu_ctx, F):
+    g = load_golden("water999_q.npz")
+    nf = synth.ncomp(F)
+    ff = gpu_ctx.field_fmm(0, g["coordinates"], g["multipoles"], g["exclusions"], 0.5, 64, 6, F, "THOLE",
+                           g["thole_df"])
+    assert max(block_rel(ff, g["fmm_thole"][:, :nf], F)) < PARITY
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 5, 6])
+def test_water_dipole_orders(gpu_ctx, p):
+    g = load_golden("water999_dipole_orders.npz")
+    ff = gpu_ctx.field_fmm(0, g["coordinates"], g["multipoles"], g["exclusions"], 0.5, 64, p, 1)
+    assert max(block_rel(ff, g[f"fmm_p{p}"], 1)) < PARITY
+
+
+@pytest.mark.parametrize("p", [7, 8, 10])
+def test_high_orders_vs_oracle(gpu_ctx, p):
+    """Above p = 6 the reference's own T tensors overflow; the oracle (exact
+    tensors) is the bar there."""
+    g = load_golden("water999_dipole_orders.npz")
+    x, m, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    ff = gpu_ctx.field_fmm(0, x, m, e, 0.5, 64, p, 1)
+    assert max(block_rel(ff, oracle.field_fmm(x, m, e, 0.5, 64, p, 1), 1)) < PARITY
+
+
+def test_theta06_ncrit16(gpu_ctx):
+    g = load_golden("water999_dipole_orders.npz")
+    ff = gpu_ctx.field_fmm(0, g["coordinates"], g["multipoles"], g["exclusions"], 0.6, 16, 5, 1)
+    assert max(block_rel(ff, g["fmm_theta06_ncrit16_p5"], 1)) < PARITY
+
+
+@pytest.mark.parametrize("name", ["charges_tiny40.npz", "charges_small300.npz"])
+def test_charges(gpu_ctx, name):
+    g = load_golden(name)
+    x, q, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    assert max(block_rel(gpu_ctx.field_direct(0, x, q, e, 2), g["field_direct"], 2)) < PARITY
+    assert max(block_rel(gpu_ctx.field_fmm(0, x, q, e, 0.5, 64, 4, 2), g["field_fmm"], 2)) < PARITY
+
+
+def _tree_equal(a, b):
+    for k in ("depth", "parent", "is_leaf", "nleaf", "mem_off", "members", "m2l_off", "m2l_src", "p2p_off",
+              "p2p_src"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    for k in ("centers", "r", "rmax"):
+        np.testing.assert_array_equal(a[k].view(np.uint64), b[k].view(np.uint64), err_msg=k)
+    assert a["max_depth"] == b["max_depth"]
+
+
+@pytest.mark.parametrize("case", ["uniform", "water", "clusters", "ties"])
+def test_tree_and_lists_bit_exact(gpu_ctx, case):
+    if case == "uniform":
+        x = synth.config(0)["coordinates"]
+        ncrit, theta = 64, 0.5
+    elif case == "water":
+        x = synth.water_box(3000, 3)[0]
+        ncrit, theta = 64, 0.5
+    elif case == "clusters":
+        x, _ = synth.gaussian_clusters(16, 512, 100.0, 3.0, 9)
+        ncrit, theta = 64, 0.6
+    else:   # lattice points: every theta = 0.5 MAC tie of the lattice is hit
+        g = np.stack(np.meshgrid(*[np.arange(24.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        x = np.asfortranarray(g * 1.5 + 0.25)
+        ncrit, theta = 16, 0.5
+    _tree_equal(gpu_ctx.tree(x, ncrit, theta), oracle.tree(x, ncrit, theta))
+
+
+def test_field_is_accumulated(gpu_ctx):
+    g = load_golden("charges_small300.npz")
+    x, q, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    base = np.asfortranarray(np.random.default_rng(0).normal(size=(x.shape[0], 10)))
+    f = gpu_ctx.field_fmm(0, x, q, e, 0.5, 64, 4, 2, field=base.copy(order="F"))
+    assert rel(f - base, g["field_fmm"]) < PARITY
+
+
+def test_device_entry_points_match_host(gpu_ctx):
+    import torch
+    g = load_golden("water999_q.npz")
+    dev = torch.device("cuda:0")
+
+    def col(a, dt=torch.float64):
+        return torch.as_tensor(np.ascontiguousarray(np.asarray(a).T), dtype=dt, device=dev).T
+
+    X, M = col(g["coordinates"]), col(g["multipoles"])
+    E = col(g["exclusions"], torch.int32)
+    df = torch.as_tensor(g["thole_df"], device=dev)
+    out = col(np.zeros((X.shape[0], 10)))
+    gpu_ctx.field_fmm_device(X, M, E, 0.5, 64, 6, 2, out, "THOLE", df)
+    ref = gpu_ctx.field_fmm(0, g["coordinates"], g["multipoles"], g["exclusions"], 0.5, 64, 6, 2, "THOLE",
+                            g["thole_df"])
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    out.zero_()
+    gpu_ctx.field_direct_device(X, M, E, 2, out, "THOLE", df)
+    assert max(block_rel(out.cpu().numpy(), g["direct_thole"], 2)) < PARITY
+
+
+def test_errors(gpu_ctx):
+    from fmm_amd.octree import FmmError
+    g = load_golden("charges_small300.npz")
+    x, q, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    with pytest.raises(FmmError, match="Unknown damp type: thole"):
+        gpu_ctx.field_fmm(0, x, q, e, 0.5, 64, 4, 1, "thole", np.ones(len(x)))
+    with pytest.raises(FmmError, match="periodic"):
+        gpu_ctx.field_fmm(0, x, q, e, 0.5, 64, 4, 1, is_periodic=True, box_matrix=np.eye(3) * 10)
+    bad = e.copy()
+    bad[3, 0] = len(x) + 5
+    with pytest.raises(FmmError, match="outside"):
+        gpu_ctx.field_direct(0, x, q, bad, 1)
+
+
+def test_missing_self_exclusion_gives_nonfinite_like_reference(gpu_ctx):
+    g = load_golden("charges_tiny40.npz")
+    x, q = g["coordinates"], g["multipoles"]
+    e = np.zeros((len(x), 1), dtype=np.int32, order="F")
+    f = gpu_ctx.field_direct(0, x, q, e, 0)
+    assert not np.isfinite(f).all()

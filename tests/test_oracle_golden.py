@@ -1,0 +1,84 @@
+"""The C++ oracle against the golden vectors written by the compiled
+reference (tests/golden/make_golden.py).  CPU only."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import block_rel, load_golden, rel
+
+PARITY = 1e-12
+
+
+def test_uniform4096_direct_and_fmm():
+    g = load_golden("uniform4096_p5.npz")
+    X, M, E = g["coordinates"], g["multipoles"], g["exclusions"]
+    fd = oracle.field_direct(X, M, E, 1)
+    assert max(block_rel(fd, g["field_direct"], 1)) < PARITY
+    ff = oracle.field_fmm(X, M, E, 0.5, 64, 5, 1)
+    assert max(block_rel(ff, g["field_fmm"], 1)) < PARITY
+
+
+@pytest.mark.parametrize("damp", ["", "THOLE", "AMOEBA", "ERF"])
+def test_water_quadrupoles_damped(damp):
+    g = load_golden("water999_q.npz")
+    x, m, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    df = g["erf_df"] if damp == "ERF" else g["thole_df"]
+    key = damp.lower() or "none"
+    fd = oracle.field_direct(x, m, e, 2, damp, df)
+    # AMOEBA has no order-0 tensor (src/tensors_damped_amoeba.F90:77): the
+    # direct potential block is identically zero up to the 1e10 artefact.
+    skip = damp == "AMOEBA"
+    assert max(block_rel(fd, g[f"direct_{key}"], 2, skip)) < PARITY
+    ff = oracle.field_fmm(x, m, e, 0.5, 64, 6, 2, damp, df)
+    assert max(block_rel(ff, g[f"fmm_{key}"], 2)) < PARITY
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 5, 6])
+def test_water_dipoles_orders(p):
+    g = load_golden("water999_dipole_orders.npz")
+    x, m, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    ff = oracle.field_fmm(x, m, e, 0.5, 64, p, 1)
+    assert max(block_rel(ff, g[f"fmm_p{p}"], 1)) < PARITY
+
+
+def test_theta06_ncrit16():
+    g = load_golden("water999_dipole_orders.npz")
+    ff = oracle.field_fmm(g["coordinates"], g["multipoles"], g["exclusions"], 0.6, 16, 5, 1)
+    assert max(block_rel(ff, g["fmm_theta06_ncrit16_p5"], 1)) < PARITY
+
+
+def test_reference_overflow_beyond_p6():
+    """The reference's int32 T-tensor coefficients overflow from tensor order
+    11 (src/tensors_recursive.F90:26, 64-93): with flang the M2L tensors are
+    recursive from order 13, i.e. p >= 7; on the gfortran branch for every
+    order, i.e. p >= 6.  The oracle is exact there and converges to the
+    direct sum while the reference does not."""
+    g = load_golden("water999_dipole_orders.npz")
+    x, m, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    d = g["direct"][:, 1:4]
+    errs = {}
+    for p in (6, 7, 8):
+        ff = oracle.field_fmm(x, m, e, 0.5, 64, p, 1)
+        errs[p] = rel(ff[:, 1:4], d)
+        ref_err = rel(g[f"fmm_p{p}"][:, 1:4], d)
+        if p >= 7:
+            assert ref_err > 5 * errs[p]
+            assert rel(ff, g[f"fmm_p{p}"]) > 1e-4
+    assert errs[8] < errs[7] < errs[6]
+    assert rel(g["fmm_gnu_p5"], g["fmm_p5"]) < PARITY          # both branches agree below the overflow
+    assert rel(g["fmm_gnu_p6"][:, 1:4], d) > 1.2 * errs[6]       # gfortran branch already wrong at p = 6
+    assert rel(g["fmm_gnu_p6"], g["fmm_p6"]) > 1e-4
+
+
+@pytest.mark.parametrize("name", ["charges_tiny40.npz", "charges_small300.npz"])
+def test_charges_small(name):
+    g = load_golden(name)
+    x, q, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    assert max(block_rel(oracle.field_direct(x, q, e, 2), g["field_direct"], 2)) < PARITY
+    assert max(block_rel(oracle.field_fmm(x, q, e, 0.5, 64, 4, 2), g["field_fmm"], 2)) < PARITY
+
+
+def test_n_le_ncrit_is_direct():
+    g = load_golden("charges_tiny40.npz")
+    # N = 40 <= ncrit = 64: field_fmm is field_direct (src/fmm.F90:1381-1383)
+    np.testing.assert_array_equal(g["field_fmm"], g["field_direct"])
