@@ -62,8 +62,12 @@ def build_fortran(verbose=True):
     subprocess.run([FLANG, "-cpp", "-O2", "-fPIC", "-J", out, "-c", src, "-o", obj], check=True)
     subprocess.run([FLANG, "-shared", "-fPIC", "-o", lib, obj, f"-L{HERE}", "-lfmm_hip",
                     f"-Wl,-rpath,{HERE}"], check=True)
+    drv = os.path.join(HERE, "fortran", "drive.x")
+    subprocess.run([FLANG, "-cpp", "-O2", "-I", out, "-o", drv, os.path.join(HERE, "fortran", "drive.F90"),
+                    f"-L{os.path.join(HERE, 'fortran')}", "-lfmm_fortran", f"-L{HERE}", "-lfmm_hip",
+                    f"-Wl,-rpath,{os.path.join(HERE, 'fortran')}", f"-Wl,-rpath,{HERE}"], check=True)
     if verbose:
-        print(f"built {lib}")
+        print(f"built {lib} and {drv}")
     return lib
 
 
