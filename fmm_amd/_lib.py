@@ -25,6 +25,7 @@ _PROTOS = {
     "fmmh_set_option": (i32, [vp, i32, f64]),
     "fmmh_phase_times": (i32, [vp, vp, i32, _c.c_char_p, i32]),
     "fmmh_stats": (i32, [vp, vp, i32]),
+    "fmmh_debug_buffer": (i32, [vp, _c.c_char_p, vp, i64]),
 }
 
 EXPORTED = tuple(_PROTOS)

@@ -38,7 +38,6 @@ struct Tables {
     ShiftTab p2m{}, m2m{}, l2l{};
     const int* mono_parent = nullptr;
     const int* mono_axis = nullptr;
-    const int* slot_st = nullptr;
     const int* xyz = nullptr;
     const int* level_base = nullptr;
     const double* cs = nullptr;       // Taylor factors for the p expansion
@@ -59,7 +58,8 @@ struct fmmh_context {
     std::vector<hipEvent_t> event_pool;
     size_t event_used = 0;
     std::vector<std::pair<std::string, double>> phase_ms;
-    int64_t stats[6] = {0, 0, 0, 0, 0, 0};
+    int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t stats_classes = 0;
     // last tree (for export)
     int num_nodes = 0, num_leaves = 0, max_depth = 0, n_m2l = 0, n_p2p = 0, n_sites = 0;
     std::vector<int> level_off;       // host copy, size KEY_LEVELS + 3
@@ -228,15 +228,6 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     for (int k = 0; k <= K; ++k) level_base[k + 1] = level_base[k] + ndeg(k) * (K - k + 1);
     std::vector<int> xyz(ncomp(K));
     for (int i = 0; i < ncomp(K); ++i) xyz[i] = comp[i].x | (comp[i].y << 8) | (comp[i].z << 16);
-    std::vector<int> slot_st((size_t)nm * nm);
-    for (int s = 0; s < nm; ++s)
-        for (int t = 0; t < nm; ++t) {
-            const XYZ& S = comp[s];
-            const XYZ& T = comp[t];
-            int k = S.x + S.y + S.z + T.x + T.y + T.z;
-            int i = idx(S.x + T.x, S.y + T.y, S.z + T.z);
-            slot_st[(size_t)s * nm + t] = level_base[k] + (i - ncomp(k - 1));
-        }
     std::vector<double> cs(nm), scale(ncomp(m));
     for (int s = 0; s < nm; ++s) cs[s] = taylor_factor(comp[s]);
     for (int s = 0; s < ncomp(m); ++s) scale[s] = taylor_factor(comp[s]);
@@ -246,7 +237,7 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     Tables t;
     t.p = p; t.m = m; t.nsrc = ncomp(msrc);
     std::vector<std::vector<int>*> ilist = {&p2m_off, &p2m_s, &p2m_e, &m2m_off, &m2m_s, &m2m_e, &l2l_off, &l2l_s,
-                                            &l2l_e, &mono_parent, &mono_axis, &slot_st, &xyz, &level_base};
+                                            &l2l_e, &mono_parent, &mono_axis, &xyz, &level_base};
     std::vector<std::vector<double>*> dlist = {&p2m_c, &m2m_c, &l2l_c, &cs, &scale};
     std::vector<size_t> ioff, doff;
     size_t ni = 0, nd = 0;
@@ -267,9 +258,8 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     t.l2l = ShiftTab{I + ioff[6], I + ioff[7], I + ioff[8], D + doff[2]};
     t.mono_parent = I + ioff[9];
     t.mono_axis = I + ioff[10];
-    t.slot_st = I + ioff[11];
-    t.xyz = I + ioff[12];
-    t.level_base = I + ioff[13];
+    t.xyz = I + ioff[11];
+    t.level_base = I + ioff[12];
     t.cs = D + doff[3];
     t.scale = D + doff[4];
     return c->tables.emplace(key, t).first->second;
@@ -405,10 +395,13 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     t.level_fill = c->buf<int>("t_level_fill", KEY_LEVELS + 3);
     FMMH_HIP(hipMemsetAsync(t.info, 0, 64 * sizeof(int), s));
     launch_centroid(X, ldx, n, t, s);
+    c->mark("t_centroid");
     launch_keys(X, ldx, n, t, keys_in, idx_in, s);
+    c->mark("t_keys");
     size_t tb = std::max(sort_pairs_u64_temp(n), exscan_i32_temp(n));
     void* temp = c->buf<char>("t_temp", tb);
     sort_pairs_u64(temp, tb, keys_in, t.keys, idx_in, t.perm, n, 63, s);
+    c->mark("t_sort");
     launch_site_levels(n, ncrit, t, s);
     exscan_i32(temp, tb, t.cnt, t.off, n, s);
     int* hp = c->h_pinned;
@@ -416,6 +409,7 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     FMMH_HIP(hipMemcpyAsync(hp + 1, t.cnt + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
     FMMH_HIP(hipMemcpyAsync(hp + 2, t.info + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     c->sync();
+    c->mark("t_levels");
     if (hp[2]) throw Error(FMMH_ERR_TREE, "octree deeper than 21 levels: >= ncrit sites share one cell");
     const int nn = hp[0] + hp[1];
     out.num_nodes = nn;
@@ -438,6 +432,7 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     FMMH_HIP(hipMemsetAsync(t.children, 0xff, (size_t)nn * 8 * sizeof(int), s));
     launch_emit_nodes(n, t, s);
     launch_node_links(nn, t, s);
+    c->mark("t_nodes");
     size_t tb2 = exscan_i32_temp(nn);
     void* temp2 = c->buf<char>("t_temp2", tb2);
     exscan_i32(temp2, tb2, t.is_leaf, leaf_scan, nn, s);
@@ -463,6 +458,7 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
 
 struct ListsOut {
     int* m2l_off = nullptr; int* m2l_src = nullptr; int n_m2l = 0;
+    unsigned long long* m2l_keys = nullptr;   // sorted (target << 32 | source), CSR order
     int* p2p_off = nullptr; int* p2p_src = nullptr; int n_p2p = 0;
 };
 
@@ -509,24 +505,78 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
             if (hp[4] & 4) cap_f *= 2;
             continue;
         }
+        c->mark("l_traverse");
         L.n_m2l = hp[2];
         L.n_p2p = hp[3];
         int bits = 32;
         while ((1ll << (bits - 32)) < nn) ++bits;
         size_t tb = std::max(sort_keys_u64_temp(std::max(L.n_m2l, 1)), sort_keys_u64_temp(std::max(L.n_p2p, 1)));
         void* temp = c->buf<char>("l_temp", tb);
-        unsigned long long* sorted = c->buf<unsigned long long>("l_sorted", std::max(L.n_m2l, L.n_p2p));
+        L.m2l_keys = c->buf<unsigned long long>("l_m2l_sorted", L.n_m2l);
+        unsigned long long* sorted = c->buf<unsigned long long>("l_p2p_sorted", L.n_p2p);
         L.m2l_off = c->buf<int>("l_m2l_off", nn + 1);
         L.m2l_src = c->buf<int>("l_m2l_src", L.n_m2l);
         L.p2p_off = c->buf<int>("l_p2p_off", nn + 1);
         L.p2p_src = c->buf<int>("l_p2p_src", L.n_p2p);
-        if (L.n_m2l > 0) sort_keys_u64(temp, tb, m2l, sorted, L.n_m2l, bits, s);
-        launch_csr(sorted, L.n_m2l, nn, L.m2l_off, L.m2l_src, s);
+        if (L.n_m2l > 0) sort_keys_u64(temp, tb, m2l, L.m2l_keys, L.n_m2l, bits, s);
+        launch_csr(L.m2l_keys, L.n_m2l, nn, L.m2l_off, L.m2l_src, s);
         if (L.n_p2p > 0) sort_keys_u64(temp, tb, p2p, sorted, L.n_p2p, bits, s);
         launch_csr(sorted, L.n_p2p, nn, L.p2p_off, L.p2p_src, s);
         return;
     }
     throw Error(FMMH_ERR_INTERNAL, "interaction list capacity did not converge");
+}
+
+// ---------------------------------------------------------------- M2L
+// Displacement classes (sort + unique), one tensor per class, then the
+// target-centric Hankel contraction.  One host sync to size the class table.
+void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, const double* CM, double* LE) {
+    hipStream_t s = c->stream;
+    const int nn = tr.num_nodes, np = L.n_m2l;
+    const int nm = ncomp(p), nT = ncomp(2 * p);
+    double* CMs = c->buf<double>("m_cms", (size_t)nn * nm);
+    launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, s);
+    int* pair_class = c->buf<int>("m_pair_class", np);
+    int* dev_scalars = c->buf<int>("m_scalars", 4);   // num_classes, error
+    FMMH_HIP(hipMemsetAsync(dev_scalars, 0, 4 * sizeof(int), s));
+    double* Tc = nullptr;
+    if (np > 0) {
+        M2LClassArgs ca{};
+        ca.pair_keys = L.m2l_keys;
+        ca.n_pairs = np;
+        ca.cx = tr.t.cx; ca.cy = tr.t.cy; ca.cz = tr.t.cz; ca.r = tr.t.r; ca.level = tr.t.level;
+        ca.class_key_in = c->buf<unsigned long long>("m_key_in", np);
+        ca.pair_idx = c->buf<int>("m_idx_in", np);
+        ca.class_key_sorted = c->buf<unsigned long long>("m_key_sorted", np);
+        ca.pair_idx_sorted = c->buf<int>("m_idx_sorted", np);
+        ca.flag = c->buf<int>("m_flag", np);
+        ca.cls = c->buf<int>("m_cls", np);
+        ca.pair_class = pair_class;
+        ca.class_key = c->buf<unsigned long long>("m_class_key", np);
+        ca.num_classes = dev_scalars;
+        ca.error = dev_scalars + 1;
+        ca.p = p;
+        ca.xyz = tb.xyz;
+        ca.level_base = tb.level_base;
+        launch_m2l_keys(ca, s);
+        size_t tbytes = std::max(sort_pairs_u64_temp(np), exscan_i32_temp(np));
+        void* temp = c->buf<char>("m_temp", tbytes);
+        sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, ca.pair_idx_sorted, np, 62, s);
+        launch_m2l_flags(ca, s);
+        exscan_i32(temp, tbytes, ca.flag, ca.cls, np, s);
+        launch_m2l_assign(ca, s);
+        FMMH_HIP(hipMemcpyAsync(c->h_pinned + 300, dev_scalars, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        c->sync();
+        if (c->h_pinned[301]) throw Error(FMMH_ERR_UNSUPPORTED, "M2L displacement outside the class key range");
+        const int ncls = c->h_pinned[300];
+        c->stats_classes = ncls;
+        Tc = c->buf<double>("m_tc", (size_t)ncls * nT);
+        ca.Tc = Tc;
+        launch_m2l_class_tensors(ca, std::min(ncls, 4096), s);
+    }
+    c->mark("m2l_classes");
+    M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE, tr.t.level, tr.t.r, nm, p};
+    launch_m2l(ma, nn, s);
 }
 
 // ---------------------------------------------------------------- field_fmm
@@ -573,11 +623,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
                 ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0};
     launch_near(na, nl, m, k.F, k.damp, s);
     c->mark("p2p");
-    int* iota = c->buf<int>("iota", nn);
-    hipLaunchKernelGGL(k_iota, dim3(nblk(nn, 256)), dim3(256), 0, s, iota, nn);
-    M2LArgs ma{iota, L.m2l_off, L.m2l_src, tr.t.cx, tr.t.cy, tr.t.cz, CM, LE, nm, p, tb.slot_st, tb.cs, tb.xyz,
-               tb.level_base};
-    launch_m2l(ma, nn, s);
+    run_m2l(c, tr, L, tb, p, CM, LE);
     c->mark("m2l");
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
                 tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf};
@@ -597,7 +643,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
     c->sync();
     std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
     c->stats[0] = nn; c->stats[1] = nl; c->stats[2] = tr.max_depth;
-    c->stats[3] = L.n_m2l; c->stats[4] = L.n_p2p; c->stats[5] = (int64_t)pairs;
+    c->stats[3] = L.n_m2l; c->stats[4] = L.n_p2p; c->stats[5] = (int64_t)pairs; c->stats[6] = c->stats_classes;
 }
 
 // ---------------------------------------------------------------- host <-> device staging
@@ -881,10 +927,21 @@ int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, in
     return rc == FMMH_OK ? std::min(count, (int)cap) : -rc;
 }
 
+int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t bytes) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        auto it = c->bufs.find(name);
+        if (it == c->bufs.end()) throw Error(FMMH_ERR_ARG, std::string("unknown buffer ") + name);
+        if ((size_t)bytes > it->second.cap) throw Error(FMMH_ERR_ARG, std::string("buffer too small: ") + name);
+        FMMH_HIP(hipStreamSynchronize(c->stream));
+        FMMH_HIP(hipMemcpy(host, it->second.p, (size_t)bytes, hipMemcpyDeviceToHost));
+    });
+}
+
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
-        for (int i = 0; i < std::min(cap, 6); ++i) out[i] = c->stats[i];
+        for (int i = 0; i < std::min(cap, 7); ++i) out[i] = c->stats[i];
     });
 }
 

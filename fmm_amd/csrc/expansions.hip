@@ -3,15 +3,13 @@
 //   M2M  src/fmm.F90:387-435, 536-540   (reference: leaf -> every ancestor;
 //        here child -> parent level by level, identical by the binomial
 //        theorem, different summation order)
-//   M2L  src/fmm.F90:826-882            undamped T of order 2p per pair
+//   (M2L, src/fmm.F90:826-882, is in m2l.hip)
 //   L2L  src/fmm.F90:981-1044           parent -> child, level by level
 //   L2P  src/fmm.F90:1046-1132          evaluation at leaf sites, added to
 //        the near field and scattered to the caller's site order
 // Shift operators run one workgroup per node with one lane per target
 // coefficient, driven by host-built (target, source, monomial, coefficient)
-// tables.  M2L builds the order-2p tensor of each pair cooperatively in LDS
-// with the Hermite recursion (one barrier per tensor degree) and contracts it
-// against the LDS-staged source moments.
+// tables.
 #include "common.h"
 #include "kernels.h"
 #include "tensors.h"
@@ -114,80 +112,6 @@ __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ n
     }
 }
 
-// ---------------------------------------------------------------- M2L
-constexpr int M2L_KMAX = 2 * 10;                       // p <= 10 for the LDS tensor
-constexpr int M2L_AUX = (M2L_KMAX + 1) * (M2L_KMAX + 2) * (M2L_KMAX + 3) * (M2L_KMAX + 4) / 24;
-constexpr int M2L_NMMAX = ncomp(10);
-
-__global__ __launch_bounds__(64) void k_m2l(M2LArgs a) {
-    __shared__ double A[M2L_AUX];
-    __shared__ double cm[M2L_NMMAX];
-    const int i = a.targets[blockIdx.x];
-    const int nm = a.nm;
-    const int K = 2 * a.p;
-    const int e0 = a.off[i], e1 = a.off[i + 1];
-    const double cxi = a.cx[i], cyi = a.cy[i], czi = a.cz[i];
-    constexpr int NQ = (M2L_NMMAX + 63) / 64;
-    double acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int e = e0; e < e1; ++e) {
-        const int j = a.src[e];
-        const double dx = cxi - a.cx[j], dy = cyi - a.cy[j], dz = czi - a.cz[j];
-        if ((int)threadIdx.x <= K) {
-            const double R2 = dx * dx + dy * dy + dz * dz;
-            const double inv = rsqrt(R2);
-            const double inv2 = inv * inv;
-            double g = inv;
-            for (int n = 1; n <= (int)threadIdx.x; ++n) g *= -(2.0 * n - 1.0) * inv2;
-            A[threadIdx.x] = g;               // degree 0: slot n
-        }
-        for (int s = threadIdx.x; s < nm; s += 64) cm[s] = a.cs[s] * a.CM[(int64_t)j * nm + s];
-        __syncthreads();
-        for (int k = 1; k <= K; ++k) {
-            const int nd = (k + 1) * (k + 2) / 2;
-            const int ndm1 = k * (k + 1) / 2;
-            const int ndm2 = (k - 1) * k / 2;
-            const int nent = nd * (K - k + 1);
-            const int base = a.level_base[k], bm1 = a.level_base[k - 1], bm2 = k >= 2 ? a.level_base[k - 2] : 0;
-            const int c0 = ncomp(k - 1), c1 = k >= 1 ? ncomp(k - 2) : 0, c2 = k >= 2 ? ncomp(k - 3) : 0;
-            for (int x = threadIdx.x; x < nent; x += 64) {
-                const int n = x / nd, l = x - n * nd;
-                const int pk = a.xyz[c0 + l];
-                const int t = pk & 255, u = (pk >> 8) & 255, v = pk >> 16;
-                double val;
-                if (t > 0) {
-                    val = dx * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t - 1, u, v) - c1)];
-                    if (t > 1) val = __builtin_fma((double)(t - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t - 2, u, v) - c2)], val);
-                } else if (u > 0) {
-                    val = dy * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u - 1, v) - c1)];
-                    if (u > 1) val = __builtin_fma((double)(u - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u - 2, v) - c2)], val);
-                } else {
-                    val = dz * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u, v - 1) - c1)];
-                    if (v > 1) val = __builtin_fma((double)(v - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u, v - 2) - c2)], val);
-                }
-                A[base + n * nd + l] = val;
-            }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int t = threadIdx.x + 64 * q;
-            if (t < nm) {
-                double sum = 0.0;
-                for (int s = 0; s < nm; ++s) sum = __builtin_fma(cm[s], A[a.slot_st[s * nm + t]], sum);
-                acc[q] += sum;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int t = threadIdx.x + 64 * q;
-        if (t < nm) a.LE[(int64_t)i * nm + t] = acc[q];
-    }
-}
-
 // ---------------------------------------------------------------- L2L / L2P
 __global__ __launch_bounds__(128) void k_l2l(DownArgs a, const int* __restrict__ nodes) {
     __shared__ double mono[MAX_NM];
@@ -278,12 +202,6 @@ void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s) {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, a, nodes);
-    FMMH_HIP(hipGetLastError());
-}
-
-void launch_m2l(const M2LArgs& a, int num_targets, hipStream_t s) {
-    if (num_targets <= 0) return;
-    hipLaunchKernelGGL(k_m2l, dim3(num_targets), dim3(64), 0, s, a);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -133,17 +133,51 @@ struct UpArgs {
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s);
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s);
 
-struct M2LArgs {
-    const int* targets;       // nodes with non-empty lists
-    const int* off; const int* src;
-    const double* cx; const double* cy; const double* cz;
-    const double* CM; double* LE; int nm; int p;
-    const int* slot_st;       // [s * nm + t] -> aux slot of T_{s+t}
-    const double* cs;         // (-1)^{|s|+1} / (sx! sy! sz!)
-    const int* xyz;           // packed index -> t | u << 8 | v << 16
-    const int* level_base;    // aux slot base per degree (K + 2 entries)
+// M2L by displacement class.  For a far pair (i, j) let u = min(r_i, r_j):
+// c_i - c_j = u v with v an integer vector (node centres are the root centre
+// plus multiples of the finest half-width involved), so
+//   T_k(c_i - c_j) = u^{-(|k|+1)} T_k(v).
+// Pairs are grouped by (v, level_i - level_j); one tensor per class, and the
+// per-pair work is the Hankel contraction L~_i(t) += sum_s T~_c[s+t] M~_j(s)
+// on box-size-scaled expansions M~_j(s) = c_s M_j(s) r_j^{-|s|}.
+struct M2LClassArgs {
+    const unsigned long long* pair_keys;   // sorted (target << 32 | source), CSR order
+    int n_pairs;
+    const double* cx; const double* cy; const double* cz; const double* r;
+    const int* level;
+    unsigned long long* class_key_in;      // per pair: packed (v, level difference)
+    int* pair_idx;                          // per pair: its CSR position
+    unsigned long long* class_key_sorted;
+    int* pair_idx_sorted;
+    int* flag;                              // first of a class in sorted order
+    int* cls;                               // exclusive scan of flag
+    int* pair_class;                        // CSR position -> class id
+    unsigned long long* class_key;          // class id -> key
+    int* num_classes;                       // device scalar
+    int* error;                             // device scalar: displacement out of range
+    double* Tc;                             // class tensors, ncomp(2p) each
+    int p;
+    const int* xyz;                         // packed index -> t | u << 8 | v << 16
+    const int* level_base;                  // aux slot base per degree (2p + 2 entries)
 };
-void launch_m2l(const M2LArgs& a, int num_targets, hipStream_t s);
+void launch_m2l_keys(const M2LClassArgs& a, hipStream_t s);
+void launch_m2l_flags(const M2LClassArgs& a, hipStream_t s);        // after the key sort
+void launch_m2l_assign(const M2LClassArgs& a, hipStream_t s);       // after the flag scan
+void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s);
+
+struct M2LArgs {
+    const int* off; const int* src;
+    const int* pair_class;
+    const double* Tc;
+    const double* CMs;        // scaled source expansions M~
+    double* LE;
+    const int* level;
+    const double* r;
+    int nm; int p;
+};
+void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
+                          hipStream_t s);
+void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s);
 
 struct DownArgs {
     const double* cx; const double* cy; const double* cz;

@@ -1,0 +1,313 @@
+// M2L, src/fmm.F90:826-882: L_i(t) += sum_{j in list(i)} sum_s c_s M_j(s) T_{s+t}(c_i - c_j),
+// undamped, |s|, |t| <= p, tensor order 2p, c_s = (-1)^{|s|+1} / (sx! sy! sz!).
+//
+// MI355X formulation.  Node centres are the root centre plus multiples of
+// the half-widths r_L = r_0 / 2^{L-1}, so for a far pair the displacement is
+// u v with u = min(r_i, r_j) and v an integer vector.  With
+// T_k(u v) = u^{-(|k|+1)} T_k(v):
+//   * pairs are grouped into classes (v, level_i - level_j) -- a few hundred
+//     to a few thousand per tree -- and one order-2p tensor is built per
+//     class (Hermite recursion in LDS, once);
+//   * source expansions are scaled by their own box size,
+//     M~_j(s) = c_s M_j(s) r_j^{-|s|}, targets accumulate
+//     L~_i(t) = sum 2^{...} sum_s T~_c[s+t] M~_j(s) and are unscaled by
+//     r_i^{-(|t|+1)} at the end; level mismatches are exact powers of two.
+// The per-pair work is then a pure Hankel contraction: one workgroup per
+// target node, one lane per target coefficient, the class tensor staged in
+// a double-buffered LDS slot (prefetched one pair ahead), the source
+// expansion read with wave-uniform scalar loads, the loops fully unrolled
+// for each expansion order so every index is a constant offset.
+// Using the exact integer v instead of the rounded c_i - c_j changes each
+// tensor by ~1e-16 relative, far inside the 1e-12 parity bar.
+#include "common.h"
+#include "kernels.h"
+
+namespace fmmh {
+
+constexpr int VBITS = 18;
+constexpr long long VOFF = 1ll << (VBITS - 1);
+constexpr int CLASS_KEY_BITS = 3 * VBITS + 8;
+
+__global__ __launch_bounds__(256) void k_m2l_keys(M2LClassArgs a) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.n_pairs) return;
+    const unsigned long long pk = a.pair_keys[e];
+    const int i = (int)(pk >> 32), j = (int)(pk & 0xffffffffull);
+    const double u = fmin(a.r[i], a.r[j]);
+    const long long vx = llrint((a.cx[i] - a.cx[j]) / u);
+    const long long vy = llrint((a.cy[i] - a.cy[j]) / u);
+    const long long vz = llrint((a.cz[i] - a.cz[j]) / u);
+    const int dl = a.level[i] - a.level[j];
+    if (llabs(vx) >= VOFF || llabs(vy) >= VOFF || llabs(vz) >= VOFF || dl < -100 || dl > 100)
+        atomicOr(a.error, 1);
+    const unsigned long long key = ((unsigned long long)(vx + VOFF) << (2 * VBITS + 8)) |
+                                   ((unsigned long long)(vy + VOFF) << (VBITS + 8)) |
+                                   ((unsigned long long)(vz + VOFF) << 8) | (unsigned long long)(dl + 128);
+    a.class_key_in[e] = key;
+    a.pair_idx[e] = e;
+}
+
+__global__ __launch_bounds__(256) void k_m2l_flags(M2LClassArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n_pairs) return;
+    a.flag[k] = (k == 0 || a.class_key_sorted[k] != a.class_key_sorted[k - 1]) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_m2l_assign(M2LClassArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n_pairs) return;
+    // cls is the exclusive scan of the class-start flags: the class id is the
+    // number of starts up to and including k, minus one
+    const int c = a.cls[k] + a.flag[k] - 1;
+    a.pair_class[a.pair_idx_sorted[k]] = c;
+    if (a.flag[k]) a.class_key[c] = a.class_key_sorted[k];
+    if (k == a.n_pairs - 1) *a.num_classes = c + 1;
+}
+
+constexpr int CT_KMAX = 2 * PMAX;
+constexpr int CT_AUX = (CT_KMAX + 1) * (CT_KMAX + 2) * (CT_KMAX + 3) * (CT_KMAX + 4) / 24;
+
+// One class tensor per workgroup iteration: T_k(v) for |k| <= 2p by the
+// Hermite recursion over degree levels (one barrier per level), scaled by
+// 2^{dl |k|} when the target is the deeper box (dl > 0).
+__global__ __launch_bounds__(64) void k_class_tensor(M2LClassArgs a) {
+    __shared__ double A[CT_AUX];
+    const int K = 2 * a.p;
+    const int nT = ncomp(K);
+    const int ncls = *a.num_classes;
+    for (int c = blockIdx.x; c < ncls; c += gridDim.x) {
+        const unsigned long long key = a.class_key[c];
+        const double vx = (double)((long long)((key >> (2 * VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
+        const double vy = (double)((long long)((key >> (VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
+        const double vz = (double)((long long)((key >> 8) & ((1ull << VBITS) - 1)) - VOFF);
+        const int dl = (int)(key & 255ull) - 128;
+        if ((int)threadIdx.x <= K) {
+            const double R2 = vx * vx + vy * vy + vz * vz;
+            const double inv = 1.0 / sqrt(R2);
+            const double inv2 = inv * inv;
+            double g = inv;
+            for (int n = 1; n <= (int)threadIdx.x; ++n) g *= -(2.0 * n - 1.0) * inv2;
+            A[threadIdx.x] = g;
+        }
+        __syncthreads();
+        for (int k = 1; k <= K; ++k) {
+            const int nd = (k + 1) * (k + 2) / 2, ndm1 = k * (k + 1) / 2, ndm2 = (k - 1) * k / 2;
+            const int nent = nd * (K - k + 1);
+            const int base = a.level_base[k], bm1 = a.level_base[k - 1], bm2 = k >= 2 ? a.level_base[k - 2] : 0;
+            const int c0 = ncomp(k - 1), c1 = ncomp(k - 2), c2 = ncomp(k - 3);
+            for (int x = threadIdx.x; x < nent; x += 64) {
+                const int n = x / nd, l = x - n * nd;
+                const int pk = a.xyz[c0 + l];
+                const int t = pk & 255, u = (pk >> 8) & 255, v = pk >> 16;
+                double val;
+                if (t > 0) {
+                    val = vx * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t - 1, u, v) - c1)];
+                    if (t > 1) val = __builtin_fma((double)(t - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t - 2, u, v) - c2)], val);
+                } else if (u > 0) {
+                    val = vy * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u - 1, v) - c1)];
+                    if (u > 1) val = __builtin_fma((double)(u - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u - 2, v) - c2)], val);
+                } else {
+                    val = vz * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u, v - 1) - c1)];
+                    if (v > 1) val = __builtin_fma((double)(v - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u, v - 2) - c2)], val);
+                }
+                A[base + n * nd + l] = val;
+            }
+            __syncthreads();
+        }
+        for (int idx = threadIdx.x; idx < nT; idx += 64) {
+            const int pk = a.xyz[idx];
+            const int k = (pk & 255) + ((pk >> 8) & 255) + (pk >> 16);
+            double val = A[a.level_base[k] + (idx - ncomp(k - 1))];
+            if (dl > 0) val = ldexp(val, dl * k);
+            a.Tc[(int64_t)c * nT + idx] = val;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scale_sources(const double* __restrict__ CM, const double* __restrict__ cs,
+                                                       const double* __restrict__ r, double* __restrict__ CMs,
+                                                       int num_nodes, int nm) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (int64_t)num_nodes * nm) return;
+    const int j = (int)(x / nm), s = (int)(x - (int64_t)j * nm);
+    int k = 0;
+    while (ncomp(k) <= s) ++k;
+    const double rinv = 1.0 / r[j];
+    double f = 1.0;
+    for (int q = 0; q < k; ++q) f *= rinv;
+    CMs[x] = cs[s] * CM[x] * f;
+}
+
+constexpr int M2L_LDS_SOURCE_P = 8;   // from this order on, source expansions are staged in LDS
+
+__host__ __device__ constexpr int tet(int k) { return k * (k + 1) * (k + 2) / 6; }
+__host__ __device__ constexpr int tri(int w) { return w * (w + 1) / 2; }
+
+// Packed decomposition of a target component t for the index of s + t:
+//   idx(s + t) = tet(|s| + |t|) + tri(ws + wt) + (sz + tz),  w = y + z
+// (packed order within a degree runs over w, then z).
+__device__ __forceinline__ int decode_target(int t) {
+    int k = 0;
+    while (ncomp(k) <= t) ++k;
+    const int l = t - ncomp(k - 1);
+    int w = 0;
+    while (tri(w + 1) <= l) ++w;
+    return k | (w << 8) | ((l - tri(w)) << 16);
+}
+
+template <int P>
+__global__ __launch_bounds__(128, 4) void k_m2l(M2LArgs a) {
+    constexpr int NM = ncomp(P);
+    constexpr int NT = ncomp(2 * P);
+    constexpr int NQ = (NM + 127) / 128;
+    constexpr int NL = (NT + 127) / 128;
+    static_assert(NQ <= 3, "at most 3 target components per lane");
+    __shared__ double Tl[2][NT];
+    __shared__ double Ms[P >= M2L_LDS_SOURCE_P ? NM : 1];
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int e0 = a.off[i], e1 = a.off[i + 1];
+    const int dec0 = decode_target(min(tid, NM - 1));
+    const int dec1 = NQ > 1 ? decode_target(min(tid + 128, NM - 1)) : 0;
+    const int dec2 = NQ > 2 ? decode_target(min(tid + 256, NM - 1)) : 0;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    double pre[NL];
+    if (e0 < e1) {
+        const double* T0 = a.Tc + (int64_t)a.pair_class[e0] * NT;
+#pragma unroll
+        for (int x = 0; x < NL; ++x)
+            if (tid + 128 * x < NT) Tl[0][tid + 128 * x] = T0[tid + 128 * x];
+    }
+    __syncthreads();
+    const int li = a.level[i];
+    for (int e = e0; e < e1; ++e) {
+        const int cur = (e - e0) & 1;
+        const bool more = e + 1 < e1;
+        if (more) {
+            const double* Tn = a.Tc + (int64_t)a.pair_class[e + 1] * NT;
+#pragma unroll
+            for (int x = 0; x < NL; ++x) pre[x] = (tid + 128 * x < NT) ? Tn[tid + 128 * x] : 0.0;
+        }
+        const int j = a.src[e];
+        const int dl = li - a.level[j];
+        const double* __restrict__ Mj = a.CMs + (int64_t)j * NM;
+        const double* __restrict__ T = Tl[cur];
+        if constexpr (P >= M2L_LDS_SOURCE_P) {
+            // large orders: the source expansion goes through LDS as well, the
+            // loops stay rolled (full unrolling would exhaust the SGPR file)
+            __syncthreads();
+            for (int x = tid; x < NM; x += 128) Ms[x] = Mj[x];
+            __syncthreads();
+        }
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) {
+            if (tid + 128 * q >= NM) break;
+            const int dec = q == 0 ? dec0 : (q == 1 ? dec1 : dec2);
+            const int k = dec & 255, w = (dec >> 8) & 255, tz = dec >> 16;
+            double sum = 0.0;
+            if constexpr (P >= M2L_LDS_SOURCE_P) {
+#pragma unroll 1
+                for (int n = 0; n <= P; ++n) {
+                    const int tb = tet(n + k) + tz;
+                    for (int ws = 0; ws <= n; ++ws) {
+                        const int base = tb + tri(ws + w);
+                        const int sb = tet(n) + tri(ws);
+#pragma unroll 4
+                        for (int sz = 0; sz <= ws; ++sz) sum = __builtin_fma(Ms[sb + sz], T[base + sz], sum);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n <= P; ++n) {
+                    const int tb = tet(n + k) + tz;
+#pragma unroll
+                    for (int ws = 0; ws <= n; ++ws) {
+                        const int base = tb + tri(ws + w);
+                        const int sb = tet(n) + tri(ws);
+#pragma unroll
+                        for (int sz = 0; sz <= ws; ++sz) sum = __builtin_fma(Mj[sb + sz], T[base + sz], sum);
+                    }
+                }
+            }
+            double scale = 1.0;
+            if (dl > 0) scale = ldexp(1.0, -dl * k);
+            else if (dl < 0) scale = ldexp(1.0, -dl * (k + 1));
+            if (q == 0) acc0 = __builtin_fma(scale, sum, acc0);
+            else if (q == 1) acc1 = __builtin_fma(scale, sum, acc1);
+            else acc2 = __builtin_fma(scale, sum, acc2);
+        }
+        if (more) {
+#pragma unroll
+            for (int x = 0; x < NL; ++x)
+                if (tid + 128 * x < NT) Tl[1 - cur][tid + 128 * x] = pre[x];
+        }
+        __syncthreads();
+    }
+    const double rinv = 1.0 / a.r[i];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int t = tid + 128 * q;
+        if (t < NM) {
+            const int dec = q == 0 ? dec0 : (q == 1 ? dec1 : dec2);
+            double f = rinv;
+            for (int k = 0; k < (dec & 255); ++k) f *= rinv;
+            a.LE[(int64_t)i * NM + t] = (q == 0 ? acc0 : (q == 1 ? acc1 : acc2)) * f;
+        }
+    }
+}
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_m2l_keys(const M2LClassArgs& a, hipStream_t s) {
+    if (a.n_pairs <= 0) return;
+    hipLaunchKernelGGL(k_m2l_keys, dim3(nblk(a.n_pairs, 256)), dim3(256), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2l_flags(const M2LClassArgs& a, hipStream_t s) {
+    if (a.n_pairs <= 0) return;
+    hipLaunchKernelGGL(k_m2l_flags, dim3(nblk(a.n_pairs, 256)), dim3(256), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2l_assign(const M2LClassArgs& a, hipStream_t s) {
+    if (a.n_pairs <= 0) return;
+    hipLaunchKernelGGL(k_m2l_assign, dim3(nblk(a.n_pairs, 256)), dim3(256), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s) {
+    if (grid <= 0) return;
+    hipLaunchKernelGGL(k_class_tensor, dim3(grid), dim3(64), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
+                          hipStream_t s) {
+    const int nm = ncomp(p);
+    hipLaunchKernelGGL(k_scale_sources, dim3(nblk((long long)num_nodes * nm, 256)), dim3(256), 0, s, CM, cs, r, CMs,
+                       num_nodes, nm);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s) {
+    if (num_nodes <= 0) return;
+    dim3 g(num_nodes), b(128);
+    switch (a.p) {
+        case 0: hipLaunchKernelGGL(k_m2l<0>, g, b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_m2l<1>, g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_m2l<2>, g, b, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_m2l<3>, g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_m2l<4>, g, b, 0, s, a); break;
+        case 5: hipLaunchKernelGGL(k_m2l<5>, g, b, 0, s, a); break;
+        case 6: hipLaunchKernelGGL(k_m2l<6>, g, b, 0, s, a); break;
+        case 7: hipLaunchKernelGGL(k_m2l<7>, g, b, 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_m2l<8>, g, b, 0, s, a); break;
+        case 9: hipLaunchKernelGGL(k_m2l<9>, g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL(k_m2l<10>, g, b, 0, s, a); break;
+    }
+    FMMH_HIP(hipGetLastError());
+}
+
+}  // namespace fmmh

@@ -295,42 +295,71 @@ __device__ __forceinline__ double norm2_3(double v0, double v1, double v2) {
 // (target, source) node pairs.  Outputs are appended in arbitrary order and
 // sorted afterwards; for a fixed target the reference's DFS appends sources
 // in ascending pre-order id, which the sort reproduces.
+// Wave-aggregated append: every lane asks for `want` slots of `counter`;
+// one atomic per wavefront, lanes get consecutive ranges in lane order.
+__device__ __forceinline__ int wave_append(int* counter, int want) {
+    const int lane = __lane_id();
+    int incl = want;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    int base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 63, 64);
+    return base + incl - want;
+}
+
 __global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
 #pragma clang fp contract(off)
     const int n = *a.n_in;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const int2 pr = a.in[e];
-        const int i = pr.x, j = pr.y;
-        const double d0 = a.cx[i] - (a.cx[j] + 0.0);
-        const double d1 = a.cy[i] - (a.cy[j] + 0.0);
-        const double d2 = a.cz[i] - (a.cz[j] + 0.0);
-        const double r = norm2_3(d0, d1, d2);
-        const double lhs = r * a.theta;
-        const double rhs = a.rmax[i] + a.rmax[j];
-        const bool li = a.is_leaf[i] != 0, lj = a.is_leaf[j] != 0;
-        if (lhs > rhs) {
-            const int pos = atomicAdd(a.n_m2l, 1);
-            if (pos < a.cap_m2l) a.m2l[pos] = ((unsigned long long)i << 32) | (unsigned)j;
-            else atomicOr(a.overflow, 1);
-        } else if (li && lj) {
-            const int pos = atomicAdd(a.n_p2p, 1);
-            if (pos < a.cap_p2p) a.p2p[pos] = ((unsigned long long)i << 32) | (unsigned)j;
-            else atomicOr(a.overflow, 2);
-        } else if (lj || ((a.rmax[i] > a.rmax[j]) && !li)) {
-            for (int o = 0; o < 8; ++o) {
-                const int c = a.children[i * 8 + o];
-                if (c < 0) continue;
-                const int pos = atomicAdd(a.n_out, 1);
-                if (pos < a.cap_out) a.out[pos] = make_int2(c, j);
-                else atomicOr(a.overflow, 4);
+    const int stride = gridDim.x * blockDim.x;
+    // the loop runs over whole wavefronts so the lane collectives see every lane
+    for (int e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); e0 < n; e0 += stride) {
+        const int e = e0 + (threadIdx.x & 63);
+        int kind = -1;                 // 0 far (M2L), 1 leaf pair (P2P), 2 descend target, 3 descend source
+        int i = 0, j = 0, nchild = 0;
+        if (e < n) {
+            const int2 pr = a.in[e];
+            i = pr.x; j = pr.y;
+            const double d0 = a.cx[i] - (a.cx[j] + 0.0);
+            const double d1 = a.cy[i] - (a.cy[j] + 0.0);
+            const double d2 = a.cz[i] - (a.cz[j] + 0.0);
+            const double r = norm2_3(d0, d1, d2);
+            const double lhs = r * a.theta;
+            const double rhs = a.rmax[i] + a.rmax[j];
+            const bool li = a.is_leaf[i] != 0, lj = a.is_leaf[j] != 0;
+            if (lhs > rhs) kind = 0;
+            else if (li && lj) kind = 1;
+            else if (lj || ((a.rmax[i] > a.rmax[j]) && !li)) kind = 2;
+            else kind = 3;
+            if (kind >= 2) {
+                const int pnode = kind == 2 ? i : j;
+                for (int o = 0; o < 8; ++o) nchild += a.children[pnode * 8 + o] >= 0 ? 1 : 0;
             }
-        } else {
+        }
+        const unsigned long long key = ((unsigned long long)i << 32) | (unsigned)j;
+        int pos = wave_append(a.n_m2l, kind == 0 ? 1 : 0);
+        if (kind == 0) {
+            if (pos < a.cap_m2l) a.m2l[pos] = key;
+            else atomicOr(a.overflow, 1);
+        }
+        pos = wave_append(a.n_p2p, kind == 1 ? 1 : 0);
+        if (kind == 1) {
+            if (pos < a.cap_p2p) a.p2p[pos] = key;
+            else atomicOr(a.overflow, 2);
+        }
+        pos = wave_append(a.n_out, nchild);
+        if (kind >= 2) {
+            const int pnode = kind == 2 ? i : j;
             for (int o = 0; o < 8; ++o) {
-                const int c = a.children[j * 8 + o];
+                const int c = a.children[pnode * 8 + o];
                 if (c < 0) continue;
-                const int pos = atomicAdd(a.n_out, 1);
-                if (pos < a.cap_out) a.out[pos] = make_int2(i, c);
+                if (pos < a.cap_out) a.out[pos] = kind == 2 ? make_int2(c, j) : make_int2(i, c);
                 else atomicOr(a.overflow, 4);
+                ++pos;
             }
         }
     }

@@ -163,10 +163,16 @@ class Context:
         return dict(zip(keys, ms[:cnt].tolist()))
 
     def stats(self):
-        out = np.zeros(6, dtype=np.int64)
-        check(self._lib.fmmh_stats(self._h, _p(out), 6))
-        keys = ("num_nodes", "num_leaves", "max_depth", "n_m2l", "n_p2p", "p2p_site_pairs")
+        out = np.zeros(7, dtype=np.int64)
+        check(self._lib.fmmh_stats(self._h, _p(out), 7))
+        keys = ("num_nodes", "num_leaves", "max_depth", "n_m2l", "n_p2p", "p2p_site_pairs", "m2l_classes")
         return dict(zip(keys, (int(v) for v in out)))
+
+    def debug_buffer(self, name, count, dtype=np.float64):
+        """First `count` elements of an internal device buffer (white-box tests)."""
+        out = np.zeros(count, dtype=dtype)
+        check(self._lib.fmmh_debug_buffer(self._h, name.encode(), _p(out), out.nbytes))
+        return out
 
     def tree(self, coordinates, ncrit, theta):
         """Octree + interaction lists (pre-order ids, 0-based) as numpy arrays."""

@@ -136,8 +136,14 @@ enum fmmh_option { FMMH_OPT_TIMING = 1 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
 
+/* Copy `bytes` from the start of the context's pooled device buffer `name`
+ * (an internal name such as "e_cm", "e_le", "m_tc") to host memory, for
+ * white-box tests of intermediate results.  Fails if the buffer is unknown
+ * or smaller than `bytes`. */
+int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t bytes);
+
 /* Counters of the last field_fmm call: {num_nodes, num_leaves, max_depth,
- * n_m2l, n_p2p, p2p_site_pairs}. */
+ * n_m2l, n_p2p, p2p_site_pairs, m2l_classes}. */
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap);
 
 #ifdef __cplusplus
