@@ -4,7 +4,8 @@
 // context's stream:
 //   tree     centroid -> keys -> sort -> levels -> scan -> nodes -> links
 //            -> leaves -> level lists -> leaf member order      (2 host syncs)
-//   lists    level-synchronous dual traversal, sorted into CSR  (1 sync/level)
+//   lists    level-synchronous dual traversal, sorted into CSR  (1 host sync)
+//   M2L      displacement classes, class tensors, Hankel contraction (1 sync)
 //   gather   sorted SoA site arrays, Taylor-scaled multipoles
 //   P2M, M2M (level by level), P2P, M2L, L2L (level by level), L2P + scatter
 // Device buffers are pooled per context and only grow.
@@ -58,8 +59,10 @@ struct fmmh_context {
     std::vector<hipEvent_t> event_pool;
     size_t event_used = 0;
     std::vector<std::pair<std::string, double>> phase_ms;
-    int64_t stats[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t stats_classes = 0;
+    double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
+    int list_retries = 0;
     // last tree (for export)
     int num_nodes = 0, num_leaves = 0, max_depth = 0, n_m2l = 0, n_p2p = 0, n_sites = 0;
     std::vector<int> level_off;       // host copy, size KEY_LEVELS + 3
@@ -297,6 +300,7 @@ __global__ void k_fill_i32(int* a, int v, int n) {
 }
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+constexpr int TRAVERSE_GRID = 1024;   // workgroups per traversal step (grid-stride over the frontier)
 
 struct Call {
     const double* X; int64_t ldx;
@@ -465,11 +469,13 @@ struct ListsOut {
 void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes;
-    int cap_f = std::max(4096, 16 * nn);
-    int cap_m = std::max(4096, 64 * nn);
-    int cap_p = std::max(4096, 32 * nn);
+    const double sc = c->list_cap_scale;
+    c->list_retries = 0;
+    int cap_f = std::max(64, (int)std::min(1e9, 64.0 * nn * sc));
+    int cap_m = std::max(64, (int)std::min(1e9, 160.0 * nn * sc));
+    int cap_p = std::max(64, (int)std::min(1e9, 64.0 * nn * sc));
     int* hp = c->h_pinned;
-    for (int attempt = 0; attempt < 8; ++attempt) {
+    for (int attempt = 0; attempt < 64; ++attempt) {
         int2* fa = c->buf<int2>("l_front_a", cap_f);
         int2* fb = c->buf<int2>("l_front_b", cap_f);
         unsigned long long* m2l = c->buf<unsigned long long>("l_m2l", cap_m);
@@ -481,30 +487,33 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
         FMMH_HIP(hipMemcpyAsync(counters, hp + 100, 5 * sizeof(int), hipMemcpyHostToDevice, s));
         std::memcpy(hp + 110, &root, sizeof(root));
         FMMH_HIP(hipMemcpyAsync(fa, hp + 110, sizeof(int2), hipMemcpyHostToDevice, s));
-        int n_in = 1;
+        // Every level of the traversal deepens one side of each surviving
+        // pair, so level_i + level_j grows by one per step: 2 max_depth steps
+        // always drain the frontier.  The frontier size stays on the device
+        // (the kernel reads it), so no host round trip per step.
+        const int steps = 2 * tr.max_depth;
         int cur = 0;
-        bool overflow = false;
-        while (n_in > 0) {
+        for (int it = 0; it < steps; ++it) {
             int2* fin = cur == 0 ? fa : fb;
             int2* fout = cur == 0 ? fb : fa;
             int* cin = counters + cur;
             int* cout = counters + (1 - cur);
             FMMH_HIP(hipMemsetAsync(cout, 0, sizeof(int), s));
-            TraverseArgs a{fin, cin, fout, cout, cap_f, m2l, counters + 2, cap_m, p2p, counters + 3, cap_p,
+            TraverseArgs a{fin, cin, cap_f, fout, cout, cap_f, m2l, counters + 2, cap_m, p2p, counters + 3, cap_p,
                            counters + 4, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf, tr.t.children, theta};
-            launch_traverse(a, std::min(nblk(n_in, 256), 4096), s);
-            FMMH_HIP(hipMemcpyAsync(hp, counters, 5 * sizeof(int), hipMemcpyDeviceToHost, s));
-            c->sync();
-            if (hp[4]) { overflow = true; break; }
-            n_in = hp[1 - cur];
+            launch_traverse(a, TRAVERSE_GRID, s);
             cur = 1 - cur;
         }
-        if (overflow) {
-            if (hp[4] & 1) cap_m *= 2;
-            if (hp[4] & 2) cap_p *= 2;
+        FMMH_HIP(hipMemcpyAsync(hp, counters, 5 * sizeof(int), hipMemcpyDeviceToHost, s));
+        c->sync();
+        if (hp[4]) {
+            ++c->list_retries;
+            if (hp[4] & 1) cap_m = (int)std::min(1e9, std::max(2.0 * cap_m, 1.5 * hp[2]));
+            if (hp[4] & 2) cap_p = (int)std::min(1e9, std::max(2.0 * cap_p, 1.5 * hp[3]));
             if (hp[4] & 4) cap_f *= 2;
             continue;
         }
+        if (hp[cur] != 0) throw Error(FMMH_ERR_INTERNAL, "dual traversal did not drain");
         c->mark("l_traverse");
         L.n_m2l = hp[2];
         L.n_p2p = hp[3];
@@ -644,6 +653,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
     std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
     c->stats[0] = nn; c->stats[1] = nl; c->stats[2] = tr.max_depth;
     c->stats[3] = L.n_m2l; c->stats[4] = L.n_p2p; c->stats[5] = (int64_t)pairs; c->stats[6] = c->stats_classes;
+    c->stats[7] = c->list_retries;
 }
 
 // ---------------------------------------------------------------- host <-> device staging
@@ -904,6 +914,7 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
         if (key == FMMH_OPT_TIMING) c->timing = value != 0.0;
+        else if (key == FMMH_OPT_LIST_CAPACITY) c->list_cap_scale = value > 0.0 ? value : 1.0;
         else throw Error(FMMH_ERR_ARG, "unknown option key " + std::to_string(key));
     });
 }
@@ -941,7 +952,7 @@ int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t b
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
-        for (int i = 0; i < std::min(cap, 7); ++i) out[i] = c->stats[i];
+        for (int i = 0; i < std::min(cap, 8); ++i) out[i] = c->stats[i];
     });
 }
 

@@ -95,7 +95,7 @@ void launch_gather(const SiteArgs& a, hipStream_t s);
 
 // ---------------------------------------------------------------- lists
 struct TraverseArgs {
-    const int2* in; const int* n_in;
+    const int2* in; const int* n_in; int cap_in;   // n_in may exceed cap_in after an overflow
     int2* out; int* n_out; int cap_out;
     unsigned long long* m2l; int* n_m2l; int cap_m2l;
     unsigned long long* p2p; int* n_p2p; int cap_p2p;

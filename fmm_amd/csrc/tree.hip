@@ -37,21 +37,47 @@ __device__ __forceinline__ int key_digit(unsigned long long key, int d) {   // d
 
 // Sequential centroid, one workgroup per axis: sum(coordinates, 1) / N.
 // flang evaluates SUM left to right; a parallel reduction would round
-// differently and move node centres.  All lanes run the same uniform loop
-// so the loads are scalar.
-__global__ __launch_bounds__(64) void k_centroid(const double* __restrict__ X, int64_t ldx, int n,
-                                                 double* __restrict__ root) {
+// differently and move node centres.
+// The chain of dependent adds is the floor; the loads must not be: waves
+// 1..3 stream the next chunk into LDS with coalesced vector loads while
+// wave 0 runs the chain over the current chunk out of LDS (broadcast reads,
+// prefetched one group ahead of the adds).
+constexpr int CENTROID_CHUNK = 2048;
+
+__global__ __launch_bounds__(256) void k_centroid(const double* __restrict__ X, int64_t ldx, int n,
+                                                  double* __restrict__ root) {
 #pragma clang fp contract(off)
+    __shared__ double buf[2][CENTROID_CHUNK];
     const double* col = X + (int64_t)blockIdx.x * ldx;
+    const int tid = threadIdx.x;
+    const int nch = (n + CENTROID_CHUNK - 1) / CENTROID_CHUNK;
+    for (int k = tid; k < CENTROID_CHUNK && k < n; k += 256) buf[0][k] = col[k];
+    __syncthreads();
     double s = 0.0;
-    int i = 0;
-    for (; i + 8 <= n; i += 8) {
-        double v0 = col[i], v1 = col[i + 1], v2 = col[i + 2], v3 = col[i + 3];
-        double v4 = col[i + 4], v5 = col[i + 5], v6 = col[i + 6], v7 = col[i + 7];
-        s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
+    for (int c = 0; c < nch; ++c) {
+        const int cur = c & 1;
+        if (tid >= 64) {
+            const int base = (c + 1) * CENTROID_CHUNK;
+            for (int k = tid - 64; k < CENTROID_CHUNK && base + k < n; k += 192) buf[1 - cur][k] = col[base + k];
+        } else {
+            const double* b = buf[cur];
+            const int len = min(CENTROID_CHUNK, n - c * CENTROID_CHUNK);
+            int k = 0;
+            if (len >= 8) {
+                double v0 = b[0], v1 = b[1], v2 = b[2], v3 = b[3], v4 = b[4], v5 = b[5], v6 = b[6], v7 = b[7];
+                for (k = 8; k + 8 <= len; k += 8) {
+                    const double w0 = b[k], w1 = b[k + 1], w2 = b[k + 2], w3 = b[k + 3];
+                    const double w4 = b[k + 4], w5 = b[k + 5], w6 = b[k + 6], w7 = b[k + 7];
+                    s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
+                    v0 = w0; v1 = w1; v2 = w2; v3 = w3; v4 = w4; v5 = w5; v6 = w6; v7 = w7;
+                }
+                s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
+            }
+            for (; k < len; ++k) s += b[k];
+        }
+        __syncthreads();
     }
-    for (; i < n; ++i) s += col[i];
-    if (threadIdx.x == 0) root[blockIdx.x] = s / (double)n;
+    if (tid == 0) root[blockIdx.x] = s / (double)n;
 }
 
 // max |x - c| per axis (exact, so any reduction order is fine).
@@ -314,7 +340,9 @@ __device__ __forceinline__ int wave_append(int* counter, int want) {
 
 __global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
 #pragma clang fp contract(off)
-    const int n = *a.n_in;
+    // after an overflowing step the count exceeds what was stored: read only
+    // the stored part (the overflow flag is already set and the host retries)
+    const int n = min(*a.n_in, a.cap_in);
     const int stride = gridDim.x * blockDim.x;
     // the loop runs over whole wavefronts so the lane collectives see every lane
     for (int e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); e0 < n; e0 += stride) {
@@ -385,7 +413,7 @@ __global__ __launch_bounds__(256) void k_csr(const unsigned long long* __restric
 static inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
 
 void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, hipStream_t s) {
-    hipLaunchKernelGGL(k_centroid, dim3(3), dim3(64), 0, s, X, ldx, n, t.root);
+    hipLaunchKernelGGL(k_centroid, dim3(3), dim3(256), 0, s, X, ldx, n, t.root);
     FMMH_HIP(hipMemsetAsync(t.maxabs, 0, 3 * sizeof(unsigned long long), s));
     int g = std::min(blocks_for(n, 256), 1024);
     hipLaunchKernelGGL(k_maxabs, dim3(g), dim3(256), 0, s, X, ldx, n, t.root, t.maxabs);

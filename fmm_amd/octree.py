@@ -153,6 +153,10 @@ class Context:
     def set_timing(self, on=True):
         check(self._lib.fmmh_set_option(self._h, 1, 1.0 if on else 0.0))
 
+    def set_list_capacity(self, scale=1.0):
+        """Scale of the initial interaction-list buffers (they grow on overflow)."""
+        check(self._lib.fmmh_set_option(self._h, 2, float(scale)))
+
     def phase_times(self):
         ms = np.zeros(PHASE_CAP)
         names = ctypes.create_string_buffer(4096)
@@ -163,9 +167,10 @@ class Context:
         return dict(zip(keys, ms[:cnt].tolist()))
 
     def stats(self):
-        out = np.zeros(7, dtype=np.int64)
-        check(self._lib.fmmh_stats(self._h, _p(out), 7))
-        keys = ("num_nodes", "num_leaves", "max_depth", "n_m2l", "n_p2p", "p2p_site_pairs", "m2l_classes")
+        out = np.zeros(8, dtype=np.int64)
+        check(self._lib.fmmh_stats(self._h, _p(out), 8))
+        keys = ("num_nodes", "num_leaves", "max_depth", "n_m2l", "n_p2p", "p2p_site_pairs", "m2l_classes",
+                "list_retries")
         return dict(zip(keys, (int(v) for v in out)))
 
     def debug_buffer(self, name, count, dtype=np.float64):

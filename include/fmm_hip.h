@@ -132,7 +132,10 @@ int fmmh_tree_export(fmmh_context* ctx, double* centers, double* r, double* rmax
  * events on the context stream when timing is enabled.  names receives a
  * ';'-separated list of phase names (capacity name_cap bytes).  Returns the
  * number of phases written. */
-enum fmmh_option { FMMH_OPT_TIMING = 1 };
+enum fmmh_option {
+    FMMH_OPT_TIMING = 1,          /* value != 0: record per-phase HIP events */
+    FMMH_OPT_LIST_CAPACITY = 2    /* scale of the initial interaction-list buffers (default 1; they grow on overflow) */
+};
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
 
@@ -143,7 +146,7 @@ int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, in
 int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t bytes);
 
 /* Counters of the last field_fmm call: {num_nodes, num_leaves, max_depth,
- * n_m2l, n_p2p, p2p_site_pairs, m2l_classes}. */
+ * n_m2l, n_p2p, p2p_site_pairs, m2l_classes, list_retries}. */
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap);
 
 #ifdef __cplusplus

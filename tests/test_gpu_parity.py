@@ -112,6 +112,22 @@ def test_tree_and_lists_bit_exact(gpu_ctx, case):
     _tree_equal(gpu_ctx.tree(x, ncrit, theta), oracle.tree(x, ncrit, theta))
 
 
+def test_list_overflow_retry_gives_same_lists_and_field(gpu_ctx):
+    """Tiny initial list buffers: every traversal step overflows, the kernels
+    read and write only inside the buffers, and the host regrows them."""
+    from fmm_amd.octree import Context
+    g = load_golden("uniform4096_p5.npz")
+    ctx = Context(0)
+    try:
+        ctx.set_list_capacity(1e-4)
+        ff = ctx.field_fmm(0, g["coordinates"], g["multipoles"], g["exclusions"], 0.5, 64, 5, 1)
+        assert ctx.stats()["list_retries"] >= 2
+        assert max(block_rel(ff, g["field_fmm"], 1)) < PARITY
+        _tree_equal(ctx.tree(g["coordinates"], 64, 0.5), oracle.tree(g["coordinates"], 64, 0.5))
+    finally:
+        ctx.close()
+
+
 def test_field_is_accumulated(gpu_ctx):
     g = load_golden("charges_small300.npz")
     x, q, e = g["coordinates"], g["multipoles"], g["exclusions"]
