@@ -19,11 +19,21 @@ namespace fmmh {
 
 constexpr int NEAR_EXCL_REG = 4;   // exclusions kept in registers; more are read from memory
 
+// Lane layout: a target block of nt <= 64 sites is replicated over
+// r = 64 / nt lane groups (group g = lane / nt, target lane % nt); group g
+// takes every r-th source, so small leaves still fill the wavefront.  Source
+// runs are staged 64 sites at a time in LDS (one coalesced load per lane per
+// array) and read back per group; at the end the groups' partial fields are
+// summed in group order through LDS (deterministic).
 template <int M, int F, int DAMP>
 __global__ __launch_bounds__(64) void k_near(NearArgs a) {
     constexpr int K = M + F;
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
+    __shared__ double sx[64], sy[64], sz[64], sdf[64];
+    __shared__ double sm[NM][64];
+    __shared__ int sorig[64];
+    __shared__ double red[NF][64];
     const int b = a.blk[blockIdx.x];
     const int lane = threadIdx.x;
     const int t0 = a.tgt_start[b];
@@ -31,8 +41,12 @@ __global__ __launch_bounds__(64) void k_near(NearArgs a) {
     const int e0 = a.list_off[b];
     const int e1 = a.list_off[b + 1];
     for (int base = 0; base < tc; base += 64) {
-        const bool active = base + lane < tc;
-        const int k = t0 + base + (active ? lane : 0);
+        const int nt = min(64, tc - base);
+        const int r = 64 / nt;                   // lane groups
+        const int g = lane / nt;                 // this lane's group (== r for idle lanes)
+        const int ti = lane - g * nt;            // this lane's target within the block
+        const bool active = g < r;
+        const int k = t0 + base + (active ? ti : 0);
         const double xi = a.xs[k], yi = a.ys[k], zi = a.zs[k];
         const double dfi = DAMP != DAMP_NONE ? a.dfs[k] : 0.0;
         int ex[NEAR_EXCL_REG];
@@ -45,37 +59,67 @@ __global__ __launch_bounds__(64) void k_near(NearArgs a) {
             const int sb = a.list_src[e];
             const int s0 = a.src_start[sb];
             const int s1 = s0 + a.src_count[sb];
-            for (int j = s0; j < s1; ++j) {
-                double dx = xi - a.xs[j];
-                double dy = yi - a.ys[j];
-                double dz = zi - a.zs[j];
-                const int oj = a.orig[j] + 1;
-                bool excluded = false;
+            for (int c0 = s0; c0 < s1; c0 += 64) {
+                const int cn = min(64, s1 - c0);
+                if (lane < cn) {
+                    const int j = c0 + lane;
+                    sx[lane] = a.xs[j];
+                    sy[lane] = a.ys[j];
+                    sz[lane] = a.zs[j];
+                    if constexpr (DAMP != DAMP_NONE) sdf[lane] = a.dfs[j];
+                    sorig[lane] = a.orig[j] + 1;
 #pragma unroll
-                for (int q = 0; q < NEAR_EXCL_REG; ++q) excluded |= (ex[q] == oj);
-                for (int q = NEAR_EXCL_REG; q < a.nexcl; ++q) excluded |= (a.excl[(int64_t)q * a.lde + k] == oj);
-                if (excluded) { dx = 1e10; dy = 1e10; dz = 1e10; }
-                double aij = 0.0;
-                if constexpr (DAMP != DAMP_NONE) aij = dfi * a.dfs[j];
-                double m[NM];
+                    for (int s = 0; s < NM; ++s) sm[s][lane] = a.ms[(int64_t)s * a.ldm + j];
+                }
+                __syncthreads();
+                if (active) {
+                    for (int jj = g; jj < cn; jj += r) {
+                        double dx = xi - sx[jj];
+                        double dy = yi - sy[jj];
+                        double dz = zi - sz[jj];
+                        const int oj = sorig[jj];
+                        bool excluded = false;
 #pragma unroll
-                for (int s = 0; s < NM; ++s) m[s] = a.ms[(int64_t)s * a.ldm + j];
-                double g[K + 1];
-                radial<K, DAMP>(dx * dx + dy * dy + dz * dz, aij, g);
-                double T[ncomp(K)];
-                md_tensor<K>(dx, dy, dz, g, T);
-                contract<M, F>(m, T, acc);
+                        for (int q = 0; q < NEAR_EXCL_REG; ++q) excluded |= (ex[q] == oj);
+                        for (int q = NEAR_EXCL_REG; q < a.nexcl; ++q)
+                            excluded |= (a.excl[(int64_t)q * a.lde + k] == oj);
+                        if (excluded) { dx = 1e10; dy = 1e10; dz = 1e10; }
+                        double aij = 0.0;
+                        if constexpr (DAMP != DAMP_NONE) aij = dfi * sdf[jj];
+                        double m[NM];
+#pragma unroll
+                        for (int s = 0; s < NM; ++s) m[s] = sm[s][jj];
+                        double gr[K + 1];
+                        radial<K, DAMP>(dx * dx + dy * dy + dz * dz, aij, gr);
+                        double T[ncomp(K)];
+                        md_tensor<K>(dx, dy, dz, gr, T);
+                        contract<M, F>(m, T, acc);
+                    }
+                }
+                __syncthreads();
             }
         }
-        if (active) {
+        // sum the r partial fields of each target in group order
+#pragma unroll
+        for (int t = 0; t < NF; ++t) red[t][lane] = acc[t];
+        __syncthreads();
+        if (lane < nt) {
+#pragma unroll
+            for (int t = 0; t < NF; ++t) {
+                double v = red[t][lane];
+                for (int q = 1; q < r; ++q) v += red[t][lane + q * nt];
+                acc[t] = v;
+            }
+            const int kk = t0 + base + lane;
             if (a.accumulate) {
 #pragma unroll
-                for (int t = 0; t < NF; ++t) a.out[(int64_t)t * a.ldo + k] += acc[t];
+                for (int t = 0; t < NF; ++t) a.out[(int64_t)t * a.ldo + kk] += acc[t];
             } else {
 #pragma unroll
-                for (int t = 0; t < NF; ++t) a.out[(int64_t)t * a.ldo + k] = acc[t];
+                for (int t = 0; t < NF; ++t) a.out[(int64_t)t * a.ldo + kk] = acc[t];
             }
         }
+        __syncthreads();
     }
 }
 
