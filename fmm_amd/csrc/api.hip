@@ -471,7 +471,7 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
     const int nn = tr.num_nodes;
     const double sc = c->list_cap_scale;
     c->list_retries = 0;
-    int cap_f = std::max(64, (int)std::min(1e9, 64.0 * nn * sc));
+    int cap_f = std::max(64, (int)std::min(1e9, 160.0 * nn * sc));
     int cap_m = std::max(64, (int)std::min(1e9, 160.0 * nn * sc));
     int cap_p = std::max(64, (int)std::min(1e9, 64.0 * nn * sc));
     int* hp = c->h_pinned;
