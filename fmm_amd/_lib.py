@@ -20,6 +20,14 @@ _PROTOS = {
                              vp, i64, i32, vp]),
     "fmmh_field_fmm_dev": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, vp, i64, i32, f64, i32, i32, i32, vp, i64,
                                  i32, vp, i64, i32, vp]),
+    "fmmh_field_direct_sets": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, i32, vp, i64, i32, i32, vp, i64, i32, vp,
+                                     i64]),
+    "fmmh_field_direct_sets_dev": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, i32, vp, i64, i32, i32, vp, i64, i32,
+                                         vp, i64]),
+    "fmmh_field_fmm_sets": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, i32, vp, i64, i32, f64, i32, i32, i32, vp, i64,
+                                  i32, vp, i64, i32, vp]),
+    "fmmh_field_fmm_sets_dev": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, i32, vp, i64, i32, f64, i32, i32, i32, vp,
+                                      i64, i32, vp, i64, i32, vp]),
     "fmmh_tree_build": (i32, [vp, vp, i64, i32, i32, f64, vp]),
     "fmmh_tree_export": (i32, [vp] + [vp] * 13),
     "fmmh_set_option": (i32, [vp, i32, f64]),

@@ -62,6 +62,7 @@ struct fmmh_context {
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
+    int m2l_mfma = 1;                 // 0 never, 1 for >= 16 sets, 2 whenever the order allows
     int list_retries = 0;
     // last tree (for export)
     int num_nodes = 0, num_leaves = 0, max_depth = 0, n_m2l = 0, n_p2p = 0, n_sites = 0;
@@ -309,6 +310,7 @@ struct Call {
     const int* E; int64_t lde; int nexcl;
     int F; double* field; int64_t ldf;
     int damp; const double* df;
+    int nsets;   // multipoles n x dim x nsets and field n x nf x nsets, one leading dimension each
 };
 
 void check_common(const Call& c, int64_t n_damping) {
@@ -316,6 +318,7 @@ void check_common(const Call& c, int64_t n_damping) {
     if (c.ldx < c.n || c.ldm < c.n || c.ldf < c.n || (c.nexcl > 0 && c.lde < c.n))
         throw Error(FMMH_ERR_ARG, "leading dimension smaller than n");
     if (c.nexcl < 0) throw Error(FMMH_ERR_ARG, "nexcl must be >= 0");
+    if (c.nsets < 1) throw Error(FMMH_ERR_ARG, "nsets must be >= 1");
     if (c.damp < 0 || c.damp > 3) throw Error(FMMH_ERR_DAMP, "Unknown damp type code");
     if (c.damp != FMMH_DAMP_NONE && (n_damping < c.n || c.df == nullptr))
         throw Error(FMMH_ERR_ARG, "damping_factors must hold n values when damping is on");
@@ -351,8 +354,8 @@ void run_direct(fmmh_context* c, const Call& k) {
     int* s_start = c->buf<int>("d_sstart", 1);
     int* s_count = c->buf<int>("d_scount", 1);
     int* iota = c->buf<int>("iota", std::max(n, nb));
-    double* ms = c->buf<double>("d_ms", (size_t)n * k.dim);
-    double* ms_raw = c->buf<double>("d_ms_raw", (size_t)n * k.dim);
+    double* ms = c->buf<double>("d_ms", (size_t)n * k.dim * k.nsets);
+    double* ms_raw = c->buf<double>("d_ms_raw", (size_t)n * k.dim * k.nsets);
     double* xs = c->buf<double>("d_xs", n);
     double* ys = c->buf<double>("d_ys", n);
     double* zs = c->buf<double>("d_zs", n);
@@ -364,11 +367,11 @@ void run_direct(fmmh_context* c, const Call& k) {
     FMMH_HIP(hipMemcpyAsync(s_start, &zero_n[0], sizeof(int), hipMemcpyHostToDevice, s));
     FMMH_HIP(hipMemcpyAsync(s_count, &zero_n[1], sizeof(int), hipMemcpyHostToDevice, s));
     SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, iota, n,
-               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale};
+               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale, k.nsets};
     launch_gather(g, s);
     c->mark("gather");
     NearArgs a{iota, start, count, loff, lsrc, s_start, s_count, xs, ys, zs, ms, n, dfs, iota, excl, n, k.nexcl,
-               k.field, k.ldf, 1};
+               k.field, k.ldf, 1, k.nsets};
     launch_near(a, nb, m, k.F, k.damp, s);
     c->mark("direct");
     FMMH_HIP(hipStreamSynchronize(s));
@@ -538,13 +541,12 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
 
 // ---------------------------------------------------------------- M2L
 // Displacement classes (sort + unique), one tensor per class, then the
-// target-centric Hankel contraction.  One host sync to size the class table.
-void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, const double* CM, double* LE) {
+// target-centric Hankel contraction of every set.  One host sync to size the
+// class table.  Many sets go through the MFMA form in chunks of up to 32 sets.
+void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets, const double* CM, double* LE) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes, np = L.n_m2l;
     const int nm = ncomp(p), nT = ncomp(2 * p);
-    double* CMs = c->buf<double>("m_cms", (size_t)nn * nm);
-    launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, s);
     int* pair_class = c->buf<int>("m_pair_class", np);
     int* dev_scalars = c->buf<int>("m_scalars", 4);   // num_classes, error
     FMMH_HIP(hipMemsetAsync(dev_scalars, 0, 4 * sizeof(int), s));
@@ -584,8 +586,20 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, const
         launch_m2l_class_tensors(ca, std::min(ncls, 4096), s);
     }
     c->mark("m2l_classes");
-    M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE, tr.t.level, tr.t.r, nm, p};
-    launch_m2l(ma, nn, s);
+    const bool mfma = c->m2l_mfma == 2 ? m2l_mfma_supported(p, 16)
+                                       : (c->m2l_mfma == 1 && m2l_mfma_supported(p, nsets));
+    const int chunk = mfma ? 32 : nsets;
+    const size_t node_stride = (size_t)nn * nm;
+    double* CMs = c->buf<double>("m_cms", node_stride * (mfma ? std::min(32, ((nsets + 15) / 16) * 16) : nsets));
+    for (int c0 = 0; c0 < nsets; c0 += chunk) {
+        const int ns = std::min(chunk, nsets - c0);
+        const int s_ld = mfma ? (ns <= 16 ? 16 : 32) : 0;
+        launch_scale_sources(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, s_ld, s);
+        M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,
+                   nm, p, ns, s_ld, nn};
+        if (mfma) launch_m2l_mfma(ma, nn, s);
+        else launch_m2l(ma, nn, s);
+    }
 }
 
 // ---------------------------------------------------------------- field_fmm
@@ -609,33 +623,34 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
     double* xs = c->buf<double>("s_xs", n);
     double* ys = c->buf<double>("s_ys", n);
     double* zs = c->buf<double>("s_zs", n);
-    double* ms_raw = c->buf<double>("s_ms_raw", (size_t)n * k.dim);
-    double* ms = c->buf<double>("s_ms", (size_t)n * k.dim);
+    const int S = k.nsets;
+    double* ms_raw = c->buf<double>("s_ms_raw", (size_t)n * k.dim * S);
+    double* ms = c->buf<double>("s_ms", (size_t)n * k.dim * S);
     double* dfs = c->buf<double>("s_dfs", n);
     int* excl = c->buf<int>("s_excl", (size_t)n * std::max(k.nexcl, 1));
     SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, tr.t.perm, n,
-               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale};
+               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale, S};
     launch_gather(g, s);
     c->mark("gather");
-    double* CM = c->buf<double>("e_cm", (size_t)nn * nm);
-    double* LE = c->buf<double>("e_le", (size_t)nn * nm);
+    double* CM = c->buf<double>("e_cm", (size_t)nn * nm * S);
+    double* LE = c->buf<double>("e_le", (size_t)nn * nm * S);
     UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
-              tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis};
+              tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, S, k.dim};
     launch_p2m(up, nl, s);
     for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
         int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
         launch_m2m(up, tr.t.level_nodes + o0, o1 - o0, s);
     }
     c->mark("upward");
-    double* near = c->buf<double>("e_near", (size_t)n * nf);
+    double* near = c->buf<double>("e_near", (size_t)n * nf * S);
     NearArgs na{tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
-                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0};
+                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S};
     launch_near(na, nl, m, k.F, k.damp, s);
     c->mark("p2p");
-    run_m2l(c, tr, L, tb, p, CM, LE);
+    run_m2l(c, tr, L, tb, p, S, CM, LE);
     c->mark("m2l");
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
-                tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf};
+                tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S};
     for (int lev = 2; lev <= tr.max_depth; ++lev) {
         int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
         launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
@@ -669,19 +684,19 @@ Staged stage_inputs(fmmh_context* c, const Call& h, int64_t n_damping) {
     st.k = h;
     st.nf = ncomp(h.F);
     double* X = c->buf<double>("h_X", (size_t)n * 3);
-    double* M = c->buf<double>("h_M", (size_t)n * h.dim);
+    double* M = c->buf<double>("h_M", (size_t)n * h.dim * h.nsets);
     int* E = c->buf<int>("h_E", (size_t)n * std::max(h.nexcl, 1));
-    double* F = c->buf<double>("h_F", (size_t)n * st.nf);
+    double* F = c->buf<double>("h_F", (size_t)n * st.nf * h.nsets);
     double* df = c->buf<double>("h_df", n);
     FMMH_HIP(hipMemcpy2DAsync(X, n * sizeof(double), h.X, h.ldx * sizeof(double), n * sizeof(double), 3,
                               hipMemcpyHostToDevice, s));
-    FMMH_HIP(hipMemcpy2DAsync(M, n * sizeof(double), h.M, h.ldm * sizeof(double), n * sizeof(double), h.dim,
-                              hipMemcpyHostToDevice, s));
+    FMMH_HIP(hipMemcpy2DAsync(M, n * sizeof(double), h.M, h.ldm * sizeof(double), n * sizeof(double),
+                              (size_t)h.dim * h.nsets, hipMemcpyHostToDevice, s));
     if (h.nexcl > 0)
         FMMH_HIP(hipMemcpy2DAsync(E, n * sizeof(int), h.E, h.lde * sizeof(int), n * sizeof(int), h.nexcl,
                                   hipMemcpyHostToDevice, s));
-    FMMH_HIP(hipMemcpy2DAsync(F, n * sizeof(double), h.field, h.ldf * sizeof(double), n * sizeof(double), st.nf,
-                              hipMemcpyHostToDevice, s));
+    FMMH_HIP(hipMemcpy2DAsync(F, n * sizeof(double), h.field, h.ldf * sizeof(double), n * sizeof(double),
+                              (size_t)st.nf * h.nsets, hipMemcpyHostToDevice, s));
     if (h.damp != FMMH_DAMP_NONE)
         FMMH_HIP(hipMemcpyAsync(df, h.df, n * sizeof(double), hipMemcpyHostToDevice, s));
     (void)n_damping;
@@ -695,13 +710,14 @@ Staged stage_inputs(fmmh_context* c, const Call& h, int64_t n_damping) {
 
 void unstage_field(fmmh_context* c, const Staged& st, double* field, int64_t ldf) {
     FMMH_HIP(hipMemcpy2DAsync(field, ldf * sizeof(double), st.k.field, st.k.n * sizeof(double),
-                              st.k.n * sizeof(double), st.nf, hipMemcpyDeviceToHost, c->stream));
+                              st.k.n * sizeof(double), (size_t)st.nf * st.k.nsets, hipMemcpyDeviceToHost,
+                              c->stream));
     c->sync();
 }
 
 Call make_call(const double* X, int64_t ldx, const double* M, int64_t ldm, int n, int dim, const int* E, int64_t lde,
-               int nexcl, int F, double* field, int64_t ldf, int damp, const double* df) {
-    return Call{X, ldx, M, ldm, n, dim, E, lde, nexcl, F, field, ldf, damp, df};
+               int nexcl, int F, double* field, int64_t ldf, int damp, const double* df, int nsets = 1) {
+    return Call{X, ldx, M, ldm, n, dim, E, lde, nexcl, F, field, ldf, damp, df, nsets};
 }
 
 void check_fmm_args(double theta, int ncrit, int p, int is_periodic) {
@@ -753,47 +769,33 @@ int fmmh_parse_damp_type(const char* s, int64_t len, int32_t* code) {
     });
 }
 
-int fmmh_field_direct(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
-                      const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
-                      const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
-                      double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
-                      int64_t n_damping) {
-    (void)comm;
+static int field_direct_impl(fmmh_context* ctx, bool host, const double* coordinates, int64_t ld_coordinates,
+                             const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
+                             const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
+                             double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                             int64_t n_damping) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
         Call h = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
-                           nexcl, field_order, field, ld_field, damp_type, damping_factors);
+                           nexcl, field_order, field, ld_field, damp_type, damping_factors, nsets);
         check_common(h, n_damping);
-        check_exclusions_host(h);
+        if (host) check_exclusions_host(h);
         c->begin_timing();
-        Staged st = stage_inputs(c, h, n_damping);
-        c->mark("h2d");
-        run_direct(c, st.k);
-        unstage_field(c, st, field, ld_field);
-        c->mark("d2h");
-        c->end_timing();
-    });
-}
-
-int fmmh_field_direct_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
-                          const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
-                          const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
-                          double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
-                          int64_t n_damping) {
-    (void)comm;
-    return guarded([&] {
-        fmmh_context* c = resolve(ctx);
-        Call k = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
-                           nexcl, field_order, field, ld_field, damp_type, damping_factors);
-        check_common(k, n_damping);
-        c->begin_timing();
-        run_direct(c, k);
+        if (host) {
+            Staged st = stage_inputs(c, h, n_damping);
+            c->mark("h2d");
+            run_direct(c, st.k);
+            unstage_field(c, st, field, ld_field);
+            c->mark("d2h");
+        } else {
+            run_direct(c, h);
+        }
         c->end_timing();
     });
 }
 
 static int field_fmm_impl(fmmh_context* ctx, bool host, const double* coordinates, int64_t ld_coordinates,
-                          const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                          const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
                           const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, double theta,
                           int32_t ncrit, int32_t expansion_order, int32_t field_order, double* field,
                           int64_t ld_field, int32_t damp_type, const double* damping_factors, int64_t n_damping,
@@ -801,7 +803,7 @@ static int field_fmm_impl(fmmh_context* ctx, bool host, const double* coordinate
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
         Call h = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
-                           nexcl, field_order, field, ld_field, damp_type, damping_factors);
+                           nexcl, field_order, field, ld_field, damp_type, damping_factors, nsets);
         // src/fmm.F90:1372-1382: damp_type first, then the periodic checks
         if (damp_type < 0 || damp_type > 3) throw Error(FMMH_ERR_DAMP, "Unknown damp type code");
         if (is_periodic && n <= ncrit)
@@ -827,13 +829,35 @@ static int field_fmm_impl(fmmh_context* ctx, bool host, const double* coordinate
     });
 }
 
+int fmmh_field_direct(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                      const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                      const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
+                      double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                      int64_t n_damping) {
+    (void)comm;
+    return field_direct_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, 1, exclusions,
+                             ld_exclusions, nexcl, field_order, field, ld_field, damp_type, damping_factors,
+                             n_damping);
+}
+
+int fmmh_field_direct_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                          const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                          const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
+                          double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                          int64_t n_damping) {
+    (void)comm;
+    return field_direct_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, 1,
+                             exclusions, ld_exclusions, nexcl, field_order, field, ld_field, damp_type,
+                             damping_factors, n_damping);
+}
+
 int fmmh_field_fmm(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
                    const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, const int32_t* exclusions,
                    int64_t ld_exclusions, int32_t nexcl, double theta, int32_t ncrit, int32_t expansion_order,
                    int32_t field_order, double* field, int64_t ld_field, int32_t damp_type,
                    const double* damping_factors, int64_t n_damping, int32_t is_periodic, const double* box_matrix) {
     (void)comm; (void)box_matrix;
-    return field_fmm_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions,
+    return field_fmm_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, 1, exclusions,
                           ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field, ld_field,
                           damp_type, damping_factors, n_damping, is_periodic);
 }
@@ -845,9 +869,55 @@ int fmmh_field_fmm_dev(fmmh_context* ctx, int32_t comm, const double* coordinate
                        int32_t damp_type, const double* damping_factors, int64_t n_damping, int32_t is_periodic,
                        const double* box_matrix) {
     (void)comm; (void)box_matrix;
-    return field_fmm_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions,
+    return field_fmm_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, 1, exclusions,
                           ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field, ld_field,
                           damp_type, damping_factors, n_damping, is_periodic);
+}
+
+int fmmh_field_direct_sets(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                           const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
+                           const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, int32_t field_order,
+                           double* field, int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                           int64_t n_damping) {
+    (void)comm;
+    return field_direct_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, nsets,
+                             exclusions, ld_exclusions, nexcl, field_order, field, ld_field, damp_type,
+                             damping_factors, n_damping);
+}
+
+int fmmh_field_direct_sets_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                               const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                               int32_t nsets, const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                               int32_t field_order, double* field, int64_t ld_field, int32_t damp_type,
+                               const double* damping_factors, int64_t n_damping) {
+    (void)comm;
+    return field_direct_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, nsets,
+                             exclusions, ld_exclusions, nexcl, field_order, field, ld_field, damp_type,
+                             damping_factors, n_damping);
+}
+
+int fmmh_field_fmm_sets(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                        const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
+                        const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, double theta, int32_t ncrit,
+                        int32_t expansion_order, int32_t field_order, double* field, int64_t ld_field,
+                        int32_t damp_type, const double* damping_factors, int64_t n_damping, int32_t is_periodic,
+                        const double* box_matrix) {
+    (void)comm; (void)box_matrix;
+    return field_fmm_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, nsets,
+                          exclusions, ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field,
+                          ld_field, damp_type, damping_factors, n_damping, is_periodic);
+}
+
+int fmmh_field_fmm_sets_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                            const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
+                            const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, double theta,
+                            int32_t ncrit, int32_t expansion_order, int32_t field_order, double* field,
+                            int64_t ld_field, int32_t damp_type, const double* damping_factors, int64_t n_damping,
+                            int32_t is_periodic, const double* box_matrix) {
+    (void)comm; (void)box_matrix;
+    return field_fmm_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, nsets,
+                          exclusions, ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field,
+                          ld_field, damp_type, damping_factors, n_damping, is_periodic);
 }
 
 int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates, int32_t n, int32_t ncrit,
@@ -915,6 +985,10 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         fmmh_context* c = resolve(ctx);
         if (key == FMMH_OPT_TIMING) c->timing = value != 0.0;
         else if (key == FMMH_OPT_LIST_CAPACITY) c->list_cap_scale = value > 0.0 ? value : 1.0;
+        else if (key == FMMH_OPT_M2L_MFMA) {
+            if (value != 0.0 && value != 1.0 && value != 2.0) throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1 or 2");
+            c->m2l_mfma = (int)value;
+        }
         else throw Error(FMMH_ERR_ARG, "unknown option key " + std::to_string(key));
     });
 }

@@ -35,10 +35,13 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
     __shared__ double mono[UP_CHUNK][MAX_NM];
     __shared__ double mch[UP_CHUNK][UP_NSRC];
     const int node = a.leaf_node[blockIdx.x];
+    const int set = blockIdx.y;
     const int s0 = a.start[node], c = a.count[node];
     const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
     const int nm = a.nm;
     const int nsrc = a.nsrc;
+    const double* __restrict__ msr = a.ms_raw + (int64_t)set * a.dim * a.ldm;
+    double* __restrict__ CM = a.CM + (int64_t)set * a.num_nodes * nm;
     double acc[(MAX_NM + 127) / 128];
 #pragma unroll
     for (int q = 0; q < (MAX_NM + 127) / 128; ++q) acc[q] = 0.0;
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
         }
         for (int x = threadIdx.x; x < cc * nsrc; x += blockDim.x) {
             const int jj = x / nsrc, sc = x % nsrc;
-            mch[jj][sc] = a.ms_raw[(int64_t)sc * a.ldm + s0 + j0 + jj];
+            mch[jj][sc] = msr[(int64_t)sc * a.ldm + s0 + j0 + jj];
         }
         __syncthreads();
 #pragma unroll
@@ -71,7 +74,7 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
 #pragma unroll
     for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
         const int t = threadIdx.x + 128 * q;
-        if (t < nm) a.CM[(int64_t)node * nm + t] = acc[q];
+        if (t < nm) CM[(int64_t)node * nm + t] = acc[q];
     }
 }
 
@@ -79,6 +82,7 @@ __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ n
     __shared__ double mono[MAX_NM];
     const int node = nodes[blockIdx.x];
     const int nm = a.nm;
+    double* __restrict__ CM = a.CM + (int64_t)blockIdx.y * a.num_nodes * nm;
     const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
     double acc[(MAX_NM + 127) / 128];
 #pragma unroll
@@ -91,7 +95,7 @@ __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ n
         if (threadIdx.x == 0)
             build_mono(mono, nm, a.cx[ch] - cx, a.cy[ch] - cy, a.cz[ch] - cz, a.mono_parent, a.mono_axis);
         __syncthreads();
-        const double* src = a.CM + (int64_t)ch * nm;
+        const double* src = CM + (int64_t)ch * nm;
 #pragma unroll
         for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
             const int t = threadIdx.x + 128 * q;
@@ -108,7 +112,7 @@ __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ n
 #pragma unroll
     for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
         const int t = threadIdx.x + 128 * q;
-        if (t < nm) a.CM[(int64_t)node * nm + t] = acc[q];
+        if (t < nm) CM[(int64_t)node * nm + t] = acc[q];
     }
 }
 
@@ -119,16 +123,17 @@ __global__ __launch_bounds__(128) void k_l2l(DownArgs a, const int* __restrict__
     const int node = nodes[blockIdx.x];
     const int par = a.parent[node];
     const int nm = a.nm;
+    double* __restrict__ LE = a.LE + (int64_t)blockIdx.y * a.num_nodes * nm;
     if (threadIdx.x == 0)
         build_mono(mono, nm, a.cx[node] - a.cx[par], a.cy[node] - a.cy[par], a.cz[node] - a.cz[par], a.mono_parent,
                    a.mono_axis);
-    for (int s = threadIdx.x; s < nm; s += blockDim.x) lp[s] = a.LE[(int64_t)par * nm + s];
+    for (int s = threadIdx.x; s < nm; s += blockDim.x) lp[s] = LE[(int64_t)par * nm + s];
     __syncthreads();
     for (int t = threadIdx.x; t < nm; t += blockDim.x) {
         const int b0 = a.l2l.off[t], b1 = a.l2l.off[t + 1];
         double sum = 0.0;
         for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.l2l.coef[b] * mono[a.l2l.e[b]], lp[a.l2l.s[b]], sum);
-        a.LE[(int64_t)node * nm + t] += sum;
+        LE[(int64_t)node * nm + t] += sum;
     }
 }
 
@@ -139,9 +144,12 @@ __global__ __launch_bounds__(64) void k_l2p(DownArgs a) {
     __shared__ double L[MAX_NM];
     __shared__ double fact[PMAX + 1];
     const int node = a.leaf_node[blockIdx.x];
+    const int set = blockIdx.y;
     const int s0 = a.start[node], c = a.count[node];
     const int nm = a.nm, p = a.p;
-    for (int s = threadIdx.x; s < nm; s += 64) L[s] = a.LE[(int64_t)node * nm + s];
+    const double* __restrict__ near = a.near + (int64_t)set * NF * a.ld_near;
+    double* __restrict__ field = a.field + (int64_t)set * NF * a.ld_field;
+    for (int s = threadIdx.x; s < nm; s += 64) L[s] = a.LE[((int64_t)set * a.num_nodes + node) * nm + s];
     if (threadIdx.x == 0) {
         double f = 1.0;
         fact[0] = 1.0;
@@ -180,50 +188,37 @@ __global__ __launch_bounds__(64) void k_l2p(DownArgs a) {
         const int o = a.perm[k];
 #pragma unroll
         for (int t = 0; t < NF; ++t) {
-            double* fp = a.field + (int64_t)t * a.ld_field + o;
-            *fp = (*fp + a.near[(int64_t)t * a.ld_near + k]) + f[t];
+            double* fp = field + (int64_t)t * a.ld_field + o;
+            *fp = (*fp + near[(int64_t)t * a.ld_near + k]) + f[t];
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_accumulate(const double* __restrict__ near, int64_t ldn,
-                                                    double* __restrict__ field, int64_t ldf, int n, int nf) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (int t = 0; t < nf; ++t) field[(int64_t)t * ldf + i] += near[(int64_t)t * ldn + i];
-}
-
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
-    hipLaunchKernelGGL(k_p2m, dim3(num_leaves), dim3(128), 0, s, a);
+    hipLaunchKernelGGL(k_p2m, dim3(num_leaves, a.nsets), dim3(128), 0, s, a);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_m2m, dim3(count), dim3(128), 0, s, a, nodes);
+    hipLaunchKernelGGL(k_m2m, dim3(count, a.nsets), dim3(128), 0, s, a, nodes);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_l2l, dim3(count), dim3(128), 0, s, a, nodes);
+    hipLaunchKernelGGL(k_l2l, dim3(count, a.nsets), dim3(128), 0, s, a, nodes);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
     switch (a.nf) {
-        case 1: hipLaunchKernelGGL(k_l2p<0>, dim3(num_leaves), dim3(64), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_l2p<1>, dim3(num_leaves), dim3(64), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_l2p<2>, dim3(num_leaves), dim3(64), 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_l2p<0>, dim3(num_leaves, a.nsets), dim3(64), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_l2p<1>, dim3(num_leaves, a.nsets), dim3(64), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_l2p<2>, dim3(num_leaves, a.nsets), dim3(64), 0, s, a); break;
     }
-    FMMH_HIP(hipGetLastError());
-}
-
-void launch_accumulate(const double* near, int64_t ld_near, double* field, int64_t ld_field, int n, int nf,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, near, ld_near, field, ld_field, n, nf);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -29,6 +29,7 @@ struct NearArgs {
     double* out;               // per sorted site, ncomp(F) components
     int64_t ldo;
     int accumulate;            // 0: out = result, 1: out += result
+    int nsets;                 // multipole sets: ms is [set][component][ldm], out is [set][component][ldo]
 };
 
 bool near_supported(int m, int F);
@@ -90,6 +91,7 @@ struct SiteArgs {
     double* dfs;
     int* excl_s;                         // component-major, leading dimension n
     const double* scale;                 // per multipole component factor (device, dim entries)
+    int nsets;                           // M is [set][component][ldm]; ms_* are [set][component][n]
 };
 void launch_gather(const SiteArgs& a, hipStream_t s);
 
@@ -129,6 +131,9 @@ struct UpArgs {
     ShiftTab p2m; ShiftTab m2m;
     const int* mono_parent;   // packed index -> packed index with one exponent less
     const int* mono_axis;     //   and which axis was removed
+    int num_nodes;            // CM is [set][node][nm]; ms_raw is [set][component][ldm]
+    int nsets;                // one grid row per set
+    int dim;                  // multipole components per site and set
 };
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s);
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s);
@@ -165,19 +170,31 @@ void launch_m2l_flags(const M2LClassArgs& a, hipStream_t s);        // after the
 void launch_m2l_assign(const M2LClassArgs& a, hipStream_t s);       // after the flag scan
 void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s);
 
+// Expansions of S multipole sets are stored set-major, CM[(set * nodes + node)
+// * nm + s] and likewise LE.  The scaled sources CMs that M2L reads are
+// set-major for k_m2l (s_ld = 0) and set-fastest, CMs[(node * nm + s) * s_ld +
+// set] with s_ld padded to 16 or 32, for the MFMA form.
 struct M2LArgs {
     const int* off; const int* src;
     const int* pair_class;
     const double* Tc;
-    const double* CMs;        // scaled source expansions M~
+    const double* CMs;
     double* LE;
     const int* level;
     const double* r;
     int nm; int p;
+    int nsets;
+    int s_ld;
+    int num_nodes;
 };
 void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
-                          hipStream_t s);
+                          int nsets, int s_ld_out, hipStream_t s);
 void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s);
+// MFMA form for many sets: one wavefront per target accumulates its n_m x S
+// local expansions (S = a.nsets <= a.s_ld, s_ld 16 or 32) over all its pairs
+// with v_mfma_f64_16x16x4f64.  Supported from 16 sets on (callers chunk by 32).
+bool m2l_mfma_supported(int p, int nsets);
+void launch_m2l_mfma(const M2LArgs& a, int num_nodes, hipStream_t s);
 
 struct DownArgs {
     const double* cx; const double* cy; const double* cz;
@@ -192,13 +209,11 @@ struct DownArgs {
     const double* near; int64_t ld_near;   // P2P result per sorted site
     double* field; int64_t ld_field;       // accumulated at original indices
     int nf;
+    int num_nodes;                         // LE is [set][node][nm]; near and field are [set][component][ld]
+    int nsets;                             // one grid row per set
 };
 void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s);
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s);
-
-// field[i, t] += near[i, t] for i < n (direct sum, original order)
-void launch_accumulate(const double* near, int64_t ld_near, double* field, int64_t ld_field, int n, int nf,
-                       hipStream_t s);
 
 // ---------------------------------------------------------------- primitives (rocPRIM)
 size_t sort_pairs_u64_temp(int n);

@@ -125,18 +125,27 @@ __global__ __launch_bounds__(64) void k_class_tensor(M2LClassArgs a) {
     }
 }
 
+// M~_j(s) = c_s M_j(s) r_j^{-|s|} from the set-major CM; written set-major
+// (s_out == 0) or set-fastest with s_out >= nsets sets per coefficient
+// (padding sets zero, for the MFMA tiles).
 __global__ __launch_bounds__(256) void k_scale_sources(const double* __restrict__ CM, const double* __restrict__ cs,
                                                        const double* __restrict__ r, double* __restrict__ CMs,
-                                                       int num_nodes, int nm) {
+                                                       int num_nodes, int nm, int nsets, int s_out) {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= (int64_t)num_nodes * nm) return;
-    const int j = (int)(x / nm), s = (int)(x - (int64_t)j * nm);
+    const int width = s_out > 0 ? s_out : nsets;
+    if (x >= (int64_t)num_nodes * nm * width) return;
+    int64_t js;
+    int set;
+    if (s_out > 0) { js = x / s_out; set = (int)(x - js * s_out); }
+    else { set = (int)(x / ((int64_t)num_nodes * nm)); js = x - (int64_t)set * num_nodes * nm; }
+    const int j = (int)(js / nm), s = (int)(js - (int64_t)j * nm);
+    if (set >= nsets) { CMs[x] = 0.0; return; }
     int k = 0;
     while (ncomp(k) <= s) ++k;
     const double rinv = 1.0 / r[j];
     double f = 1.0;
     for (int q = 0; q < k; ++q) f *= rinv;
-    CMs[x] = cs[s] * CM[x] * f;
+    CMs[x] = cs[s] * CM[(int64_t)set * num_nodes * nm + js] * f;
 }
 
 constexpr int M2L_LDS_SOURCE_P = 8;   // from this order on, source expansions are staged in LDS
@@ -166,6 +175,8 @@ __global__ __launch_bounds__(128, 4) void k_m2l(M2LArgs a) {
     __shared__ double Tl[2][NT];
     __shared__ double Ms[P >= M2L_LDS_SOURCE_P ? NM : 1];
     const int i = blockIdx.x;
+    const int set = blockIdx.y;                  // one multipole set per grid row
+    const int64_t set_off = (int64_t)set * a.num_nodes * NM;   // set-major CMs and LE
     const int tid = threadIdx.x;
     const int e0 = a.off[i], e1 = a.off[i + 1];
     const int dec0 = decode_target(min(tid, NM - 1));
@@ -191,7 +202,7 @@ __global__ __launch_bounds__(128, 4) void k_m2l(M2LArgs a) {
         }
         const int j = a.src[e];
         const int dl = li - a.level[j];
-        const double* __restrict__ Mj = a.CMs + (int64_t)j * NM;
+        const double* __restrict__ Mj = a.CMs + set_off + (int64_t)j * NM;
         const double* __restrict__ T = Tl[cur];
         if constexpr (P >= M2L_LDS_SOURCE_P) {
             // large orders: the source expansion goes through LDS as well, the
@@ -252,7 +263,7 @@ __global__ __launch_bounds__(128, 4) void k_m2l(M2LArgs a) {
             const int dec = q == 0 ? dec0 : (q == 1 ? dec1 : dec2);
             double f = rinv;
             for (int k = 0; k < (dec & 255); ++k) f *= rinv;
-            a.LE[(int64_t)i * NM + t] = (q == 0 ? acc0 : (q == 1 ? acc1 : acc2)) * f;
+            a.LE[set_off + (int64_t)i * NM + t] = (q == 0 ? acc0 : (q == 1 ? acc1 : acc2)) * f;
         }
     }
 }
@@ -284,16 +295,163 @@ void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s) {
 }
 
 void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
-                          hipStream_t s) {
+                          int nsets, int s_ld_out, hipStream_t s) {
     const int nm = ncomp(p);
-    hipLaunchKernelGGL(k_scale_sources, dim3(nblk((long long)num_nodes * nm, 256)), dim3(256), 0, s, CM, cs, r, CMs,
-                       num_nodes, nm);
+    const int width = s_ld_out > 0 ? s_ld_out : nsets;
+    hipLaunchKernelGGL(k_scale_sources, dim3(nblk((long long)num_nodes * nm * width, 256)), dim3(256), 0, s, CM, cs,
+                       r, CMs, num_nodes, nm, nsets, s_ld_out);
+    FMMH_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- MFMA form
+// For S >= 16 multipole sets, the contraction of one pair is a small GEMM
+//   L~_i[t][set] += sum_s H_c[t][s] M~_j[s][set],  H_c[t][s] = scale(t) T~_c[s+t]
+// with n_m x n_m Hankel A (built on the fly from the class tensor in LDS) and
+// n_m x S source B.  One wavefront per target node keeps the whole n_m x S
+// result in MFMA accumulators across all its pairs (v_mfma_f64_16x16x4f64:
+// A lane l -> A[l % 16][l / 16], B lane l -> B[l / 16][l % 16], accumulator
+// register q of lane l -> D[4 q + l / 16][l % 16], measured on gfx950).  The packed index of s + t for every A-fragment
+// lane is tabulated in LDS once per workgroup; the level-offset factor is a
+// per-row scale of A.
+using v4d = double __attribute__((ext_vector_type(4)));
+constexpr int M2L_MFMA_PMAX = 8;
+
+template <int P, int NTL>
+__global__ __launch_bounds__(64) void k_m2l_mfma(M2LArgs a) {
+    constexpr int NM = ncomp(P);
+    constexpr int NT = ncomp(2 * P);
+    constexpr int MT = (NM + 15) / 16;      // 16-row tiles of target coefficients
+    constexpr int KS = (NM + 3) / 4;        // 4-deep steps over source coefficients
+    constexpr int SLD = 16 * NTL;           // stored sets per coefficient
+    constexpr int NL = (NT + 63) / 64;
+    __shared__ double Tl[2][NT];
+    __shared__ short fidx[MT][KS][64];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int row = lane & 15, kq = lane >> 4;
+    const int e0 = a.off[i], e1 = a.off[i + 1];
+    // Hankel index table: A[t][s] = T[idx(s + t)] for t = mt*16 + row, s = ks*4 + kq
+    for (int mt = 0; mt < MT; ++mt) {
+        const int t = mt * 16 + row;
+        const int dt = t < NM ? decode_target(t) : 0;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int s = ks * 4 + kq;
+            short v = -1;
+            if (t < NM && s < NM) {
+                const int ds = decode_target(s);
+                const int n = (ds & 255) + (dt & 255);
+                const int w = ((ds >> 8) & 255) + ((dt >> 8) & 255);
+                const int z = (ds >> 16) + (dt >> 16);
+                v = (short)(tet(n) + tri(w) + z);
+            }
+            fidx[mt][ks][lane] = v;
+        }
+    }
+    int tdeg[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int t = mt * 16 + row;
+        tdeg[mt] = t < NM ? (decode_target(t) & 255) : 0;
+    }
+    v4d acc[MT][NTL];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    double pre[NL];
+    if (e0 < e1) {
+        const double* T0 = a.Tc + (int64_t)a.pair_class[e0] * NT;
+#pragma unroll
+        for (int x = 0; x < NL; ++x)
+            if (lane + 64 * x < NT) Tl[0][lane + 64 * x] = T0[lane + 64 * x];
+    }
+    __syncthreads();
+    const int li = a.level[i];
+    for (int e = e0; e < e1; ++e) {
+        const int cur = (e - e0) & 1;
+        const bool more = e + 1 < e1;
+        if (more) {
+            const double* Tn = a.Tc + (int64_t)a.pair_class[e + 1] * NT;
+#pragma unroll
+            for (int x = 0; x < NL; ++x) pre[x] = (lane + 64 * x < NT) ? Tn[lane + 64 * x] : 0.0;
+        }
+        const int j = a.src[e];
+        const int dl = li - a.level[j];
+        double rs[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+            rs[mt] = dl > 0 ? ldexp(1.0, -dl * tdeg[mt]) : (dl < 0 ? ldexp(1.0, -dl * (tdeg[mt] + 1)) : 1.0);
+        const double* __restrict__ B = a.CMs + (int64_t)j * NM * SLD;
+        const double* __restrict__ T = Tl[cur];
+#pragma unroll 2
+        for (int ks = 0; ks < KS; ++ks) {
+            const int s = ks * 4 + kq;
+            double b[NTL];
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) b[nt] = s < NM ? B[(int64_t)s * SLD + nt * 16 + row] : 0.0;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int f = fidx[mt][ks][lane];
+                const double av = f >= 0 ? T[f] * rs[mt] : 0.0;
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int x = 0; x < NL; ++x)
+                if (lane + 64 * x < NT) Tl[1 - cur][lane + 64 * x] = pre[x];
+        }
+        __syncthreads();
+    }
+    const double rinv = 1.0 / a.r[i];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = mt * 16 + 4 * q + kq;
+            if (t >= NM) continue;
+            const int deg = decode_target(t) & 255;
+            double f = rinv;
+            for (int k = 0; k < deg; ++k) f *= rinv;
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) {
+                const int set = nt * 16 + row;
+                if (set < a.nsets) a.LE[((int64_t)set * a.num_nodes + i) * NM + t] = acc[mt][nt][q] * f;
+            }
+        }
+    }
+}
+
+bool m2l_mfma_supported(int p, int nsets) { return p <= M2L_MFMA_PMAX && nsets >= 16; }
+
+template <int NTL>
+static void launch_mfma_p(const M2LArgs& a, int num_nodes, hipStream_t s) {
+    dim3 g(num_nodes), b(64);
+    switch (a.p) {
+        case 0: hipLaunchKernelGGL((k_m2l_mfma<0, NTL>), g, b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_m2l_mfma<1, NTL>), g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_m2l_mfma<2, NTL>), g, b, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_m2l_mfma<3, NTL>), g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_m2l_mfma<4, NTL>), g, b, 0, s, a); break;
+        case 5: hipLaunchKernelGGL((k_m2l_mfma<5, NTL>), g, b, 0, s, a); break;
+        case 6: hipLaunchKernelGGL((k_m2l_mfma<6, NTL>), g, b, 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_m2l_mfma<7, NTL>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_m2l_mfma<8, NTL>), g, b, 0, s, a); break;
+    }
+}
+
+void launch_m2l_mfma(const M2LArgs& a, int num_nodes, hipStream_t s) {
+    if (num_nodes <= 0) return;
+    if (a.s_ld == 16) launch_mfma_p<1>(a, num_nodes, s);
+    else launch_mfma_p<2>(a, num_nodes, s);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s) {
     if (num_nodes <= 0) return;
-    dim3 g(num_nodes), b(128);
+    dim3 g(num_nodes, a.nsets), b(128);
     switch (a.p) {
         case 0: hipLaunchKernelGGL(k_m2l<0>, g, b, 0, s, a); break;
         case 1: hipLaunchKernelGGL(k_m2l<1>, g, b, 0, s, a); break;

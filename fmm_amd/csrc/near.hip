@@ -2,11 +2,8 @@
 // all-pairs direct sum (src/fmm.F90:749-824) share one kernel.
 //
 // Work decomposition: one 64-lane wavefront per target block (a leaf of the
-// octree, or a run of 64 consecutive sites for the direct sum).  Each lane
-// holds one target site; sources are streamed in a wave-uniform loop so that
-// their coordinates, scaled multipoles, damping factor and original index are
-// scalar (SMEM) loads broadcast to all lanes for free -- no LDS traffic and no
-// bank conflicts.  The tensor recursion lives entirely in VGPRs (tensors.h).
+// octree, or a run of 64 consecutive sites for the direct sum); see the lane
+// layout below.  The tensor recursion lives entirely in VGPRs (tensors.h).
 //
 // Exclusions follow the reference exactly: an excluded pair is evaluated at
 // displacement (1e10, 1e10, 1e10) (src/fmm.F90:714-721, 802-806), which
@@ -25,21 +22,31 @@ constexpr int NEAR_EXCL_REG = 4;   // exclusions kept in registers; more are rea
 // runs are staged 64 sites at a time in LDS (one coalesced load per lane per
 // array) and read back per group; at the end the groups' partial fields are
 // summed in group order through LDS (deterministic).
-template <int M, int F, int DAMP>
-__global__ __launch_bounds__(64) void k_near(NearArgs a) {
+// Multipole sets: workgroup row blockIdx.y takes S consecutive sets; each pair
+// tensor is built once and contracted with the S sets' multipoles, so the
+// tensor recursion is shared (S = 1 for a single set).
+// Tensor orders up to 3 (except ERF, whose erf needs more) fit 128 VGPRs (4
+// waves per SIMD) without spilling; the S > 1 forms and the rest get more.
+template <int M, int F, int DAMP, int S>
+__global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 1)
+                                        : ((M + F <= 3 && DAMP != DAMP_ERF) ? 2 : 1)) void k_near(NearArgs a) {
     constexpr int K = M + F;
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
     __shared__ double sx[64], sy[64], sz[64], sdf[64];
-    __shared__ double sm[NM][64];
+    __shared__ double sm[S][NM][64];
     __shared__ int sorig[64];
     __shared__ double red[NF][64];
     const int b = a.blk[blockIdx.x];
+    const int set0 = blockIdx.y * S;
+    const int ns = min(S, a.nsets - set0);   // sets of this row (the rest are zero padding)
     const int lane = threadIdx.x;
     const int t0 = a.tgt_start[b];
     const int tc = a.tgt_count[b];
     const int e0 = a.list_off[b];
     const int e1 = a.list_off[b + 1];
+    const double* __restrict__ ms = a.ms + (int64_t)set0 * NM * a.ldm;
+    double* __restrict__ out = a.out + (int64_t)set0 * NF * a.ldo;
     for (int base = 0; base < tc; base += 64) {
         const int nt = min(64, tc - base);
         const int r = 64 / nt;                   // lane groups
@@ -52,9 +59,11 @@ __global__ __launch_bounds__(64) void k_near(NearArgs a) {
         int ex[NEAR_EXCL_REG];
 #pragma unroll
         for (int q = 0; q < NEAR_EXCL_REG; ++q) ex[q] = q < a.nexcl ? a.excl[(int64_t)q * a.lde + k] : -1;
-        double acc[NF];
+        double acc[S][NF];
 #pragma unroll
-        for (int t = 0; t < NF; ++t) acc[t] = 0.0;
+        for (int u = 0; u < S; ++u)
+#pragma unroll
+            for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
         for (int e = e0; e < e1; ++e) {
             const int sb = a.list_src[e];
             const int s0 = a.src_start[sb];
@@ -69,7 +78,10 @@ __global__ __launch_bounds__(64) void k_near(NearArgs a) {
                     if constexpr (DAMP != DAMP_NONE) sdf[lane] = a.dfs[j];
                     sorig[lane] = a.orig[j] + 1;
 #pragma unroll
-                    for (int s = 0; s < NM; ++s) sm[s][lane] = a.ms[(int64_t)s * a.ldm + j];
+                    for (int u = 0; u < S; ++u)
+#pragma unroll
+                        for (int s = 0; s < NM; ++s)
+                            sm[u][s][lane] = u < ns ? ms[((int64_t)u * NM + s) * a.ldm + j] : 0.0;
                 }
                 __syncthreads();
                 if (active) {
@@ -86,50 +98,65 @@ __global__ __launch_bounds__(64) void k_near(NearArgs a) {
                         if (excluded) { dx = 1e10; dy = 1e10; dz = 1e10; }
                         double aij = 0.0;
                         if constexpr (DAMP != DAMP_NONE) aij = dfi * sdf[jj];
-                        double m[NM];
-#pragma unroll
-                        for (int s = 0; s < NM; ++s) m[s] = sm[s][jj];
                         double gr[K + 1];
                         radial<K, DAMP>(dx * dx + dy * dy + dz * dz, aij, gr);
                         double T[ncomp(K)];
                         md_tensor<K>(dx, dy, dz, gr, T);
-                        contract<M, F>(m, T, acc);
+#pragma unroll
+                        for (int u = 0; u < S; ++u) {
+                            double m[NM];
+#pragma unroll
+                            for (int s = 0; s < NM; ++s) m[s] = sm[u][s][jj];
+                            contract<M, F>(m, T, acc[u]);
+                        }
                     }
                 }
                 __syncthreads();
             }
         }
-        // sum the r partial fields of each target in group order
+        // sum the r partial fields of each target in group order, set by set
 #pragma unroll
-        for (int t = 0; t < NF; ++t) red[t][lane] = acc[t];
-        __syncthreads();
-        if (lane < nt) {
+        for (int u = 0; u < S; ++u) {
 #pragma unroll
-            for (int t = 0; t < NF; ++t) {
-                double v = red[t][lane];
-                for (int q = 1; q < r; ++q) v += red[t][lane + q * nt];
-                acc[t] = v;
+            for (int t = 0; t < NF; ++t) red[t][lane] = acc[u][t];
+            __syncthreads();
+            if (lane < nt && u < ns) {
+                const int kk = t0 + base + lane;
+#pragma unroll
+                for (int t = 0; t < NF; ++t) {
+                    double v = red[t][lane];
+                    for (int q = 1; q < r; ++q) v += red[t][lane + q * nt];
+                    double* o = out + ((int64_t)u * NF + t) * a.ldo + kk;
+                    if (a.accumulate) *o += v;
+                    else *o = v;
+                }
             }
-            const int kk = t0 + base + lane;
-            if (a.accumulate) {
-#pragma unroll
-                for (int t = 0; t < NF; ++t) a.out[(int64_t)t * a.ldo + kk] += acc[t];
-            } else {
-#pragma unroll
-                for (int t = 0; t < NF; ++t) a.out[(int64_t)t * a.ldo + kk] = acc[t];
-            }
+            __syncthreads();
         }
-        __syncthreads();
+    }
+}
+
+// Sets per workgroup row: 4 where the tensor is cheap in registers, else 2.
+template <int M, int F, int DAMP>
+constexpr int near_sets() { return (M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 2; }
+
+template <int M, int F, int DAMP>
+static void launch_sets(const NearArgs& a, int nblocks, hipStream_t s) {
+    if (a.nsets == 1) {
+        hipLaunchKernelGGL((k_near<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
+    } else {
+        constexpr int S = near_sets<M, F, DAMP>();
+        hipLaunchKernelGGL((k_near<M, F, DAMP, S>), dim3(nblocks, (a.nsets + S - 1) / S), dim3(64), 0, s, a);
     }
 }
 
 template <int M, int F>
 static void launch_damp(const NearArgs& a, int nblocks, int damp, hipStream_t s) {
     switch (damp) {
-        case DAMP_NONE: hipLaunchKernelGGL((k_near<M, F, DAMP_NONE>), dim3(nblocks), dim3(64), 0, s, a); break;
-        case DAMP_THOLE: hipLaunchKernelGGL((k_near<M, F, DAMP_THOLE>), dim3(nblocks), dim3(64), 0, s, a); break;
-        case DAMP_AMOEBA: hipLaunchKernelGGL((k_near<M, F, DAMP_AMOEBA>), dim3(nblocks), dim3(64), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_near<M, F, DAMP_ERF>), dim3(nblocks), dim3(64), 0, s, a); break;
+        case DAMP_NONE: launch_sets<M, F, DAMP_NONE>(a, nblocks, s); break;
+        case DAMP_THOLE: launch_sets<M, F, DAMP_THOLE>(a, nblocks, s); break;
+        case DAMP_AMOEBA: launch_sets<M, F, DAMP_AMOEBA>(a, nblocks, s); break;
+        default: launch_sets<M, F, DAMP_ERF>(a, nblocks, s); break;
     }
 }
 

@@ -282,10 +282,11 @@ __global__ __launch_bounds__(256) void k_gather(SiteArgs a) {
     a.xs[k] = a.X[o];
     a.ys[k] = a.X[a.ldx + o];
     a.zs[k] = a.X[2 * a.ldx + o];
-    for (int c = 0; c < a.dim; ++c) {
-        const double v = a.M[(int64_t)c * a.ldm + o];
-        a.ms_raw[(int64_t)c * a.n + k] = v;
-        a.ms_scaled[(int64_t)c * a.n + k] = a.scale[c] * v;
+    // multipole column cc = set * dim + component, the same leading dimension for all sets
+    for (int cc = 0; cc < a.dim * a.nsets; ++cc) {
+        const double v = a.M[(int64_t)cc * a.ldm + o];
+        a.ms_raw[(int64_t)cc * a.n + k] = v;
+        a.ms_scaled[(int64_t)cc * a.n + k] = a.scale[cc % a.dim] * v;
     }
     a.dfs[k] = a.df ? a.df[o] : 0.0;
     for (int q = 0; q < a.nexcl; ++q) a.excl_s[(int64_t)q * a.n + k] = a.E[(int64_t)q * a.lde + o];

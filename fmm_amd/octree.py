@@ -112,6 +112,41 @@ class Context:
                                        1 if is_periodic else 0, _p(box)))
         return F
 
+    # -- many multipole sets on one set of sites --------------------------------
+    @staticmethod
+    def _prep_sets(coordinates, multipoles, exclusions, damping_factors):
+        M = np.asfortranarray(multipoles, dtype=np.float64)
+        if M.ndim == 2:
+            M = M.reshape(M.shape[0], M.shape[1], 1, order="F")
+        if M.ndim != 3:
+            raise ValueError("multipoles must be N x dim x nsets")
+        X, _, E, df = _prep(coordinates, M.reshape(M.shape[0], -1, order="F"), exclusions, damping_factors)
+        return X, M, E, df
+
+    def field_direct_sets(self, comm, coordinates, multipoles, exclusions, field_order, damp_type="",
+                          damping_factors=(), field=None):
+        """field_direct for nsets multipole sets (multipoles N x dim x nsets); returns N x nf x nsets."""
+        X, M, E, df = self._prep_sets(coordinates, multipoles, exclusions, damping_factors)
+        n, dim, ns = M.shape
+        F = np.zeros((n, ncomp(field_order), ns), order="F") if field is None else field
+        check(self._lib.fmmh_field_direct_sets(self._h, int(comm), _p(X), n, _p(M), n, n, dim, ns, _p(E), n,
+                                               E.shape[1], int(field_order), _p(F), F.shape[0],
+                                               _damp_code(damp_type), _p(df) if df.size else None, df.size))
+        return F
+
+    def field_fmm_sets(self, comm, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order,
+                       damp_type="", damping_factors=(), is_periodic=False, box_matrix=None, field=None):
+        """field_fmm for nsets multipole sets on one tree (multipoles N x dim x nsets); returns N x nf x nsets."""
+        X, M, E, df = self._prep_sets(coordinates, multipoles, exclusions, damping_factors)
+        n, dim, ns = M.shape
+        F = np.zeros((n, ncomp(field_order), ns), order="F") if field is None else field
+        box = np.zeros((3, 3), order="F") if box_matrix is None else np.asfortranarray(box_matrix, np.float64)
+        check(self._lib.fmmh_field_fmm_sets(self._h, int(comm), _p(X), n, _p(M), n, n, dim, ns, _p(E), n,
+                                            E.shape[1], float(theta), int(ncrit), int(expansion_order),
+                                            int(field_order), _p(F), F.shape[0], _damp_code(damp_type),
+                                            _p(df) if df.size else None, df.size, 1 if is_periodic else 0, _p(box)))
+        return F
+
     # -- device arrays (torch tensors already in HBM) --------------------------
     @staticmethod
     def _dptr(t):
@@ -149,6 +184,38 @@ class Context:
             _damp_code(damp_type), self._dptr(damping_factors), ndf, 0, None))
         return field
 
+    @staticmethod
+    def _ld3(t):
+        """Leading dimension of a column-major N x c x nsets tensor."""
+        if t.stride(0) != 1 or t.stride(2) != t.stride(1) * t.shape[1]:
+            raise ValueError("device set arrays must be column-major N x c x nsets: use t.permute(2, 1, 0)"
+                             ".contiguous().permute(2, 1, 0)")
+        return t.stride(1)
+
+    def field_direct_sets_device(self, coordinates, multipoles, exclusions, field_order, field, damp_type="",
+                                 damping_factors=None):
+        n, dim, ns = multipoles.shape
+        nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
+        ndf = damping_factors.numel() if damping_factors is not None else 0
+        check(self._lib.fmmh_field_direct_sets_dev(
+            self._h, 0, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles),
+            self._ld3(multipoles), n, dim, ns, self._dptr(exclusions), self._ld(exclusions), nexcl,
+            int(field_order), self._dptr(field), self._ld3(field), _damp_code(damp_type),
+            self._dptr(damping_factors), ndf))
+        return field
+
+    def field_fmm_sets_device(self, coordinates, multipoles, exclusions, theta, ncrit, expansion_order,
+                              field_order, field, damp_type="", damping_factors=None):
+        n, dim, ns = multipoles.shape
+        nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
+        ndf = damping_factors.numel() if damping_factors is not None else 0
+        check(self._lib.fmmh_field_fmm_sets_dev(
+            self._h, 0, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles),
+            self._ld3(multipoles), n, dim, ns, self._dptr(exclusions), self._ld(exclusions), nexcl, float(theta),
+            int(ncrit), int(expansion_order), int(field_order), self._dptr(field), self._ld3(field),
+            _damp_code(damp_type), self._dptr(damping_factors), ndf, 0, None))
+        return field
+
     # -- diagnostics -----------------------------------------------------------
     def set_timing(self, on=True):
         check(self._lib.fmmh_set_option(self._h, 1, 1.0 if on else 0.0))
@@ -156,6 +223,10 @@ class Context:
     def set_list_capacity(self, scale=1.0):
         """Scale of the initial interaction-list buffers (they grow on overflow)."""
         check(self._lib.fmmh_set_option(self._h, 2, float(scale)))
+
+    def set_m2l_mfma(self, mode=1):
+        """M2L engine: 0 vector ALUs, 1 FP64 MFMA from 16 sets on (default), 2 MFMA for any set count."""
+        check(self._lib.fmmh_set_option(self._h, 3, float(mode)))
 
     def phase_times(self):
         ms = np.zeros(PHASE_CAP)

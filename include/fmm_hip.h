@@ -112,6 +112,49 @@ int fmmh_field_fmm_dev(fmmh_context* ctx, int32_t comm,
                        int32_t damp_type, const double* damping_factors, int64_t n_damping,
                        int32_t is_periodic, const double* box_matrix);
 
+/* Many multipole sets on one set of sites: nsets independent field_direct /
+ * field_fmm calls (same coordinates, exclusions, damping and parameters) in
+ * one call that builds the tree and interaction lists once.  Replaces a loop
+ * of nsets calls of src/fmm.F90:1338-1352 / 1354-1400 (e.g. the induced-
+ * dipole iterations and response vectors of a polarizable embedding, which
+ * call field_fmm repeatedly on fixed coordinates).
+ * multipoles is n x dim x nsets and field n x nf x nsets, column-major with
+ * the one leading dimension ld_multipoles / ld_field (set s starts at
+ * s * dim * ld_multipoles / s * nf * ld_field).  Each set's field equals the
+ * single-set call's to rounding (the M2L of >= 16 sets runs on the FP64
+ * matrix cores, in another summation order). */
+int fmmh_field_direct_sets(fmmh_context* ctx, int32_t comm,
+                           const double* coordinates, int64_t ld_coordinates,
+                           const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
+                           const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                           int32_t field_order, double* field, int64_t ld_field,
+                           int32_t damp_type, const double* damping_factors, int64_t n_damping);
+
+int fmmh_field_direct_sets_dev(fmmh_context* ctx, int32_t comm,
+                               const double* coordinates, int64_t ld_coordinates,
+                               const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                               int32_t nsets, const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                               int32_t field_order, double* field, int64_t ld_field,
+                               int32_t damp_type, const double* damping_factors, int64_t n_damping);
+
+int fmmh_field_fmm_sets(fmmh_context* ctx, int32_t comm,
+                        const double* coordinates, int64_t ld_coordinates,
+                        const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
+                        const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                        double theta, int32_t ncrit, int32_t expansion_order,
+                        int32_t field_order, double* field, int64_t ld_field,
+                        int32_t damp_type, const double* damping_factors, int64_t n_damping,
+                        int32_t is_periodic, const double* box_matrix);
+
+int fmmh_field_fmm_sets_dev(fmmh_context* ctx, int32_t comm,
+                            const double* coordinates, int64_t ld_coordinates,
+                            const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, int32_t nsets,
+                            const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                            double theta, int32_t ncrit, int32_t expansion_order,
+                            int32_t field_order, double* field, int64_t ld_field,
+                            int32_t damp_type, const double* damping_factors, int64_t n_damping,
+                            int32_t is_periodic, const double* box_matrix);
+
 /* ---- diagnostics (no counterpart in the reference) -------------------------
  * Octree and interaction lists of field_fmm for the given sites (host
  * coordinates), exported for bit-exact comparison with the reference's
@@ -134,7 +177,9 @@ int fmmh_tree_export(fmmh_context* ctx, double* centers, double* r, double* rmax
  * number of phases written. */
 enum fmmh_option {
     FMMH_OPT_TIMING = 1,          /* value != 0: record per-phase HIP events */
-    FMMH_OPT_LIST_CAPACITY = 2    /* scale of the initial interaction-list buffers (default 1; they grow on overflow) */
+    FMMH_OPT_LIST_CAPACITY = 2,   /* scale of the initial interaction-list buffers (default 1; they grow on overflow) */
+    FMMH_OPT_M2L_MFMA = 3         /* 0: M2L on the vector ALUs; 1 (default): FP64 MFMA from 16 sets on;
+                                     2: MFMA for any set count (expansion orders <= 8) */
 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
