@@ -147,6 +147,18 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     stats = ctx.stats()
     phase_avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+
+    # Secondary figure: the same call with the tree / list cache on (fixed
+    # coordinates, as in induced-dipole iterations).  Not the headline value.
+    ctx.set_tree_reuse(True)
+    step()
+    torch.cuda.synchronize()
+    r0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    reuse_ms = (time.perf_counter() - r0) * 1e3 / args.steps
+    ctx.set_tree_reuse(False)
     value = n * sets / (ms_per_step / 1e3)
 
     pairs = stats["p2p_site_pairs"]
@@ -176,6 +188,26 @@ def main():
             errors[f"set{k}_vs_single_set_fmm"] = rel(field[:, :, k], single)
         errors["note"] = ("damping applies to near-field pairs only, as in the reference, so the damped FMM "
                           "differs from the damped direct sum by the damping of far pairs")
+    elif not args.no_error and n > 250_000:
+        # full direct sums are O(N^2): state the error on 4096 seeded target sites
+        field = out.cpu().numpy()
+        tg = np.sort(np.random.default_rng(4096).choice(n, 4096, replace=False)).astype(np.int32)
+        T = torch.as_tensor(tg, device=dev)
+        d = col(np.zeros((4096, nf)), dev)
+        ctx.field_direct_targets_device(X, M, E, T, F, d, c["damp_type"], df)
+        direct = d.cpu().numpy()
+        d.zero_()
+        ctx.field_direct_targets_device(X, M, E, T, F, d, "", None)
+        direct_u = d.cpu().numpy()
+        u = col(np.zeros((n, nf)), dev)
+        ctx.field_fmm_device(X, M, E, c["theta"], c["ncrit"], c["expansion_order"], F, u, "", None)
+        fmm_u = u.cpu().numpy()[tg]
+        errors = {
+            "field_block_vs_fp64_direct": rel(field[tg, 1:4], direct[:, 1:4]),
+            "potential_block_vs_fp64_direct": rel(field[tg, :1], direct[:, :1]),
+            "field_block_truncation": rel(fmm_u[:, 1:4], direct_u[:, 1:4]),
+            "note": "on 4096 seeded target sites; truncation = undamped FMM vs undamped direct",
+        }
     elif not args.no_error:
         field = out.cpu().numpy()
         d = col(np.zeros((n, nf)), dev)
@@ -217,6 +249,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phases_ms": phase_avg,
+        "tree_reuse": {"ms_per_step": reuse_ms, "value": n * sets / (reuse_ms / 1e3),
+                       "note": "same call with FMMH_OPT_REUSE_TREE on: octree, lists and M2L classes kept from "
+                               "the previous call with bitwise-identical coordinates"},
         "tree": stats,
         "errors": errors,
     }

@@ -28,6 +28,8 @@ _PROTOS = {
                                   i32, vp, i64, i32, vp]),
     "fmmh_field_fmm_sets_dev": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, i32, vp, i64, i32, f64, i32, i32, i32, vp,
                                       i64, i32, vp, i64, i32, vp]),
+    "fmmh_field_direct_targets_dev": (i32, [vp, vp, i64, vp, i64, i32, i32, vp, i64, i32, vp, i32, i32, vp, i64,
+                                            i32, vp, i64]),
     "fmmh_tree_build": (i32, [vp, vp, i64, i32, i32, f64, vp]),
     "fmmh_tree_export": (i32, [vp] + [vp] * 13),
     "fmmh_set_option": (i32, [vp, i32, f64]),

@@ -46,6 +46,19 @@ struct Tables {
     int nsrc = 0;
 };
 
+struct TreeOut {
+    TreeDev t{};
+    int num_nodes = 0, num_leaves = 0, max_depth = 0;
+    std::vector<int> level_off;   // KEY_LEVELS + 3 entries; level L nodes at [off[L], off[L+1])
+};
+
+struct ListsOut {
+    int* m2l_off = nullptr; int* m2l_src = nullptr; int n_m2l = 0;
+    unsigned long long* m2l_keys = nullptr;   // sorted (target << 32 | source), CSR order
+    int* p2p_off = nullptr; int* p2p_src = nullptr; int n_p2p = 0;
+};
+
+
 }  // namespace
 
 struct fmmh_context {
@@ -63,6 +76,20 @@ struct fmmh_context {
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
     int m2l_mfma = 1;                 // 0 never, 1 for >= 16 sets, 2 whenever the order allows
+    // Tree / list cache (FMMH_OPT_REUSE_TREE): the last field_fmm tree, lists
+    // and M2L classes, reused while coordinates (bitwise), n, ncrit and theta
+    // stay the same.  The device data live in the named pool buffers, which
+    // only build_tree / build_lists / the M2L class pass write.
+    bool reuse_tree = false;
+    bool cache_valid = false;
+    int cache_n = 0, cache_ncrit = 0;
+    double cache_theta = 0.0;
+    TreeOut cache_tr;
+    ListsOut cache_L;
+    int cache_class_p = -1;           // expansion order of the cached class tensors (-1: none)
+    int64_t cache_classes = 0;
+    unsigned long long cache_pairs = 0;
+    int cache_retries = 0;
     int list_retries = 0;
     // last tree (for export)
     int num_nodes = 0, num_leaves = 0, max_depth = 0, n_m2l = 0, n_p2p = 0, n_sites = 0;
@@ -274,10 +301,12 @@ __global__ void k_iota(int* a, int n) {
     if (i < n) a[i] = i;
 }
 
-__global__ void k_direct_blocks(int* start, int* count, int* list_off, int* list_src, int nb, int n) {
+// Target blocks of 64 consecutive sites from `first` on (n targets), each
+// with the one source block 0.
+__global__ void k_direct_blocks(int* start, int* count, int* list_off, int* list_src, int nb, int n, int first) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b < nb) {
-        start[b] = 64 * b;
+        start[b] = first + 64 * b;
         count[b] = min(64, n - 64 * b);
         list_off[b] = b;
         list_src[b] = 0;
@@ -361,7 +390,7 @@ void run_direct(fmmh_context* c, const Call& k) {
     double* zs = c->buf<double>("d_zs", n);
     double* dfs = c->buf<double>("d_dfs", n);
     int* excl = c->buf<int>("d_excl", (size_t)n * std::max(k.nexcl, 1));
-    hipLaunchKernelGGL(k_direct_blocks, dim3(nblk(nb, 256)), dim3(256), 0, s, start, count, loff, lsrc, nb, n);
+    hipLaunchKernelGGL(k_direct_blocks, dim3(nblk(nb, 256)), dim3(256), 0, s, start, count, loff, lsrc, nb, n, 0);
     hipLaunchKernelGGL(k_iota, dim3(nblk(std::max(n, nb), 256)), dim3(256), 0, s, iota, std::max(n, nb));
     int zero_n[2] = {0, n};
     FMMH_HIP(hipMemcpyAsync(s_start, &zero_n[0], sizeof(int), hipMemcpyHostToDevice, s));
@@ -377,13 +406,76 @@ void run_direct(fmmh_context* c, const Call& k) {
     FMMH_HIP(hipStreamSynchronize(s));
 }
 
-// ---------------------------------------------------------------- tree
-struct TreeOut {
-    TreeDev t{};
-    int num_nodes = 0, num_leaves = 0, max_depth = 0;
-    std::vector<int> level_off;   // KEY_LEVELS + 3 entries; level L nodes at [off[L], off[L+1])
-};
+// Direct sum at a subset of target sites (spot checks at sizes where the
+// full O(N^2) sum is too costly).  The targets are appended as copies after
+// the n sources in the gathered site arrays, so the near kernel sees target
+// blocks [n, n + nt) against the one source block [0, n).
+__global__ void k_target_perm(int* perm, const int* targets, int n, int nt, int* bad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[i] = i;
+    else if (i < n + nt) {
+        int t = targets[i - n];
+        if (t < 0 || t >= n) { *bad = 1; t = 0; }
+        perm[i] = t;
+    }
+}
 
+__global__ void k_add_rows(const double* __restrict__ src, int64_t lds, int first, double* __restrict__ dst,
+                           int64_t ldd, int nt, int nf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    for (int t = 0; t < nf; ++t) dst[t * ldd + i] += src[t * lds + first + i];
+}
+
+void run_direct_targets(fmmh_context* c, const Call& k, const int* targets, int nt) {
+    hipStream_t s = c->stream;
+    const int n = k.n, ntot = n + nt;
+    const int m = order_from_dim(k.dim);
+    const int nf = ncomp(k.F);
+    Tables& tb = get_tables(c, 0, m);
+    const int nb = nblk(nt, 64);
+    int* start = c->buf<int>("d_start", nb);
+    int* count = c->buf<int>("d_count", nb);
+    int* loff = c->buf<int>("d_loff", nb + 1);
+    int* lsrc = c->buf<int>("d_lsrc", nb);
+    int* s_start = c->buf<int>("d_sstart", 1);
+    int* s_count = c->buf<int>("d_scount", 1);
+    int* iota = c->buf<int>("iota", std::max(ntot, nb));
+    int* perm = c->buf<int>("d_tperm", ntot);
+    int* bad = c->buf<int>("d_tbad", 1);
+    double* ms = c->buf<double>("d_ms", (size_t)ntot * k.dim);
+    double* ms_raw = c->buf<double>("d_ms_raw", (size_t)ntot * k.dim);
+    double* xs = c->buf<double>("d_xs", ntot);
+    double* ys = c->buf<double>("d_ys", ntot);
+    double* zs = c->buf<double>("d_zs", ntot);
+    double* dfs = c->buf<double>("d_dfs", ntot);
+    int* excl = c->buf<int>("d_excl", (size_t)ntot * std::max(k.nexcl, 1));
+    double* out = c->buf<double>("d_tout", (size_t)ntot * nf);
+    FMMH_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_target_perm, dim3(nblk(ntot, 256)), dim3(256), 0, s, perm, targets, n, nt, bad);
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 410, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->sync();
+    if (c->h_pinned[410]) throw Error(FMMH_ERR_ARG, "target index outside 0..n-1");
+    hipLaunchKernelGGL(k_direct_blocks, dim3(nblk(nb, 256)), dim3(256), 0, s, start, count, loff, lsrc, nb, nt, n);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(std::max(ntot, nb), 256)), dim3(256), 0, s, iota, std::max(ntot, nb));
+    int zero_n[2] = {0, n};
+    std::memcpy(c->h_pinned + 420, zero_n, sizeof(zero_n));
+    FMMH_HIP(hipMemcpyAsync(s_start, c->h_pinned + 420, sizeof(int), hipMemcpyHostToDevice, s));
+    FMMH_HIP(hipMemcpyAsync(s_count, c->h_pinned + 421, sizeof(int), hipMemcpyHostToDevice, s));
+    SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, perm, ntot,
+               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale, 1};
+    launch_gather(g, s);
+    NearArgs a{iota, start, count, loff, lsrc, s_start, s_count, xs, ys, zs, ms, ntot, dfs, perm, excl, ntot,
+               k.nexcl, out, ntot, 0, 1};
+    launch_near(a, nb, m, k.F, k.damp, s);
+    hipLaunchKernelGGL(k_add_rows, dim3(nblk(nt, 256)), dim3(256), 0, s, out, (int64_t)ntot, n, k.field, k.ldf, nt,
+                       nf);
+    FMMH_HIP(hipGetLastError());
+    c->mark("direct_targets");
+    FMMH_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- tree
 void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit, TreeOut& out) {
     hipStream_t s = c->stream;
     TreeDev& t = out.t;
@@ -463,12 +555,6 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     c->sync();
 }
 
-struct ListsOut {
-    int* m2l_off = nullptr; int* m2l_src = nullptr; int n_m2l = 0;
-    unsigned long long* m2l_keys = nullptr;   // sorted (target << 32 | source), CSR order
-    int* p2p_off = nullptr; int* p2p_src = nullptr; int n_p2p = 0;
-};
-
 void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes;
@@ -543,15 +629,19 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
 // Displacement classes (sort + unique), one tensor per class, then the
 // target-centric Hankel contraction of every set.  One host sync to size the
 // class table.  Many sets go through the MFMA form in chunks of up to 32 sets.
-void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets, const double* CM, double* LE) {
+void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets, const double* CM, double* LE,
+             bool classes_cached) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes, np = L.n_m2l;
     const int nm = ncomp(p), nT = ncomp(2 * p);
     int* pair_class = c->buf<int>("m_pair_class", np);
-    int* dev_scalars = c->buf<int>("m_scalars", 4);   // num_classes, error
-    FMMH_HIP(hipMemsetAsync(dev_scalars, 0, 4 * sizeof(int), s));
     double* Tc = nullptr;
-    if (np > 0) {
+    if (classes_cached) {
+        Tc = np > 0 ? c->buf<double>("m_tc", (size_t)c->cache_classes * nT) : nullptr;
+        c->stats_classes = c->cache_classes;
+    } else if (np > 0) {
+        int* dev_scalars = c->buf<int>("m_scalars", 4);   // num_classes, error
+        FMMH_HIP(hipMemsetAsync(dev_scalars, 0, 4 * sizeof(int), s));
         M2LClassArgs ca{};
         ca.pair_keys = L.m2l_keys;
         ca.n_pairs = np;
@@ -584,6 +674,8 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         Tc = c->buf<double>("m_tc", (size_t)ncls * nT);
         ca.Tc = Tc;
         launch_m2l_class_tensors(ca, std::min(ncls, 4096), s);
+    } else {
+        c->stats_classes = 0;
     }
     c->mark("m2l_classes");
     const bool mfma = c->m2l_mfma == 2 ? m2l_mfma_supported(p, 16)
@@ -603,6 +695,31 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
 }
 
 // ---------------------------------------------------------------- field_fmm
+__global__ void k_coords_differ(const double* __restrict__ X, int64_t ldx, const double* __restrict__ C, int n,
+                                int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool d = false;
+    for (int q = 0; q < 3; ++q)
+        d |= __double_as_longlong(X[q * ldx + i]) != __double_as_longlong(C[(int64_t)q * n + i]);
+    if (d) *flag = 1;
+}
+
+// True when the cached tree was built from bitwise the same coordinates with
+// the same n, ncrit and theta (one device compare, one host sync).
+bool cached_tree_matches(fmmh_context* c, const Call& k, double theta, int ncrit) {
+    if (!c->cache_valid || c->cache_n != k.n || c->cache_ncrit != ncrit || c->cache_theta != theta) return false;
+    hipStream_t s = c->stream;
+    int* flag = c->buf<int>("c_flag", 1);
+    FMMH_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_coords_differ, dim3(nblk(k.n, 256)), dim3(256), 0, s, k.X, k.ldx,
+                       c->buf<double>("c_X", (size_t)k.n * 3), k.n, flag);
+    FMMH_HIP(hipGetLastError());
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 400, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->sync();
+    return c->h_pinned[400] == 0;
+}
+
 void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
     hipStream_t s = c->stream;
     const int n = k.n;
@@ -610,11 +727,20 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
     const int nf = ncomp(k.F);
     const int nm = ncomp(p);
     TreeOut tr;
-    build_tree(c, k.X, k.ldx, n, ncrit, tr);
-    c->mark("tree");
     ListsOut L;
-    build_lists(c, tr, theta, L);
-    c->mark("lists");
+    const bool reuse = c->reuse_tree && cached_tree_matches(c, k, theta, ncrit);
+    if (reuse) {
+        tr = c->cache_tr;
+        L = c->cache_L;
+        c->list_retries = c->cache_retries;
+        c->mark("tree_reused");
+    } else {
+        c->cache_valid = false;
+        build_tree(c, k.X, k.ldx, n, ncrit, tr);
+        c->mark("tree");
+        build_lists(c, tr, theta, L);
+        c->mark("lists");
+    }
     const int nn = tr.num_nodes, nl = tr.num_leaves;
     c->num_nodes = nn; c->num_leaves = nl; c->max_depth = tr.max_depth;
     c->n_m2l = L.n_m2l; c->n_p2p = L.n_p2p; c->n_sites = n;
@@ -647,7 +773,8 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
                 ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S};
     launch_near(na, nl, m, k.F, k.damp, s);
     c->mark("p2p");
-    run_m2l(c, tr, L, tb, p, S, CM, LE);
+    const bool classes_cached = reuse && c->cache_class_p == p;
+    run_m2l(c, tr, L, tb, p, S, CM, LE, classes_cached);
     c->mark("m2l");
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
                 tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S};
@@ -658,14 +785,36 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
     c->mark("l2l");
     launch_l2p(da, nl, s);
     c->mark("l2p");
-    unsigned long long* pc = c->buf<unsigned long long>("e_paircount", 1);
-    FMMH_HIP(hipMemsetAsync(pc, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_pair_count, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src, tr.t.leaf_node,
-                       tr.t.count, nl, pc);
     unsigned long long pairs = 0;
-    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 200, pc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    c->sync();
-    std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
+    if (reuse) {
+        pairs = c->cache_pairs;
+        c->sync();
+    } else {
+        unsigned long long* pc = c->buf<unsigned long long>("e_paircount", 1);
+        FMMH_HIP(hipMemsetAsync(pc, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_pair_count, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
+                           tr.t.leaf_node, tr.t.count, nl, pc);
+        FMMH_HIP(hipMemcpyAsync(c->h_pinned + 200, pc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        c->sync();
+        std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
+    }
+    if (c->reuse_tree) {
+        if (!reuse) {
+            // keep a copy of the coordinates the cached tree was built from
+            double* cx = c->buf<double>("c_X", (size_t)n * 3);
+            FMMH_HIP(hipMemcpy2DAsync(cx, n * sizeof(double), k.X, k.ldx * sizeof(double), n * sizeof(double), 3,
+                                      hipMemcpyDeviceToDevice, s));
+            c->sync();
+            c->cache_tr = tr;
+            c->cache_L = L;
+            c->cache_n = n; c->cache_ncrit = ncrit; c->cache_theta = theta;
+            c->cache_pairs = pairs;
+            c->cache_retries = c->list_retries;
+            c->cache_valid = true;
+        }
+        c->cache_class_p = p;
+        c->cache_classes = c->stats_classes;
+    }
     c->stats[0] = nn; c->stats[1] = nl; c->stats[2] = tr.max_depth;
     c->stats[3] = L.n_m2l; c->stats[4] = L.n_p2p; c->stats[5] = (int64_t)pairs; c->stats[6] = c->stats_classes;
     c->stats[7] = c->list_retries;
@@ -920,6 +1069,26 @@ int fmmh_field_fmm_sets_dev(fmmh_context* ctx, int32_t comm, const double* coord
                           ld_field, damp_type, damping_factors, n_damping, is_periodic);
 }
 
+int fmmh_field_direct_targets_dev(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates,
+                                  const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                                  const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                                  const int32_t* targets, int32_t ntargets, int32_t field_order, double* field,
+                                  int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                                  int64_t n_damping) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (ntargets < 1) throw Error(FMMH_ERR_ARG, "ntargets must be >= 1");
+        if (ld_field < ntargets) throw Error(FMMH_ERR_ARG, "ld_field smaller than ntargets");
+        Call k = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
+                           nexcl, field_order, field, n, damp_type, damping_factors);
+        check_common(k, n_damping);
+        k.ldf = ld_field;
+        c->begin_timing();
+        run_direct_targets(c, k, targets, ntargets);
+        c->end_timing();
+    });
+}
+
 int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates, int32_t n, int32_t ncrit,
                     double theta, int64_t* info) {
     return guarded([&] {
@@ -930,6 +1099,7 @@ int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coo
         double* X = c->buf<double>("h_X", (size_t)n * 3);
         FMMH_HIP(hipMemcpy2DAsync(X, n * sizeof(double), coordinates, ld_coordinates * sizeof(double),
                                   n * sizeof(double), 3, hipMemcpyHostToDevice, c->stream));
+        c->cache_valid = false;
         TreeOut tr;
         build_tree(c, X, n, n, ncrit, tr);
         ListsOut L;
@@ -985,7 +1155,10 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         fmmh_context* c = resolve(ctx);
         if (key == FMMH_OPT_TIMING) c->timing = value != 0.0;
         else if (key == FMMH_OPT_LIST_CAPACITY) c->list_cap_scale = value > 0.0 ? value : 1.0;
-        else if (key == FMMH_OPT_M2L_MFMA) {
+        else if (key == FMMH_OPT_REUSE_TREE) {
+            c->reuse_tree = value != 0.0;
+            if (!c->reuse_tree) c->cache_valid = false;
+        } else if (key == FMMH_OPT_M2L_MFMA) {
             if (value != 0.0 && value != 1.0 && value != 2.0) throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1 or 2");
             c->m2l_mfma = (int)value;
         }

@@ -216,6 +216,19 @@ class Context:
             _damp_code(damp_type), self._dptr(damping_factors), ndf, 0, None))
         return field
 
+    def field_direct_targets_device(self, coordinates, multipoles, exclusions, targets, field_order, field,
+                                    damp_type="", damping_factors=None):
+        """Direct sum at the sites `targets` (0-based int32 device tensor); field is ntargets x nf."""
+        n, dim = multipoles.shape
+        nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
+        ndf = damping_factors.numel() if damping_factors is not None else 0
+        check(self._lib.fmmh_field_direct_targets_dev(
+            self._h, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles), self._ld(multipoles),
+            n, dim, self._dptr(exclusions), self._ld(exclusions), nexcl, self._dptr(targets), targets.numel(),
+            int(field_order), self._dptr(field), self._ld(field), _damp_code(damp_type),
+            self._dptr(damping_factors), ndf))
+        return field
+
     # -- diagnostics -----------------------------------------------------------
     def set_timing(self, on=True):
         check(self._lib.fmmh_set_option(self._h, 1, 1.0 if on else 0.0))
@@ -223,6 +236,10 @@ class Context:
     def set_list_capacity(self, scale=1.0):
         """Scale of the initial interaction-list buffers (they grow on overflow)."""
         check(self._lib.fmmh_set_option(self._h, 2, float(scale)))
+
+    def set_tree_reuse(self, on=True):
+        """Reuse the octree, lists and M2L classes while coordinates, ncrit and theta are unchanged."""
+        check(self._lib.fmmh_set_option(self._h, 4, 1.0 if on else 0.0))
 
     def set_m2l_mfma(self, mode=1):
         """M2L engine: 0 vector ALUs, 1 FP64 MFMA from 16 sets on (default), 2 MFMA for any set count."""

@@ -156,7 +156,19 @@ int fmmh_field_fmm_sets_dev(fmmh_context* ctx, int32_t comm,
                             int32_t is_periodic, const double* box_matrix);
 
 /* ---- diagnostics (no counterpart in the reference) -------------------------
- * Octree and interaction lists of field_fmm for the given sites (host
+ * Spot-check direct sum: the field_direct result (src/fmm.F90:749-824) at the
+ * ntargets sites targets[] (0-based, device memory like every other array),
+ * accumulated into field (ntargets x nf, leading dimension ld_field).  Costs
+ * ntargets x n pairs; used to state FMM errors at sizes where the full direct
+ * sum is too costly. */
+int fmmh_field_direct_targets_dev(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates,
+                                  const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                                  const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                                  const int32_t* targets, int32_t ntargets, int32_t field_order, double* field,
+                                  int64_t ld_field, int32_t damp_type, const double* damping_factors,
+                                  int64_t n_damping);
+
+/* Octree and interaction lists of field_fmm for the given sites (host
  * coordinates), exported for bit-exact comparison with the reference's
  * tree (src/fmm.F90:137-385).  fmmh_tree_build keeps the result in the
  * context and returns sizes in info[5] = {num_nodes, num_leaves, max_depth,
@@ -178,8 +190,14 @@ int fmmh_tree_export(fmmh_context* ctx, double* centers, double* r, double* rmax
 enum fmmh_option {
     FMMH_OPT_TIMING = 1,          /* value != 0: record per-phase HIP events */
     FMMH_OPT_LIST_CAPACITY = 2,   /* scale of the initial interaction-list buffers (default 1; they grow on overflow) */
-    FMMH_OPT_M2L_MFMA = 3         /* 0: M2L on the vector ALUs; 1 (default): FP64 MFMA from 16 sets on;
+    FMMH_OPT_M2L_MFMA = 3,        /* 0: M2L on the vector ALUs; 1 (default): FP64 MFMA from 16 sets on;
                                      2: MFMA for any set count (expansion orders <= 8) */
+    FMMH_OPT_REUSE_TREE = 4       /* value != 0: keep the last field_fmm octree, interaction lists and M2L
+                                     class tensors, and reuse them while the coordinates (compared bitwise
+                                     on the device), n, ncrit and theta are unchanged -- e.g. the induced-
+                                     dipole iterations of a polarizable embedding, for which the reference
+                                     rebuilds every call (src/fmm.F90:1387-1398).  Results are identical
+                                     to a rebuild.  0 (default) drops the cache. */
 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);

@@ -177,3 +177,29 @@ def test_missing_self_exclusion_gives_nonfinite_like_reference(gpu_ctx):
     e = np.zeros((len(x), 1), dtype=np.int32, order="F")
     f = gpu_ctx.field_direct(0, x, q, e, 0)
     assert not np.isfinite(f).all()
+
+
+def test_direct_targets_match_full_direct(gpu_ctx):
+    """Spot-check direct sum at a subset of sites equals those rows of the
+    full direct sum (last block of 13 targets runs with lane groups, so the
+    bar is rounding, not identity)."""
+    import torch
+    g = load_golden("water999_q.npz")
+    dev = torch.device("cuda:0")
+
+    def col(a, dt=torch.float64):
+        return torch.as_tensor(np.ascontiguousarray(np.asarray(a).T), dtype=dt, device=dev).T
+
+    x, m, e, df = g["coordinates"], g["multipoles"], g["exclusions"], g["thole_df"]
+    tg = np.random.default_rng(4).choice(x.shape[0], 77, replace=False).astype(np.int32)
+    out = col(np.zeros((77, 10)))
+    gpu_ctx.field_direct_targets_device(col(x), col(m), col(e, torch.int32), torch.as_tensor(tg, device=dev), 2,
+                                        out, "THOLE", torch.as_tensor(df, device=dev))
+    full = gpu_ctx.field_direct(0, x, m, e, 2, "THOLE", df)
+    got = out.cpu().numpy()
+    assert max(block_rel(got, full[tg], 2)) < 1e-14
+    np.testing.assert_array_equal(got[:64], full[tg[:64]])
+    from fmm_amd.octree import FmmError
+    with pytest.raises(FmmError, match="target index"):
+        bad = torch.as_tensor(np.array([0, x.shape[0]], np.int32), device=dev)
+        gpu_ctx.field_direct_targets_device(col(x), col(m), col(e, torch.int32), bad, 2, col(np.zeros((2, 10))))
