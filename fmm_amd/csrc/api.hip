@@ -44,6 +44,8 @@ struct Tables {
     const double* cs = nullptr;       // Taylor factors for the p expansion
     const double* scale = nullptr;    // Taylor factors for the dim source components
     int nsrc = 0;
+    const int* gemm_fidx = nullptr;   // class-grouped M2L: A-fragment index table (p <= M2L_GEMM_PMAX)
+    const int* gemm_tdeg = nullptr;   //   and |t| per target row
 };
 
 struct TreeOut {
@@ -75,7 +77,7 @@ struct fmmh_context {
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
-    int m2l_mfma = 1;                 // 0 never, 1 for >= 16 sets, 2 whenever the order allows
+    int m2l_mfma = 1;                 // FMMH_OPT_M2L_MFMA: 0 VALU, 1 auto, 2 per-target MFMA, 3 class-grouped
     // Tree / list cache (FMMH_OPT_REUSE_TREE): the last field_fmm tree, lists
     // and M2L classes, reused while coordinates (bitwise), n, ncrit and theta
     // stay the same.  The device data live in the named pool buffers, which
@@ -262,13 +264,31 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     std::vector<double> cs(nm), scale(ncomp(m));
     for (int s = 0; s < nm; ++s) cs[s] = taylor_factor(comp[s]);
     for (int s = 0; s < ncomp(m); ++s) scale[s] = taylor_factor(comp[s]);
+    // Class-grouped M2L fragment tables: lane l of the A fragment of row tile
+    // mt and source step ks holds H[t][s], t = 16 mt + l % 16, s = 4 ks + l / 16.
+    std::vector<int> gemm_fidx, gemm_tdeg;
+    if (p <= M2L_GEMM_PMAX) {
+        const int MT = (nm + 15) / 16, KS = (nm + 3) / 4;
+        gemm_fidx.assign((size_t)MT * KS * 64, -1);
+        gemm_tdeg.assign((size_t)MT * 16, 0);
+        for (int mt = 0; mt < MT; ++mt)
+            for (int ks = 0; ks < KS; ++ks)
+                for (int l = 0; l < 64; ++l) {
+                    const int t = 16 * mt + l % 16, s = 4 * ks + l / 16;
+                    if (t < nm && s < nm)
+                        gemm_fidx[((size_t)mt * KS + ks) * 64 + l] =
+                            idx(comp[t].x + comp[s].x, comp[t].y + comp[s].y, comp[t].z + comp[s].z);
+                }
+        for (int t = 0; t < nm; ++t) gemm_tdeg[t] = comp[t].x + comp[t].y + comp[t].z;
+    }
 
     // pack into one int and one double buffer
     std::vector<std::pair<std::vector<int>*, const int**>> iparts;
     Tables t;
     t.p = p; t.m = m; t.nsrc = ncomp(msrc);
     std::vector<std::vector<int>*> ilist = {&p2m_off, &p2m_s, &p2m_e, &m2m_off, &m2m_s, &m2m_e, &l2l_off, &l2l_s,
-                                            &l2l_e, &mono_parent, &mono_axis, &xyz, &level_base};
+                                            &l2l_e, &mono_parent, &mono_axis, &xyz, &level_base, &gemm_fidx,
+                                            &gemm_tdeg};
     std::vector<std::vector<double>*> dlist = {&p2m_c, &m2m_c, &l2l_c, &cs, &scale};
     std::vector<size_t> ioff, doff;
     size_t ni = 0, nd = 0;
@@ -291,6 +311,10 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     t.mono_axis = I + ioff[10];
     t.xyz = I + ioff[11];
     t.level_base = I + ioff[12];
+    if (p <= M2L_GEMM_PMAX) {
+        t.gemm_fidx = I + ioff[13];
+        t.gemm_tdeg = I + ioff[14];
+    }
     t.cs = D + doff[3];
     t.scale = D + doff[4];
     return c->tables.emplace(key, t).first->second;
@@ -330,7 +354,8 @@ __global__ void k_fill_i32(int* a, int v, int n) {
 }
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
-constexpr int TRAVERSE_GRID = 1024;   // workgroups per traversal step (grid-stride over the frontier)
+constexpr int TRAVERSE_GRID = 1024;
+constexpr double M2L_GEMM_MAX_BYTES = 48e9;   // per-pair slot buffer of the class-grouped M2L   // workgroups per traversal step (grid-stride over the frontier)
 
 struct Call {
     const double* X; int64_t ldx;
@@ -636,9 +661,11 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     const int nm = ncomp(p), nT = ncomp(2 * p);
     int* pair_class = c->buf<int>("m_pair_class", np);
     double* Tc = nullptr;
+    int ncls = 0;
     if (classes_cached) {
-        Tc = np > 0 ? c->buf<double>("m_tc", (size_t)c->cache_classes * nT) : nullptr;
-        c->stats_classes = c->cache_classes;
+        ncls = (int)c->cache_classes;
+        Tc = np > 0 ? c->buf<double>("m_tc", (size_t)ncls * nT) : nullptr;
+        c->stats_classes = ncls;
     } else if (np > 0) {
         int* dev_scalars = c->buf<int>("m_scalars", 4);   // num_classes, error
         FMMH_HIP(hipMemsetAsync(dev_scalars, 0, 4 * sizeof(int), s));
@@ -669,7 +696,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         FMMH_HIP(hipMemcpyAsync(c->h_pinned + 300, dev_scalars, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         c->sync();
         if (c->h_pinned[301]) throw Error(FMMH_ERR_UNSUPPORTED, "M2L displacement outside the class key range");
-        const int ncls = c->h_pinned[300];
+        ncls = c->h_pinned[300];
         c->stats_classes = ncls;
         Tc = c->buf<double>("m_tc", (size_t)ncls * nT);
         ca.Tc = Tc;
@@ -680,8 +707,34 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     c->mark("m2l_classes");
     const bool mfma = c->m2l_mfma == 2 ? m2l_mfma_supported(p, 16)
                                        : (c->m2l_mfma == 1 && m2l_mfma_supported(p, nsets));
-    const int chunk = mfma ? 32 : nsets;
     const size_t node_stride = (size_t)nn * nm;
+    // class-grouped GEMM: one set at a time, per-pair slots of np x nm doubles
+    const bool gemm = !mfma && (c->m2l_mfma == 1 || c->m2l_mfma == 3) && p <= M2L_GEMM_PMAX && np > 0 &&
+                      (double)np * nm * sizeof(double) <= M2L_GEMM_MAX_BYTES;
+    if (gemm) {
+        const int* flag = c->buf<int>("m_flag", np);
+        const int* cls = c->buf<int>("m_cls", np);
+        const int* pair_csr = c->buf<int>("m_idx_sorted", np);
+        const unsigned long long* class_key = c->buf<unsigned long long>("m_class_key", np);
+        int* class_off = c->buf<int>("m_class_off", ncls + 1);
+        int* tile_cnt = c->buf<int>("m_tile_cnt", ncls + 1);
+        int* tile_off = c->buf<int>("m_tile_off", ncls + 1);
+        launch_m2l_tiles(flag, cls, np, ncls, class_off, tile_cnt, s);
+        size_t tbytes = exscan_i32_temp(ncls + 1);
+        exscan_i32(c->buf<char>("m_temp_tiles", tbytes), tbytes, tile_cnt, tile_off, ncls + 1, s);
+        const int grid = ncls + (np + M2L_GEMM_TN - 1) / M2L_GEMM_TN;   // >= tiles
+        double* pbuf = c->buf<double>("m_pbuf", (size_t)np * nm);
+        double* CMs = c->buf<double>("m_cms", node_stride * nsets);
+        launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
+        for (int set = 0; set < nsets; ++set) {
+            M2LGemmArgs ga{class_off, tile_off, ncls, pair_csr, L.m2l_src, class_key, Tc, CMs + set * node_stride,
+                           tb.gemm_fidx, tb.gemm_tdeg, pbuf, p};
+            launch_m2l_gemm(ga, grid, s);
+            launch_m2l_reduce(pbuf, L.m2l_off, tr.t.r, tb.gemm_tdeg, LE + set * node_stride, nm, nn, s);
+        }
+        return;
+    }
+    const int chunk = mfma ? 32 : nsets;
     double* CMs = c->buf<double>("m_cms", node_stride * (mfma ? std::min(32, ((nsets + 15) / 16) * 16) : nsets));
     for (int c0 = 0; c0 < nsets; c0 += chunk) {
         const int ns = std::min(chunk, nsets - c0);
@@ -1159,7 +1212,8 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
             c->reuse_tree = value != 0.0;
             if (!c->reuse_tree) c->cache_valid = false;
         } else if (key == FMMH_OPT_M2L_MFMA) {
-            if (value != 0.0 && value != 1.0 && value != 2.0) throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1 or 2");
+            if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0)
+                throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1, 2 or 3");
             c->m2l_mfma = (int)value;
         }
         else throw Error(FMMH_ERR_ARG, "unknown option key " + std::to_string(key));

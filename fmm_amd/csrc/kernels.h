@@ -196,6 +196,34 @@ void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s);
 bool m2l_mfma_supported(int p, int nsets);
 void launch_m2l_mfma(const M2LArgs& a, int num_nodes, hipStream_t s);
 
+// Class-grouped M2L: all pairs of one displacement class share one Hankel
+// matrix H_c[t][s] = scale_c(t) T~_c[s + t], so per class the M2L is the GEMM
+// H_c (n_m x n_m) x [M~_j ...] (n_m x pairs) on the FP64 matrix cores, in
+// tiles of 32 pairs.  Each pair's n_m results go to its CSR slot of pbuf;
+// launch_m2l_reduce then sums every target's slots in CSR order (fixed order,
+// deterministic) and applies r_i^{-(|t|+1)}.
+struct M2LGemmArgs {
+    const int* class_off;                // ncls + 1: first class-sorted pair of each class
+    const int* tile_off;                 // ncls + 1: first tile of each class, tile_off[ncls] = tiles
+    int ncls;
+    const int* pair_csr;                 // class-sorted pair -> CSR position
+    const int* src;                      // CSR position -> source node
+    const unsigned long long* class_key; // class -> packed (v, level difference)
+    const double* Tc;                    // class tensors, ncomp(2p) each
+    const double* CMs;                   // scaled source expansions of one set, [node][nm]
+    const int* fidx;                     // [MT][KS][64]: packed index of s + t per A-fragment lane, -1 outside
+    const int* tdeg;                     // [16 MT]: |t| of each target row, 0 past nm
+    double* pbuf;                        // [CSR position][nm] per-pair partial local coefficients
+    int p;
+};
+constexpr int M2L_GEMM_PMAX = 10;
+constexpr int M2L_GEMM_TN = 32;          // pairs per tile
+void launch_m2l_tiles(const int* flag, const int* cls, int n_pairs, int ncls, int* class_off, int* tile_cnt,
+                      hipStream_t s);
+void launch_m2l_gemm(const M2LGemmArgs& a, int grid, hipStream_t s);
+void launch_m2l_reduce(const double* pbuf, const int* off, const double* r, const int* tdeg, double* LE, int nm,
+                       int num_nodes, hipStream_t s);
+
 struct DownArgs {
     const double* cx; const double* cy; const double* cz;
     const int* parent;

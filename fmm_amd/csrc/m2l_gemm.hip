@@ -1,0 +1,191 @@
+// Class-grouped M2L (src/fmm.F90:826-882) for one multipole set at a time.
+//
+// Every far pair (i, j) belongs to a displacement class c = (v, level_i -
+// level_j) (m2l.hip), and the pair's contribution is
+//   L~_i(t) += scale_c(t) sum_s T~_c[s + t] M~_j(s),
+// so all pairs of a class apply the same n_m x n_m Hankel matrix H_c.  Per
+// class this is a GEMM, H_c x [M~_j1 M~_j2 ...], run here on
+// v_mfma_f64_16x16x4f64: a workgroup of four wavefronts takes one tile of up
+// to 32 pairs of one class, stages the class tensor and the 32 source
+// expansions in LDS, and each wavefront accumulates a quarter of the 16-row
+// target tiles.  A fragments are gathered from the class tensor through a
+// host-built index table (packed index of s + t per fragment lane); the
+// level-difference factor of the class is a per-row scale.
+//
+// Results are written per pair to its CSR slot, and k_m2l_reduce sums each
+// target's slots in CSR order, so the sum order is fixed (deterministic) and
+// independent of the tiling.
+//
+// MFMA operand layout of v_mfma_f64_16x16x4f64 (measured on gfx950): A lane
+// l -> A[l % 16][l / 16], B lane l -> B[l / 16][l % 16], accumulator
+// register q of lane l -> D[4 q + l / 16][l % 16].
+#include "common.h"
+#include "kernels.h"
+
+namespace fmmh {
+
+using v4d = double __attribute__((ext_vector_type(4)));
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+__global__ __launch_bounds__(256) void k_m2l_tiles(const int* __restrict__ flag, const int* __restrict__ cls,
+                                                   int n_pairs, int ncls, int* __restrict__ class_off,
+                                                   int* __restrict__ tile_cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_pairs && flag[k]) class_off[cls[k]] = k;   // cls = exclusive scan of flag = class id at a start
+    if (k == 0) {
+        class_off[ncls] = n_pairs;
+        tile_cnt[ncls] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_m2l_tile_count(const int* __restrict__ class_off, int ncls,
+                                                        int* __restrict__ tile_cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncls) tile_cnt[c] = (class_off[c + 1] - class_off[c] + M2L_GEMM_TN - 1) / M2L_GEMM_TN;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
+    constexpr int NM = ncomp(P);
+    constexpr int NT = ncomp(2 * P);
+    constexpr int MT = (NM + 15) / 16;          // 16-row target tiles
+    constexpr int KS = (NM + 3) / 4;            // 4-deep source steps
+    constexpr int MW = (MT + 3) / 4;            // target tiles per wavefront
+    constexpr int TN = M2L_GEMM_TN;
+    constexpr int NTL = TN / 16;
+    constexpr int BLD = NM | 1;                 // odd LDS row stride of the source tile
+    __shared__ double Ts[NT];
+    __shared__ double Bs[TN * BLD];
+    __shared__ int ecsr[TN];
+    __shared__ int jsrc[TN];
+    const int b = blockIdx.x;
+    const int total = a.tile_off[a.ncls];
+    if (b >= total) return;                     // grid is an upper bound on the tile count
+    int lo = 0, hi = a.ncls;                    // class of this tile: last c with tile_off[c] <= b
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.tile_off[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const int c = lo;
+    const int k0 = a.class_off[c] + (b - a.tile_off[c]) * TN;
+    const int cnt = min(TN, a.class_off[c + 1] - k0);
+    const int dl = (int)(a.class_key[c] & 255ull) - 128;
+    const int tid = threadIdx.x;
+    for (int x = tid; x < NT; x += 256) Ts[x] = a.Tc[(int64_t)c * NT + x];
+    if (tid < TN) {
+        const int e = tid < cnt ? a.pair_csr[k0 + tid] : -1;
+        ecsr[tid] = e;
+        jsrc[tid] = e >= 0 ? a.src[e] : -1;
+    }
+    __syncthreads();
+    for (int x = tid; x < TN * NM; x += 256) {
+        const int n = x / NM, s = x - n * NM;
+        const int j = jsrc[n];
+        Bs[n * BLD + s] = j >= 0 ? a.CMs[(int64_t)j * NM + s] : 0.0;
+    }
+    __syncthreads();
+    const int w = tid >> 6, lane = tid & 63;
+    const int c16 = lane & 15, g4 = lane >> 4;
+    double rs[MW];
+#pragma unroll
+    for (int mi = 0; mi < MW; ++mi) {
+        const int mt = w + 4 * mi;
+        const int deg = mt < MT ? a.tdeg[mt * 16 + c16] : 0;
+        rs[mi] = dl > 0 ? ldexp(1.0, -dl * deg) : (dl < 0 ? ldexp(1.0, -dl * (deg + 1)) : 1.0);
+    }
+    v4d acc[MW][NTL];
+#pragma unroll
+    for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int* __restrict__ fx = a.fidx + lane;
+#pragma unroll 2
+    for (int ks = 0; ks < KS; ++ks) {
+        const int s = ks * 4 + g4;
+        double bv[NTL];
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) bv[nt] = s < NM ? Bs[(nt * 16 + c16) * BLD + s] : 0.0;
+#pragma unroll
+        for (int mi = 0; mi < MW; ++mi) {
+            const int mt = w + 4 * mi;
+            if (mt < MT) {
+                const int f = fx[(mt * KS + ks) * 64];
+                const double av = f >= 0 ? Ts[f] * rs[mi] : 0.0;
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt)
+                    acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[nt], acc[mi][nt], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int mi = 0; mi < MW; ++mi) {
+        const int mt = w + 4 * mi;
+        if (mt >= MT) continue;
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) {
+            const int n = nt * 16 + c16;
+            if (n >= cnt) continue;
+            double* __restrict__ out = a.pbuf + (int64_t)ecsr[n] * NM;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = mt * 16 + 4 * q + g4;
+                if (t < NM) out[t] = acc[mi][nt][q];
+            }
+        }
+    }
+}
+
+// L_i(t) = r_i^{-(|t|+1)} sum over the CSR slots of target i, in CSR order.
+__global__ __launch_bounds__(128) void k_m2l_reduce(const double* __restrict__ pbuf, const int* __restrict__ off,
+                                                    const double* __restrict__ r, const int* __restrict__ tdeg,
+                                                    double* __restrict__ LE, int nm) {
+    const int i = blockIdx.x;
+    const int e0 = off[i], e1 = off[i + 1];
+    const double rinv = 1.0 / r[i];
+    for (int t = threadIdx.x; t < nm; t += 128) {
+        double sum = 0.0;
+        for (int e = e0; e < e1; ++e) sum += pbuf[(int64_t)e * nm + t];
+        double f = rinv;
+        for (int k = 0; k < tdeg[t]; ++k) f *= rinv;
+        LE[(int64_t)i * nm + t] = sum * f;
+    }
+}
+
+void launch_m2l_tiles(const int* flag, const int* cls, int n_pairs, int ncls, int* class_off, int* tile_cnt,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_m2l_tiles, dim3(nblk(std::max(n_pairs, 1), 256)), dim3(256), 0, s, flag, cls, n_pairs, ncls,
+                       class_off, tile_cnt);
+    if (ncls > 0)
+        hipLaunchKernelGGL(k_m2l_tile_count, dim3(nblk(ncls, 256)), dim3(256), 0, s, class_off, ncls, tile_cnt);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2l_gemm(const M2LGemmArgs& a, int grid, hipStream_t s) {
+    if (grid <= 0) return;
+    dim3 g(grid), b(256);
+    switch (a.p) {
+        case 0: hipLaunchKernelGGL(k_m2l_gemm<0>, g, b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_m2l_gemm<1>, g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_m2l_gemm<2>, g, b, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_m2l_gemm<3>, g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_m2l_gemm<4>, g, b, 0, s, a); break;
+        case 5: hipLaunchKernelGGL(k_m2l_gemm<5>, g, b, 0, s, a); break;
+        case 6: hipLaunchKernelGGL(k_m2l_gemm<6>, g, b, 0, s, a); break;
+        case 7: hipLaunchKernelGGL(k_m2l_gemm<7>, g, b, 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_m2l_gemm<8>, g, b, 0, s, a); break;
+        case 9: hipLaunchKernelGGL(k_m2l_gemm<9>, g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL(k_m2l_gemm<10>, g, b, 0, s, a); break;
+    }
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_m2l_reduce(const double* pbuf, const int* off, const double* r, const int* tdeg, double* LE, int nm,
+                       int num_nodes, hipStream_t s) {
+    if (num_nodes <= 0) return;
+    hipLaunchKernelGGL(k_m2l_reduce, dim3(num_nodes), dim3(128), 0, s, pbuf, off, r, tdeg, LE, nm);
+    FMMH_HIP(hipGetLastError());
+}
+
+}  // namespace fmmh

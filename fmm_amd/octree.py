@@ -242,7 +242,8 @@ class Context:
         check(self._lib.fmmh_set_option(self._h, 4, 1.0 if on else 0.0))
 
     def set_m2l_mfma(self, mode=1):
-        """M2L engine: 0 vector ALUs, 1 FP64 MFMA from 16 sets on (default), 2 MFMA for any set count."""
+        """M2L engine: 0 per-target VALU, 1 auto (default: per-target MFMA over >= 16 sets, else the
+        class-grouped MFMA GEMM), 2 per-target MFMA for any set count, 3 class-grouped always."""
         check(self._lib.fmmh_set_option(self._h, 3, float(mode)))
 
     def phase_times(self):

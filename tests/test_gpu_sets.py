@@ -51,9 +51,10 @@ def test_direct_sets_bit_identical(gpu_ctx, dim, F, damp):
 
 
 @pytest.mark.parametrize("nsets", [2, 3, 7])
-def test_fmm_sets_valu_bit_identical(gpu_ctx, mfma_mode, nsets):
-    """Below 16 sets the M2L runs per set on the vector ALUs: same arithmetic
-    as the single-set call, so the fields are identical bit for bit."""
+def test_fmm_sets_below_16_bit_identical(gpu_ctx, mfma_mode, nsets):
+    """Below 16 sets the M2L runs set by set (class-grouped GEMM): the same
+    arithmetic as the single-set call, so the fields are identical bit for
+    bit."""
     g = load_golden("water999_q.npz")
     x, e, df = g["coordinates"], g["exclusions"], g["thole_df"]
     m = _sets(x.shape[0], 4, nsets, 12)
@@ -74,6 +75,31 @@ def test_m2l_mfma_matches_valu_every_order(gpu_ctx, mfma_mode, p):
     got = gpu_ctx.field_fmm_sets(0, x, m, e, 0.5, 64, p, 1)
     for k in range(3):
         assert max(block_rel(got[:, :, k], ref[:, :, k], 1)) < PARITY
+
+
+@pytest.mark.parametrize("p", list(range(0, 11)))
+def test_m2l_class_grouped_matches_valu_every_order(gpu_ctx, mfma_mode, p):
+    """Class-grouped FP64 MFMA GEMM M2L (the default for one set) against the
+    per-target VALU contraction, every order 0..10."""
+    g = load_golden("water999_dipole_orders.npz")
+    x, m, e = g["coordinates"], g["multipoles"], g["exclusions"]
+    mfma_mode(0)
+    ref = gpu_ctx.field_fmm(0, x, m, e, 0.5, 64, p, 1)
+    mfma_mode(1)
+    got = gpu_ctx.field_fmm(0, x, m, e, 0.5, 64, p, 1)
+    assert max(block_rel(got, ref, 1)) < PARITY
+
+
+def test_m2l_class_grouped_for_many_sets(gpu_ctx, mfma_mode):
+    """Mode 3 forces the class-grouped GEMM for 18 sets (set by set)."""
+    g = load_golden("water999_q.npz")
+    x, e = g["coordinates"], g["exclusions"]
+    m = _sets(x.shape[0], 4, 18, 21)
+    ref = gpu_ctx.field_fmm_sets(0, x, m, e, 0.5, 64, 6, 1)
+    mfma_mode(3)
+    got = gpu_ctx.field_fmm_sets(0, x, m, e, 0.5, 64, 6, 1)
+    for k in range(18):
+        assert max(block_rel(got[:, :, k], ref[:, :, k], 1)) < PARITY, k
 
 
 @pytest.mark.parametrize("nsets,p", [(16, 6), (20, 5), (32, 6), (40, 8)])
