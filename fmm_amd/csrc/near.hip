@@ -18,25 +18,37 @@ constexpr int NEAR_EXCL_REG = 4;   // exclusions kept in registers; more are rea
 
 // Lane layout: a target block of nt <= 64 sites is replicated over
 // r = 64 / nt lane groups (group g = lane / nt, target lane % nt); group g
-// takes every r-th source, so small leaves still fill the wavefront.  Source
-// runs are staged 64 sites at a time in LDS (one coalesced load per lane per
-// array) and read back per group; at the end the groups' partial fields are
-// summed in group order through LDS (deterministic).
+// takes every r-th source, so small leaves still fill the wavefront.  At the
+// end the groups' partial fields are summed in group order through LDS
+// (deterministic).
+// Source staging: the source blocks of a target block's list are one
+// concatenated run (windows of NEAR_WIN list entries, whose exclusive prefix
+// of source counts is scanned across the wavefront), cut into tiles of 64
+// sources regardless of block boundaries.  Each tile is published to LDS
+// from registers that were loaded while the previous tile was computed, so
+// the global-load latency overlaps the pair arithmetic.
 // Multipole sets: workgroup row blockIdx.y takes S consecutive sets; each pair
 // tensor is built once and contracted with the S sets' multipoles, so the
 // tensor recursion is shared (S = 1 for a single set).
-// Tensor orders up to 3 (except ERF, whose erf needs more) fit 128 VGPRs (4
-// waves per SIMD) without spilling; the S > 1 forms and the rest get more.
+// Tensor orders up to 3 (except ERF, whose erf needs more) with the prefetch
+// registers fit 168 VGPRs (3 waves per SIMD) without spilling; the S > 1
+// forms and the rest get more.
+constexpr int NEAR_WIN = 64;
+
 template <int M, int F, int DAMP, int S>
-__global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 1)
+__global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 : 1)
                                         : ((M + F <= 3 && DAMP != DAMP_ERF) ? 2 : 1)) void k_near(NearArgs a) {
     constexpr int K = M + F;
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
+    constexpr bool PREF_M = NM * S <= 4;          // multipoles ride in the prefetch registers too
+    constexpr int NPM = PREF_M ? NM * S : 1;
     __shared__ double sx[64], sy[64], sz[64], sdf[64];
     __shared__ double sm[S][NM][64];
     __shared__ int sorig[64];
     __shared__ double red[NF][64];
+    __shared__ int wcum[NEAR_WIN + 1];            // window: exclusive prefix of source counts
+    __shared__ int wstart[NEAR_WIN];              // window: first sorted site of each entry
     const int b = a.blk[blockIdx.x];
     const int set0 = blockIdx.y * S;
     const int ns = min(S, a.nsets - set0);   // sets of this row (the rest are zero padding)
@@ -64,26 +76,74 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
         for (int u = 0; u < S; ++u)
 #pragma unroll
             for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
-        for (int e = e0; e < e1; ++e) {
-            const int sb = a.list_src[e];
-            const int s0 = a.src_start[sb];
-            const int s1 = s0 + a.src_count[sb];
-            for (int c0 = s0; c0 < s1; c0 += 64) {
-                const int cn = min(64, s1 - c0);
+        for (int w0 = e0; w0 < e1; w0 += NEAR_WIN) {
+            const int wn = min(NEAR_WIN, e1 - w0);
+            int cnt = 0, st = 0;
+            if (lane < wn) {
+                const int sb = a.list_src[w0 + lane];
+                st = a.src_start[sb];
+                cnt = a.src_count[sb];
+            }
+            int incl = cnt;                      // inclusive scan across the wavefront
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y = __shfl_up(incl, d);
+                if (lane >= d) incl += y;
+            }
+            __syncthreads();                     // readers of the previous window are done
+            if (lane < wn) {
+                wcum[lane] = incl - cnt;
+                wstart[lane] = st;
+            }
+            if (lane == wn - 1) wcum[wn] = incl;
+            __syncthreads();
+            const int total = wcum[wn];
+            // prefetch registers for tile position v = vbase + lane
+            double px = 0.0, py = 0.0, pz = 0.0, pdf = 0.0;
+            double pm[NPM];
+            int po = 0, pj = 0;
+            int cur = 0;                         // list entry holding the tile's first position
+            auto fetch = [&](int vbase) {
+                while (wcum[cur + 1] <= vbase) ++cur;
+                const int v = vbase + lane;
+                if (v < total) {
+                    int e = cur;
+                    while (wcum[e + 1] <= v) ++e;
+                    const int j = wstart[e] + (v - wcum[e]);
+                    pj = j;
+                    px = a.xs[j];
+                    py = a.ys[j];
+                    pz = a.zs[j];
+                    if constexpr (DAMP != DAMP_NONE) pdf = a.dfs[j];
+                    po = a.orig[j] + 1;
+                    if constexpr (PREF_M) {
+#pragma unroll
+                        for (int u = 0; u < S; ++u)
+#pragma unroll
+                            for (int s = 0; s < NM; ++s)
+                                pm[u * NM + s] = u < ns ? ms[((int64_t)u * NM + s) * a.ldm + j] : 0.0;
+                    }
+                }
+            };
+            if (total > 0) fetch(0);
+            for (int vbase = 0; vbase < total; vbase += 64) {
+                const int cn = min(64, total - vbase);
                 if (lane < cn) {
-                    const int j = c0 + lane;
-                    sx[lane] = a.xs[j];
-                    sy[lane] = a.ys[j];
-                    sz[lane] = a.zs[j];
-                    if constexpr (DAMP != DAMP_NONE) sdf[lane] = a.dfs[j];
-                    sorig[lane] = a.orig[j] + 1;
+                    sx[lane] = px;
+                    sy[lane] = py;
+                    sz[lane] = pz;
+                    if constexpr (DAMP != DAMP_NONE) sdf[lane] = pdf;
+                    sorig[lane] = po;
 #pragma unroll
                     for (int u = 0; u < S; ++u)
 #pragma unroll
-                        for (int s = 0; s < NM; ++s)
-                            sm[u][s][lane] = u < ns ? ms[((int64_t)u * NM + s) * a.ldm + j] : 0.0;
+                        for (int s = 0; s < NM; ++s) {
+                            if constexpr (PREF_M) sm[u][s][lane] = pm[u * NM + s];
+                            else sm[u][s][lane] = u < ns ? ms[((int64_t)u * NM + s) * a.ldm + pj] : 0.0;
+                        }
                 }
                 __syncthreads();
+                if (vbase + 64 < total) fetch(vbase + 64);
                 if (active) {
                     for (int jj = g; jj < cn; jj += r) {
                         double dx = xi - sx[jj];
