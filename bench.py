@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="default: mixed for config 3 (its workload), fp64 otherwise")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -124,6 +126,8 @@ def main():
     out = col(np.zeros((n, nf)), dev) if sets == 1 else col3(np.zeros((n, nf, sets)), dev)
     ctx = Context(0)
     ctx.set_timing(True)
+    precision = args.precision or ("mixed" if args.config == 3 else "fp64")
+    ctx.set_precision(precision == "mixed")
 
     def step():
         out.zero_()
@@ -241,9 +245,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if precision == "fp64" else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": c["name"], "sites": n, "sets": sets, "multipole_dim": dim, "field_order": F,
+        "config": {"workload": c["name"], "sites": n, "sets": sets, "precision": precision, "multipole_dim": dim, "field_order": F,
                    "theta": c["theta"], "ncrit": c["ncrit"], "expansion_order": c["expansion_order"],
                    "damp_type": c["damp_type"], "exclusions_per_site": int(c["exclusions"].shape[1])},
         "roofline": roofline,

@@ -78,6 +78,7 @@ struct fmmh_context {
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
     int m2l_mfma = 1;                 // FMMH_OPT_M2L_MFMA: 0 VALU, 1 auto, 2 per-target MFMA, 3 class-grouped
+    int precision = 0;                // FMMH_OPT_PRECISION: 0 FP64, 1 mixed (FP32 MFMA translations)
     // Tree / list cache (FMMH_OPT_REUSE_TREE): the last field_fmm tree, lists
     // and M2L classes, reused while coordinates (bitwise), n, ncrit and theta
     // stay the same.  The device data live in the named pool buffers, which
@@ -723,14 +724,18 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         size_t tbytes = exscan_i32_temp(ncls + 1);
         exscan_i32(c->buf<char>("m_temp_tiles", tbytes), tbytes, tile_cnt, tile_off, ncls + 1, s);
         const int grid = ncls + (np + M2L_GEMM_TN - 1) / M2L_GEMM_TN;   // >= tiles
-        double* pbuf = c->buf<double>("m_pbuf", (size_t)np * nm);
+        const bool f32 = c->precision == 1;
+        double* pbuf = f32 ? nullptr : c->buf<double>("m_pbuf", (size_t)np * nm);
+        float* pbuf32 = f32 ? c->buf<float>("m_pbuf32", (size_t)np * nm) : nullptr;
+        float* cms32 = f32 ? c->buf<float>("m_cms32", node_stride) : nullptr;
         double* CMs = c->buf<double>("m_cms", node_stride * nsets);
         launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
         for (int set = 0; set < nsets; ++set) {
+            if (f32) launch_to_f32(CMs + set * node_stride, cms32, (int64_t)node_stride, s);
             M2LGemmArgs ga{class_off, tile_off, ncls, pair_csr, L.m2l_src, class_key, Tc, CMs + set * node_stride,
-                           tb.gemm_fidx, tb.gemm_tdeg, pbuf, p};
+                           tb.gemm_fidx, tb.gemm_tdeg, pbuf, p, f32 ? 1 : 0, cms32, pbuf32};
             launch_m2l_gemm(ga, grid, s);
-            launch_m2l_reduce(pbuf, L.m2l_off, tr.t.r, tb.gemm_tdeg, LE + set * node_stride, nm, nn, s);
+            launch_m2l_reduce(ga, L.m2l_off, tr.t.r, LE + set * node_stride, nn, s);
         }
         return;
     }
@@ -1211,6 +1216,9 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         else if (key == FMMH_OPT_REUSE_TREE) {
             c->reuse_tree = value != 0.0;
             if (!c->reuse_tree) c->cache_valid = false;
+        } else if (key == FMMH_OPT_PRECISION) {
+            if (value != 0.0 && value != 1.0) throw Error(FMMH_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+            c->precision = (int)value;
         } else if (key == FMMH_OPT_M2L_MFMA) {
             if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0)
                 throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1, 2 or 3");

@@ -215,14 +215,18 @@ struct M2LGemmArgs {
     const int* tdeg;                     // [16 MT]: |t| of each target row, 0 past nm
     double* pbuf;                        // [CSR position][nm] per-pair partial local coefficients
     int p;
+    int f32;                             // mixed precision: FP32 MFMA on CMs32, slots in pbuf32
+    const float* CMs32;
+    float* pbuf32;
 };
 constexpr int M2L_GEMM_PMAX = 10;
 constexpr int M2L_GEMM_TN = 32;          // pairs per tile
 void launch_m2l_tiles(const int* flag, const int* cls, int n_pairs, int ncls, int* class_off, int* tile_cnt,
                       hipStream_t s);
 void launch_m2l_gemm(const M2LGemmArgs& a, int grid, hipStream_t s);
-void launch_m2l_reduce(const double* pbuf, const int* off, const double* r, const int* tdeg, double* LE, int nm,
-                       int num_nodes, hipStream_t s);
+void launch_m2l_reduce(const M2LGemmArgs& a, const int* off, const double* r, double* LE, int num_nodes,
+                       hipStream_t s);
+void launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 
 struct DownArgs {
     const double* cx; const double* cy; const double* cz;

@@ -16,15 +16,41 @@
 // target's slots in CSR order, so the sum order is fixed (deterministic) and
 // independent of the tiling.
 //
-// MFMA operand layout of v_mfma_f64_16x16x4f64 (measured on gfx950): A lane
-// l -> A[l % 16][l / 16], B lane l -> B[l / 16][l % 16], accumulator
-// register q of lane l -> D[4 q + l / 16][l % 16].
+// Mixed precision (F32): the same tiles on v_mfma_f32_16x16x4f32 with FP32
+// sources and FP32 accumulation over s; the class tensor stays FP64 in LDS
+// and each A element is rounded once after its row scale; per-pair results
+// are FP32 and the reduction accumulates them in FP64 (exact for any
+// realistic number of pairs per target, so no compensation term is needed).
+//
+// MFMA operand layouts (measured on gfx950): A lane l -> A[l % 16][l / 16],
+// B lane l -> B[l / 16][l % 16] for both; accumulator register q of lane l
+// -> D[4 q + l / 16][l % 16] for f64_16x16x4f64 and D[4 (l / 16) + q][l % 16]
+// for f32_16x16x4f32.
 #include "common.h"
 #include "kernels.h"
 
 namespace fmmh {
 
 using v4d = double __attribute__((ext_vector_type(4)));
+using v4f = float __attribute__((ext_vector_type(4)));
+
+template <bool F32> struct GemmT;
+template <> struct GemmT<false> {
+    using E = double;
+    using Acc = v4d;
+    static __device__ __forceinline__ Acc mfma(E a, E b, Acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int q, int g4) { return 4 * q + g4; }
+};
+template <> struct GemmT<true> {
+    using E = float;
+    using Acc = v4f;
+    static __device__ __forceinline__ Acc mfma(E a, E b, Acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int q, int g4) { return 4 * g4 + q; }
+};
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
@@ -45,8 +71,10 @@ __global__ __launch_bounds__(256) void k_m2l_tile_count(const int* __restrict__ 
     if (c < ncls) tile_cnt[c] = (class_off[c + 1] - class_off[c] + M2L_GEMM_TN - 1) / M2L_GEMM_TN;
 }
 
-template <int P>
+template <int P, bool F32>
 __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
+    using G = GemmT<F32>;
+    using E = typename G::E;
     constexpr int NM = ncomp(P);
     constexpr int NT = ncomp(2 * P);
     constexpr int MT = (NM + 15) / 16;          // 16-row target tiles
@@ -56,7 +84,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
     constexpr int NTL = TN / 16;
     constexpr int BLD = NM | 1;                 // odd LDS row stride of the source tile
     __shared__ double Ts[NT];
-    __shared__ double Bs[TN * BLD];
+    __shared__ E Bs[TN * BLD];
     __shared__ int ecsr[TN];
     __shared__ int jsrc[TN];
     const int b = blockIdx.x;
@@ -80,10 +108,11 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
         jsrc[tid] = e >= 0 ? a.src[e] : -1;
     }
     __syncthreads();
+    const E* __restrict__ cms = F32 ? (const E*)a.CMs32 : (const E*)a.CMs;
     for (int x = tid; x < TN * NM; x += 256) {
         const int n = x / NM, s = x - n * NM;
         const int j = jsrc[n];
-        Bs[n * BLD + s] = j >= 0 ? a.CMs[(int64_t)j * NM + s] : 0.0;
+        Bs[n * BLD + s] = j >= 0 ? cms[(int64_t)j * NM + s] : E(0);
     }
     __syncthreads();
     const int w = tid >> 6, lane = tid & 63;
@@ -95,27 +124,26 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
         const int deg = mt < MT ? a.tdeg[mt * 16 + c16] : 0;
         rs[mi] = dl > 0 ? ldexp(1.0, -dl * deg) : (dl < 0 ? ldexp(1.0, -dl * (deg + 1)) : 1.0);
     }
-    v4d acc[MW][NTL];
+    typename G::Acc acc[MW][NTL];
 #pragma unroll
     for (int mi = 0; mi < MW; ++mi)
 #pragma unroll
-        for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = typename G::Acc{0, 0, 0, 0};
     const int* __restrict__ fx = a.fidx + lane;
 #pragma unroll 2
     for (int ks = 0; ks < KS; ++ks) {
         const int s = ks * 4 + g4;
-        double bv[NTL];
+        E bv[NTL];
 #pragma unroll
-        for (int nt = 0; nt < NTL; ++nt) bv[nt] = s < NM ? Bs[(nt * 16 + c16) * BLD + s] : 0.0;
+        for (int nt = 0; nt < NTL; ++nt) bv[nt] = s < NM ? Bs[(nt * 16 + c16) * BLD + s] : E(0);
 #pragma unroll
         for (int mi = 0; mi < MW; ++mi) {
             const int mt = w + 4 * mi;
             if (mt < MT) {
                 const int f = fx[(mt * KS + ks) * 64];
-                const double av = f >= 0 ? Ts[f] * rs[mi] : 0.0;
+                const E av = f >= 0 ? E(Ts[f] * rs[mi]) : E(0);
 #pragma unroll
-                for (int nt = 0; nt < NTL; ++nt)
-                    acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[nt], acc[mi][nt], 0, 0, 0);
+                for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = G::mfma(av, bv[nt], acc[mi][nt]);
             }
         }
     }
@@ -127,18 +155,20 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
         for (int nt = 0; nt < NTL; ++nt) {
             const int n = nt * 16 + c16;
             if (n >= cnt) continue;
-            double* __restrict__ out = a.pbuf + (int64_t)ecsr[n] * NM;
+            E* __restrict__ out = (F32 ? (E*)a.pbuf32 : (E*)a.pbuf) + (int64_t)ecsr[n] * NM;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int t = mt * 16 + 4 * q + g4;
+                const int t = mt * 16 + G::row(q, g4);
                 if (t < NM) out[t] = acc[mi][nt][q];
             }
         }
     }
 }
 
-// L_i(t) = r_i^{-(|t|+1)} sum over the CSR slots of target i, in CSR order.
-__global__ __launch_bounds__(128) void k_m2l_reduce(const double* __restrict__ pbuf, const int* __restrict__ off,
+// L_i(t) = r_i^{-(|t|+1)} sum over the CSR slots of target i, in CSR order
+// (FP64 accumulation for either slot type).
+template <class E>
+__global__ __launch_bounds__(128) void k_m2l_reduce(const E* __restrict__ pbuf, const int* __restrict__ off,
                                                     const double* __restrict__ r, const int* __restrict__ tdeg,
                                                     double* __restrict__ LE, int nm) {
     const int i = blockIdx.x;
@@ -146,7 +176,7 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(const double* __restrict__ p
     const double rinv = 1.0 / r[i];
     for (int t = threadIdx.x; t < nm; t += 128) {
         double sum = 0.0;
-        for (int e = e0; e < e1; ++e) sum += pbuf[(int64_t)e * nm + t];
+        for (int e = e0; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nm + t];
         double f = rinv;
         for (int k = 0; k < tdeg[t]; ++k) f *= rinv;
         LE[(int64_t)i * nm + t] = sum * f;
@@ -162,29 +192,51 @@ void launch_m2l_tiles(const int* flag, const int* cls, int n_pairs, int ncls, in
     FMMH_HIP(hipGetLastError());
 }
 
-void launch_m2l_gemm(const M2LGemmArgs& a, int grid, hipStream_t s) {
-    if (grid <= 0) return;
+template <bool F32>
+static void launch_gemm_p(const M2LGemmArgs& a, int grid, hipStream_t s) {
     dim3 g(grid), b(256);
     switch (a.p) {
-        case 0: hipLaunchKernelGGL(k_m2l_gemm<0>, g, b, 0, s, a); break;
-        case 1: hipLaunchKernelGGL(k_m2l_gemm<1>, g, b, 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_m2l_gemm<2>, g, b, 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_m2l_gemm<3>, g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_m2l_gemm<4>, g, b, 0, s, a); break;
-        case 5: hipLaunchKernelGGL(k_m2l_gemm<5>, g, b, 0, s, a); break;
-        case 6: hipLaunchKernelGGL(k_m2l_gemm<6>, g, b, 0, s, a); break;
-        case 7: hipLaunchKernelGGL(k_m2l_gemm<7>, g, b, 0, s, a); break;
-        case 8: hipLaunchKernelGGL(k_m2l_gemm<8>, g, b, 0, s, a); break;
-        case 9: hipLaunchKernelGGL(k_m2l_gemm<9>, g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL(k_m2l_gemm<10>, g, b, 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_m2l_gemm<0, F32>), g, b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_m2l_gemm<1, F32>), g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_m2l_gemm<2, F32>), g, b, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_m2l_gemm<3, F32>), g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_m2l_gemm<4, F32>), g, b, 0, s, a); break;
+        case 5: hipLaunchKernelGGL((k_m2l_gemm<5, F32>), g, b, 0, s, a); break;
+        case 6: hipLaunchKernelGGL((k_m2l_gemm<6, F32>), g, b, 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_m2l_gemm<7, F32>), g, b, 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_m2l_gemm<8, F32>), g, b, 0, s, a); break;
+        case 9: hipLaunchKernelGGL((k_m2l_gemm<9, F32>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_m2l_gemm<10, F32>), g, b, 0, s, a); break;
     }
+}
+
+void launch_m2l_gemm(const M2LGemmArgs& a, int grid, hipStream_t s) {
+    if (grid <= 0) return;
+    if (a.f32) launch_gemm_p<true>(a, grid, s);
+    else launch_gemm_p<false>(a, grid, s);
     FMMH_HIP(hipGetLastError());
 }
 
-void launch_m2l_reduce(const double* pbuf, const int* off, const double* r, const int* tdeg, double* LE, int nm,
-                       int num_nodes, hipStream_t s) {
+void launch_m2l_reduce(const M2LGemmArgs& a, const int* off, const double* r, double* LE, int num_nodes,
+                       hipStream_t s) {
     if (num_nodes <= 0) return;
-    hipLaunchKernelGGL(k_m2l_reduce, dim3(num_nodes), dim3(128), 0, s, pbuf, off, r, tdeg, LE, nm);
+    const int nm = ncomp(a.p);
+    if (a.f32)
+        hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(num_nodes), dim3(128), 0, s, a.pbuf32, off, r, a.tdeg, LE, nm);
+    else
+        hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(num_nodes), dim3(128), 0, s, a.pbuf, off, r, a.tdeg, LE, nm);
+    FMMH_HIP(hipGetLastError());
+}
+
+// FP32 copy of one set's scaled source expansions (mixed precision).
+__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+void launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, n);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -237,6 +237,11 @@ class Context:
         """Scale of the initial interaction-list buffers (they grow on overflow)."""
         check(self._lib.fmmh_set_option(self._h, 2, float(scale)))
 
+    def set_precision(self, mixed=False):
+        """False: FP64 throughout (default).  True: M2L translations on FP32 MFMA (FP64 near field, P2M,
+        L2P and accumulation)."""
+        check(self._lib.fmmh_set_option(self._h, 5, 1.0 if mixed else 0.0))
+
     def set_tree_reuse(self, on=True):
         """Reuse the octree, lists and M2L classes while coordinates, ncrit and theta are unchanged."""
         check(self._lib.fmmh_set_option(self._h, 4, 1.0 if on else 0.0))

@@ -194,12 +194,17 @@ enum fmmh_option {
                                      1 (default): per-target FP64 MFMA over sets from 16 sets on (orders
                                      <= 8), else the class-grouped FP64 MFMA GEMM (orders <= 10);
                                      2: per-target MFMA for any set count; 3: class-grouped always */
-    FMMH_OPT_REUSE_TREE = 4       /* value != 0: keep the last field_fmm octree, interaction lists and M2L
+    FMMH_OPT_REUSE_TREE = 4,      /* value != 0: keep the last field_fmm octree, interaction lists and M2L
                                      class tensors, and reuse them while the coordinates (compared bitwise
                                      on the device), n, ncrit and theta are unchanged -- e.g. the induced-
                                      dipole iterations of a polarizable embedding, for which the reference
                                      rebuilds every call (src/fmm.F90:1387-1398).  Results are identical
                                      to a rebuild.  0 (default) drops the cache. */
+    FMMH_OPT_PRECISION = 5        /* 0 (default): FP64 throughout.  1: mixed -- the multipole-to-local
+                                     translations of the class-grouped M2L run on FP32 MFMA with FP32
+                                     accumulation and an FP64 sum into the local expansions; P2M, L2P,
+                                     the near field (with its damped tensors) and the field stay FP64.
+                                     Results then differ from the reference at the FP32 level. */
 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
