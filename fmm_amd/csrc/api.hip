@@ -87,6 +87,11 @@ struct fmmh_context {
     bool cache_valid = false;
     int cache_n = 0, cache_ncrit = 0;
     double cache_theta = 0.0;
+    bool cache_periodic = false;
+    bool shells_valid = false;        // periodic shell tensor sums in "p_shells" for (shells_p, shells_box)
+    int shells_p = -1;
+    double shells_box[9] = {};
+    double cache_box[9] = {};
     TreeOut cache_tr;
     ListsOut cache_L;
     int cache_class_p = -1;           // expansion order of the cached class tensors (-1: none)
@@ -345,7 +350,7 @@ __global__ void k_pair_count(const int* off, const int* src, const int* leaf_nod
     if (l >= nl) return;
     int node = leaf_node[l];
     unsigned long long s = 0;
-    for (int e = off[node]; e < off[node + 1]; ++e) s += (unsigned long long)count[src[e]];
+    for (int e = off[node]; e < off[node + 1]; ++e) s += (unsigned long long)count[src_node(src[e])];
     atomicAdd(total, s * (unsigned long long)count[node]);
 }
 
@@ -355,6 +360,43 @@ __global__ void k_fill_i32(int* a, int v, int n) {
 }
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// Periodic images (src/fmm.F90:156-175): image id 0 is the central cell, ids
+// 1..26 the neighbours in the reference's (ai, aj, ak) loop order; the
+// displacement is matmul(box_matrix, aijk) summed column by column.
+struct Periodic {
+    bool on = false;
+    int nimg = 1;
+    double box[9] = {};                  // column-major 3 x 3
+    double disp[NUM_IMAGES * 3] = {};
+};
+
+Periodic make_periodic(bool on, const double* box) {
+    Periodic P;
+    if (!on) return P;
+    P.on = true;
+    P.nimg = NUM_IMAGES;
+    std::memcpy(P.box, box, sizeof(P.box));
+    int id = 1;
+    for (int ai = -1; ai <= 1; ++ai)
+        for (int aj = -1; aj <= 1; ++aj)
+            for (int ak = -1; ak <= 1; ++ak) {
+                if (ai == 0 && aj == 0 && ak == 0) continue;
+                for (int r = 0; r < 3; ++r)
+                    P.disp[3 * id + r] = (box[r] * ai + box[r + 3] * aj) + box[r + 6] * ak;
+                ++id;
+            }
+    return P;
+}
+
+// Device copy of the image displacements (one slot per image id).
+const double* upload_disp(fmmh_context* c, const Periodic& P) {
+    double* d = c->buf<double>("p_disp", NUM_IMAGES * 3);
+    double* h = reinterpret_cast<double*>(c->h_pinned + 512);
+    std::memcpy(h, P.disp, sizeof(P.disp));
+    FMMH_HIP(hipMemcpyAsync(d, h, sizeof(P.disp), hipMemcpyHostToDevice, c->stream));
+    return d;
+}
 constexpr int TRAVERSE_GRID = 1024;
 constexpr double M2L_GEMM_MAX_BYTES = 48e9;   // per-pair slot buffer of the class-grouped M2L   // workgroups per traversal step (grid-stride over the frontier)
 
@@ -426,7 +468,7 @@ void run_direct(fmmh_context* c, const Call& k) {
     launch_gather(g, s);
     c->mark("gather");
     NearArgs a{iota, start, count, loff, lsrc, s_start, s_count, xs, ys, zs, ms, n, dfs, iota, excl, n, k.nexcl,
-               k.field, k.ldf, 1, k.nsets};
+               k.field, k.ldf, 1, k.nsets, upload_disp(c, Periodic{})};
     launch_near(a, nb, m, k.F, k.damp, s);
     c->mark("direct");
     FMMH_HIP(hipStreamSynchronize(s));
@@ -492,7 +534,7 @@ void run_direct_targets(fmmh_context* c, const Call& k, const int* targets, int 
                xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale, 1};
     launch_gather(g, s);
     NearArgs a{iota, start, count, loff, lsrc, s_start, s_count, xs, ys, zs, ms, ntot, dfs, perm, excl, ntot,
-               k.nexcl, out, ntot, 0, 1};
+               k.nexcl, out, ntot, 0, 1, upload_disp(c, Periodic{})};
     launch_near(a, nb, m, k.F, k.damp, s);
     hipLaunchKernelGGL(k_add_rows, dim3(nblk(nt, 256)), dim3(256), 0, s, out, (int64_t)ntot, n, k.field, k.ldf, nt,
                        nf);
@@ -581,7 +623,7 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     c->sync();
 }
 
-void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
+void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int nimg, const double* disp) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes;
     const double sc = c->list_cap_scale;
@@ -596,12 +638,14 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
         unsigned long long* m2l = c->buf<unsigned long long>("l_m2l", cap_m);
         unsigned long long* p2p = c->buf<unsigned long long>("l_p2p", cap_p);
         int* counters = c->buf<int>("l_counters", 8);   // n_a, n_b, n_m2l, n_p2p, overflow
-        int2 root = make_int2(0, 0);
-        int init[5] = {1, 0, 0, 0, 0};
+        int init[5] = {nimg, 0, 0, 0, 0};
         std::memcpy(hp + 100, init, sizeof(init));
         FMMH_HIP(hipMemcpyAsync(counters, hp + 100, 5 * sizeof(int), hipMemcpyHostToDevice, s));
-        std::memcpy(hp + 110, &root, sizeof(root));
-        FMMH_HIP(hipMemcpyAsync(fa, hp + 110, sizeof(int2), hipMemcpyHostToDevice, s));
+        for (int k = 0; k < nimg; ++k) {   // one (root, root) seed per image
+            int2 root = make_int2(0, src_encode(k, 0));
+            std::memcpy(hp + 110 + 2 * k, &root, sizeof(root));
+        }
+        FMMH_HIP(hipMemcpyAsync(fa, hp + 110, nimg * sizeof(int2), hipMemcpyHostToDevice, s));
         // Every level of the traversal deepens one side of each surviving
         // pair, so level_i + level_j grows by one per step: 2 max_depth steps
         // always drain the frontier.  The frontier size stays on the device
@@ -615,7 +659,8 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
             int* cout = counters + (1 - cur);
             FMMH_HIP(hipMemsetAsync(cout, 0, sizeof(int), s));
             TraverseArgs a{fin, cin, cap_f, fout, cout, cap_f, m2l, counters + 2, cap_m, p2p, counters + 3, cap_p,
-                           counters + 4, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf, tr.t.children, theta};
+                           counters + 4, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf, tr.t.children, theta,
+                           disp};
             launch_traverse(a, TRAVERSE_GRID, s);
             cur = 1 - cur;
         }
@@ -656,7 +701,7 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L) {
 // target-centric Hankel contraction of every set.  One host sync to size the
 // class table.  Many sets go through the MFMA form in chunks of up to 32 sets.
 void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets, const double* CM, double* LE,
-             bool classes_cached) {
+             bool classes_cached, const Periodic& P, const double* disp) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes, np = L.n_m2l;
     const int nm = ncomp(p), nT = ncomp(2 * p);
@@ -687,10 +732,30 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         ca.p = p;
         ca.xyz = tb.xyz;
         ca.level_base = tb.level_base;
+        ca.class_dl = c->buf<int>("m_class_dl", np);
+        ca.periodic = P.on ? 1 : 0;
+        ca.disp = disp;
+        if (P.on) {
+            ca.class_hi_in = c->buf<unsigned long long>("m_hi_in", np);
+            ca.class_hi_sorted = c->buf<unsigned long long>("m_hi_sorted", np);
+            ca.class_lo_final = c->buf<unsigned long long>("m_lo_final", np);
+            ca.class_key_hi = c->buf<unsigned long long>("m_class_key_hi", np);
+        }
         launch_m2l_keys(ca, s);
         size_t tbytes = std::max(sort_pairs_u64_temp(np), exscan_i32_temp(np));
         void* temp = c->buf<char>("m_temp", tbytes);
-        sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, ca.pair_idx_sorted, np, 62, s);
+        if (!P.on) {
+            sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, ca.pair_idx_sorted, np, 62,
+                           s);
+        } else {
+            // (hi, lo) lexicographic order by two stable radix sorts: lo first, then hi
+            int* idx1 = c->buf<int>("m_idx1", np);
+            unsigned long long* hi1 = c->buf<unsigned long long>("m_hi1", np);
+            sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, idx1, np, 63, s);
+            launch_gather_u64(ca.class_hi_in, idx1, hi1, np, s);
+            sort_pairs_u64(temp, tbytes, hi1, ca.class_hi_sorted, idx1, ca.pair_idx_sorted, np, 21, s);
+            launch_gather_u64(ca.class_key_in, ca.pair_idx_sorted, ca.class_lo_final, np, s);
+        }
         launch_m2l_flags(ca, s);
         exscan_i32(temp, tbytes, ca.flag, ca.cls, np, s);
         launch_m2l_assign(ca, s);
@@ -716,7 +781,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         const int* flag = c->buf<int>("m_flag", np);
         const int* cls = c->buf<int>("m_cls", np);
         const int* pair_csr = c->buf<int>("m_idx_sorted", np);
-        const unsigned long long* class_key = c->buf<unsigned long long>("m_class_key", np);
+        const int* class_dl = c->buf<int>("m_class_dl", np);
         int* class_off = c->buf<int>("m_class_off", ncls + 1);
         int* tile_cnt = c->buf<int>("m_tile_cnt", ncls + 1);
         int* tile_off = c->buf<int>("m_tile_off", ncls + 1);
@@ -732,7 +797,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
         for (int set = 0; set < nsets; ++set) {
             if (f32) launch_to_f32(CMs + set * node_stride, cms32, (int64_t)node_stride, s);
-            M2LGemmArgs ga{class_off, tile_off, ncls, pair_csr, L.m2l_src, class_key, Tc, CMs + set * node_stride,
+            M2LGemmArgs ga{class_off, tile_off, ncls, pair_csr, L.m2l_src, class_dl, Tc, CMs + set * node_stride,
                            tb.gemm_fidx, tb.gemm_tdeg, pbuf, p, f32 ? 1 : 0, cms32, pbuf32};
             launch_m2l_gemm(ga, grid, s);
             launch_m2l_reduce(ga, L.m2l_off, tr.t.r, LE + set * node_stride, nn, s);
@@ -752,6 +817,40 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     }
 }
 
+// ---------------------------------------------------------------- periodic
+// Lattice sum into the root local expansions and the dipole correction into
+// the field (src/fmm.F90:1392-1395), after M2L and before L2L.  The shell
+// tensor sums depend only on the box and p and are kept between calls.
+void run_periodic(fmmh_context* c, TreeOut& tr, Tables& tb, const Periodic& P, const Call& k, int p, const double* CM,
+                  double* LE) {
+    hipStream_t s = c->stream;
+    const int nT = ncomp(2 * p);
+    double* boxd = c->buf<double>("p_box", 9);
+    double* hbox = reinterpret_cast<double*>(c->h_pinned + 700);
+    std::memcpy(hbox, P.box, sizeof(P.box));
+    FMMH_HIP(hipMemcpyAsync(boxd, hbox, sizeof(P.box), hipMemcpyHostToDevice, s));
+    double* shells = c->buf<double>("p_shells", (size_t)PSUM_LEVELS * 26 * nT);
+    // volume exactly as src/fmm.F90:961-963 forms it (including its h(1,2) in the last cofactor)
+    auto H = [&](int r, int col) { return P.box[(r - 1) + 3 * (col - 1)]; };
+    const double volume = H(1, 1) * (H(2, 2) * H(3, 3) - H(2, 3) * H(3, 2)) -
+                          H(1, 2) * (H(2, 1) * H(3, 3) - H(2, 3) * H(3, 1)) +
+                          H(1, 3) * (H(1, 2) * H(3, 2) - H(2, 2) * H(3, 1));
+    const double pi = 3.14159265359;   // the reference's literal
+    PeriodicArgs pa{boxd, shells, p, tb.xyz, tb.level_base, CM, LE, tr.num_nodes, k.nsets, tb.cs, tb.m2m,
+                    tb.mono_parent, tb.mono_axis, k.X, k.ldx, k.n, k.field, k.ldf, ncomp(k.F), 4.0 * pi, 2.0 * pi,
+                    3.0 * volume, c->buf<int>("p_levels", k.nsets)};
+    if (!(c->shells_valid && c->shells_p == p && std::memcmp(c->shells_box, P.box, sizeof(P.box)) == 0)) {
+        c->shells_valid = false;
+        launch_periodic_shells(pa, s);
+        c->shells_valid = true;
+        c->shells_p = p;
+        std::memcpy(c->shells_box, P.box, sizeof(P.box));
+    }
+    launch_periodic_sum(pa, s);
+    launch_dipole_correction(pa, s);
+    c->mark("periodic");
+}
+
 // ---------------------------------------------------------------- field_fmm
 __global__ void k_coords_differ(const double* __restrict__ X, int64_t ldx, const double* __restrict__ C, int n,
                                 int* flag) {
@@ -765,8 +864,9 @@ __global__ void k_coords_differ(const double* __restrict__ X, int64_t ldx, const
 
 // True when the cached tree was built from bitwise the same coordinates with
 // the same n, ncrit and theta (one device compare, one host sync).
-bool cached_tree_matches(fmmh_context* c, const Call& k, double theta, int ncrit) {
+bool cached_tree_matches(fmmh_context* c, const Call& k, double theta, int ncrit, const Periodic& P) {
     if (!c->cache_valid || c->cache_n != k.n || c->cache_ncrit != ncrit || c->cache_theta != theta) return false;
+    if (c->cache_periodic != P.on || (P.on && std::memcmp(c->cache_box, P.box, sizeof(P.box)) != 0)) return false;
     hipStream_t s = c->stream;
     int* flag = c->buf<int>("c_flag", 1);
     FMMH_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
@@ -778,15 +878,16 @@ bool cached_tree_matches(fmmh_context* c, const Call& k, double theta, int ncrit
     return c->h_pinned[400] == 0;
 }
 
-void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
+void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, const Periodic& P) {
     hipStream_t s = c->stream;
+    const double* disp = upload_disp(c, P);
     const int n = k.n;
     const int m = order_from_dim(k.dim);
     const int nf = ncomp(k.F);
     const int nm = ncomp(p);
     TreeOut tr;
     ListsOut L;
-    const bool reuse = c->reuse_tree && cached_tree_matches(c, k, theta, ncrit);
+    const bool reuse = c->reuse_tree && cached_tree_matches(c, k, theta, ncrit, P);
     if (reuse) {
         tr = c->cache_tr;
         L = c->cache_L;
@@ -796,7 +897,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
         c->cache_valid = false;
         build_tree(c, k.X, k.ldx, n, ncrit, tr);
         c->mark("tree");
-        build_lists(c, tr, theta, L);
+        build_lists(c, tr, theta, L, P.nimg, disp);
         c->mark("lists");
     }
     const int nn = tr.num_nodes, nl = tr.num_leaves;
@@ -828,12 +929,13 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
     c->mark("upward");
     double* near = c->buf<double>("e_near", (size_t)n * nf * S);
     NearArgs na{tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
-                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S};
+                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp};
     launch_near(na, nl, m, k.F, k.damp, s);
     c->mark("p2p");
     const bool classes_cached = reuse && c->cache_class_p == p;
-    run_m2l(c, tr, L, tb, p, S, CM, LE, classes_cached);
+    run_m2l(c, tr, L, tb, p, S, CM, LE, classes_cached, P, disp);
     c->mark("m2l");
+    if (P.on) run_periodic(c, tr, tb, P, k, p, CM, LE);
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
                 tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S};
     for (int lev = 2; lev <= tr.max_depth; ++lev) {
@@ -866,6 +968,8 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p) {
             c->cache_tr = tr;
             c->cache_L = L;
             c->cache_n = n; c->cache_ncrit = ncrit; c->cache_theta = theta;
+            c->cache_periodic = P.on;
+            std::memcpy(c->cache_box, P.box, sizeof(P.box));
             c->cache_pairs = pairs;
             c->cache_retries = c->list_retries;
             c->cache_valid = true;
@@ -927,8 +1031,11 @@ Call make_call(const double* X, int64_t ldx, const double* M, int64_t ldm, int n
     return Call{X, ldx, M, ldm, n, dim, E, lde, nexcl, F, field, ldf, damp, df, nsets};
 }
 
-void check_fmm_args(double theta, int ncrit, int p, int is_periodic) {
-    if (is_periodic) throw Error(FMMH_ERR_UNSUPPORTED, "periodic FMM is not supported by this engine");
+void check_fmm_args(double theta, int ncrit, int p, int is_periodic, const double* box_matrix) {
+    if (is_periodic && box_matrix == nullptr) throw Error(FMMH_ERR_ARG, "periodic FMM needs box_matrix");
+    if (is_periodic && p < 2)
+        throw Error(FMMH_ERR_UNSUPPORTED, "periodic FMM needs expansion_order >= 2 (the dipole correction reads "
+                                          "the quadrupole trace of the root expansion)");
     if (!(theta > 0.0)) throw Error(FMMH_ERR_ARG, "theta must be > 0");
     if (ncrit < 2 || ncrit > 4096) throw Error(FMMH_ERR_ARG, "ncrit must be in 2..4096");
     if (p < 0 || p > PMAX) throw Error(FMMH_ERR_UNSUPPORTED, "expansion_order must be in 0.." + std::to_string(PMAX));
@@ -1006,7 +1113,7 @@ static int field_fmm_impl(fmmh_context* ctx, bool host, const double* coordinate
                           const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl, double theta,
                           int32_t ncrit, int32_t expansion_order, int32_t field_order, double* field,
                           int64_t ld_field, int32_t damp_type, const double* damping_factors, int64_t n_damping,
-                          int32_t is_periodic) {
+                          int32_t is_periodic, const double* box_matrix) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
         Call h = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
@@ -1015,7 +1122,7 @@ static int field_fmm_impl(fmmh_context* ctx, bool host, const double* coordinate
         if (damp_type < 0 || damp_type > 3) throw Error(FMMH_ERR_DAMP, "Unknown damp type code");
         if (is_periodic && n <= ncrit)
             throw Error(FMMH_ERR_UNSUPPORTED, "Not enough coordinates for periodic FMM, please decrease ncrit");
-        check_fmm_args(theta, ncrit, expansion_order, is_periodic);
+        check_fmm_args(theta, ncrit, expansion_order, is_periodic, box_matrix);
         check_common(h, n_damping);
         if (host) check_exclusions_host(h);
         c->begin_timing();
@@ -1027,7 +1134,7 @@ static int field_fmm_impl(fmmh_context* ctx, bool host, const double* coordinate
             c->mark("h2d");
         }
         if (n <= ncrit) run_direct(c, k);
-        else run_fmm(c, k, theta, ncrit, expansion_order);
+        else run_fmm(c, k, theta, ncrit, expansion_order, make_periodic(is_periodic != 0, box_matrix));
         if (host) {
             unstage_field(c, st, field, ld_field);
             c->mark("d2h");
@@ -1063,10 +1170,10 @@ int fmmh_field_fmm(fmmh_context* ctx, int32_t comm, const double* coordinates, i
                    int64_t ld_exclusions, int32_t nexcl, double theta, int32_t ncrit, int32_t expansion_order,
                    int32_t field_order, double* field, int64_t ld_field, int32_t damp_type,
                    const double* damping_factors, int64_t n_damping, int32_t is_periodic, const double* box_matrix) {
-    (void)comm; (void)box_matrix;
+    (void)comm;
     return field_fmm_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, 1, exclusions,
                           ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field, ld_field,
-                          damp_type, damping_factors, n_damping, is_periodic);
+                          damp_type, damping_factors, n_damping, is_periodic, box_matrix);
 }
 
 int fmmh_field_fmm_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
@@ -1075,10 +1182,10 @@ int fmmh_field_fmm_dev(fmmh_context* ctx, int32_t comm, const double* coordinate
                        int32_t expansion_order, int32_t field_order, double* field, int64_t ld_field,
                        int32_t damp_type, const double* damping_factors, int64_t n_damping, int32_t is_periodic,
                        const double* box_matrix) {
-    (void)comm; (void)box_matrix;
+    (void)comm;
     return field_fmm_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, 1, exclusions,
                           ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field, ld_field,
-                          damp_type, damping_factors, n_damping, is_periodic);
+                          damp_type, damping_factors, n_damping, is_periodic, box_matrix);
 }
 
 int fmmh_field_direct_sets(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
@@ -1109,10 +1216,10 @@ int fmmh_field_fmm_sets(fmmh_context* ctx, int32_t comm, const double* coordinat
                         int32_t expansion_order, int32_t field_order, double* field, int64_t ld_field,
                         int32_t damp_type, const double* damping_factors, int64_t n_damping, int32_t is_periodic,
                         const double* box_matrix) {
-    (void)comm; (void)box_matrix;
+    (void)comm;
     return field_fmm_impl(ctx, true, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, nsets,
                           exclusions, ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field,
-                          ld_field, damp_type, damping_factors, n_damping, is_periodic);
+                          ld_field, damp_type, damping_factors, n_damping, is_periodic, box_matrix);
 }
 
 int fmmh_field_fmm_sets_dev(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
@@ -1121,10 +1228,10 @@ int fmmh_field_fmm_sets_dev(fmmh_context* ctx, int32_t comm, const double* coord
                             int32_t ncrit, int32_t expansion_order, int32_t field_order, double* field,
                             int64_t ld_field, int32_t damp_type, const double* damping_factors, int64_t n_damping,
                             int32_t is_periodic, const double* box_matrix) {
-    (void)comm; (void)box_matrix;
+    (void)comm;
     return field_fmm_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, nsets,
                           exclusions, ld_exclusions, nexcl, theta, ncrit, expansion_order, field_order, field,
-                          ld_field, damp_type, damping_factors, n_damping, is_periodic);
+                          ld_field, damp_type, damping_factors, n_damping, is_periodic, box_matrix);
 }
 
 int fmmh_field_direct_targets_dev(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates,
@@ -1153,7 +1260,7 @@ int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coo
         fmmh_context* c = resolve(ctx);
         if (n < 1 || ld_coordinates < n) throw Error(FMMH_ERR_ARG, "bad n / leading dimension");
         if (n < ncrit) throw Error(FMMH_ERR_ARG, "the tree needs n >= ncrit");
-        check_fmm_args(theta, ncrit, 0, 0);
+        check_fmm_args(theta, ncrit, 0, 0, nullptr);
         double* X = c->buf<double>("h_X", (size_t)n * 3);
         FMMH_HIP(hipMemcpy2DAsync(X, n * sizeof(double), coordinates, ld_coordinates * sizeof(double),
                                   n * sizeof(double), 3, hipMemcpyHostToDevice, c->stream));
@@ -1161,7 +1268,7 @@ int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coo
         TreeOut tr;
         build_tree(c, X, n, n, ncrit, tr);
         ListsOut L;
-        build_lists(c, tr, theta, L);
+        build_lists(c, tr, theta, L, 1, upload_disp(c, Periodic{}));
         c->sync();
         c->num_nodes = tr.num_nodes; c->num_leaves = tr.num_leaves; c->max_depth = tr.max_depth;
         c->n_m2l = L.n_m2l; c->n_p2p = L.n_p2p; c->n_sites = n;

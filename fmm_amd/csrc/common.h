@@ -15,6 +15,18 @@ namespace fmmh {
 
 constexpr int KEY_LEVELS = 21;   // 3-bit octant digits in a 63-bit key
 constexpr int PMAX = 10;         // largest expansion order accepted
+// Interaction-list sources carry a periodic image id in their top bits:
+// src = image << IMG_SHIFT | node, image 0 = the central cell (the only one
+// without periodic boundaries), 1..26 = the neighbour cells.
+constexpr int IMG_SHIFT = 27;
+constexpr int NODE_MASK = (1 << IMG_SHIFT) - 1;
+constexpr int NUM_IMAGES = 27;
+// (the encoded id uses all 32 bits: decode it unsigned)
+__host__ __device__ __forceinline__ int src_node(int enc) { return enc & NODE_MASK; }
+__host__ __device__ __forceinline__ int src_image(int enc) { return (int)((unsigned)enc >> IMG_SHIFT); }
+__host__ __device__ __forceinline__ int src_encode(int img, int node) {
+    return (int)(((unsigned)img << IMG_SHIFT) | (unsigned)node);
+}
 constexpr int KMAX_NEAR = 6;     // largest near-field tensor order (damped tensors stop at 6)
 
 enum Damp { DAMP_NONE = 0, DAMP_THOLE = 1, DAMP_AMOEBA = 2, DAMP_ERF = 3 };

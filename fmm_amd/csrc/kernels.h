@@ -30,6 +30,7 @@ struct NearArgs {
     int64_t ldo;
     int accumulate;            // 0: out = result, 1: out += result
     int nsets;                 // multipole sets: ms is [set][component][ldm], out is [set][component][ldo]
+    const double* disp;        // per image id: source displacement (x, y, z); image 0 = (0, 0, 0)
 };
 
 bool near_supported(int m, int F);
@@ -105,6 +106,7 @@ struct TraverseArgs {
     const double* cx; const double* cy; const double* cz; const double* rmax;
     const int* is_leaf; const int* children;
     double theta;
+    const double* disp;        // per image id: displacement of the source cell; frontier sources carry the id
 };
 void launch_traverse(const TraverseArgs& a, int grid, hipStream_t s);
 void launch_csr(const unsigned long long* keys, int n_keys, int num_targets, int* off, int* src, hipStream_t s);
@@ -164,11 +166,22 @@ struct M2LClassArgs {
     int p;
     const int* xyz;                         // packed index -> t | u << 8 | v << 16
     const int* level_base;                  // aux slot base per degree (2p + 2 entries)
+    int* class_dl;                          // class id -> level_i - level_j
+    // Periodic runs: a class is (v, level difference, image, and for image
+    // classes the target level), keyed by two words sorted as (hi, lo);
+    // the image displacement D enters the class tensor as T(v - D / u).
+    int periodic;
+    unsigned long long* class_hi_in;        // per pair: image | level | level difference
+    unsigned long long* class_hi_sorted;    // hi words in final sorted order
+    unsigned long long* class_lo_final;     // lo words in final sorted order
+    unsigned long long* class_key_hi;       // class id -> hi word
+    const double* disp;                     // per image id: displacement
 };
 void launch_m2l_keys(const M2LClassArgs& a, hipStream_t s);
 void launch_m2l_flags(const M2LClassArgs& a, hipStream_t s);        // after the key sort
 void launch_m2l_assign(const M2LClassArgs& a, hipStream_t s);       // after the flag scan
 void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s);
+void launch_gather_u64(const unsigned long long* in, const int* idx, unsigned long long* out, int n, hipStream_t s);
 
 // Expansions of S multipole sets are stored set-major, CM[(set * nodes + node)
 // * nm + s] and likewise LE.  The scaled sources CMs that M2L reads are
@@ -208,7 +221,7 @@ struct M2LGemmArgs {
     int ncls;
     const int* pair_csr;                 // class-sorted pair -> CSR position
     const int* src;                      // CSR position -> source node
-    const unsigned long long* class_key; // class -> packed (v, level difference)
+    const int* class_dl;                 // class -> level_i - level_j
     const double* Tc;                    // class tensors, ncomp(2p) each
     const double* CMs;                   // scaled source expansions of one set, [node][nm]
     const int* fidx;                     // [MT][KS][64]: packed index of s + t per A-fragment lane, -1 outside
@@ -246,6 +259,32 @@ struct DownArgs {
 };
 void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s);
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s);
+
+// ---------------------------------------------------------------- periodic boundaries
+// Lattice sum beyond the 27 cells (src/fmm.F90:884-944): at shell level l
+// the 26 x 27 cells of size 3^{l-1} boxes around the central 3^l supercell
+// act on the root's local expansion with the level-l supercell multipole
+// (charge removed), l = 1..PSUM_LEVELS until the contribution's norm drops
+// below 1e-8.  The cell tensors depend only on the box and p, so their sums
+// per level are formed once (k_periodic_shells) and the level loop runs in
+// one workgroup per set.  Then the dipole correction (src/fmm.F90:946-979).
+constexpr int PSUM_LEVELS = 50;
+struct PeriodicArgs {
+    const double* box;        // 3 x 3 column-major (device)
+    double* shells;           // [PSUM_LEVELS][26][ncomp(2p)] per-neighbour shell tensor sums
+    int p;
+    const int* xyz; const int* level_base;
+    const double* CM; double* LE; int num_nodes; int nsets;   // [set][node][nm], root = node 0
+    const double* cs;         // Taylor factors c_s of the expansion components
+    ShiftTab m2m; const int* mono_parent; const int* mono_axis;
+    const double* X; int64_t ldx; int n;                       // caller-order coordinates
+    double* field; int64_t ldf; int nf;                         // [set][component][ldf]
+    double fourpi, twopi, threeV;
+    int* levels_used;         // per set: shell levels summed
+};
+void launch_periodic_shells(const PeriodicArgs& a, hipStream_t s);
+void launch_periodic_sum(const PeriodicArgs& a, hipStream_t s);
+void launch_dipole_correction(const PeriodicArgs& a, hipStream_t s);
 
 // ---------------------------------------------------------------- primitives (rocPRIM)
 size_t sort_pairs_u64_temp(int n);

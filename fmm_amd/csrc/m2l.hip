@@ -21,36 +21,55 @@
 // tensor by ~1e-16 relative, far inside the 1e-12 parity bar.
 #include "common.h"
 #include "kernels.h"
+#include "hermite.h"
 
 namespace fmmh {
 
 constexpr int VBITS = 18;
 constexpr long long VOFF = 1ll << (VBITS - 1);
 constexpr int CLASS_KEY_BITS = 3 * VBITS + 8;
+// periodic lo word: three 21-bit components (image pairs can span the box)
+constexpr int PBITS = 21;
+constexpr long long POFF = 1ll << (PBITS - 1);
 
 __global__ __launch_bounds__(256) void k_m2l_keys(M2LClassArgs a) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.n_pairs) return;
     const unsigned long long pk = a.pair_keys[e];
-    const int i = (int)(pk >> 32), j = (int)(pk & 0xffffffffull);
+    const int i = (int)(pk >> 32);
+    const int jenc = (int)(pk & 0xffffffffull);
+    const int j = src_node(jenc), img = src_image(jenc);
     const double u = fmin(a.r[i], a.r[j]);
     const long long vx = llrint((a.cx[i] - a.cx[j]) / u);
     const long long vy = llrint((a.cy[i] - a.cy[j]) / u);
     const long long vz = llrint((a.cz[i] - a.cz[j]) / u);
     const int dl = a.level[i] - a.level[j];
-    if (llabs(vx) >= VOFF || llabs(vy) >= VOFF || llabs(vz) >= VOFF || dl < -100 || dl > 100)
+    const long long vlim = a.periodic ? POFF : VOFF;
+    if (llabs(vx) >= vlim || llabs(vy) >= vlim || llabs(vz) >= vlim || dl < -100 || dl > 100)
         atomicOr(a.error, 1);
-    const unsigned long long key = ((unsigned long long)(vx + VOFF) << (2 * VBITS + 8)) |
-                                   ((unsigned long long)(vy + VOFF) << (VBITS + 8)) |
-                                   ((unsigned long long)(vz + VOFF) << 8) | (unsigned long long)(dl + 128);
-    a.class_key_in[e] = key;
+    if (a.periodic) {
+        a.class_key_in[e] = ((unsigned long long)(vx + POFF) << (2 * PBITS)) |
+                            ((unsigned long long)(vy + POFF) << PBITS) | (unsigned long long)(vz + POFF);
+        // image classes also depend on the box size u, i.e. on the target level
+        const int lev = img == 0 ? 0 : a.level[i];
+        a.class_hi_in[e] = ((unsigned long long)img << 16) | ((unsigned long long)lev << 8) |
+                           (unsigned long long)(dl + 128);
+    } else {
+        a.class_key_in[e] = ((unsigned long long)(vx + VOFF) << (2 * VBITS + 8)) |
+                            ((unsigned long long)(vy + VOFF) << (VBITS + 8)) |
+                            ((unsigned long long)(vz + VOFF) << 8) | (unsigned long long)(dl + 128);
+    }
     a.pair_idx[e] = e;
 }
 
 __global__ __launch_bounds__(256) void k_m2l_flags(M2LClassArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n_pairs) return;
-    a.flag[k] = (k == 0 || a.class_key_sorted[k] != a.class_key_sorted[k - 1]) ? 1 : 0;
+    if (a.periodic)
+        a.flag[k] = (k == 0 || a.class_lo_final[k] != a.class_lo_final[k - 1] ||
+                     a.class_hi_sorted[k] != a.class_hi_sorted[k - 1]) ? 1 : 0;
+    else
+        a.flag[k] = (k == 0 || a.class_key_sorted[k] != a.class_key_sorted[k - 1]) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_m2l_assign(M2LClassArgs a) {
@@ -60,12 +79,20 @@ __global__ __launch_bounds__(256) void k_m2l_assign(M2LClassArgs a) {
     // number of starts up to and including k, minus one
     const int c = a.cls[k] + a.flag[k] - 1;
     a.pair_class[a.pair_idx_sorted[k]] = c;
-    if (a.flag[k]) a.class_key[c] = a.class_key_sorted[k];
+    if (a.flag[k]) {
+        if (a.periodic) {
+            a.class_key[c] = a.class_lo_final[k];
+            a.class_key_hi[c] = a.class_hi_sorted[k];
+            a.class_dl[c] = (int)(a.class_hi_sorted[k] & 255ull) - 128;
+        } else {
+            a.class_key[c] = a.class_key_sorted[k];
+            a.class_dl[c] = (int)(a.class_key_sorted[k] & 255ull) - 128;
+        }
+    }
     if (k == a.n_pairs - 1) *a.num_classes = c + 1;
 }
 
-constexpr int CT_KMAX = 2 * PMAX;
-constexpr int CT_AUX = (CT_KMAX + 1) * (CT_KMAX + 2) * (CT_KMAX + 3) * (CT_KMAX + 4) / 24;
+constexpr int CT_AUX = hermite_aux_size(HERMITE_KMAX);
 
 // One class tensor per workgroup iteration: T_k(v) for |k| <= 2p by the
 // Hermite recursion over degree levels (one barrier per level), scaled by
@@ -77,47 +104,32 @@ __global__ __launch_bounds__(64) void k_class_tensor(M2LClassArgs a) {
     const int ncls = *a.num_classes;
     for (int c = blockIdx.x; c < ncls; c += gridDim.x) {
         const unsigned long long key = a.class_key[c];
-        const double vx = (double)((long long)((key >> (2 * VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
-        const double vy = (double)((long long)((key >> (VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
-        const double vz = (double)((long long)((key >> 8) & ((1ull << VBITS) - 1)) - VOFF);
-        const int dl = (int)(key & 255ull) - 128;
-        if ((int)threadIdx.x <= K) {
-            const double R2 = vx * vx + vy * vy + vz * vz;
-            const double inv = 1.0 / sqrt(R2);
-            const double inv2 = inv * inv;
-            double g = inv;
-            for (int n = 1; n <= (int)threadIdx.x; ++n) g *= -(2.0 * n - 1.0) * inv2;
-            A[threadIdx.x] = g;
-        }
-        __syncthreads();
-        for (int k = 1; k <= K; ++k) {
-            const int nd = (k + 1) * (k + 2) / 2, ndm1 = k * (k + 1) / 2, ndm2 = (k - 1) * k / 2;
-            const int nent = nd * (K - k + 1);
-            const int base = a.level_base[k], bm1 = a.level_base[k - 1], bm2 = k >= 2 ? a.level_base[k - 2] : 0;
-            const int c0 = ncomp(k - 1), c1 = ncomp(k - 2), c2 = ncomp(k - 3);
-            for (int x = threadIdx.x; x < nent; x += 64) {
-                const int n = x / nd, l = x - n * nd;
-                const int pk = a.xyz[c0 + l];
-                const int t = pk & 255, u = (pk >> 8) & 255, v = pk >> 16;
-                double val;
-                if (t > 0) {
-                    val = vx * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t - 1, u, v) - c1)];
-                    if (t > 1) val = __builtin_fma((double)(t - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t - 2, u, v) - c2)], val);
-                } else if (u > 0) {
-                    val = vy * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u - 1, v) - c1)];
-                    if (u > 1) val = __builtin_fma((double)(u - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u - 2, v) - c2)], val);
-                } else {
-                    val = vz * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t, u, v - 1) - c1)];
-                    if (v > 1) val = __builtin_fma((double)(v - 1), A[bm2 + (n + 1) * ndm2 + (xyz2idx(t, u, v - 2) - c2)], val);
-                }
-                A[base + n * nd + l] = val;
+        double vx, vy, vz;
+        const int dl = a.class_dl[c];
+        if (a.periodic) {
+            vx = (double)((long long)((key >> (2 * PBITS)) & ((1ull << PBITS) - 1)) - POFF);
+            vy = (double)((long long)((key >> PBITS) & ((1ull << PBITS) - 1)) - POFF);
+            vz = (double)((long long)(key & ((1ull << PBITS) - 1)) - POFF);
+            const unsigned long long hi = a.class_key_hi[c];
+            const int img = (int)(hi >> 16), lev = (int)((hi >> 8) & 255ull);
+            if (img != 0) {
+                // c_i - (c_j + D) = u (v - D / u), u the half-width at the deeper of the two levels
+                const int deep = lev + (dl < 0 ? -dl : 0);
+                const double u = ldexp(a.r[0], -(deep - 1));
+                vx -= a.disp[3 * img] / u;
+                vy -= a.disp[3 * img + 1] / u;
+                vz -= a.disp[3 * img + 2] / u;
             }
-            __syncthreads();
+        } else {
+            vx = (double)((long long)((key >> (2 * VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
+            vy = (double)((long long)((key >> (VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
+            vz = (double)((long long)((key >> 8) & ((1ull << VBITS) - 1)) - VOFF);
         }
+        hermite_aux(vx, vy, vz, K, A, a.xyz, a.level_base);
         for (int idx = threadIdx.x; idx < nT; idx += 64) {
             const int pk = a.xyz[idx];
             const int k = (pk & 255) + ((pk >> 8) & 255) + (pk >> 16);
-            double val = A[a.level_base[k] + (idx - ncomp(k - 1))];
+            double val = hermite_T(A, idx, a.xyz, a.level_base);
             if (dl > 0) val = ldexp(val, dl * k);
             a.Tc[(int64_t)c * nT + idx] = val;
         }
@@ -200,7 +212,7 @@ __global__ __launch_bounds__(128, 4) void k_m2l(M2LArgs a) {
 #pragma unroll
             for (int x = 0; x < NL; ++x) pre[x] = (tid + 128 * x < NT) ? Tn[tid + 128 * x] : 0.0;
         }
-        const int j = a.src[e];
+        const int j = src_node(a.src[e]);
         const int dl = li - a.level[j];
         const double* __restrict__ Mj = a.CMs + set_off + (int64_t)j * NM;
         const double* __restrict__ T = Tl[cur];
@@ -285,6 +297,19 @@ void launch_m2l_flags(const M2LClassArgs& a, hipStream_t s) {
 void launch_m2l_assign(const M2LClassArgs& a, hipStream_t s) {
     if (a.n_pairs <= 0) return;
     hipLaunchKernelGGL(k_m2l_assign, dim3(nblk(a.n_pairs, 256)), dim3(256), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_gather_u64(const unsigned long long* __restrict__ in,
+                                                    const int* __restrict__ idx, unsigned long long* __restrict__ out,
+                                                    int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = in[idx[k]];
+}
+
+void launch_gather_u64(const unsigned long long* in, const int* idx, unsigned long long* out, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_u64, dim3(nblk(n, 256)), dim3(256), 0, s, in, idx, out, n);
     FMMH_HIP(hipGetLastError());
 }
 
@@ -375,7 +400,7 @@ __global__ __launch_bounds__(64) void k_m2l_mfma(M2LArgs a) {
 #pragma unroll
             for (int x = 0; x < NL; ++x) pre[x] = (lane + 64 * x < NT) ? Tn[lane + 64 * x] : 0.0;
         }
-        const int j = a.src[e];
+        const int j = src_node(a.src[e]);
         const int dl = li - a.level[j];
         double rs[MT];
 #pragma unroll

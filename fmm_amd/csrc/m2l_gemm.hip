@@ -99,13 +99,13 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
     const int c = lo;
     const int k0 = a.class_off[c] + (b - a.tile_off[c]) * TN;
     const int cnt = min(TN, a.class_off[c + 1] - k0);
-    const int dl = (int)(a.class_key[c] & 255ull) - 128;
+    const int dl = a.class_dl[c];
     const int tid = threadIdx.x;
     for (int x = tid; x < NT; x += 256) Ts[x] = a.Tc[(int64_t)c * NT + x];
     if (tid < TN) {
         const int e = tid < cnt ? a.pair_csr[k0 + tid] : -1;
         ecsr[tid] = e;
-        jsrc[tid] = e >= 0 ? a.src[e] : -1;
+        jsrc[tid] = e >= 0 ? src_node(a.src[e]) : -1;
     }
     __syncthreads();
     const E* __restrict__ cms = F32 ? (const E*)a.CMs32 : (const E*)a.CMs;

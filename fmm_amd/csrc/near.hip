@@ -7,14 +7,17 @@
 //
 // Exclusions follow the reference exactly: an excluded pair is evaluated at
 // displacement (1e10, 1e10, 1e10) (src/fmm.F90:714-721, 802-806), which
-// leaves the -q/(sqrt(3) 1e10) potential artefact in place.
+// leaves the -q/(sqrt(3) 1e10) potential artefact in place.  Periodic image
+// sources are displaced as x_i - (x_j + D) and never excluded
+// (src/fmm.F90:711-721: exclusions apply to the central cell only).
 #include "common.h"
 #include "tensors.h"
 #include "kernels.h"
 
 namespace fmmh {
 
-constexpr int NEAR_EXCL_REG = 4;   // exclusions kept in registers; more are read from memory
+constexpr int NEAR_EXCL_REG = 4;   // exclusions kept in registers (unused ones hold -1); more are read from memory
+constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: matches no exclusion entry (0..n) or -1
 
 // Lane layout: a target block of nt <= 64 sites is replicated over
 // r = 64 / nt lane groups (group g = lane / nt, target lane % nt); group g
@@ -49,6 +52,7 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 
     __shared__ double red[NF][64];
     __shared__ int wcum[NEAR_WIN + 1];            // window: exclusive prefix of source counts
     __shared__ int wstart[NEAR_WIN];              // window: first sorted site of each entry
+    __shared__ int wimg[NEAR_WIN];                // window: periodic image id of each entry
     const int b = a.blk[blockIdx.x];
     const int set0 = blockIdx.y * S;
     const int ns = min(S, a.nsets - set0);   // sets of this row (the rest are zero padding)
@@ -78,9 +82,11 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 
             for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
         for (int w0 = e0; w0 < e1; w0 += NEAR_WIN) {
             const int wn = min(NEAR_WIN, e1 - w0);
-            int cnt = 0, st = 0;
+            int cnt = 0, st = 0, im = 0;
             if (lane < wn) {
-                const int sb = a.list_src[w0 + lane];
+                const int enc = a.list_src[w0 + lane];
+                const int sb = src_node(enc);
+                im = src_image(enc);
                 st = a.src_start[sb];
                 cnt = a.src_count[sb];
             }
@@ -94,6 +100,7 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 
             if (lane < wn) {
                 wcum[lane] = incl - cnt;
                 wstart[lane] = st;
+                wimg[lane] = im;
             }
             if (lane == wn - 1) wcum[wn] = incl;
             __syncthreads();
@@ -110,12 +117,14 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 
                     int e = cur;
                     while (wcum[e + 1] <= v) ++e;
                     const int j = wstart[e] + (v - wcum[e]);
+                    const int img = wimg[e];
+                    const double* D = a.disp + 3 * img;
                     pj = j;
-                    px = a.xs[j];
-                    py = a.ys[j];
-                    pz = a.zs[j];
+                    px = a.xs[j] + D[0];
+                    py = a.ys[j] + D[1];
+                    pz = a.zs[j] + D[2];
                     if constexpr (DAMP != DAMP_NONE) pdf = a.dfs[j];
-                    po = a.orig[j] + 1;
+                    po = img == 0 ? a.orig[j] + 1 : NEAR_IMAGE_SRC;   // exclusions: central cell only
                     if constexpr (PREF_M) {
 #pragma unroll
                         for (int u = 0; u < S; ++u)

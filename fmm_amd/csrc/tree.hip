@@ -321,7 +321,10 @@ __device__ __forceinline__ double norm2_3(double v0, double v1, double v2) {
 // One level of the dual traversal of src/fmm.F90:352-385 over a frontier of
 // (target, source) node pairs.  Outputs are appended in arbitrary order and
 // sorted afterwards; for a fixed target the reference's DFS appends sources
-// in ascending pre-order id, which the sort reproduces.
+// in ascending pre-order id, which the sort reproduces.  Periodic runs
+// traverse from one (root, root) seed per image (src/fmm.F90:337-340): the
+// source id carries the image, whose displacement enters the MAC as
+// c_i - (c_j + D) (src/fmm.F90:360-362).
 // Wave-aggregated append: every lane asks for `want` slots of `counter`;
 // one atomic per wavefront, lanes get consecutive ranges in lane order.
 __device__ __forceinline__ int wave_append(int* counter, int want) {
@@ -349,13 +352,14 @@ __global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
     for (int e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); e0 < n; e0 += stride) {
         const int e = e0 + (threadIdx.x & 63);
         int kind = -1;                 // 0 far (M2L), 1 leaf pair (P2P), 2 descend target, 3 descend source
-        int i = 0, j = 0, nchild = 0;
+        int i = 0, j = 0, img = 0, nchild = 0;
         if (e < n) {
             const int2 pr = a.in[e];
-            i = pr.x; j = pr.y;
-            const double d0 = a.cx[i] - (a.cx[j] + 0.0);
-            const double d1 = a.cy[i] - (a.cy[j] + 0.0);
-            const double d2 = a.cz[i] - (a.cz[j] + 0.0);
+            i = pr.x; j = src_node(pr.y); img = src_image(pr.y);
+            const double* D = a.disp + 3 * img;
+            const double d0 = a.cx[i] - (a.cx[j] + D[0]);
+            const double d1 = a.cy[i] - (a.cy[j] + D[1]);
+            const double d2 = a.cz[i] - (a.cz[j] + D[2]);
             const double r = norm2_3(d0, d1, d2);
             const double lhs = r * a.theta;
             const double rhs = a.rmax[i] + a.rmax[j];
@@ -369,7 +373,8 @@ __global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
                 for (int o = 0; o < 8; ++o) nchild += a.children[pnode * 8 + o] >= 0 ? 1 : 0;
             }
         }
-        const unsigned long long key = ((unsigned long long)i << 32) | (unsigned)j;
+        const int jenc = src_encode(img, j);
+        const unsigned long long key = ((unsigned long long)i << 32) | (unsigned)jenc;
         int pos = wave_append(a.n_m2l, kind == 0 ? 1 : 0);
         if (kind == 0) {
             if (pos < a.cap_m2l) a.m2l[pos] = key;
@@ -386,7 +391,8 @@ __global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
             for (int o = 0; o < 8; ++o) {
                 const int c = a.children[pnode * 8 + o];
                 if (c < 0) continue;
-                if (pos < a.cap_out) a.out[pos] = kind == 2 ? make_int2(c, j) : make_int2(i, c);
+                if (pos < a.cap_out)
+                    a.out[pos] = kind == 2 ? make_int2(c, jenc) : make_int2(i, src_encode(img, c));
                 else atomicOr(a.overflow, 4);
                 ++pos;
             }

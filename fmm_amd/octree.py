@@ -47,6 +47,20 @@ def _damp_code(damp_type):
     return code.value
 
 
+_BOXES = []
+
+
+def _periodic(is_periodic, box_matrix):
+    """(flag, host pointer) for the C ABI; the box stays alive for the call."""
+    if not is_periodic:
+        return 0, None
+    box = np.asfortranarray(box_matrix, dtype=np.float64)
+    if box.shape != (3, 3):
+        raise ValueError("box_matrix must be 3 x 3")
+    _BOXES[:] = [box]
+    return 1, _p(box)
+
+
 def _prep(coordinates, multipoles, exclusions, damping_factors):
     X = np.asfortranarray(coordinates, dtype=np.float64)
     M = np.asfortranarray(multipoles, dtype=np.float64)
@@ -173,7 +187,7 @@ class Context:
         return field
 
     def field_fmm_device(self, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order,
-                         field, damp_type="", damping_factors=None):
+                         field, damp_type="", damping_factors=None, is_periodic=False, box_matrix=None):
         n, dim = multipoles.shape
         nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
         ndf = damping_factors.numel() if damping_factors is not None else 0
@@ -181,7 +195,7 @@ class Context:
             self._h, 0, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles),
             self._ld(multipoles), n, dim, self._dptr(exclusions), self._ld(exclusions), nexcl, float(theta),
             int(ncrit), int(expansion_order), int(field_order), self._dptr(field), self._ld(field),
-            _damp_code(damp_type), self._dptr(damping_factors), ndf, 0, None))
+            _damp_code(damp_type), self._dptr(damping_factors), ndf, *_periodic(is_periodic, box_matrix)))
         return field
 
     @staticmethod
@@ -205,7 +219,8 @@ class Context:
         return field
 
     def field_fmm_sets_device(self, coordinates, multipoles, exclusions, theta, ncrit, expansion_order,
-                              field_order, field, damp_type="", damping_factors=None):
+                              field_order, field, damp_type="", damping_factors=None, is_periodic=False,
+                              box_matrix=None):
         n, dim, ns = multipoles.shape
         nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
         ndf = damping_factors.numel() if damping_factors is not None else 0
@@ -213,7 +228,7 @@ class Context:
             self._h, 0, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles),
             self._ld3(multipoles), n, dim, ns, self._dptr(exclusions), self._ld(exclusions), nexcl, float(theta),
             int(ncrit), int(expansion_order), int(field_order), self._dptr(field), self._ld3(field),
-            _damp_code(damp_type), self._dptr(damping_factors), ndf, 0, None))
+            _damp_code(damp_type), self._dptr(damping_factors), ndf, *_periodic(is_periodic, box_matrix)))
         return field
 
     def field_direct_targets_device(self, coordinates, multipoles, exclusions, targets, field_order, field,

@@ -42,7 +42,7 @@ enum fmmh_status {
     FMMH_OK = 0,
     FMMH_ERR_ARG = 1,          /* invalid sizes, leading dimensions, exclusion index out of range */
     FMMH_ERR_DAMP = 2,         /* "Unknown damp type: " -- src/fmm.F90:594, 1372-1377 */
-    FMMH_ERR_UNSUPPORTED = 3,  /* periodic boxes, orders outside the compiled range */
+    FMMH_ERR_UNSUPPORTED = 3,  /* orders outside the compiled range, displacements beyond the class keys */
     FMMH_ERR_HIP = 4,          /* HIP runtime failure (allocation, launch) */
     FMMH_ERR_TREE = 5,         /* >= ncrit sites share one cell 21 levels down */
     FMMH_ERR_INTERNAL = 6
@@ -81,8 +81,14 @@ int fmmh_field_direct(fmmh_context* ctx, int32_t comm,
                       int32_t damp_type, const double* damping_factors, int64_t n_damping);
 
 /* Replaces field_fmm (src/fmm.F90:1354-1400).  Host arrays.  n <= ncrit
- * falls back to the direct sum (src/fmm.F90:1381-1383).  is_periodic != 0
- * returns FMMH_ERR_UNSUPPORTED; box_matrix (3 x 3) may be NULL otherwise. */
+ * falls back to the direct sum (src/fmm.F90:1381-1383).
+ * is_periodic != 0: periodic boundaries as the reference implements them --
+ * 27 image cells displaced by box_matrix (3 x 3, column-major, host memory in
+ * every variant) through the interaction lists (exclusions in the central
+ * cell only), the lattice sum beyond them on the root expansion and the
+ * dipole correction (src/fmm.F90:884-979).  Needs n > ncrit ("Not enough
+ * coordinates for periodic FMM, please decrease ncrit") and
+ * expansion_order >= 2.  box_matrix may be NULL when is_periodic == 0. */
 int fmmh_field_fmm(fmmh_context* ctx, int32_t comm,
                    const double* coordinates, int64_t ld_coordinates,
                    const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
@@ -93,7 +99,7 @@ int fmmh_field_fmm(fmmh_context* ctx, int32_t comm,
                    int32_t is_periodic, const double* box_matrix);
 
 /* Device-resident forms of the two calls above: identical arguments, but
- * every array pointer is device memory of the context's device.  No host
+ * every array pointer except box_matrix is device memory of the context's device.  No host
  * copies; exclusion indices are not range-checked.  Used when the inputs are
  * already in HBM (benchmarks, GPU-side callers). */
 int fmmh_field_direct_dev(fmmh_context* ctx, int32_t comm,

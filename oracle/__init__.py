@@ -66,11 +66,23 @@ def field_direct(coordinates, multipoles, exclusions, field_order, damp_type="",
 
 
 def field_fmm(coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order, damp_type="",
-              damping_factors=(), impl="oracle"):
+              damping_factors=(), impl="oracle", box_matrix=None):
+    """box_matrix (3 x 3) given: periodic field_fmm, compiled reference only."""
     X, M, E, df = _prep(coordinates, multipoles, exclusions, damping_factors)
     n, dim = M.shape
     F = np.zeros((n, ncomp(field_order)), order="F")
     d = DAMP[damp_type.strip()]
+    if box_matrix is not None:
+        B = np.asfortranarray(box_matrix, dtype=np.float64)
+        if impl == "oracle":
+            diag = periodic_diagnostics(X, M, E, theta, ncrit, expansion_order, field_order, damp_type, df, B, F)
+            if diag is None:
+                raise ValueError("periodic FMM needs n > ncrit and expansion_order >= 2")
+            return F
+        lib = _lib("_ref/libfmmref.so" if impl == "ref" else "_ref/libfmmref_gnu.so")
+        lib.ref_field_fmm_periodic(n, dim, E.shape[1], _p(X), _p(M), _p(E), ctypes.c_double(theta), ncrit,
+                                   expansion_order, field_order, d, _p(df), _p(B), _p(F))
+        return F
     if impl == "oracle":
         _lib("liboracle.so").orc_field_fmm(n, dim, _p(X), _p(M), _p(E), E.shape[1], ctypes.c_double(theta), ncrit,
                                            expansion_order, field_order, d, _p(df), _p(F))
@@ -79,6 +91,24 @@ def field_fmm(coordinates, multipoles, exclusions, theta, ncrit, expansion_order
         lib.ref_field_fmm(n, dim, E.shape[1], _p(X), _p(M), _p(E), ctypes.c_double(theta), ncrit, expansion_order,
                           field_order, d, _p(df), _p(F))
     return F
+
+
+def periodic_diagnostics(X, M, E, theta, ncrit, p, field_order, damp_type, df, box, F, num_nodes=0):
+    """Periodic field_fmm of the restatement into F; returns (root local
+    expansion after M2L, after the lattice sum, levels summed, the field of
+    the near-field pairs alone, all local expansions after L2L when
+    num_nodes > 0) or None."""
+    n, dim = M.shape
+    nm = ncomp(p)
+    le1, le2 = np.zeros(nm), np.zeros(nm)
+    lv = ctypes.c_int(0)
+    fnear = np.zeros_like(F, order="F")
+    le_all = np.zeros(max(num_nodes, 1) * nm)
+    rc = _lib("liboracle.so").orc_field_fmm_periodic(
+        n, dim, _p(X), _p(M), _p(E), E.shape[1], ctypes.c_double(theta), ncrit, p, field_order,
+        DAMP[damp_type.strip()], _p(df), _p(box), _p(F), _p(le1), _p(le2), ctypes.byref(lv), _p(fnear),
+        _p(le_all) if num_nodes > 0 else None)
+    return None if rc else (le1, le2, lv.value, fnear, le_all.reshape(-1, nm) if num_nodes > 0 else None)
 
 
 def tensor(K, damp_type, x, y, z, a=None):

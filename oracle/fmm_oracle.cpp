@@ -23,6 +23,9 @@
 //   P2M/M2M/M2L/L2L/L2P/P2P .. field_fmm golden vectors from the compiled
 //                              reference, polynomial shift identities
 //   direct ................... golden vectors + closed-form 1e10 artefact
+//   periodic ................. field_fmm golden vectors from the compiled
+//                              reference in well-conditioned boxes (its
+//                              lattice sum converges); see periodic_sum
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off).
 
@@ -250,7 +253,8 @@ struct Tree {
     int max_depth = 0;
     int ncrit = 0;
     double theta = 0.0;
-    std::vector<std::vector<int>> m2l, p2p;   // by pre-order id, source pre-order ids
+    std::vector<std::vector<int>> m2l, p2p;   // by pre-order id, source pre-order ids (central cell)
+    std::vector<std::vector<std::vector<int>>> m2l_img, p2p_img;   // [image][pre-order id]
 };
 
 // flang's NORM2 on a 3-vector, matched bit-for-bit by a black-box probe of
@@ -379,33 +383,63 @@ static void octree_build(Tree& tr, const double* X, int ldx, int n, int ncrit, d
 
 static Node& node_by_id(Tree& tr, int id) { return tr.nodes[tr.order[id]]; }
 
-// src/fmm.F90:352-385 (non-periodic: displacement 0)
-static void interact(Tree& tr, int ci, int cj) {
+// Periodic images (src/fmm.F90:156-175): aijk over (-1..1)^3 with ai
+// outermost; displacement = matmul(box_matrix, aijk) summed column by
+// column (src/fmm.F90:360).  Image 13 is the central cell.
+struct Images {
+    int n = 1;
+    double disp[27][3] = {};
+    int central = 0;
+    double box[9] = {};
+};
+
+static Images make_images(const double* box) {
+    Images im;
+    if (!box) return im;
+    im.n = 27;
+    im.central = 13;
+    for (int k = 0; k < 9; ++k) im.box[k] = box[k];
+    int idx = 0;
+    for (int ai = -1; ai <= 1; ++ai)
+        for (int aj = -1; aj <= 1; ++aj)
+            for (int ak = -1; ak <= 1; ++ak) {
+                for (int r = 0; r < 3; ++r) im.disp[idx][r] = (box[r] * ai + box[r + 3] * aj) + box[r + 6] * ak;
+                ++idx;
+            }
+    return im;
+}
+
+// src/fmm.F90:352-385, lists of image k (displacement D)
+static void interact(Tree& tr, int ci, int cj, int k, const double D[3]) {
     Node& a = tr.nodes[ci];
     Node& b = tr.nodes[cj];
     double delta[3];
-    for (int k = 0; k < 3; ++k) delta[k] = a.c[k] - (b.c[k] + 0.0);
+    for (int q = 0; q < 3; ++q) delta[q] = a.c[q] - (b.c[q] + D[q]);
     double r = norm2_3(delta);
     if (r * tr.theta > a.rmax + b.rmax) {
-        tr.m2l[a.idx].push_back(b.idx);
+        tr.m2l_img[k][a.idx].push_back(b.idx);
     } else if (a.terminal && b.terminal) {
-        tr.p2p[a.idx].push_back(b.idx);
+        tr.p2p_img[k][a.idx].push_back(b.idx);
     } else if (b.terminal || ((a.rmax > b.rmax) && !a.terminal)) {
         for (int o = 0; o < 8; ++o)
-            if (tr.nodes[ci].child[o] >= 0) interact(tr, tr.nodes[ci].child[o], cj);
+            if (tr.nodes[ci].child[o] >= 0) interact(tr, tr.nodes[ci].child[o], cj, k, D);
     } else {
         for (int o = 0; o < 8; ++o)
-            if (tr.nodes[cj].child[o] >= 0) interact(tr, ci, tr.nodes[cj].child[o]);
+            if (tr.nodes[cj].child[o] >= 0) interact(tr, ci, tr.nodes[cj].child[o], k, D);
     }
 }
 
-// src/fmm.F90:325-350
-static void build_lists(Tree& tr) {
+// src/fmm.F90:325-350; tr.m2l / tr.p2p are the central cell's lists
+static void build_lists(Tree& tr, const Images& im) {
     size_t nn = tr.order.size();
-    tr.m2l.assign(nn, {});
-    tr.p2p.assign(nn, {});
-    interact(tr, 0, 0);
+    tr.m2l_img.assign(im.n, std::vector<std::vector<int>>(nn));
+    tr.p2p_img.assign(im.n, std::vector<std::vector<int>>(nn));
+    for (int k = 0; k < im.n; ++k) interact(tr, 0, 0, k, im.disp[k]);
+    tr.m2l = tr.m2l_img[im.central];
+    tr.p2p = tr.p2p_img[im.central];
 }
+
+static void build_lists(Tree& tr) { build_lists(tr, Images{}); }
 
 // ---------------------------------------------------------------- upward pass
 // src/fmm.F90:437-472 (P2M), monomials over the leaf members.
@@ -535,26 +569,30 @@ static void multipole_expansion(Tree& tr, const Inputs& in, int p, std::vector<d
     }
 }
 
-// src/fmm.F90:623-747 (particle_fields), serial, non-periodic
-static void particle_fields(Tree& tr, const Inputs& in) {
+// src/fmm.F90:623-747 (particle_fields), serial; image lists in order,
+// exclusions in the central cell only (src/fmm.F90:714)
+static void particle_fields(Tree& tr, const Inputs& in, const Images& im) {
     int nf = ncomp(in.field_order);
     std::vector<double> F(nf);
     size_t nn = tr.order.size();
+    for (int k = 0; k < im.n; ++k) {
+    const double* D = im.disp[k];
+    const bool central = k == im.central;
     for (size_t id = 0; id < nn; ++id) {
         Node& ni = node_by_id(tr, (int)id);
         if (!ni.terminal) continue;
         for (int li = 0; li < ni.nleaf; ++li) {
             int i = ni.leaf[li];
-            for (int jid : tr.p2p[id]) {
+            for (int jid : tr.p2p_img[k][id]) {
                 Node& nj = node_by_id(tr, jid);
                 int ns = nj.nleaf;
                 std::vector<double> dx(ns), dy(ns), dz(ns), a(ns), Mj((size_t)ns * in.dim);
                 for (int jj = 0; jj < ns; ++jj) {
                     int j = nj.leaf[jj];
-                    dx[jj] = in.X[i] - (in.X[j] + 0.0);
-                    dy[jj] = in.X[in.ldx + i] - (in.X[in.ldx + j] + 0.0);
-                    dz[jj] = in.X[2 * (size_t)in.ldx + i] - (in.X[2 * (size_t)in.ldx + j] + 0.0);
-                    if (excluded(in, i, j)) { dx[jj] = 1e10; dy[jj] = 1e10; dz[jj] = 1e10; }
+                    dx[jj] = in.X[i] - (in.X[j] + D[0]);
+                    dy[jj] = in.X[in.ldx + i] - (in.X[in.ldx + j] + D[1]);
+                    dz[jj] = in.X[2 * (size_t)in.ldx + i] - (in.X[2 * (size_t)in.ldx + j] + D[2]);
+                    if (central && excluded(in, i, j)) { dx[jj] = 1e10; dy[jj] = 1e10; dz[jj] = 1e10; }
                     if (in.damp) a[jj] = in.df[i] * in.df[j];
                     for (int c = 0; c < in.dim; ++c) Mj[(size_t)c * ns + jj] = in.M[(size_t)c * in.ldm + j];
                 }
@@ -564,29 +602,108 @@ static void particle_fields(Tree& tr, const Inputs& in) {
             }
         }
     }
+    }
 }
 
-// src/fmm.F90:826-882 (multipole_to_local), undamped, field order p
-static void multipole_to_local(Tree& tr, int p, const std::vector<double>& CM, std::vector<double>& LE) {
+// src/fmm.F90:826-882 (multipole_to_local), undamped, field order p, image by image
+static void multipole_to_local(Tree& tr, int p, const std::vector<double>& CM, std::vector<double>& LE,
+                               const Images& im) {
     int nm = ncomp(p);
     size_t nn = tr.order.size();
     LE.assign(nn * nm, 0.0);
     std::vector<double> F(nm);
+    for (int kk = 0; kk < im.n; ++kk) {
+    const double* D = im.disp[kk];
     for (size_t id = 0; id < nn; ++id) {
         Node& ni = node_by_id(tr, (int)id);
-        const std::vector<int>& lst = tr.m2l[id];
+        const std::vector<int>& lst = tr.m2l_img[kk][id];
         int ns = (int)lst.size();
         std::vector<double> dx(ns), dy(ns), dz(ns), Ms((size_t)ns * nm);
         for (int k = 0; k < ns; ++k) {
             Node& nj = node_by_id(tr, lst[k]);
-            dx[k] = ni.c[0] - (nj.c[0] + 0.0);
-            dy[k] = ni.c[1] - (nj.c[1] + 0.0);
-            dz[k] = ni.c[2] - (nj.c[2] + 0.0);
+            dx[k] = ni.c[0] - (nj.c[0] + D[0]);
+            dy[k] = ni.c[1] - (nj.c[1] + D[1]);
+            dz[k] = ni.c[2] - (nj.c[2] + D[2]);
             for (int c = 0; c < nm; ++c) Ms[(size_t)c * ns + k] = CM[(size_t)lst[k] * nm + c];
         }
         multipole_field(Ms.data(), ns, nm, ns, dx.data(), dy.data(), dz.data(), p, DAMP_NONE, nullptr, F.data());
         for (int c = 0; c < nm; ++c) LE[id * nm + c] += F[c];
     }
+    }
+}
+
+// src/fmm.F90:884-944 (periodic_sum): lattice beyond the 27 cells on the
+// root local expansion, shell by shell of 3^level supercells, root charge
+// removed, until the norm of the contribution (without the potential) is
+// below 1e-8 or 50 levels.  Returns the number of levels summed.
+static int periodic_sum(Tree& tr, int p, std::vector<double>& CM, std::vector<double>& LE, const Images& im) {
+    int nm = ncomp(p);
+    const int root = 0;                       // pre-order id of the root
+    CM[(size_t)root * nm + 0] = 0.0;          // src/fmm.F90:897
+    std::vector<double> cell(CM.begin() + (size_t)root * nm, CM.begin() + (size_t)(root + 1) * nm);
+    std::vector<double> scaled = cell;
+    std::vector<double> dx(26 * 27), dy(26 * 27), dz(26 * 27), Ms((size_t)26 * 27 * nm), F(nm);
+    const double* B = im.box;
+    auto matvec = [&](const double a[3], double d[3]) {
+        for (int r = 0; r < 3; ++r) d[r] = (B[r] * a[0] + B[r + 3] * a[1]) + B[r + 6] * a[2];
+    };
+    int level = 1;
+    for (; level <= 50; ++level) {
+        int k = 0;
+        double p3 = std::pow(3.0, level), p3m = std::pow(3.0, level - 1);
+        for (int ai = -1; ai <= 1; ++ai)
+            for (int aj = -1; aj <= 1; ++aj)
+                for (int ak = -1; ak <= 1; ++ak) {
+                    if (ai == 0 && aj == 0 && ak == 0) continue;
+                    double a3[3] = {ai * p3, aj * p3, ak * p3}, da[3];
+                    matvec(a3, da);
+                    double d3[3] = {da[0] / 3.0, da[1] / 3.0, da[2] / 3.0};
+                    multipole_accumulate_kernel(d3, cell.data(), scaled.data(), p);
+                    for (int bi = -1; bi <= 1; ++bi)
+                        for (int bj = -1; bj <= 1; ++bj)
+                            for (int bk = -1; bk <= 1; ++bk) {
+                                double b3[3] = {bi * p3m, bj * p3m, bk * p3m}, db[3];
+                                matvec(b3, db);
+                                dx[k] = da[0] + db[0];
+                                dy[k] = da[1] + db[1];
+                                dz[k] = da[2] + db[2];
+                                ++k;
+                            }
+                }
+        for (int c = 0; c < nm; ++c)
+            for (int q = 0; q < 26 * 27; ++q) Ms[(size_t)c * 26 * 27 + q] = cell[c];
+        multipole_field(Ms.data(), 26 * 27, nm, 26 * 27, dx.data(), dy.data(), dz.data(), p, DAMP_NONE, nullptr,
+                        F.data());
+        for (int c = 0; c < nm; ++c) LE[(size_t)root * nm + c] += F[c];
+        cell = scaled;
+        double sq = 0.0;
+        for (int c = 1; c < nm; ++c) sq += F[c] * F[c];
+        if (std::sqrt(sq) < 1.0e-8) break;
+    }
+    return std::min(level, 50);
+}
+
+// src/fmm.F90:946-979 (dipole_correction), with the reference's volume
+// expression (its last cofactor reads h(1,2)) and pi literal.
+static void dipole_correction(Tree& tr, const Inputs& in, int p, const std::vector<double>& CM, const Images& im) {
+    int nm = ncomp(p);
+    const double* h = im.box;
+    auto H = [&](int r, int c) { return h[(r - 1) + 3 * (c - 1)]; };
+    double volume = H(1, 1) * (H(2, 2) * H(3, 3) - H(2, 3) * H(3, 2)) - H(1, 2) * (H(2, 1) * H(3, 3) - H(2, 3) * H(3, 1)) +
+                    H(1, 3) * (H(1, 2) * H(3, 2) - H(2, 2) * H(3, 1));
+    const double pi = 3.14159265359;
+    const double* root = &CM[0];
+    double dip[3] = {root[1], root[2], root[3]};
+    double qtr = (root[4] + root[7]) + root[9];
+    (void)nm;
+    for (int i = 0; i < in.n; ++i) {
+        double s = (dip[0] * in.X[i] + dip[1] * in.X[in.ldx + i]) + dip[2] * in.X[2 * (size_t)in.ldx + i];
+        double& f0 = in.F[i];
+        f0 = (f0 + 4.0 * pi * s / (3.0 * volume)) + 2.0 * pi * qtr / (3.0 * volume);
+    }
+    if (in.field_order >= 1)
+        for (int c = 0; c < 3; ++c)
+            for (int i = 0; i < in.n; ++i) in.F[(size_t)(c + 1) * in.ldf + i] += 4.0 * pi * dip[c] / (3.0 * volume);
 }
 
 // src/fmm.F90:981-1044 (local_to_local), depth by depth
@@ -690,10 +807,40 @@ int orc_field_fmm(int n, int dim, const double* X, const double* M, const int* E
     orc::octree_build(tr, X, n, n, ncrit, theta);
     std::vector<double> CM, LE;
     orc::multipole_expansion(tr, in, p, CM);
-    orc::build_lists(tr);
-    orc::particle_fields(tr, in);
-    orc::multipole_to_local(tr, p, CM, LE);
+    orc::Images im;
+    orc::build_lists(tr, im);
+    orc::particle_fields(tr, in, im);
+    orc::multipole_to_local(tr, p, CM, LE, im);
     orc::local_to_local(tr, p, LE);
+    orc::local_to_particle(tr, in, p, LE);
+    return 0;
+}
+
+// src/fmm.F90:1354-1400, periodic (box: 3 x 3 column-major).  Optional
+// diagnostics: root local expansion after M2L (le_m2l) and after the lattice
+// sum (le_psum), levels summed (levels).  n must exceed ncrit.
+int orc_field_fmm_periodic(int n, int dim, const double* X, const double* M, const int* E, int nexcl, double theta,
+                           int ncrit, int p, int field_order, int damp, const double* df, const double* box, double* F,
+                           double* le_m2l, double* le_psum, int* levels, double* F_near, double* le_all) {
+    orc::Inputs in{n, dim, nexcl, field_order, damp, X, n, M, n, E, n, df, F, n};
+    if (n <= ncrit || p < 2) return 1;
+    orc::Tree tr;
+    orc::octree_build(tr, X, n, n, ncrit, theta);
+    std::vector<double> CM, LE;
+    orc::multipole_expansion(tr, in, p, CM);
+    orc::Images im = orc::make_images(box);
+    orc::build_lists(tr, im);
+    orc::particle_fields(tr, in, im);
+    if (F_near) std::copy(F, F + (size_t)n * orc::ncomp(field_order), F_near);
+    orc::multipole_to_local(tr, p, CM, LE, im);
+    int nm = orc::ncomp(p);
+    if (le_m2l) for (int c = 0; c < nm; ++c) le_m2l[c] = LE[c];
+    int lv = orc::periodic_sum(tr, p, CM, LE, im);
+    if (le_psum) for (int c = 0; c < nm; ++c) le_psum[c] = LE[c];
+    if (levels) *levels = lv;
+    orc::dipole_correction(tr, in, p, CM, im);
+    orc::local_to_local(tr, p, LE);
+    if (le_all) std::copy(LE.begin(), LE.end(), le_all);
     orc::local_to_particle(tr, in, p, LE);
     return 0;
 }
@@ -710,6 +857,24 @@ int orc_tensor(int K, int damp, int n, const double* x, const double* y, const d
 }
 
 double orc_norm2(const double* v) { return orc::norm2_3(v); }
+
+// Periodic interaction-list sizes: out = {M2L entries, P2P entries, P2P site
+// pairs} summed over the 27 images (box: 3 x 3 column-major).
+void orc_periodic_list_counts(int n, const double* X, int ncrit, double theta, const double* box, long long* out) {
+    orc::Tree tr;
+    orc::octree_build(tr, X, n, n, ncrit, theta);
+    orc::Images im = orc::make_images(box);
+    orc::build_lists(tr, im);
+    long long nm = 0, np = 0, sp = 0;
+    for (int k = 0; k < im.n; ++k)
+        for (size_t id = 0; id < tr.order.size(); ++id) {
+            nm += tr.m2l_img[k][id].size();
+            np += tr.p2p_img[k][id].size();
+            for (int j : tr.p2p_img[k][id])
+                sp += (long long)orc::node_by_id(tr, (int)id).nleaf * orc::node_by_id(tr, j).nleaf;
+        }
+    out[0] = nm; out[1] = np; out[2] = sp;
+}
 
 // Tree + interaction lists, exported in pre-order with 0-based ids.
 struct OrcTree { orc::Tree tr; };

@@ -63,4 +63,25 @@ contains
                            df(1:n), periodic, box)
         end if
     end subroutine ref_field_fmm
+
+    ! periodic field_fmm: box is the 3 x 3 box_matrix (column-major)
+    subroutine ref_field_fmm_periodic(n, dim, nexcl, coords, mults, excl, theta, ncrit, p, field_order, damp, df, &
+                                      box, field) bind(C, name="ref_field_fmm_periodic")
+        integer(c_int), value :: n, dim, nexcl, ncrit, p, field_order, damp
+        real(c_double), value :: theta
+        real(c_double), intent(in) :: coords(n, 3), mults(n, dim), df(*), box(3, 3)
+        integer(c_int), intent(in) :: excl(n, nexcl)
+        real(c_double), intent(inout) :: field(n, (field_order + 1) * (field_order + 2) * (field_order + 3) / 6)
+        integer(ip) :: comm
+        logical(lp) :: periodic
+        comm = 0
+        periodic = .true.
+        if (damp == 0) then
+            call field_fmm(comm, coords, mults, excl, theta, ncrit, p, field_order, field, damp_name(damp), &
+                           df(1:0), periodic, box)
+        else
+            call field_fmm(comm, coords, mults, excl, theta, ncrit, p, field_order, field, damp_name(damp), &
+                           df(1:n), periodic, box)
+        end if
+    end subroutine ref_field_fmm_periodic
 end module fmmref_bindc

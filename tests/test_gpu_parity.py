@@ -163,8 +163,8 @@ def test_errors(gpu_ctx):
     x, q, e = g["coordinates"], g["multipoles"], g["exclusions"]
     with pytest.raises(FmmError, match="Unknown damp type: thole"):
         gpu_ctx.field_fmm(0, x, q, e, 0.5, 64, 4, 1, "thole", np.ones(len(x)))
-    with pytest.raises(FmmError, match="periodic"):
-        gpu_ctx.field_fmm(0, x, q, e, 0.5, 64, 4, 1, is_periodic=True, box_matrix=np.eye(3) * 10)
+    with pytest.raises(FmmError, match="Not enough coordinates for periodic FMM"):
+        gpu_ctx.field_fmm(0, x, q, e, 0.5, 400, 4, 1, is_periodic=True, box_matrix=np.eye(3) * 10)
     bad = e.copy()
     bad[3, 0] = len(x) + 5
     with pytest.raises(FmmError, match="outside"):
