@@ -30,6 +30,12 @@ _PROTOS = {
                                       i64, i32, vp, i64, i32, vp]),
     "fmmh_field_direct_targets_dev": (i32, [vp, vp, i64, vp, i64, i32, i32, vp, i64, i32, vp, i32, i32, vp, i64,
                                             i32, vp, i64]),
+    "fmmh_get_boxes": (i32, [vp, vp, i64, i32, vp, i64, i32, f64, i32, i32, vp]),
+    "fmmh_get_boxes_export": (i32, [vp, vp, vp, vp]),
+    "fmmh_get_multipoles": (i32, [vp, i32, vp, i64, i32, vp, i64, i32, i32, vp, i64, i32, f64, i32, i32, vp]),
+    "fmmh_get_multipoles_export": (i32, [vp, vp, vp, vp]),
+    "fmmh_interpolate_electron_fields": (i32, [vp, i32, vp, i64, i32, vp, i64, i32, i32, vp, f64, i32, i32, i32, vp,
+                                               i64]),
     "fmmh_tree_build": (i32, [vp, vp, i64, i32, i32, f64, vp]),
     "fmmh_tree_export": (i32, [vp] + [vp] * 13),
     "fmmh_set_option": (i32, [vp, i32, f64]),

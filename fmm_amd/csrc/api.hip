@@ -97,6 +97,9 @@ struct fmmh_context {
     int cache_class_p = -1;           // expansion order of the cached class tensors (-1: none)
     int64_t cache_classes = 0;
     unsigned long long cache_pairs = 0;
+    // last get_boxes / get_multipoles classification, kept for the export call
+    int qm_num_near = -1, qm_num_far = 0, qm_nm = 0, qm_ndens = 0, qm_num_nodes = 0;
+    bool qm_multipoles = false;
     int cache_retries = 0;
     int list_retries = 0;
     // last tree (for export)
@@ -1043,6 +1046,53 @@ void check_fmm_args(double theta, int ncrit, int p, int is_periodic, const doubl
 
 }  // namespace
 
+// ---------------------------------------------------------------- QM/MM embedding
+// Octree of the sites, classification against the target points, and the
+// compaction of near-field sites / far boxes (src/fmm.F90:1171-1291).
+void qm_classify(fmmh_context* c, const double* X, int n, const double* T, int nt, double theta, int ncrit,
+                 TreeOut& tr) {
+    hipStream_t s = c->stream;
+    c->cache_valid = false;
+    build_tree(c, X, n, n, ncrit, tr);
+    const int nn = tr.num_nodes;
+    int* close = c->buf<int>("q_close", nn);
+    int* type = c->buf<int>("q_type", nn);
+    launch_classify(T, nt, nt, theta, tr.t, nn, close, type, s);
+    int* near_cnt = c->buf<int>("q_near_cnt", nn + 1);
+    int* far_cnt = c->buf<int>("q_far_cnt", nn + 1);
+    int* near_off = c->buf<int>("q_near_off", nn + 1);
+    int* far_off = c->buf<int>("q_far_off", nn + 1);
+    launch_qmmm_counts(type, tr.t.count, nn, near_cnt, far_cnt, s);
+    size_t tb = exscan_i32_temp(nn + 1);
+    void* temp = c->buf<char>("q_temp", tb);
+    exscan_i32(temp, tb, near_cnt, near_off, nn + 1, s);
+    exscan_i32(temp, tb, far_cnt, far_off, nn + 1, s);
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 440, near_off + nn, sizeof(int), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 441, far_off + nn, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->sync();
+    c->qm_num_near = c->h_pinned[440];
+    c->qm_num_far = c->h_pinned[441];
+    c->qm_num_nodes = nn;
+    int* near_sites = c->buf<int>("q_near", std::max(c->qm_num_near, 1));
+    int* far_nodes = c->buf<int>("q_far", std::max(c->qm_num_far, 1));
+    launch_qmmm_compact(type, tr.t, nn, near_off, far_off, near_sites, far_nodes, s);
+    c->sync();
+}
+
+const double* stage_host(fmmh_context* c, const char* name, const double* h, int64_t ld, int n, int cols) {
+    double* d = c->buf<double>(name, (size_t)n * std::max(cols, 1));
+    if (cols > 0)
+        FMMH_HIP(hipMemcpy2DAsync(d, n * sizeof(double), h, ld * sizeof(double), n * sizeof(double), cols,
+                                  hipMemcpyHostToDevice, c->stream));
+    return d;
+}
+
+void check_qm(int n, int64_t ldx, int nt, int64_t ldt, double theta, int ncrit, int p) {
+    if (n < 1 || ldx < n) throw Error(FMMH_ERR_ARG, "bad n / leading dimension of coordinates");
+    if (nt < 0 || (nt > 0 && ldt < nt)) throw Error(FMMH_ERR_ARG, "bad target count / leading dimension");
+    check_fmm_args(theta, ncrit, p, 0, nullptr);
+}
+
 // ================================================================ C ABI
 extern "C" {
 
@@ -1251,6 +1301,189 @@ int fmmh_field_direct_targets_dev(fmmh_context* ctx, const double* coordinates, 
         c->begin_timing();
         run_direct_targets(c, k, targets, ntargets);
         c->end_timing();
+    });
+}
+
+int fmmh_get_boxes(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates, int32_t n,
+                   const double* target_coordinates, int64_t ld_targets, int32_t ntargets, double theta,
+                   int32_t ncrit, int32_t expansion_order, int64_t* sizes) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        check_qm(n, ld_coordinates, ntargets, ld_targets, theta, ncrit, expansion_order);
+        const double* X = stage_host(c, "h_X", coordinates, ld_coordinates, n, 3);
+        const double* T = stage_host(c, "q_T", target_coordinates, ld_targets, ntargets, 3);
+        TreeOut tr;
+        qm_classify(c, X, n, T, ntargets, theta, ncrit, tr);
+        c->qm_multipoles = false;
+        sizes[0] = c->qm_num_near;
+        sizes[1] = c->qm_num_far;
+    });
+}
+
+// Shared export of the near-field sites and far boxes of the last classification.
+static void qm_export_common(fmmh_context* c, int32_t* idx_near_field, double* box_coordinates, int32_t* far_ids) {
+    const int nn = c->qm_num_nodes, nfar = c->qm_num_far, nnear = c->qm_num_near;
+    auto get = [&](void* dst, const char* name, size_t bytes) {
+        auto it = c->bufs.find(name);
+        if (it == c->bufs.end()) throw Error(FMMH_ERR_INTERNAL, std::string("missing buffer ") + name);
+        FMMH_HIP(hipMemcpy(dst, it->second.p, bytes, hipMemcpyDeviceToHost));
+    };
+    if (nnear > 0 && idx_near_field) get(idx_near_field, "q_near", nnear * sizeof(int));
+    std::vector<int> ids(std::max(nfar, 1));
+    if (nfar > 0) get(ids.data(), "q_far", nfar * sizeof(int));
+    if (box_coordinates && nfar > 0) {
+        std::vector<double> cx(nn), cy(nn), cz(nn);
+        get(cx.data(), "n_cx", nn * sizeof(double));
+        get(cy.data(), "n_cy", nn * sizeof(double));
+        get(cz.data(), "n_cz", nn * sizeof(double));
+        for (int k = 0; k < nfar; ++k) {
+            box_coordinates[k] = cx[ids[k]];
+            box_coordinates[nfar + k] = cy[ids[k]];
+            box_coordinates[2 * (size_t)nfar + k] = cz[ids[k]];
+        }
+    }
+    if (far_ids)
+        for (int k = 0; k < nfar; ++k) far_ids[k] = ids[k];
+}
+
+int fmmh_get_boxes_export(fmmh_context* ctx, int32_t* idx_near_field, double* box_coordinates, int32_t* box_indices) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (c->qm_num_near < 0 || c->qm_multipoles) throw Error(FMMH_ERR_ARG, "no get_boxes result on this context");
+        qm_export_common(c, idx_near_field, box_coordinates, box_indices);
+        if (box_indices)
+            for (int k = 0; k < c->qm_num_far; ++k) box_indices[k] += 1;   // 1-based node index
+    });
+}
+
+int fmmh_get_multipoles(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates, int32_t n,
+                        const double* multipoles, int64_t ld_multipoles, int32_t dim, int32_t ndens,
+                        const double* target_coordinates, int64_t ld_targets, int32_t ntargets, double theta,
+                        int32_t ncrit, int32_t expansion_order, int64_t* sizes) {
+    (void)comm;
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        check_qm(n, ld_coordinates, ntargets, ld_targets, theta, ncrit, expansion_order);
+        if (ndens < 1) throw Error(FMMH_ERR_ARG, "ndens must be >= 1");
+        if (ld_multipoles < n) throw Error(FMMH_ERR_ARG, "leading dimension of multipoles smaller than n");
+        const int m = order_from_dim(dim);
+        if (m < 0 || ncomp(m) != dim || m > NEAR_MMAX)
+            throw Error(FMMH_ERR_UNSUPPORTED, "multipole dim must be (m+1)(m+2)(m+3)/6 with m <= 2");
+        hipStream_t s = c->stream;
+        const double* X = stage_host(c, "h_X", coordinates, ld_coordinates, n, 3);
+        const double* T = stage_host(c, "q_T", target_coordinates, ld_targets, ntargets, 3);
+        const double* M = stage_host(c, "h_M", multipoles, ld_multipoles, n, dim * ndens);
+        TreeOut tr;
+        qm_classify(c, X, n, T, ntargets, theta, ncrit, tr);
+        // cell multipoles of every node for every density set (P2M + M2M)
+        const int p = expansion_order, nm = ncomp(p), nn = tr.num_nodes;
+        Tables& tb = get_tables(c, p, m);
+        double* xs = c->buf<double>("s_xs", n);
+        double* ys = c->buf<double>("s_ys", n);
+        double* zs = c->buf<double>("s_zs", n);
+        double* ms_raw = c->buf<double>("s_ms_raw", (size_t)n * dim * ndens);
+        double* msc = c->buf<double>("s_ms", (size_t)n * dim * ndens);
+        double* dfs = c->buf<double>("s_dfs", n);
+        int* excl = c->buf<int>("s_excl", n);
+        SiteArgs g{X, n, M, n, dim, nullptr, n, 0, nullptr, tr.t.perm, n, xs, ys, zs, ms_raw, msc, dfs, excl,
+                   tb.scale, ndens};
+        launch_gather(g, s);
+        double* CM = c->buf<double>("e_cm", (size_t)nn * nm * ndens);
+        UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
+                  tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, ndens, dim};
+        launch_p2m(up, tr.num_leaves, s);
+        for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
+            int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
+            launch_m2m(up, tr.t.level_nodes + o0, o1 - o0, s);
+        }
+        c->sync();
+        c->qm_multipoles = true;
+        c->qm_nm = nm;
+        c->qm_ndens = ndens;
+        sizes[0] = c->qm_num_near;
+        sizes[1] = c->qm_num_far;
+    });
+}
+
+int fmmh_get_multipoles_export(fmmh_context* ctx, int32_t* idx_near_field, double* box_coordinates,
+                               double* box_multipoles) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (c->qm_num_near < 0 || !c->qm_multipoles)
+            throw Error(FMMH_ERR_ARG, "no get_multipoles result on this context");
+        const int nfar = c->qm_num_far, nm = c->qm_nm, nd = c->qm_ndens, nn = c->qm_num_nodes;
+        std::vector<int> ids(std::max(nfar, 1));
+        qm_export_common(c, idx_near_field, box_coordinates, ids.data());
+        if (box_multipoles && nfar > 0) {
+            std::vector<double> cm((size_t)nn * nm * nd);
+            auto it = c->bufs.find("e_cm");
+            FMMH_HIP(hipMemcpy(cm.data(), it->second.p, cm.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (int d = 0; d < nd; ++d)
+                for (int t = 0; t < nm; ++t)
+                    for (int k = 0; k < nfar; ++k)
+                        box_multipoles[((size_t)d * nm + t) * nfar + k] = cm[((size_t)d * nn + ids[k]) * nm + t];
+        }
+    });
+}
+
+int fmmh_interpolate_electron_fields(fmmh_context* ctx, int32_t comm, const double* coordinates,
+                                     int64_t ld_coordinates, int32_t n, const double* local_expansion,
+                                     int64_t ld_local_expansion, int32_t nbox, int32_t ndens,
+                                     const int32_t* box_indices, double theta, int32_t ncrit,
+                                     int32_t expansion_order, int32_t field_order, double* field,
+                                     int64_t ld_field) {
+    (void)comm;
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        check_qm(n, ld_coordinates, 0, 0, theta, ncrit, expansion_order);
+        if (ndens < 1 || nbox < 0) throw Error(FMMH_ERR_ARG, "ndens must be >= 1 and nbox >= 0");
+        if (nbox > 0 && ld_local_expansion < nbox) throw Error(FMMH_ERR_ARG, "leading dimension of local_expansion");
+        if (ld_field < n) throw Error(FMMH_ERR_ARG, "leading dimension of field smaller than n");
+        if (field_order < 0 || field_order > NEAR_FMAX) throw Error(FMMH_ERR_UNSUPPORTED, "field_order must be 0..2");
+        hipStream_t s = c->stream;
+        const double* X = stage_host(c, "h_X", coordinates, ld_coordinates, n, 3);
+        c->cache_valid = false;
+        TreeOut tr;
+        build_tree(c, X, n, n, ncrit, tr);
+        const int p = expansion_order, nm = ncomp(p), nn = tr.num_nodes, nf = ncomp(field_order);
+        // local expansions: zero except the given boxes (src/fmm.F90:1322-1328), later rows win
+        std::vector<double> le((size_t)nn * nm * ndens, 0.0);
+        for (int i = 0; i < nbox; ++i) {
+            const int b = box_indices[i];
+            if (b < 1 || b > nn)
+                throw Error(FMMH_ERR_ARG, "box_indices(" + std::to_string(i + 1) + ") = " + std::to_string(b) +
+                                              " is outside 1..num_nodes (" + std::to_string(nn) + ")");
+            for (int d = 0; d < ndens; ++d)
+                for (int t = 0; t < nm; ++t)
+                    le[((size_t)d * nn + (b - 1)) * nm + t] =
+                        local_expansion[((size_t)d * nm + t) * ld_local_expansion + i];
+        }
+        double* LE = c->buf<double>("e_le", le.size());
+        FMMH_HIP(hipMemcpyAsync(LE, le.data(), le.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        Tables& tb = get_tables(c, p, 0);
+        double* xs = c->buf<double>("s_xs", n);
+        double* ys = c->buf<double>("s_ys", n);
+        double* zs = c->buf<double>("s_zs", n);
+        double* dfs = c->buf<double>("s_dfs", n);
+        int* excl = c->buf<int>("s_excl", n);
+        SiteArgs g{X, n, nullptr, n, 0, nullptr, n, 0, nullptr, tr.t.perm, n, xs, ys, zs, nullptr, nullptr, dfs,
+                   excl, tb.scale, 1};
+        launch_gather(g, s);
+        double* near = c->buf<double>("e_near", (size_t)n * nf * ndens);
+        FMMH_HIP(hipMemsetAsync(near, 0, (size_t)n * nf * ndens * sizeof(double), s));
+        double* F = c->buf<double>("h_F", (size_t)n * nf * ndens);
+        FMMH_HIP(hipMemcpy2DAsync(F, n * sizeof(double), field, ld_field * sizeof(double), n * sizeof(double),
+                                  (size_t)nf * ndens, hipMemcpyHostToDevice, s));
+        DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
+                    tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, F, n, nf, nn, ndens};
+        for (int lev = 2; lev <= tr.max_depth; ++lev) {
+            int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
+            launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
+        }
+        launch_l2p(da, tr.num_leaves, s);
+        FMMH_HIP(hipMemcpy2DAsync(field, ld_field * sizeof(double), F, n * sizeof(double), n * sizeof(double),
+                                  (size_t)nf * ndens, hipMemcpyDeviceToHost, s));
+        c->sync();
     });
 }
 

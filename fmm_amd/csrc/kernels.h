@@ -286,6 +286,21 @@ void launch_periodic_shells(const PeriodicArgs& a, hipStream_t s);
 void launch_periodic_sum(const PeriodicArgs& a, hipStream_t s);
 void launch_dipole_correction(const PeriodicArgs& a, hipStream_t s);
 
+// ---------------------------------------------------------------- QM/MM embedding
+// classify_box (src/fmm.F90:1134-1169) against a set of target points: a
+// node is too close if r theta <= rmax for any target (flang NORM2); the
+// recursion descends close internal nodes only, so a node is visited iff
+// every ancestor is close.  type: 0 not used, 1 far (box multipole), 2 close
+// leaf (its sites join the near field).
+void launch_classify(const double* targets, int64_t ldt, int ntargets, double theta, const TreeDev& t, int num_nodes,
+                     int* close, int* type, hipStream_t s);
+// Per-node counts for the compaction: near[node] = count if type 2, far[node] = 1 if type 1.
+void launch_qmmm_counts(const int* type, const int* count, int num_nodes, int* near_cnt, int* far_cnt, hipStream_t s);
+// Writes 1-based site indices of close leaves (reference leaf order) and the
+// far boxes' pre-order ids (0-based) in node order.
+void launch_qmmm_compact(const int* type, const TreeDev& t, int num_nodes, const int* near_off, const int* far_off,
+                         int* near_sites, int* far_nodes, hipStream_t s);
+
 // ---------------------------------------------------------------- primitives (rocPRIM)
 size_t sort_pairs_u64_temp(int n);
 void sort_pairs_u64(void* temp, size_t temp_bytes, const unsigned long long* kin, unsigned long long* kout,

@@ -25,7 +25,8 @@ import numpy as np
 from . import _lib
 from ._lib import FmmError, check
 
-__all__ = ["field_direct", "field_fmm", "Context", "FmmError", "ncomp"]
+__all__ = ["field_direct", "field_fmm", "get_multipoles", "get_boxes", "interpolate_electron_fields", "Context",
+           "FmmError", "ncomp"]
 
 DAMP = {"": 0, "THOLE": 1, "AMOEBA": 2, "ERF": 3}
 PHASE_CAP = 64
@@ -159,6 +160,57 @@ class Context:
                                             E.shape[1], float(theta), int(ncrit), int(expansion_order),
                                             int(field_order), _p(F), F.shape[0], _damp_code(damp_type),
                                             _p(df) if df.size else None, df.size, 1 if is_periodic else 0, _p(box)))
+        return F
+
+    # -- QM/MM embedding (src/fmm.F90:1134-1336) --------------------------------
+    def get_boxes(self, coordinates, target_coordinates, theta, ncrit, expansion_order):
+        """(idx_near_field, box_coordinates, box_indices), indices 1-based as in the reference."""
+        X = np.asfortranarray(coordinates, dtype=np.float64)
+        T = np.asfortranarray(target_coordinates, dtype=np.float64).reshape(-1, 3, order="F")
+        sizes = np.zeros(2, np.int64)
+        check(self._lib.fmmh_get_boxes(self._h, _p(X), X.shape[0], X.shape[0], _p(T), max(T.shape[0], 1),
+                                       T.shape[0], float(theta), int(ncrit), int(expansion_order), _p(sizes)))
+        near = np.zeros(max(sizes[0], 1), np.int32)
+        boxes = np.zeros((max(sizes[1], 1), 3), order="F")
+        idx = np.zeros(max(sizes[1], 1), np.int32)
+        check(self._lib.fmmh_get_boxes_export(self._h, _p(near), _p(boxes), _p(idx)))
+        return near[: sizes[0]], np.asfortranarray(boxes[: sizes[1]]), idx[: sizes[1]]
+
+    def get_multipoles(self, comm, coordinates, multipoles, target_coordinates, theta, ncrit, expansion_order):
+        """(idx_near_field, box_coordinates, box_multipoles); multipoles N x dim x ndens."""
+        X = np.asfortranarray(coordinates, dtype=np.float64)
+        M = np.asfortranarray(multipoles, dtype=np.float64)
+        if M.ndim == 2:
+            M = M.reshape(M.shape[0], M.shape[1], 1, order="F")
+        T = np.asfortranarray(target_coordinates, dtype=np.float64).reshape(-1, 3, order="F")
+        n, dim, nd = M.shape
+        sizes = np.zeros(2, np.int64)
+        check(self._lib.fmmh_get_multipoles(self._h, int(comm), _p(X), n, n, _p(M), n, dim, nd, _p(T),
+                                            max(T.shape[0], 1), T.shape[0], float(theta), int(ncrit),
+                                            int(expansion_order), _p(sizes)))
+        nf_, nb = int(sizes[0]), int(sizes[1])
+        near = np.zeros(max(nf_, 1), np.int32)
+        boxes = np.zeros((max(nb, 1), 3), order="F")
+        bm = np.zeros((max(nb, 1), ncomp(expansion_order), nd), order="F")
+        check(self._lib.fmmh_get_multipoles_export(self._h, _p(near), _p(boxes), _p(bm)))
+        return near[:nf_], np.asfortranarray(boxes[:nb]), np.asfortranarray(bm[:nb])
+
+    def interpolate_electron_fields(self, comm, coordinates, local_expansion, box_indices, theta, ncrit,
+                                    expansion_order, field_order, field=None):
+        """Field (N x nf x ndens) of local expansions placed on the given boxes (1-based indices)."""
+        X = np.asfortranarray(coordinates, dtype=np.float64)
+        L = np.asfortranarray(local_expansion, dtype=np.float64)
+        if L.ndim == 2:
+            L = L.reshape(L.shape[0], L.shape[1], 1, order="F")
+        B = np.ascontiguousarray(box_indices, dtype=np.int32)
+        n = X.shape[0]
+        nb, nm, nd = L.shape
+        if nm != ncomp(expansion_order) or B.size != nb:
+            raise ValueError("local_expansion must be nbox x ncomp(expansion_order) x ndens, one index per box")
+        F = np.zeros((n, ncomp(field_order), nd), order="F") if field is None else field
+        check(self._lib.fmmh_interpolate_electron_fields(self._h, int(comm), _p(X), n, n, _p(L), max(nb, 1), nb, nd,
+                                                         _p(B) if nb else None, float(theta), int(ncrit),
+                                                         int(expansion_order), int(field_order), _p(F), F.shape[0]))
         return F
 
     # -- device arrays (torch tensors already in HBM) --------------------------
@@ -331,3 +383,20 @@ def field_fmm(comm, coordinates, multipoles, exclusions, theta, ncrit, expansion
     """Drop-in for octree.field_fmm (src/fmm.F90:1354-1400)."""
     return _ctx().field_fmm(comm, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order,
                             damp_type, damping_factors, is_periodic, box_matrix)
+
+
+def get_multipoles(comm, coordinates, multipoles, target_coordinates, theta, ncrit, expansion_order):
+    """Drop-in for fmm.get_multipoles (src/fmm.F90:1171-1229)."""
+    return _ctx().get_multipoles(comm, coordinates, multipoles, target_coordinates, theta, ncrit, expansion_order)
+
+
+def get_boxes(coordinates, target_coordinates, theta, ncrit, expansion_order):
+    """Drop-in for fmm.get_boxes (src/fmm.F90:1231-1291)."""
+    return _ctx().get_boxes(coordinates, target_coordinates, theta, ncrit, expansion_order)
+
+
+def interpolate_electron_fields(comm, coordinates, local_expansion, box_indices, theta, ncrit, expansion_order,
+                                field_order):
+    """Drop-in for fmm.interpolate_electron_fields (src/fmm.F90:1293-1336)."""
+    return _ctx().interpolate_electron_fields(comm, coordinates, local_expansion, box_indices, theta, ncrit,
+                                              expansion_order, field_order)

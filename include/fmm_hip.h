@@ -161,6 +161,50 @@ int fmmh_field_fmm_sets_dev(fmmh_context* ctx, int32_t comm,
                             int32_t damp_type, const double* damping_factors, int64_t n_damping,
                             int32_t is_periodic, const double* box_matrix);
 
+/* ---- QM/MM embedding (src/fmm.F90:1134-1336) ---------------------------------
+ * The reference returns allocatable arrays; here each query is two calls on
+ * one context: the first builds the octree of `coordinates` (n x 3), runs
+ * classify_box against `target_coordinates` (ntargets x 3; a node is too
+ * close if norm2(target - centre) * theta <= rmax for any target, and close
+ * internal nodes are descended, src/fmm.F90:1134-1169) and returns
+ * sizes[2] = {num_near_field, num_far_field}; the export call then fills
+ * caller arrays of those sizes.  All arrays are host memory, column-major.
+ *   idx_near_field  num_near_field sites (1-based) of the close leaves, in
+ *                   node order and the reference's leaf member order
+ *   box_coordinates num_far_field x 3 centres of the far boxes, node order
+ *   box_indices     num_far_field 1-based node indices (pre-order), the
+ *                   numbering interpolate_electron_fields expects
+ *   box_multipoles  num_far_field x (p+1)(p+2)(p+3)/6 x ndens cell multipoles
+ *                   of the far boxes (P2M + M2M about the box centres) */
+
+/* Replaces get_boxes (src/fmm.F90:1231-1291). */
+int fmmh_get_boxes(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates, int32_t n,
+                   const double* target_coordinates, int64_t ld_targets, int32_t ntargets, double theta,
+                   int32_t ncrit, int32_t expansion_order, int64_t* sizes);
+int fmmh_get_boxes_export(fmmh_context* ctx, int32_t* idx_near_field, double* box_coordinates,
+                          int32_t* box_indices);
+
+/* Replaces get_multipoles (src/fmm.F90:1171-1229); multipoles is
+ * n x dim x ndens (one leading dimension), dim = (m+1)(m+2)(m+3)/6, m <= 2. */
+int fmmh_get_multipoles(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
+                        int32_t n, const double* multipoles, int64_t ld_multipoles, int32_t dim, int32_t ndens,
+                        const double* target_coordinates, int64_t ld_targets, int32_t ntargets, double theta,
+                        int32_t ncrit, int32_t expansion_order, int64_t* sizes);
+int fmmh_get_multipoles_export(fmmh_context* ctx, int32_t* idx_near_field, double* box_coordinates,
+                               double* box_multipoles);
+
+/* Replaces interpolate_electron_fields (src/fmm.F90:1293-1336): local
+ * expansions (nbox x (p+1)(p+2)(p+3)/6 x ndens) placed on the boxes
+ * box_indices (1-based, as get_boxes returns them; a later row for the same
+ * box wins), every other box zero, then L2L down the octree of `coordinates`
+ * and L2P into field (n x nf x ndens, accumulated, nf = ncomp(field_order)). */
+int fmmh_interpolate_electron_fields(fmmh_context* ctx, int32_t comm, const double* coordinates,
+                                     int64_t ld_coordinates, int32_t n, const double* local_expansion,
+                                     int64_t ld_local_expansion, int32_t nbox, int32_t ndens,
+                                     const int32_t* box_indices, double theta, int32_t ncrit,
+                                     int32_t expansion_order, int32_t field_order, double* field,
+                                     int64_t ld_field);
+
 /* ---- diagnostics (no counterpart in the reference) -------------------------
  * Spot-check direct sum: the field_direct result (src/fmm.F90:749-824) at the
  * ntargets sites targets[] (0-based, device memory like every other array),

@@ -152,3 +152,67 @@ def tree(coordinates, ncrit, theta):
     out["max_depth"] = md
     out["num_leaves"] = nl
     return out
+
+
+# ---------------------------------------------------------------- QM/MM embedding
+def get_boxes(coordinates, target_coordinates, theta, ncrit, expansion_order, impl="ref"):
+    """(idx_near_field, box_coordinates, box_indices), 1-based as the reference returns them."""
+    X = np.asfortranarray(coordinates, dtype=np.float64)
+    T = np.asfortranarray(target_coordinates, dtype=np.float64)
+    n, nt = X.shape[0], T.shape[0]
+    sizes = np.zeros(2, np.int32)
+    near = np.zeros(n, np.int32)
+    nodes = 8 * n + 8
+    bc = np.zeros((nodes, 3), order="F")
+    bi = np.zeros(nodes, np.int32)
+    if impl == "oracle":
+        _lib("liboracle.so").orc_get_boxes(n, _p(X), nt, _p(T), ctypes.c_double(theta), ncrit, nodes, _p(sizes),
+                                           _p(near), _p(bc), _p(bi))
+    else:
+        _lib("_ref/libfmmref.so").ref_get_boxes(n, _p(X), nt, _p(T), ctypes.c_double(theta), ncrit, expansion_order,
+                                                n, nodes, _p(sizes), _p(near), _p(bc), _p(bi))
+    return near[: sizes[0]].copy(), np.asfortranarray(bc[: sizes[1]]), bi[: sizes[1]].copy()
+
+
+def get_multipoles(coordinates, multipoles, target_coordinates, theta, ncrit, expansion_order, impl="ref"):
+    """(idx_near_field, box_coordinates, box_multipoles); multipoles N x dim x ndens."""
+    X = np.asfortranarray(coordinates, dtype=np.float64)
+    M = np.asfortranarray(multipoles, dtype=np.float64)
+    T = np.asfortranarray(target_coordinates, dtype=np.float64)
+    n, dim, nd = M.shape
+    nt = T.shape[0]
+    nm = ncomp(expansion_order)
+    sizes = np.zeros(2, np.int32)
+    near = np.zeros(n, np.int32)
+    nodes = 8 * n + 8
+    bc = np.zeros((nodes, 3), order="F")
+    bm = np.zeros((nodes, nm, nd), order="F")
+    if impl == "oracle":
+        _lib("liboracle.so").orc_get_multipoles(n, dim, nd, _p(X), _p(M), nt, _p(T), ctypes.c_double(theta), ncrit,
+                                                expansion_order, nodes, _p(sizes), _p(near), _p(bc), _p(bm))
+    else:
+        _lib("_ref/libfmmref.so").ref_get_multipoles(n, dim, nd, _p(X), _p(M), nt, _p(T), ctypes.c_double(theta),
+                                                     ncrit, expansion_order, n, nodes, _p(sizes), _p(near), _p(bc),
+                                                     _p(bm))
+    return near[: sizes[0]].copy(), np.asfortranarray(bc[: sizes[1]]), np.asfortranarray(bm[: sizes[1]])
+
+
+def interpolate_electron_fields(coordinates, local_expansion, box_indices, theta, ncrit, expansion_order,
+                                field_order, impl="ref", out=None):
+    """impl='ref' aborts in the reference's finalize after the field is
+    written (see tests/golden/make_golden.py); pass `out` in shared memory."""
+    X = np.asfortranarray(coordinates, dtype=np.float64)
+    L = np.asfortranarray(local_expansion, dtype=np.float64)
+    B = np.ascontiguousarray(box_indices, dtype=np.int32)
+    n = X.shape[0]
+    nb, nm, nd = L.shape
+    F = np.zeros((n, ncomp(field_order), nd), order="F") if out is None else out
+    if impl == "oracle":
+        _lib("liboracle.so").orc_interpolate_electron_fields(n, _p(X), nb, nm, nd, _p(L), _p(B),
+                                                             ctypes.c_double(theta), ncrit, expansion_order,
+                                                             field_order, _p(F))
+    else:
+        _lib("_ref/libfmmref.so").ref_interpolate_electron_fields(n, _p(X), nb, nm, nd, _p(L), _p(B),
+                                                                  ctypes.c_double(theta), ncrit, expansion_order,
+                                                                  field_order, _p(F))
+    return F

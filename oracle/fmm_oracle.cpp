@@ -784,6 +784,38 @@ static void local_to_particle(Tree& tr, const Inputs& in, int p, const std::vect
     }
 }
 
+// src/fmm.F90:1134-1169 (classify_box), recursive from the root.
+// type by pre-order id: 0 unused, 1 far box, 2 close leaf.
+static void classify_box(Tree& tr, int ci, const double* T, int ldt, int nt, double theta, std::vector<int>& type) {
+    Node& nd = tr.nodes[ci];
+    bool too_close = false;
+    for (int i = 0; i < nt; ++i) {
+        double d[3] = {T[i] - nd.c[0], T[ldt + i] - nd.c[1], T[2 * (size_t)ldt + i] - nd.c[2]};
+        double r = norm2_3(d);
+        if (r * theta <= nd.rmax) too_close = true;
+    }
+    if (too_close && !nd.terminal) {
+        for (int o = 0; o < 8; ++o)
+            if (tr.nodes[ci].child[o] >= 0) classify_box(tr, tr.nodes[ci].child[o], T, ldt, nt, theta, type);
+    } else if (too_close && nd.terminal) {
+        type[nd.idx] = 2;
+    } else {
+        type[nd.idx] = 1;
+    }
+}
+
+// near-field sites (1-based) of close leaves and far-box ids, in node order
+// (src/fmm.F90:1192-1227, 1247-1288)
+static void qm_lists(Tree& tr, const std::vector<int>& type, std::vector<int>& near, std::vector<int>& far) {
+    for (size_t id = 0; id < tr.order.size(); ++id) {
+        Node& nd = node_by_id(tr, (int)id);
+        if (type[id] == 2)
+            for (int s : nd.leaf) near.push_back(s + 1);
+        else if (type[id] == 1)
+            far.push_back((int)id);
+    }
+}
+
 }  // namespace orc
 
 // ================================================================ C ABI (ctypes)
@@ -842,6 +874,72 @@ int orc_field_fmm_periodic(int n, int dim, const double* X, const double* M, con
     orc::local_to_local(tr, p, LE);
     if (le_all) std::copy(LE.begin(), LE.end(), le_all);
     orc::local_to_particle(tr, in, p, LE);
+    return 0;
+}
+
+// get_boxes (src/fmm.F90:1231-1291): sizes = {num_near, num_far}; at most
+// cap far boxes are written (bc cap x 3, bi 1-based node ids); near holds n.
+int orc_get_boxes(int n, const double* X, int nt, const double* T, double theta, int ncrit, int cap, int* sizes,
+                  int* near, double* bc, int* bi) {
+    orc::Tree tr;
+    orc::octree_build(tr, X, n, n, ncrit, theta);
+    std::vector<int> type(tr.order.size(), 0), nr, fr;
+    orc::classify_box(tr, 0, T, nt, nt, theta, type);
+    orc::qm_lists(tr, type, nr, fr);
+    sizes[0] = (int)nr.size();
+    sizes[1] = (int)fr.size();
+    for (size_t k = 0; k < nr.size(); ++k) near[k] = nr[k];
+    for (size_t k = 0; k < fr.size() && (int)k < cap; ++k) {
+        orc::Node& nd = orc::node_by_id(tr, fr[k]);
+        for (int q = 0; q < 3; ++q) bc[(size_t)q * cap + k] = nd.c[q];
+        bi[k] = fr[k] + 1;
+    }
+    return 0;
+}
+
+// get_multipoles (src/fmm.F90:1171-1229): multipoles n x dim x nd; bm cap x ncomp(p) x nd.
+int orc_get_multipoles(int n, int dim, int nd, const double* X, const double* M, int nt, const double* T,
+                       double theta, int ncrit, int p, int cap, int* sizes, int* near, double* bc, double* bm) {
+    orc::Tree tr;
+    orc::octree_build(tr, X, n, n, ncrit, theta);
+    std::vector<int> type(tr.order.size(), 0), nr, fr;
+    orc::classify_box(tr, 0, T, nt, nt, theta, type);
+    orc::qm_lists(tr, type, nr, fr);
+    sizes[0] = (int)nr.size();
+    sizes[1] = (int)fr.size();
+    for (size_t k = 0; k < nr.size(); ++k) near[k] = nr[k];
+    int nm = orc::ncomp(p);
+    for (int d = 0; d < nd; ++d) {
+        orc::Inputs in{n, dim, 0, 0, 0, X, n, M + (size_t)d * dim * n, n, nullptr, n, nullptr, nullptr, n};
+        std::vector<double> CM;
+        orc::multipole_expansion(tr, in, p, CM);
+        for (size_t k = 0; k < fr.size() && (int)k < cap; ++k) {
+            orc::Node& ndd = orc::node_by_id(tr, fr[k]);
+            if (d == 0)
+                for (int q = 0; q < 3; ++q) bc[(size_t)q * cap + k] = ndd.c[q];
+            for (int t = 0; t < nm; ++t) bm[((size_t)d * nm + t) * cap + k] = CM[(size_t)fr[k] * nm + t];
+        }
+    }
+    return 0;
+}
+
+// interpolate_electron_fields (src/fmm.F90:1293-1336): L nb x nm x nd,
+// box ids 1-based; field n x ncomp(F) x nd accumulated.
+int orc_interpolate_electron_fields(int n, const double* X, int nb, int nm, int nd, const double* L, const int* B,
+                                    double theta, int ncrit, int p, int field_order, double* field) {
+    orc::Tree tr;
+    orc::octree_build(tr, X, n, n, ncrit, theta);
+    size_t nn = tr.order.size();
+    int nf = orc::ncomp(field_order);
+    for (int d = 0; d < nd; ++d) {
+        std::vector<double> LE(nn * nm, 0.0);
+        for (int i = 0; i < nb; ++i)
+            for (int t = 0; t < nm; ++t) LE[(size_t)(B[i] - 1) * nm + t] = L[((size_t)d * nm + t) * nb + i];
+        orc::local_to_local(tr, p, LE);
+        orc::Inputs in{n, 0, 0, field_order, 0, X, n, nullptr, n, nullptr, n, nullptr,
+                       field + (size_t)d * nf * n, n};
+        orc::local_to_particle(tr, in, p, LE);
+    }
     return 0;
 }
 

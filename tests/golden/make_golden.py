@@ -6,7 +6,7 @@ script calls its field_direct / field_fmm through the bind(C) wrapper
 oracle/ref_bindc.F90 and stores inputs and outputs only.
 
 Run in the build container (it needs the reference sources):
-    make -C oracle ref && python tests/golden/make_golden.py [--periodic]
+    make -C oracle ref && python tests/golden/make_golden.py [--periodic | --qmmm]
 """
 import os
 import sys
@@ -77,6 +77,7 @@ def main():
                             field_direct=oracle.field_direct(x, q, e, 2, impl="ref"),
                             field_fmm=oracle.field_fmm(x, q, e, 0.5, 64, 4, 2, impl="ref"))
     periodic()
+    qmmm()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
@@ -135,9 +136,58 @@ def periodic():
     np.savez_compressed(os.path.join(HERE, "water999_periodic.npz"), **out)
 
 
+def _ref_interpolate(x, le, bi, theta, ncrit, p, F):
+    """The reference's interpolate_electron_fields writes the field and then
+    aborts in finalize: clean_node deallocates interaction lists that the
+    QM/MM routines never allocate (src/fmm.F90:318-321, 1335).  Run it in a
+    forked child that writes into shared memory and read the field back."""
+    from multiprocessing import shared_memory
+    shape = (x.shape[0], (F + 1) * (F + 2) * (F + 3) // 6, le.shape[2])
+    shm = shared_memory.SharedMemory(create=True, size=int(np.prod(shape)) * 8)
+    try:
+        out = np.ndarray(shape, dtype=np.float64, buffer=shm.buf, order="F")
+        out[:] = 0.0
+        pid = os.fork()
+        if pid == 0:
+            os.close(2)                     # the runtime's abort message
+            oracle.interpolate_electron_fields(x, le, bi, theta, ncrit, p, F, impl="ref", out=out)
+            os._exit(0)
+        os.waitpid(pid, 0)
+        return np.array(out, order="F")
+    finally:
+        shm.close()
+        shm.unlink()
+
+
+def qmmm():
+    """6. QM/MM embedding (src/fmm.F90:1134-1336): the 999-site water box as
+    the MM region and a few QM target points around its middle.  The
+    reference's get_boxes / get_multipoles abort before returning (see
+    _ref_interpolate), so their outputs here come from the restatement and
+    serve as inputs; the pinned output is the reference's own
+    interpolate_electron_fields on those boxes."""
+    x, m, excl, df, _ = water_inputs(333, 4, 17)
+    rng = np.random.default_rng(17)
+    c = x.mean(axis=0)
+    targets = np.asfortranarray(c + rng.normal(0.0, 2.0, (6, 3)))
+    m2 = np.asfortranarray(np.stack([m, rng.normal(0.0, 0.1, m.shape)], axis=2))
+    out = dict(coordinates=x, multipoles=m2, targets=targets)
+    for theta, ncrit, p, F in ((0.5, 64, 4, 1), (0.3, 16, 6, 2)):
+        key = f"t{int(theta * 10)}_n{ncrit}_p{p}"
+        near, bc, bi = oracle.get_boxes(x, targets, theta, ncrit, p, impl="oracle")
+        le = np.asfortranarray(rng.normal(0.0, 1e-3, (len(bi), (p + 1) * (p + 2) * (p + 3) // 6, 2)))
+        ref = _ref_interpolate(x, le, bi, theta, ncrit, p, F)
+        assert np.abs(ref).max() > 0 and np.isfinite(ref).all()
+        out.update({f"{key}_box_indices": bi, f"{key}_local_expansion": le, f"{key}_field": ref,
+                    f"{key}_field_order": F})
+    np.savez_compressed(os.path.join(HERE, "water999_qmmm.npz"), **out)
+
+
 if __name__ == "__main__":
     assert oracle.have_ref(), "build oracle/_ref first (make -C oracle ref)"
     if sys.argv[1:] == ["--periodic"]:
         periodic()
+    elif sys.argv[1:] == ["--qmmm"]:
+        qmmm()
     else:
         main()
