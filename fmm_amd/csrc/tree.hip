@@ -35,47 +35,154 @@ __device__ __forceinline__ int key_digit(unsigned long long key, int d) {   // d
     return (int)((key >> (63 - 3 * d)) & 7ull);
 }
 
-// Sequential centroid, one workgroup per axis: sum(coordinates, 1) / N.
-// flang evaluates SUM left to right; a parallel reduction would round
-// differently and move node centres.
-// The chain of dependent adds is the floor; the loads must not be: waves
-// 1..3 stream the next chunk into LDS with coalesced vector loads while
-// wave 0 runs the chain over the current chunk out of LDS (broadcast reads,
-// prefetched one group ahead of the adds).
-constexpr int CENTROID_CHUNK = 2048;
+// Centroid, one workgroup per axis: sum(coordinates, 1) / N with flang's
+// left-to-right SUM, bit for bit (a tree built from another rounding of the
+// centroid differs from the reference's).
+//
+// The sequential chain is evaluated in blocks without giving up exactness.
+// While the running sum s stays in one binade [2^e, 2^{e+1}) (by magnitude),
+// every partial sum is an integer multiple of the unit u = 2^{e-52}, and
+// fl(s + x) = s + rint(x / u) u exactly whenever the result stays strictly
+// inside the binade and x / u is not a half-integer (a tie would be broken by
+// the parity of s).  So a block of CB_CHUNK addends becomes an exact int64
+// prefix sum of rint(x / u); the first addend whose partial sum leaves
+// (2^52, 2^53) units, or that is a tie, a non-finite value or too large,
+// ends the block: the prefix up to it is exact, that one addend is added in
+// FP64 as the reference does, and the next block starts after it with the
+// unit of the new binade.  Where blocks end early (a running sum that keeps
+// changing binade), a burst of CB_BURST addends is summed sequentially out of
+// LDS, so the cost never exceeds the plain chain by much.
+// Blocks are staged through LDS with coalesced loads (a thread then reads its
+// CB_PER consecutive addends; rows padded by one slot, conflict-free), and
+// the next block is prefetched into registers while the current one is
+// scanned.
+constexpr int CB_THREADS = 1024;
+constexpr int CB_PER = 8;
+constexpr int CB_CHUNK = CB_THREADS * CB_PER;
+constexpr int CB_LDS = CB_CHUNK + CB_CHUNK / CB_PER;
+constexpr int CB_MIN_PROGRESS = 64;
+constexpr int CB_BURST = 4096;
+static_assert(CB_BURST <= CB_LDS, "a burst is staged in the block buffer");
 
-__global__ __launch_bounds__(256) void k_centroid(const double* __restrict__ X, int64_t ldx, int n,
-                                                  double* __restrict__ root) {
+__global__ __launch_bounds__(CB_THREADS) void k_centroid(const double* __restrict__ X, int64_t ldx, int n,
+                                                         double* __restrict__ root) {
 #pragma clang fp contract(off)
-    __shared__ double buf[2][CENTROID_CHUNK];
-    const double* col = X + (int64_t)blockIdx.x * ldx;
-    const int tid = threadIdx.x;
-    const int nch = (n + CENTROID_CHUNK - 1) / CENTROID_CHUNK;
-    for (int k = tid; k < CENTROID_CHUNK && k < n; k += 256) buf[0][k] = col[k];
-    __syncthreads();
-    double s = 0.0;
-    for (int c = 0; c < nch; ++c) {
-        const int cur = c & 1;
-        if (tid >= 64) {
-            const int base = (c + 1) * CENTROID_CHUNK;
-            for (int k = tid - 64; k < CENTROID_CHUNK && base + k < n; k += 192) buf[1 - cur][k] = col[base + k];
-        } else {
-            const double* b = buf[cur];
-            const int len = min(CENTROID_CHUNK, n - c * CENTROID_CHUNK);
-            int k = 0;
-            if (len >= 8) {
-                double v0 = b[0], v1 = b[1], v2 = b[2], v3 = b[3], v4 = b[4], v5 = b[5], v6 = b[6], v7 = b[7];
-                for (k = 8; k + 8 <= len; k += 8) {
-                    const double w0 = b[k], w1 = b[k + 1], w2 = b[k + 2], w3 = b[k + 3];
-                    const double w4 = b[k + 4], w5 = b[k + 5], w6 = b[k + 6], w7 = b[k + 7];
-                    s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
-                    v0 = w0; v1 = w1; v2 = w2; v3 = w3; v4 = w4; v5 = w5; v6 = w6; v7 = w7;
-                }
-                s += v0; s += v1; s += v2; s += v3; s += v4; s += v5; s += v6; s += v7;
-            }
-            for (; k < len; ++k) s += b[k];
+    __shared__ double xs[CB_LDS];
+    __shared__ unsigned long long wtot[CB_THREADS / 64];
+    __shared__ int first_bad;
+    __shared__ double s_next;
+    const double* __restrict__ col = X + (int64_t)blockIdx.x * ldx;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    double pre[CB_PER];
+    int pre_pos = -1;
+    auto prefetch = [&](int p0) {
+#pragma unroll
+        for (int q = 0; q < CB_PER; ++q) {
+            const int k = p0 + q * CB_THREADS + tid;
+            pre[q] = k < n ? col[k] : 0.0;
+        }
+        pre_pos = p0;
+    };
+    double s = 0.0;   // uniform across the workgroup
+    int pos = 0;
+    while (pos < n) {
+        if (!(fabs(s) >= 0x1p-1022) || !isfinite(s)) {   // 0, subnormal or non-finite: no binade unit
+            s = s + col[pos];
+            ++pos;
+            continue;
+        }
+        const int e = ilogb(s);
+        const unsigned long long S0 = (unsigned long long)(long long)ldexp(s, 52 - e);
+        const int len = min(CB_CHUNK, n - pos);
+        if (pre_pos != pos) prefetch(pos);
+#pragma unroll
+        for (int q = 0; q < CB_PER; ++q) {
+            const int i = q * CB_THREADS + tid;
+            xs[i + i / CB_PER] = pre[q];
         }
         __syncthreads();
+        if (pos + len < n) prefetch(pos + len);   // the next block, if this one runs to its end
+        const int base = tid * CB_PER;
+        unsigned long long incl[CB_PER];
+        unsigned badmask = 0u;
+        unsigned long long run = 0ull;
+#pragma unroll
+        for (int q = 0; q < CB_PER; ++q) {
+            const int i = base + q;
+            unsigned long long inc = 0ull;
+            if (i < len) {
+                const double y = ldexp(xs[tid * (CB_PER + 1) + q], 52 - e);
+                bool bad = !(fabs(y) < 0x1p54);
+                if (!bad) {
+                    const double r = rint(y);
+                    bad = fabs(y - r) == 0.5;
+                    inc = bad ? 0ull : (unsigned long long)(long long)r;
+                }
+                if (bad) badmask |= 1u << q;
+            }
+            run += inc;
+            incl[q] = run;
+        }
+        // exclusive scan of the per-thread totals (unsigned: wraps only past the first invalid addend)
+        unsigned long long x = run;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned long long y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        if (tid == 0) first_bad = len;
+        if (lane == 63) wtot[wid] = x;
+        __syncthreads();
+        unsigned long long off = x - run;
+        for (int w = 0; w < wid; ++w) off += wtot[w];
+        int mine = len;
+#pragma unroll
+        for (int q = 0; q < CB_PER; ++q) {
+            const int i = base + q;
+            if (i < len && mine == len) {
+                const long long Sk = (long long)(S0 + off + incl[q]);
+                const long long a = Sk < 0 ? -Sk : Sk;
+                if (((badmask >> q) & 1u) || !(a > (1ll << 52) && a < (1ll << 53))) mine = i;
+            }
+        }
+        if (mine < len) atomicMin(&first_bad, mine);
+        __syncthreads();
+        const int f = first_bad;
+        // exact partial sum just before f (or at the end of the block)
+        const int last = (f >= len ? len : f) - 1;
+#pragma unroll
+        for (int q = 0; q < CB_PER; ++q)
+            if (base + q == last) s_next = ldexp((double)(long long)(S0 + off + incl[q]), e - 52);
+        __syncthreads();
+        if (last >= 0) s = s_next;
+        if (f >= len) {
+            pos += len;
+        } else {
+            s = s + xs[f + f / CB_PER];
+            pos += f + 1;
+            if (f < CB_MIN_PROGRESS && pos < n) {
+                const int blen = min(CB_BURST, n - pos);
+                __syncthreads();                 // every thread has read xs[f]
+                for (int k = tid; k < blen; k += CB_THREADS) xs[k] = col[pos + k];
+                __syncthreads();
+                if (tid == 0) {
+                    double t = s;
+                    int k = 0;
+                    for (; k + 8 <= blen; k += 8) {
+                        const double a0 = xs[k], a1 = xs[k + 1], a2 = xs[k + 2], a3 = xs[k + 3];
+                        const double a4 = xs[k + 4], a5 = xs[k + 5], a6 = xs[k + 6], a7 = xs[k + 7];
+                        t = t + a0; t = t + a1; t = t + a2; t = t + a3;
+                        t = t + a4; t = t + a5; t = t + a6; t = t + a7;
+                    }
+                    for (; k < blen; ++k) t = t + xs[k];
+                    s_next = t;
+                }
+                __syncthreads();
+                s = s_next;
+                pos += blen;
+            }
+        }
+        __syncthreads();   // xs, s_next and first_bad are rewritten by the next block
     }
     if (tid == 0) root[blockIdx.x] = s / (double)n;
 }
@@ -420,7 +527,7 @@ __global__ __launch_bounds__(256) void k_csr(const unsigned long long* __restric
 static inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
 
 void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, hipStream_t s) {
-    hipLaunchKernelGGL(k_centroid, dim3(3), dim3(256), 0, s, X, ldx, n, t.root);
+    hipLaunchKernelGGL(k_centroid, dim3(3), dim3(CB_THREADS), 0, s, X, ldx, n, t.root);
     FMMH_HIP(hipMemsetAsync(t.maxabs, 0, 3 * sizeof(unsigned long long), s));
     int g = std::min(blocks_for(n, 256), 1024);
     hipLaunchKernelGGL(k_maxabs, dim3(g), dim3(256), 0, s, X, ldx, n, t.root, t.maxabs);

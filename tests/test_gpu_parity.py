@@ -203,3 +203,37 @@ def test_direct_targets_match_full_direct(gpu_ctx):
     with pytest.raises(FmmError, match="target index"):
         bad = torch.as_tensor(np.array([0, x.shape[0]], np.int32), device=dev)
         gpu_ctx.field_direct_targets_device(col(x), col(m), col(e, torch.int32), bad, 2, col(np.zeros((2, 10))))
+
+
+def _adversarial_axis(kind, n, rng):
+    if kind == "uniform":
+        return rng.uniform(0.0, 300.0, n)
+    if kind == "centred":        # running sum keeps crossing binades and zero
+        return rng.uniform(-50.0, 50.0, n)
+    if kind == "quarters":       # x / ulp(s) lands on half-integers: ties broken by the parity of s
+        return np.floor(rng.uniform(0.0, 600.0, n)) / 2.0 + 0.25
+    if kind == "wide":           # magnitudes over 30 decades
+        return rng.normal(0.0, 1.0, n) * 10.0 ** rng.integers(-15, 15, n)
+    if kind == "cancel":         # partial sums far above the final sum
+        return np.where(rng.integers(0, 2, n) == 1, 1e16, -1e16 + 1.0)
+    return rng.uniform(0.0, 300.0, n) + 1e9   # offset: small addends on a large running sum
+
+
+@pytest.mark.parametrize("kind", ["uniform", "centred", "quarters", "wide", "cancel", "offset"])
+@pytest.mark.parametrize("n", [2, 17, 16385, 200001, 1048576])
+def test_root_centre_is_left_to_right_sum(gpu_ctx, kind, n):
+    """The root centre is sum(coordinates, 1) / N summed left to right, as
+    flang evaluates it (src/fmm.F90:184), bit for bit, whatever the data do
+    to the running sum (binade changes, ties, cancellation).  numpy's cumsum
+    is that sequential sum.  Degenerate inputs may not make a valid octree
+    (FMMH_ERR_TREE); the root centre is computed first either way and read
+    from the context's "t_root" buffer."""
+    rng = np.random.default_rng(n + len(kind))
+    x = np.asfortranarray(np.stack([_adversarial_axis(kind, n, rng) for _ in range(3)], 1))
+    try:
+        gpu_ctx.tree(x, min(64, n), 0.5)
+    except RuntimeError:
+        pass
+    got = gpu_ctx.debug_buffer("t_root", 3)
+    want = np.cumsum(x, axis=0)[-1] / n
+    assert np.array_equal(got.view(np.int64), want.view(np.int64)), (got, want)
