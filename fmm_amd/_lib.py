@@ -42,6 +42,15 @@ _PROTOS = {
     "fmmh_phase_times": (i32, [vp, vp, i32, _c.c_char_p, i32]),
     "fmmh_stats": (i32, [vp, vp, i32]),
     "fmmh_debug_buffer": (i32, [vp, _c.c_char_p, vp, i64]),
+    "fmmh_selector_train": (i32, [vp, vp, i64, vp, i64, i32, i32, f64, i32, i32, i32, i32, _c.c_uint64, vp]),
+    "fmmh_selector_samples": (i32, [vp, vp, vp, vp, vp]),
+    "fmmh_selector_loss": (i32, [vp, vp, i32]),
+    "fmmh_selector_nparam": (i32, []),
+    "fmmh_selector_get": (i32, [vp, vp, vp, vp]),
+    "fmmh_selector_set": (i32, [vp, vp, vp, vp]),
+    "fmmh_selector_grad": (i32, [vp, vp, vp, vp, vp, i32, _c.c_float, vp]),
+    "fmmh_selector_predict": (i32, [vp, vp, vp, vp, i32, vp]),
+    "fmmh_selection_stats": (i32, [vp, vp, i32]),
 }
 
 EXPORTED = tuple(_PROTOS)

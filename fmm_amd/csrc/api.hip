@@ -10,7 +10,9 @@
 //   P2M, M2M (level by level), P2P, M2L, L2L (level by level), L2P + scatter
 // Device buffers are pooled per context and only grow.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <random>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -102,6 +104,26 @@ struct fmmh_context {
     bool qm_multipoles = false;
     int cache_retries = 0;
     int list_retries = 0;
+    // Learned per-pair expansion orders: the trained MLP (device buffers
+    // "sel_W", "sel_norm"), its training range, and the per-call options.
+    bool sel_trained = false;
+    int sel_cap = 0;                  // expansion order the selector was trained at
+    float sel_rel_lo = 0.f, sel_rel_hi = 0.f;   // range of x2 - x3 seen in training
+    float sel_margin = 0.25f;         // FMMH_OPT_ORDER_MARGIN
+    double order_tol = 0.0;           // FMMH_OPT_ORDER_TOL: requested relative RMS field error (0: off)
+    double sel_kappa = 1.0;           // FMMH_OPT_ORDER_SAFETY
+    int spot_checks = 0;              // FMMH_OPT_SPOT_CHECKS
+    int sel_force_full = 0;           // FMMH_OPT_ORDER_FORCE_FULL: selection machinery on, every pair at p
+    // last call with selection: spot-check error, fallback flag, tolerance, order histogram
+    double sel_stats[8] = {-1, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long sel_hist[PMAX + 1] = {};
+    // last training run
+    std::vector<float> train_loss;
+    int train_nsamples = 0, train_cap = 0;
+    std::vector<int> train_pairs;     // [nsamples][2] target, source node (pre-order ids)
+    std::vector<float> train_feat;    // [nsamples][SEL_IN]
+    std::vector<double> train_err;    // [nsamples][cap + 1]
+    std::vector<double> train_fref;   // [nsamples]
     // last tree (for export)
     int num_nodes = 0, num_leaves = 0, max_depth = 0, n_m2l = 0, n_p2p = 0, n_sites = 0;
     std::vector<int> level_off;       // host copy, size KEY_LEVELS + 3
@@ -704,7 +726,7 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int ni
 // target-centric Hankel contraction of every set.  One host sync to size the
 // class table.  Many sets go through the MFMA form in chunks of up to 32 sets.
 void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets, const double* CM, double* LE,
-             bool classes_cached, const Periodic& P, const double* disp) {
+             bool classes_cached, const Periodic& P, const double* disp, const unsigned char* order = nullptr) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes, np = L.n_m2l;
     const int nm = ncomp(p), nT = ncomp(2 * p);
@@ -776,6 +798,8 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     c->mark("m2l_classes");
     const bool mfma = c->m2l_mfma == 2 ? m2l_mfma_supported(p, 16)
                                        : (c->m2l_mfma == 1 && m2l_mfma_supported(p, nsets));
+    if (order && c->m2l_mfma == 0)
+        throw Error(FMMH_ERR_UNSUPPORTED, "learned expansion orders need an MFMA M2L engine (FMMH_OPT_M2L_MFMA != 0)");
     const size_t node_stride = (size_t)nn * nm;
     // class-grouped GEMM: one set at a time, per-pair slots of np x nm doubles
     const bool gemm = !mfma && (c->m2l_mfma == 1 || c->m2l_mfma == 3) && p <= M2L_GEMM_PMAX && np > 0 &&
@@ -785,13 +809,45 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         const int* cls = c->buf<int>("m_cls", np);
         const int* pair_csr = c->buf<int>("m_idx_sorted", np);
         const int* class_dl = c->buf<int>("m_class_dl", np);
-        int* class_off = c->buf<int>("m_class_off", ncls + 1);
-        int* tile_cnt = c->buf<int>("m_tile_cnt", ncls + 1);
-        int* tile_off = c->buf<int>("m_tile_off", ncls + 1);
-        launch_m2l_tiles(flag, cls, np, ncls, class_off, tile_cnt, s);
-        size_t tbytes = exscan_i32_temp(ncls + 1);
-        exscan_i32(c->buf<char>("m_temp_tiles", tbytes), tbytes, tile_cnt, tile_off, ncls + 1, s);
-        const int grid = ncls + (np + M2L_GEMM_TN - 1) / M2L_GEMM_TN;   // >= tiles
+        const int* grp_class = nullptr;
+        const int* grp_order = nullptr;
+        int ngrp = ncls;
+        if (order) {
+            // tiles of one (class, order) group: re-sort the class-sorted pairs by (class, order)
+            unsigned long long* gk_in = c->buf<unsigned long long>("g_keys_in", np);
+            unsigned long long* gk = c->buf<unsigned long long>("g_keys", np);
+            int* gv_in = c->buf<int>("g_vals_in", np);
+            int* gv = c->buf<int>("g_vals", np);
+            int* gflag = c->buf<int>("g_flag", np);
+            int* gid = c->buf<int>("g_id", np);
+            int* gcls = c->buf<int>("g_class", np);
+            int* gord = c->buf<int>("g_order", np);
+            int* gcount = c->buf<int>("g_count", 1);
+            launch_sel_groups(pair_csr, pair_class, order, np, gk_in, gv_in, s);
+            int bits = 4;
+            while ((1ll << (bits - 4)) < (long long)ncls) ++bits;
+            size_t gb = std::max(sort_pairs_u64_temp(np), exscan_i32_temp(np));
+            void* gtemp = c->buf<char>("g_temp", gb);
+            sort_pairs_u64(gtemp, gb, gk_in, gk, gv_in, gv, np, bits, s);
+            launch_sel_group_flags(gk, np, gflag, s);
+            exscan_i32(gtemp, gb, gflag, gid, np, s);
+            launch_sel_group_info(gk, gflag, gid, np, gcls, gord, gcount, s);
+            FMMH_HIP(hipMemcpyAsync(c->h_pinned + 302, gcount, sizeof(int), hipMemcpyDeviceToHost, s));
+            c->sync();
+            ngrp = c->h_pinned[302];
+            flag = gflag;
+            cls = gid;
+            pair_csr = gv;
+            grp_class = gcls;
+            grp_order = gord;
+        }
+        int* class_off = c->buf<int>("m_class_off", ngrp + 1);
+        int* tile_cnt = c->buf<int>("m_tile_cnt", ngrp + 1);
+        int* tile_off = c->buf<int>("m_tile_off", ngrp + 1);
+        launch_m2l_tiles(flag, cls, np, ngrp, class_off, tile_cnt, s);
+        size_t tbytes = exscan_i32_temp(ngrp + 1);
+        exscan_i32(c->buf<char>("m_temp_tiles", tbytes), tbytes, tile_cnt, tile_off, ngrp + 1, s);
+        const int grid = ngrp + (np + M2L_GEMM_TN - 1) / M2L_GEMM_TN;   // >= tiles
         const bool f32 = c->precision == 1;
         double* pbuf = f32 ? nullptr : c->buf<double>("m_pbuf", (size_t)np * nm);
         float* pbuf32 = f32 ? c->buf<float>("m_pbuf32", (size_t)np * nm) : nullptr;
@@ -800,8 +856,9 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
         for (int set = 0; set < nsets; ++set) {
             if (f32) launch_to_f32(CMs + set * node_stride, cms32, (int64_t)node_stride, s);
-            M2LGemmArgs ga{class_off, tile_off, ncls, pair_csr, L.m2l_src, class_dl, Tc, CMs + set * node_stride,
-                           tb.gemm_fidx, tb.gemm_tdeg, pbuf, p, f32 ? 1 : 0, cms32, pbuf32};
+            M2LGemmArgs ga{class_off, tile_off, ngrp, pair_csr, L.m2l_src, class_dl, Tc, CMs + set * node_stride,
+                           tb.gemm_fidx, tb.gemm_tdeg, pbuf, p, f32 ? 1 : 0, cms32, pbuf32, grp_class, grp_order,
+                           order};
             launch_m2l_gemm(ga, grid, s);
             launch_m2l_reduce(ga, L.m2l_off, tr.t.r, LE + set * node_stride, nn, s);
         }
@@ -814,7 +871,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         const int s_ld = mfma ? (ns <= 16 ? 16 : 32) : 0;
         launch_scale_sources(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, s_ld, s);
         M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,
-                   nm, p, ns, s_ld, nn};
+                   nm, p, ns, s_ld, nn, order};
         if (mfma) launch_m2l_mfma(ma, nn, s);
         else launch_m2l(ma, nn, s);
     }
@@ -881,6 +938,86 @@ bool cached_tree_matches(fmmh_context* c, const Call& k, double theta, int ncrit
     return c->h_pinned[400] == 0;
 }
 
+// ---------------------------------------------------------------- learned orders
+// Per-pair expansion orders for this call (CSR positions): source norms,
+// the per-pair tolerance from the near-field RMS and the list sizes along
+// each site's path (on the device), then the MLP per pair.
+const unsigned char* select_orders(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets,
+                                   const double* CM, const double* near, int n, int nf) {
+    hipStream_t s = c->stream;
+    const int nn = tr.num_nodes, np = L.n_m2l, nm = ncomp(p);
+    double* node_norm = c->buf<double>("sel_node_norm", nn);
+    launch_sel_node_norm(CM, tr.t.r, tb.xyz, nn, nm, nsets, node_norm, s);
+    double* acc = c->buf<double>("sel_acc", 4);
+    FMMH_HIP(hipMemsetAsync(acc, 0, 4 * sizeof(double), s));
+    launch_sel_budget(near, n, n, nf, tr.t.leaf_node, tr.t.count, tr.t.parent, L.m2l_off, tr.num_leaves, acc,
+                      c->order_tol * c->sel_kappa, acc + 2, s);
+    unsigned char* order = c->buf<unsigned char>("sel_order", np);
+    unsigned long long* hist = c->buf<unsigned long long>("sel_hist", PMAX + 1);
+    FMMH_HIP(hipMemsetAsync(hist, 0, (PMAX + 1) * sizeof(unsigned long long), s));
+    SelOrderArgs oa{L.m2l_keys, np, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.r, node_norm, acc + 2,
+                    c->buf<float>("sel_W", SEL_NPARAM), c->buf<float>("sel_norm", 2 * SEL_IN), p, c->sel_cap,
+                    c->sel_force_full, c->sel_margin, c->sel_rel_lo, c->sel_rel_hi, order, hist};
+    launch_sel_orders(oa, s);
+    c->mark("select");
+    return order;
+}
+
+// Relative RMS error (field block; the potential for field order 0) of the
+// far field at spot sites against the direct sum with the reference's far
+// semantics.  FMM far field at a site = field - field before L2P - near.
+__global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__ dir, const int* __restrict__ site,
+                                                      const int* __restrict__ perm, const double* __restrict__ near,
+                                                      int64_t ldn, const double* __restrict__ field, int64_t ldf,
+                                                      const double* __restrict__ before, int64_t ldb, int nspots,
+                                                      int nf, double* __restrict__ out) {
+    __shared__ double red[2][256];
+    double e2 = 0.0, f2 = 0.0;
+    const int c0 = nf >= 4 ? 1 : 0, c1 = nf >= 4 ? 4 : 1;
+    for (int sp = threadIdx.x; sp < nspots; sp += 256) {
+        const int k = site[sp], o = perm[k];
+        for (int c = c0; c < c1; ++c) {
+            const double nr = near[(int64_t)c * ldn + k];
+            const double ffar = field[(int64_t)c * ldf + o] - before[(int64_t)c * ldb + o] - nr;
+            const double d = dir[(int64_t)sp * 4 + c];
+            e2 += (ffar - d) * (ffar - d);
+            f2 += (nr + d) * (nr + d);
+        }
+    }
+    red[0][threadIdx.x] = e2;
+    red[1][threadIdx.x] = f2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = red[0][0]; out[1] = red[1][0]; }
+}
+
+double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, const double* ys, const double* zs,
+                  const double* ms, int n, int m, int nf, const double* near, const Call& k, const double* before,
+                  int nspots) {
+    hipStream_t s = c->stream;
+    nspots = std::min(nspots, tr.num_leaves);
+    double* out = c->buf<double>("spot_out", (size_t)nspots * 4);
+    int* site = c->buf<int>("spot_site", nspots);
+    double* res = c->buf<double>("spot_res", 2);
+    SpotArgs sa{nspots, tr.num_leaves, n, tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, xs, ys, zs,
+                ms, n, out, site};
+    launch_spot_far(sa, m, s);
+    hipLaunchKernelGGL(k_spot_compare, dim3(1), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n, k.field,
+                       k.ldf, before, (int64_t)n, nspots, nf, res);
+    FMMH_HIP(hipGetLastError());
+    double* h = reinterpret_cast<double*>(c->h_pinned + 800);
+    FMMH_HIP(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    c->sync();
+    c->mark("spot_check");
+    return h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : 0.0;
+}
+
 void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, const Periodic& P) {
     hipStream_t s = c->stream;
     const double* disp = upload_disp(c, P);
@@ -936,18 +1073,76 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     launch_near(na, nl, m, k.F, k.damp, s);
     c->mark("p2p");
     const bool classes_cached = reuse && c->cache_class_p == p;
-    run_m2l(c, tr, L, tb, p, S, CM, LE, classes_cached, P, disp);
+    // learned per-pair orders (off: every pair at p, as the reference)
+    const bool select = c->order_tol > 0.0 && c->sel_trained && !P.on && L.n_m2l > 0;
+    const unsigned char* order = select ? select_orders(c, tr, L, tb, p, S, CM, near, n, nf) : nullptr;
+    run_m2l(c, tr, L, tb, p, S, CM, LE, classes_cached, P, disp, order);
     c->mark("m2l");
     if (P.on) run_periodic(c, tr, tb, P, k, p, CM, LE);
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
                 tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S};
-    for (int lev = 2; lev <= tr.max_depth; ++lev) {
-        int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
-        launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
+    auto downward = [&]() {
+        for (int lev = 2; lev <= tr.max_depth; ++lev) {
+            int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
+            launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
+        }
+        c->mark("l2l");
+        launch_l2p(da, nl, s);
+        c->mark("l2p");
+    };
+    const bool check = select && c->spot_checks > 0;
+    double* before = nullptr;
+    if (check) {
+        // the field before L2P, to isolate the far field at the spot sites and to restore on fallback
+        before = c->buf<double>("spot_before", (size_t)n * nf * S);
+        FMMH_HIP(hipMemcpy2DAsync(before, n * sizeof(double), k.field, k.ldf * sizeof(double), n * sizeof(double),
+                                  (size_t)nf * S, hipMemcpyDeviceToDevice, s));
     }
-    c->mark("l2l");
-    launch_l2p(da, nl, s);
-    c->mark("l2p");
+    downward();
+    c->sel_stats[0] = -1.0;
+    c->sel_stats[1] = 0.0;
+    if (check) {
+        const double err = spot_check(c, tr, L, xs, ys, zs, ms, n, m, nf, near, k, before, c->spot_checks);
+        c->sel_stats[0] = err;
+        if (!(err <= c->order_tol)) {
+            // the learned orders missed the tolerance: redo the far field at expansion_order
+            c->sel_stats[1] = 1.0;
+            FMMH_HIP(hipMemcpy2DAsync(k.field, k.ldf * sizeof(double), before, n * sizeof(double),
+                                      n * sizeof(double), (size_t)nf * S, hipMemcpyDeviceToDevice, s));
+            c->cache_classes = c->stats_classes;   // the class tensors of this call are in place
+            run_m2l(c, tr, L, tb, p, S, CM, LE, true, P, disp, nullptr);
+            c->mark("m2l_fallback");
+            downward();
+        }
+    }
+    if (select) {
+        unsigned long long* hh = reinterpret_cast<unsigned long long*>(c->h_pinned + 850);
+        FMMH_HIP(hipMemcpyAsync(hh, c->buf<unsigned long long>("sel_hist", PMAX + 1),
+                                (PMAX + 1) * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        double* ht = reinterpret_cast<double*>(c->h_pinned + 880);
+        FMMH_HIP(hipMemcpyAsync(ht, c->buf<double>("sel_acc", 4), 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+        c->sync();
+        double fl = 0.0, cnt = 0.0, qs = 0.0;
+        for (int q = 0; q <= PMAX; ++q) {
+            c->sel_hist[q] = hh[q];
+            fl += (double)hh[q] * 2.0 * ncomp(q) * ncomp(q) * S;
+            cnt += (double)hh[q];
+            qs += (double)hh[q] * q;
+        }
+        c->sel_stats[2] = ht[2];
+        c->sel_stats[3] = c->sel_stats[1] > 0.0 ? (double)L.n_m2l * 2.0 * nm * nm * S : fl;
+        c->sel_stats[4] = (double)L.n_m2l * 2.0 * nm * nm * S;
+        c->sel_stats[5] = cnt > 0 ? qs / cnt : 0.0;
+        c->sel_stats[6] = 1.0;
+        c->sel_stats[7] = std::sqrt(ht[0] / n);
+    } else {
+        for (int q = 0; q <= PMAX; ++q) c->sel_hist[q] = 0;
+        c->sel_stats[2] = 0.0;
+        c->sel_stats[3] = c->sel_stats[4] = (double)L.n_m2l * 2.0 * nm * nm * S;
+        c->sel_stats[5] = p;
+        c->sel_stats[6] = 0.0;
+        c->sel_stats[7] = 0.0;
+    }
     unsigned long long pairs = 0;
     if (reuse) {
         pairs = c->cache_pairs;
@@ -1559,6 +1754,20 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         } else if (key == FMMH_OPT_PRECISION) {
             if (value != 0.0 && value != 1.0) throw Error(FMMH_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
             c->precision = (int)value;
+        } else if (key == FMMH_OPT_ORDER_TOL) {
+            if (!(value >= 0.0)) throw Error(FMMH_ERR_ARG, "order tolerance must be >= 0");
+            c->order_tol = value;
+        } else if (key == FMMH_OPT_ORDER_MARGIN) {
+            if (!(value >= -1.0 && value <= 11.0)) throw Error(FMMH_ERR_ARG, "order margin must be in -1..11");
+            c->sel_margin = (float)value;
+        } else if (key == FMMH_OPT_SPOT_CHECKS) {
+            if (!(value >= 0.0 && value <= 65536.0)) throw Error(FMMH_ERR_ARG, "spot checks must be in 0..65536");
+            c->spot_checks = (int)value;
+        } else if (key == FMMH_OPT_ORDER_SAFETY) {
+            if (!(value > 0.0)) throw Error(FMMH_ERR_ARG, "order safety factor must be > 0");
+            c->sel_kappa = value;
+        } else if (key == FMMH_OPT_ORDER_FORCE_FULL) {
+            c->sel_force_full = value != 0.0 ? 1 : 0;
         } else if (key == FMMH_OPT_M2L_MFMA) {
             if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0)
                 throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1, 2 or 3");
@@ -1604,5 +1813,345 @@ int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap) {
         for (int i = 0; i < std::min(cap, 8); ++i) out[i] = c->stats[i];
     });
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- learned order selection
+namespace {
+
+// Training examples from the measured truncation errors: per sample, NTAU
+// tolerances spread (stratified, jittered) over log10 of its error curve;
+// the label is the fractional order where the suffix-maximum error curve
+// E_q = max_{q' >= q} err_{q'} crosses tau (log-linear between integer
+// orders), cap + 1 where even cap misses tau.
+constexpr int SEL_NTAU = 16;
+
+struct TrainSet {
+    std::vector<float> X, Y;
+    float rel_lo = 1e30f, rel_hi = -1e30f;
+};
+
+TrainSet build_examples(const std::vector<float>& feat, const std::vector<double>& err, int ns, int cap,
+                        unsigned long long seed) {
+    TrainSet t;
+    std::mt19937_64 rng(seed ^ 0x5eedull);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<double> env(cap + 1);
+    for (int s = 0; s < ns; ++s) {
+        const double* e = &err[(size_t)s * (cap + 1)];
+        bool ok = std::isfinite(feat[(size_t)s * SEL_IN + 2]);
+        for (int q = 0; q <= cap; ++q) ok = ok && std::isfinite(e[q]) && e[q] > 0.0;
+        if (!ok) continue;
+        double mx = 0.0;
+        for (int q = cap; q >= 0; --q) { mx = std::max(mx, e[q]); env[q] = mx; }
+        const double lo = std::log10(env[cap]) - 1.0, hi = std::log10(env[0]) + 0.5;
+        for (int k = 0; k < SEL_NTAU; ++k) {
+            const double lt = lo + (hi - lo) * (k + U(rng)) / SEL_NTAU;
+            double y = cap + 1.0;
+            for (int q = 0; q <= cap; ++q) {
+                if (std::log10(env[q]) <= lt) {
+                    if (q == 0) y = 0.0;
+                    else {
+                        const double l1 = std::log10(env[q - 1]), l2 = std::log10(env[q]);
+                        y = (q - 1) + (l1 > l2 ? (l1 - lt) / (l1 - l2) : 1.0);
+                    }
+                    break;
+                }
+            }
+            const float* f = &feat[(size_t)s * SEL_IN];
+            t.X.insert(t.X.end(), {f[0], f[1], f[2], (float)lt});
+            t.Y.push_back((float)y);
+            const float rel = f[2] - (float)lt;
+            t.rel_lo = std::min(t.rel_lo, rel);
+            t.rel_hi = std::max(t.rel_hi, rel);
+        }
+    }
+    return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmmh_selector_train(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates,
+                        const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, double theta,
+                        int32_t ncrit, int32_t cap, int32_t nsamples, int32_t epochs, uint64_t seed, double* info) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        hipStream_t s = c->stream;
+        const int m = order_from_dim(dim);
+        if (m < 0 || ncomp(m) != dim || m > NEAR_MMAX) throw Error(FMMH_ERR_ARG, "dim must be 1, 4 or 10");
+        check_fmm_args(theta, ncrit, cap, 0, nullptr);
+        if (cap < 1 || cap > M2L_GEMM_PMAX) throw Error(FMMH_ERR_ARG, "cap must be in 1..10");
+        if (n <= ncrit || ld_coordinates < n || ld_multipoles < n) throw Error(FMMH_ERR_ARG, "need n > ncrit");
+        if (nsamples < 1 || nsamples > (1 << 16)) throw Error(FMMH_ERR_ARG, "nsamples must be in 1..65536");
+        if (epochs < 1 || epochs > 100000) throw Error(FMMH_ERR_ARG, "epochs must be in 1..100000");
+        auto t0 = std::chrono::steady_clock::now();
+        double* X = c->buf<double>("h_X", (size_t)n * 3);
+        double* M = c->buf<double>("h_M", (size_t)n * dim);
+        FMMH_HIP(hipMemcpy2DAsync(X, n * sizeof(double), coordinates, ld_coordinates * sizeof(double),
+                                  n * sizeof(double), 3, hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpy2DAsync(M, n * sizeof(double), multipoles, ld_multipoles * sizeof(double),
+                                  n * sizeof(double), dim, hipMemcpyHostToDevice, s));
+        c->cache_valid = false;
+        TreeOut tr;
+        build_tree(c, X, n, n, ncrit, tr);
+        ListsOut L;
+        build_lists(c, tr, theta, L, 1, upload_disp(c, Periodic{}));
+        if (L.n_m2l == 0) throw Error(FMMH_ERR_ARG, "the tree has no M2L pairs to learn from");
+        Tables& tb = get_tables(c, cap, m);
+        const int nn = tr.num_nodes, nm = ncomp(cap);
+        double* xs = c->buf<double>("s_xs", n);
+        double* ys = c->buf<double>("s_ys", n);
+        double* zs = c->buf<double>("s_zs", n);
+        double* ms_raw = c->buf<double>("s_ms_raw", (size_t)n * dim);
+        double* ms = c->buf<double>("s_ms", (size_t)n * dim);
+        double* dfs = c->buf<double>("s_dfs", n);
+        int* excl = c->buf<int>("s_excl", n);
+        SiteArgs g{X, n, M, n, dim, nullptr, n, 0, nullptr, tr.t.perm, n, xs, ys, zs, ms_raw, ms, dfs, excl,
+                   tb.scale, 1};
+        launch_gather(g, s);
+        double* CM = c->buf<double>("e_cm", (size_t)nn * nm);
+        UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
+                  tr.t.children, CM, nm, cap, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, 1, dim};
+        launch_p2m(up, tr.num_leaves, s);
+        for (int lev = tr.max_depth - 1; lev >= 1; --lev)
+            launch_m2m(up, tr.t.level_nodes + tr.level_off[lev], tr.level_off[lev + 1] - tr.level_off[lev], s);
+        double* node_norm = c->buf<double>("sel_node_norm", nn);
+        launch_sel_node_norm(CM, tr.t.r, tb.xyz, nn, nm, 1, node_norm, s);
+        // sampled pairs: uniform over the M2L lists (seeded)
+        std::mt19937_64 rng(seed);
+        std::vector<int> pick(nsamples);
+        for (auto& v : pick) v = (int)(rng() % (unsigned long long)L.n_m2l);
+        int* dpick = c->buf<int>("sel_pick", nsamples);
+        FMMH_HIP(hipMemcpyAsync(dpick, pick.data(), nsamples * sizeof(int), hipMemcpyHostToDevice, s));
+        float* feat = c->buf<float>("sel_feat", (size_t)nsamples * SEL_IN);
+        double* err = c->buf<double>("sel_err", (size_t)nsamples * (cap + 1));
+        double* fref = c->buf<double>("sel_fref", nsamples);
+        SelSampleArgs sa{L.m2l_keys, dpick, nsamples, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.r, tr.t.start, tr.t.count,
+                         xs, ys, zs, ms, n, CM, nm, cap, tb.cs, tb.xyz, tb.level_base, node_norm, feat, err, fref};
+        launch_sel_sample(sa, m, s);
+        std::vector<unsigned long long> keys(nsamples);
+        c->train_feat.assign((size_t)nsamples * SEL_IN, 0.f);
+        c->train_err.assign((size_t)nsamples * (cap + 1), 0.0);
+        c->train_fref.assign(nsamples, 0.0);
+        std::vector<unsigned long long> all_keys(L.n_m2l);
+        FMMH_HIP(hipMemcpyAsync(c->train_feat.data(), feat, c->train_feat.size() * sizeof(float),
+                                hipMemcpyDeviceToHost, s));
+        FMMH_HIP(hipMemcpyAsync(c->train_err.data(), err, c->train_err.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+        FMMH_HIP(hipMemcpyAsync(c->train_fref.data(), fref, nsamples * sizeof(double), hipMemcpyDeviceToHost, s));
+        FMMH_HIP(hipMemcpyAsync(all_keys.data(), L.m2l_keys, L.n_m2l * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, s));
+        c->sync();
+        c->train_pairs.assign((size_t)nsamples * 2, 0);
+        for (int k = 0; k < nsamples; ++k) {
+            c->train_pairs[2 * k] = (int)(all_keys[pick[k]] >> 32);
+            c->train_pairs[2 * k + 1] = src_node((int)(all_keys[pick[k]] & 0xffffffffull));
+        }
+        c->train_nsamples = nsamples;
+        c->train_cap = cap;
+        auto t1 = std::chrono::steady_clock::now();
+        TrainSet ts = build_examples(c->train_feat, c->train_err, nsamples, cap, seed);
+        const int ne = (int)ts.Y.size();
+        if (ne < 64) throw Error(FMMH_ERR_ARG, "too few usable training samples");
+        // feature standardisation (model constants)
+        float norm[2 * SEL_IN];
+        for (int f = 0; f < SEL_IN; ++f) {
+            double mu = 0.0, m2 = 0.0;
+            for (int e = 0; e < ne; ++e) mu += ts.X[(size_t)e * SEL_IN + f];
+            mu /= ne;
+            for (int e = 0; e < ne; ++e) { const double d = ts.X[(size_t)e * SEL_IN + f] - mu; m2 += d * d; }
+            const double sd = std::sqrt(m2 / ne);
+            norm[f] = (float)mu;
+            norm[SEL_IN + f] = (float)(sd > 1e-12 ? 1.0 / sd : 1.0);
+        }
+        // He-uniform initial weights; the output bias starts at the mean label
+        std::vector<float> W(SEL_NPARAM, 0.f);
+        auto fill = [&](int off, int count, int fan_in) {
+            const double lim = std::sqrt(6.0 / fan_in);
+            std::uniform_real_distribution<double> u(-lim, lim);
+            for (int x = 0; x < count; ++x) W[off + x] = (float)u(rng);
+        };
+        fill(0, SEL_H * SEL_IN, SEL_IN);
+        fill(SEL_H * SEL_IN + SEL_H, SEL_H * SEL_H, SEL_H);
+        fill(SEL_H * SEL_IN + SEL_H + SEL_H * SEL_H + SEL_H, SEL_H, SEL_H);
+        double ym = 0.0;
+        for (float y : ts.Y) ym += y;
+        W[SEL_NPARAM - 1] = (float)(ym / ne);
+        const int batch = std::min(1024, (ne / 64) * 64);
+        const int nblk = batch / 64;
+        const int steps_per_epoch = std::max(1, ne / batch);
+        const int steps = epochs * steps_per_epoch;
+        float* dX = c->buf<float>("sel_tX", (size_t)ne * SEL_IN);
+        float* dY = c->buf<float>("sel_tY", ne);
+        float* dW = c->buf<float>("sel_W", SEL_NPARAM);
+        float* dnorm = c->buf<float>("sel_norm", 2 * SEL_IN);
+        float* part = c->buf<float>("sel_part", (size_t)nblk * (SEL_NPARAM + 1));
+        float* grad = c->buf<float>("sel_grad", SEL_NPARAM + 1);
+        float* am = c->buf<float>("sel_adam_m", SEL_NPARAM);
+        float* av = c->buf<float>("sel_adam_v", SEL_NPARAM);
+        int* step = c->buf<int>("sel_step", 1);
+        float* loss_log = c->buf<float>("sel_loss", steps);
+        FMMH_HIP(hipMemcpyAsync(dX, ts.X.data(), ts.X.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dY, ts.Y.data(), ts.Y.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dW, W.data(), SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dnorm, norm, sizeof(norm), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemsetAsync(am, 0, SEL_NPARAM * sizeof(float), s));
+        FMMH_HIP(hipMemsetAsync(av, 0, SEL_NPARAM * sizeof(float), s));
+        FMMH_HIP(hipMemsetAsync(step, 0, sizeof(int), s));
+        c->sync();
+        // a multiplier coprime to ne for the per-epoch sample permutation
+        unsigned long long pa = 2654435761ull % (unsigned long long)ne;
+        auto gcd = [](unsigned long long a, unsigned long long b) { while (b) { unsigned long long t = a % b; a = b; b = t; } return a; };
+        while (pa < 2 || gcd(pa, (unsigned long long)ne) != 1) ++pa;
+        MlpArgs ma{dX, dY, ne, dnorm, dW, part, grad, am, av, step, loss_log, batch, nblk, 1, pa,
+                   (unsigned long long)(seed % (unsigned long long)ne), 3e-3f, 0.9f, 0.999f, 1e-8f, 4.0f,
+                   std::max(1, steps / 4)};
+        // the training steps run as replays of a captured graph of `chunk` steps
+        // (the step counter lives on the device, so every replay advances it)
+        const int chunk = std::min(steps, 32);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        FMMH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < chunk; ++k) launch_mlp_train_step(ma, s);
+        FMMH_HIP(hipStreamEndCapture(s, &graph));
+        FMMH_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        const int replays = steps / chunk;
+        for (int r = 0; r < replays; ++r) FMMH_HIP(hipGraphLaunch(exec, s));
+        for (int k = replays * chunk; k < steps; ++k) launch_mlp_train_step(ma, s);
+        c->sync();
+        FMMH_HIP(hipGraphExecDestroy(exec));
+        FMMH_HIP(hipGraphDestroy(graph));
+        auto t2 = std::chrono::steady_clock::now();
+        c->train_loss.assign(steps, 0.f);
+        FMMH_HIP(hipMemcpy(c->train_loss.data(), loss_log, steps * sizeof(float), hipMemcpyDeviceToHost));
+        c->sel_trained = true;
+        c->sel_cap = cap;
+        c->sel_rel_lo = ts.rel_lo;
+        c->sel_rel_hi = ts.rel_hi;
+        double tail = 0.0;
+        const int nt = std::min(steps, 16);
+        for (int k = steps - nt; k < steps; ++k) tail += c->train_loss[k];
+        if (info) {
+            info[0] = ne;
+            info[1] = nsamples;
+            info[2] = c->train_loss[0];
+            info[3] = tail / nt;
+            info[4] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            info[5] = std::chrono::duration<double, std::milli>(t2 - t1).count();
+            info[6] = steps;
+            info[7] = replays;
+        }
+    });
+}
+
+int fmmh_selector_samples(fmmh_context* ctx, int32_t* pairs, float* features, double* errors, double* field_rms) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (c->train_nsamples == 0) throw Error(FMMH_ERR_ARG, "no training samples on this context");
+        if (pairs) std::copy(c->train_pairs.begin(), c->train_pairs.end(), pairs);
+        if (features) std::copy(c->train_feat.begin(), c->train_feat.end(), features);
+        if (errors) std::copy(c->train_err.begin(), c->train_err.end(), errors);
+        if (field_rms) std::copy(c->train_fref.begin(), c->train_fref.end(), field_rms);
+    });
+}
+
+int fmmh_selector_loss(fmmh_context* ctx, float* loss, int32_t cap) {
+    int count = 0;
+    int rc = guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        count = (int)c->train_loss.size();
+        for (int k = 0; k < std::min(count, (int)cap); ++k) loss[k] = c->train_loss[k];
+    });
+    return rc == FMMH_OK ? count : -rc;
+}
+
+int fmmh_selector_get(fmmh_context* ctx, float* weights, float* norm, double* meta) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (!c->sel_trained) throw Error(FMMH_ERR_ARG, "no trained order selector on this context");
+        c->sync();
+        FMMH_HIP(hipMemcpy(weights, c->buf<float>("sel_W", SEL_NPARAM), SEL_NPARAM * sizeof(float),
+                           hipMemcpyDeviceToHost));
+        FMMH_HIP(hipMemcpy(norm, c->buf<float>("sel_norm", 2 * SEL_IN), 2 * SEL_IN * sizeof(float),
+                           hipMemcpyDeviceToHost));
+        meta[0] = c->sel_cap;
+        meta[1] = c->sel_rel_lo;
+        meta[2] = c->sel_rel_hi;
+        meta[3] = c->sel_margin;
+    });
+}
+
+int fmmh_selector_set(fmmh_context* ctx, const float* weights, const float* norm, const double* meta) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (!(meta[0] >= 1 && meta[0] <= PMAX) || !(meta[1] <= meta[2]))
+            throw Error(FMMH_ERR_ARG, "selector meta must be {cap 1..10, rel_lo <= rel_hi, margin}");
+        FMMH_HIP(hipMemcpy(c->buf<float>("sel_W", SEL_NPARAM), weights, SEL_NPARAM * sizeof(float),
+                           hipMemcpyHostToDevice));
+        FMMH_HIP(hipMemcpy(c->buf<float>("sel_norm", 2 * SEL_IN), norm, 2 * SEL_IN * sizeof(float),
+                           hipMemcpyHostToDevice));
+        c->sel_cap = (int)meta[0];
+        c->sel_rel_lo = (float)meta[1];
+        c->sel_rel_hi = (float)meta[2];
+        c->sel_margin = (float)meta[3];
+        c->sel_trained = true;
+    });
+}
+
+int fmmh_selector_grad(fmmh_context* ctx, const float* weights, const float* norm, const float* features,
+                       const float* targets, int32_t ns, float asym, float* grad_loss) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (ns < 1 || ns > 65536) throw Error(FMMH_ERR_ARG, "ns must be in 1..65536");
+        hipStream_t s = c->stream;
+        const int nblk = (ns + 63) / 64;
+        float* dX = c->buf<float>("chk_X", (size_t)ns * SEL_IN);
+        float* dY = c->buf<float>("chk_Y", ns);
+        float* dW = c->buf<float>("chk_W", SEL_NPARAM);
+        float* dn = c->buf<float>("chk_norm", 2 * SEL_IN);
+        float* part = c->buf<float>("chk_part", (size_t)nblk * (SEL_NPARAM + 1));
+        float* grad = c->buf<float>("chk_grad", SEL_NPARAM + 1);
+        FMMH_HIP(hipMemcpyAsync(dX, features, (size_t)ns * SEL_IN * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dY, targets, ns * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dW, weights, SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dn, norm, 2 * SEL_IN * sizeof(float), hipMemcpyHostToDevice, s));
+        MlpArgs ma{dX, dY, ns, dn, dW, part, grad, nullptr, nullptr, nullptr, nullptr, ns, nblk, 0, 0, 0,
+                   0.f, 0.f, 0.f, 0.f, asym, 1};
+        launch_mlp_grad(ma, s);
+        FMMH_HIP(hipMemcpyAsync(grad_loss, grad, (SEL_NPARAM + 1) * sizeof(float), hipMemcpyDeviceToHost, s));
+        c->sync();
+    });
+}
+
+int fmmh_selector_predict(fmmh_context* ctx, const float* weights, const float* norm, const float* features,
+                          int32_t ns, float* out) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        if (ns < 1) throw Error(FMMH_ERR_ARG, "ns must be >= 1");
+        hipStream_t s = c->stream;
+        float* dX = c->buf<float>("chk_X", (size_t)ns * SEL_IN);
+        float* dY = c->buf<float>("chk_Y", ns);
+        float* dW = c->buf<float>("chk_W", SEL_NPARAM);
+        float* dn = c->buf<float>("chk_norm", 2 * SEL_IN);
+        FMMH_HIP(hipMemcpyAsync(dX, features, (size_t)ns * SEL_IN * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dW, weights, SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice, s));
+        FMMH_HIP(hipMemcpyAsync(dn, norm, 2 * SEL_IN * sizeof(float), hipMemcpyHostToDevice, s));
+        launch_sel_predict(dW, dn, dX, ns, dY, s);
+        FMMH_HIP(hipMemcpyAsync(out, dY, ns * sizeof(float), hipMemcpyDeviceToHost, s));
+        c->sync();
+    });
+}
+
+int fmmh_selection_stats(fmmh_context* ctx, double* out, int32_t cap) {
+    return guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        for (int i = 0; i < std::min(cap, 8); ++i) out[i] = c->sel_stats[i];
+        for (int q = 0; q <= PMAX && 8 + q < cap; ++q) out[8 + q] = (double)c->sel_hist[q];
+    });
+}
+
+int32_t fmmh_selector_nparam(void) { return SEL_NPARAM; }
 
 }  // extern "C"

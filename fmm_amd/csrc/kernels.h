@@ -199,6 +199,7 @@ struct M2LArgs {
     int nsets;
     int s_ld;
     int num_nodes;
+    const unsigned char* order;          // per CSR position: learned expansion order (null: p; MFMA form only)
 };
 void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
                           int nsets, int s_ld_out, hipStream_t s);
@@ -231,6 +232,14 @@ struct M2LGemmArgs {
     int f32;                             // mixed precision: FP32 MFMA on CMs32, slots in pbuf32
     const float* CMs32;
     float* pbuf32;
+    // Per-pair expansion orders (learned order selection): tiles are then
+    // runs of one (class, order) group; grp_class / grp_order give each
+    // group's class and order, and only the leading ncomp(order) rows and
+    // columns of the Hankel matrix are applied.  order[CSR position] tells the
+    // reduction how many coefficients a slot holds.  All null: every pair at p.
+    const int* grp_class;
+    const int* grp_order;
+    const unsigned char* order;
 };
 constexpr int M2L_GEMM_PMAX = 10;
 constexpr int M2L_GEMM_TN = 32;          // pairs per tile
@@ -239,6 +248,83 @@ void launch_m2l_tiles(const int* flag, const int* cls, int n_pairs, int ncls, in
 void launch_m2l_gemm(const M2LGemmArgs& a, int grid, hipStream_t s);
 void launch_m2l_reduce(const M2LGemmArgs& a, const int* off, const double* r, double* LE, int num_nodes,
                        hipStream_t s);
+
+// ---------------------------------------------------------------- learned order selection
+constexpr int SEL_IN = 4;                // MLP inputs
+constexpr int SEL_H = 64;                // units per hidden layer
+constexpr int SEL_NPARAM = SEL_H * SEL_IN + SEL_H + SEL_H * SEL_H + SEL_H + SEL_H + 1;
+constexpr int SEL_NS = 32;               // target sites per training sample
+
+struct SelSampleArgs {
+    const unsigned long long* pair_keys;  // CSR-sorted (target << 32 | source)
+    const int* sample_pair;               // per sample: CSR position of its pair
+    int nsamples;
+    const double* cx; const double* cy; const double* cz; const double* r;
+    const int* start; const int* count;
+    const double* xs; const double* ys; const double* zs;
+    const double* ms; int64_t ldm;        // sorted Taylor-scaled multipoles, component-major
+    const double* CM; int nm; int cap;    // cell multipoles at order cap (one set)
+    const double* cs;                     // Taylor factors of the expansion components
+    const int* xyz; const int* level_base;
+    const double* node_norm;              // A_j per node
+    float* feat;                          // [nsamples][SEL_IN] (x3 left 0)
+    double* err;                          // [nsamples][cap + 1] truncation error per order
+    double* fref;                         // [nsamples] RMS exact field at the sampled sites
+};
+void launch_sel_sample(const SelSampleArgs& a, int m, hipStream_t s);
+void launch_sel_node_norm(const double* CM, const double* r, const int* xyz, int num_nodes, int nm, int nsets,
+                          double* out, hipStream_t s);
+
+struct MlpArgs {
+    const float* X; const float* Y; int ns;   // raw features [ns][SEL_IN], targets [ns]
+    const float* norm;                        // [2 SEL_IN]: feature mean, 1 / std
+    float* W;                                 // [SEL_NPARAM]
+    float* part;                              // [nblk][SEL_NPARAM + 1] workgroup partial gradients (+ loss)
+    float* grad;                              // [SEL_NPARAM + 1] minibatch gradient (+ loss)
+    float* adam_m; float* adam_v;
+    int* step;                                // device step counter (null: 0)
+    float* loss_log;                          // per step: minibatch loss
+    int batch; int nblk;                      // samples per step, workgroups per step (64 samples each)
+    int use_perm;                             // 0: samples 0..batch-1; 1: permuted running position
+    unsigned long long perm_a, perm_c;
+    float lr, beta1, beta2, eps, asym;
+    int lr_half;                              // steps per halving of the step size
+};
+void launch_mlp_grad(const MlpArgs& a, hipStream_t s);        // part + grad
+void launch_mlp_train_step(const MlpArgs& a, hipStream_t s);  // grad, Adam, step + 1
+void launch_sel_predict(const float* W, const float* norm, const float* X, int ns, float* Y, hipStream_t s);
+
+struct SelOrderArgs {
+    const unsigned long long* pair_keys; int n_pairs;
+    const double* cx; const double* cy; const double* cz; const double* r;
+    const double* node_norm;
+    const double* tau;                        // device scalar: per-pair field tolerance
+    const float* W; const float* norm;
+    int p; int cap_train; int force_p;
+    float margin, rel_lo, rel_hi;             // training range of x2 - x3
+    unsigned char* order;                     // per CSR position
+    unsigned long long* hist;                 // [PMAX + 1] pairs per order
+};
+void launch_sel_orders(const SelOrderArgs& a, hipStream_t s);
+void launch_sel_groups(const int* pair_csr, const int* pair_class, const unsigned char* order, int n,
+                       unsigned long long* keys, int* vals, hipStream_t s);
+void launch_sel_group_flags(const unsigned long long* keys, int n, int* flag, hipStream_t s);
+void launch_sel_group_info(const unsigned long long* keys, const int* flag, const int* gid, int n, int* grp_class,
+                           int* grp_order, int* ngroups, hipStream_t s);
+void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, const int* leaf_node, const int* count,
+                       const int* parent, const int* m2l_off, int num_leaves, double* acc, double eps_kappa,
+                       double* tau, hipStream_t s);
+
+struct SpotArgs {
+    int nspots; int num_leaves; int n;
+    const int* leaf_node; const int* start; const int* count;
+    const int* p2p_off; const int* p2p_src;
+    const double* xs; const double* ys; const double* zs;
+    const double* ms; int64_t ldm;
+    double* out;                              // [nspots][4] far-field direct sum
+    int* site;                                // [nspots] sorted site index
+};
+void launch_spot_far(const SpotArgs& a, int m, hipStream_t s);
 void launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 
 struct DownArgs {

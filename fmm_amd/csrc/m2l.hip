@@ -402,6 +402,9 @@ __global__ __launch_bounds__(64) void k_m2l_mfma(M2LArgs a) {
         }
         const int j = src_node(a.src[e]);
         const int dl = li - a.level[j];
+        // learned order q of this pair: only the leading ncomp(q) rows and columns
+        const int nq = a.order ? ncomp(a.order[e]) : NM;
+        const int KSq = (nq + 3) / 4;
         double rs[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
@@ -409,15 +412,15 @@ __global__ __launch_bounds__(64) void k_m2l_mfma(M2LArgs a) {
         const double* __restrict__ B = a.CMs + (int64_t)j * NM * SLD;
         const double* __restrict__ T = Tl[cur];
 #pragma unroll 2
-        for (int ks = 0; ks < KS; ++ks) {
+        for (int ks = 0; ks < KSq; ++ks) {
             const int s = ks * 4 + kq;
             double b[NTL];
 #pragma unroll
-            for (int nt = 0; nt < NTL; ++nt) b[nt] = s < NM ? B[(int64_t)s * SLD + nt * 16 + row] : 0.0;
+            for (int nt = 0; nt < NTL; ++nt) b[nt] = s < nq ? B[(int64_t)s * SLD + nt * 16 + row] : 0.0;
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const int f = fidx[mt][ks][lane];
-                const double av = f >= 0 ? T[f] * rs[mt] : 0.0;
+                const double av = (f >= 0 && mt * 16 + row < nq) ? T[f] * rs[mt] : 0.0;
 #pragma unroll
                 for (int nt = 0; nt < NTL; ++nt)
                     acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[nt], acc[mt][nt], 0, 0, 0);

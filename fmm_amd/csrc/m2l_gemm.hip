@@ -96,12 +96,17 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
         if (a.tile_off[mid] <= b) lo = mid;
         else hi = mid;
     }
-    const int c = lo;
+    const int c = lo;                           // class, or (class, order) group
+    const int cls = a.grp_class ? a.grp_class[c] : c;
+    const int q = a.grp_order ? a.grp_order[c] : P;
+    const int nq = ncomp(q);                    // coefficients applied for this tile's pairs
+    const int MTq = (nq + 15) / 16, KSq = (nq + 3) / 4;
     const int k0 = a.class_off[c] + (b - a.tile_off[c]) * TN;
     const int cnt = min(TN, a.class_off[c + 1] - k0);
-    const int dl = a.class_dl[c];
+    const int dl = a.class_dl[cls];
     const int tid = threadIdx.x;
-    for (int x = tid; x < NT; x += 256) Ts[x] = a.Tc[(int64_t)c * NT + x];
+    const int ntq = ncomp(2 * q);
+    for (int x = tid; x < ntq; x += 256) Ts[x] = a.Tc[(int64_t)cls * NT + x];
     if (tid < TN) {
         const int e = tid < cnt ? a.pair_csr[k0 + tid] : -1;
         ecsr[tid] = e;
@@ -112,7 +117,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
     for (int x = tid; x < TN * NM; x += 256) {
         const int n = x / NM, s = x - n * NM;
         const int j = jsrc[n];
-        Bs[n * BLD + s] = j >= 0 ? cms[(int64_t)j * NM + s] : E(0);
+        Bs[n * BLD + s] = (j >= 0 && s < nq) ? cms[(int64_t)j * NM + s] : E(0);
     }
     __syncthreads();
     const int w = tid >> 6, lane = tid & 63;
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
         for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = typename G::Acc{0, 0, 0, 0};
     const int* __restrict__ fx = a.fidx + lane;
 #pragma unroll 2
-    for (int ks = 0; ks < KS; ++ks) {
+    for (int ks = 0; ks < KSq; ++ks) {
         const int s = ks * 4 + g4;
         E bv[NTL];
 #pragma unroll
@@ -139,9 +144,9 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
 #pragma unroll
         for (int mi = 0; mi < MW; ++mi) {
             const int mt = w + 4 * mi;
-            if (mt < MT) {
+            if (mt < MTq) {
                 const int f = fx[(mt * KS + ks) * 64];
-                const E av = f >= 0 ? E(Ts[f] * rs[mi]) : E(0);
+                const E av = (f >= 0 && mt * 16 + c16 < nq) ? E(Ts[f] * rs[mi]) : E(0);
 #pragma unroll
                 for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = G::mfma(av, bv[nt], acc[mi][nt]);
             }
@@ -150,16 +155,16 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
 #pragma unroll
     for (int mi = 0; mi < MW; ++mi) {
         const int mt = w + 4 * mi;
-        if (mt >= MT) continue;
+        if (mt >= MTq) continue;
 #pragma unroll
         for (int nt = 0; nt < NTL; ++nt) {
             const int n = nt * 16 + c16;
             if (n >= cnt) continue;
             E* __restrict__ out = (F32 ? (E*)a.pbuf32 : (E*)a.pbuf) + (int64_t)ecsr[n] * NM;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int t = mt * 16 + G::row(q, g4);
-                if (t < NM) out[t] = acc[mi][nt][q];
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const int t = mt * 16 + G::row(r4, g4);
+                if (t < nq) out[t] = acc[mi][nt][r4];
             }
         }
     }
@@ -167,16 +172,24 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
 
 // L_i(t) = r_i^{-(|t|+1)} sum over the CSR slots of target i, in CSR order
 // (FP64 accumulation for either slot type).
+// A slot of a pair with a learned order q holds only its first ncomp(q) coefficients.
 template <class E>
 __global__ __launch_bounds__(128) void k_m2l_reduce(const E* __restrict__ pbuf, const int* __restrict__ off,
                                                     const double* __restrict__ r, const int* __restrict__ tdeg,
+                                                    const unsigned char* __restrict__ order,
                                                     double* __restrict__ LE, int nm) {
     const int i = blockIdx.x;
     const int e0 = off[i], e1 = off[i + 1];
     const double rinv = 1.0 / r[i];
     for (int t = threadIdx.x; t < nm; t += 128) {
         double sum = 0.0;
-        for (int e = e0; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nm + t];
+        if (order) {
+            const int dt = tdeg[t];
+            for (int e = e0; e < e1; ++e)
+                if (dt <= order[e]) sum += (double)pbuf[(int64_t)e * nm + t];
+        } else {
+            for (int e = e0; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nm + t];
+        }
         double f = rinv;
         for (int k = 0; k < tdeg[t]; ++k) f *= rinv;
         LE[(int64_t)i * nm + t] = sum * f;
@@ -222,9 +235,11 @@ void launch_m2l_reduce(const M2LGemmArgs& a, const int* off, const double* r, do
     if (num_nodes <= 0) return;
     const int nm = ncomp(a.p);
     if (a.f32)
-        hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(num_nodes), dim3(128), 0, s, a.pbuf32, off, r, a.tdeg, LE, nm);
+        hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(num_nodes), dim3(128), 0, s, a.pbuf32, off, r, a.tdeg, a.order,
+                           LE, nm);
     else
-        hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(num_nodes), dim3(128), 0, s, a.pbuf, off, r, a.tdeg, LE, nm);
+        hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(num_nodes), dim3(128), 0, s, a.pbuf, off, r, a.tdeg, a.order,
+                           LE, nm);
     FMMH_HIP(hipGetLastError());
 }
 

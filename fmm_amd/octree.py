@@ -327,6 +327,87 @@ class Context:
         keys = names.value.decode().split(";") if names.value else []
         return dict(zip(keys, ms[:cnt].tolist()))
 
+    # -- learned per-pair expansion orders --------------------------------------
+    def train_selector(self, coordinates, multipoles, theta, ncrit, cap, nsamples=4096, epochs=40, seed=1):
+        """Measure M2L truncation errors on a sample system and train the order
+        selector on the GPU.  Returns a dict of training figures."""
+        X = np.asfortranarray(coordinates, dtype=np.float64)
+        M = np.asfortranarray(multipoles, dtype=np.float64)
+        n, dim = M.shape
+        info = np.zeros(8)
+        check(self._lib.fmmh_selector_train(self._h, _p(X), n, _p(M), n, n, dim, float(theta), int(ncrit), int(cap),
+                                            int(nsamples), int(epochs), int(seed), _p(info)))
+        keys = ("examples", "samples", "first_loss", "final_loss", "sample_ms", "train_ms", "steps",
+                "graph_replays")
+        return dict(zip(keys, info.tolist()))
+
+    def selector_samples(self, nsamples, cap):
+        pairs = np.zeros((nsamples, 2), np.int32)
+        feat = np.zeros((nsamples, 4), np.float32)
+        err = np.zeros((nsamples, cap + 1))
+        fref = np.zeros(nsamples)
+        check(self._lib.fmmh_selector_samples(self._h, _p(pairs), _p(feat), _p(err), _p(fref)))
+        return pairs, feat, err, fref
+
+    def selector_loss(self):
+        cnt = self._lib.fmmh_selector_loss(self._h, None, 0)
+        if cnt < 0:
+            check(-cnt)
+        out = np.zeros(max(cnt, 1), np.float32)
+        self._lib.fmmh_selector_loss(self._h, _p(out), cnt)
+        return out[:cnt]
+
+    def selector_weights(self):
+        W = np.zeros(self._lib.fmmh_selector_nparam(), np.float32)
+        norm = np.zeros(8, np.float32)
+        meta = np.zeros(4)
+        check(self._lib.fmmh_selector_get(self._h, _p(W), _p(norm), _p(meta)))
+        return W, norm, meta
+
+    def set_selector_weights(self, W, norm, meta):
+        W = np.ascontiguousarray(W, np.float32)
+        norm = np.ascontiguousarray(norm, np.float32)
+        meta = np.ascontiguousarray(meta, np.float64)
+        check(self._lib.fmmh_selector_set(self._h, _p(W), _p(norm), _p(meta)))
+
+    def selector_grad(self, W, norm, X, Y, asym=4.0):
+        """Mean loss and its gradient at W for the examples (X, Y), by the training kernels."""
+        W = np.ascontiguousarray(W, np.float32)
+        norm = np.ascontiguousarray(norm, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        out = np.zeros(W.size + 1, np.float32)
+        check(self._lib.fmmh_selector_grad(self._h, _p(W), _p(norm), _p(X), _p(Y), X.shape[0], float(asym),
+                                           _p(out)))
+        return float(out[-1]), out[:-1]
+
+    def selector_predict(self, W, norm, X):
+        W = np.ascontiguousarray(W, np.float32)
+        norm = np.ascontiguousarray(norm, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0], np.float32)
+        check(self._lib.fmmh_selector_predict(self._h, _p(W), _p(norm), _p(X), X.shape[0], _p(out)))
+        return out
+
+    def set_order_tolerance(self, tol, spot_checks=0, margin=None, safety=None, force_full=False):
+        """tol > 0: learned per-pair M2L orders for a relative RMS far-field error of about tol
+        (needs a trained selector); spot_checks > 0 validates each call and falls back to
+        expansion_order when the check fails."""
+        check(self._lib.fmmh_set_option(self._h, 6, float(tol)))
+        check(self._lib.fmmh_set_option(self._h, 8, float(spot_checks)))
+        if margin is not None:
+            check(self._lib.fmmh_set_option(self._h, 7, float(margin)))
+        if safety is not None:
+            check(self._lib.fmmh_set_option(self._h, 9, float(safety)))
+        check(self._lib.fmmh_set_option(self._h, 10, 1.0 if force_full else 0.0))
+
+    def selection_stats(self):
+        out = np.zeros(19)
+        check(self._lib.fmmh_selection_stats(self._h, _p(out), 19))
+        return dict(spot_error=float(out[0]), fallback=bool(out[1]), tau=float(out[2]), m2l_flops=float(out[3]),
+                    m2l_flops_full=float(out[4]), mean_order=float(out[5]), active=bool(out[6]),
+                    near_rms=float(out[7]), order_hist=out[8:].astype(np.int64))
+
     def stats(self):
         out = np.zeros(8, dtype=np.int64)
         check(self._lib.fmmh_stats(self._h, _p(out), 8))

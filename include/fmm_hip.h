@@ -250,11 +250,27 @@ enum fmmh_option {
                                      dipole iterations of a polarizable embedding, for which the reference
                                      rebuilds every call (src/fmm.F90:1387-1398).  Results are identical
                                      to a rebuild.  0 (default) drops the cache. */
-    FMMH_OPT_PRECISION = 5        /* 0 (default): FP64 throughout.  1: mixed -- the multipole-to-local
+    FMMH_OPT_PRECISION = 5,       /* 0 (default): FP64 throughout.  1: mixed -- the multipole-to-local
                                      translations of the class-grouped M2L run on FP32 MFMA with FP32
                                      accumulation and an FP64 sum into the local expansions; P2M, L2P,
                                      the near field (with its damped tensors) and the field stay FP64.
                                      Results then differ from the reference at the FP32 level. */
+    /* Learned per-pair expansion orders (no counterpart in the reference, whose M2L applies
+     * expansion_order to every pair).  With a trained selector on the context
+     * (fmmh_selector_train / fmmh_selector_set) and ORDER_TOL > 0, field_fmm picks an order
+     * q <= expansion_order for every M2L pair so that the far field keeps a relative RMS error
+     * of about ORDER_TOL; without a trained selector, or for periodic calls, every pair keeps
+     * expansion_order. */
+    FMMH_OPT_ORDER_TOL = 6,       /* requested relative RMS field error of the far field (0 = off, default) */
+    FMMH_OPT_ORDER_MARGIN = 7,    /* added to the predicted order before rounding up (default 0.25) */
+    FMMH_OPT_SPOT_CHECKS = 8,     /* > 0: after the call, compare the far field at this many sites (the
+                                     first site of evenly spaced leaves) against direct sums with the
+                                     reference's far semantics (undamped, no exclusions); if the relative
+                                     RMS error exceeds ORDER_TOL, redo M2L..L2P at expansion_order
+                                     (fallback).  0 (default): no check. */
+    FMMH_OPT_ORDER_SAFETY = 9,    /* factor on the per-pair tolerance (default 1) */
+    FMMH_OPT_ORDER_FORCE_FULL = 10 /* != 0: run the selection machinery but keep every pair at
+                                     expansion_order (ablation) */
 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
@@ -268,6 +284,55 @@ int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t b
 /* Counters of the last field_fmm call: {num_nodes, num_leaves, max_depth,
  * n_m2l, n_p2p, p2p_site_pairs, m2l_classes, list_retries}. */
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap);
+
+/* ---- learned per-pair expansion orders (no counterpart in the reference) ------
+ * A multilayer perceptron (4 inputs, 2 hidden layers of 64 ReLU units, FP32) maps
+ * an M2L pair (i, j) to the expansion order its contribution needs:
+ *   x0 = r_i / |R|, x1 = r_j / |R|   (box half-widths over the centre distance R)
+ *   x2 = log10(A_j / |R|^2)          (A_j = 2-norm of M_j(s) r_j^{-|s|}, |s| <= p, max over sets)
+ *   x3 = log10(tau)                  (the pair's absolute field tolerance)
+ * and the call uses q = ceil(y + margin) clamped to [0, expansion_order].
+ *
+ * fmmh_selector_train builds the octree of a sample system (host arrays,
+ * column-major n x 3 coordinates and n x dim multipoles, dim 1/4/10), runs the
+ * upward pass at order cap, samples nsamples M2L pairs, measures each pair's
+ * truncation error at every order 0..cap against the direct sum over its source
+ * box at up to 32 sites of its target box (on the GPU), turns the error curves
+ * into examples, and trains the network on the GPU (Adam, asymmetric squared
+ * loss that charges under-predicted orders 4x) for `epochs` passes.  info[8] =
+ * {examples, samples, first-step loss, final loss, sampling ms, training ms,
+ * steps, graph replays}. */
+int fmmh_selector_train(fmmh_context* ctx, const double* coordinates, int64_t ld_coordinates,
+                        const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim, double theta,
+                        int32_t ncrit, int32_t cap, int32_t nsamples, int32_t epochs, uint64_t seed, double* info);
+/* Samples of the last training run: pairs[ns][2] (target, source node, 0-based
+ * pre-order ids of the octree of the training system), features[ns][4] (x3 = 0),
+ * errors[ns][cap + 1], field_rms[ns] (RMS exact field at the sampled sites).
+ * Any pointer may be NULL. */
+int fmmh_selector_samples(fmmh_context* ctx, int32_t* pairs, float* features, double* errors, double* field_rms);
+/* Per-step minibatch loss of the last training run; returns the step count. */
+int fmmh_selector_loss(fmmh_context* ctx, float* loss, int32_t cap);
+int32_t fmmh_selector_nparam(void);
+/* Trained weights (fmmh_selector_nparam() floats: W1[64][4], b1[64], W2[64][64],
+ * b2[64], W3[64], b3), feature normalisation norm[8] (mean[4], 1/std[4]) and
+ * meta[4] = {cap, lowest x2 - x3, highest x2 - x3 of the training set, margin};
+ * pairs with x2 - x3 above that range keep expansion_order. */
+int fmmh_selector_get(fmmh_context* ctx, float* weights, float* norm, double* meta);
+int fmmh_selector_set(fmmh_context* ctx, const float* weights, const float* norm, const double* meta);
+/* Loss and gradient of the training objective for one batch of ns examples
+ * (features[ns][4], targets[ns]) at the given weights: grad_loss[nparam] =
+ * gradient, grad_loss[nparam] = mean loss.  Same kernels as training. */
+int fmmh_selector_grad(fmmh_context* ctx, const float* weights, const float* norm, const float* features,
+                       const float* targets, int32_t ns, float asym, float* grad_loss);
+/* Network outputs y for ns raw feature rows (the inference forward pass). */
+int fmmh_selector_predict(fmmh_context* ctx, const float* weights, const float* norm, const float* features,
+                          int32_t ns, float* out);
+/* Selection figures of the last field_fmm call: out[0] spot-check relative
+ * error (-1: not checked), [1] 1 if it fell back to expansion_order, [2] per-pair
+ * tolerance tau, [3] M2L flops at the chosen orders (2 n_m(q)^2 per pair and set),
+ * [4] M2L flops at expansion_order, [5] mean order, [6] 1 if selection ran,
+ * [7] RMS near field used for tau, [8 + q] pairs at order q (q = 0..10). */
+int fmmh_selection_stats(fmmh_context* ctx, double* out, int32_t cap);
 
 #ifdef __cplusplus
 }
