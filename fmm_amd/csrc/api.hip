@@ -379,6 +379,11 @@ __global__ void k_pair_count(const int* off, const int* src, const int* leaf_nod
     atomicAdd(total, s * (unsigned long long)count[node]);
 }
 
+__global__ void k_invert_perm(const int* __restrict__ perm, int* __restrict__ inv, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) inv[perm[k]] = k;
+}
+
 __global__ void k_fill_i32(int* a, int v, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -1068,8 +1073,10 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     }
     c->mark("upward");
     double* near = c->buf<double>("e_near", (size_t)n * nf * S);
+    int* inv = c->buf<int>("s_inv", n);
+    hipLaunchKernelGGL(k_invert_perm, dim3(nblk(n, 256)), dim3(256), 0, s, tr.t.perm, inv, n);
     NearArgs na{tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
-                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp};
+                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp, inv};
     launch_near(na, nl, m, k.F, k.damp, s);
     c->mark("p2p");
     const bool classes_cached = reuse && c->cache_class_p == p;

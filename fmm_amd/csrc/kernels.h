@@ -31,6 +31,7 @@ struct NearArgs {
     int accumulate;            // 0: out = result, 1: out += result
     int nsets;                 // multipole sets: ms is [set][component][ldm], out is [set][component][ldo]
     const double* disp;        // per image id: source displacement (x, y, z); image 0 = (0, 0, 0)
+    const int* inv;            // original (0-based) site index -> sorted position (null: identity)
 };
 
 bool near_supported(int m, int F);

@@ -10,14 +10,20 @@
 // leaves the -q/(sqrt(3) 1e10) potential artefact in place.  Periodic image
 // sources are displaced as x_i - (x_j + D) and never excluded
 // (src/fmm.F90:711-721: exclusions apply to the central cell only).
+// They are applied outside the pair loop: the loop evaluates every pair at
+// its actual displacement except a site with itself (one integer compare),
+// and k_near_excl then adds, per target site and distinct exclusion entry j
+// of its near field, the pair at (1e10, 1e10, 1e10) minus the pair at its
+// actual displacement (for j = i only the former); a site that does not
+// exclude itself gets its self pair at displacement 0, i.e. a non-finite
+// field, as in the reference.
 #include "common.h"
 #include "tensors.h"
 #include "kernels.h"
 
 namespace fmmh {
 
-constexpr int NEAR_EXCL_REG = 4;   // exclusions kept in registers (unused ones hold -1); more are read from memory
-constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: matches no exclusion entry (0..n) or -1
+constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: never the target's own id
 
 // Lane layout: a target block of nt <= 64 sites is replicated over
 // r = 64 / nt lane groups (group g = lane / nt, target lane % nt); group g
@@ -72,9 +78,7 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 
         const int k = t0 + base + (active ? ti : 0);
         const double xi = a.xs[k], yi = a.ys[k], zi = a.zs[k];
         const double dfi = DAMP != DAMP_NONE ? a.dfs[k] : 0.0;
-        int ex[NEAR_EXCL_REG];
-#pragma unroll
-        for (int q = 0; q < NEAR_EXCL_REG; ++q) ex[q] = q < a.nexcl ? a.excl[(int64_t)q * a.lde + k] : -1;
+        const int myo = a.orig[k] + 1;           // 1-based original index of the target
         double acc[S][NF];
 #pragma unroll
         for (int u = 0; u < S; ++u)
@@ -155,28 +159,34 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 
                 if (vbase + 64 < total) fetch(vbase + 64);
                 if (active) {
                     for (int jj = g; jj < cn; jj += r) {
-                        double dx = xi - sx[jj];
-                        double dy = yi - sy[jj];
-                        double dz = zi - sz[jj];
-                        const int oj = sorig[jj];
-                        bool excluded = false;
-#pragma unroll
-                        for (int q = 0; q < NEAR_EXCL_REG; ++q) excluded |= (ex[q] == oj);
-                        for (int q = NEAR_EXCL_REG; q < a.nexcl; ++q)
-                            excluded |= (a.excl[(int64_t)q * a.lde + k] == oj);
-                        if (excluded) { dx = 1e10; dy = 1e10; dz = 1e10; }
+                        const double dx = xi - sx[jj];
+                        const double dy = yi - sy[jj];
+                        const double dz = zi - sz[jj];
+                        const double R2 = dx * dx + dy * dy + dz * dz;
+                        // the self pair is left to k_near_excl: 1/R = 0 makes its terms exactly 0
+                        const double inv = sorig[jj] == myo ? 0.0 : rsqrt(R2);
                         double aij = 0.0;
                         if constexpr (DAMP != DAMP_NONE) aij = dfi * sdf[jj];
                         double gr[K + 1];
-                        radial<K, DAMP>(dx * dx + dy * dy + dz * dz, aij, gr);
-                        double T[ncomp(K)];
-                        md_tensor<K>(dx, dy, dz, gr, T);
+                        radial_inv<K, DAMP>(R2, inv, aij, gr);
+                        if constexpr (has_closed_form<M, F>()) {
 #pragma unroll
-                        for (int u = 0; u < S; ++u) {
-                            double m[NM];
+                            for (int u = 0; u < S; ++u) {
+                                double m[NM];
 #pragma unroll
-                            for (int s = 0; s < NM; ++s) m[s] = sm[u][s][jj];
-                            contract<M, F>(m, T, acc[u]);
+                                for (int s = 0; s < NM; ++s) m[s] = sm[u][s][jj];
+                                contract_closed<M, F>(dx, dy, dz, gr, m, acc[u]);
+                            }
+                        } else {
+                            double T[ncomp(K)];
+                            md_tensor<K>(dx, dy, dz, gr, T);
+#pragma unroll
+                            for (int u = 0; u < S; ++u) {
+                                double m[NM];
+#pragma unroll
+                                for (int s = 0; s < NM; ++s) m[s] = sm[u][s][jj];
+                                contract<M, F>(m, T, acc[u]);
+                            }
                         }
                     }
                 }
@@ -205,6 +215,80 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 
     }
 }
 
+// Exclusions of the near field (see the header comment).  One lane per
+// target site of a target block; the exclusion entries of the site are
+// de-duplicated (the reference tests membership), and an entry counts only
+// if its site lies in a central-cell source block of the block's list.
+template <int M, int F, int DAMP, int S>
+__global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
+    constexpr int NM = ncomp(M);
+    constexpr int NF = ncomp(F);
+    const int b = a.blk[blockIdx.x];
+    const int set0 = blockIdx.y * S;
+    const int ns = min(S, a.nsets - set0);
+    const int t0 = a.tgt_start[b], tc = a.tgt_count[b];
+    const int e0 = a.list_off[b], e1 = a.list_off[b + 1];
+    const double* __restrict__ ms = a.ms + (int64_t)set0 * NM * a.ldm;
+    double* __restrict__ out = a.out + (int64_t)set0 * NF * a.ldo;
+    for (int base = 0; base < tc; base += 64) {
+        const int jt = base + (int)threadIdx.x;
+        if (jt >= tc) break;
+        const int k = t0 + jt;
+        const int myo = a.orig[k] + 1;
+        const double xi = a.xs[k], yi = a.ys[k], zi = a.zs[k];
+        const double dfi = DAMP != DAMP_NONE ? a.dfs[k] : 0.0;
+        double acc[S][NF];
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+#pragma unroll
+            for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
+        auto add = [&](int j, double dx, double dy, double dz, double sign) {
+            double aij = 0.0;
+            if constexpr (DAMP != DAMP_NONE) aij = dfi * a.dfs[j];
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                if (u >= ns) break;
+                double m[NM];
+#pragma unroll
+                for (int s = 0; s < NM; ++s) m[s] = sign * ms[((int64_t)u * NM + s) * a.ldm + j];
+                pair_field<M, F, DAMP>(dx, dy, dz, dx * dx + dy * dy + dz * dz, aij, m, acc[u]);
+            }
+        };
+        bool self = false;
+        for (int q = 0; q < a.nexcl; ++q) {
+            const int e = a.excl[(int64_t)q * a.lde + k];
+            if (e <= 0) continue;
+            bool dup = false;
+            for (int q2 = 0; q2 < q; ++q2) dup |= a.excl[(int64_t)q2 * a.lde + k] == e;
+            if (dup) continue;
+            if (e == myo) {
+                self = true;
+                add(k, 1e10, 1e10, 1e10, 1.0);
+                continue;
+            }
+            const int j = a.inv ? a.inv[e - 1] : e - 1;   // sorted position of the excluded site
+            bool near = false;
+            for (int x = e0; x < e1 && !near; ++x) {
+                const int enc = a.list_src[x];
+                if (src_image(enc) != 0) continue;
+                const int sb = src_node(enc);
+                const int st = a.src_start[sb];
+                near = j >= st && j < st + a.src_count[sb];
+            }
+            if (!near) continue;
+            add(j, 1e10, 1e10, 1e10, 1.0);
+            add(j, xi - a.xs[j], yi - a.ys[j], zi - a.zs[j], -1.0);
+        }
+        if (!self) add(k, 0.0, 0.0, 0.0, 1.0);   // the reference's self pair: non-finite
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            if (u >= ns) break;
+#pragma unroll
+            for (int t = 0; t < NF; ++t) out[((int64_t)u * NF + t) * a.ldo + k] += acc[u][t];
+        }
+    }
+}
+
 // Sets per workgroup row: 4 where the tensor is cheap in registers, else 2.
 template <int M, int F, int DAMP>
 constexpr int near_sets() { return (M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 2; }
@@ -213,9 +297,11 @@ template <int M, int F, int DAMP>
 static void launch_sets(const NearArgs& a, int nblocks, hipStream_t s) {
     if (a.nsets == 1) {
         hipLaunchKernelGGL((k_near<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_near_excl<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
     } else {
         constexpr int S = near_sets<M, F, DAMP>();
         hipLaunchKernelGGL((k_near<M, F, DAMP, S>), dim3(nblocks, (a.nsets + S - 1) / S), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S>), dim3(nblocks, (a.nsets + S - 1) / S), dim3(64), 0, s, a);
     }
 }
 

@@ -81,10 +81,10 @@ __device__ __forceinline__ double poly_eval(double x) {
     return s;
 }
 
-// g[0..K] for one pair; R2 = |r|^2, a = damping factor product.
+// g[0..K] for one pair; R2 = |r|^2, inv = 1/|r| (0 makes every g_n exactly
+// 0: a pair that must not contribute), a = damping factor product.
 template <int K, int DAMP>
-__device__ __forceinline__ void radial(double R2, double a, double (&g)[K + 1]) {
-    const double inv = rsqrt(R2);
+__device__ __forceinline__ void radial_inv(double R2, double inv, double a, double (&g)[K + 1]) {
     const double inv2 = inv * inv;
     double b = inv;
     if constexpr (DAMP == DAMP_NONE) {
@@ -124,6 +124,11 @@ __device__ __forceinline__ void radial(double R2, double a, double (&g)[K + 1]) 
         if constexpr (K >= 5) { b *= -9.0 * inv2; g[5] = b * (er - e * poly_eval<DAMP_ERF, 5>(w)); }
         if constexpr (K >= 6) { b *= -11.0 * inv2; g[6] = b * (er - e * poly_eval<DAMP_ERF, 6>(w)); }
     }
+}
+
+template <int K, int DAMP>
+__device__ __forceinline__ void radial(double R2, double a, double (&g)[K + 1]) {
+    radial_inv<K, DAMP>(R2, rsqrt(R2), a, g);
 }
 
 // Register-resident Hermite recursion; T[i] for packed index i, orders <= K.
@@ -186,6 +191,90 @@ __device__ __forceinline__ void contract(const double (&m)[ncomp(M)], const doub
             const Exps et = idx2xyz(t), es = idx2xyz(s);
             acc[t] = __builtin_fma(m[s], T[xyz2idx(et.x + es.x, et.y + es.y, et.z + es.z)], acc[t]);
         }
+    }
+}
+
+// Closed forms of the same contraction for the source / field orders the
+// near field uses most, built directly from the radial derivatives g_n
+// (d_c g_n = x_c g_{n+1}): with the source as the operator
+// D = m_0 + sum_a m_a d_a + sum_{a<=b} m_ab d_a d_b,
+//   D f   = m_0 g0 + (d + tr) g1 + Q g2,       d = sum_a m_a x_a, tr = m_xx + m_yy + m_zz,
+//                                             Q = sum_{a<=b} m_ab x_a x_b, q_c = d_c Q
+//   d_c D f = x_c A1 + m_c g1 + q_c g2,      A1 = m_0 g1 + (d + tr) g2 + Q g3
+//   d_c d_e D f (m_ab = 0) = delta_ce A1 + x_c x_e B + g2 (x_c m_e + m_c x_e),  B = m_0 g2 + d g3.
+// Far fewer operations than building every T component and contracting
+// them; the orders without a closed form here (M = F = 2) use contract().
+template <int M, int F>
+constexpr bool has_closed_form() { return M <= 2 && F <= 2 && !(M == 2 && F == 2); }
+
+template <int M, int F>
+__device__ __forceinline__ void contract_closed(double x, double y, double z, const double (&g)[M + F + 1],
+                                                const double (&m)[ncomp(M)], double (&acc)[ncomp(F)]) {
+    static_assert(has_closed_form<M, F>(), "no closed form for these orders");
+    const double m0 = m[0];
+    double dt = 0.0, d = 0.0, Q = 0.0, qx = 0.0, qy = 0.0, qz = 0.0;
+    if constexpr (M >= 1) {
+        d = __builtin_fma(m[1], x, __builtin_fma(m[2], y, m[3] * z));
+        dt = d;
+    }
+    if constexpr (M >= 2) {
+        // packed quadrupole: 4 xx, 5 xy, 6 xz, 7 yy, 8 yz, 9 zz
+        dt = d + ((m[4] + m[7]) + m[9]);
+        qx = __builtin_fma(2.0 * m[4], x, __builtin_fma(m[5], y, m[6] * z));
+        qy = __builtin_fma(m[5], x, __builtin_fma(2.0 * m[7], y, m[8] * z));
+        qz = __builtin_fma(m[6], x, __builtin_fma(m[8], y, 2.0 * m[9] * z));
+        Q = 0.5 * __builtin_fma(qx, x, __builtin_fma(qy, y, qz * z));
+    }
+    acc[0] = __builtin_fma(m0, g[0], acc[0]);
+    if constexpr (M >= 1) acc[0] = __builtin_fma(dt, g[1], acc[0]);
+    if constexpr (M >= 2) acc[0] = __builtin_fma(Q, g[2], acc[0]);
+    if constexpr (F >= 1) {
+        double A1 = m0 * g[1];
+        if constexpr (M >= 1) A1 = __builtin_fma(dt, g[2], A1);
+        if constexpr (M >= 2) A1 = __builtin_fma(Q, g[3], A1);
+        acc[1] = __builtin_fma(x, A1, acc[1]);
+        acc[2] = __builtin_fma(y, A1, acc[2]);
+        acc[3] = __builtin_fma(z, A1, acc[3]);
+        if constexpr (M >= 1) {
+            acc[1] = __builtin_fma(m[1], g[1], acc[1]);
+            acc[2] = __builtin_fma(m[2], g[1], acc[2]);
+            acc[3] = __builtin_fma(m[3], g[1], acc[3]);
+        }
+        if constexpr (M >= 2) {
+            acc[1] = __builtin_fma(qx, g[2], acc[1]);
+            acc[2] = __builtin_fma(qy, g[2], acc[2]);
+            acc[3] = __builtin_fma(qz, g[2], acc[3]);
+        }
+        if constexpr (F >= 2) {
+            double B = m0 * g[2];
+            if constexpr (M >= 1) B = __builtin_fma(d, g[3], B);
+            const double vx = B * x, vy = B * y, vz = B * z;
+            double ux = 0.0, uy = 0.0, uz = 0.0;
+            if constexpr (M >= 1) { ux = g[2] * m[1]; uy = g[2] * m[2]; uz = g[2] * m[3]; }
+            acc[4] = __builtin_fma(x, __builtin_fma(2.0, ux, vx), acc[4] + A1);
+            acc[5] = __builtin_fma(x, vy + uy, __builtin_fma(ux, y, acc[5]));
+            acc[6] = __builtin_fma(x, vz + uz, __builtin_fma(ux, z, acc[6]));
+            acc[7] = __builtin_fma(y, __builtin_fma(2.0, uy, vy), acc[7] + A1);
+            acc[8] = __builtin_fma(y, vz + uz, __builtin_fma(uy, z, acc[8]));
+            acc[9] = __builtin_fma(z, __builtin_fma(2.0, uz, vz), acc[9] + A1);
+        }
+    }
+}
+
+// One pair's contribution acc[t] += sum_s m[s] T_{s+t}(x, y, z) with damping
+// factor product a; closed form where there is one.
+template <int M, int F, int DAMP>
+__device__ __forceinline__ void pair_field(double x, double y, double z, double R2, double a,
+                                           const double (&m)[ncomp(M)], double (&acc)[ncomp(F)]) {
+    constexpr int K = M + F;
+    double gr[K + 1];
+    radial<K, DAMP>(R2, a, gr);
+    if constexpr (has_closed_form<M, F>()) {
+        contract_closed<M, F>(x, y, z, gr, m, acc);
+    } else {
+        double T[ncomp(K)];
+        md_tensor<K>(x, y, z, gr, T);
+        contract<M, F>(m, T, acc);
     }
 }
 
