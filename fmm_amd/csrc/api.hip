@@ -379,6 +379,29 @@ __global__ void k_pair_count(const int* off, const int* src, const int* leaf_nod
     atomicAdd(total, s * (unsigned long long)count[node]);
 }
 
+// Near-field work of each leaf (source tile passes of its wavefront, as
+// k_near walks them), as a sort key that puts the heaviest leaves first:
+// the hardware starts workgroups in index order, so the long blocks no
+// longer finish last on an otherwise idle GPU.
+__global__ void k_leaf_work_keys(const int* __restrict__ off, const int* __restrict__ src,
+                                 const int* __restrict__ leaf_node, const int* __restrict__ count, int nl,
+                                 unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nl) return;
+    const int node = leaf_node[l];
+    long long total = 0;
+    for (int e = off[node]; e < off[node + 1]; ++e) total += count[src_node(src[e])];
+    const int c = count[node];
+    long long work = 0;
+    for (int b = 0; b < c; b += 64) {
+        const int nt = min(64, c - b), r = 64 / nt;
+        work += (total + r - 1) / r;
+    }
+    const unsigned long long w = (unsigned long long)min(work, 0xffffffffll);
+    keys[l] = ((0xffffffffull - w) << 32) | (unsigned long long)l;
+    vals[l] = node;
+}
+
 __global__ void k_invert_perm(const int* __restrict__ perm, int* __restrict__ inv, int n) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) inv[perm[k]] = k;
@@ -1075,7 +1098,16 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     double* near = c->buf<double>("e_near", (size_t)n * nf * S);
     int* inv = c->buf<int>("s_inv", n);
     hipLaunchKernelGGL(k_invert_perm, dim3(nblk(n, 256)), dim3(256), 0, s, tr.t.perm, inv, n);
-    NearArgs na{tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
+    // leaves in order of decreasing near-field work
+    unsigned long long* wk_in = c->buf<unsigned long long>("s_work_keys_in", nl);
+    unsigned long long* wk = c->buf<unsigned long long>("s_work_keys", nl);
+    int* wv_in = c->buf<int>("s_work_vals_in", nl);
+    int* blk = c->buf<int>("s_work_blk", nl);
+    hipLaunchKernelGGL(k_leaf_work_keys, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
+                       tr.t.leaf_node, tr.t.count, nl, wk_in, wv_in);
+    size_t wb = sort_pairs_u64_temp(nl);
+    sort_pairs_u64(c->buf<char>("s_work_temp", wb), wb, wk_in, wk, wv_in, blk, nl, 64, s);
+    NearArgs na{blk, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
                 ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp, inv};
     launch_near(na, nl, m, k.F, k.damp, s);
     c->mark("p2p");

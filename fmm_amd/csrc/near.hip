@@ -45,7 +45,7 @@ constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: never
 constexpr int NEAR_WIN = 64;
 
 template <int M, int F, int DAMP, int S>
-__global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 3 : 1)
+__global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 1)
                                         : ((M + F <= 3 && DAMP != DAMP_ERF) ? 2 : 1)) void k_near(NearArgs a) {
     constexpr int K = M + F;
     constexpr int NM = ncomp(M);
