@@ -201,13 +201,16 @@ struct M2LArgs {
     int s_ld;
     int num_nodes;
     const unsigned char* order;          // per CSR position: learned expansion order (null: p; MFMA form only)
+    const int* fidx;                     // MFMA form: A-fragment index table (see M2LGemmArgs)
+    const float* CMs32;                  // MFMA form, mixed precision: FP32 copy of CMs (null: FP64)
 };
 void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
                           int nsets, int s_ld_out, hipStream_t s);
 void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s);
-// MFMA form for many sets: one wavefront per target accumulates its n_m x S
-// local expansions (S = a.nsets <= a.s_ld, s_ld 16 or 32) over all its pairs
-// with v_mfma_f64_16x16x4f64.  Supported from 16 sets on (callers chunk by 32).
+// MFMA form for many sets: one wavefront per (target, 16 sets) accumulates
+// its n_m x 16 local expansions over all its pairs (FP64 MFMA, or FP32 MFMA
+// per pair with FP64 accumulation when CMs32 is set).  CMs is set-fastest with
+// s_ld >= nsets, a multiple of 16.  Supported from 16 sets on, orders <= 10.
 bool m2l_mfma_supported(int p, int nsets);
 void launch_m2l_mfma(const M2LArgs& a, int num_nodes, hipStream_t s);
 

@@ -19,6 +19,8 @@
 // for each expansion order so every index is a constant offset.
 // Using the exact integer v instead of the rounded c_i - c_j changes each
 // tensor by ~1e-16 relative, far inside the 1e-12 parity bar.
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 #include "hermite.h"
@@ -329,66 +331,68 @@ void launch_scale_sources(const double* CM, const double* cs, const double* r, d
 }
 
 // ---------------------------------------------------------------- MFMA form
-// For S >= 16 multipole sets, the contraction of one pair is a small GEMM
+// For many multipole sets, the contraction of one pair is a small GEMM
 //   L~_i[t][set] += sum_s H_c[t][s] M~_j[s][set],  H_c[t][s] = scale(t) T~_c[s+t]
-// with n_m x n_m Hankel A (built on the fly from the class tensor in LDS) and
-// n_m x S source B.  One wavefront per target node keeps the whole n_m x S
-// result in MFMA accumulators across all its pairs (v_mfma_f64_16x16x4f64:
-// A lane l -> A[l % 16][l / 16], B lane l -> B[l / 16][l % 16], accumulator
-// register q of lane l -> D[4 q + l / 16][l % 16], measured on gfx950).  The packed index of s + t for every A-fragment
-// lane is tabulated in LDS once per workgroup; the level-offset factor is a
-// per-row scale of A.
+// with the n_m x n_m Hankel matrix A gathered from the class tensor (staged
+// in a double-buffered LDS slot, prefetched one pair ahead) through the
+// host-built fragment index table (packed index of s + t per A-fragment
+// lane, read from global memory: L2-resident) and an n_m x 16 source block B
+// (16 sets).  One wavefront per (target node, group of 16 sets) keeps the
+// whole n_m x 16 result in MFMA accumulators across all its pairs.
+//   FP64 (v_mfma_f64_16x16x4f64): accumulators stay FP64 over every pair.
+//   FP32 (mixed precision, v_mfma_f32_16x16x4f32): the source block is
+//   staged per pair in LDS as FP32, each pair's 16-row tiles are accumulated
+//   over s in FP32, then added to FP64 accumulators, so the sum over a
+//   target's pairs is FP64.
+// MFMA operand layouts (measured on gfx950): A lane l -> A[l % 16][l / 16],
+// B lane l -> B[l / 16][l % 16]; accumulator register q of lane l ->
+// D[4 q + l / 16][l % 16] for f64_16x16x4f64, D[4 (l / 16) + q][l % 16] for
+// f32_16x16x4f32.  Learned per-pair orders (a.order) shorten both loops.
 using v4d = double __attribute__((ext_vector_type(4)));
-constexpr int M2L_MFMA_PMAX = 8;
+using v4f = float __attribute__((ext_vector_type(4)));
+constexpr int M2L_MFMA_PMAX = 10;
+constexpr int M2L_SETS_TILE = 16;
 
-template <int P, int NTL>
-__global__ __launch_bounds__(64) void k_m2l_mfma(M2LArgs a) {
+template <int P, bool F32>
+constexpr int mfma_waves() { return ((ncomp(P) + 15) / 16) < 4 ? ((ncomp(P) + 15) / 16) : 4; }
+
+template <int P, bool F32>
+__global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int NM = ncomp(P);
     constexpr int NT = ncomp(2 * P);
     constexpr int MT = (NM + 15) / 16;      // 16-row tiles of target coefficients
     constexpr int KS = (NM + 3) / 4;        // 4-deep steps over source coefficients
-    constexpr int SLD = 16 * NTL;           // stored sets per coefficient
-    constexpr int NL = (NT + 63) / 64;
+    constexpr int NW = mfma_waves<P, F32>();  // wavefronts per workgroup, tile mt to wave mt % NW
+    constexpr int MW = (MT + NW - 1) / NW;  // row tiles per wavefront
+    constexpr int NTH = 64 * NW;
+    constexpr int NL = (NT + NTH - 1) / NTH;
+    using Acc = typename std::conditional<F32, v4f, v4d>::type;
     __shared__ double Tl[2][NT];
-    __shared__ short fidx[MT][KS][64];
+    __shared__ int sdec[4 * KS];            // decoded source components (degree | w << 8 | z << 16)
     const int i = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int sg = blockIdx.y * M2L_SETS_TILE;    // first set of this workgroup
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, lane = tid & 63;
     const int row = lane & 15, kq = lane >> 4;
     const int e0 = a.off[i], e1 = a.off[i + 1];
-    // Hankel index table: A[t][s] = T[idx(s + t)] for t = mt*16 + row, s = ks*4 + kq
-    for (int mt = 0; mt < MT; ++mt) {
-        const int t = mt * 16 + row;
-        const int dt = t < NM ? decode_target(t) : 0;
-        for (int ks = 0; ks < KS; ++ks) {
-            const int s = ks * 4 + kq;
-            short v = -1;
-            if (t < NM && s < NM) {
-                const int ds = decode_target(s);
-                const int n = (ds & 255) + (dt & 255);
-                const int w = ((ds >> 8) & 255) + ((dt >> 8) & 255);
-                const int z = (ds >> 16) + (dt >> 16);
-                v = (short)(tet(n) + tri(w) + z);
-            }
-            fidx[mt][ks][lane] = v;
-        }
+    for (int x = tid; x < 4 * KS; x += NTH) sdec[x] = x < NM ? decode_target(x) : 0;
+    // A fragment of row tile mt holds H[t = 16 mt + row][s = 4 ks + kq]; the
+    // packed index of s + t is tet(|s| + |t|) + tri(w_s + w_t) + (z_s + z_t)
+    int tdec[MW];
+#pragma unroll
+    for (int mi = 0; mi < MW; ++mi) {
+        const int t = (w + NW * mi) * 16 + row;
+        tdec[mi] = t < NM ? decode_target(t) : -1;
     }
-    int tdeg[MT];
+    v4d acc[MW];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const int t = mt * 16 + row;
-        tdeg[mt] = t < NM ? (decode_target(t) & 255) : 0;
-    }
-    v4d acc[MT][NTL];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int mi = 0; mi < MW; ++mi) acc[mi] = v4d{0.0, 0.0, 0.0, 0.0};
     double pre[NL];
     if (e0 < e1) {
         const double* T0 = a.Tc + (int64_t)a.pair_class[e0] * NT;
 #pragma unroll
         for (int x = 0; x < NL; ++x)
-            if (lane + 64 * x < NT) Tl[0][lane + 64 * x] = T0[lane + 64 * x];
+            if (tid + NTH * x < NT) Tl[0][tid + NTH * x] = T0[tid + NTH * x];
     }
     __syncthreads();
     const int li = a.level[i];
@@ -398,82 +402,111 @@ __global__ __launch_bounds__(64) void k_m2l_mfma(M2LArgs a) {
         if (more) {
             const double* Tn = a.Tc + (int64_t)a.pair_class[e + 1] * NT;
 #pragma unroll
-            for (int x = 0; x < NL; ++x) pre[x] = (lane + 64 * x < NT) ? Tn[lane + 64 * x] : 0.0;
+            for (int x = 0; x < NL; ++x) pre[x] = (tid + NTH * x < NT) ? Tn[tid + NTH * x] : 0.0;
         }
         const int j = src_node(a.src[e]);
         const int dl = li - a.level[j];
         // learned order q of this pair: only the leading ncomp(q) rows and columns
         const int nq = a.order ? ncomp(a.order[e]) : NM;
         const int KSq = (nq + 3) / 4;
-        double rs[MT];
+        // level-offset scale of row t as a power of two: 2^-(dl |t|) or 2^(dl (|t| + 1))
+        int sh[MW];
+        bool rv[MW];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-            rs[mt] = dl > 0 ? ldexp(1.0, -dl * tdeg[mt]) : (dl < 0 ? ldexp(1.0, -dl * (tdeg[mt] + 1)) : 1.0);
-        const double* __restrict__ B = a.CMs + (int64_t)j * NM * SLD;
+        for (int mi = 0; mi < MW; ++mi) {
+            const int deg = tdec[mi] & 255;
+            sh[mi] = dl > 0 ? -dl * deg : (dl < 0 ? -dl * (deg + 1) : 0);
+            rv[mi] = tdec[mi] >= 0 && (w + NW * mi) * 16 + row < nq;
+        }
         const double* __restrict__ T = Tl[cur];
+        const int64_t boff = (int64_t)j * NM * a.s_ld + sg + row;
+        Acc part[MW];
+        if constexpr (F32) {
+#pragma unroll
+            for (int mi = 0; mi < MW; ++mi) part[mi] = Acc{0, 0, 0, 0};
+        }
 #pragma unroll 2
         for (int ks = 0; ks < KSq; ++ks) {
             const int s = ks * 4 + kq;
-            double b[NTL];
+            const int ds = sdec[s];
+            const bool sv = s < nq;
+            typename std::conditional<F32, float, double>::type b;
+            if constexpr (F32) b = sv ? a.CMs32[boff + (int64_t)s * a.s_ld] : 0.0f;
+            else b = sv ? a.CMs[boff + (int64_t)s * a.s_ld] : 0.0;
 #pragma unroll
-            for (int nt = 0; nt < NTL; ++nt) b[nt] = s < nq ? B[(int64_t)s * SLD + nt * 16 + row] : 0.0;
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const int f = fidx[mt][ks][lane];
-                const double av = (f >= 0 && mt * 16 + row < nq) ? T[f] * rs[mt] : 0.0;
-#pragma unroll
-                for (int nt = 0; nt < NTL; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[nt], acc[mt][nt], 0, 0, 0);
+            for (int mi = 0; mi < MW; ++mi) {
+                const int dt = tdec[mi];
+                const int kk = (dt & 255) + (ds & 255);
+                const int ww = ((dt >> 8) & 255) + ((ds >> 8) & 255);
+                const int f = tet(kk) + tri(ww) + ((dt >> 16) & 255) + (ds >> 16);
+                const double av = (rv[mi] && sv) ? ldexp(T[f], sh[mi]) : 0.0;
+                if constexpr (F32)
+                    part[mi] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av, b, part[mi], 0, 0, 0);
+                else
+                    acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, acc[mi], 0, 0, 0);
             }
+        }
+        if constexpr (F32) {
+            // this pair's FP32 tiles into the FP64 accumulators (the f32 layout keeps row
+            // 4 (l / 16) + q in register q, the f64 one row 4 q + l / 16; remapped at the end)
+#pragma unroll
+            for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[mi][q] += (double)part[mi][q];
         }
         if (more) {
 #pragma unroll
             for (int x = 0; x < NL; ++x)
-                if (lane + 64 * x < NT) Tl[1 - cur][lane + 64 * x] = pre[x];
+                if (tid + NTH * x < NT) Tl[1 - cur][tid + NTH * x] = pre[x];
         }
         __syncthreads();
     }
     const double rinv = 1.0 / a.r[i];
+    const int set = sg + row;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
+    for (int mi = 0; mi < MW; ++mi) {
+        const int mt = w + NW * mi;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int t = mt * 16 + 4 * q + kq;
-            if (t >= NM) continue;
+            const int t = mt * 16 + (F32 ? 4 * kq + q : 4 * q + kq);
+            if (mt >= MT || t >= NM) continue;
             const int deg = decode_target(t) & 255;
             double f = rinv;
             for (int k = 0; k < deg; ++k) f *= rinv;
-#pragma unroll
-            for (int nt = 0; nt < NTL; ++nt) {
-                const int set = nt * 16 + row;
-                if (set < a.nsets) a.LE[((int64_t)set * a.num_nodes + i) * NM + t] = acc[mt][nt][q] * f;
-            }
+            if (set < a.nsets) a.LE[((int64_t)set * a.num_nodes + i) * NM + t] = acc[mi][q] * f;
         }
     }
 }
 
 bool m2l_mfma_supported(int p, int nsets) { return p <= M2L_MFMA_PMAX && nsets >= 16; }
 
-template <int NTL>
+template <int P, bool F32>
+static void launch_mfma_one(const M2LArgs& a, int num_nodes, hipStream_t s) {
+    dim3 g(num_nodes, (a.nsets + M2L_SETS_TILE - 1) / M2L_SETS_TILE), b(64 * mfma_waves<P, F32>());
+    hipLaunchKernelGGL((k_m2l_mfma<P, F32>), g, b, 0, s, a);
+}
+
+template <bool F32>
 static void launch_mfma_p(const M2LArgs& a, int num_nodes, hipStream_t s) {
-    dim3 g(num_nodes), b(64);
     switch (a.p) {
-        case 0: hipLaunchKernelGGL((k_m2l_mfma<0, NTL>), g, b, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_m2l_mfma<1, NTL>), g, b, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_m2l_mfma<2, NTL>), g, b, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_m2l_mfma<3, NTL>), g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_m2l_mfma<4, NTL>), g, b, 0, s, a); break;
-        case 5: hipLaunchKernelGGL((k_m2l_mfma<5, NTL>), g, b, 0, s, a); break;
-        case 6: hipLaunchKernelGGL((k_m2l_mfma<6, NTL>), g, b, 0, s, a); break;
-        case 7: hipLaunchKernelGGL((k_m2l_mfma<7, NTL>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_m2l_mfma<8, NTL>), g, b, 0, s, a); break;
+        case 0: launch_mfma_one<0, F32>(a, num_nodes, s); break;
+        case 1: launch_mfma_one<1, F32>(a, num_nodes, s); break;
+        case 2: launch_mfma_one<2, F32>(a, num_nodes, s); break;
+        case 3: launch_mfma_one<3, F32>(a, num_nodes, s); break;
+        case 4: launch_mfma_one<4, F32>(a, num_nodes, s); break;
+        case 5: launch_mfma_one<5, F32>(a, num_nodes, s); break;
+        case 6: launch_mfma_one<6, F32>(a, num_nodes, s); break;
+        case 7: launch_mfma_one<7, F32>(a, num_nodes, s); break;
+        case 8: launch_mfma_one<8, F32>(a, num_nodes, s); break;
+        case 9: launch_mfma_one<9, F32>(a, num_nodes, s); break;
+        default: launch_mfma_one<10, F32>(a, num_nodes, s); break;
     }
 }
 
 void launch_m2l_mfma(const M2LArgs& a, int num_nodes, hipStream_t s) {
     if (num_nodes <= 0) return;
-    if (a.s_ld == 16) launch_mfma_p<1>(a, num_nodes, s);
-    else launch_mfma_p<2>(a, num_nodes, s);
+    if (a.CMs32) launch_mfma_p<true>(a, num_nodes, s);
+    else launch_mfma_p<false>(a, num_nodes, s);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -45,3 +45,25 @@ def test_mixed_error_budget_vs_direct(mixed):
 def test_mixed_option_errors(gpu_ctx):
     assert gpu_ctx._lib.fmmh_set_option(gpu_ctx.handle, 5, 2.0) == 1
     assert b"precision" in gpu_ctx._lib.fmmh_last_error()
+
+
+@pytest.mark.parametrize("p", [2, 6, 10])
+def test_mixed_per_target_mfma_for_many_sets(mixed, p):
+    """Many sets: the per-target MFMA M2L in FP32 (FP64 sum over pairs) differs
+    from the FP64 form at the FP32 level only, for every set, including sets
+    beyond the first group of 16."""
+    g = load_golden("water999_dipole_orders.npz")
+    x, e = g["coordinates"], g["exclusions"]
+    n = x.shape[0]
+    rng = np.random.default_rng(p)
+    S = 20
+    m = np.zeros((n, 4, S), order="F")
+    for k in range(S):
+        m[:, 0, k] = g["multipoles"][:, 0]
+        m[:, 1:4, k] = rng.normal(0.0, 0.05, (n, 3))
+    ref = mixed.field_fmm_sets(0, x, m, e, 0.5, 64, p, 1)
+    mixed.set_precision(True)
+    got = mixed.field_fmm_sets(0, x, m, e, 0.5, 64, p, 1)
+    for k in (0, 15, 16, S - 1):
+        d = max(block_rel(got[:, :, k], ref[:, :, k], 1))
+        assert 1e-12 < d < 2e-6, (k, d)

@@ -62,7 +62,7 @@ def test_fmm_sets_below_16_bit_identical(gpu_ctx, mfma_mode, nsets):
     np.testing.assert_array_equal(got, _single(gpu_ctx, x, m, e, 2, "THOLE", df, p=6))
 
 
-@pytest.mark.parametrize("p", list(range(0, 9)))
+@pytest.mark.parametrize("p", list(range(0, 11)))
 def test_m2l_mfma_matches_valu_every_order(gpu_ctx, mfma_mode, p):
     """MFMA M2L forced on for 3 sets (zero-padded to 16) against the VALU
     M2L, every order the MFMA form supports: checks the fragment layout."""
