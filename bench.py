@@ -81,6 +81,100 @@ def cpu_baseline(seconds_hint=True):
                       f"THOLE, theta 0.5, ncrit 64, p 6, field order 2; {dt:.1f} s"}
 
 
+def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.35, 10), tol=1e-6, sets=32, spots=1024):
+    """Throughput at <= 1e-6 relative RMS field error against the FP64 direct
+    sum: the plain FP64 FMM (one multipole set, every pair at expansion_order)
+    against the batched (sets on one tree), mixed-precision (FP32 MFMA M2L),
+    learned-order FMM.  Workload: the 200,001-site water geometry above,
+    undamped (the FMM's far field then has the direct sum's semantics, so the
+    error is the method's own), charges plus per-set random dipoles; errors on
+    4096 seeded target sites for the first and last set.  The selector is
+    trained on the GPU on a 1/20-scale box of the same recipe (the training
+    time is reported, not counted in the batched step)."""
+    import torch
+    from fmm_amd import synth
+    from fmm_amd.octree import Context
+    c = synth.config(1)
+    x = c["coordinates"]
+    n = x.shape[0]
+    rng = np.random.default_rng(2024)
+    mh = np.zeros((n, 4, sets), order="F")
+    for k in range(sets):
+        mh[:, 0, k] = c["multipoles"][:, 0]
+        mh[:, 1:4, k] = rng.normal(0.0, 0.05, (n, 3))
+    X = col(x, dev)
+    E = col(c["exclusions"], dev, torch.int32)
+    tg = np.sort(np.random.default_rng(4096).choice(n, 4096, replace=False)).astype(np.int32)
+    T = torch.as_tensor(tg, device=dev)
+    ctx = Context(0)
+    ctx.set_timing(True)
+    ref = {}
+    for k in (0, sets - 1):
+        d = col(np.zeros((4096, 4)), dev)
+        ctx.field_direct_targets_device(X, col(np.asfortranarray(mh[:, :, k]), dev), E, T, 1, d, "", None)
+        ref[k] = d.cpu().numpy()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    M1 = col(np.asfortranarray(mh[:, :, 0]), dev)
+    o1 = col(np.zeros((n, 4)), dev)
+    th_p, p_p = plain
+
+    def f_plain():
+        o1.zero_()
+        ctx.field_fmm_device(X, M1, E, th_p, 64, p_p, 1, o1, "", None)
+    ms_p = timed(f_plain)
+    err_p = rel(o1.cpu().numpy()[tg, 1:4], ref[0][:, 1:4])
+    # the learned order selector, trained on a smaller box of the same recipe
+    th_b, p_b = batched
+    ct = synth.config(1, scale=0.05)
+    mt = np.zeros((ct["coordinates"].shape[0], 4))
+    mt[:, 0] = ct["multipoles"][:, 0]
+    mt[:, 1:4] = np.random.default_rng(7).normal(0.0, 0.05, (mt.shape[0], 3))
+    info = ctx.train_selector(ct["coordinates"], np.asfortranarray(mt), th_b, 64, p_b, nsamples=4096, epochs=60,
+                              seed=5)
+    MS = col3(mh, dev)
+    os_ = col3(np.zeros((n, 4, sets)), dev)
+    ctx.set_precision(True)
+    ctx.set_order_tolerance(tol, spot_checks=spots)
+
+    def f_batched():
+        os_.zero_()
+        ctx.field_fmm_sets_device(X, MS, E, th_b, 64, p_b, 1, os_, "", None)
+    ms_b = timed(f_batched)
+    st = ctx.selection_stats()
+    phases = ctx.phase_times()
+    ob = os_.cpu().numpy()
+    err_b = [rel(ob[tg, 1:4, k], ref[k][:, 1:4]) for k in (0, sets - 1)]
+    ctx.set_order_tolerance(0.0)
+    ctx.set_precision(False)
+    ctx.close()
+    v_p = n / (ms_p / 1e3)
+    v_b = n * sets / (ms_b / 1e3)
+    return {
+        "workload": "water-200k-undamped-dipole (configs[1] geometry, no damping)",
+        "error_metric": "relative RMS of the field block vs FP64 field_direct on 4096 seeded sites",
+        "plain_fp64": {"theta": th_p, "expansion_order": p_p, "sets": 1, "ms_per_call": ms_p, "value": v_p,
+                       "error": err_p},
+        "batched_mixed_learned": {"theta": th_b, "expansion_order_cap": p_b, "sets": sets, "ms_per_call": ms_b,
+                                  "value": v_b, "error_set0": err_b[0], "error_last_set": err_b[1],
+                                  "order_tolerance": tol, "spot_checks": spots, "spot_error": st["spot_error"],
+                                  "fell_back": st["fallback"], "mean_order": st["mean_order"],
+                                  "m2l_flops_vs_full": st["m2l_flops"] / st["m2l_flops_full"],
+                                  "phases_ms": {k: round(v, 3) for k, v in phases.items()},
+                                  "selector_training": info},
+        "speedup": v_b / v_p,
+        "both_within_1e-6": bool(err_p <= 1e-6 and max(err_b) <= 1e-6),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +187,18 @@ def main():
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for HBM counters) and exit")
+    ap.add_argument("--no-target", action="store_true",
+                    help="skip the comparison at <= 1e-6 error (plain FP64 vs batched mixed learned-order)")
+    ap.add_argument("--target-only", action="store_true", help="only run that comparison and print it")
+    ap.add_argument("--target-plain", default="0.3,9", help="theta,p of the plain FP64 side")
+    ap.add_argument("--target-batched", default="0.35,10", help="theta,cap of the batched side")
     args = ap.parse_args()
+    tp = tuple(float(v) if i == 0 else int(v) for i, v in enumerate(args.target_plain.split(",")))
+    tb_ = tuple(float(v) if i == 0 else int(v) for i, v in enumerate(args.target_batched.split(",")))
+    if args.target_only:
+        import torch
+        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_)))
+        return
     if args.copy_calibration:
         import torch
         src = torch.ones(args.copy_calibration // 8, dtype=torch.float64, device="cuda:0")
@@ -234,6 +339,7 @@ def main():
                     "of far pairs)",
         }
     cpu = None if args.no_cpu else cpu_baseline()
+    target = None if (args.no_target or args.config != 1) else target_comparison(dev, min(args.steps, 3), tp, tb_)
     line = {
         "metric": "site-field evals/s at <=1e-6 rel err vs FP64 direct; % of FP64/FP32 peak",
         "value": value,
@@ -258,6 +364,7 @@ def main():
                                "the previous call with bitwise-identical coordinates"},
         "tree": stats,
         "errors": errors,
+        "target_1e-6": target,
     }
     ctx.close()
     print(json.dumps(line))

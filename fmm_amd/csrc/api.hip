@@ -894,13 +894,14 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     }
     // MFMA form: up to 64 sets per pass (one wavefront per target and 16 sets)
     const int chunk = mfma ? 64 : nsets;
-    const int s_ld_max = mfma ? ((std::min(chunk, nsets) + 15) / 16) * 16 : nsets;
+    auto pad_sets = [](int ns) { return ns <= 16 ? 16 : ((ns + 31) / 32) * 32; };
+    const int s_ld_max = mfma ? pad_sets(std::min(chunk, nsets)) : nsets;
     double* CMs = c->buf<double>("m_cms", node_stride * s_ld_max);
     const bool f32 = mfma && c->precision == 1;
     float* CMs32 = f32 ? c->buf<float>("m_cms32", node_stride * s_ld_max) : nullptr;
     for (int c0 = 0; c0 < nsets; c0 += chunk) {
         const int ns = std::min(chunk, nsets - c0);
-        const int s_ld = mfma ? ((ns + 15) / 16) * 16 : 0;
+        const int s_ld = mfma ? pad_sets(ns) : 0;
         launch_scale_sources(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, s_ld, s);
         if (f32) launch_to_f32(CMs, CMs32, (int64_t)node_stride * s_ld, s);
         M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,

@@ -356,7 +356,14 @@ constexpr int M2L_SETS_TILE = 16;
 template <int P, bool F32>
 constexpr int mfma_waves() { return ((ncomp(P) + 15) / 16) < 4 ? ((ncomp(P) + 15) / 16) : 4; }
 
-template <int P, bool F32>
+// NTL = 16-set tiles per workgroup: an A fragment, gathered once, feeds NTL MFMAs.
+// FP64: the row scale is applied to each A element and the pair accumulates
+// straight into the FP64 MFMA accumulators.  FP32: the class tensor is kept
+// as FP32 in LDS, the pair's tiles are accumulated over s in FP32 with
+// unscaled A, and each tile is scaled by its rows' powers of two while it is
+// added to the FP64 accumulators (rows beyond the pair's order are skipped
+// there).
+template <int P, bool F32, int NTL>
 __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int NM = ncomp(P);
     constexpr int NT = ncomp(2 * P);
@@ -366,11 +373,12 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int MW = (MT + NW - 1) / NW;  // row tiles per wavefront
     constexpr int NTH = 64 * NW;
     constexpr int NL = (NT + NTH - 1) / NTH;
+    using TE = typename std::conditional<F32, float, double>::type;
     using Acc = typename std::conditional<F32, v4f, v4d>::type;
-    __shared__ double Tl[2][NT];
+    __shared__ TE Tl[2][NT];
     __shared__ int sdec[4 * KS];            // decoded source components (degree | w << 8 | z << 16)
     const int i = blockIdx.x;
-    const int sg = blockIdx.y * M2L_SETS_TILE;    // first set of this workgroup
+    const int sg = blockIdx.y * M2L_SETS_TILE * NTL;   // first set of this workgroup
     const int tid = threadIdx.x;
     const int w = tid >> 6, lane = tid & 63;
     const int row = lane & 15, kq = lane >> 4;
@@ -384,15 +392,28 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
         const int t = (w + NW * mi) * 16 + row;
         tdec[mi] = t < NM ? decode_target(t) : -1;
     }
-    v4d acc[MW];
+    // FP32 accumulator register q holds row 4 kq + q of its tile: its degree
+    int qdeg[F32 ? MW : 1][4];
+    if constexpr (F32) {
 #pragma unroll
-    for (int mi = 0; mi < MW; ++mi) acc[mi] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = (w + NW * mi) * 16 + 4 * kq + q;
+                qdeg[mi][q] = t < NM ? (decode_target(t) & 255) : 99;
+            }
+    }
+    v4d acc[MW][NTL];
+#pragma unroll
+    for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = v4d{0.0, 0.0, 0.0, 0.0};
     double pre[NL];
     if (e0 < e1) {
         const double* T0 = a.Tc + (int64_t)a.pair_class[e0] * NT;
 #pragma unroll
         for (int x = 0; x < NL; ++x)
-            if (tid + NTH * x < NT) Tl[0][tid + NTH * x] = T0[tid + NTH * x];
+            if (tid + NTH * x < NT) Tl[0][tid + NTH * x] = (TE)T0[tid + NTH * x];
     }
     __syncthreads();
     const int li = a.level[i];
@@ -407,7 +428,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
         const int j = src_node(a.src[e]);
         const int dl = li - a.level[j];
         // learned order q of this pair: only the leading ncomp(q) rows and columns
-        const int nq = a.order ? ncomp(a.order[e]) : NM;
+        const int qo = a.order ? (int)a.order[e] : P;
+        const int nq = ncomp(qo);
         const int KSq = (nq + 3) / 4;
         // level-offset scale of row t as a power of two: 2^-(dl |t|) or 2^(dl (|t| + 1))
         int sh[MW];
@@ -418,51 +440,68 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
             sh[mi] = dl > 0 ? -dl * deg : (dl < 0 ? -dl * (deg + 1) : 0);
             rv[mi] = tdec[mi] >= 0 && (w + NW * mi) * 16 + row < nq;
         }
-        const double* __restrict__ T = Tl[cur];
+        const TE* __restrict__ T = Tl[cur];
         const int64_t boff = (int64_t)j * NM * a.s_ld + sg + row;
-        Acc part[MW];
+        Acc part[MW][NTL];
         if constexpr (F32) {
 #pragma unroll
-            for (int mi = 0; mi < MW; ++mi) part[mi] = Acc{0, 0, 0, 0};
+            for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt) part[mi][nt] = Acc{0, 0, 0, 0};
         }
-#pragma unroll 2
+#pragma unroll 8
         for (int ks = 0; ks < KSq; ++ks) {
             const int s = ks * 4 + kq;
             const int ds = sdec[s];
             const bool sv = s < nq;
-            typename std::conditional<F32, float, double>::type b;
-            if constexpr (F32) b = sv ? a.CMs32[boff + (int64_t)s * a.s_ld] : 0.0f;
-            else b = sv ? a.CMs[boff + (int64_t)s * a.s_ld] : 0.0;
+            TE b[NTL];
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) {
+                if constexpr (F32) b[nt] = sv ? a.CMs32[boff + (int64_t)s * a.s_ld + 16 * nt] : 0.0f;
+                else b[nt] = sv ? a.CMs[boff + (int64_t)s * a.s_ld + 16 * nt] : 0.0;
+            }
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi) {
                 const int dt = tdec[mi];
                 const int kk = (dt & 255) + (ds & 255);
                 const int ww = ((dt >> 8) & 255) + ((ds >> 8) & 255);
                 const int f = tet(kk) + tri(ww) + ((dt >> 16) & 255) + (ds >> 16);
-                const double av = (rv[mi] && sv) ? ldexp(T[f], sh[mi]) : 0.0;
-                if constexpr (F32)
-                    part[mi] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av, b, part[mi], 0, 0, 0);
-                else
-                    acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, acc[mi], 0, 0, 0);
+                if constexpr (F32) {
+                    const float av = dt >= 0 ? T[f] : 0.0f;
+#pragma unroll
+                    for (int nt = 0; nt < NTL; ++nt)
+                        part[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b[nt], part[mi][nt], 0, 0, 0);
+                } else {
+                    const double av = rv[mi] ? ldexp(T[f], sh[mi]) : 0.0;
+#pragma unroll
+                    for (int nt = 0; nt < NTL; ++nt)
+                        acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[nt], acc[mi][nt], 0, 0, 0);
+                }
             }
         }
         if constexpr (F32) {
-            // this pair's FP32 tiles into the FP64 accumulators (the f32 layout keeps row
-            // 4 (l / 16) + q in register q, the f64 one row 4 q + l / 16; remapped at the end)
+            // this pair's FP32 tiles, row-scaled, into the FP64 accumulators (the f32 layout keeps
+            // row 4 kq + q in register q, the f64 one row 4 q + kq; remapped at the end)
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[mi][q] += (double)part[mi][q];
+                for (int q = 0; q < 4; ++q) {
+                    const int deg = qdeg[mi][q];
+                    const int shq = dl > 0 ? -dl * deg : (dl < 0 ? -dl * (deg + 1) : 0);
+                    const bool ok = deg <= qo;
+#pragma unroll
+                    for (int nt = 0; nt < NTL; ++nt)
+                        if (ok) acc[mi][nt][q] += ldexp((double)part[mi][nt][q], shq);
+                }
         }
         if (more) {
 #pragma unroll
             for (int x = 0; x < NL; ++x)
-                if (tid + NTH * x < NT) Tl[1 - cur][tid + NTH * x] = pre[x];
+                if (tid + NTH * x < NT) Tl[1 - cur][tid + NTH * x] = (TE)pre[x];
         }
         __syncthreads();
     }
     const double rinv = 1.0 / a.r[i];
-    const int set = sg + row;
 #pragma unroll
     for (int mi = 0; mi < MW; ++mi) {
         const int mt = w + NW * mi;
@@ -473,17 +512,28 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
             const int deg = decode_target(t) & 255;
             double f = rinv;
             for (int k = 0; k < deg; ++k) f *= rinv;
-            if (set < a.nsets) a.LE[((int64_t)set * a.num_nodes + i) * NM + t] = acc[mi][q] * f;
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) {
+                const int set = sg + 16 * nt + row;
+                if (set < a.nsets) a.LE[((int64_t)set * a.num_nodes + i) * NM + t] = acc[mi][nt][q] * f;
+            }
         }
     }
 }
 
 bool m2l_mfma_supported(int p, int nsets) { return p <= M2L_MFMA_PMAX && nsets >= 16; }
 
+// 32 sets and more: two 16-set tiles per workgroup
 template <int P, bool F32>
 static void launch_mfma_one(const M2LArgs& a, int num_nodes, hipStream_t s) {
-    dim3 g(num_nodes, (a.nsets + M2L_SETS_TILE - 1) / M2L_SETS_TILE), b(64 * mfma_waves<P, F32>());
-    hipLaunchKernelGGL((k_m2l_mfma<P, F32>), g, b, 0, s, a);
+    const dim3 b(64 * mfma_waves<P, F32>());
+    if (a.nsets > M2L_SETS_TILE && a.s_ld % (2 * M2L_SETS_TILE) == 0) {
+        const dim3 g(num_nodes, (a.nsets + 2 * M2L_SETS_TILE - 1) / (2 * M2L_SETS_TILE));
+        hipLaunchKernelGGL((k_m2l_mfma<P, F32, 2>), g, b, 0, s, a);
+    } else {
+        const dim3 g(num_nodes, (a.nsets + M2L_SETS_TILE - 1) / M2L_SETS_TILE);
+        hipLaunchKernelGGL((k_m2l_mfma<P, F32, 1>), g, b, 0, s, a);
+    }
 }
 
 template <bool F32>
