@@ -379,10 +379,15 @@ __global__ void k_pair_count(const int* off, const int* src, const int* leaf_nod
     atomicAdd(total, s * (unsigned long long)count[node]);
 }
 
-// Near-field work of each leaf (source tile passes of its wavefront, as
-// k_near walks them), as a sort key that puts the heaviest leaves first:
-// the hardware starts workgroups in index order, so the long blocks no
-// longer finish last on an otherwise idle GPU.
+// Order of the near-field workgroups.  Workgroup b runs on XCD b % 8 (the
+// dispatcher deals workgroups round robin), and each XCD has its own L2: the
+// leaves are cut into 8 contiguous Morton ranges, range x goes to the slots
+// b = 8 r + x, so an XCD's L2 holds one region's sources; within a range the
+// heaviest leaves (source tile passes of the wavefront, as k_near walks them)
+// come first, because the hardware starts workgroups in index order and the
+// long blocks should not finish last on an idle GPU.
+constexpr int NUM_XCD = 8;
+
 __global__ void k_leaf_work_keys(const int* __restrict__ off, const int* __restrict__ src,
                                  const int* __restrict__ leaf_node, const int* __restrict__ count, int nl,
                                  unsigned long long* __restrict__ keys, int* __restrict__ vals) {
@@ -398,8 +403,21 @@ __global__ void k_leaf_work_keys(const int* __restrict__ off, const int* __restr
         work += (total + r - 1) / r;
     }
     const unsigned long long w = (unsigned long long)min(work, 0xffffffffll);
-    keys[l] = ((0xffffffffull - w) << 32) | (unsigned long long)l;
+    const unsigned long long x = (unsigned long long)(((long long)l * NUM_XCD) / nl);   // Morton range
+    keys[l] = (x << 56) | ((0xffffffffull - w) << 24) | (unsigned long long)(l & 0xffffff);
     vals[l] = node;
+}
+
+// sorted position k (range x, rank r within it) -> workgroup slot 8 r + x; unused slots hold -1
+__global__ void k_leaf_scatter(const int* __restrict__ sorted_nodes, int nl, int per, int* __restrict__ blk) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nl) return;
+    // range x holds leaves [ceil(x nl / 8), ceil((x + 1) nl / 8)) of the Morton order
+    int x = (int)(((long long)k * NUM_XCD) / nl);
+    while (x + 1 < NUM_XCD && ((long long)(x + 1) * nl + NUM_XCD - 1) / NUM_XCD <= k) ++x;
+    while (x > 0 && ((long long)x * nl + NUM_XCD - 1) / NUM_XCD > k) --x;
+    const int first = (int)(((long long)x * nl + NUM_XCD - 1) / NUM_XCD);
+    blk[(k - first) * NUM_XCD + x] = sorted_nodes[k];
 }
 
 __global__ void k_invert_perm(const int* __restrict__ perm, int* __restrict__ inv, int n) {
@@ -1108,15 +1126,23 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     unsigned long long* wk_in = c->buf<unsigned long long>("s_work_keys_in", nl);
     unsigned long long* wk = c->buf<unsigned long long>("s_work_keys", nl);
     int* wv_in = c->buf<int>("s_work_vals_in", nl);
-    int* blk = c->buf<int>("s_work_blk", nl);
     hipLaunchKernelGGL(k_leaf_work_keys, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
                        tr.t.leaf_node, tr.t.count, nl, wk_in, wv_in);
     size_t wb = sort_pairs_u64_temp(nl);
-    sort_pairs_u64(c->buf<char>("s_work_temp", wb), wb, wk_in, wk, wv_in, blk, nl, 64, s);
-    NearArgs na{blk, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
+    int* sorted_nodes = c->buf<int>("s_work_sorted", nl);
+    sort_pairs_u64(c->buf<char>("s_work_temp", wb), wb, wk_in, wk, wv_in, sorted_nodes, nl, 64, s);
+    const int per = (nl + NUM_XCD - 1) / NUM_XCD + 1;   // slots per XCD range
+    int* slots = c->buf<int>("s_work_blk", (size_t)per * NUM_XCD);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk((long long)per * NUM_XCD, 256)), dim3(256), 0, s, slots, -1,
+                       per * NUM_XCD);
+    hipLaunchKernelGGL(k_leaf_scatter, dim3(nblk(nl, 256)), dim3(256), 0, s, sorted_nodes, nl, per, slots);
+    NearArgs na{slots, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
                 ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp, inv};
-    launch_near(na, nl, m, k.F, k.damp, s);
+    c->mark("p2p_order");
+    launch_near_part(na, per * NUM_XCD, m, k.F, k.damp, 1, s);
     c->mark("p2p");
+    launch_near_part(na, per * NUM_XCD, m, k.F, k.damp, 2, s);
+    c->mark("p2p_excl");
     const bool classes_cached = reuse && c->cache_class_p == p;
     // learned per-pair orders (off: every pair at p, as the reference)
     const bool select = c->order_tol > 0.0 && c->sel_trained && !P.on && L.n_m2l > 0;

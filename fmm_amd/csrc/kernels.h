@@ -36,6 +36,8 @@ struct NearArgs {
 
 bool near_supported(int m, int F);
 void launch_near(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s);
+// part 1: the pair kernel only; part 2: the exclusion correction only
+void launch_near_part(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
 
 // ---------------------------------------------------------------- tree
 struct TreeDev {

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
     __shared__ int wstart[NEAR_WIN];              // window: first sorted site of each entry
     __shared__ int wimg[NEAR_WIN];                // window: periodic image id of each entry
     const int b = a.blk[blockIdx.x];
+    if (b < 0) return;                         // an empty slot of the block order
     const int set0 = blockIdx.y * S;
     const int ns = min(S, a.nsets - set0);   // sets of this row (the rest are zero padding)
     const int lane = threadIdx.x;
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
     const int b = a.blk[blockIdx.x];
+    if (b < 0) return;
     const int set0 = blockIdx.y * S;
     const int ns = min(S, a.nsets - set0);
     const int t0 = a.tgt_start[b], tc = a.tgt_count[b];
@@ -293,47 +295,54 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
 template <int M, int F, int DAMP>
 constexpr int near_sets() { return (M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 2; }
 
+// part 1: the pair kernel; part 2: the exclusion correction (after it on the stream)
 template <int M, int F, int DAMP>
-static void launch_sets(const NearArgs& a, int nblocks, hipStream_t s) {
+static void launch_sets(const NearArgs& a, int nblocks, int part, hipStream_t s) {
     if (a.nsets == 1) {
-        hipLaunchKernelGGL((k_near<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
-        hipLaunchKernelGGL((k_near_excl<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
+        if (part == 1) hipLaunchKernelGGL((k_near<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((k_near_excl<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
     } else {
         constexpr int S = near_sets<M, F, DAMP>();
-        hipLaunchKernelGGL((k_near<M, F, DAMP, S>), dim3(nblocks, (a.nsets + S - 1) / S), dim3(64), 0, s, a);
-        hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S>), dim3(nblocks, (a.nsets + S - 1) / S), dim3(64), 0, s, a);
+        const dim3 g(nblocks, (a.nsets + S - 1) / S);
+        if (part == 1) hipLaunchKernelGGL((k_near<M, F, DAMP, S>), g, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S>), g, dim3(64), 0, s, a);
     }
 }
 
 template <int M, int F>
-static void launch_damp(const NearArgs& a, int nblocks, int damp, hipStream_t s) {
+static void launch_damp(const NearArgs& a, int nblocks, int damp, int part, hipStream_t s) {
     switch (damp) {
-        case DAMP_NONE: launch_sets<M, F, DAMP_NONE>(a, nblocks, s); break;
-        case DAMP_THOLE: launch_sets<M, F, DAMP_THOLE>(a, nblocks, s); break;
-        case DAMP_AMOEBA: launch_sets<M, F, DAMP_AMOEBA>(a, nblocks, s); break;
-        default: launch_sets<M, F, DAMP_ERF>(a, nblocks, s); break;
+        case DAMP_NONE: launch_sets<M, F, DAMP_NONE>(a, nblocks, part, s); break;
+        case DAMP_THOLE: launch_sets<M, F, DAMP_THOLE>(a, nblocks, part, s); break;
+        case DAMP_AMOEBA: launch_sets<M, F, DAMP_AMOEBA>(a, nblocks, part, s); break;
+        default: launch_sets<M, F, DAMP_ERF>(a, nblocks, part, s); break;
     }
 }
 
 template <int M>
-static void launch_f(const NearArgs& a, int nblocks, int F, int damp, hipStream_t s) {
+static void launch_f(const NearArgs& a, int nblocks, int F, int damp, int part, hipStream_t s) {
     switch (F) {
-        case 0: launch_damp<M, 0>(a, nblocks, damp, s); break;
-        case 1: launch_damp<M, 1>(a, nblocks, damp, s); break;
-        default: launch_damp<M, 2>(a, nblocks, damp, s); break;
+        case 0: launch_damp<M, 0>(a, nblocks, damp, part, s); break;
+        case 1: launch_damp<M, 1>(a, nblocks, damp, part, s); break;
+        default: launch_damp<M, 2>(a, nblocks, damp, part, s); break;
     }
 }
 
 bool near_supported(int m, int F) { return m >= 0 && m <= NEAR_MMAX && F >= 0 && F <= NEAR_FMAX; }
 
-void launch_near(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s) {
+void launch_near_part(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s) {
     if (nblocks <= 0) return;
     switch (m) {
-        case 0: launch_f<0>(a, nblocks, F, damp, s); break;
-        case 1: launch_f<1>(a, nblocks, F, damp, s); break;
-        default: launch_f<2>(a, nblocks, F, damp, s); break;
+        case 0: launch_f<0>(a, nblocks, F, damp, part, s); break;
+        case 1: launch_f<1>(a, nblocks, F, damp, part, s); break;
+        default: launch_f<2>(a, nblocks, F, damp, part, s); break;
     }
     FMMH_HIP(hipGetLastError());
+}
+
+void launch_near(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s) {
+    launch_near_part(a, nblocks, m, F, damp, 1, s);
+    launch_near_part(a, nblocks, m, F, damp, 2, s);
 }
 
 }  // namespace fmmh
