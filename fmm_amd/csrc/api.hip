@@ -1112,7 +1112,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     double* CM = c->buf<double>("e_cm", (size_t)nn * nm * S);
     double* LE = c->buf<double>("e_le", (size_t)nn * nm * S);
     UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
-              tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, S, k.dim};
+              tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, S, k.dim, tb.xyz};
     launch_p2m(up, nl, s);
     for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
         int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
@@ -1151,7 +1151,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->mark("m2l");
     if (P.on) run_periodic(c, tr, tb, P, k, p, CM, LE);
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
-                tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S};
+                tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S, tb.xyz};
     auto downward = [&]() {
         for (int lev = 2; lev <= tr.max_depth; ++lev) {
             int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
@@ -1656,7 +1656,7 @@ int fmmh_get_multipoles(fmmh_context* ctx, int32_t comm, const double* coordinat
         launch_gather(g, s);
         double* CM = c->buf<double>("e_cm", (size_t)nn * nm * ndens);
         UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
-                  tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, ndens, dim};
+                  tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, ndens, dim, tb.xyz};
         launch_p2m(up, tr.num_leaves, s);
         for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
             int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
@@ -1741,7 +1741,7 @@ int fmmh_interpolate_electron_fields(fmmh_context* ctx, int32_t comm, const doub
         FMMH_HIP(hipMemcpy2DAsync(F, n * sizeof(double), field, ld_field * sizeof(double), n * sizeof(double),
                                   (size_t)nf * ndens, hipMemcpyHostToDevice, s));
         DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
-                    tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, F, n, nf, nn, ndens};
+                    tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, F, n, nf, nn, ndens, tb.xyz};
         for (int lev = 2; lev <= tr.max_depth; ++lev) {
             int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
             launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
@@ -1984,7 +1984,7 @@ int fmmh_selector_train(fmmh_context* ctx, const double* coordinates, int64_t ld
         launch_gather(g, s);
         double* CM = c->buf<double>("e_cm", (size_t)nn * nm);
         UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
-                  tr.t.children, CM, nm, cap, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, 1, dim};
+                  tr.t.children, CM, nm, cap, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, 1, dim, tb.xyz};
         launch_p2m(up, tr.num_leaves, s);
         for (int lev = tr.max_depth - 1; lev >= 1; --lev)
             launch_m2m(up, tr.t.level_nodes + tr.level_off[lev], tr.level_off[lev + 1] - tr.level_off[lev], s);

@@ -31,9 +31,33 @@ __device__ __forceinline__ void build_mono(double* mono, int nm, double dx, doub
     }
 }
 
+// The same monomials for nv displacement vectors (d[v] = (dx, dy, dz)), built
+// by the whole workgroup: per-axis powers first, then one product per entry
+// (xyz[e] = ex | ey << 8 | ez << 16).  mono is [nv][ld], pw holds 3 (PMAX + 1)
+// doubles per vector.  Ends with a barrier.
+__device__ __forceinline__ void build_mono_block(double* mono, int ld, int nm, int nv, const double* d3,
+                                                 double* pw, const int* __restrict__ xyz, int p) {
+    for (int x = threadIdx.x; x < 3 * nv; x += blockDim.x) {
+        const double d = d3[x];
+        double* w = pw + x * (PMAX + 1);
+        double v = 1.0;
+        for (int k = 0; k <= p; ++k) { w[k] = v; v *= d; }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < nv * nm; x += blockDim.x) {
+        const int v = x / nm, e = x - v * nm;
+        const int pk = xyz[e];
+        const double* w = pw + 3 * v * (PMAX + 1);
+        mono[v * ld + e] = (w[pk & 255] * w[(PMAX + 1) + ((pk >> 8) & 255)]) * w[2 * (PMAX + 1) + (pk >> 16)];
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
     __shared__ double mono[UP_CHUNK][MAX_NM];
     __shared__ double mch[UP_CHUNK][UP_NSRC];
+    __shared__ double d3[UP_CHUNK * 3];
+    __shared__ double pw[UP_CHUNK * 3 * (PMAX + 1)];
     const int node = a.leaf_node[blockIdx.x];
     const int set = blockIdx.y;
     const int s0 = a.start[node], c = a.count[node];
@@ -47,10 +71,13 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
     for (int q = 0; q < (MAX_NM + 127) / 128; ++q) acc[q] = 0.0;
     for (int j0 = 0; j0 < c; j0 += UP_CHUNK) {
         const int cc = min(UP_CHUNK, c - j0);
-        if ((int)threadIdx.x < cc) {
-            const int j = s0 + j0 + threadIdx.x;
-            build_mono(mono[threadIdx.x], nm, a.xs[j] - cx, a.ys[j] - cy, a.zs[j] - cz, a.mono_parent, a.mono_axis);
+        if ((int)threadIdx.x < 3 * cc) {
+            const int jj = threadIdx.x / 3, ax = threadIdx.x - 3 * jj;
+            const int j = s0 + j0 + jj;
+            d3[threadIdx.x] = ax == 0 ? a.xs[j] - cx : (ax == 1 ? a.ys[j] - cy : a.zs[j] - cz);
         }
+        __syncthreads();
+        build_mono_block(&mono[0][0], MAX_NM, nm, cc, d3, pw, a.xyz, a.p);
         for (int x = threadIdx.x; x < cc * nsrc; x += blockDim.x) {
             const int jj = x / nsrc, sc = x % nsrc;
             mch[jj][sc] = msr[(int64_t)sc * a.ldm + s0 + j0 + jj];
@@ -79,7 +106,9 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
 }
 
 __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ nodes) {
-    __shared__ double mono[MAX_NM];
+    __shared__ double monos[8][MAX_NM];
+    __shared__ double d3[24];
+    __shared__ double pw[24 * (PMAX + 1)];
     const int node = nodes[blockIdx.x];
     const int nm = a.nm;
     double* __restrict__ CM = a.CM + (int64_t)blockIdx.y * a.num_nodes * nm;
@@ -87,14 +116,19 @@ __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ n
     double acc[(MAX_NM + 127) / 128];
 #pragma unroll
     for (int q = 0; q < (MAX_NM + 127) / 128; ++q) acc[q] = 0.0;
+    if (threadIdx.x < 24) {
+        const int o = threadIdx.x / 3, ax = threadIdx.x - 3 * o;
+        const int ch = a.children[node * 8 + o];
+        d3[threadIdx.x] = ch < 0 ? 0.0 : (ax == 0 ? a.cx[ch] - cx : (ax == 1 ? a.cy[ch] - cy : a.cz[ch] - cz));
+    }
+    __syncthreads();
+    build_mono_block(&monos[0][0], MAX_NM, nm, 8, d3, pw, a.xyz, a.p);
     bool any = false;
     for (int o = 0; o < 8; ++o) {
         const int ch = a.children[node * 8 + o];
         if (ch < 0) continue;
         any = true;
-        if (threadIdx.x == 0)
-            build_mono(mono, nm, a.cx[ch] - cx, a.cy[ch] - cy, a.cz[ch] - cz, a.mono_parent, a.mono_axis);
-        __syncthreads();
+        const double* __restrict__ mono = monos[o];
         const double* src = CM + (int64_t)ch * nm;
 #pragma unroll
         for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
@@ -106,7 +140,6 @@ __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ n
                 acc[q] += sum;
             }
         }
-        __syncthreads();
     }
     if (!any) return;   // leaves keep their P2M moments
 #pragma unroll
@@ -120,15 +153,19 @@ __global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ n
 __global__ __launch_bounds__(128) void k_l2l(DownArgs a, const int* __restrict__ nodes) {
     __shared__ double mono[MAX_NM];
     __shared__ double lp[MAX_NM];
+    __shared__ double d3[3];
+    __shared__ double pw[3 * (PMAX + 1)];
     const int node = nodes[blockIdx.x];
     const int par = a.parent[node];
     const int nm = a.nm;
     double* __restrict__ LE = a.LE + (int64_t)blockIdx.y * a.num_nodes * nm;
-    if (threadIdx.x == 0)
-        build_mono(mono, nm, a.cx[node] - a.cx[par], a.cy[node] - a.cy[par], a.cz[node] - a.cz[par], a.mono_parent,
-                   a.mono_axis);
+    if (threadIdx.x < 3) {
+        const int ax = threadIdx.x;
+        d3[ax] = ax == 0 ? a.cx[node] - a.cx[par] : (ax == 1 ? a.cy[node] - a.cy[par] : a.cz[node] - a.cz[par]);
+    }
     for (int s = threadIdx.x; s < nm; s += blockDim.x) lp[s] = LE[(int64_t)par * nm + s];
     __syncthreads();
+    build_mono_block(mono, MAX_NM, nm, 1, d3, pw, a.xyz, a.p);
     for (int t = threadIdx.x; t < nm; t += blockDim.x) {
         const int b0 = a.l2l.off[t], b1 = a.l2l.off[t + 1];
         double sum = 0.0;

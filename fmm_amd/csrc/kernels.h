@@ -139,6 +139,7 @@ struct UpArgs {
     int num_nodes;            // CM is [set][node][nm]; ms_raw is [set][component][ldm]
     int nsets;                // one grid row per set
     int dim;                  // multipole components per site and set
+    const int* xyz;           // packed index -> ex | ey << 8 | ez << 16
 };
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s);
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s);
@@ -348,6 +349,7 @@ struct DownArgs {
     int nf;
     int num_nodes;                         // LE is [set][node][nm]; near and field are [set][component][ld]
     int nsets;                             // one grid row per set
+    const int* xyz;                        // packed index -> ex | ey << 8 | ez << 16
 };
 void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s);
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s);

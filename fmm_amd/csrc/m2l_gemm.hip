@@ -185,9 +185,11 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(const E* __restrict__ pbuf, 
         double sum = 0.0;
         if (order) {
             const int dt = tdeg[t];
+#pragma unroll 8
             for (int e = e0; e < e1; ++e)
                 if (dt <= order[e]) sum += (double)pbuf[(int64_t)e * nm + t];
         } else {
+#pragma unroll 8
             for (int e = e0; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nm + t];
         }
         double f = rinv;
