@@ -51,6 +51,7 @@ _PROTOS = {
     "fmmh_selector_grad": (i32, [vp, vp, vp, vp, vp, i32, _c.c_float, vp]),
     "fmmh_selector_predict": (i32, [vp, vp, vp, vp, i32, vp]),
     "fmmh_selection_stats": (i32, [vp, vp, i32]),
+    "fmmh_tuning": (i32, [vp, vp, vp, vp, i32]),
 }
 
 EXPORTED = tuple(_PROTOS)

@@ -114,6 +114,12 @@ struct fmmh_context {
     double sel_kappa = 1.0;           // FMMH_OPT_ORDER_SAFETY
     int spot_checks = 0;              // FMMH_OPT_SPOT_CHECKS
     int sel_force_full = 0;           // FMMH_OPT_ORDER_FORCE_FULL: selection machinery on, every pair at p
+    // Tile-shape autotuner (FMMH_OPT_AUTOTUNE): the many-sets M2L variant chosen per
+    // (order, precision, sets in the chunk, learned orders on), timed on the first call that needs it.
+    int autotune = 1;
+    std::map<long long, int> tuned;
+    std::vector<double> tune_ms;      // last tuning: milliseconds per variant (-1: not valid)
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
     // last call with selection: spot-check error, fallback flag, tolerance, order histogram
     double sel_stats[8] = {-1, 0, 0, 0, 0, 0, 0, 0};
     unsigned long long sel_hist[PMAX + 1] = {};
@@ -924,8 +930,38 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         if (f32) launch_to_f32(CMs, CMs32, (int64_t)node_stride * s_ld, s);
         M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,
                    nm, p, ns, s_ld, nn, order, tb.gemm_fidx, CMs32};
-        if (mfma) launch_m2l_mfma(ma, nn, s);
-        else launch_m2l(ma, nn, s);
+        if (!mfma) {
+            launch_m2l(ma, nn, s);
+            continue;
+        }
+        const long long key = ((long long)p << 32) | ((long long)(f32 ? 1 : 0) << 24) | ((long long)ns << 8) |
+                              (order ? 1 : 0);
+        auto it = c->tuned.find(key);
+        if (it != c->tuned.end() || !c->autotune) {
+            launch_m2l_mfma(ma, nn, it != c->tuned.end() ? it->second : m2l_mfma_default_variant(ma), s);
+            continue;
+        }
+        // first call of this shape: time every valid variant on this call's pairs; each one writes
+        // the same local expansions (same sums in the same order), so the last run's output stands
+        if (!c->tune_ev[0]) {
+            FMMH_HIP(hipEventCreate(&c->tune_ev[0]));
+            FMMH_HIP(hipEventCreate(&c->tune_ev[1]));
+        }
+        int best = m2l_mfma_default_variant(ma);
+        float best_ms = 1e30f;
+        c->tune_ms.assign(M2L_MFMA_VARIANTS, -1.0);
+        for (int v = 0; v < M2L_MFMA_VARIANTS; ++v) {
+            if (!m2l_mfma_variant_ok(ma, v)) continue;
+            FMMH_HIP(hipEventRecord(c->tune_ev[0], s));
+            launch_m2l_mfma(ma, nn, v, s);
+            FMMH_HIP(hipEventRecord(c->tune_ev[1], s));
+            FMMH_HIP(hipEventSynchronize(c->tune_ev[1]));
+            float ms = 0.f;
+            FMMH_HIP(hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
+            c->tune_ms[v] = ms;
+            if (ms < best_ms) { best_ms = ms; best = v; }
+        }
+        c->tuned[key] = best;
     }
 }
 
@@ -1837,6 +1873,9 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         } else if (key == FMMH_OPT_ORDER_SAFETY) {
             if (!(value > 0.0)) throw Error(FMMH_ERR_ARG, "order safety factor must be > 0");
             c->sel_kappa = value;
+        } else if (key == FMMH_OPT_AUTOTUNE) {
+            c->autotune = value != 0.0 ? 1 : 0;
+            if (value < 0.0) c->tuned.clear();   // forget earlier choices
         } else if (key == FMMH_OPT_ORDER_FORCE_FULL) {
             c->sel_force_full = value != 0.0 ? 1 : 0;
         } else if (key == FMMH_OPT_M2L_MFMA) {
@@ -2224,5 +2263,23 @@ int fmmh_selection_stats(fmmh_context* ctx, double* out, int32_t cap) {
 }
 
 int32_t fmmh_selector_nparam(void) { return SEL_NPARAM; }
+
+int fmmh_tuning(fmmh_context* ctx, int64_t* keys, int32_t* variants, double* last_ms, int32_t cap) {
+    int count = 0;
+    int rc = guarded([&] {
+        fmmh_context* c = resolve(ctx);
+        for (auto& kv : c->tuned) {
+            if (count < cap) {
+                if (keys) keys[count] = kv.first;
+                if (variants) variants[count] = kv.second;
+            }
+            ++count;
+        }
+        if (last_ms)
+            for (int v = 0; v < M2L_MFMA_VARIANTS; ++v)
+                last_ms[v] = v < (int)c->tune_ms.size() ? c->tune_ms[v] : -1.0;
+    });
+    return rc == FMMH_OK ? count : -rc;
+}
 
 }  // extern "C"

@@ -215,7 +215,11 @@ void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s);
 // per pair with FP64 accumulation when CMs32 is set).  CMs is set-fastest with
 // s_ld >= nsets, a multiple of 16.  Supported from 16 sets on, orders <= 10.
 bool m2l_mfma_supported(int p, int nsets);
-void launch_m2l_mfma(const M2LArgs& a, int num_nodes, hipStream_t s);
+// Tile-shape variants 0..3 (see m2l.hip); an invalid variant falls back to the default.
+constexpr int M2L_MFMA_VARIANTS = 4;
+bool m2l_mfma_variant_ok(const M2LArgs& a, int variant);
+int m2l_mfma_default_variant(const M2LArgs& a);
+void launch_m2l_mfma(const M2LArgs& a, int num_nodes, int variant, hipStream_t s);
 
 // Class-grouped M2L: all pairs of one displacement class share one Hankel
 // matrix H_c[t][s] = scale_c(t) T~_c[s + t], so per class the M2L is the GEMM

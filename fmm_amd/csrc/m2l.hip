@@ -363,7 +363,16 @@ constexpr int mfma_waves() { return ((ncomp(P) + 15) / 16) < 4 ? ((ncomp(P) + 15
 // unscaled A, and each tile is scaled by its rows' powers of two while it is
 // added to the FP64 accumulators (rows beyond the pair's order are skipped
 // there).
+// The next pair's class tensor and, where 2 buffers fit in 64 KB (STAGE), its
+// source block (n_m x 16 NTL sets) are loaded into registers with wide
+// coalesced loads while the current pair is multiplied, then published to the
+// other LDS buffer: the k-loop reads only LDS.
 template <int P, bool F32, int NTL>
+constexpr bool mfma_stage_b() {
+    return 2 * ncomp(P) * 16 * NTL * (F32 ? 4 : 8) <= 65536;
+}
+
+template <int P, bool F32, int NTL, bool STG>
 __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int NM = ncomp(P);
     constexpr int NT = ncomp(2 * P);
@@ -373,16 +382,21 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int MW = (MT + NW - 1) / NW;  // row tiles per wavefront
     constexpr int NTH = 64 * NW;
     constexpr int NL = (NT + NTH - 1) / NTH;
+    constexpr bool STAGE = STG && mfma_stage_b<P, F32, NTL>();
+    constexpr int BW = 16 * NTL;            // sets per workgroup
+    constexpr int NBL = STAGE ? (NM * BW + NTH - 1) / NTH : 1;
     using TE = typename std::conditional<F32, float, double>::type;
     using Acc = typename std::conditional<F32, v4f, v4d>::type;
     __shared__ TE Tl[2][NT];
+    __shared__ TE Bl[STAGE ? 2 : 1][STAGE ? NM * BW : 1];
     __shared__ int sdec[4 * KS];            // decoded source components (degree | w << 8 | z << 16)
     const int i = blockIdx.x;
-    const int sg = blockIdx.y * M2L_SETS_TILE * NTL;   // first set of this workgroup
+    const int sg = blockIdx.y * BW;         // first set of this workgroup
     const int tid = threadIdx.x;
     const int w = tid >> 6, lane = tid & 63;
     const int row = lane & 15, kq = lane >> 4;
     const int e0 = a.off[i], e1 = a.off[i + 1];
+    const TE* __restrict__ cms = F32 ? (const TE*)a.CMs32 : (const TE*)a.CMs;
     for (int x = tid; x < 4 * KS; x += NTH) sdec[x] = x < NM ? decode_target(x) : 0;
     // A fragment of row tile mt holds H[t = 16 mt + row][s = 4 ks + kq]; the
     // packed index of s + t is tet(|s| + |t|) + tri(w_s + w_t) + (z_s + z_t)
@@ -408,23 +422,44 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     for (int mi = 0; mi < MW; ++mi)
 #pragma unroll
         for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    // prefetch registers: class tensor (NL) and source block (NBL) of a pair
     double pre[NL];
-    if (e0 < e1) {
-        const double* T0 = a.Tc + (int64_t)a.pair_class[e0] * NT;
+    TE preb[NBL];
+    auto fetch = [&](int e) {
+        const double* Tn = a.Tc + (int64_t)a.pair_class[e] * NT;
+#pragma unroll
+        for (int x = 0; x < NL; ++x) pre[x] = (tid + NTH * x < NT) ? Tn[tid + NTH * x] : 0.0;
+        if constexpr (STAGE) {
+            const int j = src_node(a.src[e]);
+            const int nb = (a.order ? ncomp(a.order[e]) : NM) * BW;
+            const TE* __restrict__ B = cms + (int64_t)j * NM * a.s_ld + sg;
+#pragma unroll
+            for (int x = 0; x < NBL; ++x) {
+                const int y = tid + NTH * x;   // y = s * BW + set
+                preb[x] = y < nb ? B[(int64_t)(y / BW) * a.s_ld + (y % BW)] : TE(0);
+            }
+        }
+    };
+    auto publish = [&](int buf) {
 #pragma unroll
         for (int x = 0; x < NL; ++x)
-            if (tid + NTH * x < NT) Tl[0][tid + NTH * x] = (TE)T0[tid + NTH * x];
+            if (tid + NTH * x < NT) Tl[buf][tid + NTH * x] = (TE)pre[x];
+        if constexpr (STAGE) {
+#pragma unroll
+            for (int x = 0; x < NBL; ++x)
+                if (tid + NTH * x < NM * BW) Bl[buf][tid + NTH * x] = preb[x];
+        }
+    };
+    if (e0 < e1) {
+        fetch(e0);
+        publish(0);
     }
     __syncthreads();
     const int li = a.level[i];
     for (int e = e0; e < e1; ++e) {
         const int cur = (e - e0) & 1;
         const bool more = e + 1 < e1;
-        if (more) {
-            const double* Tn = a.Tc + (int64_t)a.pair_class[e + 1] * NT;
-#pragma unroll
-            for (int x = 0; x < NL; ++x) pre[x] = (tid + NTH * x < NT) ? Tn[tid + NTH * x] : 0.0;
-        }
+        if (more) fetch(e + 1);
         const int j = src_node(a.src[e]);
         const int dl = li - a.level[j];
         // learned order q of this pair: only the leading ncomp(q) rows and columns
@@ -449,7 +484,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
 #pragma unroll
                 for (int nt = 0; nt < NTL; ++nt) part[mi][nt] = Acc{0, 0, 0, 0};
         }
-#pragma unroll 8
+#pragma unroll 4
         for (int ks = 0; ks < KSq; ++ks) {
             const int s = ks * 4 + kq;
             const int ds = sdec[s];
@@ -457,8 +492,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
             TE b[NTL];
 #pragma unroll
             for (int nt = 0; nt < NTL; ++nt) {
-                if constexpr (F32) b[nt] = sv ? a.CMs32[boff + (int64_t)s * a.s_ld + 16 * nt] : 0.0f;
-                else b[nt] = sv ? a.CMs[boff + (int64_t)s * a.s_ld + 16 * nt] : 0.0;
+                if constexpr (STAGE) b[nt] = sv ? Bl[cur][s * BW + 16 * nt + row] : TE(0);
+                else b[nt] = sv ? cms[boff + (int64_t)s * a.s_ld + 16 * nt] : TE(0);
             }
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi) {
@@ -494,11 +529,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
                         if (ok) acc[mi][nt][q] += ldexp((double)part[mi][nt][q], shq);
                 }
         }
-        if (more) {
-#pragma unroll
-            for (int x = 0; x < NL; ++x)
-                if (tid + NTH * x < NT) Tl[1 - cur][tid + NTH * x] = (TE)pre[x];
-        }
+        if (more) publish(1 - cur);
         __syncthreads();
     }
     const double rinv = 1.0 / a.r[i];
@@ -523,40 +554,71 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
 
 bool m2l_mfma_supported(int p, int nsets) { return p <= M2L_MFMA_PMAX && nsets >= 16; }
 
-// 32 sets and more: two 16-set tiles per workgroup
+// Variants of the many-sets form (tile shapes): bit 0 = two 16-set tiles per
+// workgroup (needs s_ld a multiple of 32), bit 1 = source blocks read from
+// global memory instead of staged in LDS.  Every variant computes the same
+// sums in the same order; they differ in speed only.
 template <int P, bool F32>
-static void launch_mfma_one(const M2LArgs& a, int num_nodes, hipStream_t s) {
+static bool mfma_variant_ok(const M2LArgs& a, int v) {
+    const bool two = v & 1, stage = !(v & 2);
+    if (two && !(a.nsets > M2L_SETS_TILE && a.s_ld % (2 * M2L_SETS_TILE) == 0)) return false;
+    if (stage && !(two ? mfma_stage_b<P, F32, 2>() : mfma_stage_b<P, F32, 1>())) return false;
+    return true;
+}
+
+template <int P, bool F32>
+static void launch_mfma_one(const M2LArgs& a, int num_nodes, int v, hipStream_t s) {
     const dim3 b(64 * mfma_waves<P, F32>());
-    if (a.nsets > M2L_SETS_TILE && a.s_ld % (2 * M2L_SETS_TILE) == 0) {
-        const dim3 g(num_nodes, (a.nsets + 2 * M2L_SETS_TILE - 1) / (2 * M2L_SETS_TILE));
-        hipLaunchKernelGGL((k_m2l_mfma<P, F32, 2>), g, b, 0, s, a);
-    } else {
-        const dim3 g(num_nodes, (a.nsets + M2L_SETS_TILE - 1) / M2L_SETS_TILE);
-        hipLaunchKernelGGL((k_m2l_mfma<P, F32, 1>), g, b, 0, s, a);
+    const int sets_per = (v & 1) ? 2 * M2L_SETS_TILE : M2L_SETS_TILE;
+    const dim3 g(num_nodes, (a.nsets + sets_per - 1) / sets_per);
+    switch (v) {
+        case 0: hipLaunchKernelGGL((k_m2l_mfma<P, F32, 1, true>), g, b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_m2l_mfma<P, F32, 2, true>), g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_m2l_mfma<P, F32, 1, false>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_m2l_mfma<P, F32, 2, false>), g, b, 0, s, a); break;
     }
+}
+
+template <int P, bool F32>
+static bool mfma_p(const M2LArgs& a, int num_nodes, int v, bool launch, hipStream_t s) {
+    if (!mfma_variant_ok<P, F32>(a, v)) return false;
+    if (launch) launch_mfma_one<P, F32>(a, num_nodes, v, s);
+    return true;
 }
 
 template <bool F32>
-static void launch_mfma_p(const M2LArgs& a, int num_nodes, hipStream_t s) {
+static bool mfma_dispatch(const M2LArgs& a, int num_nodes, int v, bool launch, hipStream_t s) {
     switch (a.p) {
-        case 0: launch_mfma_one<0, F32>(a, num_nodes, s); break;
-        case 1: launch_mfma_one<1, F32>(a, num_nodes, s); break;
-        case 2: launch_mfma_one<2, F32>(a, num_nodes, s); break;
-        case 3: launch_mfma_one<3, F32>(a, num_nodes, s); break;
-        case 4: launch_mfma_one<4, F32>(a, num_nodes, s); break;
-        case 5: launch_mfma_one<5, F32>(a, num_nodes, s); break;
-        case 6: launch_mfma_one<6, F32>(a, num_nodes, s); break;
-        case 7: launch_mfma_one<7, F32>(a, num_nodes, s); break;
-        case 8: launch_mfma_one<8, F32>(a, num_nodes, s); break;
-        case 9: launch_mfma_one<9, F32>(a, num_nodes, s); break;
-        default: launch_mfma_one<10, F32>(a, num_nodes, s); break;
+        case 0: return mfma_p<0, F32>(a, num_nodes, v, launch, s);
+        case 1: return mfma_p<1, F32>(a, num_nodes, v, launch, s);
+        case 2: return mfma_p<2, F32>(a, num_nodes, v, launch, s);
+        case 3: return mfma_p<3, F32>(a, num_nodes, v, launch, s);
+        case 4: return mfma_p<4, F32>(a, num_nodes, v, launch, s);
+        case 5: return mfma_p<5, F32>(a, num_nodes, v, launch, s);
+        case 6: return mfma_p<6, F32>(a, num_nodes, v, launch, s);
+        case 7: return mfma_p<7, F32>(a, num_nodes, v, launch, s);
+        case 8: return mfma_p<8, F32>(a, num_nodes, v, launch, s);
+        case 9: return mfma_p<9, F32>(a, num_nodes, v, launch, s);
+        default: return mfma_p<10, F32>(a, num_nodes, v, launch, s);
     }
 }
 
-void launch_m2l_mfma(const M2LArgs& a, int num_nodes, hipStream_t s) {
+bool m2l_mfma_variant_ok(const M2LArgs& a, int v) {
+    return a.CMs32 ? mfma_dispatch<true>(a, 0, v, false, nullptr) : mfma_dispatch<false>(a, 0, v, false, nullptr);
+}
+
+// Default variant without tuning: two tiles where their staged blocks fit, else one tile, staged.
+int m2l_mfma_default_variant(const M2LArgs& a) {
+    if (m2l_mfma_variant_ok(a, 1)) return 1;
+    if (m2l_mfma_variant_ok(a, 0)) return 0;
+    return m2l_mfma_variant_ok(a, 3) ? 3 : 2;
+}
+
+void launch_m2l_mfma(const M2LArgs& a, int num_nodes, int variant, hipStream_t s) {
     if (num_nodes <= 0) return;
-    if (a.CMs32) launch_mfma_p<true>(a, num_nodes, s);
-    else launch_mfma_p<false>(a, num_nodes, s);
+    const int v = m2l_mfma_variant_ok(a, variant) ? variant : m2l_mfma_default_variant(a);
+    if (a.CMs32) mfma_dispatch<true>(a, num_nodes, v, true, s);
+    else mfma_dispatch<false>(a, num_nodes, v, true, s);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -401,6 +401,21 @@ class Context:
             check(self._lib.fmmh_set_option(self._h, 9, float(safety)))
         check(self._lib.fmmh_set_option(self._h, 10, 1.0 if force_full else 0.0))
 
+    def set_autotune(self, on=True, reset=False):
+        """Tile-shape autotuner of the many-sets M2L (on by default); reset forgets earlier choices."""
+        check(self._lib.fmmh_set_option(self._h, 11, -1.0 if reset else (1.0 if on else 0.0)))
+
+    def tuning(self):
+        """{key: variant} of the autotuner and the per-variant times (ms) of its last tuning run."""
+        n = self._lib.fmmh_tuning(self._h, None, None, None, 0)
+        if n < 0:
+            check(-n)
+        keys = np.zeros(max(n, 1), np.int64)
+        var = np.zeros(max(n, 1), np.int32)
+        ms = np.zeros(4)
+        self._lib.fmmh_tuning(self._h, _p(keys), _p(var), _p(ms), n)
+        return {int(k): int(v) for k, v in zip(keys[:n], var[:n])}, ms
+
     def selection_stats(self):
         out = np.zeros(19)
         check(self._lib.fmmh_selection_stats(self._h, _p(out), 19))

@@ -269,8 +269,13 @@ enum fmmh_option {
                                      RMS error exceeds ORDER_TOL, redo M2L..L2P at expansion_order
                                      (fallback).  0 (default): no check. */
     FMMH_OPT_ORDER_SAFETY = 9,    /* factor on the per-pair tolerance (default 1) */
-    FMMH_OPT_ORDER_FORCE_FULL = 10 /* != 0: run the selection machinery but keep every pair at
+    FMMH_OPT_ORDER_FORCE_FULL = 10, /* != 0: run the selection machinery but keep every pair at
                                      expansion_order (ablation) */
+    FMMH_OPT_AUTOTUNE = 11        /* 1 (default): the first call of each M2L shape for many sets (order,
+                                     precision, sets per pass, learned orders) times every tile-shape
+                                     variant of that kernel on its own pairs and keeps the fastest for the
+                                     context's later calls (results do not depend on the choice).  0: fixed
+                                     defaults.  -1: forget the choices made so far and keep tuning on. */
 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
@@ -333,6 +338,12 @@ int fmmh_selector_predict(fmmh_context* ctx, const float* weights, const float* 
  * [4] M2L flops at expansion_order, [5] mean order, [6] 1 if selection ran,
  * [7] RMS near field used for tau, [8 + q] pairs at order q (q = 0..10). */
 int fmmh_selection_stats(fmmh_context* ctx, double* out, int32_t cap);
+
+/* Tile-shape choices of the autotuner (FMMH_OPT_AUTOTUNE): up to cap (key, variant) pairs, key = order << 32 |
+ * fp32 << 24 | sets << 8 | learned-orders, variant bit 0 = 32 sets per workgroup, bit 1 = source blocks from
+ * global memory instead of LDS; last_ms[4] = the times of the last tuning (-1: variant not valid).  Returns the
+ * number of choices held. */
+int fmmh_tuning(fmmh_context* ctx, int64_t* keys, int32_t* variants, double* last_ms, int32_t cap);
 
 #ifdef __cplusplus
 }

@@ -182,3 +182,33 @@ def test_sets_errors(gpu_ctx):
         gpu_ctx.field_fmm_sets(0, x, m, e, 0.5, 64, 5, 1)
     with pytest.raises(FmmError, match="mode"):
         gpu_ctx.set_m2l_mfma(5)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("nsets", [16, 20, 32])
+def test_autotuned_tile_shape_gives_identical_results(nsets, mixed):
+    """The first many-sets call times every valid tile-shape variant of the
+    per-target MFMA M2L and keeps the fastest; every variant computes the same
+    sums in the same order, so the tuned result equals the untuned default bit
+    for bit, and later calls reuse the choice."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from fmm_amd.octree import Context
+    g = load_golden("water999_dipole_orders.npz")
+    x, e = g["coordinates"], g["exclusions"]
+    m = _sets(x.shape[0], 4, nsets, 29)
+    ctx = Context(0)
+    try:
+        ctx.set_precision(mixed)
+        ctx.set_autotune(False)
+        ref = ctx.field_fmm_sets(0, x, m, e, 0.5, 64, 7, 1)
+        ctx.set_autotune(True)
+        tuned = ctx.field_fmm_sets(0, x, m, e, 0.5, 64, 7, 1)
+        choices, ms = ctx.tuning()
+        assert len(choices) == 1 and (ms >= 0).sum() >= 1
+        again = ctx.field_fmm_sets(0, x, m, e, 0.5, 64, 7, 1)
+        assert ctx.tuning()[0] == choices
+        assert np.array_equal(tuned, ref) and np.array_equal(again, ref)
+    finally:
+        ctx.close()
