@@ -923,13 +923,19 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     double* CMs = c->buf<double>("m_cms", node_stride * s_ld_max);
     const bool f32 = mfma && c->precision == 1;
     float* CMs32 = f32 ? c->buf<float>("m_cms32", node_stride * s_ld_max) : nullptr;
+    void* Tcp = nullptr;
+    if (mfma && ncls > 0) {
+        const size_t tb_bytes = (size_t)ncls * m2l_mfma_class_stride(p, f32) * (f32 ? sizeof(float) : sizeof(double));
+        Tcp = c->buf<char>("m_tcp", tb_bytes);
+        launch_m2l_class_pad(Tc, Tcp, ncls, p, f32, s);
+    }
     for (int c0 = 0; c0 < nsets; c0 += chunk) {
         const int ns = std::min(chunk, nsets - c0);
         const int s_ld = mfma ? pad_sets(ns) : 0;
         launch_scale_sources(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, s_ld, s);
         if (f32) launch_to_f32(CMs, CMs32, (int64_t)node_stride * s_ld, s);
         M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,
-                   nm, p, ns, s_ld, nn, order, tb.gemm_fidx, CMs32};
+                   nm, p, ns, s_ld, nn, order, tb.gemm_fidx, CMs32, Tcp};
         if (!mfma) {
             launch_m2l(ma, nn, s);
             continue;

@@ -206,7 +206,11 @@ struct M2LArgs {
     const unsigned char* order;          // per CSR position: learned expansion order (null: p; MFMA form only)
     const int* fidx;                     // MFMA form: A-fragment index table (see M2LGemmArgs)
     const float* CMs32;                  // MFMA form, mixed precision: FP32 copy of CMs (null: FP64)
+    const void* Tcp;                     // MFMA form: class tensors in its precision, stride m2l_mfma_class_stride
 };
+// Padded class-tensor copy for the MFMA form (16-byte aligned rows of FP32 or FP64).
+int m2l_mfma_class_stride(int p, bool f32);
+void launch_m2l_class_pad(const double* Tc, void* Tcp, int ncls, int p, bool f32, hipStream_t s);
 void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
                           int nsets, int s_ld_out, hipStream_t s);
 void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s);

@@ -367,17 +367,33 @@ constexpr int mfma_waves() { return ((ncomp(P) + 15) / 16) < 4 ? ((ncomp(P) + 15
 // source block (n_m x 16 NTL sets) are loaded into registers with wide
 // coalesced loads while the current pair is multiplied, then published to the
 // other LDS buffer: the k-loop reads only LDS.
-template <int P, bool F32, int NTL>
-constexpr bool mfma_stage_b() {
-    return 2 * ncomp(P) * 16 * NTL * (F32 ? 4 : 8) <= 65536;
+// One 16-byte chunk per lane from global memory straight into LDS: lane l of the
+// wave writes bytes [16 l, 16 l + 16) after the wave-uniform LDS address lds.
+__device__ __forceinline__ void dma16(const void* g, void* lds) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
 }
 
+template <int P, bool F32>
+constexpr int mfma_kpad() { return ((ncomp(P) + 3) / 4 + 1) / 2 * 2; }   // k-steps, even
+
+template <int P, bool F32, int NTL>
+constexpr bool mfma_stage_b() {
+    return 2 * 4 * (mfma_kpad<P, F32>() + 2) * 16 * NTL * (F32 ? 4 : 8) <= 65536;
+}
+
+// The k-loop is branch-free: the A element of lane (row, kq) is read from the
+// LDS class tensor at byte offset G[row part + source part] + z parts, where
+// G[d][w] = sizeof(element) (tet(d) + tri(w)) sits in LDS and the row and
+// source parts are pre-scaled byte offsets; rows past n_m read a valid
+// (finite) element and are never stored; source rows past the pair's order
+// multiply zeros (staged blocks are zero-filled there, global reads are
+// selected to zero).  Two k-steps per iteration.
 template <int P, bool F32, int NTL, bool STG>
 __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int NM = ncomp(P);
     constexpr int NT = ncomp(2 * P);
     constexpr int MT = (NM + 15) / 16;      // 16-row tiles of target coefficients
-    constexpr int KS = (NM + 3) / 4;        // 4-deep steps over source coefficients
+    constexpr int KSP = mfma_kpad<P, F32>();  // 4-deep steps over source coefficients, padded
     constexpr int NW = mfma_waves<P, F32>();  // wavefronts per workgroup, tile mt to wave mt % NW
     constexpr int MW = (MT + NW - 1) / NW;  // row tiles per wavefront
     constexpr int NTH = 64 * NW;
@@ -385,11 +401,19 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr bool STAGE = STG && mfma_stage_b<P, F32, NTL>();
     constexpr int BW = 16 * NTL;            // sets per workgroup
     constexpr int NBL = STAGE ? (NM * BW + NTH - 1) / NTH : 1;
+    constexpr int NGD = 2 * P + 1;
     using TE = typename std::conditional<F32, float, double>::type;
     using Acc = typename std::conditional<F32, v4f, v4d>::type;
-    __shared__ TE Tl[2][NT];
-    __shared__ TE Bl[STAGE ? 2 : 1][STAGE ? NM * BW : 1];
-    __shared__ int sdec[4 * KS];            // decoded source components (degree | w << 8 | z << 16)
+    constexpr int ES = sizeof(TE);
+    constexpr int BROWS = STAGE ? 4 * (KSP + 2) * BW : 1;   // + the pipeline's read-ahead
+    constexpr int CH = 16 / ES;             // elements per 16-byte chunk
+    constexpr int NTP = (NT + CH - 1) / CH * CH;
+    constexpr int CPR = BW / CH;            // chunks per staged source row
+    __shared__ __align__(16) TE Tl[2][NTP];
+    __shared__ __align__(16) TE Bl[2][BROWS];   // staged source block [s][set]
+    __shared__ int sdec[4 * (KSP + 8)];     // source s: 4 (|s| NGD + w_s) | ES z_s << 16 (+ read-ahead)
+    __shared__ int gtab[NGD * NGD];         // ES (tet(d) + tri(w)) at d NGD + w
+    __shared__ unsigned char degl[16 * MT]; // degree of target row t (99 past n_m)
     const int i = blockIdx.x;
     const int sg = blockIdx.y * BW;         // first set of this workgroup
     const int tid = threadIdx.x;
@@ -397,123 +421,225 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     const int row = lane & 15, kq = lane >> 4;
     const int e0 = a.off[i], e1 = a.off[i + 1];
     const TE* __restrict__ cms = F32 ? (const TE*)a.CMs32 : (const TE*)a.CMs;
-    for (int x = tid; x < 4 * KS; x += NTH) sdec[x] = x < NM ? decode_target(x) : 0;
-    // A fragment of row tile mt holds H[t = 16 mt + row][s = 4 ks + kq]; the
-    // packed index of s + t is tet(|s| + |t|) + tri(w_s + w_t) + (z_s + z_t)
-    int tdec[MW];
+    for (int x = tid; x < 4 * (KSP + 8); x += NTH) {
+        const int d = x < NM ? decode_target(x) : 0;
+        sdec[x] = 4 * ((d & 255) * NGD + ((d >> 8) & 255)) | (ES * (d >> 16)) << 16;
+    }
+    for (int x = tid; x < 16 * MT; x += NTH) degl[x] = x < NM ? (decode_target(x) & 255) : 99;
+    for (int x = tid; x < NGD * NGD; x += NTH) gtab[x] = ES * (tet(x / NGD) + tri(x % NGD));
+    // A fragment of row tile mt holds H[t = 16 mt + row][s = 4 ks + kq]
+    int trow[MW], tz[MW], tdeg[MW];
 #pragma unroll
     for (int mi = 0; mi < MW; ++mi) {
         const int t = (w + NW * mi) * 16 + row;
-        tdec[mi] = t < NM ? decode_target(t) : -1;
-    }
-    // FP32 accumulator register q holds row 4 kq + q of its tile: its degree
-    int qdeg[F32 ? MW : 1][4];
-    if constexpr (F32) {
-#pragma unroll
-        for (int mi = 0; mi < MW; ++mi)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int t = (w + NW * mi) * 16 + 4 * kq + q;
-                qdeg[mi][q] = t < NM ? (decode_target(t) & 255) : 99;
-            }
+        const int d = t < NM ? decode_target(t) : 0;
+        trow[mi] = 4 * ((d & 255) * NGD + ((d >> 8) & 255));
+        tz[mi] = ES * (d >> 16);
+        tdeg[mi] = t < NM ? (d & 255) : 99;
     }
     v4d acc[MW][NTL];
 #pragma unroll
     for (int mi = 0; mi < MW; ++mi)
 #pragma unroll
         for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = v4d{0.0, 0.0, 0.0, 0.0};
-    // prefetch registers: class tensor (NL) and source block (NBL) of a pair
-    double pre[NL];
-    TE preb[NBL];
-    auto fetch = [&](int e) {
-        const double* Tn = a.Tc + (int64_t)a.pair_class[e] * NT;
-#pragma unroll
-        for (int x = 0; x < NL; ++x) pre[x] = (tid + NTH * x < NT) ? Tn[tid + NTH * x] : 0.0;
+    // Pair info (class, source node, its level, order) is read one pair ahead and is complete
+    // before the next pair's copies start, so no vector load waits behind them.
+    // Next pair's class tensor and (STAGE) source block rows 0..n_q-1 into the other LDS
+    // buffer: STAGE first zero-fills the rows past n_q that its k-steps read, then copies
+    // 16-byte chunks straight from global memory into LDS (global_load_lds_dwordx4: lane l
+    // of a wave fills bytes 16 l.. after the wave's LDS base), waited for with vmcnt before
+    // the pair's closing barrier; otherwise the class tensor goes through registers.
+    double pre[STAGE ? 1 : NL];
+    const TE* __restrict__ tcp = (const TE*)a.Tcp;
+    // Pair info runs two pairs ahead: when pair e starts, pair e + 1's class, source node and
+    // order are complete (loaded during pair e - 1, covered by its closing wait) and its
+    // copies start at once; pair e + 1's source level and pair e + 2's raw info are loaded
+    // while pair e is multiplied.
+    int cls_n = 0, j_n = 0, q_n = P, lv_n = 0;   // next pair (lv_n: in flight during this pair)
+    int cls_r = 0, src_r = 0, q_r = P;            // the pair after it, raw
+    auto load_raw = [&](int e) {
+        cls_r = a.pair_class[e];
+        src_r = a.src[e];
+        q_r = a.order ? (int)a.order[e] : P;
+    };
+    auto take_raw = [&]() {
+        cls_n = __builtin_amdgcn_readfirstlane(cls_r);
+        j_n = __builtin_amdgcn_readfirstlane(src_node(src_r));
+        q_n = __builtin_amdgcn_readfirstlane(q_r);
+    };
+    const int brow = lane / CPR;
+    const int boff_l = brow * a.s_ld + (lane % CPR) * CH;
+    auto fetch = [&](int buf) {
         if constexpr (STAGE) {
-            const int j = src_node(a.src[e]);
-            const int nb = (a.order ? ncomp(a.order[e]) : NM) * BW;
-            const TE* __restrict__ B = cms + (int64_t)j * NM * a.s_ld + sg;
+            const int nqn = ncomp(q_n);
+            const int kend = 4 * (((nqn + 3) / 4 + 1) & ~1);
+            for (int x = nqn * BW + tid; x < kend * BW; x += NTH) Bl[buf][x] = TE(0);
+            const TE* __restrict__ Tn = tcp + (int64_t)cls_n * NTP;
+            for (int k = w; k * 64 * CH < NTP; k += NW)
+                if (64 * k * CH + lane * CH < NTP) dma16(Tn + 64 * k * CH + lane * CH, &Tl[buf][64 * k * CH]);
+            // instruction k copies rows 64 k / CPR + lane / CPR, chunk lane % CPR of each
+            const TE* __restrict__ B = cms + (int64_t)j_n * NM * a.s_ld + sg;
+            for (int k = w; k * 64 < nqn * CPR; k += NW)
+                if (k * (64 / CPR) + brow < nqn) dma16(B + (int64_t)k * (64 / CPR) * a.s_ld + boff_l, &Bl[buf][64 * k * CH]);
+        } else {
+            const double* Tn = a.Tc + (int64_t)cls_n * NT;
 #pragma unroll
-            for (int x = 0; x < NBL; ++x) {
-                const int y = tid + NTH * x;   // y = s * BW + set
-                preb[x] = y < nb ? B[(int64_t)(y / BW) * a.s_ld + (y % BW)] : TE(0);
-            }
+            for (int x = 0; x < NL; ++x) pre[x] = (tid + NTH * x < NT) ? Tn[tid + NTH * x] : 0.0;
         }
     };
     auto publish = [&](int buf) {
-#pragma unroll
-        for (int x = 0; x < NL; ++x)
-            if (tid + NTH * x < NT) Tl[buf][tid + NTH * x] = (TE)pre[x];
         if constexpr (STAGE) {
+            __builtin_amdgcn_s_waitcnt(0);
+        } else {
 #pragma unroll
-            for (int x = 0; x < NBL; ++x)
-                if (tid + NTH * x < NM * BW) Bl[buf][tid + NTH * x] = preb[x];
+            for (int x = 0; x < NL; ++x)
+                if (tid + NTH * x < NT) Tl[buf][tid + NTH * x] = (TE)pre[x];
         }
     };
+    const int li = a.level[i];
     if (e0 < e1) {
-        fetch(e0);
+        load_raw(e0);
+        take_raw();
+        fetch(0);
+        lv_n = a.level[j_n];
+        if (e0 + 1 < e1) load_raw(e0 + 1);
         publish(0);
     }
     __syncthreads();
-    const int li = a.level[i];
+    const char* __restrict__ gt = (const char*)gtab;
     for (int e = e0; e < e1; ++e) {
         const int cur = (e - e0) & 1;
         const bool more = e + 1 < e1;
-        if (more) fetch(e + 1);
-        const int j = src_node(a.src[e]);
-        const int dl = li - a.level[j];
-        // learned order q of this pair: only the leading ncomp(q) rows and columns
-        const int qo = a.order ? (int)a.order[e] : P;
-        const int nq = ncomp(qo);
-        const int KSq = (nq + 3) / 4;
-        // level-offset scale of row t as a power of two: 2^-(dl |t|) or 2^(dl (|t| + 1))
-        int sh[MW];
-        bool rv[MW];
-#pragma unroll
-        for (int mi = 0; mi < MW; ++mi) {
-            const int deg = tdec[mi] & 255;
-            sh[mi] = dl > 0 ? -dl * deg : (dl < 0 ? -dl * (deg + 1) : 0);
-            rv[mi] = tdec[mi] >= 0 && (w + NW * mi) * 16 + row < nq;
+        // this pair: source node, level offset, learned order q (only the leading ncomp(q)
+        // rows and columns)
+        const int j = j_n;
+        const int dl = li - __builtin_amdgcn_readfirstlane(lv_n);
+        const int qo = q_n;
+        if (more) {
+            take_raw();
+            fetch(1 - cur);
+            lv_n = a.level[j_n];
+            if (e + 2 < e1) load_raw(e + 2);
         }
-        const TE* __restrict__ T = Tl[cur];
+        const int nq = ncomp(qo);
+        const int KSq = ((nq + 3) / 4 + 1) & ~1;
+        // level-offset scale of row t as a power of two, 2^-(dl |t|) or 2^(dl (|t| + 1)), i.e.
+        // 2^(sa |t| + sb); rows past the pair's order get 2^-2000, which flushes them to zero
+        const int sa = -dl, sb = dl < 0 ? -dl : 0;
+        int sh[F32 ? 1 : MW];
+        if constexpr (!F32) {
+#pragma unroll
+            for (int mi = 0; mi < MW; ++mi) sh[mi] = tdeg[mi] <= qo ? __mul24(sa, tdeg[mi]) + sb : -2000;
+        }
+        int tzc[MW];
+#pragma unroll
+        for (int mi = 0; mi < MW; ++mi) tzc[mi] = tz[mi] + cur * NTP * ES;
+        const char* __restrict__ T = (const char*)&Tl[0][0];
+        const TE* __restrict__ Bc = Bl[cur];
         const int64_t boff = (int64_t)j * NM * a.s_ld + sg + row;
         Acc part[MW][NTL];
-        if constexpr (F32) {
+        // Software pipeline over step pairs (k0, k0 + 1): while the MFMAs of pair k0 run, the
+        // tensor elements and source rows of pair k0 + 2, the G lookups of pair k0 + 4 and the
+        // source decodes of pair k0 + 6 are in flight (LDS returns in order, so each wait only
+        // covers loads issued an iteration earlier).  Offsets are byte offsets into Tl.
+        auto decode = [&](int k, int (&d)[2]) {
 #pragma unroll
-            for (int mi = 0; mi < MW; ++mi)
+            for (int u = 0; u < 2; ++u) d[u] = sdec[(k + u) * 4 + kq];
+        };
+        auto lookup = [&](const int (&d)[2], int (&o)[2][MW]) {
 #pragma unroll
-                for (int nt = 0; nt < NTL; ++nt) part[mi][nt] = Acc{0, 0, 0, 0};
-        }
-#pragma unroll 4
-        for (int ks = 0; ks < KSq; ++ks) {
-            const int s = ks * 4 + kq;
-            const int ds = sdec[s];
-            const bool sv = s < nq;
-            TE b[NTL];
+            for (int u = 0; u < 2; ++u)
 #pragma unroll
-            for (int nt = 0; nt < NTL; ++nt) {
-                if constexpr (STAGE) b[nt] = sv ? Bl[cur][s * BW + 16 * nt + row] : TE(0);
-                else b[nt] = sv ? cms[boff + (int64_t)s * a.s_ld + 16 * nt] : TE(0);
-            }
+                for (int mi = 0; mi < MW; ++mi) o[u][mi] = *(const int*)(gt + trow[mi] + (d[u] & 0xffff));
+        };
+        auto finish = [&](const int (&d)[2], int (&o)[2][MW]) {
 #pragma unroll
-            for (int mi = 0; mi < MW; ++mi) {
-                const int dt = tdec[mi];
-                const int kk = (dt & 255) + (ds & 255);
-                const int ww = ((dt >> 8) & 255) + ((ds >> 8) & 255);
-                const int f = tet(kk) + tri(ww) + ((dt >> 16) & 255) + (ds >> 16);
-                if constexpr (F32) {
-                    const float av = dt >= 0 ? T[f] : 0.0f;
+            for (int u = 0; u < 2; ++u)
 #pragma unroll
-                    for (int nt = 0; nt < NTL; ++nt)
-                        part[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b[nt], part[mi][nt], 0, 0, 0);
-                } else {
-                    const double av = rv[mi] ? ldexp(T[f], sh[mi]) : 0.0;
+                for (int mi = 0; mi < MW; ++mi) o[u][mi] += tzc[mi] + (d[u] >> 16);
+        };
+        auto loadA = [&](const int (&o)[2][MW], TE (&x)[2][MW]) {
 #pragma unroll
-                    for (int nt = 0; nt < NTL; ++nt)
-                        acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[nt], acc[mi][nt], 0, 0, 0);
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int mi = 0; mi < MW; ++mi) x[u][mi] = *(const TE*)(T + o[u][mi]);
+        };
+        auto loadB = [&](int k, TE (&x)[2][NTL]) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int s = (k + u) * 4 + kq;
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt) {
+                    if constexpr (STAGE) {
+                        x[u][nt] = Bc[s * BW + 16 * nt + row];
+                    } else {
+                        const bool sv = s < nq;
+                        const TE v = cms[boff + (int64_t)(sv ? s : 0) * a.s_ld + 16 * nt];
+                        x[u][nt] = sv ? v : TE(0);
+                    }
                 }
             }
+        };
+        auto multiply = [&](const TE (&x)[2][MW], const TE (&y)[2][NTL], auto first) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int mi = 0; mi < MW; ++mi) {
+                    if constexpr (F32) {
+#pragma unroll
+                        for (int nt = 0; nt < NTL; ++nt) {
+                            if (decltype(first)::value && u == 0)
+                                part[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u][mi], y[u][nt], Acc{0, 0, 0, 0}, 0, 0, 0);
+                            else
+                                part[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u][mi], y[u][nt], part[mi][nt], 0, 0, 0);
+                        }
+                    } else {
+                        const double av = ldexp(x[u][mi], sh[mi]);
+#pragma unroll
+                        for (int nt = 0; nt < NTL; ++nt)
+                            acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, y[u][nt], acc[mi][nt], 0, 0, 0);
+                    }
+                }
+        };
+        TE av[2][MW], bv[2][NTL];
+        int go[2][MW], dsn[2];
+        {
+            int d[2];
+            decode(0, d);
+            lookup(d, go);
+            finish(d, go);
+            loadA(go, av);
+            loadB(0, bv);
+            decode(2, d);
+            lookup(d, go);
+            finish(d, go);
+            decode(4, dsn);
         }
+        auto iteration = [&](int k0, auto first) {
+            TE an[2][MW], bn[2][NTL];
+            int gn[2][MW], dn[2];
+            loadA(go, an);
+            loadB(k0 + 2, bn);
+            lookup(dsn, gn);
+            decode(k0 + 6, dn);
+            __builtin_amdgcn_sched_barrier(0);
+            multiply(av, bv, first);
+            __builtin_amdgcn_sched_barrier(0);
+            finish(dsn, gn);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                dsn[u] = dn[u];
+#pragma unroll
+                for (int mi = 0; mi < MW; ++mi) {
+                    av[u][mi] = an[u][mi];
+                    go[u][mi] = gn[u][mi];
+                }
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt) bv[u][nt] = bn[u][nt];
+            }
+        };
+        iteration(0, std::true_type{});
+        for (int k0 = 2; k0 < KSq; k0 += 2) iteration(k0, std::false_type{});
         if constexpr (F32) {
             // this pair's FP32 tiles, row-scaled, into the FP64 accumulators (the f32 layout keeps
             // row 4 kq + q in register q, the f64 one row 4 q + kq; remapped at the end)
@@ -521,12 +647,10 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
             for (int mi = 0; mi < MW; ++mi)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int deg = qdeg[mi][q];
-                    const int shq = dl > 0 ? -dl * deg : (dl < 0 ? -dl * (deg + 1) : 0);
-                    const bool ok = deg <= qo;
+                    const int deg = degl[(w + NW * mi) * 16 + 4 * kq + q];
+                    const int shq = deg <= qo ? __mul24(sa, deg) + sb : -2000;
 #pragma unroll
-                    for (int nt = 0; nt < NTL; ++nt)
-                        if (ok) acc[mi][nt][q] += ldexp((double)part[mi][nt][q], shq);
+                    for (int nt = 0; nt < NTL; ++nt) acc[mi][nt][q] += ldexp((double)part[mi][nt][q], shq);
                 }
         }
         if (more) publish(1 - cur);
@@ -550,6 +674,30 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
             }
         }
     }
+}
+
+int m2l_mfma_class_stride(int p, bool f32) {
+    const int ch = f32 ? 4 : 2;
+    return (ncomp(2 * p) + ch - 1) / ch * ch;
+}
+
+template <typename TE>
+__global__ void k_class_pad(const double* __restrict__ Tc, TE* __restrict__ Tcp, int64_t n, int nT, int ntp) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const int64_t c = x / ntp;
+    const int k = (int)(x - c * ntp);
+    Tcp[x] = k < nT ? (TE)Tc[c * nT + k] : TE(0);
+}
+
+void launch_m2l_class_pad(const double* Tc, void* Tcp, int ncls, int p, bool f32, hipStream_t s) {
+    const int ntp = m2l_mfma_class_stride(p, f32);
+    const int64_t n = (int64_t)ncls * ntp;
+    if (n == 0) return;
+    const int grid = (int)((n + 255) / 256);
+    if (f32) hipLaunchKernelGGL(k_class_pad<float>, dim3(grid), dim3(256), 0, s, Tc, (float*)Tcp, n, ncomp(2 * p), ntp);
+    else hipLaunchKernelGGL(k_class_pad<double>, dim3(grid), dim3(256), 0, s, Tc, (double*)Tcp, n, ncomp(2 * p), ntp);
+    FMMH_HIP(hipGetLastError());
 }
 
 bool m2l_mfma_supported(int p, int nsets) { return p <= M2L_MFMA_PMAX && nsets >= 16; }
