@@ -59,8 +59,12 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
     __shared__ int wcum[NEAR_WIN + 1];            // window: exclusive prefix of source counts
     __shared__ int wstart[NEAR_WIN];              // window: first sorted site of each entry
     __shared__ int wimg[NEAR_WIN];                // window: periodic image id of each entry
+    __shared__ double etab[DAMP != DAMP_NONE ? 32 : 1];   // 2^(j/32) for exp_table
     const int b = a.blk[blockIdx.x];
     if (b < 0) return;                         // an empty slot of the block order
+    if constexpr (DAMP != DAMP_NONE) {
+        if (threadIdx.x < 32) etab[threadIdx.x] = kExp2By32[threadIdx.x];   // read after the window barriers
+    }
     const int set0 = blockIdx.y * S;
     const int ns = min(S, a.nsets - set0);   // sets of this row (the rest are zero padding)
     const int lane = threadIdx.x;
@@ -165,11 +169,12 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
                         const double dz = zi - sz[jj];
                         const double R2 = dx * dx + dy * dy + dz * dz;
                         // the self pair is left to k_near_excl: 1/R = 0 makes its terms exactly 0
-                        const double inv = sorig[jj] == myo ? 0.0 : rsqrt(R2);
+                        const double rinv = rsqrt_pos(R2);
+                        const double inv = sorig[jj] == myo ? 0.0 : rinv;
                         double aij = 0.0;
                         if constexpr (DAMP != DAMP_NONE) aij = dfi * sdf[jj];
                         double gr[K + 1];
-                        radial_inv<K, DAMP>(R2, inv, aij, gr);
+                        radial_inv<K, DAMP, DAMP != DAMP_NONE>(R2, inv, aij, gr, etab);
                         if constexpr (has_closed_form<M, F>()) {
 #pragma unroll
                             for (int u = 0; u < S; ++u) {
@@ -224,8 +229,13 @@ template <int M, int F, int DAMP, int S>
 __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
+    __shared__ double etab[DAMP != DAMP_NONE ? 32 : 1];   // 2^(j/32) for exp_table
     const int b = a.blk[blockIdx.x];
     if (b < 0) return;
+    if constexpr (DAMP != DAMP_NONE) {
+        if (threadIdx.x < 32) etab[threadIdx.x] = kExp2By32[threadIdx.x];
+        __syncthreads();
+    }
     const int set0 = blockIdx.y * S;
     const int ns = min(S, a.nsets - set0);
     const int t0 = a.tgt_start[b], tc = a.tgt_count[b];
@@ -253,7 +263,7 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
                 double m[NM];
 #pragma unroll
                 for (int s = 0; s < NM; ++s) m[s] = sign * ms[((int64_t)u * NM + s) * a.ldm + j];
-                pair_field<M, F, DAMP>(dx, dy, dz, dx * dx + dy * dy + dz * dz, aij, m, acc[u]);
+                pair_field<M, F, DAMP>(dx, dy, dz, dx * dx + dy * dy + dz * dz, aij, m, acc[u], etab);
             }
         };
         bool self = false;

@@ -72,19 +72,77 @@ struct Poly {
     static constexpr PolyTable tab = make_poly(KIND);
 };
 
+// a * b + c with the constant c read from scalar registers (v_fma_f64, VOP3): the compiler
+// otherwise keeps such constants in VGPRs and copies one before every accumulate-in-place
+// FMA of a Horner chain.  Plain arithmetic, no memory access.
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
 template <int KIND, int N>
 __device__ __forceinline__ double poly_eval(double x) {
     constexpr int D = Poly<KIND>::tab.deg[N];
-    double s = Poly<KIND>::tab.c[N][D];
+    if constexpr (D == 0) {
+        return Poly<KIND>::tab.c[N][0];
+    } else {
+        const double top = Poly<KIND>::tab.c[N][D];
+        double s = fma_sc(x, top, Poly<KIND>::tab.c[N][D - 1]);
 #pragma unroll
-    for (int i = D - 1; i >= 0; --i) s = __builtin_fma(s, x, Poly<KIND>::tab.c[N][i]);
-    return s;
+        for (int i = D - 2; i >= 0; --i) s = fma_sc(s, x, Poly<KIND>::tab.c[N][i]);
+        return s;
+    }
+}
+
+// 2^(j/32), j = 0..31, correctly rounded (decimal arithmetic at 60 digits, then rounded to
+// the nearest double); staged in LDS by the kernels that call exp_table.
+static __constant__ const double kExp2By32[32] = {
+    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
+    0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
+    0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,
+    0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,
+    0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,
+    0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0,
+};
+
+// e^x for x <= 0 by Tang's table-driven method: x = (32 m + j) ln2/32 + r, |r| <= ln2/64,
+// e^x = 2^m 2^(j/32) e^r, with e^r - 1 from its degree-6 Taylor polynomial (remainder below
+// 2^-58 relative) and ln2/32 split in two parts (the high part has 32 significant bits, so
+// k ln2/32 is exact for |k| < 2^21).  About 1 ulp; 0 below -745 (arguments under -1000
+// are clamped there).  tab: kExp2By32, usually staged in LDS.
+__device__ __forceinline__ double exp_table(double x, const double* tab) {
+    x = fmax(x, -1000.0);
+    const double kd = __builtin_rint(x * 0x1.71547652b82fep+5);        // 32 / ln2
+    double r = __builtin_fma(kd, -0x1.62e42fee00000p-6, x);             // ln2/32, high part
+    r = __builtin_fma(kd, -0x1.a39ef35793c76p-38, r);                   // ln2/32, low part
+    const int k = (int)kd;
+    double q = fma_sc(r, 1.0 / 720.0, 1.0 / 120.0);
+    q = fma_sc(q, r, 1.0 / 24.0);
+    q = fma_sc(q, r, 1.0 / 6.0);
+    q = __builtin_fma(q, r, 0.5);
+    q = __builtin_fma(q, r, 1.0);
+    q *= r;                                                              // e^r - 1
+    const double t = tab[k & 31];
+    return ldexp(__builtin_fma(t, q, t), k >> 5);
+}
+
+// 1/sqrt(x) for finite x > 0: the hardware estimate y refined by one Newton-Raphson step
+// carried to second order, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2.  x = 0 gives NaN.
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = __builtin_fma(-x * y, y, 1.0);
+    return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
 // g[0..K] for one pair; R2 = |r|^2, inv = 1/|r| (0 makes every g_n exactly
 // 0: a pair that must not contribute), a = damping factor product.
-template <int K, int DAMP>
-__device__ __forceinline__ void radial_inv(double R2, double inv, double a, double (&g)[K + 1]) {
+// TAB: exponentials by exp_table on etab (kExp2By32), else the library exp.
+template <int K, int DAMP, bool TAB = false>
+__device__ __forceinline__ void radial_inv(double R2, double inv, double a, double (&g)[K + 1],
+                                           const double* etab = nullptr) {
     const double inv2 = inv * inv;
     double b = inv;
     if constexpr (DAMP == DAMP_NONE) {
@@ -93,7 +151,7 @@ __device__ __forceinline__ void radial_inv(double R2, double inv, double a, doub
         for (int n = 1; n <= K; ++n) { b *= -(2.0 * n - 1.0) * inv2; g[n] = b; }
     } else if constexpr (DAMP == DAMP_THOLE) {
         const double u = a * (R2 * inv);
-        const double e = exp(-u);
+        const double e = TAB ? exp_table(-u, etab) : exp(-u);
         g[0] = b * (1.0 - e * poly_eval<DAMP_THOLE, 0>(u));
         if constexpr (K >= 1) { b *= -1.0 * inv2; g[1] = b * (1.0 - e * poly_eval<DAMP_THOLE, 1>(u)); }
         if constexpr (K >= 2) { b *= -3.0 * inv2; g[2] = b * (1.0 - e * poly_eval<DAMP_THOLE, 2>(u)); }
@@ -104,7 +162,7 @@ __device__ __forceinline__ void radial_inv(double R2, double inv, double a, doub
     } else if constexpr (DAMP == DAMP_AMOEBA) {
         const double R = R2 * inv;
         const double v = a * a * a * (R2 * R);
-        const double e = exp(-v);
+        const double e = TAB ? exp_table(-v, etab) : exp(-v);
         g[0] = 0.0;
         if constexpr (K >= 1) { b *= -1.0 * inv2; g[1] = b * (1.0 - e * poly_eval<DAMP_AMOEBA, 1>(v)); }
         if constexpr (K >= 2) { b *= -3.0 * inv2; g[2] = b * (1.0 - e * poly_eval<DAMP_AMOEBA, 2>(v)); }
@@ -114,7 +172,7 @@ __device__ __forceinline__ void radial_inv(double R2, double inv, double a, doub
         if constexpr (K >= 6) { b *= -11.0 * inv2; g[6] = b * (1.0 - e * poly_eval<DAMP_AMOEBA, 6>(v)); }
     } else {
         const double w = sqrt(a) * (R2 * inv);
-        const double e = 1.1283791670955126 * exp(-w * w);   // 2/sqrt(pi) e^{-w^2}
+        const double e = 1.1283791670955126 * (TAB ? exp_table(-w * w, etab) : exp(-w * w));   // 2/sqrt(pi) e^{-w^2}
         const double er = erf(w);
         g[0] = b * (er - e * poly_eval<DAMP_ERF, 0>(w));
         if constexpr (K >= 1) { b *= -1.0 * inv2; g[1] = b * (er - e * poly_eval<DAMP_ERF, 1>(w)); }
@@ -262,13 +320,15 @@ __device__ __forceinline__ void contract_closed(double x, double y, double z, co
 }
 
 // One pair's contribution acc[t] += sum_s m[s] T_{s+t}(x, y, z) with damping
-// factor product a; closed form where there is one.
+// factor product a; closed form where there is one.  The radial part is computed as
+// the near-field pair loop computes it (rsqrt_pos, exp_table on etab).
 template <int M, int F, int DAMP>
 __device__ __forceinline__ void pair_field(double x, double y, double z, double R2, double a,
-                                           const double (&m)[ncomp(M)], double (&acc)[ncomp(F)]) {
+                                           const double (&m)[ncomp(M)], double (&acc)[ncomp(F)],
+                                           const double* etab) {
     constexpr int K = M + F;
     double gr[K + 1];
-    radial<K, DAMP>(R2, a, gr);
+    radial_inv<K, DAMP, DAMP != DAMP_NONE>(R2, rsqrt_pos(R2), a, gr, etab);
     if constexpr (has_closed_form<M, F>()) {
         contract_closed<M, F>(x, y, z, gr, m, acc);
     } else {
