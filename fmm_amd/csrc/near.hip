@@ -25,11 +25,12 @@ namespace fmmh {
 
 constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: never the target's own id
 
-// Lane layout: a target block of nt <= 64 sites is replicated over
-// r = 64 / nt lane groups (group g = lane / nt, target lane % nt); group g
-// takes every r-th source, so small leaves still fill the wavefront.  At the
-// end the groups' partial fields are summed in group order through LDS
-// (deterministic).
+// Lane layout: a target block is taken in passes of nt <= 64 sites, each
+// replicated over r = 64 / nt lane groups (group g = lane / nt, target
+// lane % nt); group g takes every r-th source, so small leaves still fill the
+// wavefront, and the pass size is chosen per block so that few lanes idle.
+// At the end of a pass the groups' partial fields are summed in group order
+// through LDS (deterministic).
 // Source staging: the source blocks of a target block's list are one
 // concatenated run (windows of NEAR_WIN list entries, whose exclusive prefix
 // of source counts is scanned across the wavefront), cut into tiles of 64
@@ -74,8 +75,26 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
     const int e1 = a.list_off[b + 1];
     const double* __restrict__ ms = a.ms + (int64_t)set0 * NM * a.ldm;
     double* __restrict__ out = a.out + (int64_t)set0 * NF * a.ldo;
-    for (int base = 0; base < tc; base += 64) {
-        const int nt = min(64, tc - base);
+    // Targets per pass: c passes of ct = ceil(tc / c) targets, each replicated r = 64 / ct
+    // times, with c minimizing c (ceil(64 / r) + 2) -- the pair iterations per 64-source tile
+    // plus about two iterations' worth of re-staging the tile (measured: weights 1 and 2 run
+    // alike, 0 is 17 % slower) -- so that a block of, say, 40 targets runs as 2 passes of 20
+    // (r = 3) instead of one pass with 24 idle lanes.
+    int ct = min(tc, 64);
+    {
+        int best = 1 << 30;
+        for (int c = (tc + 63) / 64; c <= 16; ++c) {
+            const int t = (tc + c - 1) / c;
+            const int rr = 64 / t;
+            const int cost = c * ((64 + rr - 1) / rr + 2);
+            if (cost < best) {
+                best = cost;
+                ct = t;
+            }
+        }
+    }
+    for (int base = 0; base < tc; base += ct) {
+        const int nt = min(ct, tc - base);
         const int r = 64 / nt;                   // lane groups
         const int g = lane / nt;                 // this lane's group (== r for idle lanes)
         const int ti = lane - g * nt;            // this lane's target within the block

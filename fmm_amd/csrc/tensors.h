@@ -149,6 +149,30 @@ __device__ __forceinline__ void radial_inv(double R2, double inv, double a, doub
         g[0] = b;
 #pragma unroll
         for (int n = 1; n <= K; ++n) { b *= -(2.0 * n - 1.0) * inv2; g[n] = b; }
+    } else if constexpr (DAMP == DAMP_THOLE && K <= 3) {
+        // the first four polynomials differ by single leading terms: 1 + u/2, 1 + u (1 + u/2),
+        // + u^3/6, + u^4/30 (the table's values), so they are built one from the next
+        static_assert(Poly<DAMP_THOLE>::tab.deg[3] == 4, "Thole polynomial table");
+        const double u = a * (R2 * inv);
+        const double e = TAB ? exp_table(-u, etab) : exp(-u);
+        const double p0 = __builtin_fma(u, 0.5, 1.0);
+        g[0] = b * (1.0 - e * p0);
+        if constexpr (K >= 1) {
+            const double p1 = __builtin_fma(u, p0, 1.0);
+            b *= -1.0 * inv2;
+            g[1] = b * (1.0 - e * p1);
+            if constexpr (K >= 2) {
+                const double u2 = u * u;
+                const double p2 = fma_sc(u2 * u, Poly<DAMP_THOLE>::tab.c[2][3], p1);
+                b *= -3.0 * inv2;
+                g[2] = b * (1.0 - e * p2);
+                if constexpr (K >= 3) {
+                    const double p3 = fma_sc(u2 * u2, Poly<DAMP_THOLE>::tab.c[3][4], p2);
+                    b *= -5.0 * inv2;
+                    g[3] = b * (1.0 - e * p3);
+                }
+            }
+        }
     } else if constexpr (DAMP == DAMP_THOLE) {
         const double u = a * (R2 * inv);
         const double e = TAB ? exp_table(-u, etab) : exp(-u);
@@ -304,17 +328,27 @@ __device__ __forceinline__ void contract_closed(double x, double y, double z, co
             acc[3] = __builtin_fma(qz, g[2], acc[3]);
         }
         if constexpr (F >= 2) {
+            // Phi_ce = delta_ce A1 + x_c w_e + u_c x_e,  u = g2 m,  w = B r + u
             double B = m0 * g[2];
             if constexpr (M >= 1) B = __builtin_fma(d, g[3], B);
-            const double vx = B * x, vy = B * y, vz = B * z;
-            double ux = 0.0, uy = 0.0, uz = 0.0;
-            if constexpr (M >= 1) { ux = g[2] * m[1]; uy = g[2] * m[2]; uz = g[2] * m[3]; }
-            acc[4] = __builtin_fma(x, __builtin_fma(2.0, ux, vx), acc[4] + A1);
-            acc[5] = __builtin_fma(x, vy + uy, __builtin_fma(ux, y, acc[5]));
-            acc[6] = __builtin_fma(x, vz + uz, __builtin_fma(ux, z, acc[6]));
-            acc[7] = __builtin_fma(y, __builtin_fma(2.0, uy, vy), acc[7] + A1);
-            acc[8] = __builtin_fma(y, vz + uz, __builtin_fma(uy, z, acc[8]));
-            acc[9] = __builtin_fma(z, __builtin_fma(2.0, uz, vz), acc[9] + A1);
+            if constexpr (M >= 1) {
+                const double ux = g[2] * m[1], uy = g[2] * m[2], uz = g[2] * m[3];
+                const double wx = __builtin_fma(B, x, ux), wy = __builtin_fma(B, y, uy), wz = __builtin_fma(B, z, uz);
+                acc[4] = __builtin_fma(x, wx + ux, acc[4] + A1);
+                acc[5] = __builtin_fma(x, wy, __builtin_fma(ux, y, acc[5]));
+                acc[6] = __builtin_fma(x, wz, __builtin_fma(ux, z, acc[6]));
+                acc[7] = __builtin_fma(y, wy + uy, acc[7] + A1);
+                acc[8] = __builtin_fma(y, wz, __builtin_fma(uy, z, acc[8]));
+                acc[9] = __builtin_fma(z, wz + uz, acc[9] + A1);
+            } else {
+                const double wx = B * x, wy = B * y, wz = B * z;
+                acc[4] = __builtin_fma(x, wx, acc[4] + A1);
+                acc[5] = __builtin_fma(x, wy, acc[5]);
+                acc[6] = __builtin_fma(x, wz, acc[6]);
+                acc[7] = __builtin_fma(y, wy, acc[7] + A1);
+                acc[8] = __builtin_fma(y, wz, acc[8]);
+                acc[9] = __builtin_fma(z, wz, acc[9] + A1);
+            }
         }
     }
 }

@@ -198,7 +198,11 @@ def test_direct_targets_match_full_direct(gpu_ctx):
     full = gpu_ctx.field_direct(0, x, m, e, 2, "THOLE", df)
     got = out.cpu().numpy()
     assert max(block_rel(got, full[tg], 2)) < 1e-14
-    np.testing.assert_array_equal(got[:64], full[tg[:64]])
+    # the first 64 targets form one full block here; sites that also sit in a full
+    # 64-site block of the all-sites sum get the same per-lane arithmetic, hence identity
+    same = np.nonzero(tg[:64] < (x.shape[0] // 64) * 64)[0]
+    assert same.size >= 50
+    np.testing.assert_array_equal(got[same], full[tg[same]])
     from fmm_amd.octree import FmmError
     with pytest.raises(FmmError, match="target index"):
         bad = torch.as_tensor(np.array([0, x.shape[0]], np.int32), device=dev)
