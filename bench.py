@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # 1.000): 17,020 KiB x 2 read + 18,600 KiB written.  The profiler cannot run
 # inside this process, so the figure is the recorded measurement of
 # profiles/k_near_water200k_thole_p6.md.
-K_NEAR_TRAFFIC_BYTES = (17020.0 * 2.0 + 18600.0) * 1024.0
+K_NEAR_TRAFFIC_BYTES = (21476.0 * 2.0 + 20906.0) * 1024.0
 
 
 def c_T(K):

@@ -601,45 +601,58 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
                     }
                 }
         };
-        TE av[2][MW], bv[2][NTL];
-        int go[2][MW], dsn[2];
+        // two register sets of pipeline state, used in turn (no copies between iterations)
+        TE a1[2][MW], b1[2][NTL], a2[2][MW], b2[2][NTL];
+        int g1[2][MW], d1[2], g2[2][MW], d2[2];
         {
             int d[2];
             decode(0, d);
-            lookup(d, go);
-            finish(d, go);
-            loadA(go, av);
-            loadB(0, bv);
+            lookup(d, g1);
+            finish(d, g1);
+            loadA(g1, a1);
+            loadB(0, b1);
             decode(2, d);
-            lookup(d, go);
-            finish(d, go);
-            decode(4, dsn);
+            lookup(d, g1);
+            finish(d, g1);
+            decode(4, d1);
         }
-        auto iteration = [&](int k0, auto first) {
-            TE an[2][MW], bn[2][NTL];
-            int gn[2][MW], dn[2];
-            loadA(go, an);
-            loadB(k0 + 2, bn);
-            lookup(dsn, gn);
-            decode(k0 + 6, dn);
+        // state (A, Bv: pair k0; G: offsets of k0 + 2; D: decodes of k0 + 4) -> the same for k0 + 2
+        auto iteration = [&](int k0, auto first, const TE (&A)[2][MW], const TE (&Bv)[2][NTL], const int (&G)[2][MW],
+                             const int (&D)[2], TE (&An)[2][MW], TE (&Bn)[2][NTL], int (&Gn)[2][MW], int (&Dn)[2]) {
+            loadA(G, An);
+            loadB(k0 + 2, Bn);
+            lookup(D, Gn);
+            decode(k0 + 6, Dn);
             __builtin_amdgcn_sched_barrier(0);
-            multiply(av, bv, first);
+            multiply(A, Bv, first);
             __builtin_amdgcn_sched_barrier(0);
-            finish(dsn, gn);
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                dsn[u] = dn[u];
-#pragma unroll
-                for (int mi = 0; mi < MW; ++mi) {
-                    av[u][mi] = an[u][mi];
-                    go[u][mi] = gn[u][mi];
-                }
-#pragma unroll
-                for (int nt = 0; nt < NTL; ++nt) bv[u][nt] = bn[u][nt];
-            }
+            finish(D, Gn);
         };
-        iteration(0, std::true_type{});
-        for (int k0 = 2; k0 < KSq; k0 += 2) iteration(k0, std::false_type{});
+        iteration(0, std::true_type{}, a1, b1, g1, d1, a2, b2, g2, d2);
+        if constexpr (F32) {
+            for (int k0 = 2; k0 < KSq; k0 += 4) {
+                iteration(k0, std::false_type{}, a2, b2, g2, d2, a1, b1, g1, d1);
+                if (k0 + 2 >= KSq) break;
+                iteration(k0 + 2, std::false_type{}, a1, b1, g1, d1, a2, b2, g2, d2);
+            }
+        } else {
+            // FP64: the accumulators stay in place across pairs; one set of state, copied
+            // (unrolling by two doubles the accumulator registers here)
+            for (int k0 = 2; k0 < KSq; k0 += 2) {
+                iteration(k0, std::false_type{}, a2, b2, g2, d2, a1, b1, g1, d1);
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    d2[u] = d1[u];
+#pragma unroll
+                    for (int mi = 0; mi < MW; ++mi) {
+                        a2[u][mi] = a1[u][mi];
+                        g2[u][mi] = g1[u][mi];
+                    }
+#pragma unroll
+                    for (int nt = 0; nt < NTL; ++nt) b2[u][nt] = b1[u][nt];
+                }
+            }
+        }
         if constexpr (F32) {
             // this pair's FP32 tiles, row-scaled, into the FP64 accumulators (the f32 layout keeps
             // row 4 kq + q in register q, the f64 one row 4 q + kq; remapped at the end)
