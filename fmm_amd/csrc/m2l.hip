@@ -580,11 +580,16 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
                 }
             }
         };
+        // row tiles wholly past the pair's order are skipped (a wave-uniform test)
+        bool tile_on[MW];
+#pragma unroll
+        for (int mi = 0; mi < MW; ++mi) tile_on[mi] = (w + NW * mi) * 16 < nq;
         auto multiply = [&](const TE (&x)[2][MW], const TE (&y)[2][NTL], auto first) {
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
                 for (int mi = 0; mi < MW; ++mi) {
+                    if (!tile_on[mi]) continue;
                     if constexpr (F32) {
 #pragma unroll
                         for (int nt = 0; nt < NTL; ++nt) {
@@ -657,7 +662,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
             // this pair's FP32 tiles, row-scaled, into the FP64 accumulators (the f32 layout keeps
             // row 4 kq + q in register q, the f64 one row 4 q + kq; remapped at the end)
 #pragma unroll
-            for (int mi = 0; mi < MW; ++mi)
+            for (int mi = 0; mi < MW; ++mi) {
+                if (!tile_on[mi]) continue;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int deg = degl[(w + NW * mi) * 16 + 4 * kq + q];
@@ -665,6 +671,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
 #pragma unroll
                     for (int nt = 0; nt < NTL; ++nt) acc[mi][nt][q] += ldexp((double)part[mi][nt][q], shq);
                 }
+            }
         }
         if (more) publish(1 - cur);
         __syncthreads();
