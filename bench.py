@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for HBM counters) and exit")
+    ap.add_argument("--peak-calibration", action="store_true",
+                    help="only time large FP32 and FP64 library GEMMs (sustained matrix-core rates) and exit")
     ap.add_argument("--no-target", action="store_true",
                     help="skip the comparison at <= 1e-6 error (plain FP64 vs batched mixed learned-order)")
     ap.add_argument("--target-only", action="store_true", help="only run that comparison and print it")
@@ -205,6 +207,25 @@ def main():
     if args.target_only:
         import torch
         print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_)))
+        return
+    if args.peak_calibration:
+        import torch
+        torch.backends.cuda.matmul.allow_tf32 = False
+        res = {}
+        for name, dt, nsz in (("fp32", torch.float32, 8192), ("fp64", torch.float64, 8192)):
+            a = torch.randn(nsz, nsz, dtype=dt, device="cuda:0")
+            b = torch.randn(nsz, nsz, dtype=dt, device="cuda:0")
+            for _ in range(3):
+                c_ = a @ b
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 10
+            for _ in range(reps):
+                c_ = a @ b
+            torch.cuda.synchronize()
+            dt_s = (time.perf_counter() - t0) / reps
+            res[name + "_gemm_tflops"] = 2.0 * nsz ** 3 / dt_s / 1e12
+        print(json.dumps(res))
         return
     if args.copy_calibration:
         import torch
