@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the near-field kernel on the default workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one run each) scaled by the same
 # counters on a plain 1 GiB device copy (FETCH_SIZE factor 2.000, WRITE_SIZE
-# 1.000): 17,020 KiB x 2 read + 18,600 KiB written.  The profiler cannot run
+# 1.000): 21,476 KiB x 2 read + 20,906 KiB written.  The profiler cannot run
 # inside this process, so the figure is the recorded measurement of
 # profiles/k_near_water200k_thole_p6.md.
 K_NEAR_TRAFFIC_BYTES = (21476.0 * 2.0 + 20906.0) * 1024.0
@@ -182,15 +182,108 @@ def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, se
     }
 
 
+def bench_uneven(dev, args):
+    """64 water boxes of 10k-500k sites, 8 multipole sets each, THOLE, field order 1,
+    learned M2L orders (cap 10, tolerance 1e-6, 1,024 spot checks per system), mixed precision.  One
+    step = one field_fmm_sets call per system, inputs resident in HBM; the selector is trained on a
+    1/20-scale box of the idx-1 recipe (time reported, not counted)."""
+    import torch
+    from fmm_amd import synth
+    from fmm_amd.octree import Context
+    c = synth.config(4, scale=args.scale)
+    th, cap, F, sets = c["theta"], c["expansion_order"], c["field_order"], c["sets"]
+    t_start = time.perf_counter()
+
+    def progress(msg):
+        print(f"[config 4 {time.perf_counter() - t_start:7.1f} s] {msg}", file=sys.stderr, flush=True)
+    progress(f"{len(c['systems'])} systems generated")
+    systems = []
+    for sy in c["systems"]:
+        n = sy["coordinates"].shape[0]
+        systems.append((n, col(sy["coordinates"], dev), col3(sy["multipoles"], dev),
+                        col(sy["exclusions"], dev, torch.int32), torch.as_tensor(sy["damping_factors"], device=dev),
+                        col3(np.zeros((n, synth.ncomp(F), sets)), dev)))
+    ctx = Context(0)
+    if args.m2l_mfma is not None:
+        ctx.set_m2l_mfma(args.m2l_mfma)
+    ct = synth.config(1, scale=0.05)
+    mt = np.zeros((ct["coordinates"].shape[0], 4))
+    mt[:, 0] = ct["multipoles"][:, 0]
+    mt[:, 1:4] = np.random.default_rng(7).normal(0.0, 0.05, (mt.shape[0], 3))
+    info = ctx.train_selector(ct["coordinates"], np.asfortranarray(mt), th, c["ncrit"], cap, nsamples=4096,
+                              epochs=60, seed=5)
+    progress("selector trained")
+    ctx.set_precision(True)
+    ctx.set_order_tolerance(c["order_tolerance"], spot_checks=1024)
+    ctx.set_timing(True)
+    sel = []
+
+    def step(record=False):
+        for k, (n, X, M, E, df, out) in enumerate(systems):
+            out.zero_()
+            ctx.field_fmm_sets_device(X, M, E, th, c["ncrit"], cap, F, out, c["damp_type"], df)
+            if record:
+                st = ctx.selection_stats()
+                sel.append((n, st, ctx.phase_times()["p2p"], ctx.stats()["p2p_site_pairs"]))
+            if k % 16 == 15:
+                progress(f"system {k + 1}")
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        progress(f"warm-up step {w + 1} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        progress(f"timed step {i + 1} done")
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    step(record=True)
+    total = sum(n for n, *_ in systems)
+    value = total * sets / (ms_per_step / 1e3)
+    fpp = p2p_flops_per_pair(1, F, c["damp_type"], sets)
+    pairs = sum(r[3] for r in sel)
+    p2p_ms = sum(r[2] for r in sel)
+    achieved = pairs * fpp / (p2p_ms / 1e3) / 1e12
+    ctx.close()
+    return {
+        "metric": "site-field evals/s at <=1e-6 rel err vs FP64 direct; % of FP64/FP32 peak",
+        "value": value, "unit": "site-field evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": c["name"], "systems": len(systems), "sites": total,
+                   "sites_min": min(n for n, *_ in systems), "sites_max": max(n for n, *_ in systems),
+                   "sets": sets, "precision": "mixed", "multipole_dim": 4, "field_order": F, "theta": th,
+                   "ncrit": c["ncrit"], "expansion_order_cap": cap, "damp_type": c["damp_type"],
+                   "order_tolerance": c["order_tolerance"], "spot_checks": 1024},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_near (P2P, FP64 VALU)",
+                     "kernel_ms": p2p_ms, "flop_per_pair": fpp, "site_pairs": pairs,
+                     "note": "summed over the 64 calls of one step"},
+        "cpu_baseline": None,
+        "selection": {"fell_back": sum(int(r[1]["fallback"]) for r in sel),
+                      "max_spot_error": max(r[1]["spot_error"] for r in sel),
+                      "mean_order": float(np.average([r[1]["mean_order"] for r in sel], weights=[r[0] for r in sel])),
+                      "m2l_flops_vs_full": sum(r[1]["m2l_flops"] for r in sel) / sum(r[1]["m2l_flops_full"] for r in sel),
+                      "note": "spot error: far field at 1,024 sites per system against the direct far-field sum "
+                              "(damping applies to near-field pairs only, as in the reference)"},
+        "selector_training": info,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="site-count scale of the uneven batch (config 4)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
                     help="default: mixed for config 3 (its workload), fp64 otherwise")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--m2l-mfma", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="M2L engine option (FMMH_OPT_M2L_MFMA); default: the library's choice")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for HBM counters) and exit")
@@ -239,6 +332,10 @@ def main():
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.config == 4:
+        import torch
+        print(json.dumps(bench_uneven(torch.device("cuda:0"), args)))
+        return
 
     import torch
     from fmm_amd import synth
@@ -259,6 +356,8 @@ def main():
     out = col(np.zeros((n, nf)), dev) if sets == 1 else col3(np.zeros((n, nf, sets)), dev)
     ctx = Context(0)
     ctx.set_timing(True)
+    if args.m2l_mfma is not None:
+        ctx.set_m2l_mfma(args.m2l_mfma)
     precision = args.precision or ("mixed" if args.config == 3 else "fp64")
     ctx.set_precision(precision == "mixed")
 

@@ -112,6 +112,8 @@ struct fmmh_context {
     float sel_margin = 0.25f;         // FMMH_OPT_ORDER_MARGIN
     double order_tol = 0.0;           // FMMH_OPT_ORDER_TOL: requested relative RMS field error (0: off)
     double sel_kappa = 1.0;           // FMMH_OPT_ORDER_SAFETY
+    double sel_adapt = 1.0;           // tolerance scale: tightened after each fallback, reset by the options above
+    double sel_full_err = -1.0;       // last call: spot-check error of the full-order result after a fallback
     int spot_checks = 0;              // FMMH_OPT_SPOT_CHECKS
     int sel_force_full = 0;           // FMMH_OPT_ORDER_FORCE_FULL: selection machinery on, every pair at p
     // Tile-shape autotuner (FMMH_OPT_AUTOTUNE): the many-sets M2L variant chosen per
@@ -474,8 +476,9 @@ const double* upload_disp(fmmh_context* c, const Periodic& P) {
     FMMH_HIP(hipMemcpyAsync(d, h, sizeof(P.disp), hipMemcpyHostToDevice, c->stream));
     return d;
 }
-constexpr int TRAVERSE_GRID = 1024;
-constexpr double M2L_GEMM_MAX_BYTES = 48e9;   // per-pair slot buffer of the class-grouped M2L   // workgroups per traversal step (grid-stride over the frontier)
+constexpr int TRAVERSE_GRID = 1024;           // workgroups per traversal step (grid-stride over the frontier)
+constexpr double M2L_GEMM_MAX_BYTES = 48e9;   // per-pair slot buffer of the class-grouped M2L
+constexpr int M2L_MFMA_MIN_SETS = 6;          // auto engine: per-target MFMA from this many sets on
 
 struct Call {
     const double* X; int64_t ldx;
@@ -848,14 +851,18 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         c->stats_classes = 0;
     }
     c->mark("m2l_classes");
+    // auto: the per-target MFMA form from M2L_MFMA_MIN_SETS sets on (a 16-set tile, padded, costs about
+    // as much as 6 passes of the class-grouped GEMM at p 10), or whenever the GEMM's slot buffer does not fit
+    const bool gemm_fits = p <= M2L_GEMM_PMAX && np > 0 && (double)np * nm * sizeof(double) <= M2L_GEMM_MAX_BYTES;
     const bool mfma = c->m2l_mfma == 2 ? m2l_mfma_supported(p, 16)
-                                       : (c->m2l_mfma == 1 && m2l_mfma_supported(p, nsets));
+                                       : (c->m2l_mfma == 1 && (m2l_mfma_supported(p, nsets) ||
+                                                               (m2l_mfma_supported(p, 16) &&
+                                                                (nsets >= M2L_MFMA_MIN_SETS || !gemm_fits))));
     if (order && c->m2l_mfma == 0)
         throw Error(FMMH_ERR_UNSUPPORTED, "learned expansion orders need an MFMA M2L engine (FMMH_OPT_M2L_MFMA != 0)");
     const size_t node_stride = (size_t)nn * nm;
     // class-grouped GEMM: one set at a time, per-pair slots of np x nm doubles
-    const bool gemm = !mfma && (c->m2l_mfma == 1 || c->m2l_mfma == 3) && p <= M2L_GEMM_PMAX && np > 0 &&
-                      (double)np * nm * sizeof(double) <= M2L_GEMM_MAX_BYTES;
+    const bool gemm = !mfma && (c->m2l_mfma == 1 || c->m2l_mfma == 3) && gemm_fits;
     if (gemm) {
         const int* flag = c->buf<int>("m_flag", np);
         const int* cls = c->buf<int>("m_cls", np);
@@ -1045,7 +1052,7 @@ const unsigned char* select_orders(fmmh_context* c, TreeOut& tr, ListsOut& L, Ta
     double* acc = c->buf<double>("sel_acc", 4);
     FMMH_HIP(hipMemsetAsync(acc, 0, 4 * sizeof(double), s));
     launch_sel_budget(near, n, n, nf, tr.t.leaf_node, tr.t.count, tr.t.parent, L.m2l_off, tr.num_leaves, acc,
-                      c->order_tol * c->sel_kappa, acc + 2, s);
+                      c->order_tol * c->sel_kappa * c->sel_adapt, acc + 2, s);
     unsigned char* order = c->buf<unsigned char>("sel_order", np);
     unsigned long long* hist = c->buf<unsigned long long>("sel_hist", PMAX + 1);
     FMMH_HIP(hipMemsetAsync(hist, 0, (PMAX + 1) * sizeof(unsigned long long), s));
@@ -1214,6 +1221,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     downward();
     c->sel_stats[0] = -1.0;
     c->sel_stats[1] = 0.0;
+    c->sel_full_err = -1.0;
     if (check) {
         const double err = spot_check(c, tr, L, xs, ys, zs, ms, n, m, nf, near, k, before, c->spot_checks);
         c->sel_stats[0] = err;
@@ -1226,6 +1234,15 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
             run_m2l(c, tr, L, tb, p, S, CM, LE, true, P, disp, nullptr);
             c->mark("m2l_fallback");
             downward();
+            // the full-order result at the same sites; where it meets the tolerance, later calls get a
+            // per-pair tolerance tighter by about the factor the learned orders missed (where it does
+            // not, the tolerance is out of reach at this expansion_order and tightening cannot help)
+            const double err_full = spot_check(c, tr, L, xs, ys, zs, ms, n, m, nf, near, k, before, c->spot_checks);
+            c->sel_full_err = err_full;
+            if (err_full <= c->order_tol) {
+                const double f = err > 0.0 && std::isfinite(err) ? 0.8 * c->order_tol / err : 0.5;
+                c->sel_adapt = std::max(c->sel_adapt * std::min(0.5, std::max(0.125, f)), 1.0 / 256.0);
+            }
         }
     }
     if (select) {
@@ -1870,6 +1887,7 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         } else if (key == FMMH_OPT_ORDER_TOL) {
             if (!(value >= 0.0)) throw Error(FMMH_ERR_ARG, "order tolerance must be >= 0");
             c->order_tol = value;
+            c->sel_adapt = 1.0;
         } else if (key == FMMH_OPT_ORDER_MARGIN) {
             if (!(value >= -1.0 && value <= 11.0)) throw Error(FMMH_ERR_ARG, "order margin must be in -1..11");
             c->sel_margin = (float)value;
@@ -1879,6 +1897,7 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         } else if (key == FMMH_OPT_ORDER_SAFETY) {
             if (!(value > 0.0)) throw Error(FMMH_ERR_ARG, "order safety factor must be > 0");
             c->sel_kappa = value;
+            c->sel_adapt = 1.0;
         } else if (key == FMMH_OPT_AUTOTUNE) {
             c->autotune = value != 0.0 ? 1 : 0;
             if (value < 0.0) c->tuned.clear();   // forget earlier choices
@@ -2143,6 +2162,7 @@ int fmmh_selector_train(fmmh_context* ctx, const double* coordinates, int64_t ld
         c->train_loss.assign(steps, 0.f);
         FMMH_HIP(hipMemcpy(c->train_loss.data(), loss_log, steps * sizeof(float), hipMemcpyDeviceToHost));
         c->sel_trained = true;
+        c->sel_adapt = 1.0;
         c->sel_cap = cap;
         c->sel_rel_lo = ts.rel_lo;
         c->sel_rel_hi = ts.rel_hi;
@@ -2213,6 +2233,7 @@ int fmmh_selector_set(fmmh_context* ctx, const float* weights, const float* norm
         c->sel_rel_hi = (float)meta[2];
         c->sel_margin = (float)meta[3];
         c->sel_trained = true;
+        c->sel_adapt = 1.0;
     });
 }
 
@@ -2265,6 +2286,8 @@ int fmmh_selection_stats(fmmh_context* ctx, double* out, int32_t cap) {
         fmmh_context* c = resolve(ctx);
         for (int i = 0; i < std::min(cap, 8); ++i) out[i] = c->sel_stats[i];
         for (int q = 0; q <= PMAX && 8 + q < cap; ++q) out[8 + q] = (double)c->sel_hist[q];
+        if (cap > 9 + PMAX) out[9 + PMAX] = c->sel_adapt;
+        if (cap > 10 + PMAX) out[10 + PMAX] = c->sel_full_err;
     });
 }
 

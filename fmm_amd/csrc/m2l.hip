@@ -376,9 +376,11 @@ __device__ __forceinline__ void dma16(const void* g, void* lds) {
 template <int P, bool F32>
 constexpr int mfma_kpad() { return ((ncomp(P) + 3) / 4 + 1) / 2 * 2; }   // k-steps, even
 
+// Source blocks are staged in LDS when the two buffers take at most 80 KB (with the class tensors
+// and tables a workgroup then stays under 96 KB: p 10 FP32 with 16 sets runs one workgroup per CU).
 template <int P, bool F32, int NTL>
 constexpr bool mfma_stage_b() {
-    return 2 * 4 * (mfma_kpad<P, F32>() + 2) * 16 * NTL * (F32 ? 4 : 8) <= 65536;
+    return 2 * 4 * (mfma_kpad<P, F32>() + 2) * 16 * NTL * (F32 ? 4 : 8) <= 81920;
 }
 
 // The k-loop is branch-free: the A element of lane (row, kq) is read from the

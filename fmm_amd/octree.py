@@ -417,11 +417,12 @@ class Context:
         return {int(k): int(v) for k, v in zip(keys[:n], var[:n])}, ms
 
     def selection_stats(self):
-        out = np.zeros(19)
-        check(self._lib.fmmh_selection_stats(self._h, _p(out), 19))
+        out = np.zeros(21)
+        check(self._lib.fmmh_selection_stats(self._h, _p(out), 21))
         return dict(spot_error=float(out[0]), fallback=bool(out[1]), tau=float(out[2]), m2l_flops=float(out[3]),
                     m2l_flops_full=float(out[4]), mean_order=float(out[5]), active=bool(out[6]),
-                    near_rms=float(out[7]), order_hist=out[8:].astype(np.int64))
+                    near_rms=float(out[7]), order_hist=out[8:19].astype(np.int64), tolerance_scale=float(out[19]),
+                    full_order_spot_error=float(out[20]))
 
     def stats(self):
         out = np.zeros(8, dtype=np.int64)

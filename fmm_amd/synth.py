@@ -109,7 +109,8 @@ def gaussian_clusters(n_clusters, per_cluster, side, sigma, seed):
 def config(idx, scale=1.0):
     """The five benchmark workloads.  `scale` < 1 shrinks the site count
     (for tests) keeping every other parameter.  Returns a dict of the call
-    arguments of field_fmm plus 'name'."""
+    arguments of field_fmm plus 'name'; idx 4 (many systems) returns the
+    per-system arrays under 'systems'."""
     seed = 1000 + idx
     if idx == 0:
         n = max(int(4096 * scale), 8)
@@ -146,4 +147,26 @@ def config(idx, scale=1.0):
         return dict(name="clusters-1M-erf-quadrupole", coordinates=x, multipoles=m,
                     exclusions=self_exclusions(n), theta=0.6, ncrit=64, expansion_order=8,
                     field_order=1, damp_type="ERF", damping_factors=df, sets=1)
+    if idx == 4:
+        # An uneven batch: 64 water boxes built as for idx 1, sizes log-uniform between 10k and 500k sites
+        # (times scale), THOLE, 8 multipole sets each (the water charges plus per-set dipoles N(0, 0.05)),
+        # field order 1, learned M2L orders within a cap of 10 at a 1e-6 tolerance.  theta is not given for
+        # this workload: at idx 1's 0.5 even order 10 leaves ~3e-5 of far-field truncation error, and at 0.3
+        # the largest THOLE systems still miss 1e-6 at order 10 (spot errors 1.0e-6 at 211k and 2.2e-6 at 494k
+        # sites: damping shrinks the near field the error is relative to), so a 1e-6 tolerance could only fall
+        # back there; 0.25 meets it on every system with learned orders (mean order 7-8).
+        rng = np.random.default_rng(seed)
+        nsys, sets = 64, 8
+        sizes = np.exp(rng.uniform(np.log(10_000.0), np.log(500_000.0), nsys)) * scale
+        systems = []
+        for k, size in enumerate(sizes):
+            x, q, excl, df, r2 = water_box(max(int(size / 3), 2), seed * 100 + k)
+            n = x.shape[0]
+            m = np.zeros((n, 4, sets), order="F")
+            for u in range(sets):
+                m[:, 0, u] = q
+                m[:, 1:4, u] = r2.normal(0.0, 0.05, (n, 3))
+            systems.append(dict(coordinates=x, multipoles=m, exclusions=excl, damping_factors=df))
+        return dict(name="water-uneven-64-systems-8sets-thole", systems=systems, theta=0.25, ncrit=64,
+                    expansion_order=10, field_order=1, damp_type="THOLE", sets=sets, order_tolerance=1e-6)
     raise ValueError(f"unknown config {idx}")

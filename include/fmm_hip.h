@@ -336,7 +336,12 @@ int fmmh_selector_predict(fmmh_context* ctx, const float* weights, const float* 
  * error (-1: not checked), [1] 1 if it fell back to expansion_order, [2] per-pair
  * tolerance tau, [3] M2L flops at the chosen orders (2 n_m(q)^2 per pair and set),
  * [4] M2L flops at expansion_order, [5] mean order, [6] 1 if selection ran,
- * [7] RMS near field used for tau, [8 + q] pairs at order q (q = 0..10). */
+ * [7] RMS near field used for tau, [8 + q] pairs at order q (q = 0..10), [19] the
+ * context's tolerance scale for later calls (1; after a fallback whose full-order
+ * result meets the tolerance, multiplied by about tol / error, at most 1/2 and at
+ * least 1/8, floor 1/256; reset to 1 by setting FMMH_OPT_ORDER_TOL or
+ * FMMH_OPT_ORDER_SAFETY and by training or loading weights), [20] the spot-check
+ * error of the full-order result after a fallback (-1: no fallback). */
 int fmmh_selection_stats(fmmh_context* ctx, double* out, int32_t cap);
 
 /* Tile-shape choices of the autotuner (FMMH_OPT_AUTOTUNE): up to cap (key, variant) pairs, key = order << 32 |

@@ -227,6 +227,37 @@ def test_impossible_tolerance_falls_back_to_expansion_order(trained):
         st = ctx.selection_stats()
         assert st["fallback"] and st["spot_error"] > 1e-13
         assert np.array_equal(sel, full)
+        # the full-order result misses 1e-13 too: the tolerance is out of reach, nothing is tightened
+        assert st["full_order_spot_error"] > 1e-13
+        assert st["tolerance_scale"] == 1.0
+    finally:
+        ctx.close()
+
+
+def test_fallback_tightens_later_calls_when_full_order_meets_tolerance(trained):
+    """Per-pair tolerances 1000x too loose: the learned orders miss 1e-4, the
+    full-order result meets it, so later calls get a per-pair tolerance
+    tighter by about the factor missed (at most 1/2, at least 1/8 per
+    fallback); setting the tolerance again resets it."""
+    from fmm_amd.octree import Context
+    _, _, _, (W, norm, meta) = trained
+    x, m, e, _ = _water(2000, 43)
+    ctx = Context(0)
+    try:
+        ctx.set_selector_weights(W, norm, meta)
+        full = _fmm(ctx, x, m, e, 0.4, 8)
+        ctx.set_order_tolerance(1e-4, spot_checks=256, safety=1000.0)
+        sel = _fmm(ctx, x, m, e, 0.4, 8)
+        st = ctx.selection_stats()
+        assert st["fallback"] and st["spot_error"] > 1e-4
+        assert 0.0 <= st["full_order_spot_error"] <= 1e-4
+        assert np.array_equal(sel, full)
+        assert 0.125 - 1e-15 <= st["tolerance_scale"] <= 0.5
+        _fmm(ctx, x, m, e, 0.4, 8)
+        st2 = ctx.selection_stats()
+        assert st2["tau"] == pytest.approx(st["tau"] * st["tolerance_scale"], rel=1e-12)
+        ctx.set_order_tolerance(1e-4, spot_checks=256, safety=1000.0)
+        assert ctx.selection_stats()["tolerance_scale"] == 1.0
     finally:
         ctx.close()
 
