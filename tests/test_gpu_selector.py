@@ -321,3 +321,42 @@ def test_learned_orders_with_many_sets(trained):
             assert rel(sel[:, 1:4, k], d[:, 1:4]) <= 2e-4 + rel(full[:, 1:4, k], d[:, 1:4])
     finally:
         ctx.close()
+
+
+def test_uneven_batch_of_systems_meets_tolerance():
+    """configs[4] at 1/50 scale: water systems of uneven size, 8 THOLE dipole
+    sets each, learned orders (cap 10, tolerance 1e-6, spot checks) in mixed
+    precision with a selector trained at the batch's theta.  Every system
+    either passes its spot check, with every set within 2e-6 of the same call
+    at full order, or falls back to exactly the full-order result."""
+    from fmm_amd.octree import Context
+    c = synth.config(4, scale=0.02)
+    sizes = np.array([sy["coordinates"].shape[0] for sy in c["systems"]])
+    order = np.argsort(sizes)
+    th, cap = c["theta"], c["expansion_order"]
+    xt, mt, _, _ = _water(3000, 61)
+    ctx = Context(0)
+    try:
+        ctx.train_selector(xt, mt, th, c["ncrit"], cap, nsamples=2048, epochs=40, seed=7)
+        ctx.set_precision(True)
+        for idx in (order[0], order[20], order[44], order[-1]):
+            sy = c["systems"][idx]
+            x, m, e, df = sy["coordinates"], sy["multipoles"], sy["exclusions"], sy["damping_factors"]
+            ctx.set_order_tolerance(0.0)
+            full = ctx.field_fmm_sets(0, x, m, e, th, c["ncrit"], cap, 1, "THOLE", df)
+            ctx.set_order_tolerance(c["order_tolerance"], spot_checks=256)
+            got = ctx.field_fmm_sets(0, x, m, e, th, c["ncrit"], cap, 1, "THOLE", df)
+            st = ctx.selection_stats()
+            if ctx.stats()["n_m2l"] == 0:
+                # too small for any far pair: nothing to select, the plain result
+                assert not st["active"] and np.array_equal(got, full)
+                continue
+            assert st["active"] and st["spot_error"] >= 0.0, (sizes[idx], st)
+            if st["fallback"]:
+                assert np.array_equal(got, full)
+            else:
+                assert st["spot_error"] <= c["order_tolerance"]
+                for k in range(c["sets"]):
+                    assert rel(got[:, 1:4, k], full[:, 1:4, k]) <= 2e-6, (sizes[idx], k)
+    finally:
+        ctx.close()
