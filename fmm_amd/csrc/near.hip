@@ -27,7 +27,8 @@ constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: never
 
 // Lane layout: a target block is taken in passes of nt <= 64 sites, each
 // replicated over r = 64 / nt lane groups (group g = lane / nt, target
-// lane % nt); group g takes every r-th source, so small leaves still fill the
+// lane % nt); group g takes every r-th source of the run (positions g mod r,
+// whatever the tile size), so small leaves still fill the
 // wavefront, and the pass size is chosen per block so that few lanes idle.
 // At the end of a pass the groups' partial fields are summed in group order
 // through LDS (deterministic).
@@ -45,17 +46,29 @@ constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: never
 // forms and the rest get more.
 constexpr int NEAR_WIN = 64;
 
+// Source-tile size for S sets of NM components: the largest power of two <= 64 with S NM TILE
+// doubles in at most 8 KB (4 sets of quadrupoles: 16 sites, 3 waves per SIMD instead of 1.5;
+// 4 sets of dipoles keep 64, measured faster than 32 there).
+constexpr int near_tile(int snm) {
+    int t = 64;
+    while (t > 8 && snm * t * 8 > 8192) t /= 2;
+    return t;
+}
+
 template <int M, int F, int DAMP, int S>
 __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 1)
                                         : ((M + F <= 3 && DAMP != DAMP_ERF) ? 2 : 1)) void k_near(NearArgs a) {
     constexpr int K = M + F;
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
-    constexpr bool PREF_M = NM * S <= 4;          // multipoles ride in the prefetch registers too
-    constexpr int NPM = PREF_M ? NM * S : 1;
-    __shared__ double sx[64], sy[64], sz[64], sdf[64];
-    __shared__ double sm[S][NM][64];
-    __shared__ int sorig[64];
+    // Source tile: TILE sites, sized so that the staged multipoles take at most 8 KB (S NM TILE
+    // doubles), which keeps several workgroups per CU; all 64 lanes prefetch the tile's multipoles
+    // cooperatively (NPM registers each, element e = lane + 64 i of the [S][NM][TILE] block).
+    constexpr int TILE = near_tile(S * NM);
+    constexpr int NPM = (S * NM * TILE + 63) / 64;
+    __shared__ double sx[TILE], sy[TILE], sz[TILE], sdf[TILE];
+    __shared__ double sm[S][NM][TILE];
+    __shared__ int sorig[TILE];
     __shared__ double red[NF][64];
     __shared__ int wcum[NEAR_WIN + 1];            // window: exclusive prefix of source counts
     __shared__ int wstart[NEAR_WIN];              // window: first sorted site of each entry
@@ -133,15 +146,17 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
             if (lane == wn - 1) wcum[wn] = incl;
             __syncthreads();
             const int total = wcum[wn];
-            // prefetch registers for tile position v = vbase + lane
+            // prefetch registers for tile position v = vbase + lane (lane < TILE) and the tile's
+            // multipoles (all lanes)
             double px = 0.0, py = 0.0, pz = 0.0, pdf = 0.0;
             double pm[NPM];
-            int po = 0, pj = 0;
+            int po = 0;
             int cur = 0;                         // list entry holding the tile's first position
             auto fetch = [&](int vbase) {
                 while (wcum[cur + 1] <= vbase) ++cur;
                 const int v = vbase + lane;
-                if (v < total) {
+                int pj = -1;
+                if (lane < TILE && v < total) {
                     int e = cur;
                     while (wcum[e + 1] <= v) ++e;
                     const int j = wstart[e] + (v - wcum[e]);
@@ -153,36 +168,37 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
                     pz = a.zs[j] + D[2];
                     if constexpr (DAMP != DAMP_NONE) pdf = a.dfs[j];
                     po = img == 0 ? a.orig[j] + 1 : NEAR_IMAGE_SRC;   // exclusions: central cell only
-                    if constexpr (PREF_M) {
+                }
 #pragma unroll
-                        for (int u = 0; u < S; ++u)
-#pragma unroll
-                            for (int s = 0; s < NM; ++s)
-                                pm[u * NM + s] = u < ns ? ms[((int64_t)u * NM + s) * a.ldm + j] : 0.0;
-                    }
+                for (int i = 0; i < NPM; ++i) {
+                    const int el = lane + 64 * i;           // [u][s][t] of the tile's multipole block
+                    const int t = el % TILE, us = el / TILE;
+                    const int jt = __shfl(pj, t);           // source of tile position t (-1: none)
+                    const int u = us / NM, sc = us - u * NM;
+                    pm[i] = (el < S * NM * TILE && u < ns && jt >= 0) ? ms[((int64_t)u * NM + sc) * a.ldm + jt] : 0.0;
                 }
             };
             if (total > 0) fetch(0);
-            for (int vbase = 0; vbase < total; vbase += 64) {
-                const int cn = min(64, total - vbase);
+            for (int vbase = 0; vbase < total; vbase += TILE) {
+                const int cn = min(TILE, total - vbase);
                 if (lane < cn) {
                     sx[lane] = px;
                     sy[lane] = py;
                     sz[lane] = pz;
                     if constexpr (DAMP != DAMP_NONE) sdf[lane] = pdf;
                     sorig[lane] = po;
+                }
 #pragma unroll
-                    for (int u = 0; u < S; ++u)
-#pragma unroll
-                        for (int s = 0; s < NM; ++s) {
-                            if constexpr (PREF_M) sm[u][s][lane] = pm[u * NM + s];
-                            else sm[u][s][lane] = u < ns ? ms[((int64_t)u * NM + s) * a.ldm + pj] : 0.0;
-                        }
+                for (int i = 0; i < NPM; ++i) {
+                    const int el = lane + 64 * i;
+                    if (el < S * NM * TILE) (&sm[0][0][0])[el] = pm[i];
                 }
                 __syncthreads();
-                if (vbase + 64 < total) fetch(vbase + 64);
+                if (vbase + TILE < total) fetch(vbase + TILE);
                 if (active) {
-                    for (int jj = g; jj < cn; jj += r) {
+                    // group g takes the run positions v = g mod r, in increasing order: the sums do not
+                    // depend on the tile size (single- and many-set forms add in the same order)
+                    for (int jj = ((g - vbase) % r + r) % r; jj < cn; jj += r) {
                         const double dx = xi - sx[jj];
                         const double dy = yi - sy[jj];
                         const double dz = zi - sz[jj];
