@@ -231,30 +231,296 @@ __global__ __launch_bounds__(64) void k_l2p(DownArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- many sets
+// The same four operators for SB sets per workgroup: monomials, shift tables and
+// staged inputs are shared by the block of sets, and every (coefficient, set)
+// output is summed in exactly the order of the one-set kernels above, so a set
+// of a many-set call equals the one-set call bit for bit.
+constexpr int SHIFT_SB = 8;        // sets per workgroup (P2M, M2M, L2L; L2P with field order <= 1)
+constexpr int SHIFT_Q = (MAX_NM + 255) / 256;
+
+template <int SB>
+__global__ __launch_bounds__(128) void k_p2m_sb(UpArgs a) {
+    __shared__ double mono[UP_CHUNK][MAX_NM];
+    __shared__ double mch[UP_CHUNK][UP_NSRC][SB];
+    __shared__ double d3[UP_CHUNK * 3];
+    __shared__ double pw[UP_CHUNK * 3 * (PMAX + 1)];
+    constexpr int Q = (MAX_NM + 127) / 128;
+    const int node = a.leaf_node[blockIdx.x];
+    const int set0 = blockIdx.y * SB;
+    const int nsb = min(SB, a.nsets - set0);
+    const int s0 = a.start[node], c = a.count[node];
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    const int nm = a.nm;
+    const int nsrc = a.nsrc;
+    double acc[Q][SB];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int u = 0; u < SB; ++u) acc[q][u] = 0.0;
+    for (int j0 = 0; j0 < c; j0 += UP_CHUNK) {
+        const int cc = min(UP_CHUNK, c - j0);
+        if ((int)threadIdx.x < 3 * cc) {
+            const int jj = threadIdx.x / 3, ax = threadIdx.x - 3 * jj;
+            const int j = s0 + j0 + jj;
+            d3[threadIdx.x] = ax == 0 ? a.xs[j] - cx : (ax == 1 ? a.ys[j] - cy : a.zs[j] - cz);
+        }
+        __syncthreads();
+        build_mono_block(&mono[0][0], MAX_NM, nm, cc, d3, pw, a.xyz, a.p);
+        for (int x = threadIdx.x; x < SB * nsrc * cc; x += blockDim.x) {
+            const int u = x / (nsrc * cc), r = x - u * nsrc * cc, sc = r / cc, jj = r - sc * cc;
+            mch[jj][sc][u] = u < nsb ? a.ms_raw[((int64_t)(set0 + u) * a.dim + sc) * a.ldm + s0 + j0 + jj] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int t = threadIdx.x + 128 * q;
+            if (t < nm) {
+                const int b0 = a.p2m.off[t], b1 = a.p2m.off[t + 1];
+                for (int b = b0; b < b1; ++b) {
+                    const int sc = a.p2m.s[b], e = a.p2m.e[b];
+                    double sum[SB];
+#pragma unroll
+                    for (int u = 0; u < SB; ++u) sum[u] = 0.0;
+                    for (int jj = 0; jj < cc; ++jj) {
+                        const double mo = mono[jj][e];
+#pragma unroll
+                        for (int u = 0; u < SB; ++u) sum[u] = __builtin_fma(mo, mch[jj][sc][u], sum[u]);
+                    }
+                    const double cf = a.p2m.coef[b];
+#pragma unroll
+                    for (int u = 0; u < SB; ++u) acc[q][u] = __builtin_fma(cf, sum[u], acc[q][u]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int t = threadIdx.x + 128 * q;
+        if (t < nm)
+#pragma unroll
+            for (int u = 0; u < SB; ++u)
+                if (u < nsb) a.CM[((int64_t)(set0 + u) * a.num_nodes + node) * nm + t] = acc[q][u];
+    }
+}
+
+template <int SB>
+__global__ __launch_bounds__(256) void k_m2m_sb(UpArgs a, const int* __restrict__ nodes) {
+    __shared__ double monos[8][MAX_NM];
+    __shared__ double chb[MAX_NM][SB];
+    __shared__ double d3[24];
+    __shared__ double pw[24 * (PMAX + 1)];
+    const int node = nodes[blockIdx.x];
+    const int set0 = blockIdx.y * SB;
+    const int nsb = min(SB, a.nsets - set0);
+    const int nm = a.nm;
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    double acc[SHIFT_Q][SB];
+#pragma unroll
+    for (int q = 0; q < SHIFT_Q; ++q)
+#pragma unroll
+        for (int u = 0; u < SB; ++u) acc[q][u] = 0.0;
+    if (threadIdx.x < 24) {
+        const int o = threadIdx.x / 3, ax = threadIdx.x - 3 * o;
+        const int ch = a.children[node * 8 + o];
+        d3[threadIdx.x] = ch < 0 ? 0.0 : (ax == 0 ? a.cx[ch] - cx : (ax == 1 ? a.cy[ch] - cy : a.cz[ch] - cz));
+    }
+    __syncthreads();
+    build_mono_block(&monos[0][0], MAX_NM, nm, 8, d3, pw, a.xyz, a.p);
+    bool any = false;
+    for (int o = 0; o < 8; ++o) {
+        const int ch = a.children[node * 8 + o];
+        if (ch < 0) continue;
+        any = true;
+        __syncthreads();                      // readers of the previous child are done
+        for (int x = threadIdx.x; x < SB * nm; x += blockDim.x) {
+            const int u = x / nm, sc = x - u * nm;
+            chb[sc][u] = u < nsb ? a.CM[((int64_t)(set0 + u) * a.num_nodes + ch) * nm + sc] : 0.0;
+        }
+        __syncthreads();
+        const double* __restrict__ mono = monos[o];
+#pragma unroll
+        for (int q = 0; q < SHIFT_Q; ++q) {
+            const int t = threadIdx.x + 256 * q;
+            if (t < nm) {
+                const int b0 = a.m2m.off[t], b1 = a.m2m.off[t + 1];
+                double sum[SB];
+#pragma unroll
+                for (int u = 0; u < SB; ++u) sum[u] = 0.0;
+                for (int b = b0; b < b1; ++b) {
+                    const double cf = a.m2m.coef[b] * mono[a.m2m.e[b]];
+                    const int sc = a.m2m.s[b];
+#pragma unroll
+                    for (int u = 0; u < SB; ++u) sum[u] = __builtin_fma(cf, chb[sc][u], sum[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < SB; ++u) acc[q][u] += sum[u];
+            }
+        }
+    }
+    if (!any) return;   // leaves keep their P2M moments
+#pragma unroll
+    for (int q = 0; q < SHIFT_Q; ++q) {
+        const int t = threadIdx.x + 256 * q;
+        if (t < nm)
+#pragma unroll
+            for (int u = 0; u < SB; ++u)
+                if (u < nsb) a.CM[((int64_t)(set0 + u) * a.num_nodes + node) * nm + t] = acc[q][u];
+    }
+}
+
+template <int SB>
+__global__ __launch_bounds__(256) void k_l2l_sb(DownArgs a, const int* __restrict__ nodes) {
+    __shared__ double mono[MAX_NM];
+    __shared__ double lp[MAX_NM][SB];
+    __shared__ double d3[3];
+    __shared__ double pw[3 * (PMAX + 1)];
+    const int node = nodes[blockIdx.x];
+    const int par = a.parent[node];
+    const int set0 = blockIdx.y * SB;
+    const int nsb = min(SB, a.nsets - set0);
+    const int nm = a.nm;
+    if (threadIdx.x < 3) {
+        const int ax = threadIdx.x;
+        d3[ax] = ax == 0 ? a.cx[node] - a.cx[par] : (ax == 1 ? a.cy[node] - a.cy[par] : a.cz[node] - a.cz[par]);
+    }
+    for (int x = threadIdx.x; x < SB * nm; x += blockDim.x) {
+        const int u = x / nm, sc = x - u * nm;
+        lp[sc][u] = u < nsb ? a.LE[((int64_t)(set0 + u) * a.num_nodes + par) * nm + sc] : 0.0;
+    }
+    __syncthreads();
+    build_mono_block(mono, MAX_NM, nm, 1, d3, pw, a.xyz, a.p);
+    for (int t = threadIdx.x; t < nm; t += blockDim.x) {
+        const int b0 = a.l2l.off[t], b1 = a.l2l.off[t + 1];
+        double sum[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) sum[u] = 0.0;
+        for (int b = b0; b < b1; ++b) {
+            const double cf = a.l2l.coef[b] * mono[a.l2l.e[b]];
+            const int sc = a.l2l.s[b];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) sum[u] = __builtin_fma(cf, lp[sc][u], sum[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+            if (u < nsb) a.LE[((int64_t)(set0 + u) * a.num_nodes + node) * nm + t] += sum[u];
+    }
+}
+
+template <int F, int SB>
+__global__ __launch_bounds__(64) void k_l2p_sb(DownArgs a) {
+    constexpr int NF = ncomp(F);
+    __shared__ double L[SB][MAX_NM];
+    __shared__ double fact[PMAX + 1];
+    const int node = a.leaf_node[blockIdx.x];
+    const int set0 = blockIdx.y * SB;
+    const int nsb = min(SB, a.nsets - set0);
+    const int s0 = a.start[node], c = a.count[node];
+    const int nm = a.nm, p = a.p;
+    for (int x = threadIdx.x; x < SB * nm; x += 64) {
+        const int u = x / nm, sc = x - u * nm;
+        L[u][sc] = u < nsb ? a.LE[((int64_t)(set0 + u) * a.num_nodes + node) * nm + sc] : 0.0;
+    }
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        fact[0] = 1.0;
+        for (int n = 1; n <= PMAX; ++n) { f *= (double)n; fact[n] = 1.0 / f; }
+    }
+    __syncthreads();
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    for (int base = 0; base < c; base += 64) {
+        const int jj = base + threadIdx.x;
+        if (jj >= c) break;
+        const int k = s0 + jj;
+        const double dx = a.xs[k] - cx, dy = a.ys[k] - cy, dz = a.zs[k] - cz;
+        double f[SB][NF];
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+#pragma unroll
+            for (int t = 0; t < NF; ++t) f[u][t] = 0.0;
+        double px = 1.0;
+        for (int ex = 0; ex <= p; ++ex) {
+            double pxy = px;
+            for (int ey = 0; ey + ex <= p; ++ey) {
+                double m = pxy * fact[ex] * fact[ey];
+                for (int ez = 0; ez + ey + ex <= p; ++ez) {
+                    const double mz = m * fact[ez];
+                    const int ne = ex + ey + ez;
+#pragma unroll
+                    for (int t = 0; t < NF; ++t) {
+                        const Exps et = idx2xyz(t);
+                        if (et.x + et.y + et.z + ne <= p) {
+                            const int li = xyz2idx(et.x + ex, et.y + ey, et.z + ez);
+#pragma unroll
+                            for (int u = 0; u < SB; ++u) f[u][t] = __builtin_fma(mz, L[u][li], f[u][t]);
+                        }
+                    }
+                    m *= dz;
+                }
+                pxy *= dy;
+            }
+            px *= dx;
+        }
+        const int o = a.perm[k];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            if (u >= nsb) break;
+            const double* __restrict__ near = a.near + (int64_t)(set0 + u) * NF * a.ld_near;
+            double* __restrict__ field = a.field + (int64_t)(set0 + u) * NF * a.ld_field;
+#pragma unroll
+            for (int t = 0; t < NF; ++t) {
+                double* fp = field + (int64_t)t * a.ld_field + o;
+                *fp = (*fp + near[(int64_t)t * a.ld_near + k]) + f[u][t];
+            }
+        }
+    }
+}
+
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
-    hipLaunchKernelGGL(k_p2m, dim3(num_leaves, a.nsets), dim3(128), 0, s, a);
+    if (a.nsets == 1) hipLaunchKernelGGL(k_p2m, dim3(num_leaves, 1), dim3(128), 0, s, a);
+    else hipLaunchKernelGGL(k_p2m_sb<SHIFT_SB>, dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(128), 0, s, a);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_m2m, dim3(count, a.nsets), dim3(128), 0, s, a, nodes);
+    if (a.nsets == 1) hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(128), 0, s, a, nodes);
+    else hipLaunchKernelGGL(k_m2m_sb<SHIFT_SB>, dim3(count, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(256), 0, s, a, nodes);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_l2l, dim3(count, a.nsets), dim3(128), 0, s, a, nodes);
+    if (a.nsets == 1) hipLaunchKernelGGL(k_l2l, dim3(count, 1), dim3(128), 0, s, a, nodes);
+    else hipLaunchKernelGGL(k_l2l_sb<SHIFT_SB>, dim3(count, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(256), 0, s, a, nodes);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
-    switch (a.nf) {
-        case 1: hipLaunchKernelGGL(k_l2p<0>, dim3(num_leaves, a.nsets), dim3(64), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_l2p<1>, dim3(num_leaves, a.nsets), dim3(64), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_l2p<2>, dim3(num_leaves, a.nsets), dim3(64), 0, s, a); break;
+    if (a.nsets == 1) {
+        switch (a.nf) {
+            case 1: hipLaunchKernelGGL(k_l2p<0>, dim3(num_leaves, 1), dim3(64), 0, s, a); break;
+            case 4: hipLaunchKernelGGL(k_l2p<1>, dim3(num_leaves, 1), dim3(64), 0, s, a); break;
+            default: hipLaunchKernelGGL(k_l2p<2>, dim3(num_leaves, 1), dim3(64), 0, s, a); break;
+        }
+    } else {
+        constexpr int SB2 = SHIFT_SB / 2;   // field gradients: 10 components per set in registers
+        switch (a.nf) {
+            case 1:
+                hipLaunchKernelGGL((k_l2p_sb<0, SHIFT_SB>), dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB),
+                                   dim3(64), 0, s, a);
+                break;
+            case 4:
+                hipLaunchKernelGGL((k_l2p_sb<1, SHIFT_SB>), dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB),
+                                   dim3(64), 0, s, a);
+                break;
+            default:
+                hipLaunchKernelGGL((k_l2p_sb<2, SB2>), dim3(num_leaves, (a.nsets + SB2 - 1) / SB2), dim3(64), 0, s, a);
+                break;
+        }
     }
     FMMH_HIP(hipGetLastError());
 }
