@@ -394,6 +394,21 @@ def main():
         step()
     torch.cuda.synchronize()
     reuse_ms = (time.perf_counter() - r0) * 1e3 / args.steps
+    # and with the device work of those calls replayed as a HIP graph (timing events off: a graph
+    # holds no per-phase events); the first call runs eagerly, the second captures
+    ctx.set_timing(False)
+    ctx.set_graph(True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    r0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - r0) * 1e3 / args.steps
+    graph_replayed = ctx.stats()["graph"] == 2
+    ctx.set_graph(False)
+    ctx.set_timing(True)
     ctx.set_tree_reuse(False)
     value = n * sets / (ms_per_step / 1e3)
 
@@ -491,7 +506,9 @@ def main():
         "phases_ms": phase_avg,
         "tree_reuse": {"ms_per_step": reuse_ms, "value": n * sets / (reuse_ms / 1e3),
                        "note": "same call with FMMH_OPT_REUSE_TREE on: octree, lists and M2L classes kept from "
-                               "the previous call with bitwise-identical coordinates"},
+                               "the previous call with bitwise-identical coordinates",
+                       "graph_ms_per_step": graph_ms, "graph_value": n * sets / (graph_ms / 1e3),
+                       "graph_replayed": graph_replayed},
         "tree": stats,
         "errors": errors,
         "target_1e-6": target,

@@ -76,7 +76,7 @@ struct fmmh_context {
     std::vector<hipEvent_t> event_pool;
     size_t event_used = 0;
     std::vector<std::pair<std::string, double>> phase_ms;
-    int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
     int m2l_mfma = 1;                 // FMMH_OPT_M2L_MFMA: 0 VALU, 1 auto, 2 per-target MFMA, 3 class-grouped
@@ -97,6 +97,21 @@ struct fmmh_context {
     TreeOut cache_tr;
     ListsOut cache_L;
     int cache_class_p = -1;           // expansion order of the cached class tensors (-1: none)
+    bool cache_work_valid = false;    // near-field block order of the cached tree ("s_work_blk", "s_inv")
+    int cache_tiles_p = -1;           // class-grouped GEMM tile tables of the cached classes (-1: none)
+    // Graph replay (FMMH_OPT_GRAPH): a tree-reuse call runs as a captured hipGraph of its device work
+    // once a previous call of the same shape ran eagerly (tuning done, buffers allocated).  The shape key
+    // holds every pointer, size and option the captured work depends on, the pool generation (bumped by
+    // any buffer reallocation, so a replay never sees a freed address) and the cache generation (bumped
+    // whenever the tree, the M2L classes, their tile tables or the near-field block order are rebuilt,
+    // so a replay never skips a rebuild its captured work relied on).
+    int graph_opt = 0;
+    std::string graph_key, graph_pending;
+    hipGraphExec_t graph_exec = nullptr;
+    bool capturing = false;
+    int64_t pool_gen = 0;
+    int64_t cache_gen = 0;
+    int64_t graph_replays = 0;
     int64_t cache_classes = 0;
     unsigned long long cache_pairs = 0;
     // last get_boxes / get_multipoles classification, kept for the export call
@@ -146,6 +161,7 @@ struct fmmh_context {
             size_t nb = std::max(bytes, b.cap + b.cap / 2);
             FMMH_HIP(hipMalloc(&b.p, nb));
             b.cap = nb;
+            ++pool_gen;
         }
         return (T*)b.p;
     }
@@ -177,6 +193,43 @@ struct fmmh_context {
         }
     }
     void sync() { FMMH_HIP(hipStreamSynchronize(stream)); }
+    void graph_exec_replace(hipGraph_t g) {
+        if (graph_exec) {
+            FMMH_HIP(hipGraphExecDestroy(graph_exec));
+            graph_exec = nullptr;
+        }
+        const hipError_t e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+        FMMH_HIP(hipGraphDestroy(g));
+        FMMH_HIP(e);
+    }
+};
+
+// Stream capture of a region: begins on construction when asked; end() returns the graph; if the
+// region throws, the destructor ends the capture and drops it, so the stream is usable again.
+struct GraphCapture {
+    hipStream_t s;
+    bool active;
+    bool* flag;   // true while capturing (no host synchronisation, no autotuning)
+    GraphCapture(hipStream_t st, bool on, bool* f) : s(st), active(on), flag(f) {
+        if (active) {
+            FMMH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            *flag = true;
+        }
+    }
+    hipGraph_t end() {
+        hipGraph_t g = nullptr;
+        active = false;
+        *flag = false;
+        FMMH_HIP(hipStreamEndCapture(s, &g));
+        return g;
+    }
+    ~GraphCapture() {
+        if (active) {
+            *flag = false;
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        }
+    }
 };
 
 namespace {
@@ -821,6 +874,8 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
             ca.class_lo_final = c->buf<unsigned long long>("m_lo_final", np);
             ca.class_key_hi = c->buf<unsigned long long>("m_class_key_hi", np);
         }
+        c->cache_tiles_p = -1;
+        ++c->cache_gen;
         launch_m2l_keys(ca, s);
         size_t tbytes = std::max(sort_pairs_u64_temp(np), exscan_i32_temp(np));
         void* temp = c->buf<char>("m_temp", tbytes);
@@ -903,9 +958,14 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         int* class_off = c->buf<int>("m_class_off", ngrp + 1);
         int* tile_cnt = c->buf<int>("m_tile_cnt", ngrp + 1);
         int* tile_off = c->buf<int>("m_tile_off", ngrp + 1);
-        launch_m2l_tiles(flag, cls, np, ngrp, class_off, tile_cnt, s);
-        size_t tbytes = exscan_i32_temp(ngrp + 1);
-        exscan_i32(c->buf<char>("m_temp_tiles", tbytes), tbytes, tile_cnt, tile_off, ngrp + 1, s);
+        // tile tables of the (cached) classes without learned orders are kept with them
+        if (!(classes_cached && !order && c->cache_tiles_p == p)) {
+            launch_m2l_tiles(flag, cls, np, ngrp, class_off, tile_cnt, s);
+            size_t tbytes = exscan_i32_temp(ngrp + 1);
+            exscan_i32(c->buf<char>("m_temp_tiles", tbytes), tbytes, tile_cnt, tile_off, ngrp + 1, s);
+            c->cache_tiles_p = order ? -1 : p;
+            ++c->cache_gen;
+        }
         const int grid = ngrp + (np + M2L_GEMM_TN - 1) / M2L_GEMM_TN;   // >= tiles
         const bool f32 = c->precision == 1;
         double* pbuf = f32 ? nullptr : c->buf<double>("m_pbuf", (size_t)np * nm);
@@ -950,7 +1010,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         const long long key = ((long long)p << 32) | ((long long)(f32 ? 1 : 0) << 24) | ((long long)ns << 8) |
                               (order ? 1 : 0);
         auto it = c->tuned.find(key);
-        if (it != c->tuned.end() || !c->autotune) {
+        if (it != c->tuned.end() || !c->autotune || c->capturing) {
             launch_m2l_mfma(ma, nn, it != c->tuned.end() ? it->second : m2l_mfma_default_variant(ma), s);
             continue;
         }
@@ -1136,6 +1196,9 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         c->mark("tree_reused");
     } else {
         c->cache_valid = false;
+        c->cache_work_valid = false;
+        c->cache_tiles_p = -1;
+        ++c->cache_gen;
         build_tree(c, k.X, k.ldx, n, ncrit, tr);
         c->mark("tree");
         build_lists(c, tr, theta, L, P.nimg, disp);
@@ -1145,11 +1208,37 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->num_nodes = nn; c->num_leaves = nl; c->max_depth = tr.max_depth;
     c->n_m2l = L.n_m2l; c->n_p2p = L.n_p2p; c->n_sites = n;
     c->level_off = tr.level_off;
+    const int S = k.nsets;
+    // learned per-pair orders (off: every pair at p, as the reference)
+    const bool select = c->order_tol > 0.0 && c->sel_trained && !P.on && L.n_m2l > 0;
+    // graph replay: a tree-reuse call without timing events, selection or periodic images
+    bool replay = false, capture = false;
+    std::string gkey;
+    if (c->graph_opt && reuse && !c->timing && !select && !P.on) {
+        char kb[512];
+        std::snprintf(kb, sizeof kb, "%p %lld %p %lld %d %d %d %p %lld %d %d %p %lld %d %p | %.17g %d %d | %d %d %d %zu | %lld %lld",
+                      (const void*)k.X, (long long)k.ldx, (const void*)k.M, (long long)k.ldm, k.n, k.dim, k.nsets,
+                      (const void*)k.E, (long long)k.lde, k.nexcl, k.F, (const void*)k.field, (long long)k.ldf,
+                      k.damp, (const void*)k.df, theta, ncrit, p, c->precision, c->m2l_mfma, c->autotune,
+                      c->tuned.size(), (long long)c->pool_gen, (long long)c->cache_gen);
+        gkey = kb;
+        if (c->graph_exec && c->graph_key == gkey) replay = true;
+        else if (c->graph_pending == gkey) capture = true;
+        else c->graph_pending = gkey;   // this call runs eagerly; the next one of this shape is captured
+    }
+    c->sel_stats[0] = -1.0;
+    c->sel_stats[1] = 0.0;
+    c->sel_full_err = -1.0;
+    if (replay) {
+        FMMH_HIP(hipGraphLaunch(c->graph_exec, s));
+        ++c->graph_replays;
+        c->stats_classes = c->cache_classes;   // a replay runs with the cached classes
+    } else {
+    GraphCapture cap(s, capture, &c->capturing);
     Tables& tb = get_tables(c, p, m);
     double* xs = c->buf<double>("s_xs", n);
     double* ys = c->buf<double>("s_ys", n);
     double* zs = c->buf<double>("s_zs", n);
-    const int S = k.nsets;
     double* ms_raw = c->buf<double>("s_ms_raw", (size_t)n * k.dim * S);
     double* ms = c->buf<double>("s_ms", (size_t)n * k.dim * S);
     double* dfs = c->buf<double>("s_dfs", n);
@@ -1170,21 +1259,27 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->mark("upward");
     double* near = c->buf<double>("e_near", (size_t)n * nf * S);
     int* inv = c->buf<int>("s_inv", n);
-    hipLaunchKernelGGL(k_invert_perm, dim3(nblk(n, 256)), dim3(256), 0, s, tr.t.perm, inv, n);
+    // the near-field block order (and the sorted-site inverse permutation) depend on the tree only
+    const bool work_cached = reuse && c->cache_work_valid;
+    if (!work_cached) hipLaunchKernelGGL(k_invert_perm, dim3(nblk(n, 256)), dim3(256), 0, s, tr.t.perm, inv, n);
     // leaves in order of decreasing near-field work
-    unsigned long long* wk_in = c->buf<unsigned long long>("s_work_keys_in", nl);
-    unsigned long long* wk = c->buf<unsigned long long>("s_work_keys", nl);
-    int* wv_in = c->buf<int>("s_work_vals_in", nl);
-    hipLaunchKernelGGL(k_leaf_work_keys, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
-                       tr.t.leaf_node, tr.t.count, nl, wk_in, wv_in);
-    size_t wb = sort_pairs_u64_temp(nl);
-    int* sorted_nodes = c->buf<int>("s_work_sorted", nl);
-    sort_pairs_u64(c->buf<char>("s_work_temp", wb), wb, wk_in, wk, wv_in, sorted_nodes, nl, 64, s);
     const int per = (nl + NUM_XCD - 1) / NUM_XCD + 1;   // slots per XCD range
     int* slots = c->buf<int>("s_work_blk", (size_t)per * NUM_XCD);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nblk((long long)per * NUM_XCD, 256)), dim3(256), 0, s, slots, -1,
-                       per * NUM_XCD);
-    hipLaunchKernelGGL(k_leaf_scatter, dim3(nblk(nl, 256)), dim3(256), 0, s, sorted_nodes, nl, per, slots);
+    if (!work_cached) {
+        unsigned long long* wk_in = c->buf<unsigned long long>("s_work_keys_in", nl);
+        unsigned long long* wk = c->buf<unsigned long long>("s_work_keys", nl);
+        int* wv_in = c->buf<int>("s_work_vals_in", nl);
+        hipLaunchKernelGGL(k_leaf_work_keys, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
+                           tr.t.leaf_node, tr.t.count, nl, wk_in, wv_in);
+        size_t wb = sort_pairs_u64_temp(nl);
+        int* sorted_nodes = c->buf<int>("s_work_sorted", nl);
+        sort_pairs_u64(c->buf<char>("s_work_temp", wb), wb, wk_in, wk, wv_in, sorted_nodes, nl, 64, s);
+        hipLaunchKernelGGL(k_fill_i32, dim3(nblk((long long)per * NUM_XCD, 256)), dim3(256), 0, s, slots, -1,
+                           per * NUM_XCD);
+        hipLaunchKernelGGL(k_leaf_scatter, dim3(nblk(nl, 256)), dim3(256), 0, s, sorted_nodes, nl, per, slots);
+        c->cache_work_valid = c->reuse_tree;
+        ++c->cache_gen;
+    }
     NearArgs na{slots, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
                 ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp, inv};
     c->mark("p2p_order");
@@ -1193,8 +1288,6 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     launch_near_part(na, per * NUM_XCD, m, k.F, k.damp, 2, s);
     c->mark("p2p_excl");
     const bool classes_cached = reuse && c->cache_class_p == p;
-    // learned per-pair orders (off: every pair at p, as the reference)
-    const bool select = c->order_tol > 0.0 && c->sel_trained && !P.on && L.n_m2l > 0;
     const unsigned char* order = select ? select_orders(c, tr, L, tb, p, S, CM, near, n, nf) : nullptr;
     run_m2l(c, tr, L, tb, p, S, CM, LE, classes_cached, P, disp, order);
     c->mark("m2l");
@@ -1219,9 +1312,6 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
                                   (size_t)nf * S, hipMemcpyDeviceToDevice, s));
     }
     downward();
-    c->sel_stats[0] = -1.0;
-    c->sel_stats[1] = 0.0;
-    c->sel_full_err = -1.0;
     if (check) {
         const double err = spot_check(c, tr, L, xs, ys, zs, ms, n, m, nf, near, k, before, c->spot_checks);
         c->sel_stats[0] = err;
@@ -1245,6 +1335,13 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
             }
         }
     }
+    if (capture) {
+        c->graph_exec_replace(cap.end());
+        c->graph_key = gkey;
+        c->graph_pending.clear();
+        FMMH_HIP(hipGraphLaunch(c->graph_exec, s));
+    }
+    }   // device body (not replayed)
     if (select) {
         unsigned long long* hh = reinterpret_cast<unsigned long long*>(c->h_pinned + 850);
         FMMH_HIP(hipMemcpyAsync(hh, c->buf<unsigned long long>("sel_hist", PMAX + 1),
@@ -1308,6 +1405,8 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->stats[0] = nn; c->stats[1] = nl; c->stats[2] = tr.max_depth;
     c->stats[3] = L.n_m2l; c->stats[4] = L.n_p2p; c->stats[5] = (int64_t)pairs; c->stats[6] = c->stats_classes;
     c->stats[7] = c->list_retries;
+    c->stats[8] = replay ? 2 : (capture ? 1 : 0);
+    c->stats[9] = c->graph_replays;
 }
 
 // ---------------------------------------------------------------- host <-> device staging
@@ -1439,6 +1538,7 @@ int fmmh_context_destroy(fmmh_context* ctx) {
             FMMH_HIP(hipFree(kv.second.dbuf));
         }
         for (hipEvent_t e : ctx->event_pool) FMMH_HIP(hipEventDestroy(e));
+        if (ctx->graph_exec) FMMH_HIP(hipGraphExecDestroy(ctx->graph_exec));
         FMMH_HIP(hipHostFree(ctx->h_pinned));
         FMMH_HIP(hipStreamDestroy(ctx->stream));
         delete ctx;
@@ -1876,6 +1976,7 @@ int fmmh_tree_export(fmmh_context* ctx, double* centers, double* r, double* rmax
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
+        ++c->cache_gen;   // any option change: captured graphs are not replayed again
         if (key == FMMH_OPT_TIMING) c->timing = value != 0.0;
         else if (key == FMMH_OPT_LIST_CAPACITY) c->list_cap_scale = value > 0.0 ? value : 1.0;
         else if (key == FMMH_OPT_REUSE_TREE) {
@@ -1898,6 +1999,8 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
             if (!(value > 0.0)) throw Error(FMMH_ERR_ARG, "order safety factor must be > 0");
             c->sel_kappa = value;
             c->sel_adapt = 1.0;
+        } else if (key == FMMH_OPT_GRAPH) {
+            c->graph_opt = value != 0.0 ? 1 : 0;
         } else if (key == FMMH_OPT_AUTOTUNE) {
             c->autotune = value != 0.0 ? 1 : 0;
             if (value < 0.0) c->tuned.clear();   // forget earlier choices
@@ -1945,7 +2048,7 @@ int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t b
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
-        for (int i = 0; i < std::min(cap, 8); ++i) out[i] = c->stats[i];
+        for (int i = 0; i < std::min(cap, 10); ++i) out[i] = c->stats[i];
     });
 }
 

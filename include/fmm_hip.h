@@ -271,11 +271,18 @@ enum fmmh_option {
     FMMH_OPT_ORDER_SAFETY = 9,    /* factor on the per-pair tolerance (default 1) */
     FMMH_OPT_ORDER_FORCE_FULL = 10, /* != 0: run the selection machinery but keep every pair at
                                      expansion_order (ablation) */
-    FMMH_OPT_AUTOTUNE = 11        /* 1 (default): the first call of each M2L shape for many sets (order,
+    FMMH_OPT_AUTOTUNE = 11,       /* 1 (default): the first call of each M2L shape for many sets (order,
                                      precision, sets per pass, learned orders) times every tile-shape
                                      variant of that kernel on its own pairs and keeps the fastest for the
                                      context's later calls (results do not depend on the choice).  0: fixed
                                      defaults.  -1: forget the choices made so far and keep tuning on. */
+    FMMH_OPT_GRAPH = 12           /* != 0: tree-reuse calls (FMMH_OPT_REUSE_TREE) without timing, learned
+                                     orders or periodic images run their device work as a HIP graph: the
+                                     first call of a shape runs eagerly, the next one captures and launches
+                                     the graph, later ones replay it.  A shape is every pointer, size and
+                                     option of the call; any reallocation of the context's buffers or
+                                     rebuild of its cached tree data starts over.  Results are identical
+                                     to eager calls.  0 (default): eager. */
 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
@@ -287,7 +294,8 @@ int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, in
 int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t bytes);
 
 /* Counters of the last field_fmm call: {num_nodes, num_leaves, max_depth,
- * n_m2l, n_p2p, p2p_site_pairs, m2l_classes, list_retries}. */
+ * n_m2l, n_p2p, p2p_site_pairs, m2l_classes, list_retries, graph (0: eager,
+ * 1: captured and launched, 2: replayed; FMMH_OPT_GRAPH), graph replays so far}. */
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap);
 
 /* ---- learned per-pair expansion orders (no counterpart in the reference) ------
