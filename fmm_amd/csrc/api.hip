@@ -76,7 +76,7 @@ struct fmmh_context {
     std::vector<hipEvent_t> event_pool;
     size_t event_used = 0;
     std::vector<std::pair<std::string, double>> phase_ms;
-    int64_t stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
     int m2l_mfma = 1;                 // FMMH_OPT_M2L_MFMA: 0 VALU, 1 auto, 2 per-target MFMA, 3 class-grouped
@@ -98,6 +98,8 @@ struct fmmh_context {
     ListsOut cache_L;
     int cache_class_p = -1;           // expansion order of the cached class tensors (-1: none)
     bool cache_work_valid = false;    // near-field block order of the cached tree ("s_work_blk", "s_inv")
+    int near_nv = 0, near_grid = 0, near_np = 0;   // near-field chunks, workgroup slots, partial sites
+    long long near_chunk_cap = 0;     // FMMH_OPT_NEAR_CHUNK (0: automatic)
     int cache_tiles_p = -1;           // class-grouped GEMM tile tables of the cached classes (-1: none)
     // Graph replay (FMMH_OPT_GRAPH): a tree-reuse call runs as a captured hipGraph of its device work
     // once a previous call of the same shape ran eagerly (tuning done, buffers allocated).  The shape key
@@ -442,43 +444,173 @@ __global__ void k_pair_count(const int* off, const int* src, const int* leaf_nod
 
 // Order of the near-field workgroups.  Workgroup b runs on XCD b % 8 (the
 // dispatcher deals workgroups round robin), and each XCD has its own L2: the
-// leaves are cut into 8 contiguous Morton ranges, range x goes to the slots
-// b = 8 r + x, so an XCD's L2 holds one region's sources; within a range the
-// heaviest leaves (source tile passes of the wavefront, as k_near walks them)
-// come first, because the hardware starts workgroups in index order and the
-// long blocks should not finish last on an idle GPU.
+// leaves are cut into 8 contiguous Morton ranges of about equal pair work,
+// range x goes to the slots b = 8 r + x, so an XCD's L2 holds one region's
+// sources.  A leaf whose work exceeds a cap (the total over NEAR_CHUNK_DIV,
+// at least NEAR_CHUNK_MIN source-tile iterations) is cut into chunks of
+// consecutive list entries with about equal source counts; each chunk is its
+// own workgroup and writes a partial field, which k_near_fix sums in chunk
+// order (deterministic).  Within a range the heaviest chunks come first (a
+// stable sort on the range and the chunk's work in 1/4096 of the cap),
+// because the hardware starts workgroups in index order and long workgroups
+// should not finish last on an idle GPU.
 constexpr int NUM_XCD = 8;
+constexpr int NEAR_CHUNK_DIV = 16384;
+constexpr long long NEAR_CHUNK_MIN = 256;
+constexpr int NEAR_CHUNK_MAX = 64;   // chunks per leaf
 
-__global__ void k_leaf_work_keys(const int* __restrict__ off, const int* __restrict__ src,
-                                 const int* __restrict__ leaf_node, const int* __restrict__ count, int nl,
-                                 unsigned long long* __restrict__ keys, int* __restrict__ vals) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
+// Work of one leaf in source-tile iterations of k_near's wavefront: the same pass sizing as the kernel
+// (c passes of ct targets, r = 64 / ct lane groups), plus a fixed cost per pass.
+__device__ long long leaf_work(int tc, long long total) {
+    int ct = min(tc, 64), best = 1 << 30;
+    for (int c = (tc + 63) / 64; c <= 16; ++c) {
+        const int t = (tc + c - 1) / c;
+        const int rr = 64 / t;
+        const int cost = c * ((64 + rr - 1) / rr + 2);
+        if (cost < best) {
+            best = cost;
+            ct = t;
+        }
+    }
+    long long w = 0;
+    for (int b = 0; b < tc; b += ct) {
+        const int r = 64 / min(ct, tc - b);
+        w += (total + r - 1) / r + 8;
+    }
+    return w;
+}
+
+__global__ void k_leaf_work(const int* __restrict__ off, const int* __restrict__ src,
+                            const int* __restrict__ leaf_node, const int* __restrict__ count, int nl,
+                            long long* __restrict__ work) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nl) return;
     const int node = leaf_node[l];
     long long total = 0;
     for (int e = off[node]; e < off[node + 1]; ++e) total += count[src_node(src[e])];
-    const int c = count[node];
-    long long work = 0;
-    for (int b = 0; b < c; b += 64) {
-        const int nt = min(64, c - b), r = 64 / nt;
-        work += (total + r - 1) / r;
-    }
-    const unsigned long long w = (unsigned long long)min(work, 0xffffffffll);
-    const unsigned long long x = (unsigned long long)(((long long)l * NUM_XCD) / nl);   // Morton range
-    keys[l] = (x << 56) | ((0xffffffffull - w) << 24) | (unsigned long long)(l & 0xffffff);
-    vals[l] = node;
+    work[l] = leaf_work(count[node], total);
 }
 
+// Exclusive prefix sums of n int64 values in one workgroup of 1024 threads (pre[n] = total); each
+// thread sums a contiguous run, so the integer result does not depend on scheduling.
+__global__ __launch_bounds__(1024) void k_scan_i64(const long long* __restrict__ in, long long* __restrict__ pre,
+                                                   int n) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int a = min(n, t * per), b = min(n, a + per);
+    long long s = 0;
+    for (int i = a; i < b; ++i) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        long long run = 0;
+        for (int i = 0; i < 1024; ++i) {
+            const long long v = part[i];
+            part[i] = run;
+            run += v;
+        }
+        pre[n] = run;
+    }
+    __syncthreads();
+    long long run = part[t];
+    for (int i = a; i < b; ++i) {
+        pre[i] = run;
+        run += in[i];
+    }
+}
+
+// Per leaf: chunk count K (1 = whole list), XCD range, partial sites (K tc when chunked; with one
+// extra slot for the totals at index nl so that exclusive scans over nl + 1 entries give them).
+__global__ void k_leaf_chunks(const long long* __restrict__ work, const long long* __restrict__ pre,
+                              const int* __restrict__ off, const int* __restrict__ leaf_node,
+                              const int* __restrict__ count, int nl, long long cap_set, int* __restrict__ lk,
+                              int* __restrict__ lx, int* __restrict__ lps, int* __restrict__ range_cnt) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > nl) return;
+    if (l == nl) {
+        lk[nl] = 0;
+        lps[nl] = 0;
+        return;
+    }
+    const long long total = pre[nl];
+    const long long cap = cap_set > 0 ? cap_set : max(total / NEAR_CHUNK_DIV, NEAR_CHUNK_MIN);
+    const int node = leaf_node[l];
+    const int entries = off[node + 1] - off[node];
+    const long long w = work[l];
+    int K = (int)min((w + cap - 1) / cap, (long long)NEAR_CHUNK_MAX);
+    K = max(1, min(K, entries));
+    // range by the work midpoint of the leaf: monotone in l, about equal work per range
+    const int x = total > 0 ? (int)min((long long)(NUM_XCD - 1), ((pre[l] + w / 2) * NUM_XCD) / total) : 0;
+    lk[l] = K;
+    lx[l] = x;
+    lps[l] = K > 1 ? K * count[node] : 0;
+    atomicAdd(&range_cnt[x], K);
+}
+
+// Per leaf: its chunks (consecutive list entries, boundaries at about equal cumulative source counts,
+// at least one entry each) and their sort keys: range, then decreasing work, then chunk id.
+__global__ void k_leaf_vblocks(const long long* __restrict__ work, const long long* __restrict__ pre,
+                               long long cap_set, const int* __restrict__ off,
+                               const int* __restrict__ src, const int* __restrict__ leaf_node,
+                               const int* __restrict__ count, int nl, const int* __restrict__ lk,
+                               const int* __restrict__ lx, const int* __restrict__ lvb, const int* __restrict__ lpart,
+                               int* __restrict__ vb_node, int* __restrict__ vb_e0, int* __restrict__ vb_e1,
+                               int* __restrict__ vb_part, unsigned long long* __restrict__ keys,
+                               int* __restrict__ vals) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nl) return;
+    const int node = leaf_node[l];
+    const int e0 = off[node], e1 = off[node + 1];
+    const int K = lk[l];
+    const int v0 = lvb[l];
+    const unsigned long long x = (unsigned long long)lx[l];
+    const long long cap = cap_set > 0 ? cap_set : max(pre[nl] / NEAR_CHUNK_DIV, NEAR_CHUNK_MIN);
+    long long total = 0;
+    for (int e = e0; e < e1; ++e) total += count[src_node(src[e])];
+    int e = e0;
+    long long cum = 0;
+    for (int k = 0; k < K; ++k) {
+        const int a = e;
+        long long csrc = 0;
+        if (k == K - 1) {
+            e = e1;
+            csrc = total - cum;
+        } else {
+            // close the chunk once the cumulative count reaches (k + 1) total / K, leaving at least
+            // one entry for each later chunk
+            const long long goal = ((long long)(k + 1) * total) / K;
+            do {
+                const long long c = count[src_node(src[e])];
+                cum += c;
+                csrc += c;
+                ++e;
+            } while (cum < goal && e < e1 - (K - 1 - k));
+        }
+        const int v = v0 + k;
+        vb_node[v] = node;
+        vb_e0[v] = a;
+        vb_e1[v] = e;
+        vb_part[v] = K > 1 ? lpart[l] + k * count[node] : -1;
+        const long long cw = total > 0 ? (work[l] * csrc) / total : work[l] / K;
+        // work in 1/4096 of the cap, 16 bits; ties keep the chunk id order (stable sort)
+        const unsigned long long w = (unsigned long long)min((cw << 12) / cap, 0xffffll);
+        keys[v] = (x << 16) | (0xffffull - w);
+        vals[v] = v;
+    }
+}
+
+struct RangeStarts {
+    int first[NUM_XCD + 1];
+};
+
 // sorted position k (range x, rank r within it) -> workgroup slot 8 r + x; unused slots hold -1
-__global__ void k_leaf_scatter(const int* __restrict__ sorted_nodes, int nl, int per, int* __restrict__ blk) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nl) return;
-    // range x holds leaves [ceil(x nl / 8), ceil((x + 1) nl / 8)) of the Morton order
-    int x = (int)(((long long)k * NUM_XCD) / nl);
-    while (x + 1 < NUM_XCD && ((long long)(x + 1) * nl + NUM_XCD - 1) / NUM_XCD <= k) ++x;
-    while (x > 0 && ((long long)x * nl + NUM_XCD - 1) / NUM_XCD > k) --x;
-    const int first = (int)(((long long)x * nl + NUM_XCD - 1) / NUM_XCD);
-    blk[(k - first) * NUM_XCD + x] = sorted_nodes[k];
+__global__ void k_vb_scatter(const unsigned long long* __restrict__ keys, const int* __restrict__ sorted, int nv,
+                             RangeStarts rs, int* __restrict__ blk) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nv) return;
+    const int x = (int)(keys[k] >> 16);
+    blk[(k - rs.first[x]) * NUM_XCD + x] = sorted[k];
 }
 
 __global__ void k_invert_perm(const int* __restrict__ perm, int* __restrict__ inv, int n) {
@@ -833,78 +965,95 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int ni
 // Displacement classes (sort + unique), one tensor per class, then the
 // target-centric Hankel contraction of every set.  One host sync to size the
 // class table.  Many sets go through the MFMA form in chunks of up to 32 sets.
+// M2L displacement classes of the pair list (sort + unique), in two halves around one host round trip
+// for the class count: m2l_classes_begin launches the key, sort, flag, scan and assign kernels and the
+// count's readback (false when there are no pairs); after a stream sync m2l_classes_end sizes and fills
+// the class tensors.  The count is left in cache_classes and stats_classes.
+bool m2l_classes_begin(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, const Periodic& P,
+                       const double* disp, M2LClassArgs& ca) {
+    hipStream_t s = c->stream;
+    const int np = L.n_m2l;
+    if (np <= 0) return false;
+    int* pair_class = c->buf<int>("m_pair_class", np);
+    int* dev_scalars = c->buf<int>("m_scalars", 4);   // num_classes, error
+    FMMH_HIP(hipMemsetAsync(dev_scalars, 0, 4 * sizeof(int), s));
+    ca = M2LClassArgs{};
+    ca.pair_keys = L.m2l_keys;
+    ca.n_pairs = np;
+    ca.cx = tr.t.cx; ca.cy = tr.t.cy; ca.cz = tr.t.cz; ca.r = tr.t.r; ca.level = tr.t.level;
+    ca.class_key_in = c->buf<unsigned long long>("m_key_in", np);
+    ca.pair_idx = c->buf<int>("m_idx_in", np);
+    ca.class_key_sorted = c->buf<unsigned long long>("m_key_sorted", np);
+    ca.pair_idx_sorted = c->buf<int>("m_idx_sorted", np);
+    ca.flag = c->buf<int>("m_flag", np);
+    ca.cls = c->buf<int>("m_cls", np);
+    ca.pair_class = pair_class;
+    ca.class_key = c->buf<unsigned long long>("m_class_key", np);
+    ca.num_classes = dev_scalars;
+    ca.error = dev_scalars + 1;
+    ca.p = p;
+    ca.xyz = tb.xyz;
+    ca.level_base = tb.level_base;
+    ca.class_dl = c->buf<int>("m_class_dl", np);
+    ca.periodic = P.on ? 1 : 0;
+    ca.disp = disp;
+    if (P.on) {
+        ca.class_hi_in = c->buf<unsigned long long>("m_hi_in", np);
+        ca.class_hi_sorted = c->buf<unsigned long long>("m_hi_sorted", np);
+        ca.class_lo_final = c->buf<unsigned long long>("m_lo_final", np);
+        ca.class_key_hi = c->buf<unsigned long long>("m_class_key_hi", np);
+    }
+    c->cache_tiles_p = -1;
+    ++c->cache_gen;
+    launch_m2l_keys(ca, s);
+    size_t tbytes = std::max(sort_pairs_u64_temp(np), exscan_i32_temp(np));
+    void* temp = c->buf<char>("m_temp", tbytes);
+    if (!P.on) {
+        sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, ca.pair_idx_sorted, np, 62,
+                       s);
+    } else {
+        // (hi, lo) lexicographic order by two stable radix sorts: lo first, then hi
+        int* idx1 = c->buf<int>("m_idx1", np);
+        unsigned long long* hi1 = c->buf<unsigned long long>("m_hi1", np);
+        sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, idx1, np, 63, s);
+        launch_gather_u64(ca.class_hi_in, idx1, hi1, np, s);
+        sort_pairs_u64(temp, tbytes, hi1, ca.class_hi_sorted, idx1, ca.pair_idx_sorted, np, 21, s);
+        launch_gather_u64(ca.class_key_in, ca.pair_idx_sorted, ca.class_lo_final, np, s);
+    }
+    launch_m2l_flags(ca, s);
+    exscan_i32(temp, tbytes, ca.flag, ca.cls, np, s);
+    launch_m2l_assign(ca, s);
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 300, dev_scalars, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    return true;
+}
+
+void m2l_classes_end(fmmh_context* c, int p, bool pending, M2LClassArgs& ca) {
+    int ncls = 0;
+    if (pending) {
+        if (c->h_pinned[301]) throw Error(FMMH_ERR_UNSUPPORTED, "M2L displacement outside the class key range");
+        ncls = c->h_pinned[300];
+        ca.Tc = c->buf<double>("m_tc", (size_t)ncls * ncomp(2 * p));
+        launch_m2l_class_tensors(ca, std::min(ncls, 4096), c->stream);
+    }
+    c->stats_classes = ncls;
+    c->cache_classes = ncls;
+}
+
 void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets, const double* CM, double* LE,
              bool classes_cached, const Periodic& P, const double* disp, const unsigned char* order = nullptr) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes, np = L.n_m2l;
     const int nm = ncomp(p), nT = ncomp(2 * p);
     int* pair_class = c->buf<int>("m_pair_class", np);
-    double* Tc = nullptr;
-    int ncls = 0;
-    if (classes_cached) {
-        ncls = (int)c->cache_classes;
-        Tc = np > 0 ? c->buf<double>("m_tc", (size_t)ncls * nT) : nullptr;
-        c->stats_classes = ncls;
-    } else if (np > 0) {
-        int* dev_scalars = c->buf<int>("m_scalars", 4);   // num_classes, error
-        FMMH_HIP(hipMemsetAsync(dev_scalars, 0, 4 * sizeof(int), s));
+    if (!classes_cached) {
         M2LClassArgs ca{};
-        ca.pair_keys = L.m2l_keys;
-        ca.n_pairs = np;
-        ca.cx = tr.t.cx; ca.cy = tr.t.cy; ca.cz = tr.t.cz; ca.r = tr.t.r; ca.level = tr.t.level;
-        ca.class_key_in = c->buf<unsigned long long>("m_key_in", np);
-        ca.pair_idx = c->buf<int>("m_idx_in", np);
-        ca.class_key_sorted = c->buf<unsigned long long>("m_key_sorted", np);
-        ca.pair_idx_sorted = c->buf<int>("m_idx_sorted", np);
-        ca.flag = c->buf<int>("m_flag", np);
-        ca.cls = c->buf<int>("m_cls", np);
-        ca.pair_class = pair_class;
-        ca.class_key = c->buf<unsigned long long>("m_class_key", np);
-        ca.num_classes = dev_scalars;
-        ca.error = dev_scalars + 1;
-        ca.p = p;
-        ca.xyz = tb.xyz;
-        ca.level_base = tb.level_base;
-        ca.class_dl = c->buf<int>("m_class_dl", np);
-        ca.periodic = P.on ? 1 : 0;
-        ca.disp = disp;
-        if (P.on) {
-            ca.class_hi_in = c->buf<unsigned long long>("m_hi_in", np);
-            ca.class_hi_sorted = c->buf<unsigned long long>("m_hi_sorted", np);
-            ca.class_lo_final = c->buf<unsigned long long>("m_lo_final", np);
-            ca.class_key_hi = c->buf<unsigned long long>("m_class_key_hi", np);
-        }
-        c->cache_tiles_p = -1;
-        ++c->cache_gen;
-        launch_m2l_keys(ca, s);
-        size_t tbytes = std::max(sort_pairs_u64_temp(np), exscan_i32_temp(np));
-        void* temp = c->buf<char>("m_temp", tbytes);
-        if (!P.on) {
-            sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, ca.pair_idx_sorted, np, 62,
-                           s);
-        } else {
-            // (hi, lo) lexicographic order by two stable radix sorts: lo first, then hi
-            int* idx1 = c->buf<int>("m_idx1", np);
-            unsigned long long* hi1 = c->buf<unsigned long long>("m_hi1", np);
-            sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, idx1, np, 63, s);
-            launch_gather_u64(ca.class_hi_in, idx1, hi1, np, s);
-            sort_pairs_u64(temp, tbytes, hi1, ca.class_hi_sorted, idx1, ca.pair_idx_sorted, np, 21, s);
-            launch_gather_u64(ca.class_key_in, ca.pair_idx_sorted, ca.class_lo_final, np, s);
-        }
-        launch_m2l_flags(ca, s);
-        exscan_i32(temp, tbytes, ca.flag, ca.cls, np, s);
-        launch_m2l_assign(ca, s);
-        FMMH_HIP(hipMemcpyAsync(c->h_pinned + 300, dev_scalars, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        c->sync();
-        if (c->h_pinned[301]) throw Error(FMMH_ERR_UNSUPPORTED, "M2L displacement outside the class key range");
-        ncls = c->h_pinned[300];
-        c->stats_classes = ncls;
-        Tc = c->buf<double>("m_tc", (size_t)ncls * nT);
-        ca.Tc = Tc;
-        launch_m2l_class_tensors(ca, std::min(ncls, 4096), s);
-    } else {
-        c->stats_classes = 0;
+        const bool pending = m2l_classes_begin(c, tr, L, tb, p, P, disp, ca);
+        if (pending) c->sync();
+        m2l_classes_end(c, p, pending, ca);
     }
+    const int ncls = (int)c->cache_classes;
+    double* Tc = np > 0 ? c->buf<double>("m_tc", (size_t)ncls * nT) : nullptr;
+    c->stats_classes = ncls;
     c->mark("m2l_classes");
     // auto: the per-target MFMA form from M2L_MFMA_MIN_SETS sets on (a 16-set tile, padded, costs about
     // as much as 6 passes of the class-grouped GEMM at p 10), or whenever the GEMM's slot buffer does not fit
@@ -1179,6 +1328,77 @@ double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, c
     return h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : 0.0;
 }
 
+// Near-field workgroup order (leaves, and chunks of heavy leaves, in the XCD-aware order), in two
+// halves around one host round trip: near_order_begin launches the per-leaf work, its prefix sums, the
+// chunk counts and the readback of the sizes; after a stream sync near_order_end lays out the chunks,
+// sorts them and fills the workgroup slots ("s_work_blk", near_grid of them).
+void near_order_begin(fmmh_context* c, TreeOut& tr, ListsOut& L) {
+    hipStream_t s = c->stream;
+    const int nl = tr.num_leaves;
+    long long* work = c->buf<long long>("s_work", nl);
+    long long* pre = c->buf<long long>("s_work_pre", nl + 1);
+    int* lk = c->buf<int>("s_leaf_k", nl + 1);
+    int* lx = c->buf<int>("s_leaf_x", nl);
+    int* lps = c->buf<int>("s_leaf_ps", nl + 1);
+    int* lvb = c->buf<int>("s_leaf_vb", nl + 1);
+    int* lpart = c->buf<int>("s_leaf_part", nl + 1);
+    int* range_cnt = c->buf<int>("s_range_cnt", NUM_XCD);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, range_cnt, 0, NUM_XCD);
+    hipLaunchKernelGGL(k_leaf_work, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
+                       tr.t.leaf_node, tr.t.count, nl, work);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, s, work, pre, nl);
+    hipLaunchKernelGGL(k_leaf_chunks, dim3(nblk(nl + 1, 256)), dim3(256), 0, s, work, pre, L.p2p_off,
+                       tr.t.leaf_node, tr.t.count, nl, c->near_chunk_cap, lk, lx, lps, range_cnt);
+    size_t eb = exscan_i32_temp(nl + 1);
+    void* etmp = c->buf<char>("s_work_temp", eb);
+    exscan_i32(etmp, eb, lk, lvb, nl + 1, s);
+    exscan_i32(etmp, eb, lps, lpart, nl + 1, s);
+    int* hp = c->h_pinned + 900;
+    FMMH_HIP(hipMemcpyAsync(hp, range_cnt, NUM_XCD * sizeof(int), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(hp + NUM_XCD, lvb + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(hp + NUM_XCD + 1, lpart + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+}
+
+void near_order_end(fmmh_context* c, TreeOut& tr, ListsOut& L) {
+    hipStream_t s = c->stream;
+    const int nl = tr.num_leaves;
+    long long* work = c->buf<long long>("s_work", nl);
+    long long* pre = c->buf<long long>("s_work_pre", nl + 1);
+    int* lk = c->buf<int>("s_leaf_k", nl + 1);
+    int* lx = c->buf<int>("s_leaf_x", nl);
+    int* lvb = c->buf<int>("s_leaf_vb", nl + 1);
+    int* lpart = c->buf<int>("s_leaf_part", nl + 1);
+    int* hp = c->h_pinned + 900;
+    const int nv = hp[NUM_XCD];
+    if (nv <= 0 || nv > 0xffffff) throw Error(FMMH_ERR_UNSUPPORTED, "near-field workgroup count out of range");
+    RangeStarts rs;
+    int per = 1;
+    rs.first[0] = 0;
+    for (int x = 0; x < NUM_XCD; ++x) {
+        rs.first[x + 1] = rs.first[x] + hp[x];
+        per = std::max(per, hp[x]);
+    }
+    c->near_nv = nv;
+    c->near_grid = per * NUM_XCD;
+    c->near_np = hp[NUM_XCD + 1];
+    int* vb_node = c->buf<int>("s_vb_node", nv);
+    int* vb_e0 = c->buf<int>("s_vb_e0", nv);
+    int* vb_e1 = c->buf<int>("s_vb_e1", nv);
+    int* vb_part = c->buf<int>("s_vb_part", nv);
+    unsigned long long* vk_in = c->buf<unsigned long long>("s_vb_keys_in", nv);
+    unsigned long long* vk = c->buf<unsigned long long>("s_vb_keys", nv);
+    int* vv_in = c->buf<int>("s_vb_vals_in", nv);
+    int* vv = c->buf<int>("s_vb_sorted", nv);
+    hipLaunchKernelGGL(k_leaf_vblocks, dim3(nblk(nl, 256)), dim3(256), 0, s, work, pre, c->near_chunk_cap,
+                       L.p2p_off, L.p2p_src, tr.t.leaf_node, tr.t.count, nl, lk, lx, lvb, lpart, vb_node, vb_e0,
+                       vb_e1, vb_part, vk_in, vv_in);
+    size_t wb = sort_pairs_u64_temp(nv);
+    sort_pairs_u64(c->buf<char>("s_vb_temp", wb), wb, vk_in, vk, vv_in, vv, nv, 19, s);   // 3 + 16 key bits
+    int* slots = c->buf<int>("s_work_blk", (size_t)c->near_grid);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(c->near_grid, 256)), dim3(256), 0, s, slots, -1, c->near_grid);
+    hipLaunchKernelGGL(k_vb_scatter, dim3(nblk(nv, 256)), dim3(256), 0, s, vk, vv, nv, rs, slots);
+}
+
 void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, const Periodic& P) {
     hipStream_t s = c->stream;
     const double* disp = upload_disp(c, P);
@@ -1236,6 +1456,21 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     } else {
     GraphCapture cap(s, capture, &c->capturing);
     Tables& tb = get_tables(c, p, m);
+    // tree-dependent preparation, kept with a cached tree: the near-field workgroup order and the M2L
+    // displacement classes, sized through one host round trip
+    const bool work_cached = reuse && c->cache_work_valid;
+    const bool classes_cached = reuse && c->cache_class_p == p;
+    if (!work_cached) near_order_begin(c, tr, L);
+    M2LClassArgs ca{};
+    const bool cls_pending = !classes_cached && m2l_classes_begin(c, tr, L, tb, p, P, disp, ca);
+    if (!work_cached || cls_pending) c->sync();
+    if (!work_cached) {
+        near_order_end(c, tr, L);
+        c->cache_work_valid = c->reuse_tree;
+        ++c->cache_gen;
+    }
+    if (!classes_cached) m2l_classes_end(c, p, cls_pending, ca);
+    c->mark("prep");
     double* xs = c->buf<double>("s_xs", n);
     double* ys = c->buf<double>("s_ys", n);
     double* zs = c->buf<double>("s_zs", n);
@@ -1259,37 +1494,25 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->mark("upward");
     double* near = c->buf<double>("e_near", (size_t)n * nf * S);
     int* inv = c->buf<int>("s_inv", n);
-    // the near-field block order (and the sorted-site inverse permutation) depend on the tree only
-    const bool work_cached = reuse && c->cache_work_valid;
+    // the sorted-site inverse permutation depends on the tree only
     if (!work_cached) hipLaunchKernelGGL(k_invert_perm, dim3(nblk(n, 256)), dim3(256), 0, s, tr.t.perm, inv, n);
-    // leaves in order of decreasing near-field work
-    const int per = (nl + NUM_XCD - 1) / NUM_XCD + 1;   // slots per XCD range
-    int* slots = c->buf<int>("s_work_blk", (size_t)per * NUM_XCD);
-    if (!work_cached) {
-        unsigned long long* wk_in = c->buf<unsigned long long>("s_work_keys_in", nl);
-        unsigned long long* wk = c->buf<unsigned long long>("s_work_keys", nl);
-        int* wv_in = c->buf<int>("s_work_vals_in", nl);
-        hipLaunchKernelGGL(k_leaf_work_keys, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
-                           tr.t.leaf_node, tr.t.count, nl, wk_in, wv_in);
-        size_t wb = sort_pairs_u64_temp(nl);
-        int* sorted_nodes = c->buf<int>("s_work_sorted", nl);
-        sort_pairs_u64(c->buf<char>("s_work_temp", wb), wb, wk_in, wk, wv_in, sorted_nodes, nl, 64, s);
-        hipLaunchKernelGGL(k_fill_i32, dim3(nblk((long long)per * NUM_XCD, 256)), dim3(256), 0, s, slots, -1,
-                           per * NUM_XCD);
-        hipLaunchKernelGGL(k_leaf_scatter, dim3(nblk(nl, 256)), dim3(256), 0, s, sorted_nodes, nl, per, slots);
-        c->cache_work_valid = c->reuse_tree;
-        ++c->cache_gen;
-    }
-    NearArgs na{slots, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start, tr.t.count, xs, ys, zs,
-                ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp, inv};
+    const int nv = c->near_nv, ngrid = c->near_grid;
+    const int np_sites = c->near_np;
+    NearArgs na{c->buf<int>("s_work_blk", (size_t)ngrid), tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, tr.t.start,
+                tr.t.count, xs, ys, zs, ms, n, dfs, tr.t.perm, excl, n, k.nexcl, near, n, 0, S, disp, inv,
+                c->buf<int>("s_vb_node", nv), c->buf<int>("s_vb_e0", nv), c->buf<int>("s_vb_e1", nv),
+                c->buf<int>("s_vb_part", nv),
+                np_sites > 0 ? c->buf<double>("s_near_part", (size_t)np_sites * nf * S) : nullptr, np_sites};
     c->mark("p2p_order");
-    launch_near_part(na, per * NUM_XCD, m, k.F, k.damp, 1, s);
+    launch_near_part(na, ngrid, m, k.F, k.damp, 1, s);
+    if (np_sites > 0)
+        launch_near_fix(na, tr.t.leaf_node, c->buf<int>("s_leaf_k", nl + 1), c->buf<int>("s_leaf_part", nl + 1),
+                        nl, nf, s);
     c->mark("p2p");
-    launch_near_part(na, per * NUM_XCD, m, k.F, k.damp, 2, s);
+    launch_near_part(na, ngrid, m, k.F, k.damp, 2, s);
     c->mark("p2p_excl");
-    const bool classes_cached = reuse && c->cache_class_p == p;
     const unsigned char* order = select ? select_orders(c, tr, L, tb, p, S, CM, near, n, nf) : nullptr;
-    run_m2l(c, tr, L, tb, p, S, CM, LE, classes_cached, P, disp, order);
+    run_m2l(c, tr, L, tb, p, S, CM, LE, true, P, disp, order);
     c->mark("m2l");
     if (P.on) run_periodic(c, tr, tb, P, k, p, CM, LE);
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
@@ -1407,6 +1630,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->stats[7] = c->list_retries;
     c->stats[8] = replay ? 2 : (capture ? 1 : 0);
     c->stats[9] = c->graph_replays;
+    c->stats[10] = c->near_nv;
 }
 
 // ---------------------------------------------------------------- host <-> device staging
@@ -1999,6 +2223,10 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
             if (!(value > 0.0)) throw Error(FMMH_ERR_ARG, "order safety factor must be > 0");
             c->sel_kappa = value;
             c->sel_adapt = 1.0;
+        } else if (key == FMMH_OPT_NEAR_CHUNK) {
+            if (!(value >= 0.0 && value < 1e15)) throw Error(FMMH_ERR_ARG, "near chunk cap must be >= 0");
+            c->near_chunk_cap = (long long)value;
+            c->cache_work_valid = false;   // the cached block order follows the cap
         } else if (key == FMMH_OPT_GRAPH) {
             c->graph_opt = value != 0.0 ? 1 : 0;
         } else if (key == FMMH_OPT_AUTOTUNE) {
@@ -2048,7 +2276,7 @@ int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t b
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
-        for (int i = 0; i < std::min(cap, 10); ++i) out[i] = c->stats[i];
+        for (int i = 0; i < std::min(cap, 11); ++i) out[i] = c->stats[i];
     });
 }
 

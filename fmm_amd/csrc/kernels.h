@@ -32,12 +32,26 @@ struct NearArgs {
     int nsets;                 // multipole sets: ms is [set][component][ldm], out is [set][component][ldo]
     const double* disp;        // per image id: source displacement (x, y, z); image 0 = (0, 0, 0)
     const int* inv;            // original (0-based) site index -> sorted position (null: identity)
+    // Chunked target blocks (null vb_node: blk holds target block ids, each with its whole list).
+    // blk then holds chunk ids; chunk v covers list entries [vb_e0[v], vb_e1[v]) of block vb_node[v]
+    // and writes its partial field to part at sites vb_part[v] + i (vb_part -1: the block's only
+    // chunk, which writes out directly).
+    const int* vb_node = nullptr;
+    const int* vb_e0 = nullptr;
+    const int* vb_e1 = nullptr;
+    const int* vb_part = nullptr;
+    double* part = nullptr;    // [set][component][ldp]
+    int64_t ldp = 0;
 };
 
 bool near_supported(int m, int F);
 void launch_near(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s);
 // part 1: the pair kernel only; part 2: the exclusion correction only
 void launch_near_part(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
+// Sum of the chunks' partial fields of every chunked leaf, in chunk order, into out (after part 1).
+// lk[l]: chunks of leaf l (1: not chunked), lpart[l]: its first partial site.
+void launch_near_fix(const NearArgs& a, const int* leaf_node, const int* lk, const int* lpart, int nl, int nf,
+                     hipStream_t s);
 
 // ---------------------------------------------------------------- tree
 struct TreeDev {

@@ -74,8 +74,19 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
     __shared__ int wstart[NEAR_WIN];              // window: first sorted site of each entry
     __shared__ int wimg[NEAR_WIN];                // window: periodic image id of each entry
     __shared__ double etab[DAMP != DAMP_NONE ? 32 : 1];   // 2^(j/32) for exp_table
-    const int b = a.blk[blockIdx.x];
-    if (b < 0) return;                         // an empty slot of the block order
+    const int vb = a.blk[blockIdx.x];
+    if (vb < 0) return;                        // an empty slot of the block order
+    int b, e0, e1, pbase = -1;                 // target block, its list entries, partial sites
+    if (a.vb_node) {
+        b = a.vb_node[vb];
+        e0 = a.vb_e0[vb];
+        e1 = a.vb_e1[vb];
+        pbase = a.vb_part[vb];
+    } else {
+        b = vb;
+        e0 = a.list_off[b];
+        e1 = a.list_off[b + 1];
+    }
     if constexpr (DAMP != DAMP_NONE) {
         if (threadIdx.x < 32) etab[threadIdx.x] = kExp2By32[threadIdx.x];   // read after the window barriers
     }
@@ -84,8 +95,6 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
     const int lane = threadIdx.x;
     const int t0 = a.tgt_start[b];
     const int tc = a.tgt_count[b];
-    const int e0 = a.list_off[b];
-    const int e1 = a.list_off[b + 1];
     const double* __restrict__ ms = a.ms + (int64_t)set0 * NM * a.ldm;
     double* __restrict__ out = a.out + (int64_t)set0 * NF * a.ldo;
     // Targets per pass: c passes of ct = ceil(tc / c) targets, each replicated r = 64 / ct
@@ -246,9 +255,13 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
                 for (int t = 0; t < NF; ++t) {
                     double v = red[t][lane];
                     for (int q = 1; q < r; ++q) v += red[t][lane + q * nt];
-                    double* o = out + ((int64_t)u * NF + t) * a.ldo + kk;
-                    if (a.accumulate) *o += v;
-                    else *o = v;
+                    if (pbase >= 0) {
+                        a.part[((int64_t)(set0 + u) * NF + t) * a.ldp + pbase + base + lane] = v;
+                    } else {
+                        double* o = out + ((int64_t)u * NF + t) * a.ldo + kk;
+                        if (a.accumulate) *o += v;
+                        else *o = v;
+                    }
                 }
             }
             __syncthreads();
@@ -265,8 +278,13 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
     __shared__ double etab[DAMP != DAMP_NONE ? 32 : 1];   // 2^(j/32) for exp_table
-    const int b = a.blk[blockIdx.x];
-    if (b < 0) return;
+    const int vb = a.blk[blockIdx.x];
+    if (vb < 0) return;
+    int b = vb;
+    if (a.vb_node) {
+        b = a.vb_node[vb];
+        if (a.vb_e0[vb] != a.list_off[b]) return;   // a leaf's corrections run once, with its first chunk
+    }
     if constexpr (DAMP != DAMP_NONE) {
         if (threadIdx.x < 32) etab[threadIdx.x] = kExp2By32[threadIdx.x];
         __syncthreads();
@@ -382,6 +400,35 @@ void launch_near_part(const NearArgs& a, int nblocks, int m, int F, int damp, in
         case 1: launch_f<1>(a, nblocks, F, damp, part, s); break;
         default: launch_f<2>(a, nblocks, F, damp, part, s); break;
     }
+    FMMH_HIP(hipGetLastError());
+}
+
+// One workgroup per leaf; leaves that are not chunked return at once.
+__global__ void k_near_fix(NearArgs a, const int* __restrict__ leaf_node, const int* __restrict__ lk,
+                           const int* __restrict__ lpart, int nf) {
+    const int l = blockIdx.x;
+    const int K = lk[l];
+    if (K <= 1) return;
+    const int node = leaf_node[l];
+    const int t0 = a.tgt_start[node], tc = a.tgt_count[node];
+    const int p0 = lpart[l];
+    const int total = tc * nf * a.nsets;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int i = idx % tc;
+        const int ut = idx / tc;                  // set * nf + component
+        const double* src = a.part + (int64_t)ut * a.ldp + p0 + i;
+        double v = src[0];
+        for (int k = 1; k < K; ++k) v += src[(int64_t)k * tc];
+        double* o = a.out + (int64_t)ut * a.ldo + t0 + i;
+        if (a.accumulate) *o += v;
+        else *o = v;
+    }
+}
+
+void launch_near_fix(const NearArgs& a, const int* leaf_node, const int* lk, const int* lpart, int nl, int nf,
+                     hipStream_t s) {
+    if (nl <= 0 || !a.part) return;
+    hipLaunchKernelGGL(k_near_fix, dim3(nl), dim3(256), 0, s, a, leaf_node, lk, lpart, nf);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -405,6 +405,10 @@ class Context:
         """Tile-shape autotuner of the many-sets M2L (on by default); reset forgets earlier choices."""
         check(self._lib.fmmh_set_option(self._h, 11, -1.0 if reset else (1.0 if on else 0.0)))
 
+    def set_near_chunk(self, cap=0):
+        """Cap on the pair work of one near-field workgroup (source-tile iterations; 0: automatic)."""
+        check(self._lib.fmmh_set_option(self._h, 13, float(cap)))
+
     def set_graph(self, on=True):
         """Replay tree-reuse calls of an unchanged shape as a captured HIP graph."""
         check(self._lib.fmmh_set_option(self._h, 12, 1.0 if on else 0.0))
@@ -429,10 +433,10 @@ class Context:
                     full_order_spot_error=float(out[20]))
 
     def stats(self):
-        out = np.zeros(10, dtype=np.int64)
-        check(self._lib.fmmh_stats(self._h, _p(out), 10))
+        out = np.zeros(11, dtype=np.int64)
+        check(self._lib.fmmh_stats(self._h, _p(out), 11))
         keys = ("num_nodes", "num_leaves", "max_depth", "n_m2l", "n_p2p", "p2p_site_pairs", "m2l_classes",
-                "list_retries", "graph", "graph_replays")
+                "list_retries", "graph", "graph_replays", "near_workgroups")
         return dict(zip(keys, (int(v) for v in out)))
 
     def debug_buffer(self, name, count, dtype=np.float64):

@@ -276,13 +276,18 @@ enum fmmh_option {
                                      variant of that kernel on its own pairs and keeps the fastest for the
                                      context's later calls (results do not depend on the choice).  0: fixed
                                      defaults.  -1: forget the choices made so far and keep tuning on. */
-    FMMH_OPT_GRAPH = 12           /* != 0: tree-reuse calls (FMMH_OPT_REUSE_TREE) without timing, learned
+    FMMH_OPT_GRAPH = 12,          /* != 0: tree-reuse calls (FMMH_OPT_REUSE_TREE) without timing, learned
                                      orders or periodic images run their device work as a HIP graph: the
                                      first call of a shape runs eagerly, the next one captures and launches
                                      the graph, later ones replay it.  A shape is every pointer, size and
                                      option of the call; any reallocation of the context's buffers or
                                      rebuild of its cached tree data starts over.  Results are identical
                                      to eager calls.  0 (default): eager. */
+    FMMH_OPT_NEAR_CHUNK = 13      /* near-field work split: a leaf whose pair work exceeds a cap (in
+                                     source-tile iterations of one wavefront) runs as several workgroups
+                                     over consecutive parts of its source list, whose partial fields are
+                                     then summed in a fixed order.  0 (default): cap = total work / 16384,
+                                     at least 256.  > 0: that cap (tests, tuning). */
 };
 int fmmh_set_option(fmmh_context* ctx, int32_t key, double value);
 int fmmh_phase_times(fmmh_context* ctx, double* ms, int32_t cap, char* names, int32_t name_cap);
@@ -295,7 +300,8 @@ int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t b
 
 /* Counters of the last field_fmm call: {num_nodes, num_leaves, max_depth,
  * n_m2l, n_p2p, p2p_site_pairs, m2l_classes, list_retries, graph (0: eager,
- * 1: captured and launched, 2: replayed; FMMH_OPT_GRAPH), graph replays so far}. */
+ * 1: captured and launched, 2: replayed; FMMH_OPT_GRAPH), graph replays so far,
+ * near-field workgroups (leaves plus extra chunks; FMMH_OPT_NEAR_CHUNK)}. */
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap);
 
 /* ---- learned per-pair expansion orders (no counterpart in the reference) ------
