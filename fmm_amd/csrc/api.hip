@@ -480,15 +480,28 @@ __device__ long long leaf_work(int tc, long long total) {
     return w;
 }
 
-__global__ void k_leaf_work(const int* __restrict__ off, const int* __restrict__ src,
-                            const int* __restrict__ leaf_node, const int* __restrict__ count, int nl,
-                            long long* __restrict__ work) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per leaf: the source count of every list entry (ecnt, kept for the chunk split) and
+// the leaf's total and work.
+__global__ __launch_bounds__(64) void k_leaf_work(const int* __restrict__ off, const int* __restrict__ src,
+                                                  const int* __restrict__ leaf_node, const int* __restrict__ count,
+                                                  int nl, int* __restrict__ ecnt, long long* __restrict__ tot,
+                                                  long long* __restrict__ work) {
+    const int l = blockIdx.x;
     if (l >= nl) return;
     const int node = leaf_node[l];
-    long long total = 0;
-    for (int e = off[node]; e < off[node + 1]; ++e) total += count[src_node(src[e])];
-    work[l] = leaf_work(count[node], total);
+    const int e0 = off[node], e1 = off[node + 1];
+    long long t = 0;
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += 64) {
+        const int c = count[src_node(src[e])];
+        ecnt[e] = c;
+        t += c;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d);
+    if (threadIdx.x == 0) {
+        tot[l] = t;
+        work[l] = leaf_work(count[node], t);
+    }
 }
 
 // Exclusive prefix sums of n int64 values in one workgroup of 1024 threads (pre[n] = total); each
@@ -520,19 +533,10 @@ __global__ __launch_bounds__(1024) void k_scan_i64(const long long* __restrict__
     }
 }
 
-// Per leaf: chunk count K (1 = whole list), XCD range, partial sites (K tc when chunked; with one
-// extra slot for the totals at index nl so that exclusive scans over nl + 1 entries give them).
-__global__ void k_leaf_chunks(const long long* __restrict__ work, const long long* __restrict__ pre,
-                              const int* __restrict__ off, const int* __restrict__ leaf_node,
-                              const int* __restrict__ count, int nl, long long cap_set, int* __restrict__ lk,
-                              int* __restrict__ lx, int* __restrict__ lps, int* __restrict__ range_cnt) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l > nl) return;
-    if (l == nl) {
-        lk[nl] = 0;
-        lps[nl] = 0;
-        return;
-    }
+__device__ void leaf_chunk(const long long* __restrict__ work, const long long* __restrict__ pre,
+                           const int* __restrict__ off, const int* __restrict__ leaf_node,
+                           const int* __restrict__ count, int nl, long long cap_set, int l, int* __restrict__ lk,
+                           int* __restrict__ lx, int* __restrict__ lps, int* hist) {
     const long long total = pre[nl];
     const long long cap = cap_set > 0 ? cap_set : max(total / NEAR_CHUNK_DIV, NEAR_CHUNK_MIN);
     const int node = leaf_node[l];
@@ -545,29 +549,60 @@ __global__ void k_leaf_chunks(const long long* __restrict__ work, const long lon
     lk[l] = K;
     lx[l] = x;
     lps[l] = K > 1 ? K * count[node] : 0;
-    atomicAdd(&range_cnt[x], K);
+    atomicAdd(&hist[x], K);
 }
 
-// Per leaf: its chunks (consecutive list entries, boundaries at about equal cumulative source counts,
-// at least one entry each) and their sort keys: range, then decreasing work, then chunk id.
-__global__ void k_leaf_vblocks(const long long* __restrict__ work, const long long* __restrict__ pre,
-                               long long cap_set, const int* __restrict__ off,
-                               const int* __restrict__ src, const int* __restrict__ leaf_node,
-                               const int* __restrict__ count, int nl, const int* __restrict__ lk,
-                               const int* __restrict__ lx, const int* __restrict__ lvb, const int* __restrict__ lpart,
-                               int* __restrict__ vb_node, int* __restrict__ vb_e0, int* __restrict__ vb_e1,
-                               int* __restrict__ vb_part, unsigned long long* __restrict__ keys,
-                               int* __restrict__ vals) {
+// Per leaf: chunk count K (1 = whole list), XCD range, partial sites (K tc when chunked; with one
+// extra slot for the totals at index nl so that exclusive scans over nl + 1 entries give them).
+__global__ void k_leaf_chunks(const long long* __restrict__ work, const long long* __restrict__ pre,
+                              const int* __restrict__ off, const int* __restrict__ leaf_node,
+                              const int* __restrict__ count, int nl, long long cap_set, int* __restrict__ lk,
+                              int* __restrict__ lx, int* __restrict__ lps, int* __restrict__ range_cnt) {
+    __shared__ int hist[NUM_XCD];
+    if (threadIdx.x < NUM_XCD) hist[threadIdx.x] = 0;
+    __syncthreads();
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nl) leaf_chunk(work, pre, off, leaf_node, count, nl, cap_set, l, lk, lx, lps, hist);
+    if (l == nl) {
+        lk[nl] = 0;
+        lps[nl] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < NUM_XCD && hist[threadIdx.x]) atomicAdd(&range_cnt[threadIdx.x], hist[threadIdx.x]);
+}
+
+
+// Per leaf (one wavefront): its chunks (consecutive list entries, boundaries at about equal cumulative
+// source counts, at least one entry each) and their sort keys: range, then decreasing work.  The split
+// runs on lane 0 over the entry counts staged in LDS.
+constexpr int VB_STAGE = 512;
+__global__ __launch_bounds__(64) void k_leaf_vblocks(const long long* __restrict__ work,
+                                                     const long long* __restrict__ tot,
+                                                     const long long* __restrict__ pre, long long cap_set,
+                                                     const int* __restrict__ off, const int* __restrict__ ecnt,
+                                                     const int* __restrict__ leaf_node,
+                                                     const int* __restrict__ count, int nl,
+                                                     const int* __restrict__ lk, const int* __restrict__ lx,
+                                                     const int* __restrict__ lvb, const int* __restrict__ lpart,
+                                                     int* __restrict__ vb_node, int* __restrict__ vb_e0,
+                                                     int* __restrict__ vb_e1, int* __restrict__ vb_part,
+                                                     unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+    __shared__ int sc[VB_STAGE];
+    const int l = blockIdx.x;
     if (l >= nl) return;
     const int node = leaf_node[l];
     const int e0 = off[node], e1 = off[node + 1];
     const int K = lk[l];
+    const bool staged = e1 - e0 <= VB_STAGE;
+    if (K > 1 && staged)
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += 64) sc[e - e0] = ecnt[e];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     const int v0 = lvb[l];
     const unsigned long long x = (unsigned long long)lx[l];
     const long long cap = cap_set > 0 ? cap_set : max(pre[nl] / NEAR_CHUNK_DIV, NEAR_CHUNK_MIN);
-    long long total = 0;
-    for (int e = e0; e < e1; ++e) total += count[src_node(src[e])];
+    const long long total = tot[l];
+    const long long wl = work[l];
     int e = e0;
     long long cum = 0;
     for (int k = 0; k < K; ++k) {
@@ -581,7 +616,7 @@ __global__ void k_leaf_vblocks(const long long* __restrict__ work, const long lo
             // one entry for each later chunk
             const long long goal = ((long long)(k + 1) * total) / K;
             do {
-                const long long c = count[src_node(src[e])];
+                const long long c = staged ? sc[e - e0] : ecnt[e];
                 cum += c;
                 csrc += c;
                 ++e;
@@ -592,7 +627,7 @@ __global__ void k_leaf_vblocks(const long long* __restrict__ work, const long lo
         vb_e0[v] = a;
         vb_e1[v] = e;
         vb_part[v] = K > 1 ? lpart[l] + k * count[node] : -1;
-        const long long cw = total > 0 ? (work[l] * csrc) / total : work[l] / K;
+        const long long cw = total > 0 ? (wl * csrc) / total : wl / K;
         // work in 1/4096 of the cap, 16 bits; ties keep the chunk id order (stable sort)
         const unsigned long long w = (unsigned long long)min((cw << 12) / cap, 0xffffll);
         keys[v] = (x << 16) | (0xffffull - w);
@@ -902,10 +937,14 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int ni
         int2* fb = c->buf<int2>("l_front_b", cap_f);
         unsigned long long* m2l = c->buf<unsigned long long>("l_m2l", cap_m);
         unsigned long long* p2p = c->buf<unsigned long long>("l_p2p", cap_p);
-        int* counters = c->buf<int>("l_counters", 8);   // n_a, n_b, n_m2l, n_p2p, overflow
+        // n_a, n_b, n_m2l, n_p2p, overflow, each on its own 128-byte line (the traversal's atomics on
+        // them do not contend for one line)
+        constexpr int CL = 32;
+        int* counters = c->buf<int>("l_counters", 5 * CL);
         int init[5] = {nimg, 0, 0, 0, 0};
         std::memcpy(hp + 100, init, sizeof(init));
-        FMMH_HIP(hipMemcpyAsync(counters, hp + 100, 5 * sizeof(int), hipMemcpyHostToDevice, s));
+        for (int q = 0; q < 5; ++q)
+            FMMH_HIP(hipMemcpyAsync(counters + q * CL, hp + 100 + q, sizeof(int), hipMemcpyHostToDevice, s));
         for (int k = 0; k < nimg; ++k) {   // one (root, root) seed per image
             int2 root = make_int2(0, src_encode(k, 0));
             std::memcpy(hp + 110 + 2 * k, &root, sizeof(root));
@@ -920,16 +959,17 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int ni
         for (int it = 0; it < steps; ++it) {
             int2* fin = cur == 0 ? fa : fb;
             int2* fout = cur == 0 ? fb : fa;
-            int* cin = counters + cur;
-            int* cout = counters + (1 - cur);
+            int* cin = counters + cur * CL;
+            int* cout = counters + (1 - cur) * CL;
             FMMH_HIP(hipMemsetAsync(cout, 0, sizeof(int), s));
-            TraverseArgs a{fin, cin, cap_f, fout, cout, cap_f, m2l, counters + 2, cap_m, p2p, counters + 3, cap_p,
-                           counters + 4, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf, tr.t.children, theta,
-                           disp};
+            TraverseArgs a{fin, cin, cap_f, fout, cout, cap_f, m2l, counters + 2 * CL, cap_m, p2p, counters + 3 * CL,
+                           cap_p, counters + 4 * CL, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf,
+                           tr.t.children, theta, disp};
             launch_traverse(a, TRAVERSE_GRID, s);
             cur = 1 - cur;
         }
-        FMMH_HIP(hipMemcpyAsync(hp, counters, 5 * sizeof(int), hipMemcpyDeviceToHost, s));
+        for (int q = 0; q < 5; ++q)
+            FMMH_HIP(hipMemcpyAsync(hp + q, counters + q * CL, sizeof(int), hipMemcpyDeviceToHost, s));
         c->sync();
         if (hp[4]) {
             ++c->list_retries;
@@ -1344,8 +1384,9 @@ void near_order_begin(fmmh_context* c, TreeOut& tr, ListsOut& L) {
     int* lpart = c->buf<int>("s_leaf_part", nl + 1);
     int* range_cnt = c->buf<int>("s_range_cnt", NUM_XCD);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, range_cnt, 0, NUM_XCD);
-    hipLaunchKernelGGL(k_leaf_work, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
-                       tr.t.leaf_node, tr.t.count, nl, work);
+    hipLaunchKernelGGL(k_leaf_work, dim3(nl), dim3(64), 0, s, L.p2p_off, L.p2p_src, tr.t.leaf_node, tr.t.count,
+                       nl, c->buf<int>("s_entry_cnt", std::max(L.n_p2p, 1)), c->buf<long long>("s_leaf_tot", nl),
+                       work);
     hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, s, work, pre, nl);
     hipLaunchKernelGGL(k_leaf_chunks, dim3(nblk(nl + 1, 256)), dim3(256), 0, s, work, pre, L.p2p_off,
                        tr.t.leaf_node, tr.t.count, nl, c->near_chunk_cap, lk, lx, lps, range_cnt);
@@ -1389,9 +1430,10 @@ void near_order_end(fmmh_context* c, TreeOut& tr, ListsOut& L) {
     unsigned long long* vk = c->buf<unsigned long long>("s_vb_keys", nv);
     int* vv_in = c->buf<int>("s_vb_vals_in", nv);
     int* vv = c->buf<int>("s_vb_sorted", nv);
-    hipLaunchKernelGGL(k_leaf_vblocks, dim3(nblk(nl, 256)), dim3(256), 0, s, work, pre, c->near_chunk_cap,
-                       L.p2p_off, L.p2p_src, tr.t.leaf_node, tr.t.count, nl, lk, lx, lvb, lpart, vb_node, vb_e0,
-                       vb_e1, vb_part, vk_in, vv_in);
+    hipLaunchKernelGGL(k_leaf_vblocks, dim3(nl), dim3(64), 0, s, work, c->buf<long long>("s_leaf_tot", nl), pre,
+                       c->near_chunk_cap, L.p2p_off, c->buf<int>("s_entry_cnt", std::max(L.n_p2p, 1)),
+                       tr.t.leaf_node, tr.t.count, nl, lk, lx, lvb, lpart, vb_node, vb_e0, vb_e1, vb_part, vk_in,
+                       vv_in);
     size_t wb = sort_pairs_u64_temp(nv);
     sort_pairs_u64(c->buf<char>("s_vb_temp", wb), wb, vk_in, vk, vv_in, vv, nv, 19, s);   // 3 + 16 key bits
     int* slots = c->buf<int>("s_work_blk", (size_t)c->near_grid);

@@ -338,13 +338,24 @@ __global__ __launch_bounds__(256) void k_emit_nodes(int n, TreeDev t) {
     }
 }
 
+// Child links, and the level counts and maximum level (gathered per workgroup in LDS, then one global
+// atomic per level and workgroup: per-node global atomics on a few addresses serialise).
 __global__ __launch_bounds__(256) void k_node_links(int num_nodes, TreeDev t) {
+    __shared__ int cnt[KEY_LEVELS + 2];
+    __shared__ int lmax;
+    if (threadIdx.x < KEY_LEVELS + 2) cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) lmax = 0;
+    __syncthreads();
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= num_nodes) return;
-    const int L = t.level[id];
-    if (id > 0) t.children[t.parent[id] * 8 + t.octant[id]] = id;
-    atomicAdd(&t.info[3 + L], 1);
-    atomicMax(&t.info[2], L);
+    if (id < num_nodes) {
+        const int L = t.level[id];
+        if (id > 0) t.children[t.parent[id] * 8 + t.octant[id]] = id;
+        atomicAdd(&cnt[L], 1);
+        atomicMax(&lmax, L);
+    }
+    __syncthreads();
+    if (threadIdx.x < KEY_LEVELS + 2 && cnt[threadIdx.x]) atomicAdd(&t.info[3 + threadIdx.x], cnt[threadIdx.x]);
+    if (threadIdx.x == 0) atomicMax(&t.info[2], lmax);
 }
 
 __global__ __launch_bounds__(256) void k_leaf_index(int num_nodes, const int* __restrict__ leaf_scan, TreeDev t) {
@@ -359,12 +370,24 @@ __global__ __launch_bounds__(256) void k_leaf_index(int num_nodes, const int* __
     }
 }
 
+// Nodes grouped by level (any order within a level): positions from LDS counters per workgroup and one
+// global atomic per level and workgroup.
 __global__ __launch_bounds__(256) void k_level_scatter(int num_nodes, const int* __restrict__ level_off, TreeDev t) {
+    __shared__ int cnt[KEY_LEVELS + 2];
+    __shared__ int base[KEY_LEVELS + 2];
+    if (threadIdx.x < KEY_LEVELS + 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= num_nodes) return;
-    const int L = t.level[id];
-    const int pos = atomicAdd(&t.level_fill[L], 1);
-    t.level_nodes[level_off[L] + pos] = id;
+    int L = 0, pos = 0;
+    if (id < num_nodes) {
+        L = t.level[id];
+        pos = atomicAdd(&cnt[L], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < KEY_LEVELS + 2 && cnt[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&t.level_fill[threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (id < num_nodes) t.level_nodes[level_off[L] + base[L] + pos] = id;
 }
 
 // Leaf members in ascending site index (the reference's insertion order).
@@ -432,32 +455,54 @@ __device__ __forceinline__ double norm2_3(double v0, double v1, double v2) {
 // traverse from one (root, root) seed per image (src/fmm.F90:337-340): the
 // source id carries the image, whose displacement enters the MAC as
 // c_i - (c_j + D) (src/fmm.F90:360-362).
-// Wave-aggregated append: every lane asks for `want` slots of `counter`;
-// one atomic per wavefront, lanes get consecutive ranges in lane order.
-__device__ __forceinline__ int wave_append(int* counter, int want) {
-    const int lane = __lane_id();
-    int incl = want;
+// Workgroup-aggregated append to the three output lists: every thread asks for w[c] slots of counter
+// c; one atomic per workgroup and counter (the counters sit on separate cache lines), threads get
+// consecutive ranges in thread order.  Every thread of the workgroup must call it.
+constexpr int TRAVERSE_THREADS = 256;
+__device__ __forceinline__ void block_append3(int* const cnt[3], const int w[3], int pos[3]) {
+    __shared__ int tot[3][TRAVERSE_THREADS / 64];
+    __shared__ int base[3][TRAVERSE_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl[3] = {w[0], w[1], w[2]};
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int v = __shfl_up(incl[c], d, 64);
+            if (lane >= d) incl[c] += v;
+        }
     }
-    const int total = __shfl(incl, 63, 64);
-    int base = 0;
-    if (lane == 63 && total > 0) base = atomicAdd(counter, total);
-    base = __shfl(base, 63, 64);
-    return base + incl - want;
+    if (lane == 63) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tot[c][wid] = incl[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        int run = 0, b[TRAVERSE_THREADS / 64];
+#pragma unroll
+        for (int q = 0; q < TRAVERSE_THREADS / 64; ++q) {
+            b[q] = run;
+            run += tot[c][q];
+        }
+        const int g = run > 0 ? atomicAdd(cnt[c], run) : 0;
+#pragma unroll
+        for (int q = 0; q < TRAVERSE_THREADS / 64; ++q) base[c][q] = g + b[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pos[c] = base[c][wid] + incl[c] - w[c];
 }
 
-__global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
+__global__ __launch_bounds__(TRAVERSE_THREADS) void k_traverse(TraverseArgs a) {
 #pragma clang fp contract(off)
     // after an overflowing step the count exceeds what was stored: read only
     // the stored part (the overflow flag is already set and the host retries)
     const int n = min(*a.n_in, a.cap_in);
-    const int stride = gridDim.x * blockDim.x;
-    // the loop runs over whole wavefronts so the lane collectives see every lane
-    for (int e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); e0 < n; e0 += stride) {
-        const int e = e0 + (threadIdx.x & 63);
+    const int stride = gridDim.x * TRAVERSE_THREADS;
+    // the loop runs over whole workgroups so the workgroup collectives see every thread
+    for (int e0 = blockIdx.x * TRAVERSE_THREADS; e0 < n; e0 += stride) {
+        const int e = e0 + (int)threadIdx.x;
         int kind = -1;                 // 0 far (M2L), 1 leaf pair (P2P), 2 descend target, 3 descend source
         int i = 0, j = 0, img = 0, nchild = 0;
         if (e < n) {
@@ -482,17 +527,19 @@ __global__ __launch_bounds__(256) void k_traverse(TraverseArgs a) {
         }
         const int jenc = src_encode(img, j);
         const unsigned long long key = ((unsigned long long)i << 32) | (unsigned)jenc;
-        int pos = wave_append(a.n_m2l, kind == 0 ? 1 : 0);
+        int* const cnt[3] = {a.n_m2l, a.n_p2p, a.n_out};
+        const int want[3] = {kind == 0 ? 1 : 0, kind == 1 ? 1 : 0, nchild};
+        int got[3];
+        block_append3(cnt, want, got);
         if (kind == 0) {
-            if (pos < a.cap_m2l) a.m2l[pos] = key;
+            if (got[0] < a.cap_m2l) a.m2l[got[0]] = key;
             else atomicOr(a.overflow, 1);
         }
-        pos = wave_append(a.n_p2p, kind == 1 ? 1 : 0);
         if (kind == 1) {
-            if (pos < a.cap_p2p) a.p2p[pos] = key;
+            if (got[1] < a.cap_p2p) a.p2p[got[1]] = key;
             else atomicOr(a.overflow, 2);
         }
-        pos = wave_append(a.n_out, nchild);
+        int pos = got[2];
         if (kind >= 2) {
             const int pnode = kind == 2 ? i : j;
             for (int o = 0; o < 8; ++o) {
@@ -580,7 +627,7 @@ void launch_gather(const SiteArgs& a, hipStream_t s) {
 }
 
 void launch_traverse(const TraverseArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_traverse, dim3(grid), dim3(TRAVERSE_THREADS), 0, s, a);
     FMMH_HIP(hipGetLastError());
 }
 
