@@ -34,10 +34,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the near-field kernel on the default workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one run each) scaled by the same
 # counters on a plain 1 GiB device copy (FETCH_SIZE factor 2.000, WRITE_SIZE
-# 1.000): 21,476 KiB x 2 read + 20,906 KiB written.  The profiler cannot run
+# 1.000): 43,544 KiB x 2 read + 83,700 KiB written (the written part includes
+# the partial fields of chunked leaves).  The profiler cannot run
 # inside this process, so the figure is the recorded measurement of
 # profiles/k_near_water200k_thole_p6.md.
-K_NEAR_TRAFFIC_BYTES = (21476.0 * 2.0 + 20906.0) * 1024.0
+K_NEAR_TRAFFIC_BYTES = (43544.0 * 2.0 + 83700.0) * 1024.0
 
 
 def c_T(K):

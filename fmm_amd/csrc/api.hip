@@ -432,14 +432,22 @@ __global__ void k_direct_blocks(int* start, int* count, int* list_off, int* list
     if (b == 0) list_off[nb] = nb;
 }
 
-__global__ void k_pair_count(const int* off, const int* src, const int* leaf_node, const int* count, int nl,
-                             unsigned long long* total) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nl) return;
-    int node = leaf_node[l];
-    unsigned long long s = 0;
-    for (int e = off[node]; e < off[node + 1]; ++e) s += (unsigned long long)count[src_node(src[e])];
-    atomicAdd(total, s * (unsigned long long)count[node]);
+// Site pairs of the near field: sum over leaves of (sites) x (sources of its list), from the per-leaf
+// source totals; one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_pair_count(const long long* __restrict__ tot, const int* __restrict__ leaf_node,
+                                                    const int* __restrict__ count, int nl,
+                                                    unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long wsum[4];
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = l < nl ? (unsigned long long)tot[l] * (unsigned long long)count[leaf_node[l]] : 0ull;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd(total, t);
+    }
 }
 
 // Order of the near-field workgroups.  Workgroup b runs on XCD b % 8 (the
@@ -508,25 +516,31 @@ __global__ __launch_bounds__(64) void k_leaf_work(const int* __restrict__ off, c
 // thread sums a contiguous run, so the integer result does not depend on scheduling.
 __global__ __launch_bounds__(1024) void k_scan_i64(const long long* __restrict__ in, long long* __restrict__ pre,
                                                    int n) {
-    __shared__ long long part[1024];
-    const int t = threadIdx.x;
+    __shared__ long long wtot[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n + 1023) / 1024;
     const int a = min(n, t * per), b = min(n, a + per);
     long long s = 0;
     for (int i = a; i < b; ++i) s += in[i];
-    part[t] = s;
+    long long incl = s;   // inclusive scan over the wavefront, then over the 16 wavefront totals
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) wtot[w] = incl;
     __syncthreads();
     if (t == 0) {
         long long run = 0;
-        for (int i = 0; i < 1024; ++i) {
-            const long long v = part[i];
-            part[i] = run;
+        for (int q = 0; q < 16; ++q) {
+            const long long v = wtot[q];
+            wtot[q] = run;
             run += v;
         }
         pre[n] = run;
     }
     __syncthreads();
-    long long run = part[t];
+    long long run = wtot[w] + incl - s;
     for (int i = a; i < b; ++i) {
         pre[i] = run;
         run += in[i];
@@ -1642,8 +1656,8 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     } else {
         unsigned long long* pc = c->buf<unsigned long long>("e_paircount", 1);
         FMMH_HIP(hipMemsetAsync(pc, 0, sizeof(unsigned long long), s));
-        hipLaunchKernelGGL(k_pair_count, dim3(nblk(nl, 256)), dim3(256), 0, s, L.p2p_off, L.p2p_src,
-                           tr.t.leaf_node, tr.t.count, nl, pc);
+        hipLaunchKernelGGL(k_pair_count, dim3(nblk(nl, 256)), dim3(256), 0, s,
+                           c->buf<long long>("s_leaf_tot", nl), tr.t.leaf_node, tr.t.count, nl, pc);
         FMMH_HIP(hipMemcpyAsync(c->h_pinned + 200, pc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         c->sync();
         std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
