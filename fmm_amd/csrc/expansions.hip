@@ -105,47 +105,55 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
     }
 }
 
-__global__ __launch_bounds__(128) void k_m2m(UpArgs a, const int* __restrict__ nodes) {
+// One workgroup per parent node: the children's moments are staged in LDS, every (child, coefficient)
+// sum runs on its own lane (a child's terms in table order), and each coefficient adds its children's
+// sums in child order -- the same order as summing child by child, so the many-sets form below (which
+// does that) stays bit-identical.
+__global__ __launch_bounds__(256) void k_m2m(UpArgs a, const int* __restrict__ nodes) {
     __shared__ double monos[8][MAX_NM];
+    __shared__ double srcs[8][MAX_NM];
+    __shared__ double part[8][MAX_NM];
     __shared__ double d3[24];
     __shared__ double pw[24 * (PMAX + 1)];
+    __shared__ int kids[8];
     const int node = nodes[blockIdx.x];
     const int nm = a.nm;
     double* __restrict__ CM = a.CM + (int64_t)blockIdx.y * a.num_nodes * nm;
     const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
-    double acc[(MAX_NM + 127) / 128];
-#pragma unroll
-    for (int q = 0; q < (MAX_NM + 127) / 128; ++q) acc[q] = 0.0;
+    if (threadIdx.x < 8) kids[threadIdx.x] = a.children[node * 8 + threadIdx.x];
     if (threadIdx.x < 24) {
         const int o = threadIdx.x / 3, ax = threadIdx.x - 3 * o;
         const int ch = a.children[node * 8 + o];
         d3[threadIdx.x] = ch < 0 ? 0.0 : (ax == 0 ? a.cx[ch] - cx : (ax == 1 ? a.cy[ch] - cy : a.cz[ch] - cz));
     }
     __syncthreads();
-    build_mono_block(&monos[0][0], MAX_NM, nm, 8, d3, pw, a.xyz, a.p);
     bool any = false;
-    for (int o = 0; o < 8; ++o) {
-        const int ch = a.children[node * 8 + o];
-        if (ch < 0) continue;
-        any = true;
-        const double* __restrict__ mono = monos[o];
-        const double* src = CM + (int64_t)ch * nm;
 #pragma unroll
-        for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
-            const int t = threadIdx.x + 128 * q;
-            if (t < nm) {
-                const int b0 = a.m2m.off[t], b1 = a.m2m.off[t + 1];
-                double sum = 0.0;
-                for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.m2m.coef[b] * mono[a.m2m.e[b]], src[a.m2m.s[b]], sum);
-                acc[q] += sum;
-            }
-        }
-    }
+    for (int o = 0; o < 8; ++o) any |= kids[o] >= 0;
     if (!any) return;   // leaves keep their P2M moments
+    for (int x = threadIdx.x; x < 8 * nm; x += blockDim.x) {
+        const int o = x / nm, t = x - o * nm;
+        const int ch = kids[o];
+        srcs[o][t] = ch >= 0 ? CM[(int64_t)ch * nm + t] : 0.0;
+    }
+    build_mono_block(&monos[0][0], MAX_NM, nm, 8, d3, pw, a.xyz, a.p);
+    for (int x = threadIdx.x; x < 8 * nm; x += blockDim.x) {
+        const int o = x / nm, t = x - o * nm;
+        if (kids[o] < 0) continue;
+        const double* __restrict__ mono = monos[o];
+        const double* __restrict__ src = srcs[o];
+        const int b0 = a.m2m.off[t], b1 = a.m2m.off[t + 1];
+        double sum = 0.0;
+        for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.m2m.coef[b] * mono[a.m2m.e[b]], src[a.m2m.s[b]], sum);
+        part[o][t] = sum;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nm; t += blockDim.x) {
+        double acc = 0.0;
 #pragma unroll
-    for (int q = 0; q < (MAX_NM + 127) / 128; ++q) {
-        const int t = threadIdx.x + 128 * q;
-        if (t < nm) CM[(int64_t)node * nm + t] = acc[q];
+        for (int o = 0; o < 8; ++o)
+            if (kids[o] >= 0) acc += part[o][t];
+        CM[(int64_t)node * nm + t] = acc;
     }
 }
 
@@ -486,7 +494,7 @@ void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
 
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s) {
     if (count <= 0) return;
-    if (a.nsets == 1) hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(128), 0, s, a, nodes);
+    if (a.nsets == 1) hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(256), 0, s, a, nodes);
     else hipLaunchKernelGGL(k_m2m_sb<SHIFT_SB>, dim3(count, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(256), 0, s, a, nodes);
     FMMH_HIP(hipGetLastError());
 }
