@@ -53,8 +53,9 @@ __device__ __forceinline__ void build_mono_block(double* mono, int ld, int nm, i
     __syncthreads();
 }
 
+// LDS: the monomials of a chunk of sites, [UP_CHUNK][nm] doubles (dynamic, sized by the order).
 __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
-    __shared__ double mono[UP_CHUNK][MAX_NM];
+    extern __shared__ double mono[];
     __shared__ double mch[UP_CHUNK][UP_NSRC];
     __shared__ double d3[UP_CHUNK * 3];
     __shared__ double pw[UP_CHUNK * 3 * (PMAX + 1)];
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
             d3[threadIdx.x] = ax == 0 ? a.xs[j] - cx : (ax == 1 ? a.ys[j] - cy : a.zs[j] - cz);
         }
         __syncthreads();
-        build_mono_block(&mono[0][0], MAX_NM, nm, cc, d3, pw, a.xyz, a.p);
+        build_mono_block(mono, nm, nm, cc, d3, pw, a.xyz, a.p);
         for (int x = threadIdx.x; x < cc * nsrc; x += blockDim.x) {
             const int jj = x / nsrc, sc = x % nsrc;
             mch[jj][sc] = msr[(int64_t)sc * a.ldm + s0 + j0 + jj];
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
                 for (int b = b0; b < b1; ++b) {
                     const int s = a.p2m.s[b], e = a.p2m.e[b];
                     double sum = 0.0;
-                    for (int jj = 0; jj < cc; ++jj) sum = __builtin_fma(mono[jj][e], mch[jj][s], sum);
+                    for (int jj = 0; jj < cc; ++jj) sum = __builtin_fma(mono[jj * nm + e], mch[jj][s], sum);
                     acc[q] = __builtin_fma(a.p2m.coef[b], sum, acc[q]);
                 }
             }
@@ -109,15 +110,17 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
 // sum runs on its own lane (a child's terms in table order), and each coefficient adds its children's
 // sums in child order -- the same order as summing child by child, so the many-sets form below (which
 // does that) stays bit-identical.
+// LDS: monomials, staged child moments and partial sums, 3 x [8][nm] doubles (dynamic).
 __global__ __launch_bounds__(256) void k_m2m(UpArgs a, const int* __restrict__ nodes) {
-    __shared__ double monos[8][MAX_NM];
-    __shared__ double srcs[8][MAX_NM];
-    __shared__ double part[8][MAX_NM];
+    extern __shared__ double m2m_lds[];
     __shared__ double d3[24];
     __shared__ double pw[24 * (PMAX + 1)];
     __shared__ int kids[8];
     const int node = nodes[blockIdx.x];
     const int nm = a.nm;
+    double* monos = m2m_lds;
+    double* srcs = m2m_lds + 8 * nm;
+    double* part = m2m_lds + 16 * nm;
     double* __restrict__ CM = a.CM + (int64_t)blockIdx.y * a.num_nodes * nm;
     const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
     if (threadIdx.x < 8) kids[threadIdx.x] = a.children[node * 8 + threadIdx.x];
@@ -134,25 +137,25 @@ __global__ __launch_bounds__(256) void k_m2m(UpArgs a, const int* __restrict__ n
     for (int x = threadIdx.x; x < 8 * nm; x += blockDim.x) {
         const int o = x / nm, t = x - o * nm;
         const int ch = kids[o];
-        srcs[o][t] = ch >= 0 ? CM[(int64_t)ch * nm + t] : 0.0;
+        srcs[o * nm + t] = ch >= 0 ? CM[(int64_t)ch * nm + t] : 0.0;
     }
-    build_mono_block(&monos[0][0], MAX_NM, nm, 8, d3, pw, a.xyz, a.p);
+    build_mono_block(monos, nm, nm, 8, d3, pw, a.xyz, a.p);
     for (int x = threadIdx.x; x < 8 * nm; x += blockDim.x) {
         const int o = x / nm, t = x - o * nm;
         if (kids[o] < 0) continue;
-        const double* __restrict__ mono = monos[o];
-        const double* __restrict__ src = srcs[o];
+        const double* __restrict__ mono = monos + o * nm;
+        const double* __restrict__ src = srcs + o * nm;
         const int b0 = a.m2m.off[t], b1 = a.m2m.off[t + 1];
         double sum = 0.0;
         for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.m2m.coef[b] * mono[a.m2m.e[b]], src[a.m2m.s[b]], sum);
-        part[o][t] = sum;
+        part[o * nm + t] = sum;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nm; t += blockDim.x) {
         double acc = 0.0;
 #pragma unroll
         for (int o = 0; o < 8; ++o)
-            if (kids[o] >= 0) acc += part[o][t];
+            if (kids[o] >= 0) acc += part[o * nm + t];
         CM[(int64_t)node * nm + t] = acc;
     }
 }
@@ -182,19 +185,20 @@ __global__ __launch_bounds__(128) void k_l2l(DownArgs a, const int* __restrict__
     }
 }
 
-// f_t = sum_e L_{t+e} d^e / (ex! ey! ez!), |t| <= F, |t + e| <= p, one lane per site.
-template <int F>
-__global__ __launch_bounds__(64) void k_l2p(DownArgs a) {
+// f_t = sum_e L_{t+e} d^e / (ex! ey! ez!), |t| <= F, |t + e| <= p, one lane per site, added to the near
+// field and scattered to the caller's site order.  The order is a template parameter: every loop is
+// unrolled, so the packed indices are constants (LDS reads with immediate offsets) and the |t + e| <= p
+// tests vanish.  The operations and their order are those of k_l2p_sb (runtime order, many sets), so a
+// set of a many-set call equals the one-set call bit for bit.
+template <int P, int F>
+__global__ __launch_bounds__(64) void k_l2p_p(DownArgs a) {
     constexpr int NF = ncomp(F);
-    __shared__ double L[MAX_NM];
+    constexpr int NM = ncomp(P);
+    __shared__ double L[NM];
     __shared__ double fact[PMAX + 1];
     const int node = a.leaf_node[blockIdx.x];
-    const int set = blockIdx.y;
     const int s0 = a.start[node], c = a.count[node];
-    const int nm = a.nm, p = a.p;
-    const double* __restrict__ near = a.near + (int64_t)set * NF * a.ld_near;
-    double* __restrict__ field = a.field + (int64_t)set * NF * a.ld_field;
-    for (int s = threadIdx.x; s < nm; s += 64) L[s] = a.LE[((int64_t)set * a.num_nodes + node) * nm + s];
+    for (int s = threadIdx.x; s < NM; s += 64) L[s] = a.LE[(int64_t)node * NM + s];
     if (threadIdx.x == 0) {
         double f = 1.0;
         fact[0] = 1.0;
@@ -211,17 +215,19 @@ __global__ __launch_bounds__(64) void k_l2p(DownArgs a) {
 #pragma unroll
         for (int t = 0; t < NF; ++t) f[t] = 0.0;
         double px = 1.0;
-        for (int ex = 0; ex <= p; ++ex) {
+#pragma unroll
+        for (int ex = 0; ex <= P; ++ex) {
             double pxy = px;
-            for (int ey = 0; ey + ex <= p; ++ey) {
+#pragma unroll
+            for (int ey = 0; ey + ex <= P; ++ey) {
                 double m = pxy * fact[ex] * fact[ey];
-                for (int ez = 0; ez + ey + ex <= p; ++ez) {
+#pragma unroll
+                for (int ez = 0; ez + ey + ex <= P; ++ez) {
                     const double mz = m * fact[ez];
-                    const int ne = ex + ey + ez;
 #pragma unroll
                     for (int t = 0; t < NF; ++t) {
                         const Exps et = idx2xyz(t);
-                        if (et.x + et.y + et.z + ne <= p)
+                        if (et.x + et.y + et.z + ex + ey + ez <= P)
                             f[t] = __builtin_fma(mz, L[xyz2idx(et.x + ex, et.y + ey, et.z + ez)], f[t]);
                     }
                     m *= dz;
@@ -233,9 +239,27 @@ __global__ __launch_bounds__(64) void k_l2p(DownArgs a) {
         const int o = a.perm[k];
 #pragma unroll
         for (int t = 0; t < NF; ++t) {
-            double* fp = field + (int64_t)t * a.ld_field + o;
-            *fp = (*fp + near[(int64_t)t * a.ld_near + k]) + f[t];
+            double* fp = a.field + (int64_t)t * a.ld_field + o;
+            *fp = (*fp + a.near[(int64_t)t * a.ld_near + k]) + f[t];
         }
+    }
+}
+
+template <int F>
+static void launch_l2p_p(const DownArgs& a, int num_leaves, hipStream_t s) {
+    const dim3 g(num_leaves, 1);
+    switch (a.p) {
+        case 0: hipLaunchKernelGGL((k_l2p_p<0, F>), g, dim3(64), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_l2p_p<1, F>), g, dim3(64), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_l2p_p<2, F>), g, dim3(64), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_l2p_p<3, F>), g, dim3(64), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_l2p_p<4, F>), g, dim3(64), 0, s, a); break;
+        case 5: hipLaunchKernelGGL((k_l2p_p<5, F>), g, dim3(64), 0, s, a); break;
+        case 6: hipLaunchKernelGGL((k_l2p_p<6, F>), g, dim3(64), 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_l2p_p<7, F>), g, dim3(64), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_l2p_p<8, F>), g, dim3(64), 0, s, a); break;
+        case 9: hipLaunchKernelGGL((k_l2p_p<9, F>), g, dim3(64), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_l2p_p<10, F>), g, dim3(64), 0, s, a); break;
     }
 }
 
@@ -487,14 +511,16 @@ __global__ __launch_bounds__(64) void k_l2p_sb(DownArgs a) {
 
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
-    if (a.nsets == 1) hipLaunchKernelGGL(k_p2m, dim3(num_leaves, 1), dim3(128), 0, s, a);
+    if (a.nsets == 1)
+        hipLaunchKernelGGL(k_p2m, dim3(num_leaves, 1), dim3(128), (size_t)UP_CHUNK * a.nm * sizeof(double), s, a);
     else hipLaunchKernelGGL(k_p2m_sb<SHIFT_SB>, dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(128), 0, s, a);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s) {
     if (count <= 0) return;
-    if (a.nsets == 1) hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(256), 0, s, a, nodes);
+    if (a.nsets == 1)
+        hipLaunchKernelGGL(k_m2m, dim3(count, 1), dim3(256), (size_t)24 * a.nm * sizeof(double), s, a, nodes);
     else hipLaunchKernelGGL(k_m2m_sb<SHIFT_SB>, dim3(count, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(256), 0, s, a, nodes);
     FMMH_HIP(hipGetLastError());
 }
@@ -510,9 +536,9 @@ void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
     if (a.nsets == 1) {
         switch (a.nf) {
-            case 1: hipLaunchKernelGGL(k_l2p<0>, dim3(num_leaves, 1), dim3(64), 0, s, a); break;
-            case 4: hipLaunchKernelGGL(k_l2p<1>, dim3(num_leaves, 1), dim3(64), 0, s, a); break;
-            default: hipLaunchKernelGGL(k_l2p<2>, dim3(num_leaves, 1), dim3(64), 0, s, a); break;
+            case 1: launch_l2p_p<0>(a, num_leaves, s); break;
+            case 4: launch_l2p_p<1>(a, num_leaves, s); break;
+            default: launch_l2p_p<2>(a, num_leaves, s); break;
         }
     } else {
         constexpr int SB2 = SHIFT_SB / 2;   // field gradients: 10 components per set in registers
