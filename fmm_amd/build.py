@@ -14,8 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfmm_hip.so")
-SOURCES = ["prims.hip", "tree.hip", "near.hip", "expansions.hip", "m2l.hip", "m2l_gemm.hip", "periodic.hip", "qmmm.hip",
-           "selector.hip", "api.hip"]
+SOURCES = ["prims.hip", "tree.hip", "near.hip", "expansions.hip", "shift_mfma.hip", "m2l.hip", "m2l_gemm.hip",
+           "periodic.hip", "qmmm.hip", "selector.hip", "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"

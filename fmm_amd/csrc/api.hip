@@ -38,6 +38,9 @@ struct Tables {
     int p = -1, m = -1;
     int* ibuf = nullptr;
     double* dbuf = nullptr;
+    float* fbuf = nullptr;
+    const float* m2m_frag = nullptr;  // mixed precision: M2M / L2L matrices per octant, MFMA fragment order
+    const float* l2l_frag = nullptr;
     ShiftTab p2m{}, m2m{}, l2l{};
     const int* mono_parent = nullptr;
     const int* mono_axis = nullptr;
@@ -411,6 +414,36 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     }
     t.cs = D + doff[3];
     t.scale = D + doff[4];
+    // Box-scaled M2M / L2L matrices per octant for the FP32 MFMA shifts (shift_mfma.hip):
+    // M2M S[t][s] = c(t, s) u^(t-s) 2^-|t|, L2L S[t][s] = c(t, s) u^(s-t) 2^-|s|, u = (+-1, +-1, +-1) by
+    // octant bits; lane l of the fragment (mt, ks) holds S[16 mt + l % 16][4 ks + l / 16].
+    if (p >= 1) {
+        const int KS = (nm + 3) / 4;
+        const size_t nfrag = (size_t)shift_mfma_frag_floats(p);
+        std::vector<float> hf(2 * 8 * nfrag, 0.f);
+        auto fill = [&](float* dst, const std::vector<int>& off, const std::vector<int>& ss,
+                        const std::vector<int>& ee, const std::vector<double>& cc, bool l2l, int o) {
+            const int ux = (o & 1) ? 1 : -1, uy = (o & 2) ? 1 : -1, uz = (o & 4) ? 1 : -1;
+            for (int tt = 0; tt < nm; ++tt)
+                for (int b = off[tt]; b < off[tt + 1]; ++b) {
+                    const int sv = ss[b];
+                    const XYZ& E = comp[ee[b]];
+                    const XYZ& T = comp[l2l ? sv : tt];
+                    const double sign = ((E.x & 1) ? ux : 1) * ((E.y & 1) ? uy : 1) * ((E.z & 1) ? uz : 1);
+                    const double v = cc[b] * sign * std::ldexp(1.0, -(T.x + T.y + T.z));
+                    const int mt = tt / 16, ks = sv / 4, l = tt % 16 + 16 * (sv % 4);
+                    dst[((size_t)mt * KS + ks) * 64 + l] = (float)v;
+                }
+        };
+        for (int o = 0; o < 8; ++o) {
+            fill(hf.data() + (size_t)o * nfrag, m2m_off, m2m_s, m2m_e, m2m_c, false, o);
+            fill(hf.data() + (size_t)(8 + o) * nfrag, l2l_off, l2l_s, l2l_e, l2l_c, true, o);
+        }
+        FMMH_HIP(hipMalloc((void**)&t.fbuf, hf.size() * sizeof(float)));
+        FMMH_HIP(hipMemcpy(t.fbuf, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice));
+        t.m2m_frag = t.fbuf;
+        t.l2l_frag = t.fbuf + 8 * nfrag;
+    }
     return c->tables.emplace(key, t).first->second;
 }
 
@@ -1543,9 +1576,27 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
               tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, S, k.dim, tb.xyz};
     launch_p2m(up, nl, s);
+    // mixed precision: M2M and L2L on FP32 MFMA (box-scaled matrices per octant), FP64 accumulation,
+    // over per-level lists of internal nodes
+    const bool shift32 = c->precision == 1 && p >= 1 && tb.m2m_frag;
+    int* inodes = nullptr;
+    int* icnt = nullptr;
+    if (shift32) {
+        inodes = c->buf<int>("s_internal", nn);
+        icnt = c->buf<int>("s_internal_cnt", KEY_LEVELS + 2);
+        LevelStarts ls{};
+        for (int L = 0; L < KEY_LEVELS + 3; ++L) ls.off[L] = tr.level_off[L];
+        launch_internal_nodes(nn, tr.t.level, tr.t.children, ls, inodes, icnt, s);
+    }
     for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
         int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
-        launch_m2m(up, tr.t.level_nodes + o0, o1 - o0, s);
+        if (shift32) {
+            ShiftMfmaArgs sa{inodes + o0, icnt + lev, o1 - o0, tr.t.children, tr.t.r, CM, CM, (int64_t)nn * nm, S,
+                             tb.m2m_frag, tb.xyz};
+            launch_shift_mfma(sa, p, false, s);
+        } else {
+            launch_m2m(up, tr.t.level_nodes + o0, o1 - o0, s);
+        }
     }
     c->mark("upward");
     double* near = c->buf<double>("e_near", (size_t)n * nf * S);
@@ -1575,8 +1626,15 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
                 tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S, tb.xyz};
     auto downward = [&]() {
         for (int lev = 2; lev <= tr.max_depth; ++lev) {
-            int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
-            launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
+            if (shift32) {   // the parents of this level's nodes, on FP32 MFMA
+                int o0 = tr.level_off[lev - 1], o1 = tr.level_off[lev];
+                ShiftMfmaArgs sa{inodes + o0, icnt + lev - 1, o1 - o0, tr.t.children, tr.t.r, LE, LE,
+                                 (int64_t)nn * nm, S, tb.l2l_frag, tb.xyz};
+                launch_shift_mfma(sa, p, true, s);
+            } else {
+                int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
+                launch_l2l(da, tr.t.level_nodes + o0, o1 - o0, s);
+            }
         }
         c->mark("l2l");
         launch_l2p(da, nl, s);
@@ -1816,6 +1874,7 @@ int fmmh_context_destroy(fmmh_context* ctx) {
         for (auto& kv : ctx->tables) {
             FMMH_HIP(hipFree(kv.second.ibuf));
             FMMH_HIP(hipFree(kv.second.dbuf));
+            if (kv.second.fbuf) FMMH_HIP(hipFree(kv.second.fbuf));
         }
         for (hipEvent_t e : ctx->event_pool) FMMH_HIP(hipEventDestroy(e));
         if (ctx->graph_exec) FMMH_HIP(hipGraphExecDestroy(ctx->graph_exec));

@@ -356,6 +356,27 @@ struct SpotArgs {
 void launch_spot_far(const SpotArgs& a, int m, hipStream_t s);
 void launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 
+// M2M / L2L on FP32 MFMA for one level (shift_mfma.hip): columns are (node of the level, set) pairs.
+struct LevelStarts {
+    int off[KEY_LEVELS + 3];       // first slot of each level in a per-level node array
+};
+struct ShiftMfmaArgs {
+    const int* nodes;              // the level's internal nodes
+    const int* count;              //   their number (device)
+    int max_count;                 //   and an upper bound (grid size)
+    const int* children;           // [node][8] by octant, -1 where absent
+    const double* r;               // node half-widths
+    const double* src; double* dst;   // [set][node][nm]: M2M children -> parents, L2L parents -> children
+    int64_t set_stride;            // num_nodes * nm
+    int nsets;
+    const float* A;                // 8 shift matrices (one per octant) in MFMA fragment order
+    const int* xyz;                // packed index -> ex | ey << 8 | ez << 16
+};
+int shift_mfma_frag_floats(int p);
+void launch_shift_mfma(const ShiftMfmaArgs& a, int p, bool l2l, hipStream_t s);
+void launch_internal_nodes(int num_nodes, const int* level, const int* children, const LevelStarts& ls, int* out,
+                           int* cnt, hipStream_t s);
+
 struct DownArgs {
     const double* cx; const double* cy; const double* cz;
     const int* parent;

@@ -1,8 +1,9 @@
-"""Mixed precision (FMMH_OPT_PRECISION = 1): the M2L translations run on FP32
-MFMA with an FP64 sum into the local expansions; everything else is FP64.
-Checks that the FP32 path is the one that runs (results differ from FP64 at
-the FP32 level, not at rounding level) and that its error stays far below
-the truncation error the FMM already has against the direct sum."""
+"""Mixed precision (FMMH_OPT_PRECISION = 1): the M2M, M2L and L2L translations
+run on FP32 MFMA with FP64 sums into the expansions; P2M, L2P and the near
+field stay FP64.  Checks that the FP32 paths are the ones that run (results
+differ from FP64 at the FP32 level, not at rounding level) and that their
+error stays far below the truncation error the FMM already has against the
+direct sum."""
 import numpy as np
 import pytest
 
@@ -67,3 +68,34 @@ def test_mixed_per_target_mfma_for_many_sets(mixed, p):
     for k in (0, 15, 16, S - 1):
         d = max(block_rel(got[:, :, k], ref[:, :, k], 1))
         assert 1e-12 < d < 2e-6, (k, d)
+
+
+@pytest.mark.parametrize("p,nsets", [(3, 1), (6, 1), (10, 1), (6, 20)])
+def test_mixed_m2m_moments_close_to_fp64(mixed, p, nsets):
+    """The FP32 M2M (box-scaled shift matrices per octant) against the FP64
+    one: leaf moments (P2M, FP64 in both) are identical, every internal
+    node's moments agree to FP32 level relative to the node's scaled norm."""
+    g = load_golden("water999_dipole_orders.npz")
+    x, e = g["coordinates"], g["exclusions"]
+    n = x.shape[0]
+    rng = np.random.default_rng(7 + p)
+    m = np.zeros((n, 4, nsets), order="F")
+    for k in range(nsets):
+        m[:, 0, k] = g["multipoles"][:, 0]
+        m[:, 1:4, k] = rng.normal(0.0, 0.05, (n, 3))
+    tr = mixed.tree(x, 64, 0.5)
+    nn = len(tr["r"])
+    nm = (p + 1) * (p + 2) * (p + 3) // 6
+    deg = np.concatenate([[k] * ((k + 1) * (k + 2) // 2) for k in range(p + 1)])
+    mixed.field_fmm_sets(0, x, m, e, 0.5, 64, p, 1)
+    c64 = mixed.debug_buffer("e_cm", nn * nm * nsets).reshape(nsets, nn, nm)
+    mixed.set_precision(True)
+    mixed.field_fmm_sets(0, x, m, e, 0.5, 64, p, 1)
+    c32 = mixed.debug_buffer("e_cm", nn * nm * nsets).reshape(nsets, nn, nm)
+    leaf = tr["is_leaf"] != 0
+    np.testing.assert_array_equal(c32[:, leaf], c64[:, leaf])
+    scale = tr["r"][:, None] ** (-deg[None, :])            # box-scaled moments M(s) r^-|s|
+    for k in range(nsets):
+        a, b = c32[k][~leaf] * scale[~leaf], c64[k][~leaf] * scale[~leaf]
+        err = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+        assert err.max() < 2e-6 and err.max() > 1e-12, (k, err.max())
