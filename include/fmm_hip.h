@@ -250,11 +250,11 @@ enum fmmh_option {
                                      dipole iterations of a polarizable embedding, for which the reference
                                      rebuilds every call (src/fmm.F90:1387-1398).  Results are identical
                                      to a rebuild.  0 (default) drops the cache. */
-    FMMH_OPT_PRECISION = 5,       /* 0 (default): FP64 throughout.  1: mixed -- the multipole-to-local
-                                     translations of the class-grouped M2L run on FP32 MFMA with FP32
-                                     accumulation and an FP64 sum into the local expansions; P2M, L2P,
-                                     the near field (with its damped tensors) and the field stay FP64.
-                                     Results then differ from the reference at the FP32 level. */
+    FMMH_OPT_PRECISION = 5,       /* 0 (default): FP64 throughout.  1: mixed -- the M2M, M2L and L2L
+                                     translations run on FP32 MFMA with FP32 accumulation and FP64 sums
+                                     into the expansions; P2M, L2P, the near field (with its damped
+                                     tensors) and the field stay FP64.  Results then differ from the
+                                     reference at the FP32 level. */
     /* Learned per-pair expansion orders (no counterpart in the reference, whose M2L applies
      * expansion_order to every pair).  With a trained selector on the context
      * (fmmh_selector_train / fmmh_selector_set) and ORDER_TOL > 0, field_fmm picks an order
