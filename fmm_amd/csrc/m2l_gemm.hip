@@ -135,20 +135,35 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
 #pragma unroll
         for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = typename G::Acc{0, 0, 0, 0};
     const int* __restrict__ fx = a.fidx + lane;
-#pragma unroll 2
-    for (int ks = 0; ks < KSq; ++ks) {
-        const int s = ks * 4 + g4;
-        E bv[NTL];
+    // the fragment indices come from global memory (L2): KC source steps' worth are loaded together,
+    // so one wait covers KC x MW loads instead of one per step
+    constexpr int KC = 8;
+    for (int k0 = 0; k0 < KSq; k0 += KC) {
+        int fi[KC][MW];
 #pragma unroll
-        for (int nt = 0; nt < NTL; ++nt) bv[nt] = s < NM ? Bs[(nt * 16 + c16) * BLD + s] : E(0);
+        for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
-        for (int mi = 0; mi < MW; ++mi) {
-            const int mt = w + 4 * mi;
-            if (mt < MTq) {
-                const int f = fx[(mt * KS + ks) * 64];
-                const E av = (f >= 0 && mt * 16 + c16 < nq) ? E(Ts[f] * rs[mi]) : E(0);
+            for (int mi = 0; mi < MW; ++mi) {
+                const int ks = k0 + kk, mt = w + 4 * mi;
+                fi[kk][mi] = (ks < KSq && mt < MTq) ? fx[(mt * KS + ks) * 64] : -1;
+            }
 #pragma unroll
-                for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = G::mfma(av, bv[nt], acc[mi][nt]);
+        for (int kk = 0; kk < KC; ++kk) {
+            const int ks = k0 + kk;
+            if (ks >= KSq) break;
+            const int s = ks * 4 + g4;
+            E bv[NTL];
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) bv[nt] = s < NM ? Bs[(nt * 16 + c16) * BLD + s] : E(0);
+#pragma unroll
+            for (int mi = 0; mi < MW; ++mi) {
+                const int mt = w + 4 * mi;
+                if (mt < MTq) {
+                    const int f = fi[kk][mi];
+                    const E av = (f >= 0 && mt * 16 + c16 < nq) ? E(Ts[f] * rs[mi]) : E(0);
+#pragma unroll
+                    for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = G::mfma(av, bv[nt], acc[mi][nt]);
+                }
             }
         }
     }
