@@ -204,8 +204,16 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(const E* __restrict__ pbuf, 
             for (int e = e0; e < e1; ++e)
                 if (dt <= order[e]) sum += (double)pbuf[(int64_t)e * nm + t];
         } else {
-#pragma unroll 8
-            for (int e = e0; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nm + t];
+            // 16 slots loaded per batch (the adds stay in CSR order)
+            int e = e0;
+            for (; e + 16 <= e1; e += 16) {
+                E v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) v[u] = pbuf[(int64_t)(e + u) * nm + t];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) sum += (double)v[u];
+            }
+            for (; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nm + t];
         }
         double f = rinv;
         for (int k = 0; k < tdeg[t]; ++k) f *= rinv;
