@@ -94,15 +94,23 @@ __global__ __launch_bounds__(256) void k_m2l_assign(M2LClassArgs a) {
     if (k == a.n_pairs - 1) *a.num_classes = c + 1;
 }
 
-constexpr int CT_AUX = hermite_aux_size(HERMITE_KMAX);
 
 // One class tensor per workgroup iteration: T_k(v) for |k| <= 2p by the
 // Hermite recursion over degree levels (one barrier per level), scaled by
 // 2^{dl |k|} when the target is the deeper box (dl > 0).
+// LDS (dynamic, sized by the order): the recursion's auxiliary array, hermite_aux_size(2p) doubles,
+// then the packed-index exponent table (ncomp(2p) ints) and the level bases (2p + 2 ints), staged
+// once per workgroup.
 __global__ __launch_bounds__(64) void k_class_tensor(M2LClassArgs a) {
-    __shared__ double A[CT_AUX];
+    extern __shared__ double ct_lds[];
     const int K = 2 * a.p;
     const int nT = ncomp(K);
+    double* A = ct_lds;
+    int* sxyz = reinterpret_cast<int*>(ct_lds + hermite_aux_size(K));
+    int* slb = sxyz + nT;
+    for (int i = threadIdx.x; i < nT; i += 64) sxyz[i] = a.xyz[i];
+    for (int i = threadIdx.x; i < K + 2; i += 64) slb[i] = a.level_base[i];
+    __syncthreads();
     const int ncls = *a.num_classes;
     for (int c = blockIdx.x; c < ncls; c += gridDim.x) {
         const unsigned long long key = a.class_key[c];
@@ -127,11 +135,11 @@ __global__ __launch_bounds__(64) void k_class_tensor(M2LClassArgs a) {
             vy = (double)((long long)((key >> (VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
             vz = (double)((long long)((key >> 8) & ((1ull << VBITS) - 1)) - VOFF);
         }
-        hermite_aux(vx, vy, vz, K, A, a.xyz, a.level_base);
+        hermite_aux(vx, vy, vz, K, A, sxyz, slb);
         for (int idx = threadIdx.x; idx < nT; idx += 64) {
-            const int pk = a.xyz[idx];
+            const int pk = sxyz[idx];
             const int k = (pk & 255) + ((pk >> 8) & 255) + (pk >> 16);
-            double val = hermite_T(A, idx, a.xyz, a.level_base);
+            double val = hermite_T(A, idx, sxyz, slb);
             if (dl > 0) val = ldexp(val, dl * k);
             a.Tc[(int64_t)c * nT + idx] = val;
         }
@@ -317,7 +325,9 @@ void launch_gather_u64(const unsigned long long* in, const int* idx, unsigned lo
 
 void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s) {
     if (grid <= 0) return;
-    hipLaunchKernelGGL(k_class_tensor, dim3(grid), dim3(64), 0, s, a);
+    const int K = 2 * a.p;
+    const size_t lds = (size_t)hermite_aux_size(K) * sizeof(double) + (size_t)(ncomp(K) + K + 2) * sizeof(int);
+    hipLaunchKernelGGL(k_class_tensor, dim3(grid), dim3(64), lds, s, a);
     FMMH_HIP(hipGetLastError());
 }
 
