@@ -54,6 +54,28 @@ __device__ __forceinline__ void build_mono_block(double* mono, int ld, int nm, i
 }
 
 // LDS: the monomials of a chunk of sites, [UP_CHUNK][nm] doubles (dynamic, sized by the order).
+// sum over the table terms b0..b1 of coef[b] mono[e[b]] src[s[b]], in table order; the terms' table
+// entries are loaded eight at a time (one wait per batch instead of one per term).
+__device__ __forceinline__ double shift_sum(const ShiftTab& tab, int b0, int b1, const double* mono,
+                                            const double* src) {
+    double sum = 0.0;
+    int b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        double cf[8];
+        int ee[8], ss[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            cf[u] = tab.coef[b + u];
+            ee[u] = tab.e[b + u];
+            ss[u] = tab.s[b + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum = __builtin_fma(cf[u] * mono[ee[u]], src[ss[u]], sum);
+    }
+    for (; b < b1; ++b) sum = __builtin_fma(tab.coef[b] * mono[tab.e[b]], src[tab.s[b]], sum);
+    return sum;
+}
+
 __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
     extern __shared__ double mono[];
     __shared__ double mch[UP_CHUNK][UP_NSRC];
@@ -145,10 +167,7 @@ __global__ __launch_bounds__(256) void k_m2m(UpArgs a, const int* __restrict__ n
         if (kids[o] < 0) continue;
         const double* __restrict__ mono = monos + o * nm;
         const double* __restrict__ src = srcs + o * nm;
-        const int b0 = a.m2m.off[t], b1 = a.m2m.off[t + 1];
-        double sum = 0.0;
-        for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.m2m.coef[b] * mono[a.m2m.e[b]], src[a.m2m.s[b]], sum);
-        part[o * nm + t] = sum;
+        part[o * nm + t] = shift_sum(a.m2m, a.m2m.off[t], a.m2m.off[t + 1], mono, src);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nm; t += blockDim.x) {
@@ -177,12 +196,8 @@ __global__ __launch_bounds__(128) void k_l2l(DownArgs a, const int* __restrict__
     for (int s = threadIdx.x; s < nm; s += blockDim.x) lp[s] = LE[(int64_t)par * nm + s];
     __syncthreads();
     build_mono_block(mono, MAX_NM, nm, 1, d3, pw, a.xyz, a.p);
-    for (int t = threadIdx.x; t < nm; t += blockDim.x) {
-        const int b0 = a.l2l.off[t], b1 = a.l2l.off[t + 1];
-        double sum = 0.0;
-        for (int b = b0; b < b1; ++b) sum = __builtin_fma(a.l2l.coef[b] * mono[a.l2l.e[b]], lp[a.l2l.s[b]], sum);
-        LE[(int64_t)node * nm + t] += sum;
-    }
+    for (int t = threadIdx.x; t < nm; t += blockDim.x)
+        LE[(int64_t)node * nm + t] += shift_sum(a.l2l, a.l2l.off[t], a.l2l.off[t + 1], mono, lp);
 }
 
 // f_t = sum_e L_{t+e} d^e / (ex! ey! ez!), |t| <= F, |t + e| <= p, one lane per site, added to the near
