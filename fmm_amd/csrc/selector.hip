@@ -171,34 +171,52 @@ void launch_sel_sample(const SelSampleArgs& a, int m, hipStream_t s) {
 }
 
 // A_j = |(M_j(s) r_j^{-|s|})_s|_2 over the nm expansion components, max over sets.
-__global__ __launch_bounds__(256) void k_sel_node_norm(const double* __restrict__ CM, const double* __restrict__ r,
-                                                       const int* __restrict__ xyz, int num_nodes, int nm, int nsets,
-                                                       double* __restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per node: lane l takes the coefficients s = l, l + 64, ... of every set (their box
+// scale r^-|s| computed once), the squares of a set are summed across the wavefront in a fixed order,
+// and the node keeps the largest set norm.
+constexpr int NORM_PER_LANE = (ncomp(PMAX) + 63) / 64;
+
+__global__ __launch_bounds__(64) void k_sel_node_norm(const double* __restrict__ CM, const double* __restrict__ r,
+                                                      const int* __restrict__ xyz, int num_nodes, int nm, int nsets,
+                                                      double* __restrict__ out) {
+    const int j = blockIdx.x;
     if (j >= num_nodes) return;
+    const int lane = threadIdx.x;
     const double rinv = 1.0 / r[j];
+    double f[NORM_PER_LANE];
+#pragma unroll
+    for (int q = 0; q < NORM_PER_LANE; ++q) {
+        const int s = lane + 64 * q;
+        double v = 0.0;
+        if (s < nm) {
+            const int ps = xyz[s];
+            const int d = (ps & 255) + ((ps >> 8) & 255) + (ps >> 16);
+            v = 1.0;
+            for (int k = 0; k < d; ++k) v *= rinv;
+        }
+        f[q] = v;
+    }
     double best = 0.0;
     for (int set = 0; set < nsets; ++set) {
         const double* c = CM + ((int64_t)set * num_nodes + j) * nm;
-        double s2 = 0.0, f = 1.0;
-        int deg = 0;
-        for (int s = 0; s < nm; ++s) {
-            const int ps = xyz[s];
-            const int d = (ps & 255) + ((ps >> 8) & 255) + (ps >> 16);
-            while (deg < d) { f *= rinv; ++deg; }
-            const double v = c[s] * f;
-            s2 += v * v;
+        double s2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < NORM_PER_LANE; ++q) {
+            const int s = lane + 64 * q;
+            const double v = s < nm ? c[s] * f[q] : 0.0;
+            s2 = __builtin_fma(v, v, s2);
         }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) s2 += __shfl_xor(s2, d);
         best = fmax(best, s2);
     }
-    out[j] = sqrt(best);
+    if (lane == 0) out[j] = sqrt(best);
 }
 
 void launch_sel_node_norm(const double* CM, const double* r, const int* xyz, int num_nodes, int nm, int nsets,
                           double* out, hipStream_t s) {
     if (num_nodes <= 0) return;
-    hipLaunchKernelGGL(k_sel_node_norm, dim3((num_nodes + 255) / 256), dim3(256), 0, s, CM, r, xyz, num_nodes, nm,
-                       nsets, out);
+    hipLaunchKernelGGL(k_sel_node_norm, dim3(num_nodes), dim3(64), 0, s, CM, r, xyz, num_nodes, nm, nsets, out);
     FMMH_HIP(hipGetLastError());
 }
 
