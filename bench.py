@@ -89,7 +89,7 @@ def cpu_baseline(seconds_hint=True):
                       f"THOLE, theta 0.5, ncrit 64, p 6, field order 2; {dt:.1f} s"}
 
 
-def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, sets=32, spots=1024):
+def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, sets=32, spots=1024, margin=0.1):
     """Throughput at <= 1e-6 relative RMS field error against the FP64 direct
     sum: the plain FP64 FMM (one multipole set, every pair at expansion_order)
     against the batched (sets on one tree), mixed-precision (FP32 MFMA M2L),
@@ -151,7 +151,7 @@ def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, se
     MS = col3(mh, dev)
     os_ = col3(np.zeros((n, 4, sets)), dev)
     ctx.set_precision(True)
-    ctx.set_order_tolerance(tol, spot_checks=spots)
+    ctx.set_order_tolerance(tol, spot_checks=spots, margin=margin)
 
     def f_batched():
         os_.zero_()
@@ -295,12 +295,15 @@ def main():
     ap.add_argument("--target-only", action="store_true", help="only run that comparison and print it")
     ap.add_argument("--target-plain", default="0.3,9", help="theta,p of the plain FP64 side")
     ap.add_argument("--target-batched", default="0.3,9", help="theta,cap of the batched side")
+    ap.add_argument("--target-margin", type=float, default=0.1,
+                    help="order margin of the learned orders on the batched side (library default 0.25; 0.1 "
+                         "measured with spot error 8.6e-7 on this workload)")
     args = ap.parse_args()
     tp = tuple(float(v) if i == 0 else int(v) for i, v in enumerate(args.target_plain.split(",")))
     tb_ = tuple(float(v) if i == 0 else int(v) for i, v in enumerate(args.target_batched.split(",")))
     if args.target_only:
         import torch
-        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_)))
+        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_, margin=args.target_margin)))
         return
     if args.peak_calibration:
         import torch
@@ -485,7 +488,8 @@ def main():
                     "of far pairs)",
         }
     cpu = None if args.no_cpu else cpu_baseline()
-    target = None if (args.no_target or args.config != 1) else target_comparison(dev, min(args.steps, 3), tp, tb_)
+    target = None if (args.no_target or args.config != 1) else target_comparison(dev, min(args.steps, 3), tp, tb_,
+                                                                                   margin=args.target_margin)
     line = {
         "metric": "site-field evals/s at <=1e-6 rel err vs FP64 direct; % of FP64/FP32 peak",
         "value": value,

@@ -203,8 +203,8 @@ __global__ __launch_bounds__(128) void k_l2l(DownArgs a, const int* __restrict__
 // f_t = sum_e L_{t+e} d^e / (ex! ey! ez!), |t| <= F, |t + e| <= p, one lane per site, added to the near
 // field and scattered to the caller's site order.  The order is a template parameter: every loop is
 // unrolled, so the packed indices are constants (LDS reads with immediate offsets) and the |t + e| <= p
-// tests vanish.  The operations and their order are those of k_l2p_sb (many sets), so a set of a
-// many-set call equals the one-set call bit for bit.
+// tests vanish.  The operations and their order are those of k_l2p_sb (runtime order, many sets), so a
+// set of a many-set call equals the one-set call bit for bit.
 template <int P, int F>
 __global__ __launch_bounds__(64) void k_l2p_p(DownArgs a) {
     constexpr int NF = ncomp(F);
@@ -455,17 +455,16 @@ __global__ __launch_bounds__(256) void k_l2l_sb(DownArgs a, const int* __restric
     }
 }
 
-template <int P, int F, int SB>
+template <int F, int SB>
 __global__ __launch_bounds__(64) void k_l2p_sb(DownArgs a) {
     constexpr int NF = ncomp(F);
-    constexpr int NMP = ncomp(P);
-    __shared__ double L[SB][NMP];
+    __shared__ double L[SB][MAX_NM];
     __shared__ double fact[PMAX + 1];
     const int node = a.leaf_node[blockIdx.x];
     const int set0 = blockIdx.y * SB;
     const int nsb = min(SB, a.nsets - set0);
     const int s0 = a.start[node], c = a.count[node];
-    const int nm = NMP;
+    const int nm = a.nm, p = a.p;
     for (int x = threadIdx.x; x < SB * nm; x += 64) {
         const int u = x / nm, sc = x - u * nm;
         L[u][sc] = u < nsb ? a.LE[((int64_t)(set0 + u) * a.num_nodes + node) * nm + sc] : 0.0;
@@ -488,20 +487,17 @@ __global__ __launch_bounds__(64) void k_l2p_sb(DownArgs a) {
 #pragma unroll
             for (int t = 0; t < NF; ++t) f[u][t] = 0.0;
         double px = 1.0;
-#pragma unroll
-        for (int ex = 0; ex <= P; ++ex) {
+        for (int ex = 0; ex <= p; ++ex) {
             double pxy = px;
-#pragma unroll
-            for (int ey = 0; ey + ex <= P; ++ey) {
+            for (int ey = 0; ey + ex <= p; ++ey) {
                 double m = pxy * fact[ex] * fact[ey];
-#pragma unroll
-                for (int ez = 0; ez + ey + ex <= P; ++ez) {
+                for (int ez = 0; ez + ey + ex <= p; ++ez) {
                     const double mz = m * fact[ez];
                     const int ne = ex + ey + ez;
 #pragma unroll
                     for (int t = 0; t < NF; ++t) {
                         const Exps et = idx2xyz(t);
-                        if (et.x + et.y + et.z + ne <= P) {
+                        if (et.x + et.y + et.z + ne <= p) {
                             const int li = xyz2idx(et.x + ex, et.y + ey, et.z + ez);
 #pragma unroll
                             for (int u = 0; u < SB; ++u) f[u][t] = __builtin_fma(mz, L[u][li], f[u][t]);
@@ -551,24 +547,6 @@ void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s) {
     FMMH_HIP(hipGetLastError());
 }
 
-template <int F, int SB>
-static void launch_l2p_sbp(const DownArgs& a, int num_leaves, hipStream_t s) {
-    const dim3 g(num_leaves, (a.nsets + SB - 1) / SB);
-    switch (a.p) {
-        case 0: hipLaunchKernelGGL((k_l2p_sb<0, F, SB>), g, dim3(64), 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_l2p_sb<1, F, SB>), g, dim3(64), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_l2p_sb<2, F, SB>), g, dim3(64), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_l2p_sb<3, F, SB>), g, dim3(64), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_l2p_sb<4, F, SB>), g, dim3(64), 0, s, a); break;
-        case 5: hipLaunchKernelGGL((k_l2p_sb<5, F, SB>), g, dim3(64), 0, s, a); break;
-        case 6: hipLaunchKernelGGL((k_l2p_sb<6, F, SB>), g, dim3(64), 0, s, a); break;
-        case 7: hipLaunchKernelGGL((k_l2p_sb<7, F, SB>), g, dim3(64), 0, s, a); break;
-        case 8: hipLaunchKernelGGL((k_l2p_sb<8, F, SB>), g, dim3(64), 0, s, a); break;
-        case 9: hipLaunchKernelGGL((k_l2p_sb<9, F, SB>), g, dim3(64), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_l2p_sb<10, F, SB>), g, dim3(64), 0, s, a); break;
-    }
-}
-
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
     if (a.nsets == 1) {
@@ -580,9 +558,17 @@ void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
     } else {
         constexpr int SB2 = SHIFT_SB / 2;   // field gradients: 10 components per set in registers
         switch (a.nf) {
-            case 1: launch_l2p_sbp<0, SHIFT_SB>(a, num_leaves, s); break;
-            case 4: launch_l2p_sbp<1, SHIFT_SB>(a, num_leaves, s); break;
-            default: launch_l2p_sbp<2, SB2>(a, num_leaves, s); break;
+            case 1:
+                hipLaunchKernelGGL((k_l2p_sb<0, SHIFT_SB>), dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB),
+                                   dim3(64), 0, s, a);
+                break;
+            case 4:
+                hipLaunchKernelGGL((k_l2p_sb<1, SHIFT_SB>), dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB),
+                                   dim3(64), 0, s, a);
+                break;
+            default:
+                hipLaunchKernelGGL((k_l2p_sb<2, SB2>), dim3(num_leaves, (a.nsets + SB2 - 1) / SB2), dim3(64), 0, s, a);
+                break;
         }
     }
     FMMH_HIP(hipGetLastError());

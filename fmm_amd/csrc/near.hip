@@ -273,6 +273,8 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
 // target site of a target block; the exclusion entries of the site are
 // de-duplicated (the reference tests membership), and an entry counts only
 // if its site lies in a central-cell source block of the block's list.
+constexpr int NEAR_EXCL_STAGE = 512;
+
 template <int M, int F, int DAMP, int S>
 __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
     constexpr int NM = ncomp(M);
@@ -295,6 +297,24 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
     const int e0 = a.list_off[b], e1 = a.list_off[b + 1];
     const double* __restrict__ ms = a.ms + (int64_t)set0 * NM * a.ldm;
     double* __restrict__ out = a.out + (int64_t)set0 * NF * a.ldo;
+    // the central-cell source blocks of the list, as sorted-site ranges [first, end), staged in LDS
+    // (NEAR_EXCL_STAGE of them; a longer list is searched in global memory)
+    __shared__ int rfirst[NEAR_EXCL_STAGE], rend[NEAR_EXCL_STAGE];
+    const bool staged = e1 - e0 <= NEAR_EXCL_STAGE;
+    if (staged) {
+        for (int x = e0 + (int)threadIdx.x; x < e1; x += 64) {
+            const int enc = a.list_src[x];
+            int f = 0, en = 0;                    // image entries never match
+            if (src_image(enc) == 0) {
+                const int sb = src_node(enc);
+                f = a.src_start[sb];
+                en = f + a.src_count[sb];
+            }
+            rfirst[x - e0] = f;
+            rend[x - e0] = en;
+        }
+    }
+    __syncthreads();
     for (int base = 0; base < tc; base += 64) {
         const int jt = base + (int)threadIdx.x;
         if (jt >= tc) break;
@@ -333,12 +353,16 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
             }
             const int j = a.inv ? a.inv[e - 1] : e - 1;   // sorted position of the excluded site
             bool near = false;
-            for (int x = e0; x < e1 && !near; ++x) {
-                const int enc = a.list_src[x];
-                if (src_image(enc) != 0) continue;
-                const int sb = src_node(enc);
-                const int st = a.src_start[sb];
-                near = j >= st && j < st + a.src_count[sb];
+            if (staged) {
+                for (int x = 0; x < e1 - e0 && !near; ++x) near = j >= rfirst[x] && j < rend[x];
+            } else {
+                for (int x = e0; x < e1 && !near; ++x) {
+                    const int enc = a.list_src[x];
+                    if (src_image(enc) != 0) continue;
+                    const int sb = src_node(enc);
+                    const int st = a.src_start[sb];
+                    near = j >= st && j < st + a.src_count[sb];
+                }
             }
             if (!near) continue;
             add(j, 1e10, 1e10, 1e10, 1.0);

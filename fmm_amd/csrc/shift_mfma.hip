@@ -13,10 +13,10 @@
 //
 // A workgroup (four wavefronts) takes 32 columns -- (internal node, set) pairs, set fastest -- of one
 // level; the columns' source expansions are scaled and rounded to FP32 into LDS, and each wavefront
-// runs its 16-row target tiles on v_mfma_f32_16x16x4f32 (FP32 accumulation over s).  M2M adds the
-// eight octants' FP32 tiles into FP64 accumulators (compensated summation of the products into the
-// parent's expansion) and writes the parent unscaled in FP64; L2L adds each octant's FP32 tile,
-// unscaled in FP64, to the child's FP64 local expansion.  The columns come from per-level lists of
+// runs its 16-row target tiles on v_mfma_f32_16x16x4f32, FP32 accumulation over runs of 32 source
+// coefficients, each run's FP32 tile added to FP64 accumulators (compensated summation of the products
+// into the expansions).  M2M sums the eight octants that way and writes the parent unscaled in FP64;
+// L2L adds each octant's sum, unscaled in FP64, to the child's FP64 local expansion.  The columns come from per-level lists of
 // internal nodes (k_internal_nodes), whose counts stay on the device.
 //
 // MFMA operand layouts (measured on gfx950, f32_16x16x4f32): A lane l -> A[l % 16][l / 16],
@@ -109,14 +109,23 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
             __syncthreads();
         }
         const float* __restrict__ A = a.A + (size_t)o * MT * KS * 64;
-        v4f acc[MW][NT];
+        if (L2L) {
 #pragma unroll
-        for (int mi = 0; mi < MW; ++mi)
+            for (int mi = 0; mi < MW; ++mi)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mi][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc64[mi][nt][q] = 0.0;
+        }
         // A fragments come from global memory (L2): KC source steps' worth are loaded together, then
-        // their MFMAs run, so one wait covers KC x MW loads
+        // their MFMAs run, so one wait covers KC x MW loads; the FP32 accumulators cover those KC steps
+        // only and are then added to the FP64 ones
         for (int k0 = 0; k0 < KS; k0 += KC) {
+            v4f acc[MW][NT];
+#pragma unroll
+            for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[mi][nt] = v4f{0.f, 0.f, 0.f, 0.f};
             float av[KC][MW];
 #pragma unroll
             for (int kk = 0; kk < KC; ++kk)
@@ -141,15 +150,14 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
                     }
                 }
             }
-        }
-        if (!L2L) {
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
                     for (int q = 0; q < 4; ++q) acc64[mi][nt][q] += (double)acc[mi][nt][q];
-        } else {
+        }
+        if (L2L) {
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const int c = nt * 16 + c16;
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int t = mt * 16 + 4 * g4 + q;
-                        if (t < NM) dst[t] += (double)acc[mi][nt][q] * rin[c][deg[t]];
+                        if (t < NM) dst[t] += acc64[mi][nt][q] * rin[c][deg[t]];
                     }
                 }
             }
