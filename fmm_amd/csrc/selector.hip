@@ -614,6 +614,8 @@ void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, const int
 // field of those sources.  The near list of the leaf is sorted by node id,
 // which is also sorted-site order, so membership is a binary search over
 // its site ranges.
+constexpr int SPOT_STAGE = 1024;
+
 template <int M>
 __global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
     constexpr int NM = ncomp(M);
@@ -624,19 +626,41 @@ __global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
     const int site = a.start[node];
     const double x = a.xs[site], y = a.ys[site], z = a.zs[site];
     const int e0 = a.p2p_off[node], e1 = a.p2p_off[node + 1];
+    // the near source leaves' site ranges [first, end), ascending (the list is in pre-order, which is
+    // the sorted-site order), staged in LDS when they fit
+    __shared__ int rfirst[SPOT_STAGE], rend[SPOT_STAGE];
+    const int ne = e1 - e0;
+    const bool staged = ne <= SPOT_STAGE;
+    if (staged)
+        for (int x = threadIdx.x; x < ne; x += blockDim.x) {
+            const int sn = src_node(a.p2p_src[e0 + x]);
+            rfirst[x] = a.start[sn];
+            rend[x] = a.start[sn] + a.count[sn];
+        }
+    __syncthreads();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int jj = threadIdx.x; jj < a.n; jj += blockDim.x) {
-        // is jj inside the site range of a near source leaf?
-        int lo = e0, hi = e1;   // last entry with start <= jj
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (a.start[src_node(a.p2p_src[mid])] <= jj) lo = mid;
-            else hi = mid;
-        }
+        // is jj inside the site range of a near source leaf?  (last range with first <= jj)
         bool near = false;
-        if (lo < e1) {
-            const int sn = src_node(a.p2p_src[lo]);
-            near = jj >= a.start[sn] && jj < a.start[sn] + a.count[sn];
+        if (staged) {
+            int lo = 0, hi = ne;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (rfirst[mid] <= jj) lo = mid;
+                else hi = mid;
+            }
+            near = ne > 0 && jj >= rfirst[lo] && jj < rend[lo];
+        } else {
+            int lo = e0, hi = e1;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (a.start[src_node(a.p2p_src[mid])] <= jj) lo = mid;
+                else hi = mid;
+            }
+            if (lo < e1) {
+                const int sn = src_node(a.p2p_src[lo]);
+                near = jj >= a.start[sn] && jj < a.start[sn] + a.count[sn];
+            }
         }
         if (near) continue;
         const double dx = x - a.xs[jj], dy = y - a.ys[jj], dz = z - a.zs[jj];
