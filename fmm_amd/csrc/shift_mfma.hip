@@ -34,6 +34,11 @@ __global__ void k_fill_cnt(int* c, int n) {
 
 constexpr int SHIFT_NT = 2;   // 16-column tiles per workgroup (each A fragment feeds SHIFT_NT MFMAs)
 constexpr int KC = 8;         // source steps whose A fragments are loaded together
+// Row-tile groups: G workgroups split a column block's 16-row target tiles (at most two per wavefront), so
+// that the top levels, with a handful of nodes, do not serialise all tiles of all octants on four waves;
+// L2L also takes one octant per workgroup.  Every output element is still computed by one wavefront in
+// the same order.
+constexpr int shift_row_groups(int p) { return (((ncomp(p) + 15) / 16) + 7) / 8; }
 
 template <int P, bool L2L>
 __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
@@ -41,7 +46,8 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
     constexpr int MT = (NM + 15) / 16;          // 16-row target tiles
     constexpr int KS = (NM + 3) / 4;            // 4-deep source steps
     constexpr int KP = KS * 4;
-    constexpr int MW = (MT + 3) / 4;            // target tiles per wavefront
+    constexpr int G = shift_row_groups(P);      // workgroups sharing the row tiles of one column block
+    constexpr int MW = (MT + 4 * G - 1) / (4 * G);   // target tiles per wavefront
     constexpr int NT = SHIFT_NT;
     constexpr int NC = 16 * NT;                 // columns per workgroup
     __shared__ float Bs[NC][KP + 1];
@@ -55,6 +61,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
     if (col0 >= ncol) return;                   // the grid is sized for every node of the level
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int c16 = lane & 15, g4 = lane >> 4;
+    const int mrow = 4 * (int)blockIdx.y + w;   // this wavefront's first row tile; then every 4 G-th
     if (tid < NC) {
         const int col = col0 + tid;
         const int node = col < ncol ? a.nodes[col / a.nsets] : -1;
@@ -102,9 +109,10 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
         stage(0);
         __syncthreads();
     }
-    for (int o = 0; o < 8; ++o) {
+    const int o_first = L2L ? (int)blockIdx.z : 0, o_end = L2L ? o_first + 1 : 8;
+    for (int o = o_first; o < o_end; ++o) {
         if (!L2L) {
-            if (o > 0) __syncthreads();          // readers of the previous octant's columns are done
+            if (o > o_first) __syncthreads();    // readers of the previous octant's columns are done
             stage(o);
             __syncthreads();
         }
@@ -131,7 +139,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
             for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
                 for (int mi = 0; mi < MW; ++mi) {
-                    const int ks = k0 + kk, mt = w + 4 * mi;
+                    const int ks = k0 + kk, mt = mrow + 4 * G * mi;
                     av[kk][mi] = (ks < KS && mt < MT) ? A[((size_t)mt * KS + ks) * 64 + lane] : 0.f;
                 }
 #pragma unroll
@@ -143,7 +151,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
                 for (int nt = 0; nt < NT; ++nt) b[nt] = Bs[nt * 16 + c16][ks * 4 + g4];
 #pragma unroll
                 for (int mi = 0; mi < MW; ++mi) {
-                    if (w + 4 * mi < MT) {
+                    if (mrow + 4 * G * mi < MT) {
 #pragma unroll
                         for (int nt = 0; nt < NT; ++nt)
                             acc[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk][mi], b[nt], acc[mi][nt], 0, 0, 0);
@@ -167,7 +175,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
                 double* __restrict__ dst = a.dst + (int64_t)cset[c] * a.set_stride + (int64_t)ch * NM;
 #pragma unroll
                 for (int mi = 0; mi < MW; ++mi) {
-                    const int mt = w + 4 * mi;
+                    const int mt = mrow + 4 * G * mi;
                     if (mt >= MT) continue;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
@@ -187,7 +195,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
             double* __restrict__ dst = a.dst + (int64_t)cset[c] * a.set_stride + (int64_t)node * NM;
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi) {
-                const int mt = w + 4 * mi;
+                const int mt = mrow + 4 * G * mi;
                 if (mt >= MT) continue;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -241,7 +249,8 @@ int shift_mfma_frag_floats(int p) {
 
 template <bool L2L>
 static void launch_p(const ShiftMfmaArgs& a, int p, hipStream_t s) {
-    const dim3 g((unsigned)(((long long)a.max_count * a.nsets + 16 * SHIFT_NT - 1) / (16 * SHIFT_NT)));
+    const dim3 g((unsigned)(((long long)a.max_count * a.nsets + 16 * SHIFT_NT - 1) / (16 * SHIFT_NT)),
+                 shift_row_groups(p), L2L ? 8 : 1);
     switch (p) {
         case 1: hipLaunchKernelGGL((k_shift_mfma<1, L2L>), g, dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL((k_shift_mfma<2, L2L>), g, dim3(256), 0, s, a); break;
