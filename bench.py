@@ -167,7 +167,7 @@ def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, se
     v_p = n / (ms_p / 1e3)
     v_b = n * sets / (ms_b / 1e3)
     return {
-        "workload": "water-200k-undamped-dipole (configs[1] geometry, no damping)",
+        "workload": "water-200k-undamped-dipole (the default water geometry, no damping)",
         "error_metric": "relative RMS of the field block vs FP64 field_direct on 4096 seeded sites",
         "plain_fp64": {"theta": th_p, "expansion_order": p_p, "sets": 1, "ms_per_call": ms_p, "value": v_p,
                        "error": err_p},
@@ -184,10 +184,10 @@ def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, se
 
 
 def bench_uneven(dev, args):
-    """64 water boxes of 10k-500k sites, 8 multipole sets each, THOLE, field order 1,
+    """Uneven batch: 64 water boxes of 10k-500k sites, 8 multipole sets each, THOLE, field order 1,
     learned M2L orders (cap 10, tolerance 1e-6, 1,024 spot checks per system), mixed precision.  One
     step = one field_fmm_sets call per system, inputs resident in HBM; the selector is trained on a
-    1/20-scale box of the idx-1 recipe (time reported, not counted)."""
+    1/20-scale box of the default water recipe (time reported, not counted)."""
     import torch
     from fmm_amd import synth
     from fmm_amd.octree import Context

@@ -148,11 +148,11 @@ def config(idx, scale=1.0):
                     exclusions=self_exclusions(n), theta=0.6, ncrit=64, expansion_order=8,
                     field_order=1, damp_type="ERF", damping_factors=df, sets=1)
     if idx == 4:
-        # An uneven batch: 64 water boxes built as for idx 1, sizes log-uniform between 10k and 500k sites
-        # (times scale), THOLE, 8 multipole sets each (the water charges plus per-set dipoles N(0, 0.05)),
-        # field order 1, learned M2L orders within a cap of 10 at a 1e-6 tolerance.  theta is not given for
-        # this workload: at idx 1's 0.5 even order 10 leaves ~3e-5 of far-field truncation error, and at 0.3
-        # the largest THOLE systems still miss 1e-6 at order 10 (spot errors 1.0e-6 at 211k and 2.2e-6 at 494k
+        # An uneven batch: 64 water boxes built as the default water box, sizes log-uniform between 10k and
+        # 500k sites (times scale), THOLE, 8 multipole sets each (the water charges plus per-set dipoles
+        # N(0, 0.05)), field order 1, learned M2L orders within a cap of 10 at a 1e-6 tolerance.  theta is not
+        # given for this workload: at the default water box's 0.5 even order 10 leaves ~3e-5 of far-field
+        # truncation error, and at 0.3 the largest THOLE systems still miss 1e-6 at order 10 (spot errors 1.0e-6 at 211k and 2.2e-6 at 494k
         # sites: damping shrinks the near field the error is relative to), so a 1e-6 tolerance could only fall
         # back there; 0.25 meets it on every system with learned orders (mean order 7-8).
         rng = np.random.default_rng(seed)

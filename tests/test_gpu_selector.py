@@ -324,7 +324,7 @@ def test_learned_orders_with_many_sets(trained):
 
 
 def test_uneven_batch_of_systems_meets_tolerance():
-    """configs[4] at 1/50 scale: water systems of uneven size, 8 THOLE dipole
+    """The uneven batch at 1/50 scale: water systems of uneven size, 8 THOLE dipole
     sets each, learned orders (cap 10, tolerance 1e-6, spot checks) in mixed
     precision with a selector trained at the batch's theta.  Every system
     either passes its spot check, with every set within 2e-6 of the same call
