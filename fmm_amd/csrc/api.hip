@@ -745,7 +745,12 @@ const double* upload_disp(fmmh_context* c, const Periodic& P) {
 }
 constexpr int TRAVERSE_GRID = 1024;           // workgroups per traversal step (grid-stride over the frontier)
 constexpr double M2L_GEMM_MAX_BYTES = 48e9;   // per-pair slot buffer of the class-grouped M2L
-constexpr int M2L_MFMA_MIN_SETS = 6;          // auto engine: per-target MFMA from this many sets on
+constexpr int M2L_MFMA_MIN_SETS = 6;
+// Mixed precision: a level's M2M runs on FP32 MFMA when it has at least this many (node, set) columns;
+// the top levels, with a handful of nodes, are bound by one workgroup's latency (eight octants in turn)
+// and run the FP64 kernel instead (faster there, and exact).  L2L, one octant per workgroup, always runs
+// on FP32 MFMA.
+constexpr long long SHIFT_MFMA_MIN_COLS = 4096;          // auto engine: per-target MFMA from this many sets on
 
 struct Call {
     const double* X; int64_t ldx;
@@ -1590,7 +1595,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     }
     for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
         int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
-        if (shift32) {
+        if (shift32 && (long long)(o1 - o0) * S >= SHIFT_MFMA_MIN_COLS) {
             ShiftMfmaArgs sa{inodes + o0, icnt + lev, o1 - o0, tr.t.children, tr.t.r, CM, CM, (int64_t)nn * nm, S,
                              tb.m2m_frag, tb.xyz};
             launch_shift_mfma(sa, p, false, s);
@@ -1626,7 +1631,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
                 tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S, tb.xyz};
     auto downward = [&]() {
         for (int lev = 2; lev <= tr.max_depth; ++lev) {
-            if (shift32) {   // the parents of this level's nodes, on FP32 MFMA
+            if (shift32) {   // the parents of this level's nodes, on FP32 MFMA (one octant per workgroup)
                 int o0 = tr.level_off[lev - 1], o1 = tr.level_off[lev];
                 ShiftMfmaArgs sa{inodes + o0, icnt + lev - 1, o1 - o0, tr.t.children, tr.t.r, LE, LE,
                                  (int64_t)nn * nm, S, tb.l2l_frag, tb.xyz};

@@ -70,18 +70,19 @@ def test_mixed_per_target_mfma_for_many_sets(mixed, p):
         assert 1e-12 < d < 2e-6, (k, d)
 
 
-@pytest.mark.parametrize("p,nsets", [(3, 1), (6, 1), (10, 1), (6, 20)])
+@pytest.mark.parametrize("p,nsets", [(3, 8), (6, 8), (10, 8), (6, 20)])
 def test_mixed_m2m_moments_close_to_fp64(mixed, p, nsets):
     """The FP32 M2M (box-scaled shift matrices per octant) against the FP64
-    one: leaf moments (P2M, FP64 in both) are identical, every internal
-    node's moments agree to FP32 level relative to the node's scaled norm."""
-    g = load_golden("water999_dipole_orders.npz")
-    x, e = g["coordinates"], g["exclusions"]
+    one on a 30k-site water box: leaf moments (P2M, FP64 in both) are
+    identical; the levels with few (node, set) columns keep the FP64 kernel
+    (identical too); every other internal node agrees to FP32 level relative
+    to the node's scaled norm, and some do differ (the FP32 form ran)."""
+    from fmm_amd import synth
+    x, q, e, _, rng = synth.water_box(10000, 7 + p)
     n = x.shape[0]
-    rng = np.random.default_rng(7 + p)
     m = np.zeros((n, 4, nsets), order="F")
     for k in range(nsets):
-        m[:, 0, k] = g["multipoles"][:, 0]
+        m[:, 0, k] = q
         m[:, 1:4, k] = rng.normal(0.0, 0.05, (n, 3))
     tr = mixed.tree(x, 64, 0.5)
     nn = len(tr["r"])
@@ -95,7 +96,10 @@ def test_mixed_m2m_moments_close_to_fp64(mixed, p, nsets):
     leaf = tr["is_leaf"] != 0
     np.testing.assert_array_equal(c32[:, leaf], c64[:, leaf])
     scale = tr["r"][:, None] ** (-deg[None, :])            # box-scaled moments M(s) r^-|s|
+    worst = 0.0
     for k in range(nsets):
         a, b = c32[k][~leaf] * scale[~leaf], c64[k][~leaf] * scale[~leaf]
         err = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
-        assert err.max() < 2e-6 and err.max() > 1e-12, (k, err.max())
+        assert err.max() < 2e-6, (k, err.max())
+        worst = max(worst, err.max())
+    assert worst > 1e-12
