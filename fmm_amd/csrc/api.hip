@@ -310,7 +310,9 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     if (it != c->tables.end()) return it->second;
     const int nm = ncomp(p);
     const int K = 2 * p;
-    std::vector<XYZ> comp = packed_components(std::max(K, m));
+    // the packed-index table xyz also serves the field components of L2P (orders <= NEAR_FMAX)
+    const int KX = std::max(K, NEAR_FMAX);
+    std::vector<XYZ> comp = packed_components(std::max(KX, m));
     auto idx = [](int x, int y, int z) { return xyz2idx(x, y, z); };
     std::vector<int> ints;
     std::vector<double> dbl;
@@ -356,8 +358,8 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     }
     std::vector<int> level_base(K + 2, 0);
     for (int k = 0; k <= K; ++k) level_base[k + 1] = level_base[k] + ndeg(k) * (K - k + 1);
-    std::vector<int> xyz(ncomp(K));
-    for (int i = 0; i < ncomp(K); ++i) xyz[i] = comp[i].x | (comp[i].y << 8) | (comp[i].z << 16);
+    std::vector<int> xyz(ncomp(KX));
+    for (int i = 0; i < ncomp(KX); ++i) xyz[i] = comp[i].x | (comp[i].y << 8) | (comp[i].z << 16);
     std::vector<double> cs(nm), scale(ncomp(m));
     for (int s = 0; s < nm; ++s) cs[s] = taylor_factor(comp[s]);
     for (int s = 0; s < ncomp(m); ++s) scale[s] = taylor_factor(comp[s]);
@@ -705,6 +707,23 @@ __global__ void k_fill_i32(int* a, int v, int n) {
     if (i < n) a[i] = v;
 }
 
+// Device flag of exclusion entries above n (only the device-array entry points can pass them; the host
+// entry points check the entries before copying).  Cleared by a kernel, so that a captured call clears it
+// on every replay; read back with the call's last synchronisation.
+constexpr int PIN_BAD_EXCL = 960;   // int slot of the pinned readback area
+int* bad_excl_flag(fmmh_context* c) {
+    int* f = c->buf<int>("x_bad_excl", 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, f, 0, 1);
+    return f;
+}
+void bad_excl_readback(fmmh_context* c) {
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + PIN_BAD_EXCL, c->buf<int>("x_bad_excl", 1), sizeof(int),
+                            hipMemcpyDeviceToHost, c->stream));
+}
+void bad_excl_check(fmmh_context* c) {   // after the stream synchronisation that follows the readback
+    if (c->h_pinned[PIN_BAD_EXCL]) throw Error(FMMH_ERR_ARG, "exclusion index outside 0..n");
+}
+
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
 // Periodic images (src/fmm.F90:156-175): image id 0 is the central cell, ids
@@ -776,7 +795,8 @@ void check_common(const Call& c, int64_t n_damping) {
     if (!near_supported(m, c.F))
         throw Error(FMMH_ERR_UNSUPPORTED, "multipole order " + std::to_string(m) + " / field order " +
                                               std::to_string(c.F) + " outside the compiled range (<= " +
-                                              std::to_string(NEAR_MMAX) + " / <= " + std::to_string(NEAR_FMAX) + ")");
+                                              std::to_string(NEAR_MMAX) + " / <= " + std::to_string(NEAR_FMAX) +
+                                              ", sum <= " + std::to_string(NEAR_KMAX) + ")");
 }
 
 void check_exclusions_host(const Call& c) {
@@ -821,9 +841,14 @@ void run_direct(fmmh_context* c, const Call& k) {
     c->mark("gather");
     NearArgs a{iota, start, count, loff, lsrc, s_start, s_count, xs, ys, zs, ms, n, dfs, iota, excl, n, k.nexcl,
                k.field, k.ldf, 1, k.nsets, upload_disp(c, Periodic{})};
+    a.nsites = n;
+    a.bad = bad_excl_flag(c);
+    a.coinc_list = 1;
     launch_near(a, nb, m, k.F, k.damp, s);
     c->mark("direct");
+    bad_excl_readback(c);
     FMMH_HIP(hipStreamSynchronize(s));
+    bad_excl_check(c);
 }
 
 // Direct sum at a subset of target sites (spot checks at sizes where the
@@ -887,12 +912,17 @@ void run_direct_targets(fmmh_context* c, const Call& k, const int* targets, int 
     launch_gather(g, s);
     NearArgs a{iota, start, count, loff, lsrc, s_start, s_count, xs, ys, zs, ms, ntot, dfs, perm, excl, ntot,
                k.nexcl, out, ntot, 0, 1, upload_disp(c, Periodic{})};
+    a.nsites = n;
+    a.bad = bad_excl_flag(c);
+    a.coinc_list = 1;
     launch_near(a, nb, m, k.F, k.damp, s);
     hipLaunchKernelGGL(k_add_rows, dim3(nblk(nt, 256)), dim3(256), 0, s, out, (int64_t)ntot, n, k.field, k.ldf, nt,
                        nf);
     FMMH_HIP(hipGetLastError());
     c->mark("direct_targets");
+    bad_excl_readback(c);
     FMMH_HIP(hipStreamSynchronize(s));
+    bad_excl_check(c);
 }
 
 // ---------------------------------------------------------------- tree
@@ -1350,10 +1380,10 @@ const unsigned char* select_orders(fmmh_context* c, TreeOut& tr, ListsOut& L, Ta
     const int nn = tr.num_nodes, np = L.n_m2l, nm = ncomp(p);
     double* node_norm = c->buf<double>("sel_node_norm", nn);
     launch_sel_node_norm(CM, tr.t.r, tb.xyz, nn, nm, nsets, node_norm, s);
-    double* acc = c->buf<double>("sel_acc", 4);
-    FMMH_HIP(hipMemsetAsync(acc, 0, 4 * sizeof(double), s));
-    launch_sel_budget(near, n, n, nf, tr.t.leaf_node, tr.t.count, tr.t.parent, L.m2l_off, tr.num_leaves, acc,
-                      c->order_tol * c->sel_kappa * c->sel_adapt, acc + 2, s);
+    double* acc = c->buf<double>("sel_acc", 4 + (size_t)nsets);
+    FMMH_HIP(hipMemsetAsync(acc, 0, (4 + (size_t)nsets) * sizeof(double), s));
+    launch_sel_budget(near, n, n, nf, nsets, tr.t.leaf_node, tr.t.count, tr.t.parent, L.m2l_off, tr.num_leaves,
+                      acc, c->order_tol * c->sel_kappa * c->sel_adapt, s);
     unsigned char* order = c->buf<unsigned char>("sel_order", np);
     unsigned long long* hist = c->buf<unsigned long long>("sel_hist", PMAX + 1);
     FMMH_HIP(hipMemsetAsync(hist, 0, (PMAX + 1) * sizeof(unsigned long long), s));
@@ -1367,36 +1397,50 @@ const unsigned char* select_orders(fmmh_context* c, TreeOut& tr, ListsOut& L, Ta
 
 // Relative RMS error (field block; the potential for field order 0) of the
 // far field at spot sites against the direct sum with the reference's far
-// semantics.  FMM far field at a site = field - field before L2P - near.
+// semantics, per multipole set; out[0] = the largest over the sets (sets
+// without field count 0).  FMM far field at a site = field - field before
+// L2P - near.  One workgroup, the sets in turn.
 __global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__ dir, const int* __restrict__ site,
                                                       const int* __restrict__ perm, const double* __restrict__ near,
                                                       int64_t ldn, const double* __restrict__ field, int64_t ldf,
                                                       const double* __restrict__ before, int64_t ldb, int nspots,
-                                                      int nf, double* __restrict__ out) {
+                                                      int nf, int nsets, double* __restrict__ out) {
     __shared__ double red[2][256];
-    double e2 = 0.0, f2 = 0.0;
     const int c0 = nf >= 4 ? 1 : 0, c1 = nf >= 4 ? 4 : 1;
-    for (int sp = threadIdx.x; sp < nspots; sp += 256) {
-        const int k = site[sp], o = perm[k];
-        for (int c = c0; c < c1; ++c) {
-            const double nr = near[(int64_t)c * ldn + k];
-            const double ffar = field[(int64_t)c * ldf + o] - before[(int64_t)c * ldb + o] - nr;
-            const double d = dir[(int64_t)sp * 4 + c];
-            e2 += (ffar - d) * (ffar - d);
-            f2 += (nr + d) * (nr + d);
+    double worst = 0.0;
+    for (int set = 0; set < nsets; ++set) {
+        const double* __restrict__ nr = near + (int64_t)set * nf * ldn;
+        const double* __restrict__ fl = field + (int64_t)set * nf * ldf;
+        const double* __restrict__ bf = before + (int64_t)set * nf * ldb;
+        const double* __restrict__ dr = dir + (int64_t)set * nspots * 4;
+        double e2 = 0.0, f2 = 0.0;
+        for (int sp = threadIdx.x; sp < nspots; sp += 256) {
+            const int k = site[sp], o = perm[k];
+            for (int c = c0; c < c1; ++c) {
+                const double nv = nr[(int64_t)c * ldn + k];
+                const double ffar = fl[(int64_t)c * ldf + o] - bf[(int64_t)c * ldb + o] - nv;
+                const double d = dr[(int64_t)sp * 4 + c];
+                e2 += (ffar - d) * (ffar - d);
+                f2 += (nv + d) * (nv + d);
+            }
         }
-    }
-    red[0][threadIdx.x] = e2;
-    red[1][threadIdx.x] = f2;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + w];
-            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        red[0][threadIdx.x] = e2;
+        red[1][threadIdx.x] = f2;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + w];
+                red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && red[1][0] > 0.0) {
+            const double e = sqrt(red[0][0] / red[1][0]);
+            worst = (e > worst || e != e) ? e : worst;   // a non-finite error wins
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { out[0] = red[0][0]; out[1] = red[1][0]; }
+    if (threadIdx.x == 0) out[0] = worst;
 }
 
 double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, const double* ys, const double* zs,
@@ -1404,20 +1448,20 @@ double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, c
                   int nspots) {
     hipStream_t s = c->stream;
     nspots = std::min(nspots, tr.num_leaves);
-    double* out = c->buf<double>("spot_out", (size_t)nspots * 4);
+    double* out = c->buf<double>("spot_out", (size_t)nspots * 4 * k.nsets);
     int* site = c->buf<int>("spot_site", nspots);
     double* res = c->buf<double>("spot_res", 2);
     SpotArgs sa{nspots, tr.num_leaves, n, tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, xs, ys, zs,
-                ms, n, out, site};
+                ms, n, out, site, k.nsets};
     launch_spot_far(sa, m, s);
     hipLaunchKernelGGL(k_spot_compare, dim3(1), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n, k.field,
-                       k.ldf, before, (int64_t)n, nspots, nf, res);
+                       k.ldf, before, (int64_t)n, nspots, nf, k.nsets, res);
     FMMH_HIP(hipGetLastError());
     double* h = reinterpret_cast<double*>(c->h_pinned + 800);
-    FMMH_HIP(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(h, res, sizeof(double), hipMemcpyDeviceToHost, s));
     c->sync();
     c->mark("spot_check");
-    return h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : 0.0;
+    return h[0];
 }
 
 // Near-field workgroup order (leaves, and chunks of heavy leaves, in the XCD-aware order), in two
@@ -1524,7 +1568,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->level_off = tr.level_off;
     const int S = k.nsets;
     // learned per-pair orders (off: every pair at p, as the reference)
-    const bool select = c->order_tol > 0.0 && c->sel_trained && !P.on && L.n_m2l > 0;
+    const bool select = c->order_tol > 0.0 && c->sel_trained && !P.on && L.n_m2l > 0 && m <= SPOT_MMAX;
     // graph replay: a tree-reuse call without timing events, selection or periodic images
     bool replay = false, capture = false;
     std::string gkey;
@@ -1615,6 +1659,9 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
                 c->buf<int>("s_vb_node", nv), c->buf<int>("s_vb_e0", nv), c->buf<int>("s_vb_e1", nv),
                 c->buf<int>("s_vb_part", nv),
                 np_sites > 0 ? c->buf<double>("s_near_part", (size_t)np_sites * nf * S) : nullptr, np_sites};
+    na.nsites = n;
+    na.bad = bad_excl_flag(c);
+    na.coinc_list = 0;   // sites at one position share a leaf
     c->mark("p2p_order");
     launch_near_part(na, ngrid, m, k.F, k.damp, 1, s);
     if (np_sites > 0)
@@ -1684,6 +1731,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         FMMH_HIP(hipGraphLaunch(c->graph_exec, s));
     }
     }   // device body (not replayed)
+    bad_excl_readback(c);
     if (select) {
         unsigned long long* hh = reinterpret_cast<unsigned long long*>(c->h_pinned + 850);
         FMMH_HIP(hipMemcpyAsync(hh, c->buf<unsigned long long>("sel_hist", PMAX + 1),
@@ -1744,6 +1792,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         c->cache_class_p = p;
         c->cache_classes = c->stats_classes;
     }
+    bad_excl_check(c);   // read back before the synchronisations above
     c->stats[0] = nn; c->stats[1] = nl; c->stats[2] = tr.max_depth;
     c->stats[3] = L.n_m2l; c->stats[4] = L.n_p2p; c->stats[5] = (int64_t)pairs; c->stats[6] = c->stats_classes;
     c->stats[7] = c->list_retries;
@@ -2137,7 +2186,8 @@ int fmmh_get_multipoles(fmmh_context* ctx, int32_t comm, const double* coordinat
         if (ld_multipoles < n) throw Error(FMMH_ERR_ARG, "leading dimension of multipoles smaller than n");
         const int m = order_from_dim(dim);
         if (m < 0 || ncomp(m) != dim || m > NEAR_MMAX)
-            throw Error(FMMH_ERR_UNSUPPORTED, "multipole dim must be (m+1)(m+2)(m+3)/6 with m <= 2");
+            throw Error(FMMH_ERR_UNSUPPORTED, "multipole dim must be (m+1)(m+2)(m+3)/6 with m <= " +
+                                                  std::to_string(NEAR_MMAX));
         hipStream_t s = c->stream;
         const double* X = stage_host(c, "h_X", coordinates, ld_coordinates, n, 3);
         const double* T = stage_host(c, "q_T", target_coordinates, ld_targets, ntargets, 3);
@@ -2208,7 +2258,8 @@ int fmmh_interpolate_electron_fields(fmmh_context* ctx, int32_t comm, const doub
         if (ndens < 1 || nbox < 0) throw Error(FMMH_ERR_ARG, "ndens must be >= 1 and nbox >= 0");
         if (nbox > 0 && ld_local_expansion < nbox) throw Error(FMMH_ERR_ARG, "leading dimension of local_expansion");
         if (ld_field < n) throw Error(FMMH_ERR_ARG, "leading dimension of field smaller than n");
-        if (field_order < 0 || field_order > NEAR_FMAX) throw Error(FMMH_ERR_UNSUPPORTED, "field_order must be 0..2");
+        if (field_order < 0 || field_order > NEAR_FMAX)
+            throw Error(FMMH_ERR_UNSUPPORTED, "field_order must be 0.." + std::to_string(NEAR_FMAX));
         hipStream_t s = c->stream;
         const double* X = stage_host(c, "h_X", coordinates, ld_coordinates, n, 3);
         c->cache_valid = false;
@@ -2466,7 +2517,7 @@ int fmmh_selector_train(fmmh_context* ctx, const double* coordinates, int64_t ld
         fmmh_context* c = resolve(ctx);
         hipStream_t s = c->stream;
         const int m = order_from_dim(dim);
-        if (m < 0 || ncomp(m) != dim || m > NEAR_MMAX) throw Error(FMMH_ERR_ARG, "dim must be 1, 4 or 10");
+        if (m < 0 || ncomp(m) != dim || m > SPOT_MMAX) throw Error(FMMH_ERR_ARG, "dim must be 1, 4 or 10");
         check_fmm_args(theta, ncrit, cap, 0, nullptr);
         if (cap < 1 || cap > M2L_GEMM_PMAX) throw Error(FMMH_ERR_ARG, "cap must be in 1..10");
         if (n <= ncrit || ld_coordinates < n || ld_multipoles < n) throw Error(FMMH_ERR_ARG, "need n > ncrit");

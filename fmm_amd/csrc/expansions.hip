@@ -17,8 +17,11 @@
 namespace fmmh {
 
 constexpr int MAX_NM = ncomp(PMAX);          // expansion coefficients per node
-constexpr int UP_CHUNK = 16;                 // sites staged per P2M pass
-constexpr int UP_NSRC = ncomp(NEAR_MMAX);    // source multipole components staged per site
+constexpr int UP_CHUNK = 16;                 // sites staged per P2M pass, at most
+// Sites per P2M pass for nsrc source components (fewer for high multipole orders, whose staged
+// components would crowd LDS); the one-set and many-sets kernels use the same passes, so they sum
+// in the same order.
+__host__ __device__ constexpr int up_chunk(int nsrc) { return nsrc <= 20 ? UP_CHUNK : (nsrc <= 35 ? 8 : 4); }
 
 // monomials d^e for every packed e with |e| <= p, built from one-step parents.
 __device__ __forceinline__ void build_mono(double* mono, int nm, double dx, double dy, double dz,
@@ -76,9 +79,9 @@ __device__ __forceinline__ double shift_sum(const ShiftTab& tab, int b0, int b1,
     return sum;
 }
 
+// LDS (dynamic): monomials [chunk][nm], then staged source components [chunk][nsrc].
 __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
-    extern __shared__ double mono[];
-    __shared__ double mch[UP_CHUNK][UP_NSRC];
+    extern __shared__ double p2m_lds[];
     __shared__ double d3[UP_CHUNK * 3];
     __shared__ double pw[UP_CHUNK * 3 * (PMAX + 1)];
     const int node = a.leaf_node[blockIdx.x];
@@ -87,13 +90,16 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
     const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
     const int nm = a.nm;
     const int nsrc = a.nsrc;
+    const int chunk = up_chunk(nsrc);
+    double* mono = p2m_lds;
+    double* mch = p2m_lds + chunk * nm;
     const double* __restrict__ msr = a.ms_raw + (int64_t)set * a.dim * a.ldm;
     double* __restrict__ CM = a.CM + (int64_t)set * a.num_nodes * nm;
     double acc[(MAX_NM + 127) / 128];
 #pragma unroll
     for (int q = 0; q < (MAX_NM + 127) / 128; ++q) acc[q] = 0.0;
-    for (int j0 = 0; j0 < c; j0 += UP_CHUNK) {
-        const int cc = min(UP_CHUNK, c - j0);
+    for (int j0 = 0; j0 < c; j0 += chunk) {
+        const int cc = min(chunk, c - j0);
         if ((int)threadIdx.x < 3 * cc) {
             const int jj = threadIdx.x / 3, ax = threadIdx.x - 3 * jj;
             const int j = s0 + j0 + jj;
@@ -103,7 +109,7 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
         build_mono_block(mono, nm, nm, cc, d3, pw, a.xyz, a.p);
         for (int x = threadIdx.x; x < cc * nsrc; x += blockDim.x) {
             const int jj = x / nsrc, sc = x % nsrc;
-            mch[jj][sc] = msr[(int64_t)sc * a.ldm + s0 + j0 + jj];
+            mch[jj * nsrc + sc] = msr[(int64_t)sc * a.ldm + s0 + j0 + jj];
         }
         __syncthreads();
 #pragma unroll
@@ -114,7 +120,7 @@ __global__ __launch_bounds__(128) void k_p2m(UpArgs a) {
                 for (int b = b0; b < b1; ++b) {
                     const int s = a.p2m.s[b], e = a.p2m.e[b];
                     double sum = 0.0;
-                    for (int jj = 0; jj < cc; ++jj) sum = __builtin_fma(mono[jj * nm + e], mch[jj][s], sum);
+                    for (int jj = 0; jj < cc; ++jj) sum = __builtin_fma(mono[jj * nm + e], mch[jj * nsrc + s], sum);
                     acc[q] = __builtin_fma(a.p2m.coef[b], sum, acc[q]);
                 }
             }
@@ -286,10 +292,11 @@ static void launch_l2p_p(const DownArgs& a, int num_leaves, hipStream_t s) {
 constexpr int SHIFT_SB = 8;        // sets per workgroup (P2M, M2M, L2L; L2P with field order <= 1)
 constexpr int SHIFT_Q = (MAX_NM + 255) / 256;
 
+// LDS (dynamic): staged source components [chunk][nsrc][SB].
 template <int SB>
 __global__ __launch_bounds__(128) void k_p2m_sb(UpArgs a) {
     __shared__ double mono[UP_CHUNK][MAX_NM];
-    __shared__ double mch[UP_CHUNK][UP_NSRC][SB];
+    extern __shared__ double mch[];
     __shared__ double d3[UP_CHUNK * 3];
     __shared__ double pw[UP_CHUNK * 3 * (PMAX + 1)];
     constexpr int Q = (MAX_NM + 127) / 128;
@@ -300,13 +307,14 @@ __global__ __launch_bounds__(128) void k_p2m_sb(UpArgs a) {
     const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
     const int nm = a.nm;
     const int nsrc = a.nsrc;
+    const int chunk = up_chunk(nsrc);
     double acc[Q][SB];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
         for (int u = 0; u < SB; ++u) acc[q][u] = 0.0;
-    for (int j0 = 0; j0 < c; j0 += UP_CHUNK) {
-        const int cc = min(UP_CHUNK, c - j0);
+    for (int j0 = 0; j0 < c; j0 += chunk) {
+        const int cc = min(chunk, c - j0);
         if ((int)threadIdx.x < 3 * cc) {
             const int jj = threadIdx.x / 3, ax = threadIdx.x - 3 * jj;
             const int j = s0 + j0 + jj;
@@ -316,7 +324,7 @@ __global__ __launch_bounds__(128) void k_p2m_sb(UpArgs a) {
         build_mono_block(&mono[0][0], MAX_NM, nm, cc, d3, pw, a.xyz, a.p);
         for (int x = threadIdx.x; x < SB * nsrc * cc; x += blockDim.x) {
             const int u = x / (nsrc * cc), r = x - u * nsrc * cc, sc = r / cc, jj = r - sc * cc;
-            mch[jj][sc][u] = u < nsb ? a.ms_raw[((int64_t)(set0 + u) * a.dim + sc) * a.ldm + s0 + j0 + jj] : 0.0;
+            mch[(jj * nsrc + sc) * SB + u] = u < nsb ? a.ms_raw[((int64_t)(set0 + u) * a.dim + sc) * a.ldm + s0 + j0 + jj] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -332,7 +340,7 @@ __global__ __launch_bounds__(128) void k_p2m_sb(UpArgs a) {
                     for (int jj = 0; jj < cc; ++jj) {
                         const double mo = mono[jj][e];
 #pragma unroll
-                        for (int u = 0; u < SB; ++u) sum[u] = __builtin_fma(mo, mch[jj][sc][u], sum[u]);
+                        for (int u = 0; u < SB; ++u) sum[u] = __builtin_fma(mo, mch[(jj * nsrc + sc) * SB + u], sum[u]);
                     }
                     const double cf = a.p2m.coef[b];
 #pragma unroll
@@ -524,11 +532,59 @@ __global__ __launch_bounds__(64) void k_l2p_sb(DownArgs a) {
     }
 }
 
+// L2P for field orders above 2 (any set count, one set per grid row): one thread per (site, field
+// component), the terms of k_l2p_p in the same order with the orders as arguments.
+__global__ __launch_bounds__(256) void k_l2p_rt(DownArgs a) {
+    __shared__ double L[MAX_NM];
+    __shared__ double fact[PMAX + 1];
+    const int node = a.leaf_node[blockIdx.x];
+    const int set = blockIdx.y;
+    const int s0 = a.start[node], c = a.count[node];
+    const int nm = a.nm, p = a.p, nf = a.nf;
+    for (int x = threadIdx.x; x < nm; x += blockDim.x) L[x] = a.LE[((int64_t)set * a.num_nodes + node) * nm + x];
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        fact[0] = 1.0;
+        for (int n = 1; n <= PMAX; ++n) { f *= (double)n; fact[n] = 1.0 / f; }
+    }
+    __syncthreads();
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    const double* __restrict__ near = a.near + (int64_t)set * nf * a.ld_near;
+    double* __restrict__ field = a.field + (int64_t)set * nf * a.ld_field;
+    for (int x = threadIdx.x; x < c * nf; x += blockDim.x) {
+        const int jj = x / nf, t = x - jj * nf;
+        const int k = s0 + jj;
+        const int pk = a.xyz[t];
+        const int tx = pk & 255, ty = (pk >> 8) & 255, tz = pk >> 16;
+        const int rest = p - (tx + ty + tz);   // |e| <= p - |t|
+        const double dx = a.xs[k] - cx, dy = a.ys[k] - cy, dz = a.zs[k] - cz;
+        double f = 0.0, px = 1.0;
+        for (int ex = 0; ex <= rest; ++ex) {
+            double pxy = px;
+            for (int ey = 0; ey + ex <= rest; ++ey) {
+                double m = pxy * fact[ex] * fact[ey];
+                for (int ez = 0; ez + ey + ex <= rest; ++ez) {
+                    f = __builtin_fma(m * fact[ez], L[xyz2idx(tx + ex, ty + ey, tz + ez)], f);
+                    m *= dz;
+                }
+                pxy *= dy;
+            }
+            px *= dx;
+        }
+        const int o = a.perm[k];
+        double* fp = field + (int64_t)t * a.ld_field + o;
+        *fp = (*fp + near[(int64_t)t * a.ld_near + k]) + f;
+    }
+}
+
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
+    const size_t chunk = up_chunk(a.nsrc);
     if (a.nsets == 1)
-        hipLaunchKernelGGL(k_p2m, dim3(num_leaves, 1), dim3(128), (size_t)UP_CHUNK * a.nm * sizeof(double), s, a);
-    else hipLaunchKernelGGL(k_p2m_sb<SHIFT_SB>, dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(128), 0, s, a);
+        hipLaunchKernelGGL(k_p2m, dim3(num_leaves, 1), dim3(128), chunk * (a.nm + a.nsrc) * sizeof(double), s, a);
+    else
+        hipLaunchKernelGGL(k_p2m_sb<SHIFT_SB>, dim3(num_leaves, (a.nsets + SHIFT_SB - 1) / SHIFT_SB), dim3(128),
+                           chunk * a.nsrc * SHIFT_SB * sizeof(double), s, a);
     FMMH_HIP(hipGetLastError());
 }
 
@@ -549,7 +605,9 @@ void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s) {
 
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
-    if (a.nsets == 1) {
+    if (a.nf > ncomp(2)) {
+        hipLaunchKernelGGL(k_l2p_rt, dim3(num_leaves, a.nsets), dim3(256), 0, s, a);
+    } else if (a.nsets == 1) {
         switch (a.nf) {
             case 1: launch_l2p_p<0>(a, num_leaves, s); break;
             case 4: launch_l2p_p<1>(a, num_leaves, s); break;

@@ -5,8 +5,11 @@
 
 namespace fmmh {
 
-constexpr int NEAR_MMAX = 2;   // multipole order compiled into the near-field kernel
-constexpr int NEAR_FMAX = 2;   // field order compiled into the near-field kernel
+// Orders compiled into the near-field kernels: source multipole order m <= 6 (dim <= 84), field order
+// F <= 6, m + F <= 7.  m, F <= 2 have their own tuned forms (near.hip); the rest are near_wide*.hip.
+constexpr int NEAR_MMAX = 6;
+constexpr int NEAR_FMAX = 6;
+constexpr int NEAR_KMAX = 7;
 
 struct NearArgs {
     const int* blk;            // workgroup -> target block row (a leaf's node id, or a chunk id)
@@ -42,10 +45,25 @@ struct NearArgs {
     const int* vb_part = nullptr;
     double* part = nullptr;    // [set][component][ldp]
     int64_t ldp = 0;
+    int nsites = 0;            // sites of the call: exclusion entries above it are not sites
+    int* bad = nullptr;        // set to 1 when an exclusion entry is above nsites (null: not reported)
+    int coinc_list = 0;        // sites at one position: 0 in the target block (octree leaves), 1 in its list
+    int m_rt = 0, f_rt = 0;    // source and field orders (the runtime-order exclusion kernel)
 };
 
 bool near_supported(int m, int F);
 void launch_near(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s);
+// Exclusion correction with the orders as arguments (one set per workgroup row), for the orders outside
+// m, F <= 2.
+void launch_near_excl_rt(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s);
+// The orders outside m, F <= 2, by tensor order m + F (3 and 4, 5, 6, 7; a: no damping and THOLE, b: AMOEBA
+// and ERF); false if (m, F, damp) is not theirs.
+bool launch_near_wide_k34(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
+bool launch_near_wide_k5(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
+bool launch_near_wide_k6a(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
+bool launch_near_wide_k6b(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
+bool launch_near_wide_k7a(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
+bool launch_near_wide_k7b(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
 // part 1: the pair kernel only; part 2: the exclusion correction only
 void launch_near_part(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s);
 // Sum of the chunks' partial fields of every chunked leaf, in chunk order, into out (after part 1).
@@ -340,19 +358,26 @@ void launch_sel_groups(const int* pair_csr, const int* pair_class, const unsigne
 void launch_sel_group_flags(const unsigned long long* keys, int n, int* flag, hipStream_t s);
 void launch_sel_group_info(const unsigned long long* keys, const int* flag, const int* gid, int n, int* grp_class,
                            int* grp_order, int* ngroups, hipStream_t s);
-void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, const int* leaf_node, const int* count,
-                       const int* parent, const int* m2l_off, int num_leaves, double* acc, double eps_kappa,
-                       double* tau, hipStream_t s);
+// Per-pair tolerance tau = eps_kappa * (RMS near field) / sqrt(mean M2L pairs on a site's path), with the
+// smallest non-zero near-field RMS over the nsets sets (near is [set][component][ldn]).  acc holds
+// 4 + nsets doubles: [0] the sum of squares used, [1] the path sum, [2] tau, [4 + s] set s's sum of squares.
+void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, int nsets, const int* leaf_node,
+                       const int* count, const int* parent, const int* m2l_off, int num_leaves, double* acc,
+                       double eps_kappa, hipStream_t s);
 
 struct SpotArgs {
     int nspots; int num_leaves; int n;
     const int* leaf_node; const int* start; const int* count;
     const int* p2p_off; const int* p2p_src;
     const double* xs; const double* ys; const double* zs;
-    const double* ms; int64_t ldm;
-    double* out;                              // [nspots][4] far-field direct sum
+    const double* ms; int64_t ldm;            // Taylor-scaled multipoles [set][component][ldm]
+    double* out;                              // [set][nspots][4] far-field direct sum
     int* site;                                // [nspots] sorted site index
+    int nsets;
 };
+// Source multipole orders the spot checks (and so the learned per-pair orders) cover; calls with higher
+// source orders keep expansion_order for every pair.
+constexpr int SPOT_MMAX = 2;
 void launch_spot_far(const SpotArgs& a, int m, hipStream_t s);
 void launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 

@@ -1,428 +1,55 @@
-// Near-field pair sums: the FMM P2P phase (src/fmm.F90:623-747) and the
-// all-pairs direct sum (src/fmm.F90:749-824) share one kernel.
-//
-// Work decomposition: one 64-lane wavefront per target block (a leaf of the
-// octree, or a run of 64 consecutive sites for the direct sum); see the lane
-// layout below.  The tensor recursion lives entirely in VGPRs (tensors.h).
-//
-// Exclusions follow the reference exactly: an excluded pair is evaluated at
-// displacement (1e10, 1e10, 1e10) (src/fmm.F90:714-721, 802-806), which
-// leaves the -q/(sqrt(3) 1e10) potential artefact in place.  Periodic image
-// sources are displaced as x_i - (x_j + D) and never excluded
-// (src/fmm.F90:711-721: exclusions apply to the central cell only).
-// They are applied outside the pair loop: the loop evaluates every pair at
-// its actual displacement except a site with itself (one integer compare),
-// and k_near_excl then adds, per target site and distinct exclusion entry j
-// of its near field, the pair at (1e10, 1e10, 1e10) minus the pair at its
-// actual displacement (for j = i only the former); a site that does not
-// exclude itself gets its self pair at displacement 0, i.e. a non-finite
-// field, as in the reference.
-#include "common.h"
-#include "tensors.h"
-#include "kernels.h"
+// Near-field pair sums, orders <= 2: instantiations of the kernels of
+// near_kernels.h for source multipole order m <= 2 and field order F <= 2
+// (every set count, several sets per workgroup row), the dispatch over all
+// supported orders (the rest are in near_wide*.hip), and the sum of the
+// partial fields of chunked leaves.
+#include "near_kernels.h"
 
 namespace fmmh {
-
-constexpr int NEAR_IMAGE_SRC = -2; // source id of periodic-image sources: never the target's own id
-
-// Lane layout: a target block is taken in passes of nt <= 64 sites, each
-// replicated over r = 64 / nt lane groups (group g = lane / nt, target
-// lane % nt); group g takes every r-th source of the run (positions g mod r,
-// whatever the tile size), so small leaves still fill the
-// wavefront, and the pass size is chosen per block so that few lanes idle.
-// At the end of a pass the groups' partial fields are summed in group order
-// through LDS (deterministic).
-// Source staging: the source blocks of a target block's list are one
-// concatenated run (windows of NEAR_WIN list entries, whose exclusive prefix
-// of source counts is scanned across the wavefront), cut into tiles of 64
-// sources regardless of block boundaries.  Each tile is published to LDS
-// from registers that were loaded while the previous tile was computed, so
-// the global-load latency overlaps the pair arithmetic.
-// Multipole sets: workgroup row blockIdx.y takes S consecutive sets; each pair
-// tensor is built once and contracted with the S sets' multipoles, so the
-// tensor recursion is shared (S = 1 for a single set).
-// Tensor orders up to 3 (except ERF, whose erf needs more) with the prefetch
-// registers fit 168 VGPRs (3 waves per SIMD) without spilling; the S > 1
-// forms and the rest get more.
-constexpr int NEAR_WIN = 64;
-
-// Source-tile size for S sets of NM components: the largest power of two <= 64 with S NM TILE
-// doubles in at most 8 KB (4 sets of quadrupoles: 16 sites, 3 waves per SIMD instead of 1.5;
-// 4 sets of dipoles keep 64, measured faster than 32 there).
-constexpr int near_tile(int snm) {
-    int t = 64;
-    while (t > 8 && snm * t * 8 > 8192) t /= 2;
-    return t;
-}
-
-template <int M, int F, int DAMP, int S>
-__global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 1)
-                                        : ((M + F <= 3 && DAMP != DAMP_ERF) ? 2 : 1)) void k_near(NearArgs a) {
-    constexpr int K = M + F;
-    constexpr int NM = ncomp(M);
-    constexpr int NF = ncomp(F);
-    // Source tile: TILE sites, sized so that the staged multipoles take at most 8 KB (S NM TILE
-    // doubles), which keeps several workgroups per CU; all 64 lanes prefetch the tile's multipoles
-    // cooperatively (NPM registers each, element e = lane + 64 i of the [S][NM][TILE] block).
-    constexpr int TILE = near_tile(S * NM);
-    constexpr int NPM = (S * NM * TILE + 63) / 64;
-    __shared__ double sx[TILE], sy[TILE], sz[TILE], sdf[TILE];
-    __shared__ double sm[S][NM][TILE];
-    __shared__ int sorig[TILE];
-    __shared__ double red[NF][64];
-    __shared__ int wcum[NEAR_WIN + 1];            // window: exclusive prefix of source counts
-    __shared__ int wstart[NEAR_WIN];              // window: first sorted site of each entry
-    __shared__ int wimg[NEAR_WIN];                // window: periodic image id of each entry
-    __shared__ double etab[DAMP != DAMP_NONE ? 32 : 1];   // 2^(j/32) for exp_table
-    const int vb = a.blk[blockIdx.x];
-    if (vb < 0) return;                        // an empty slot of the block order
-    int b, e0, e1, pbase = -1;                 // target block, its list entries, partial sites
-    if (a.vb_node) {
-        b = a.vb_node[vb];
-        e0 = a.vb_e0[vb];
-        e1 = a.vb_e1[vb];
-        pbase = a.vb_part[vb];
-    } else {
-        b = vb;
-        e0 = a.list_off[b];
-        e1 = a.list_off[b + 1];
-    }
-    if constexpr (DAMP != DAMP_NONE) {
-        if (threadIdx.x < 32) etab[threadIdx.x] = kExp2By32[threadIdx.x];   // read after the window barriers
-    }
-    const int set0 = blockIdx.y * S;
-    const int ns = min(S, a.nsets - set0);   // sets of this row (the rest are zero padding)
-    const int lane = threadIdx.x;
-    const int t0 = a.tgt_start[b];
-    const int tc = a.tgt_count[b];
-    const double* __restrict__ ms = a.ms + (int64_t)set0 * NM * a.ldm;
-    double* __restrict__ out = a.out + (int64_t)set0 * NF * a.ldo;
-    // Targets per pass: c passes of ct = ceil(tc / c) targets, each replicated r = 64 / ct
-    // times, with c minimizing c (ceil(64 / r) + 2) -- the pair iterations per 64-source tile
-    // plus about two iterations' worth of re-staging the tile (measured: weights 1 and 2 run
-    // alike, 0 is 17 % slower) -- so that a block of, say, 40 targets runs as 2 passes of 20
-    // (r = 3) instead of one pass with 24 idle lanes.
-    int ct = min(tc, 64);
-    {
-        int best = 1 << 30;
-        for (int c = (tc + 63) / 64; c <= 16; ++c) {
-            const int t = (tc + c - 1) / c;
-            const int rr = 64 / t;
-            const int cost = c * ((64 + rr - 1) / rr + 2);
-            if (cost < best) {
-                best = cost;
-                ct = t;
-            }
-        }
-    }
-    for (int base = 0; base < tc; base += ct) {
-        const int nt = min(ct, tc - base);
-        const int r = 64 / nt;                   // lane groups
-        const int g = lane / nt;                 // this lane's group (== r for idle lanes)
-        const int ti = lane - g * nt;            // this lane's target within the block
-        const bool active = g < r;
-        const int k = t0 + base + (active ? ti : 0);
-        const double xi = a.xs[k], yi = a.ys[k], zi = a.zs[k];
-        const double dfi = DAMP != DAMP_NONE ? a.dfs[k] : 0.0;
-        const int myo = a.orig[k] + 1;           // 1-based original index of the target
-        double acc[S][NF];
-#pragma unroll
-        for (int u = 0; u < S; ++u)
-#pragma unroll
-            for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
-        for (int w0 = e0; w0 < e1; w0 += NEAR_WIN) {
-            const int wn = min(NEAR_WIN, e1 - w0);
-            int cnt = 0, st = 0, im = 0;
-            if (lane < wn) {
-                const int enc = a.list_src[w0 + lane];
-                const int sb = src_node(enc);
-                im = src_image(enc);
-                st = a.src_start[sb];
-                cnt = a.src_count[sb];
-            }
-            int incl = cnt;                      // inclusive scan across the wavefront
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int y = __shfl_up(incl, d);
-                if (lane >= d) incl += y;
-            }
-            __syncthreads();                     // readers of the previous window are done
-            if (lane < wn) {
-                wcum[lane] = incl - cnt;
-                wstart[lane] = st;
-                wimg[lane] = im;
-            }
-            if (lane == wn - 1) wcum[wn] = incl;
-            __syncthreads();
-            const int total = wcum[wn];
-            // prefetch registers for tile position v = vbase + lane (lane < TILE) and the tile's
-            // multipoles (all lanes)
-            double px = 0.0, py = 0.0, pz = 0.0, pdf = 0.0;
-            double pm[NPM];
-            int po = 0;
-            int cur = 0;                         // list entry holding the tile's first position
-            auto fetch = [&](int vbase) {
-                while (wcum[cur + 1] <= vbase) ++cur;
-                const int v = vbase + lane;
-                int pj = -1;
-                if (lane < TILE && v < total) {
-                    int e = cur;
-                    while (wcum[e + 1] <= v) ++e;
-                    const int j = wstart[e] + (v - wcum[e]);
-                    const int img = wimg[e];
-                    const double* D = a.disp + 3 * img;
-                    pj = j;
-                    px = a.xs[j] + D[0];
-                    py = a.ys[j] + D[1];
-                    pz = a.zs[j] + D[2];
-                    if constexpr (DAMP != DAMP_NONE) pdf = a.dfs[j];
-                    po = img == 0 ? a.orig[j] + 1 : NEAR_IMAGE_SRC;   // exclusions: central cell only
-                }
-#pragma unroll
-                for (int i = 0; i < NPM; ++i) {
-                    const int el = lane + 64 * i;           // [u][s][t] of the tile's multipole block
-                    const int t = el % TILE, us = el / TILE;
-                    const int jt = __shfl(pj, t);           // source of tile position t (-1: none)
-                    const int u = us / NM, sc = us - u * NM;
-                    pm[i] = (el < S * NM * TILE && u < ns && jt >= 0) ? ms[((int64_t)u * NM + sc) * a.ldm + jt] : 0.0;
-                }
-            };
-            if (total > 0) fetch(0);
-            for (int vbase = 0; vbase < total; vbase += TILE) {
-                const int cn = min(TILE, total - vbase);
-                if (lane < cn) {
-                    sx[lane] = px;
-                    sy[lane] = py;
-                    sz[lane] = pz;
-                    if constexpr (DAMP != DAMP_NONE) sdf[lane] = pdf;
-                    sorig[lane] = po;
-                }
-#pragma unroll
-                for (int i = 0; i < NPM; ++i) {
-                    const int el = lane + 64 * i;
-                    if (el < S * NM * TILE) (&sm[0][0][0])[el] = pm[i];
-                }
-                __syncthreads();
-                if (vbase + TILE < total) fetch(vbase + TILE);
-                if (active) {
-                    // group g takes the run positions v = g mod r, in increasing order: the sums do not
-                    // depend on the tile size (single- and many-set forms add in the same order)
-                    for (int jj = ((g - vbase) % r + r) % r; jj < cn; jj += r) {
-                        const double dx = xi - sx[jj];
-                        const double dy = yi - sy[jj];
-                        const double dz = zi - sz[jj];
-                        const double R2 = dx * dx + dy * dy + dz * dz;
-                        // the self pair is left to k_near_excl: 1/R = 0 makes its terms exactly 0
-                        const double rinv = rsqrt_pos(R2);
-                        const double inv = sorig[jj] == myo ? 0.0 : rinv;
-                        double aij = 0.0;
-                        if constexpr (DAMP != DAMP_NONE) aij = dfi * sdf[jj];
-                        double gr[K + 1];
-                        radial_inv<K, DAMP, DAMP != DAMP_NONE>(R2, inv, aij, gr, etab);
-                        if constexpr (has_closed_form<M, F>()) {
-#pragma unroll
-                            for (int u = 0; u < S; ++u) {
-                                double m[NM];
-#pragma unroll
-                                for (int s = 0; s < NM; ++s) m[s] = sm[u][s][jj];
-                                contract_closed<M, F>(dx, dy, dz, gr, m, acc[u]);
-                            }
-                        } else {
-                            double T[ncomp(K)];
-                            md_tensor<K>(dx, dy, dz, gr, T);
-#pragma unroll
-                            for (int u = 0; u < S; ++u) {
-                                double m[NM];
-#pragma unroll
-                                for (int s = 0; s < NM; ++s) m[s] = sm[u][s][jj];
-                                contract<M, F>(m, T, acc[u]);
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        // sum the r partial fields of each target in group order, set by set
-#pragma unroll
-        for (int u = 0; u < S; ++u) {
-#pragma unroll
-            for (int t = 0; t < NF; ++t) red[t][lane] = acc[u][t];
-            __syncthreads();
-            if (lane < nt && u < ns) {
-                const int kk = t0 + base + lane;
-#pragma unroll
-                for (int t = 0; t < NF; ++t) {
-                    double v = red[t][lane];
-                    for (int q = 1; q < r; ++q) v += red[t][lane + q * nt];
-                    if (pbase >= 0) {
-                        a.part[((int64_t)(set0 + u) * NF + t) * a.ldp + pbase + base + lane] = v;
-                    } else {
-                        double* o = out + ((int64_t)u * NF + t) * a.ldo + kk;
-                        if (a.accumulate) *o += v;
-                        else *o = v;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Exclusions of the near field (see the header comment).  One lane per
-// target site of a target block; the exclusion entries of the site are
-// de-duplicated (the reference tests membership), and an entry counts only
-// if its site lies in a central-cell source block of the block's list.
-constexpr int NEAR_EXCL_STAGE = 512;
-
-template <int M, int F, int DAMP, int S>
-__global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
-    constexpr int NM = ncomp(M);
-    constexpr int NF = ncomp(F);
-    __shared__ double etab[DAMP != DAMP_NONE ? 32 : 1];   // 2^(j/32) for exp_table
-    const int vb = a.blk[blockIdx.x];
-    if (vb < 0) return;
-    int b = vb;
-    if (a.vb_node) {
-        b = a.vb_node[vb];
-        if (a.vb_e0[vb] != a.list_off[b]) return;   // a leaf's corrections run once, with its first chunk
-    }
-    if constexpr (DAMP != DAMP_NONE) {
-        if (threadIdx.x < 32) etab[threadIdx.x] = kExp2By32[threadIdx.x];
-        __syncthreads();
-    }
-    const int set0 = blockIdx.y * S;
-    const int ns = min(S, a.nsets - set0);
-    const int t0 = a.tgt_start[b], tc = a.tgt_count[b];
-    const int e0 = a.list_off[b], e1 = a.list_off[b + 1];
-    const double* __restrict__ ms = a.ms + (int64_t)set0 * NM * a.ldm;
-    double* __restrict__ out = a.out + (int64_t)set0 * NF * a.ldo;
-    // the central-cell source blocks of the list, as sorted-site ranges [first, end), staged in LDS
-    // (NEAR_EXCL_STAGE of them; a longer list is searched in global memory)
-    __shared__ int rfirst[NEAR_EXCL_STAGE], rend[NEAR_EXCL_STAGE];
-    const bool staged = e1 - e0 <= NEAR_EXCL_STAGE;
-    if (staged) {
-        for (int x = e0 + (int)threadIdx.x; x < e1; x += 64) {
-            const int enc = a.list_src[x];
-            int f = 0, en = 0;                    // image entries never match
-            if (src_image(enc) == 0) {
-                const int sb = src_node(enc);
-                f = a.src_start[sb];
-                en = f + a.src_count[sb];
-            }
-            rfirst[x - e0] = f;
-            rend[x - e0] = en;
-        }
-    }
-    __syncthreads();
-    for (int base = 0; base < tc; base += 64) {
-        const int jt = base + (int)threadIdx.x;
-        if (jt >= tc) break;
-        const int k = t0 + jt;
-        const int myo = a.orig[k] + 1;
-        const double xi = a.xs[k], yi = a.ys[k], zi = a.zs[k];
-        const double dfi = DAMP != DAMP_NONE ? a.dfs[k] : 0.0;
-        double acc[S][NF];
-#pragma unroll
-        for (int u = 0; u < S; ++u)
-#pragma unroll
-            for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
-        auto add = [&](int j, double dx, double dy, double dz, double sign) {
-            double aij = 0.0;
-            if constexpr (DAMP != DAMP_NONE) aij = dfi * a.dfs[j];
-#pragma unroll
-            for (int u = 0; u < S; ++u) {
-                if (u >= ns) break;
-                double m[NM];
-#pragma unroll
-                for (int s = 0; s < NM; ++s) m[s] = sign * ms[((int64_t)u * NM + s) * a.ldm + j];
-                pair_field<M, F, DAMP>(dx, dy, dz, dx * dx + dy * dy + dz * dz, aij, m, acc[u], etab);
-            }
-        };
-        bool self = false;
-        for (int q = 0; q < a.nexcl; ++q) {
-            const int e = a.excl[(int64_t)q * a.lde + k];
-            if (e <= 0) continue;
-            bool dup = false;
-            for (int q2 = 0; q2 < q; ++q2) dup |= a.excl[(int64_t)q2 * a.lde + k] == e;
-            if (dup) continue;
-            if (e == myo) {
-                self = true;
-                add(k, 1e10, 1e10, 1e10, 1.0);
-                continue;
-            }
-            const int j = a.inv ? a.inv[e - 1] : e - 1;   // sorted position of the excluded site
-            bool near = false;
-            if (staged) {
-                for (int x = 0; x < e1 - e0 && !near; ++x) near = j >= rfirst[x] && j < rend[x];
-            } else {
-                for (int x = e0; x < e1 && !near; ++x) {
-                    const int enc = a.list_src[x];
-                    if (src_image(enc) != 0) continue;
-                    const int sb = src_node(enc);
-                    const int st = a.src_start[sb];
-                    near = j >= st && j < st + a.src_count[sb];
-                }
-            }
-            if (!near) continue;
-            add(j, 1e10, 1e10, 1e10, 1.0);
-            add(j, xi - a.xs[j], yi - a.ys[j], zi - a.zs[j], -1.0);
-        }
-        if (!self) add(k, 0.0, 0.0, 0.0, 1.0);   // the reference's self pair: non-finite
-#pragma unroll
-        for (int u = 0; u < S; ++u) {
-            if (u >= ns) break;
-#pragma unroll
-            for (int t = 0; t < NF; ++t) out[((int64_t)u * NF + t) * a.ldo + k] += acc[u][t];
-        }
-    }
-}
-
-// Sets per workgroup row: 4 where the tensor is cheap in registers, else 2.
-template <int M, int F, int DAMP>
-constexpr int near_sets() { return (M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 2; }
-
-// part 1: the pair kernel; part 2: the exclusion correction (after it on the stream)
-template <int M, int F, int DAMP>
-static void launch_sets(const NearArgs& a, int nblocks, int part, hipStream_t s) {
-    if (a.nsets == 1) {
-        if (part == 1) hipLaunchKernelGGL((k_near<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((k_near_excl<M, F, DAMP, 1>), dim3(nblocks), dim3(64), 0, s, a);
-    } else {
-        constexpr int S = near_sets<M, F, DAMP>();
-        const dim3 g(nblocks, (a.nsets + S - 1) / S);
-        if (part == 1) hipLaunchKernelGGL((k_near<M, F, DAMP, S>), g, dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S>), g, dim3(64), 0, s, a);
-    }
-}
-
-template <int M, int F>
-static void launch_damp(const NearArgs& a, int nblocks, int damp, int part, hipStream_t s) {
-    switch (damp) {
-        case DAMP_NONE: launch_sets<M, F, DAMP_NONE>(a, nblocks, part, s); break;
-        case DAMP_THOLE: launch_sets<M, F, DAMP_THOLE>(a, nblocks, part, s); break;
-        case DAMP_AMOEBA: launch_sets<M, F, DAMP_AMOEBA>(a, nblocks, part, s); break;
-        default: launch_sets<M, F, DAMP_ERF>(a, nblocks, part, s); break;
-    }
-}
 
 template <int M>
 static void launch_f(const NearArgs& a, int nblocks, int F, int damp, int part, hipStream_t s) {
     switch (F) {
-        case 0: launch_damp<M, 0>(a, nblocks, damp, part, s); break;
-        case 1: launch_damp<M, 1>(a, nblocks, damp, part, s); break;
-        default: launch_damp<M, 2>(a, nblocks, damp, part, s); break;
+        case 0: launch_damp<M, 0, false>(a, nblocks, damp, part, s); break;
+        case 1: launch_damp<M, 1, false>(a, nblocks, damp, part, s); break;
+        default: launch_damp<M, 2, false>(a, nblocks, damp, part, s); break;
     }
 }
 
-bool near_supported(int m, int F) { return m >= 0 && m <= NEAR_MMAX && F >= 0 && F <= NEAR_FMAX; }
+bool near_supported(int m, int F) {
+    return m >= 0 && F >= 0 && m <= NEAR_MMAX && F <= NEAR_FMAX && m + F <= NEAR_KMAX;
+}
 
 void launch_near_part(const NearArgs& a, int nblocks, int m, int F, int damp, int part, hipStream_t s) {
     if (nblocks <= 0) return;
-    switch (m) {
-        case 0: launch_f<0>(a, nblocks, F, damp, part, s); break;
-        case 1: launch_f<1>(a, nblocks, F, damp, part, s); break;
-        default: launch_f<2>(a, nblocks, F, damp, part, s); break;
+    if (!near_supported(m, F)) throw Error(FMMH_ERR_UNSUPPORTED, "near-field orders outside the compiled range");
+    if (m <= 2 && F <= 2) {
+        switch (m) {
+            case 0: launch_f<0>(a, nblocks, F, damp, part, s); break;
+            case 1: launch_f<1>(a, nblocks, F, damp, part, s); break;
+            default: launch_f<2>(a, nblocks, F, damp, part, s); break;
+        }
+    } else if (!(launch_near_wide_k34(a, nblocks, m, F, damp, part, s) ||
+                 launch_near_wide_k5(a, nblocks, m, F, damp, part, s) ||
+                 launch_near_wide_k6a(a, nblocks, m, F, damp, part, s) ||
+                 launch_near_wide_k6b(a, nblocks, m, F, damp, part, s) ||
+                 launch_near_wide_k7a(a, nblocks, m, F, damp, part, s) ||
+                 launch_near_wide_k7b(a, nblocks, m, F, damp, part, s))) {
+        throw Error(FMMH_ERR_INTERNAL, "no near-field kernel for these orders");
+    }
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_near_excl_rt(const NearArgs& a, int nblocks, int m, int F, int damp, hipStream_t s) {
+    NearArgs b = a;
+    b.m_rt = m;
+    b.f_rt = F;
+    const dim3 g(nblocks, a.nsets);
+    switch (damp) {
+        case DAMP_NONE: hipLaunchKernelGGL((k_near_excl<-1, -1, DAMP_NONE, 1>), g, dim3(64), 0, s, b); break;
+        case DAMP_THOLE: hipLaunchKernelGGL((k_near_excl<-1, -1, DAMP_THOLE, 1>), g, dim3(64), 0, s, b); break;
+        case DAMP_AMOEBA: hipLaunchKernelGGL((k_near_excl<-1, -1, DAMP_AMOEBA, 1>), g, dim3(64), 0, s, b); break;
+        default: hipLaunchKernelGGL((k_near_excl<-1, -1, DAMP_ERF, 1>), g, dim3(64), 0, s, b); break;
     }
     FMMH_HIP(hipGetLastError());
 }

@@ -555,26 +555,31 @@ void launch_sel_group_info(const unsigned long long* keys, const int* flag, cons
 // into acc[1]: the per-pair tolerance is
 //   tau = eps * kappa * sqrt(acc[0] / n) / sqrt(acc[1] / n),
 // the relative RMS budget spread over the pairs a site sees.
+// grid.y = set: the near-field sum of squares of each set (field block; the potential for field order 0);
+// row 0 also sums the M2L list sizes along each site's path.
 __global__ __launch_bounds__(256) void k_sel_budget(const double* __restrict__ near, int64_t ldn, int n, int nf,
                                                     const int* __restrict__ leaf_node, const int* __restrict__ count,
                                                     const int* __restrict__ parent, const int* __restrict__ m2l_off,
                                                     int num_leaves, double* __restrict__ acc) {
     __shared__ double red[2][256];
+    const int set = blockIdx.y;
+    const double* __restrict__ nr = near + (int64_t)set * nf * ldn;
     double f2 = 0.0, paths = 0.0;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         if (nf >= 4) {
-            for (int c = 1; c < 4; ++c) { const double v = near[(int64_t)c * ldn + k]; f2 += v * v; }
+            for (int c = 1; c < 4; ++c) { const double v = nr[(int64_t)c * ldn + k]; f2 += v * v; }
         } else {
-            const double v = near[k];
+            const double v = nr[k];
             f2 += v * v;
         }
     }
-    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < num_leaves; l += gridDim.x * blockDim.x) {
-        int node = leaf_node[l];
-        long long s = 0;
-        while (node >= 0) { s += m2l_off[node + 1] - m2l_off[node]; node = parent[node]; }
-        paths += (double)count[leaf_node[l]] * (double)s;
-    }
+    if (set == 0)
+        for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < num_leaves; l += gridDim.x * blockDim.x) {
+            int node = leaf_node[l];
+            long long s = 0;
+            while (node >= 0) { s += m2l_off[node + 1] - m2l_off[node]; node = parent[node]; }
+            paths += (double)count[leaf_node[l]] * (double)s;
+        }
     red[0][threadIdx.x] = f2;
     red[1][threadIdx.x] = paths;
     __syncthreads();
@@ -586,23 +591,31 @@ __global__ __launch_bounds__(256) void k_sel_budget(const double* __restrict__ n
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        atomicAdd(&acc[0], red[0][0]);
-        atomicAdd(&acc[1], red[1][0]);
+        atomicAdd(&acc[4 + set], red[0][0]);
+        if (set == 0) atomicAdd(&acc[1], red[1][0]);
     }
 }
 
-__global__ void k_sel_tau(const double* __restrict__ acc, int n, double eps_kappa, double* __restrict__ tau) {
-    const double frms = sqrt(acc[0] / n);
+// tau from the smallest non-zero near-field RMS over the sets (a set without near field has no far
+// field to keep either)
+__global__ void k_sel_tau(double* __restrict__ acc, int n, int nsets, double eps_kappa) {
+    double f2 = 0.0;
+    for (int s = 0; s < nsets; ++s) {
+        const double v = acc[4 + s];
+        if (v > 0.0 && (f2 == 0.0 || v < f2)) f2 = v;
+    }
+    acc[0] = f2;
+    const double frms = sqrt(f2 / n);
     const double neff = fmax(acc[1] / n, 1.0);
-    *tau = eps_kappa * frms / sqrt(neff);
+    acc[2] = eps_kappa * frms / sqrt(neff);
 }
 
-void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, const int* leaf_node, const int* count,
-                       const int* parent, const int* m2l_off, int num_leaves, double* acc, double eps_kappa,
-                       double* tau, hipStream_t s) {
-    hipLaunchKernelGGL(k_sel_budget, dim3(64), dim3(256), 0, s, near, ldn, n, nf, leaf_node, count, parent, m2l_off,
-                       num_leaves, acc);
-    hipLaunchKernelGGL(k_sel_tau, dim3(1), dim3(1), 0, s, acc, n, eps_kappa, tau);
+void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, int nsets, const int* leaf_node,
+                       const int* count, const int* parent, const int* m2l_off, int num_leaves, double* acc,
+                       double eps_kappa, hipStream_t s) {
+    hipLaunchKernelGGL(k_sel_budget, dim3(64, nsets), dim3(256), 0, s, near, ldn, n, nf, leaf_node, count, parent,
+                       m2l_off, num_leaves, acc);
+    hipLaunchKernelGGL(k_sel_tau, dim3(1), dim3(1), 0, s, acc, n, nsets, eps_kappa);
     FMMH_HIP(hipGetLastError());
 }
 
@@ -610,17 +623,21 @@ void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, const int
 // Far-field part of the direct sum at spot sites, with the reference's far
 // semantics: every source whose leaf is not in the P2P list of the site's
 // leaf, undamped and without exclusions.  One workgroup per spot site
-// (the first site of an evenly spaced leaf); out[spot][4] = potential +
+// (the first site of an evenly spaced leaf) and block of SPOT_SB sets (each
+// source's tensor is shared by the block); out[set][spot][4] = potential +
 // field of those sources.  The near list of the leaf is sorted by node id,
 // which is also sorted-site order, so membership is a binary search over
 // its site ranges.
 constexpr int SPOT_STAGE = 1024;
+constexpr int SPOT_SB = 8;
 
 template <int M>
 __global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
     constexpr int NM = ncomp(M);
     __shared__ double red[4][256];
     const int spot = blockIdx.x;
+    const int set0 = blockIdx.y * SPOT_SB;
+    const int nsb = min(SPOT_SB, a.nsets - set0);
     const int leaf = (int)(((long long)spot * a.num_leaves) / a.nspots);
     const int node = a.leaf_node[leaf];
     const int site = a.start[node];
@@ -638,7 +655,11 @@ __global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
             rend[x] = a.start[sn] + a.count[sn];
         }
     __syncthreads();
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[SPOT_SB][4];
+#pragma unroll
+    for (int u = 0; u < SPOT_SB; ++u)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[u][c] = 0.0;
     for (int jj = threadIdx.x; jj < a.n; jj += blockDim.x) {
         // is jj inside the site range of a near source leaf?  (last range with first <= jj)
         bool near = false;
@@ -668,28 +689,37 @@ __global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
         radial<M + 1, DAMP_NONE>(dx * dx + dy * dy + dz * dz, 0.0, g);
         double T[ncomp(M + 1)];
         md_tensor<M + 1>(dx, dy, dz, g, T);
-        double m[NM];
 #pragma unroll
-        for (int s = 0; s < NM; ++s) m[s] = a.ms[(int64_t)s * a.ldm + jj];
-        contract<M, 1>(m, T, acc);
+        for (int u = 0; u < SPOT_SB; ++u) {
+            if (u >= nsb) break;
+            const double* __restrict__ msu = a.ms + (int64_t)(set0 + u) * NM * a.ldm;
+            double m[NM];
+#pragma unroll
+            for (int s = 0; s < NM; ++s) m[s] = msu[(int64_t)s * a.ldm + jj];
+            contract<M, 1>(m, T, acc[u]);
+        }
     }
-    for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    for (int u = 0; u < nsb; ++u) {
+        for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = acc[u][c];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w)
+                for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x < 4) a.out[((int64_t)(set0 + u) * a.nspots + spot) * 4 + threadIdx.x] = red[threadIdx.x][0];
         __syncthreads();
     }
-    if (threadIdx.x < 4) a.out[(int64_t)spot * 4 + threadIdx.x] = red[threadIdx.x][0];
-    if (threadIdx.x == 0) a.site[spot] = site;
+    if (threadIdx.x == 0 && blockIdx.y == 0) a.site[spot] = site;
 }
 
 void launch_spot_far(const SpotArgs& a, int m, hipStream_t s) {
     if (a.nspots <= 0) return;
+    const dim3 g(a.nspots, (a.nsets + SPOT_SB - 1) / SPOT_SB);
     switch (m) {
-        case 0: hipLaunchKernelGGL(k_spot_far<0>, dim3(a.nspots), dim3(256), 0, s, a); break;
-        case 1: hipLaunchKernelGGL(k_spot_far<1>, dim3(a.nspots), dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_spot_far<2>, dim3(a.nspots), dim3(256), 0, s, a); break;
+        case 0: hipLaunchKernelGGL(k_spot_far<0>, g, dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_spot_far<1>, g, dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_spot_far<2>, g, dim3(256), 0, s, a); break;
     }
     FMMH_HIP(hipGetLastError());
 }

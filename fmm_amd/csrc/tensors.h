@@ -143,6 +143,7 @@ __device__ __forceinline__ double rsqrt_pos(double x) {
 template <int K, int DAMP, bool TAB = false>
 __device__ __forceinline__ void radial_inv(double R2, double inv, double a, double (&g)[K + 1],
                                            const double* etab = nullptr) {
+    static_assert(DAMP == DAMP_NONE || K <= KMAX_NEAR, "damped tensors stop at order 6");
     const double inv2 = inv * inv;
     double b = inv;
     if constexpr (DAMP == DAMP_NONE) {
@@ -261,16 +262,28 @@ __host__ __device__ constexpr Exps idx2xyz(int i) {
     return Exps{0, 0, 0};
 }
 
-// acc[t] += sum_s m[s] T_{s+t}, |s| <= M, |t| <= F; m already carries the
+// Tensor order a pair of source order M and field order F needs.  The damped
+// kinds stop at order 6: the reference's Tn_damp_* fill T0..T6 only and leave
+// the higher components at the zero they were allocated with
+// (src/tensors_damped_thole.F90:55-62, same lines in _amoeba and _erf;
+// src/fmm.F90:576-577), so their terms with |s| + |t| > 6 vanish.
+template <int M, int F, int DAMP>
+__host__ __device__ constexpr int pair_tensor_order() {
+    return (DAMP != DAMP_NONE && M + F > KMAX_NEAR) ? KMAX_NEAR : M + F;
+}
+
+// acc[t] += sum_s m[s] T_{s+t}, |s| <= M, |t| <= F, |s + t| <= KT (T holds the
+// orders <= KT; KT = M + F unless the kind truncates); m already carries the
 // Taylor/symmetry factor (-1)^{|s|+1} / (sx! sy! sz!) of src/fmm.F90:597-612.
-template <int M, int F>
-__device__ __forceinline__ void contract(const double (&m)[ncomp(M)], const double (&T)[ncomp(M + F)],
+template <int M, int F, int KT = M + F>
+__device__ __forceinline__ void contract(const double (&m)[ncomp(M)], const double (&T)[ncomp(KT)],
                                          double (&acc)[ncomp(F)]) {
 #pragma unroll
     for (int t = 0; t < ncomp(F); ++t) {
 #pragma unroll
         for (int s = 0; s < ncomp(M); ++s) {
             const Exps et = idx2xyz(t), es = idx2xyz(s);
+            if (et.x + et.y + et.z + es.x + es.y + es.z > KT) continue;
             acc[t] = __builtin_fma(m[s], T[xyz2idx(et.x + es.x, et.y + es.y, et.z + es.z)], acc[t]);
         }
     }
@@ -356,20 +369,31 @@ __device__ __forceinline__ void contract_closed(double x, double y, double z, co
 // One pair's contribution acc[t] += sum_s m[s] T_{s+t}(x, y, z) with damping
 // factor product a; closed form where there is one.  The radial part is computed as
 // the near-field pair loop computes it (rsqrt_pos, exp_table on etab).
+// The pair's contribution from its radial derivatives gr[0..KT] (KT =
+// pair_tensor_order): closed form where there is one, else the tensor and the
+// truncated contraction.
+template <int M, int F, int DAMP>
+__device__ __forceinline__ void pair_contract(double x, double y, double z,
+                                              const double (&gr)[pair_tensor_order<M, F, DAMP>() + 1],
+                                              const double (&m)[ncomp(M)], double (&acc)[ncomp(F)]) {
+    constexpr int KT = pair_tensor_order<M, F, DAMP>();
+    if constexpr (has_closed_form<M, F>()) {
+        contract_closed<M, F>(x, y, z, gr, m, acc);
+    } else {
+        double T[ncomp(KT)];
+        md_tensor<KT>(x, y, z, gr, T);
+        contract<M, F, KT>(m, T, acc);
+    }
+}
+
 template <int M, int F, int DAMP>
 __device__ __forceinline__ void pair_field(double x, double y, double z, double R2, double a,
                                            const double (&m)[ncomp(M)], double (&acc)[ncomp(F)],
                                            const double* etab) {
-    constexpr int K = M + F;
-    double gr[K + 1];
-    radial_inv<K, DAMP, DAMP != DAMP_NONE>(R2, rsqrt_pos(R2), a, gr, etab);
-    if constexpr (has_closed_form<M, F>()) {
-        contract_closed<M, F>(x, y, z, gr, m, acc);
-    } else {
-        double T[ncomp(K)];
-        md_tensor<K>(x, y, z, gr, T);
-        contract<M, F>(m, T, acc);
-    }
+    constexpr int KT = pair_tensor_order<M, F, DAMP>();
+    double gr[KT + 1];
+    radial_inv<KT, DAMP, DAMP != DAMP_NONE>(R2, rsqrt_pos(R2), a, gr, etab);
+    pair_contract<M, F, DAMP>(x, y, z, gr, m, acc);
 }
 
 }  // namespace fmmh

@@ -13,6 +13,11 @@
  *     multipoles n x dim (dim = (m+1)(m+2)(m+3)/6 packed q,x,y,z,xx,xy,...),
  *     exclusions n x nexcl (1-based site indices, 0 = padding), field
  *     n x (F+1)(F+2)(F+3)/6.  ld_* is the leading dimension (>= n) of each.
+ *   - Orders: source multipole order m <= 6 (dim <= 84) and field order
+ *     F <= 6 with m + F <= 7 (src/fmm.F90:567-577 takes any; other orders
+ *     return FMMH_ERR_UNSUPPORTED).  The damped kinds contract tensors up to
+ *     order 6 only, as the reference, whose damped tensors stop at T6
+ *     (src/tensors_damped_thole.F90:55-62).
  *   - field is ACCUMULATED into (field += result), like the reference, whose
  *     intent(out) dummy is never zeroed (src/fmm.F90:731, 813, 1113).
  *   - Exclusions are honoured the reference's way: an excluded pair is
@@ -100,8 +105,9 @@ int fmmh_field_fmm(fmmh_context* ctx, int32_t comm,
 
 /* Device-resident forms of the two calls above: identical arguments, but
  * every array pointer except box_matrix is device memory of the context's device.  No host
- * copies; exclusion indices are not range-checked.  Used when the inputs are
- * already in HBM (benchmarks, GPU-side callers). */
+ * copies; an exclusion entry above n is found by the exclusion pass, skipped, and reported
+ * as FMMH_ERR_ARG after the call (the field is then not meaningful).  Used when the inputs
+ * are already in HBM (benchmarks, GPU-side callers). */
 int fmmh_field_direct_dev(fmmh_context* ctx, int32_t comm,
                           const double* coordinates, int64_t ld_coordinates,
                           const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
@@ -185,7 +191,7 @@ int fmmh_get_boxes_export(fmmh_context* ctx, int32_t* idx_near_field, double* bo
                           int32_t* box_indices);
 
 /* Replaces get_multipoles (src/fmm.F90:1171-1229); multipoles is
- * n x dim x ndens (one leading dimension), dim = (m+1)(m+2)(m+3)/6, m <= 2. */
+ * n x dim x ndens (one leading dimension), dim = (m+1)(m+2)(m+3)/6, m <= 6. */
 int fmmh_get_multipoles(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,
                         int32_t n, const double* multipoles, int64_t ld_multipoles, int32_t dim, int32_t ndens,
                         const double* target_coordinates, int64_t ld_targets, int32_t ntargets, double theta,
@@ -258,16 +264,16 @@ enum fmmh_option {
     /* Learned per-pair expansion orders (no counterpart in the reference, whose M2L applies
      * expansion_order to every pair).  With a trained selector on the context
      * (fmmh_selector_train / fmmh_selector_set) and ORDER_TOL > 0, field_fmm picks an order
-     * q <= expansion_order for every M2L pair so that the far field keeps a relative RMS error
-     * of about ORDER_TOL; without a trained selector, or for periodic calls, every pair keeps
-     * expansion_order. */
+     * q <= expansion_order for every M2L pair so that the far field of every multipole set keeps
+     * a relative RMS error of about ORDER_TOL; without a trained selector, for periodic calls and
+     * for source multipole orders above 2, every pair keeps expansion_order. */
     FMMH_OPT_ORDER_TOL = 6,       /* requested relative RMS field error of the far field (0 = off, default) */
     FMMH_OPT_ORDER_MARGIN = 7,    /* added to the predicted order before rounding up (default 0.25) */
     FMMH_OPT_SPOT_CHECKS = 8,     /* > 0: after the call, compare the far field at this many sites (the
                                      first site of evenly spaced leaves) against direct sums with the
-                                     reference's far semantics (undamped, no exclusions); if the relative
-                                     RMS error exceeds ORDER_TOL, redo M2L..L2P at expansion_order
-                                     (fallback).  0 (default): no check. */
+                                     reference's far semantics (undamped, no exclusions), set by set; if
+                                     the relative RMS error of any set exceeds ORDER_TOL, redo M2L..L2P
+                                     at expansion_order (fallback).  0 (default): no check. */
     FMMH_OPT_ORDER_SAFETY = 9,    /* factor on the per-pair tolerance (default 1) */
     FMMH_OPT_ORDER_FORCE_FULL = 10, /* != 0: run the selection machinery but keep every pair at
                                      expansion_order (ablation) */
@@ -347,10 +353,11 @@ int fmmh_selector_grad(fmmh_context* ctx, const float* weights, const float* nor
 int fmmh_selector_predict(fmmh_context* ctx, const float* weights, const float* norm, const float* features,
                           int32_t ns, float* out);
 /* Selection figures of the last field_fmm call: out[0] spot-check relative
- * error (-1: not checked), [1] 1 if it fell back to expansion_order, [2] per-pair
- * tolerance tau, [3] M2L flops at the chosen orders (2 n_m(q)^2 per pair and set),
- * [4] M2L flops at expansion_order, [5] mean order, [6] 1 if selection ran,
- * [7] RMS near field used for tau, [8 + q] pairs at order q (q = 0..10), [19] the
+ * error, the largest over the multipole sets (-1: not checked), [1] 1 if it fell
+ * back to expansion_order, [2] per-pair tolerance tau, [3] M2L flops at the chosen
+ * orders (2 n_m(q)^2 per pair and set), [4] M2L flops at expansion_order, [5] mean
+ * order, [6] 1 if selection ran, [7] RMS near field used for tau (the smallest
+ * non-zero one over the sets), [8 + q] pairs at order q (q = 0..10), [19] the
  * context's tolerance scale for later calls (1; after a fallback whose full-order
  * result meets the tolerance, multiplied by about tol / error, at most 1/2 and at
  * least 1/8, floor 1/256; reset to 1 by setting FMMH_OPT_ORDER_TOL or

@@ -23,14 +23,17 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-BLOCKS = [(0, 1), (1, 4), (4, 10), (10, 20)]
+def ncomp(k):
+    return (k + 1) * (k + 2) * (k + 3) // 6
 
 
 def block_rel(a, b, field_order, skip_potential=False):
+    """Relative error per output-order block: components of degree k are columns ncomp(k-1)..ncomp(k)-1."""
     out = []
-    for k, (lo, hi) in enumerate(BLOCKS[: field_order + 1]):
-        if skip_potential and k == 0:
+    for k in range(field_order + 1):
+        if skip_potential and k == 0 and field_order > 0:
             continue
+        lo, hi = ncomp(k - 1), ncomp(k)
         out.append(rel(a[:, lo:hi], b[:, lo:hi]))
     return out
 

@@ -6,7 +6,7 @@ script calls its field_direct / field_fmm through the bind(C) wrapper
 oracle/ref_bindc.F90 and stores inputs and outputs only.
 
 Run in the build container (it needs the reference sources):
-    make -C oracle ref && python tests/golden/make_golden.py [--periodic | --qmmm]
+    make -C oracle ref && python tests/golden/make_golden.py [--periodic | --qmmm | --wide | --coincident]
 """
 import os
 import sys
@@ -78,6 +78,8 @@ def main():
                             field_fmm=oracle.field_fmm(x, q, e, 0.5, 64, 4, 2, impl="ref"))
     periodic()
     qmmm()
+    wide_orders()
+    coincident()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
@@ -136,6 +138,88 @@ def periodic():
     np.savez_compressed(os.path.join(HERE, "water999_periodic.npz"), **out)
 
 
+def wide_orders():
+    """7. Source multipole orders m > 2 and field orders F > 2 (the reference
+    infers m from dim, src/fmm.F90:567, and builds tensors of order m + F,
+    :571-577).  Box A, the 999-site water box with multipoles to order 4:
+    octupoles with field order 3 for every damping kind, hexadecapoles with
+    field order 3 undamped and THOLE (tensor order 7: the damped tensors stop
+    at order 6 and the reference drops the rest, src/tensors_damped_thole.F90:
+    55-62), quadrupoles with field order 4.  Box B, 333 sites with multipoles
+    to order 6 and ncrit 16: charges with field order 6, dipoles with field
+    order 5, m = 5 and m = 6 sources.  Each row of a field is independent, so
+    to keep the file small only seeded subsets of rows are stored (rows_a,
+    rows_b, 0-based)."""
+    rng = np.random.default_rng(19)
+    out = {}
+    xa, m10, ea, dfa, erfa = water_inputs(333, 10, 19)
+    na = xa.shape[0]
+    ma = np.zeros((na, 35))
+    ma[:, :10] = m10
+    ma[:, 10:] = rng.normal(0.0, 0.02, (na, 25))
+    ma = np.asfortranarray(ma)
+    xb, m10b, eb, dfb, erfb = water_inputs(111, 10, 23)
+    nb = xb.shape[0]
+    mb = np.zeros((nb, 84))
+    mb[:, :10] = m10b
+    mb[:, 10:] = rng.normal(0.0, 0.02, (nb, 74))
+    mb = np.asfortranarray(mb)
+    rows_a = np.sort(rng.choice(na, 300, replace=False)).astype(np.int32)
+    rows_b = np.sort(rng.choice(nb, 160, replace=False)).astype(np.int32)
+    out.update(a_coordinates=xa, a_multipoles=ma, a_exclusions=ea, a_thole_df=dfa, a_erf_df=erfa, rows_a=rows_a,
+               b_coordinates=xb, b_multipoles=mb, b_exclusions=eb, b_thole_df=dfb, b_erf_df=erfb, rows_b=rows_b)
+    # (box, dim, field order, damping, ncrit, expansion order)
+    cases = [("a", 20, 3, "", 64, 5), ("a", 20, 3, "THOLE", 64, 5), ("a", 20, 3, "AMOEBA", 64, 5),
+             ("a", 20, 3, "ERF", 64, 5), ("a", 35, 3, "", 64, 6), ("a", 35, 3, "THOLE", 64, 6),
+             ("a", 10, 4, "THOLE", 64, 6), ("b", 1, 6, "ERF", 16, 5), ("b", 4, 5, "AMOEBA", 16, 5),
+             ("b", 56, 2, "", 16, 6), ("b", 56, 1, "THOLE", 16, 5), ("b", 84, 1, "", 16, 6),
+             ("b", 84, 0, "AMOEBA", 16, 4)]
+    keys = []
+    for box, dim, F, damp, ncrit, p in cases:
+        x, e = out[f"{box}_coordinates"], out[f"{box}_exclusions"]
+        d = out[f"{box}_erf_df"] if damp == "ERF" else out[f"{box}_thole_df"]
+        mm = np.asfortranarray(out[f"{box}_multipoles"][:, :dim])
+        rows = out[f"rows_{box}"]
+        key = f"{box}_d{dim}_f{F}_{damp.lower() or 'none'}_n{ncrit}_p{p}"
+        out[f"direct_{key}"] = np.asfortranarray(oracle.field_direct(x, mm, e, F, damp, d, impl="ref")[rows])
+        out[f"fmm_{key}"] = np.asfortranarray(oracle.field_fmm(x, mm, e, 0.5, ncrit, p, F, damp, d, impl="ref")[rows])
+        keys.append(key)
+    out["cases"] = np.array(keys)
+    np.savez_compressed(os.path.join(HERE, "water_wide_orders.npz"), **out)
+
+
+def coincident():
+    """8. Sites at one position.  The reference evaluates every listed pair,
+    so two sites at the same position give a non-finite field unless they
+    exclude each other, in which case the pair sits at the 1e10 displacement
+    like any excluded pair (src/fmm.F90:714-721, 802-806).  300 random sites
+    with charges and dipoles; sites 10 and 11 coincide and exclude each other,
+    sites 20 and 21 coincide without excluding each other (their rows are
+    non-finite in the reference, the others finite)."""
+    rng = np.random.default_rng(29)
+    n = 300
+    x = rng.uniform(0.0, 12.0, (n, 3))
+    x[11] = x[10]
+    x[21] = x[20]
+    x = np.asfortranarray(x)
+    m = np.zeros((n, 4))
+    m[:, 0] = rng.normal(0.0, 0.5, n)
+    m[:, 0] -= m[:, 0].mean()
+    m[:, 1:] = rng.normal(0.0, 0.1, (n, 3))
+    m = np.asfortranarray(m)
+    e = np.zeros((n, 2), np.int32)
+    e[:, 0] = np.arange(1, n + 1)
+    e[10, 1], e[11, 1] = 12, 11
+    e = np.asfortranarray(e)
+    df = np.full(n, 0.6)
+    out = dict(coordinates=x, multipoles=m, exclusions=e, thole_df=df)
+    for damp in ("", "THOLE"):
+        key = damp.lower() or "none"
+        out[f"direct_{key}"] = oracle.field_direct(x, m, e, 2, damp, df, impl="ref")
+        out[f"fmm_{key}"] = oracle.field_fmm(x, m, e, 0.5, 16, 5, 2, damp, df, impl="ref")
+    np.savez_compressed(os.path.join(HERE, "coincident300.npz"), **out)
+
+
 def _ref_interpolate(x, le, bi, theta, ncrit, p, F):
     """The reference's interpolate_electron_fields writes the field and then
     aborts in finalize: clean_node deallocates interaction lists that the
@@ -172,7 +256,7 @@ def qmmm():
     targets = np.asfortranarray(c + rng.normal(0.0, 2.0, (6, 3)))
     m2 = np.asfortranarray(np.stack([m, rng.normal(0.0, 0.1, m.shape)], axis=2))
     out = dict(coordinates=x, multipoles=m2, targets=targets)
-    for theta, ncrit, p, F in ((0.5, 64, 4, 1), (0.3, 16, 6, 2)):
+    for theta, ncrit, p, F in ((0.5, 64, 4, 1), (0.3, 16, 6, 2), (0.5, 32, 5, 4)):
         key = f"t{int(theta * 10)}_n{ncrit}_p{p}"
         near, bc, bi = oracle.get_boxes(x, targets, theta, ncrit, p, impl="oracle")
         le = np.asfortranarray(rng.normal(0.0, 1e-3, (len(bi), (p + 1) * (p + 2) * (p + 3) // 6, 2)))
@@ -189,5 +273,9 @@ if __name__ == "__main__":
         periodic()
     elif sys.argv[1:] == ["--qmmm"]:
         qmmm()
+    elif sys.argv[1:] == ["--wide"]:
+        wide_orders()
+    elif sys.argv[1:] == ["--coincident"]:
+        coincident()
     else:
         main()

@@ -10,7 +10,7 @@ import oracle
 from conftest import block_rel, load_golden
 
 pytestmark = pytest.mark.gpu
-CASES = [("t5_n64_p4", 0.5, 64, 4), ("t3_n16_p6", 0.3, 16, 6)]
+CASES = [("t5_n64_p4", 0.5, 64, 4), ("t3_n16_p6", 0.3, 16, 6), ("t5_n32_p5", 0.5, 32, 5)]
 
 
 @pytest.mark.parametrize("key,theta,ncrit,p", CASES)

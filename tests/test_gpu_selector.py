@@ -323,6 +323,41 @@ def test_learned_orders_with_many_sets(trained):
         ctx.close()
 
 
+@pytest.mark.parametrize("S", [2, 6])
+def test_learned_orders_with_unlike_sets(trained, S):
+    """Sets of very different size in one call (the last set: dipoles 1000x smaller, no charges): the
+    per-pair tolerance comes from the smallest near-field RMS over the sets and the spot checks cover
+    every set, so every set meets the tolerance relative to its own field, or the call falls back to
+    exactly the full-order result."""
+    from fmm_amd.octree import Context
+    _, _, _, (W, norm, meta) = trained
+    x, m, e, _ = _water(2000, 59)
+    n = x.shape[0]
+    rng = np.random.default_rng(13)
+    M = np.zeros((n, 4, S), order="F")
+    for k in range(S - 1):
+        M[:, 0, k] = m[:, 0]
+        M[:, 1:4, k] = rng.normal(0.0, 0.05, (n, 3))
+    M[:, 1:4, S - 1] = rng.normal(0.0, 5e-5, (n, 3))
+    ctx = Context(0)
+    try:
+        ctx.set_selector_weights(W, norm, meta)
+        full = ctx.field_fmm_sets(0, x, M, e, 0.4, 64, 8, 1)
+        ctx.set_order_tolerance(1e-4, spot_checks=256)
+        sel = ctx.field_fmm_sets(0, x, M, e, 0.4, 64, 8, 1)
+        st = ctx.selection_stats()
+        assert st["active"]
+        if st["fallback"]:
+            assert np.array_equal(sel, full)
+        else:
+            assert st["spot_error"] <= 1e-4
+            for k in range(S):
+                d = ctx.field_direct(0, x, np.asfortranarray(M[:, :, k]), e, 1)
+                assert rel(sel[:, 1:4, k], d[:, 1:4]) <= 2e-4 + rel(full[:, 1:4, k], d[:, 1:4]), k
+    finally:
+        ctx.close()
+
+
 def test_uneven_batch_of_systems_meets_tolerance():
     """The uneven batch at 1/50 scale: water systems of uneven size, 8 THOLE dipole
     sets each, learned orders (cap 10, tolerance 1e-6, spot checks) in mixed

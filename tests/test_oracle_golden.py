@@ -107,3 +107,46 @@ def test_periodic_restatement_vs_reference(key):
     diag = oracle.periodic_diagnostics(X, M, E, 0.5, 64, p, F, damp, np.ascontiguousarray(df), box, out)
     assert max(block_rel(out, g[key], F, damp == "AMOEBA")) < 1e-12
     assert diag[2] == int(g[key + "_levels"])
+
+
+def _wide_cases():
+    return [str(k) for k in load_golden("water_wide_orders.npz")["cases"]]
+
+
+def _wide_inputs(g, key):
+    """(coordinates, multipoles, exclusions, damping factors, field order, ncrit, p, damp, rows) of a case
+    key box_d<dim>_f<F>_<damp>_n<ncrit>_p<p>."""
+    box, d, f, damp, n, p = key.split("_")
+    dim, F, ncrit, p = int(d[1:]), int(f[1:]), int(n[1:]), int(p[1:])
+    damp = "" if damp == "none" else damp.upper()
+    df = g[f"{box}_erf_df"] if damp == "ERF" else g[f"{box}_thole_df"]
+    m = np.asfortranarray(g[f"{box}_multipoles"][:, :dim])
+    return g[f"{box}_coordinates"], m, g[f"{box}_exclusions"], df, F, ncrit, p, damp, g[f"rows_{box}"]
+
+
+@pytest.mark.parametrize("key", _wide_cases())
+def test_wide_orders(key):
+    """Source orders m > 2, field orders F > 2, damped tensor order 7 (truncated at 6 like the reference)."""
+    g = load_golden("water_wide_orders.npz")
+    x, m, e, df, F, ncrit, p, damp, rows = _wide_inputs(g, key)
+    skip = damp == "AMOEBA"
+    fd = oracle.field_direct(x, m, e, F, damp, df)[rows]
+    assert max(block_rel(fd, g[f"direct_{key}"], F, skip)) < PARITY
+    ff = oracle.field_fmm(x, m, e, 0.5, ncrit, p, F, damp, df)[rows]
+    assert max(block_rel(ff, g[f"fmm_{key}"], F)) < PARITY
+
+
+@pytest.mark.parametrize("damp", ["", "THOLE"])
+def test_coincident_sites(damp):
+    """Sites 10/11 share a position and exclude each other (finite, the pair at the 1e10 displacement);
+    sites 20/21 share one without an exclusion (non-finite in the reference)."""
+    g = load_golden("coincident300.npz")
+    x, m, e, df = g["coordinates"], g["multipoles"], g["exclusions"], g["thole_df"]
+    key = damp.lower() or "none"
+    fin = np.ones(x.shape[0], bool)
+    fin[[20, 21]] = False
+    for got, ref in ((oracle.field_direct(x, m, e, 2, damp, df), g[f"direct_{key}"]),
+                     (oracle.field_fmm(x, m, e, 0.5, 16, 5, 2, damp, df), g[f"fmm_{key}"])):
+        assert not np.isfinite(ref[~fin]).all(axis=1).any() and np.isfinite(ref[fin]).all()
+        assert not np.isfinite(got[~fin]).all(axis=1).any() and np.isfinite(got[fin]).all()
+        assert max(block_rel(got[fin], ref[fin], 2)) < PARITY
