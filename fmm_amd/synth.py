@@ -155,18 +155,28 @@ def config(idx, scale=1.0):
         # truncation error, and at 0.3 the largest THOLE systems still miss 1e-6 at order 10 (spot errors 1.0e-6 at 211k and 2.2e-6 at 494k
         # sites: damping shrinks the near field the error is relative to), so a 1e-6 tolerance could only fall
         # back there; 0.25 meets it on every system with learned orders (mean order 7-8).
-        rng = np.random.default_rng(seed)
-        nsys, sets = 64, 8
-        sizes = np.exp(rng.uniform(np.log(10_000.0), np.log(500_000.0), nsys)) * scale
-        systems = []
-        for k, size in enumerate(sizes):
-            x, q, excl, df, r2 = water_box(max(int(size / 3), 2), seed * 100 + k)
-            n = x.shape[0]
-            m = np.zeros((n, 4, sets), order="F")
-            for u in range(sets):
-                m[:, 0, u] = q
-                m[:, 1:4, u] = r2.normal(0.0, 0.05, (n, 3))
-            systems.append(dict(coordinates=x, multipoles=m, exclusions=excl, damping_factors=df))
+        sizes = uneven_sizes(scale)
+        systems = [uneven_system(k, size) for k, size in enumerate(sizes)]
         return dict(name="water-uneven-64-systems-8sets-thole", systems=systems, theta=0.25, ncrit=64,
-                    expansion_order=10, field_order=1, damp_type="THOLE", sets=sets, order_tolerance=1e-6)
+                    expansion_order=10, field_order=1, damp_type="THOLE", sets=UNEVEN_SETS, order_tolerance=1e-6)
     raise ValueError(f"unknown config {idx}")
+
+
+UNEVEN_SYSTEMS, UNEVEN_SETS = 64, 8
+
+
+def uneven_sizes(scale=1.0):
+    """Site counts (before rounding to whole molecules) of the uneven batch's systems."""
+    rng = np.random.default_rng(1004)
+    return np.exp(rng.uniform(np.log(10_000.0), np.log(500_000.0), UNEVEN_SYSTEMS)) * scale
+
+
+def uneven_system(k, size):
+    """System k of the uneven batch (a water box of about `size` sites, 8 dipole sets)."""
+    x, q, excl, df, r2 = water_box(max(int(size / 3), 2), 1004 * 100 + k)
+    n = x.shape[0]
+    m = np.zeros((n, 4, UNEVEN_SETS), order="F")
+    for u in range(UNEVEN_SETS):
+        m[:, 0, u] = q
+        m[:, 1:4, u] = r2.normal(0.0, 0.05, (n, 3))
+    return dict(coordinates=x, multipoles=m, exclusions=excl, damping_factors=df)

@@ -51,12 +51,22 @@ def _prep(coordinates, multipoles, exclusions, damping_factors):
 
 
 def field_direct(coordinates, multipoles, exclusions, field_order, damp_type="", damping_factors=(),
-                 impl="oracle"):
-    """impl: 'oracle' (C++ restatement), 'ref' or 'ref_gnu' (compiled reference)."""
+                 impl="oracle", rows=None):
+    """impl: 'oracle' (C++ restatement), 'ref' or 'ref_gnu' (compiled reference).
+    rows (oracle only): evaluate only these target sites (0-based); returns len(rows) x nf."""
     X, M, E, df = _prep(coordinates, multipoles, exclusions, damping_factors)
     n, dim = M.shape
-    F = np.zeros((n, ncomp(field_order)), order="F")
     d = DAMP[damp_type.strip()]
+    if rows is not None:
+        assert impl == "oracle", "rows: oracle only"
+        R = np.ascontiguousarray(rows, dtype=np.int32)
+        F = np.zeros((R.size, ncomp(field_order)), order="F")
+        rc = _lib("liboracle.so").orc_field_direct_rows(n, dim, _p(X), _p(M), _p(E), E.shape[1], field_order, d,
+                                                        _p(df), _p(R), R.size, _p(F))
+        if rc:
+            raise ValueError("rows outside 0..n-1")
+        return F
+    F = np.zeros((n, ncomp(field_order)), order="F")
     if impl == "oracle":
         _lib("liboracle.so").orc_field_direct(n, dim, _p(X), _p(M), _p(E), E.shape[1], field_order, d, _p(df), _p(F))
     else:

@@ -511,12 +511,17 @@ static bool excluded(const Inputs& in, int i, int j) {
     return false;
 }
 
-// src/fmm.F90:749-824 (particle_fields_direct) via src/fmm.F90:1338-1352
-static void field_direct(const Inputs& in) {
+// src/fmm.F90:749-824 (particle_fields_direct) via src/fmm.F90:1338-1352.
+// rows: the target sites (0-based) whose rows of the reference's loop over i
+// are evaluated (each row is independent); F row r receives site rows[r].
+// null: every site, F row i for site i.
+static void field_direct(const Inputs& in, const int* rows = nullptr, int nrows = -1) {
     int n = in.n;
     int nf = ncomp(in.field_order);
     std::vector<double> dx(n), dy(n), dz(n), a(n), F(nf);
-    for (int i = 0; i < n; ++i) {
+    if (!rows) nrows = n;
+    for (int r = 0; r < nrows; ++r) {
+        const int i = rows ? rows[r] : r;
         for (int j = 0; j < n; ++j) {
             dx[j] = in.X[i] - in.X[j];
             dy[j] = in.X[in.ldx + i] - in.X[in.ldx + j];
@@ -529,7 +534,7 @@ static void field_direct(const Inputs& in) {
         }
         if (in.damp) for (int j = 0; j < n; ++j) a[j] = in.df[i] * in.df[j];
         multipole_field(in.M, in.ldm, in.dim, n, dx.data(), dy.data(), dz.data(), in.field_order, in.damp, a.data(), F.data());
-        for (int t = 0; t < nf; ++t) in.F[(size_t)t * in.ldf + i] += F[t];
+        for (int t = 0; t < nf; ++t) in.F[(size_t)t * in.ldf + r] += F[t];
     }
 }
 
@@ -827,6 +832,16 @@ int orc_field_direct(int n, int dim, const double* X, const double* M, const int
                      int damp, const double* df, double* F) {
     orc::Inputs in{n, dim, nexcl, field_order, damp, X, n, M, n, E, n, df, F, n};
     orc::field_direct(in);
+    return 0;
+}
+
+// field_direct rows only: F is nrows x ncomp(field_order) (leading dimension nrows), row r for site rows[r].
+int orc_field_direct_rows(int n, int dim, const double* X, const double* M, const int* E, int nexcl, int field_order,
+                          int damp, const double* df, const int* rows, int nrows, double* F) {
+    for (int r = 0; r < nrows; ++r)
+        if (rows[r] < 0 || rows[r] >= n) return 1;
+    orc::Inputs in{n, dim, nexcl, field_order, damp, X, n, M, n, E, n, df, F, nrows};
+    orc::field_direct(in, rows, nrows);
     return 0;
 }
 

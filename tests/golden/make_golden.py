@@ -6,7 +6,7 @@ script calls its field_direct / field_fmm through the bind(C) wrapper
 oracle/ref_bindc.F90 and stores inputs and outputs only.
 
 Run in the build container (it needs the reference sources):
-    make -C oracle ref && python tests/golden/make_golden.py [--periodic | --qmmm | --wide | --coincident]
+    make -C oracle ref && python tests/golden/make_golden.py [--periodic | --qmmm | --wide | --coincident | --configs]
 """
 import os
 import sys
@@ -80,6 +80,7 @@ def main():
     qmmm()
     wide_orders()
     coincident()
+    reduced_configs()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
@@ -220,6 +221,70 @@ def coincident():
     np.savez_compressed(os.path.join(HERE, "coincident300.npz"), **out)
 
 
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.asfortranarray(a).tobytes(order="F")).hexdigest()
+
+
+def _oracle_fmm_set(args):
+    x, m, e, theta, ncrit, p, F, damp, df, rows = args
+    return oracle.field_fmm(x, m, e, theta, ncrit, p, F, damp, df)[rows]
+
+
+def reduced_configs():
+    """9. The benchmark workloads at reduced size, where the full-size ones
+    cannot be run by the reference or the restatement in seconds:
+      - water20k: the 200k-site water box's recipe at 1/10 scale (19,998
+        sites, THOLE, dipoles, theta 0.5, ncrit 64, p 6, field order 2), the
+        reference's field_fmm;
+      - clusters12k: the 1M-site cluster recipe at 24 of its 2048 clusters
+        (12,288 sites, ERF, quadrupoles, theta 0.6, ncrit 64, field order 1):
+        the reference's field_fmm at p 6, the restatement's at p 8 (where the
+        reference's tensors overflow) and the restatement's direct sum at 512
+        seeded sites;
+      - uneven k: systems 0 and 23 of the uneven batch (10,032 and 14,079
+        sites, 8 THOLE dipole sets, field order 1), the restatement's
+        field_fmm at the batch's full order (p 10, theta 0.25) for every set.
+    Rows: seeded subsets (each row of a field is independent).  Inputs are
+    stored except where the generator is integer-exact (exclusions, and the
+    uneven systems' multipoles: fixed charges plus PCG64 normal draws, checked
+    by their sha256)."""
+    from multiprocessing import Pool
+    rng = np.random.default_rng(31)
+    out = {}
+    c = synth.config(1, scale=0.1)
+    x, m, e, df = c["coordinates"], c["multipoles"], c["exclusions"], c["damping_factors"]
+    rows = np.sort(rng.choice(x.shape[0], 2048, replace=False)).astype(np.int32)
+    out.update(water20k_coordinates=x, water20k_multipoles=m, water20k_damping_factors=df, water20k_rows=rows,
+               water20k_exclusions_sha=_sha(e))
+    out["water20k_fmm_ref_p6"] = np.asfortranarray(
+        oracle.field_fmm(x, m, e, 0.5, 64, 6, 2, "THOLE", df, impl="ref")[rows])
+    c = synth.config(3, scale=24 / 2048)
+    x, m, e, df = c["coordinates"], c["multipoles"], c["exclusions"], c["damping_factors"]
+    rows = np.sort(rng.choice(x.shape[0], 2048, replace=False)).astype(np.int32)
+    spots = np.sort(rng.choice(x.shape[0], 512, replace=False)).astype(np.int32)
+    out.update(clusters12k_coordinates=x, clusters12k_multipoles=m, clusters12k_damping_factors=df,
+               clusters12k_rows=rows, clusters12k_spots=spots)
+    out["clusters12k_fmm_ref_p6"] = np.asfortranarray(
+        oracle.field_fmm(x, m, e, 0.6, 64, 6, 1, "ERF", df, impl="ref")[rows])
+    out["clusters12k_fmm_oracle_p8"] = np.asfortranarray(oracle.field_fmm(x, m, e, 0.6, 64, 8, 1, "ERF", df)[rows])
+    out["clusters12k_direct_spots"] = oracle.field_direct(x, m, e, 1, "ERF", df, rows=spots)
+    sizes = synth.uneven_sizes()
+    for k in (0, 23):
+        sy = synth.uneven_system(k, sizes[k])
+        x, m, e, df = sy["coordinates"], sy["multipoles"], sy["exclusions"], sy["damping_factors"]
+        rows = np.sort(rng.choice(x.shape[0], 1024, replace=False)).astype(np.int32)
+        jobs = [(x, np.asfortranarray(m[:, :, u]), e, 0.25, 64, 10, 1, "THOLE", df, rows)
+                for u in range(m.shape[2])]
+        with Pool(min(8, os.cpu_count() or 1)) as pool:
+            full = pool.map(_oracle_fmm_set, jobs)
+        out.update({f"uneven{k}_coordinates": x, f"uneven{k}_damping_factors": df, f"uneven{k}_rows": rows,
+                    f"uneven{k}_multipoles_sha": _sha(m), f"uneven{k}_exclusions_sha": _sha(e),
+                    f"uneven{k}_size": sizes[k],
+                    f"uneven{k}_fmm_oracle_p10": np.asfortranarray(np.stack(full, axis=2))})
+    np.savez_compressed(os.path.join(HERE, "configs_reduced.npz"), **out)
+
+
 def _ref_interpolate(x, le, bi, theta, ncrit, p, F):
     """The reference's interpolate_electron_fields writes the field and then
     aborts in finalize: clean_node deallocates interaction lists that the
@@ -277,5 +342,7 @@ if __name__ == "__main__":
         wide_orders()
     elif sys.argv[1:] == ["--coincident"]:
         coincident()
+    elif sys.argv[1:] == ["--configs"]:
+        reduced_configs()
     else:
         main()

@@ -150,3 +150,47 @@ def test_coincident_sites(damp):
         assert not np.isfinite(ref[~fin]).all(axis=1).any() and np.isfinite(ref[fin]).all()
         assert not np.isfinite(got[~fin]).all(axis=1).any() and np.isfinite(got[fin]).all()
         assert max(block_rel(got[fin], ref[fin], 2)) < PARITY
+
+
+def test_direct_rows_equal_full_direct():
+    """The target-rows form of the restatement's direct sum gives the full sum's rows bit for bit."""
+    g = load_golden("uniform4096_p5.npz")
+    X, M, E = g["coordinates"], g["multipoles"], g["exclusions"]
+    rows = np.array([4095, 0, 17, 1000, 17, 2048], dtype=np.int32)
+    full = oracle.field_direct(X, M, E, 1)
+    np.testing.assert_array_equal(oracle.field_direct(X, M, E, 1, rows=rows), full[rows])
+    assert max(block_rel(full[rows], g["field_direct"][rows], 1)) < PARITY
+
+
+def test_reduced_configs_fixture_matches_the_generators():
+    """The reduced-size workloads of configs_reduced.npz are the benchmark recipes (fmm_amd.synth) at
+    reduced size: regenerated here, the stored inputs match bit for bit."""
+    from fmm_amd import synth
+    g = load_golden("configs_reduced.npz")
+    c = synth.config(1, scale=0.1)
+    for k in ("coordinates", "multipoles", "damping_factors"):
+        np.testing.assert_array_equal(c[k], g[f"water20k_{k}"])
+    c = synth.config(3, scale=24 / 2048)
+    for k in ("coordinates", "multipoles", "damping_factors"):
+        np.testing.assert_array_equal(c[k], g[f"clusters12k_{k}"])
+    sizes = synth.uneven_sizes()
+    for k in (0, 23):
+        sy = synth.uneven_system(k, sizes[k])
+        np.testing.assert_array_equal(sy["coordinates"], g[f"uneven{k}_coordinates"])
+        np.testing.assert_array_equal(sy["damping_factors"], g[f"uneven{k}_damping_factors"])
+
+
+@pytest.mark.parametrize("case", ["water20k", "clusters12k"])
+def test_reduced_configs_oracle_vs_reference(case):
+    from fmm_amd import synth
+    g = load_golden("configs_reduced.npz")
+    x, m, df = g[f"{case}_coordinates"], g[f"{case}_multipoles"], g[f"{case}_damping_factors"]
+    rows = g[f"{case}_rows"]
+    if case == "water20k":
+        e = synth.config(1, scale=0.1)["exclusions"]
+        f = oracle.field_fmm(x, m, e, 0.5, 64, 6, 2, "THOLE", df)[rows]
+        assert max(block_rel(f, g["water20k_fmm_ref_p6"], 2)) < PARITY
+    else:
+        e = synth.self_exclusions(x.shape[0])
+        f = oracle.field_fmm(x, m, e, 0.6, 64, 6, 1, "ERF", df)[rows]
+        assert max(block_rel(f, g["clusters12k_fmm_ref_p6"], 1)) < PARITY
