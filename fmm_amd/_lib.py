@@ -16,6 +16,8 @@ _PROTOS = {
     "fmmh_parse_damp_type": (i32, [_c.c_char_p, i64, _c.POINTER(i32)]),
     "fmmh_field_direct": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, vp, i64, i32, i32, vp, i64, i32, vp, i64]),
     "fmmh_field_direct_dev": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, vp, i64, i32, i32, vp, i64, i32, vp, i64]),
+    "fmmh_field_direct_dev_noalloc": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, vp, i64, i32, i32, vp, i64, i32, vp,
+                                            i64]),
     "fmmh_field_fmm": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, vp, i64, i32, f64, i32, i32, i32, vp, i64, i32,
                              vp, i64, i32, vp]),
     "fmmh_field_fmm_dev": (i32, [vp, i32, vp, i64, vp, i64, i32, i32, vp, i64, i32, f64, i32, i32, i32, vp, i64,

@@ -260,11 +260,42 @@ int guarded(F&& f) {
     }
 }
 
+// ---------------------------------------------------------------- fixed working memory
+// fmmh_field_direct_dev_noalloc keeps its working arrays in this array, one copy per device that the
+// library runs on, held until the process ends.  Host code never uses the name `scratch` as an address
+// (on the host it is a stand-in, not device memory): the device address of each device's copy is taken
+// once, with that device current, when the first context on the device is created, and kept in
+// g_scratch_ptr.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)FMMH_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The calling thread's current device must be `device`.
+void scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw Error(FMMH_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    FMMH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(FMMH_ERR_HIP, "hipGetSymbolAddress returned no address for the fixed working memory");
+    g_scratch_ptr[device] = ptr;
+}
+
+void* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    void* ptr = device >= 0 && device < MAX_DEVICES ? g_scratch_ptr[device] : nullptr;
+    if (!ptr) throw Error(FMMH_ERR_INTERNAL, "fixed working memory of the device was not set up");
+    return ptr;
+}
+
 fmmh_context* make_context(int device) {
     fmmh_context* c = new fmmh_context;
     if (device < 0) FMMH_HIP(hipGetDevice(&device));
     c->device = device;
     FMMH_HIP(hipSetDevice(device));
+    scratch_setup(device);
     FMMH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     FMMH_HIP(hipHostMalloc((void**)&c->h_pinned, 4096, hipHostMallocDefault));
     return c;
@@ -847,6 +878,104 @@ void run_direct(fmmh_context* c, const Call& k) {
     launch_near(a, nb, m, k.F, k.damp, s);
     c->mark("direct");
     bad_excl_readback(c);
+    FMMH_HIP(hipStreamSynchronize(s));
+    bad_excl_check(c);
+}
+
+// ---------------------------------------------------------------- direct sum without allocation
+// run_direct with every working array carved out of the fixed per-device memory (`base`, as returned by
+// scratch_ptr): no allocation, no host-to-device copy.  Layout (each part 256-byte aligned): xs, ys, zs,
+// dfs (n doubles each), Taylor-scaled multipoles (n x dim doubles; the direct sum needs no raw copy),
+// exclusions (n x nexcl ints), site order (max(n, blocks) ints),
+// block start / count / list offsets / list sources, source block start and count, exclusion flag,
+// image displacement, Taylor factors.
+struct NoallocLayout {
+    size_t xs, ys, zs, dfs, ms, excl, iota, start, count, loff, lsrc, sstart, scount, bad, disp, scale, total;
+};
+
+NoallocLayout noalloc_layout(int n, int dim, int nexcl) {
+    const size_t nb = (size_t)nblk(n, 64);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) / 256 * 256;
+        return at;
+    };
+    NoallocLayout L{};
+    L.xs = take((size_t)n * 8);
+    L.ys = take((size_t)n * 8);
+    L.zs = take((size_t)n * 8);
+    L.dfs = take((size_t)n * 8);
+    L.ms = take((size_t)n * dim * 8);
+    L.excl = take((size_t)n * std::max(nexcl, 1) * 4);
+    L.iota = take(std::max((size_t)n, nb) * 4);
+    L.start = take(nb * 4);
+    L.count = take(nb * 4);
+    L.loff = take((nb + 1) * 4);
+    L.lsrc = take(nb * 4);
+    L.sstart = take(4);
+    L.scount = take(4);
+    L.bad = take(4);
+    L.disp = take(3 * 8);
+    L.scale = take((size_t)dim * 8);
+    L.total = off;
+    return L;
+}
+
+// The one source block [0, n), the central-cell image displacement, the exclusion flag and the Taylor
+// factors (-1)^{|s|+1} / (sx! sy! sz!) of the dim source components (src/fmm.F90:597-612), on the device.
+__global__ void k_noalloc_setup(int n, int dim, int* sstart, int* scount, int* bad, double* disp, double* scale) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+        *sstart = 0;
+        *scount = n;
+        *bad = 0;
+    }
+    if (t < 3) disp[t] = 0.0;
+    for (int s = t; s < dim; s += blockDim.x) {
+        int k = 0;
+        while (ncomp(k) <= s) ++k;
+        int sx = 0, sy = 0, sz = 0;
+        for (int x = k; x >= 0; --x)
+            for (int y = k - x; y >= 0; --y)
+                if (xyz2idx(x, y, k - x - y) == s) { sx = x; sy = y; sz = k - x - y; }
+        double f = 1.0;
+        for (int i = 2; i <= sx; ++i) f *= i;
+        for (int i = 2; i <= sy; ++i) f *= i;
+        for (int i = 2; i <= sz; ++i) f *= i;
+        scale[s] = ((k + 1) % 2 ? -1.0 : 1.0) / f;
+    }
+}
+
+void run_direct_noalloc(fmmh_context* c, const Call& k, void* base) {
+    hipStream_t s = c->stream;
+    const int n = k.n;
+    const int m = order_from_dim(k.dim);
+    const int nb = nblk(n, 64);
+    const NoallocLayout L = noalloc_layout(n, k.dim, k.nexcl);
+    if (L.total > NOALLOC_SCRATCH_BYTES)
+        throw Error(FMMH_ERR_UNSUPPORTED, "field_direct without allocation: the working arrays of " +
+                                              std::to_string(n) + " sites (" + std::to_string(L.total) +
+                                              " bytes) exceed the fixed " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                              " bytes");
+    unsigned char* b = static_cast<unsigned char*>(base);
+    auto D = [&](size_t off) { return reinterpret_cast<double*>(b + off); };
+    auto I = [&](size_t off) { return reinterpret_cast<int*>(b + off); };
+    hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(64), 0, s, n, k.dim, I(L.sstart), I(L.scount), I(L.bad),
+                       D(L.disp), D(L.scale));
+    hipLaunchKernelGGL(k_direct_blocks, dim3(nblk(nb, 256)), dim3(256), 0, s, I(L.start), I(L.count), I(L.loff),
+                       I(L.lsrc), nb, n, 0);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(std::max(n, nb), 256)), dim3(256), 0, s, I(L.iota), std::max(n, nb));
+    SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, I(L.iota), n,
+               D(L.xs), D(L.ys), D(L.zs), nullptr, D(L.ms), D(L.dfs), I(L.excl), D(L.scale), 1};
+    launch_gather(g, s);
+    NearArgs a{I(L.iota), I(L.start), I(L.count), I(L.loff), I(L.lsrc), I(L.sstart), I(L.scount), D(L.xs), D(L.ys),
+               D(L.zs), D(L.ms), n, D(L.dfs), I(L.iota), I(L.excl), n, k.nexcl, k.field, k.ldf, 1, 1, D(L.disp)};
+    a.nsites = n;
+    a.bad = I(L.bad);
+    a.coinc_list = 1;
+    launch_near(a, nb, m, k.F, k.damp, s);
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + PIN_BAD_EXCL, I(L.bad), sizeof(int), hipMemcpyDeviceToHost, s));
     FMMH_HIP(hipStreamSynchronize(s));
     bad_excl_check(c);
 }
@@ -2031,6 +2160,22 @@ int fmmh_field_direct_dev(fmmh_context* ctx, int32_t comm, const double* coordin
     return field_direct_impl(ctx, false, coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, 1,
                              exclusions, ld_exclusions, nexcl, field_order, field, ld_field, damp_type,
                              damping_factors, n_damping);
+}
+
+int fmmh_field_direct_dev_noalloc(fmmh_context* ctx, int32_t comm, const double* coordinates,
+                                  int64_t ld_coordinates, const double* multipoles, int64_t ld_multipoles, int32_t n,
+                                  int32_t dim, const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                                  int32_t field_order, double* field, int64_t ld_field, int32_t damp_type,
+                                  const double* damping_factors, int64_t n_damping) {
+    (void)comm;
+    return guarded([&] {
+        if (!ctx) throw Error(FMMH_ERR_ARG, "field_direct without allocation needs a context (fmmh_context_create)");
+        FMMH_HIP(hipSetDevice(ctx->device));
+        Call h = make_call(coordinates, ld_coordinates, multipoles, ld_multipoles, n, dim, exclusions, ld_exclusions,
+                           nexcl, field_order, field, ld_field, damp_type, damping_factors, 1);
+        check_common(h, n_damping);
+        run_direct_noalloc(ctx, h, scratch_ptr(ctx->device));
+    });
 }
 
 int fmmh_field_fmm(fmmh_context* ctx, int32_t comm, const double* coordinates, int64_t ld_coordinates,

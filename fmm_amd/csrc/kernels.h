@@ -123,7 +123,7 @@ struct SiteArgs {
     const int* perm;
     int n;
     double* xs; double* ys; double* zs;
-    double* ms_raw; double* ms_scaled;   // component-major, leading dimension n
+    double* ms_raw; double* ms_scaled;   // component-major, leading dimension n (ms_raw null: not written)
     double* dfs;
     int* excl_s;                         // component-major, leading dimension n
     const double* scale;                 // per multipole component factor (device, dim entries)

@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void k_gather(SiteArgs a) {
     // multipole column cc = set * dim + component, the same leading dimension for all sets
     for (int cc = 0; cc < a.dim * a.nsets; ++cc) {
         const double v = a.M[(int64_t)cc * a.ldm + o];
-        a.ms_raw[(int64_t)cc * a.n + k] = v;
+        if (a.ms_raw) a.ms_raw[(int64_t)cc * a.n + k] = v;
         a.ms_scaled[(int64_t)cc * a.n + k] = a.scale[cc % a.dim] * v;
     }
     a.dfs[k] = a.df ? a.df[o] : 0.0;

@@ -238,6 +238,19 @@ class Context:
             self._dptr(field), self._ld(field), _damp_code(damp_type), self._dptr(damping_factors), ndf))
         return field
 
+    def field_direct_device_noalloc(self, coordinates, multipoles, exclusions, field_order, field, damp_type="",
+                                    damping_factors=None):
+        """field_direct_device with the working arrays in the library's fixed per-device memory: no
+        allocation (fmmh_field_direct_dev_noalloc)."""
+        n, dim = multipoles.shape
+        nexcl = exclusions.shape[1] if exclusions.dim() == 2 else 1
+        ndf = damping_factors.numel() if damping_factors is not None else 0
+        check(self._lib.fmmh_field_direct_dev_noalloc(
+            self._h, 0, self._dptr(coordinates), self._ld(coordinates), self._dptr(multipoles),
+            self._ld(multipoles), n, dim, self._dptr(exclusions), self._ld(exclusions), nexcl, int(field_order),
+            self._dptr(field), self._ld(field), _damp_code(damp_type), self._dptr(damping_factors), ndf))
+        return field
+
     def field_fmm_device(self, coordinates, multipoles, exclusions, theta, ncrit, expansion_order, field_order,
                          field, damp_type="", damping_factors=None, is_periodic=False, box_matrix=None):
         n, dim = multipoles.shape

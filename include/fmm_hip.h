@@ -124,6 +124,28 @@ int fmmh_field_fmm_dev(fmmh_context* ctx, int32_t comm,
                        int32_t damp_type, const double* damping_factors, int64_t n_damping,
                        int32_t is_periodic, const double* box_matrix);
 
+/* fmmh_field_direct_dev without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Same arguments and results as fmmh_field_direct_dev (bit for bit), one
+ * multipole set; ctx must be a context made by fmmh_context_create (NULL is
+ * refused).  The working arrays live in a fixed buffer of
+ * FMMH_NOALLOC_SCRATCH_BYTES per device, reserved when the library is loaded
+ * on a device and held until the process ends; calls whose working arrays do
+ * not fit return FMMH_ERR_UNSUPPORTED (about 128 bytes per site for
+ * quadrupoles and 3 exclusions, plus 8 per further multipole component and 4
+ * per further exclusion: ~260,000 quadrupole sites).  Every call on one
+ * device uses the same buffer, whatever the context: calls that overlap on
+ * one device race.  Kernels of source / field orders above 2 use per-lane
+ * private memory, which the HIP runtime reserves on their first launch. */
+#define FMMH_NOALLOC_SCRATCH_BYTES (32ll << 20)
+int fmmh_field_direct_dev_noalloc(fmmh_context* ctx, int32_t comm,
+                                  const double* coordinates, int64_t ld_coordinates,
+                                  const double* multipoles, int64_t ld_multipoles, int32_t n, int32_t dim,
+                                  const int32_t* exclusions, int64_t ld_exclusions, int32_t nexcl,
+                                  int32_t field_order, double* field, int64_t ld_field,
+                                  int32_t damp_type, const double* damping_factors, int64_t n_damping);
+
 /* Many multipole sets on one set of sites: nsets independent field_direct /
  * field_fmm calls (same coordinates, exclusions, damping and parameters) in
  * one call that builds the tree and interaction lists once.  Replaces a loop
