@@ -33,12 +33,23 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector = FP64 matrix peak
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the near-field kernel on the default workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (one run each) scaled by the same
-# counters on a plain 1 GiB device copy (FETCH_SIZE factor 2.000, WRITE_SIZE
-# 1.000): 43,544 KiB x 2 read + 83,700 KiB written (the written part includes
-# the partial fields of chunked leaves).  The profiler cannot run
-# inside this process, so the figure is the recorded measurement of
-# profiles/k_near_water200k_thole_p6.md.
-K_NEAR_TRAFFIC_BYTES = (43544.0 * 2.0 + 83700.0) * 1024.0
+# counters on a plain 1 GiB device copy (factors in the profile).  The profiler
+# cannot run inside this process, so the figure is the recorded measurement of
+# profiles/k_near_water200k_thole_p6.md, valid for the near-field sources it
+# was measured on: K_NEAR_TRAFFIC_SRC is the sha256 prefix of those sources
+# (near_src_sha()); with other sources the bench line reports null.
+K_NEAR_TRAFFIC_BYTES = None
+K_NEAR_TRAFFIC_SRC = None
+NEAR_SOURCES = ("fmm_amd/csrc/near_kernels.h", "fmm_amd/csrc/near.hip", "fmm_amd/csrc/tensors.h")
+
+
+def near_src_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for f in NEAR_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def c_T(K):
@@ -71,25 +82,102 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def cpu_baseline(seconds_hint=True):
-    """The reference Fortran (compiled from its own sources into oracle/_ref)
-    on one host core, on a 1/10-scale sample of the same workload."""
+def _host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline():
+    """The reference Fortran (compiled from its own sources into oracle/_ref) on one host core: value = its
+    field_fmm with the shipped flags (-O1 -g) on a 1/10-scale sample of the benchmarked workload; beside it
+    the same sample at -O3, and the 4,096-site quadrupole workload in full (field_fmm and field_direct)."""
     import oracle
     from fmm_amd import synth
     if not oracle.have_ref():
         return None
     c = synth.config(1, scale=0.1)
     n = c["coordinates"].shape[0]
-    t0 = time.perf_counter()
-    oracle.field_fmm(c["coordinates"], c["multipoles"], c["exclusions"], c["theta"], c["ncrit"],
-                     c["expansion_order"], c["field_order"], c["damp_type"], c["damping_factors"], impl="ref")
-    dt = time.perf_counter() - t0
+    args = (c["coordinates"], c["multipoles"], c["exclusions"], c["theta"], c["ncrit"], c["expansion_order"],
+            c["field_order"], c["damp_type"], c["damping_factors"])
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+    dt = timed(lambda: oracle.field_fmm(*args, impl="ref"))
+    extra = {}
+    if oracle.have_ref("_o3"):
+        dt3 = timed(lambda: oracle.field_fmm(*args, impl="ref_o3"))
+        extra["same_sample_O3"] = {"seconds": dt3, "value": n / dt3}
+    c0 = synth.config(0)
+    n0 = c0["coordinates"].shape[0]
+    a0 = (c0["coordinates"], c0["multipoles"], c0["exclusions"])
+    t_f = timed(lambda: oracle.field_fmm(*a0, c0["theta"], c0["ncrit"], c0["expansion_order"], c0["field_order"],
+                                         impl="ref"))
+    t_d = timed(lambda: oracle.field_direct(*a0, c0["field_order"], impl="ref"))
+    extra["uniform4096_quadrupole_field_fmm"] = {"seconds": t_f, "value": n0 / t_f}
+    extra["uniform4096_quadrupole_field_direct"] = {"seconds": t_d, "value": n0 / t_d}
+    model, cores = _host_cpu()
     return {"value": n / dt, "unit": "site-field evals/s", "cores": 1, "kind": "reference",
-            "sample": f"reference field_fmm (flang -O1 -g, serial) on the same recipe at 1/10 scale: {n} sites, "
-                      f"THOLE, theta 0.5, ncrit 64, p 6, field order 2; {dt:.1f} s"}
+            "sample": f"reference field_fmm (flang -O1 -g as shipped, serial) on the benchmarked recipe at 1/10 "
+                      f"scale: {n} sites, THOLE, theta 0.5, ncrit 64, p 6, field order 2; {dt:.1f} s",
+            "host_cpu": model, "host_cores": cores, "more": extra}
 
 
-def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, sets=32, spots=1024, margin=0.1):
+def plain_sweep(dev, steps, thetas=(0.25, 0.3, 0.35, 0.4), orders=(7, 8, 9, 10)):
+    """The plain FP64 FMM (one set, every pair at expansion_order) on the undamped 200,001-site water
+    geometry of target_comparison, over theta x expansion order: ms per call and the relative RMS error of
+    the field block against the FP64 direct sum at 4096 seeded sites.  The cheapest setting within 1e-6 is
+    the plain side of the comparison."""
+    import torch
+    from fmm_amd import synth
+    from fmm_amd.octree import Context
+    c = synth.config(1)
+    x = c["coordinates"]
+    n = x.shape[0]
+    rng = np.random.default_rng(2024)
+    m = np.zeros((n, 4), order="F")
+    m[:, 0] = c["multipoles"][:, 0]
+    m[:, 1:4] = rng.normal(0.0, 0.05, (n, 3))
+    X, M, E = col(x, dev), col(m, dev), col(c["exclusions"], dev, torch.int32)
+    tg = np.sort(np.random.default_rng(4096).choice(n, 4096, replace=False)).astype(np.int32)
+    ctx = Context(0)
+    d = col(np.zeros((4096, 4)), dev)
+    ctx.field_direct_targets_device(X, M, E, torch.as_tensor(tg, device=dev), 1, d, "", None)
+    ref = d.cpu().numpy()
+    o = col(np.zeros((n, 4)), dev)
+    rows = []
+    for th in thetas:
+        for p in orders:
+            def f():
+                o.zero_()
+                ctx.field_fmm_device(X, M, E, th, 64, p, 1, o, "", None)
+            f()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                f()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            err = rel(o.cpu().numpy()[tg, 1:4], ref[:, 1:4])
+            rows.append({"theta": th, "expansion_order": p, "ms_per_call": ms, "error": err})
+            print(f"plain sweep theta {th} p {p}: {ms:.2f} ms, error {err:.3e}", file=sys.stderr, flush=True)
+    ctx.close()
+    ok = [r for r in rows if r["error"] <= 1e-6]
+    best = min(ok, key=lambda r: r["ms_per_call"]) if ok else None
+    return {"workload": "water-200k-undamped-dipole", "rows": rows, "cheapest_within_1e-6": best}
+
+
+def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, sets=32, spots=1024,
+                      margins=(0.25, 0.1)):
     """Throughput at <= 1e-6 relative RMS field error against the FP64 direct
     sum: the plain FP64 FMM (one multipole set, every pair at expansion_order)
     against the batched (sets on one tree), mixed-precision (FP32 MFMA M2L),
@@ -151,36 +239,42 @@ def target_comparison(dev, steps, plain=(0.3, 9), batched=(0.3, 9), tol=1e-6, se
     MS = col3(mh, dev)
     os_ = col3(np.zeros((n, 4, sets)), dev)
     ctx.set_precision(True)
-    ctx.set_order_tolerance(tol, spot_checks=spots, margin=margin)
-
-    def f_batched():
-        os_.zero_()
-        ctx.field_fmm_sets_device(X, MS, E, th_b, 64, p_b, 1, os_, "", None)
-    ms_b = timed(f_batched)
-    st = ctx.selection_stats()
-    phases = ctx.phase_times()
-    ob = os_.cpu().numpy()
-    err_b = [rel(ob[tg, 1:4, k], ref[k][:, 1:4]) for k in (0, sets - 1)]
-    ctx.set_order_tolerance(0.0)
-    ctx.set_precision(False)
-    ctx.close()
     v_p = n / (ms_p / 1e3)
-    v_b = n * sets / (ms_b / 1e3)
-    return {
+    out = {
         "workload": "water-200k-undamped-dipole (the default water geometry, no damping)",
         "error_metric": "relative RMS of the field block vs FP64 field_direct on 4096 seeded sites",
         "plain_fp64": {"theta": th_p, "expansion_order": p_p, "sets": 1, "ms_per_call": ms_p, "value": v_p,
                        "error": err_p},
-        "batched_mixed_learned": {"theta": th_b, "expansion_order_cap": p_b, "sets": sets, "ms_per_call": ms_b,
-                                  "value": v_b, "error_set0": err_b[0], "error_last_set": err_b[1],
-                                  "order_tolerance": tol, "spot_checks": spots, "spot_error": st["spot_error"],
-                                  "fell_back": st["fallback"], "mean_order": st["mean_order"],
-                                  "m2l_flops_vs_full": st["m2l_flops"] / st["m2l_flops_full"],
-                                  "phases_ms": {k: round(v, 3) for k, v in phases.items()},
-                                  "selector_training": info},
-        "speedup": v_b / v_p,
-        "both_within_1e-6": bool(err_p <= 1e-6 and max(err_b) <= 1e-6),
     }
+    for i, margin in enumerate(margins):
+        ctx.set_order_tolerance(tol, spot_checks=spots, margin=margin)
+
+        def f_batched():
+            os_.zero_()
+            ctx.field_fmm_sets_device(X, MS, E, th_b, 64, p_b, 1, os_, "", None)
+        ms_b = timed(f_batched)
+        st = ctx.selection_stats()
+        phases = ctx.phase_times()
+        ob = os_.cpu().numpy()
+        err_b = [rel(ob[tg, 1:4, k], ref[k][:, 1:4]) for k in (0, sets - 1)]
+        v_b = n * sets / (ms_b / 1e3)
+        res = {"theta": th_b, "expansion_order_cap": p_b, "sets": sets, "ms_per_call": ms_b, "value": v_b,
+               "error_set0": err_b[0], "error_last_set": err_b[1], "order_tolerance": tol, "order_margin": margin,
+               "spot_checks": spots, "spot_error": st["spot_error"], "fell_back": st["fallback"],
+               "mean_order": st["mean_order"], "m2l_flops_vs_full": st["m2l_flops"] / st["m2l_flops_full"],
+               "phases_ms": {k: round(v, 3) for k, v in phases.items()}, "speedup": v_b / v_p,
+               "within_1e-6": bool(err_p <= 1e-6 and max(err_b) <= 1e-6)}
+        if i == 0:
+            res["selector_training"] = info
+            out["batched_mixed_learned"] = res
+            out["speedup"] = res["speedup"]
+            out["both_within_1e-6"] = res["within_1e-6"]
+        else:
+            out[f"batched_mixed_learned_margin_{margin:g}"] = res
+    ctx.set_order_tolerance(0.0)
+    ctx.set_precision(False)
+    ctx.close()
+    return out
 
 
 def bench_uneven(dev, args):
@@ -295,15 +389,18 @@ def main():
     ap.add_argument("--target-only", action="store_true", help="only run that comparison and print it")
     ap.add_argument("--target-plain", default="0.3,9", help="theta,p of the plain FP64 side")
     ap.add_argument("--target-batched", default="0.3,9", help="theta,cap of the batched side")
-    ap.add_argument("--target-margin", type=float, default=0.1,
-                    help="order margin of the learned orders on the batched side (library default 0.25; 0.1 "
-                         "measured with spot error 8.6e-7 on this workload)")
+    ap.add_argument("--plain-sweep", action="store_true",
+                    help="only sweep the plain FP64 side over theta x expansion order (time and error) and exit")
     args = ap.parse_args()
     tp = tuple(float(v) if i == 0 else int(v) for i, v in enumerate(args.target_plain.split(",")))
     tb_ = tuple(float(v) if i == 0 else int(v) for i, v in enumerate(args.target_batched.split(",")))
     if args.target_only:
         import torch
-        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_, margin=args.target_margin)))
+        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_)))
+        return
+    if args.plain_sweep:
+        import torch
+        print(json.dumps(plain_sweep(torch.device("cuda:0"), args.steps)))
         return
     if args.peak_calibration:
         import torch
@@ -415,18 +512,45 @@ def main():
     ctx.set_timing(True)
     ctx.set_tree_reuse(False)
     value = n * sets / (ms_per_step / 1e3)
+    # The drop-in call as a Fortran caller makes it (fmmh_field_fmm: host arrays in, host field out, the
+    # exclusion entries checked on the host, pageable copies both ways), on the same inputs
+    host_call = None
+    if sets == 1:
+        hx, hm, he = c["coordinates"], c["multipoles"], c["exclusions"]
+        hdf = c["damping_factors"] if c["damp_type"] else ()
+        hf = np.zeros((n, nf), order="F")
+
+        def host_step():
+            hf[:] = 0.0
+            ctx.field_fmm(0, hx, hm, he, c["theta"], c["ncrit"], c["expansion_order"], F, c["damp_type"], hdf,
+                          field=hf)
+        host_step()
+        h0 = time.perf_counter()
+        for _ in range(args.steps):
+            host_step()
+        host_ms = (time.perf_counter() - h0) * 1e3 / args.steps
+        hp = ctx.phase_times()
+        host_call = {"ms_per_call": host_ms, "value": n / (host_ms / 1e3),
+                     "h2d_ms": hp.get("h2d"), "d2h_ms": hp.get("d2h"),
+                     "bytes_in": int(hx.nbytes + hm.nbytes + he.nbytes + (hdf.nbytes if c["damp_type"] else 0)
+                                     + hf.nbytes), "bytes_out": int(hf.nbytes),
+                     "note": "fmmh_field_fmm on host (numpy, column-major) arrays: host check of the exclusion "
+                             "entries, pageable copies in (inputs and the field it accumulates into) and out"}
 
     pairs = stats["p2p_site_pairs"]
     fpp = p2p_flops_per_pair(m, F, c["damp_type"], sets)
     p2p_ms = phase_avg["p2p"]
     achieved = pairs * fpp / (p2p_ms / 1e3) / 1e12
+    src = near_src_sha()
+    traffic_ok = args.config == 1 and precision == "fp64" and K_NEAR_TRAFFIC_SRC == src
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": K_NEAR_TRAFFIC_BYTES if args.config == 1 and precision == "fp64" else None,
+                "traffic": K_NEAR_TRAFFIC_BYTES if traffic_ok else None,
                 "kernel": "k_near (P2P, FP64 VALU)", "kernel_ms": p2p_ms, "flop_per_pair": fpp,
-                "site_pairs": pairs,
+                "site_pairs": pairs, "near_src_sha": src,
                 "note": "bound: FP64 compute (the vector-ALU peak equals the FP64 matrix peak on MI355X); "
-                        "traffic = measured HBM bytes per launch (compulsory ~30 MB)"}
+                        "traffic = HBM bytes per launch recorded by the profile of these near-field sources "
+                        "(null when the sources differ from the profiled ones; compulsory ~30 MB)"}
 
     errors = None
     if not args.no_error and sets > 1:
@@ -488,11 +612,17 @@ def main():
                     "of far pairs)",
         }
     cpu = None if args.no_cpu else cpu_baseline()
-    target = None if (args.no_target or args.config != 1) else target_comparison(dev, min(args.steps, 3), tp, tb_,
-                                                                                   margin=args.target_margin)
+    target = None if (args.no_target or args.config != 1) else target_comparison(dev, min(args.steps, 3), tp, tb_)
     line = {
         "metric": "site-field evals/s at <=1e-6 rel err vs FP64 direct; % of FP64/FP32 peak",
         "value": value,
+        "value_note": ("site-field evaluations/s of the named workload at its own settings, whose error is "
+                       "stated in `errors` (it is not a 1e-6 setting); the throughput at <= 1e-6 is "
+                       "`value_at_1e-6`"),
+        "value_at_1e-6": (None if target is None else
+                          {"batched_mixed_learned": target["batched_mixed_learned"]["value"],
+                           "plain_fp64": target["plain_fp64"]["value"], "speedup": target["speedup"],
+                           "workload": target["workload"]}),
         "unit": "site-field evals/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -514,6 +644,7 @@ def main():
                                "the previous call with bitwise-identical coordinates",
                        "graph_ms_per_step": graph_ms, "graph_value": n * sets / (graph_ms / 1e3),
                        "graph_replayed": graph_replayed},
+        "drop_in_host_call": host_call,
         "tree": stats,
         "errors": errors,
         "target_1e-6": target,

@@ -82,6 +82,7 @@ struct fmmh_context {
     int64_t stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
+    double list_need[3] = {0, 0, 0};  // frontier capacity that sufficed, M2L and P2P entries (x 1.25), per node
     int m2l_mfma = 1;                 // FMMH_OPT_M2L_MFMA: 0 VALU, 1 auto, 2 per-target MFMA, 3 class-grouped
     int precision = 0;                // FMMH_OPT_PRECISION: 0 FP64, 1 mixed (FP32 MFMA translations)
     // Tree / list cache (FMMH_OPT_REUSE_TREE): the last field_fmm tree, lists
@@ -198,6 +199,12 @@ struct fmmh_context {
         }
     }
     void sync() { FMMH_HIP(hipStreamSynchronize(stream)); }
+    // An event the host waits on for a readback while later work stays queued behind it.
+    hipEvent_t rb_event = nullptr;
+    hipEvent_t readback_event() {
+        if (!rb_event) FMMH_HIP(hipEventCreateWithFlags(&rb_event, hipEventDisableTiming));
+        return rb_event;
+    }
     void graph_exec_replace(hipGraph_t g) {
         if (graph_exec) {
             FMMH_HIP(hipGraphExecDestroy(graph_exec));
@@ -1130,83 +1137,88 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     launch_leaf_index(nn, leaf_scan, t, s);
     launch_level_scatter(nn, level_off_d, t, s);
     launch_leaf_sort(out.num_leaves, ncrit, t, s);
-    // the pinned area is reused by the next readback; the level offsets copy must land first
-    c->sync();
+    // (no wait here: the level offsets' pinned slots hp[64..] are not reused before the call's last wait)
 }
 
-void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int nimg, const double* disp) {
+// overlap (may be empty): device work that does not depend on the lists, queued once behind the first
+// traversal's readback, so that the GPU runs it while the host waits for the list sizes.
+template <class Overlap>
+void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int nimg, const double* disp,
+                 Overlap&& overlap) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes;
     const double sc = c->list_cap_scale;
     c->list_retries = 0;
-    int cap_f = std::max(64, (int)std::min(1e9, 160.0 * nn * sc));
-    int cap_m = std::max(64, (int)std::min(1e9, 160.0 * nn * sc));
-    int cap_p = std::max(64, (int)std::min(1e9, 64.0 * nn * sc));
+    // initial capacities: entries per node that fit the usual trees, or, when larger, what the context's
+    // previous tree needed per node (repeated calls at small theta then do not overflow and retry)
+    int cap_f = std::max(64, (int)std::min(1e9, std::max(160.0, c->list_need[0]) * nn * sc));
+    int cap_m = std::max(64, (int)std::min(1e9, std::max(160.0, c->list_need[1]) * nn * sc));
+    int cap_p = std::max(64, (int)std::min(1e9, std::max(64.0, c->list_need[2]) * nn * sc));
     int* hp = c->h_pinned;
     for (int attempt = 0; attempt < 64; ++attempt) {
         int2* fa = c->buf<int2>("l_front_a", cap_f);
         int2* fb = c->buf<int2>("l_front_b", cap_f);
         unsigned long long* m2l = c->buf<unsigned long long>("l_m2l", cap_m);
         unsigned long long* p2p = c->buf<unsigned long long>("l_p2p", cap_p);
-        // n_a, n_b, n_m2l, n_p2p, overflow, each on its own 128-byte line (the traversal's atomics on
-        // them do not contend for one line)
+        // three frontier counters (rotating: step k reads f[k % 3], appends to f[(k + 1) % 3] and zeroes
+        // f[(k + 2) % 3], which neither it nor the previous step's readers touch), M2L, P2P and overflow,
+        // each on its own 128-byte line (the traversal's atomics on them do not contend for one line)
         constexpr int CL = 32;
-        int* counters = c->buf<int>("l_counters", 5 * CL);
-        int init[5] = {nimg, 0, 0, 0, 0};
-        std::memcpy(hp + 100, init, sizeof(init));
-        for (int q = 0; q < 5; ++q)
-            FMMH_HIP(hipMemcpyAsync(counters + q * CL, hp + 100 + q, sizeof(int), hipMemcpyHostToDevice, s));
-        for (int k = 0; k < nimg; ++k) {   // one (root, root) seed per image
-            int2 root = make_int2(0, src_encode(k, 0));
-            std::memcpy(hp + 110 + 2 * k, &root, sizeof(root));
-        }
-        FMMH_HIP(hipMemcpyAsync(fa, hp + 110, nimg * sizeof(int2), hipMemcpyHostToDevice, s));
+        int* counters = c->buf<int>("l_counters", 6 * CL);
+        int* counts_out = c->buf<int>("l_counts", 8);
+        launch_traverse_init(counters, CL, nimg, fa, s);   // one (root, root) seed per image
         // Every level of the traversal deepens one side of each surviving
         // pair, so level_i + level_j grows by one per step: 2 max_depth steps
         // always drain the frontier.  The frontier size stays on the device
         // (the kernel reads it), so no host round trip per step.
         const int steps = 2 * tr.max_depth;
-        int cur = 0;
         for (int it = 0; it < steps; ++it) {
-            int2* fin = cur == 0 ? fa : fb;
-            int2* fout = cur == 0 ? fb : fa;
-            int* cin = counters + cur * CL;
-            int* cout = counters + (1 - cur) * CL;
-            FMMH_HIP(hipMemsetAsync(cout, 0, sizeof(int), s));
-            TraverseArgs a{fin, cin, cap_f, fout, cout, cap_f, m2l, counters + 2 * CL, cap_m, p2p, counters + 3 * CL,
-                           cap_p, counters + 4 * CL, tr.t.cx, tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf,
-                           tr.t.children, theta, disp};
+            int2* fin = it % 2 == 0 ? fa : fb;
+            int2* fout = it % 2 == 0 ? fb : fa;
+            TraverseArgs a{fin, counters + (it % 3) * CL, cap_f, fout, counters + ((it + 1) % 3) * CL, cap_f, m2l,
+                           counters + 3 * CL, cap_m, p2p, counters + 4 * CL, cap_p, counters + 5 * CL, tr.t.cx,
+                           tr.t.cy, tr.t.cz, tr.t.rmax, tr.t.is_leaf, tr.t.children, theta, disp,
+                           counters + ((it + 2) % 3) * CL};
             launch_traverse(a, TRAVERSE_GRID, s);
-            cur = 1 - cur;
         }
-        for (int q = 0; q < 5; ++q)
-            FMMH_HIP(hipMemcpyAsync(hp + q, counters + q * CL, sizeof(int), hipMemcpyDeviceToHost, s));
-        c->sync();
-        if (hp[4]) {
+        launch_traverse_counts(counters, CL, counts_out, s);
+        FMMH_HIP(hipMemcpyAsync(hp, counts_out, 6 * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (attempt == 0) {
+            c->mark("l_traverse");
+            FMMH_HIP(hipEventRecord(c->readback_event(), s));
+            overlap();
+            FMMH_HIP(hipEventSynchronize(c->readback_event()));
+        } else {
+            c->sync();
+        }
+        if (hp[5]) {
             ++c->list_retries;
-            if (hp[4] & 1) cap_m = (int)std::min(1e9, std::max(2.0 * cap_m, 1.5 * hp[2]));
-            if (hp[4] & 2) cap_p = (int)std::min(1e9, std::max(2.0 * cap_p, 1.5 * hp[3]));
-            if (hp[4] & 4) cap_f *= 2;
+            if (hp[5] & 1) cap_m = (int)std::min(1e9, std::max(2.0 * cap_m, 1.5 * hp[3]));
+            if (hp[5] & 2) cap_p = (int)std::min(1e9, std::max(2.0 * cap_p, 1.5 * hp[4]));
+            if (hp[5] & 4) cap_f *= 2;
             continue;
         }
-        if (hp[cur] != 0) throw Error(FMMH_ERR_INTERNAL, "dual traversal did not drain");
-        c->mark("l_traverse");
-        L.n_m2l = hp[2];
-        L.n_p2p = hp[3];
-        int bits = 32;
-        while ((1ll << (bits - 32)) < nn) ++bits;
-        size_t tb = std::max(sort_keys_u64_temp(std::max(L.n_m2l, 1)), sort_keys_u64_temp(std::max(L.n_p2p, 1)));
+        if (hp[steps % 3] != 0) throw Error(FMMH_ERR_INTERNAL, "dual traversal did not drain");
+        if (sc == 1.0) {   // entries per node that sufficed (lists with a margin; not under a forced capacity)
+            c->list_need[0] = (double)cap_f / std::max(nn, 1);
+            c->list_need[1] = 1.25 * hp[3] / std::max(nn, 1);
+            c->list_need[2] = 1.25 * hp[4] / std::max(nn, 1);
+        }
+        L.n_m2l = hp[3];
+        L.n_p2p = hp[4];
+        // CSR by target, sources ascending (the reference's append order, src/fmm.F90:352-385)
+        const size_t tb = exscan_i32_temp(nn + 1);
         void* temp = c->buf<char>("l_temp", tb);
         L.m2l_keys = c->buf<unsigned long long>("l_m2l_sorted", L.n_m2l);
-        unsigned long long* sorted = c->buf<unsigned long long>("l_p2p_sorted", L.n_p2p);
         L.m2l_off = c->buf<int>("l_m2l_off", nn + 1);
         L.m2l_src = c->buf<int>("l_m2l_src", L.n_m2l);
         L.p2p_off = c->buf<int>("l_p2p_off", nn + 1);
         L.p2p_src = c->buf<int>("l_p2p_src", L.n_p2p);
-        if (L.n_m2l > 0) sort_keys_u64(temp, tb, m2l, L.m2l_keys, L.n_m2l, bits, s);
-        launch_csr(L.m2l_keys, L.n_m2l, nn, L.m2l_off, L.m2l_src, s);
-        if (L.n_p2p > 0) sort_keys_u64(temp, tb, p2p, sorted, L.n_p2p, bits, s);
-        launch_csr(sorted, L.n_p2p, nn, L.p2p_off, L.p2p_src, s);
+        int* cnt = c->buf<int>("l_target_cnt", nn + 1);
+        unsigned* tmp = c->buf<unsigned>("l_list_tmp", std::max(L.n_m2l, L.n_p2p));
+        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(nn + 1, 256)), dim3(256), 0, s, cnt, 0, nn + 1);
+        launch_list_csr(m2l, L.n_m2l, nn, cnt, L.m2l_off, tmp, temp, tb, L.m2l_src, L.m2l_keys, s);
+        launch_list_csr(p2p, L.n_p2p, nn, cnt, L.p2p_off, tmp, temp, tb, L.p2p_src, nullptr, s);
         return;
     }
     throw Error(FMMH_ERR_INTERNAL, "interaction list capacity did not converge");
@@ -1528,12 +1540,16 @@ const unsigned char* select_orders(fmmh_context* c, TreeOut& tr, ListsOut& L, Ta
 // far field at spot sites against the direct sum with the reference's far
 // semantics, per multipole set; out[0] = the largest over the sets (sets
 // without field count 0).  FMM far field at a site = field - field before
-// L2P - near.  One workgroup, the sets in turn.
+// L2P - near; the direct far field = the sum over all sources (the source
+// chunks' parts dir[chunk][set][spot][4], in chunk order) minus the sum over
+// the P2P list's sources (dnear[set][spot][4]).  One workgroup, the sets in
+// turn.
 __global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__ dir, const int* __restrict__ site,
                                                       const int* __restrict__ perm, const double* __restrict__ near,
                                                       int64_t ldn, const double* __restrict__ field, int64_t ldf,
                                                       const double* __restrict__ before, int64_t ldb, int nspots,
-                                                      int nf, int nsets, double* __restrict__ out) {
+                                                      int nf, int nsets, int nchunks, const double* __restrict__ dnear,
+                                                      double* __restrict__ out) {
     __shared__ double red[2][256];
     const int c0 = nf >= 4 ? 1 : 0, c1 = nf >= 4 ? 4 : 1;
     double worst = 0.0;
@@ -1541,14 +1557,15 @@ __global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__
         const double* __restrict__ nr = near + (int64_t)set * nf * ldn;
         const double* __restrict__ fl = field + (int64_t)set * nf * ldf;
         const double* __restrict__ bf = before + (int64_t)set * nf * ldb;
-        const double* __restrict__ dr = dir + (int64_t)set * nspots * 4;
         double e2 = 0.0, f2 = 0.0;
         for (int sp = threadIdx.x; sp < nspots; sp += 256) {
             const int k = site[sp], o = perm[k];
             for (int c = c0; c < c1; ++c) {
                 const double nv = nr[(int64_t)c * ldn + k];
                 const double ffar = fl[(int64_t)c * ldf + o] - bf[(int64_t)c * ldb + o] - nv;
-                const double d = dr[(int64_t)sp * 4 + c];
+                double d = 0.0;
+                for (int ch = 0; ch < nchunks; ++ch) d += dir[(((int64_t)ch * nsets + set) * nspots + sp) * 4 + c];
+                d -= dnear[((int64_t)set * nspots + sp) * 4 + c];
                 e2 += (ffar - d) * (ffar - d);
                 f2 += (nv + d) * (nv + d);
             }
@@ -1577,14 +1594,16 @@ double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, c
                   int nspots) {
     hipStream_t s = c->stream;
     nspots = std::min(nspots, tr.num_leaves);
-    double* out = c->buf<double>("spot_out", (size_t)nspots * 4 * k.nsets);
+    const int nchunks = spot_chunks(n);
+    double* out = c->buf<double>("spot_out", (size_t)nchunks * nspots * 4 * k.nsets);
     int* site = c->buf<int>("spot_site", nspots);
     double* res = c->buf<double>("spot_res", 2);
+    double* out_near = c->buf<double>("spot_out_near", (size_t)nspots * 4 * k.nsets);
     SpotArgs sa{nspots, tr.num_leaves, n, tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, xs, ys, zs,
-                ms, n, out, site, k.nsets};
+                ms, n, out, out_near, site, k.nsets};
     launch_spot_far(sa, m, s);
     hipLaunchKernelGGL(k_spot_compare, dim3(1), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n, k.field,
-                       k.ldf, before, (int64_t)n, nspots, nf, k.nsets, res);
+                       k.ldf, before, (int64_t)n, nspots, nf, k.nsets, nchunks, out_near, res);
     FMMH_HIP(hipGetLastError());
     double* h = reinterpret_cast<double*>(c->h_pinned + 800);
     FMMH_HIP(hipMemcpyAsync(h, res, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1675,6 +1694,60 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     const int nm = ncomp(p);
     TreeOut tr;
     ListsOut L;
+    const int S = k.nsets;
+    Tables& tb = get_tables(c, p, m);
+    // Sorted site arrays, expansions and the upward pass (gather, P2M, M2M): they need the tree but not the
+    // lists, so a call that builds its tree runs them while the host waits for the list sizes.
+    double *xs = nullptr, *ys = nullptr, *zs = nullptr, *ms_raw = nullptr, *ms = nullptr, *dfs = nullptr;
+    double *CM = nullptr, *LE = nullptr;
+    int* excl = nullptr;
+    int* inodes = nullptr;
+    int* icnt = nullptr;
+    bool shift32 = false;
+    UpArgs up{};
+    bool upward_done = false;
+    auto upward_pass = [&]() {
+        const int nn = tr.num_nodes, nl = tr.num_leaves;
+        xs = c->buf<double>("s_xs", n);
+        ys = c->buf<double>("s_ys", n);
+        zs = c->buf<double>("s_zs", n);
+        ms_raw = c->buf<double>("s_ms_raw", (size_t)n * k.dim * S);
+        ms = c->buf<double>("s_ms", (size_t)n * k.dim * S);
+        dfs = c->buf<double>("s_dfs", n);
+        excl = c->buf<int>("s_excl", (size_t)n * std::max(k.nexcl, 1));
+        SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, tr.t.perm, n,
+                   xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale, S};
+        launch_gather(g, s);
+        c->mark("gather");
+        CM = c->buf<double>("e_cm", (size_t)nn * nm * S);
+        LE = c->buf<double>("e_le", (size_t)nn * nm * S);
+        up = UpArgs{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
+                    tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, S, k.dim,
+                    tb.xyz};
+        launch_p2m(up, nl, s);
+        // mixed precision: M2M and L2L on FP32 MFMA (box-scaled matrices per octant), FP64 accumulation,
+        // over per-level lists of internal nodes
+        shift32 = c->precision == 1 && p >= 1 && tb.m2m_frag;
+        if (shift32) {
+            inodes = c->buf<int>("s_internal", nn);
+            icnt = c->buf<int>("s_internal_cnt", KEY_LEVELS + 2);
+            LevelStarts ls{};
+            for (int L = 0; L < KEY_LEVELS + 3; ++L) ls.off[L] = tr.level_off[L];
+            launch_internal_nodes(nn, tr.t.level, tr.t.children, ls, inodes, icnt, s);
+        }
+        for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
+            int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
+            if (shift32 && (long long)(o1 - o0) * S >= SHIFT_MFMA_MIN_COLS) {
+                ShiftMfmaArgs sa{inodes + o0, icnt + lev, o1 - o0, tr.t.children, tr.t.r, CM, CM, (int64_t)nn * nm,
+                                 S, tb.m2m_frag, tb.xyz};
+                launch_shift_mfma(sa, p, false, s);
+            } else {
+                launch_m2m(up, tr.t.level_nodes + o0, o1 - o0, s);
+            }
+        }
+        c->mark("upward");
+        upward_done = true;
+    };
     const bool reuse = c->reuse_tree && cached_tree_matches(c, k, theta, ncrit, P);
     if (reuse) {
         tr = c->cache_tr;
@@ -1688,14 +1761,13 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         ++c->cache_gen;
         build_tree(c, k.X, k.ldx, n, ncrit, tr);
         c->mark("tree");
-        build_lists(c, tr, theta, L, P.nimg, disp);
+        build_lists(c, tr, theta, L, P.nimg, disp, upward_pass);
         c->mark("lists");
     }
     const int nn = tr.num_nodes, nl = tr.num_leaves;
     c->num_nodes = nn; c->num_leaves = nl; c->max_depth = tr.max_depth;
     c->n_m2l = L.n_m2l; c->n_p2p = L.n_p2p; c->n_sites = n;
     c->level_off = tr.level_off;
-    const int S = k.nsets;
     // learned per-pair orders (off: every pair at p, as the reference)
     const bool select = c->order_tol > 0.0 && c->sel_trained && !P.on && L.n_m2l > 0 && m <= SPOT_MMAX;
     // graph replay: a tree-reuse call without timing events, selection or periodic images
@@ -1722,7 +1794,6 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         c->stats_classes = c->cache_classes;   // a replay runs with the cached classes
     } else {
     GraphCapture cap(s, capture, &c->capturing);
-    Tables& tb = get_tables(c, p, m);
     // tree-dependent preparation, kept with a cached tree: the near-field workgroup order and the M2L
     // displacement classes, sized through one host round trip
     const bool work_cached = reuse && c->cache_work_valid;
@@ -1738,45 +1809,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     }
     if (!classes_cached) m2l_classes_end(c, p, cls_pending, ca);
     c->mark("prep");
-    double* xs = c->buf<double>("s_xs", n);
-    double* ys = c->buf<double>("s_ys", n);
-    double* zs = c->buf<double>("s_zs", n);
-    double* ms_raw = c->buf<double>("s_ms_raw", (size_t)n * k.dim * S);
-    double* ms = c->buf<double>("s_ms", (size_t)n * k.dim * S);
-    double* dfs = c->buf<double>("s_dfs", n);
-    int* excl = c->buf<int>("s_excl", (size_t)n * std::max(k.nexcl, 1));
-    SiteArgs g{k.X, k.ldx, k.M, k.ldm, k.dim, k.E, k.lde, k.nexcl, k.damp ? k.df : nullptr, tr.t.perm, n,
-               xs, ys, zs, ms_raw, ms, dfs, excl, tb.scale, S};
-    launch_gather(g, s);
-    c->mark("gather");
-    double* CM = c->buf<double>("e_cm", (size_t)nn * nm * S);
-    double* LE = c->buf<double>("e_le", (size_t)nn * nm * S);
-    UpArgs up{xs, ys, zs, ms_raw, n, tr.t.leaf_node, tr.t.start, tr.t.count, tr.t.cx, tr.t.cy, tr.t.cz,
-              tr.t.children, CM, nm, p, tb.nsrc, tb.p2m, tb.m2m, tb.mono_parent, tb.mono_axis, nn, S, k.dim, tb.xyz};
-    launch_p2m(up, nl, s);
-    // mixed precision: M2M and L2L on FP32 MFMA (box-scaled matrices per octant), FP64 accumulation,
-    // over per-level lists of internal nodes
-    const bool shift32 = c->precision == 1 && p >= 1 && tb.m2m_frag;
-    int* inodes = nullptr;
-    int* icnt = nullptr;
-    if (shift32) {
-        inodes = c->buf<int>("s_internal", nn);
-        icnt = c->buf<int>("s_internal_cnt", KEY_LEVELS + 2);
-        LevelStarts ls{};
-        for (int L = 0; L < KEY_LEVELS + 3; ++L) ls.off[L] = tr.level_off[L];
-        launch_internal_nodes(nn, tr.t.level, tr.t.children, ls, inodes, icnt, s);
-    }
-    for (int lev = tr.max_depth - 1; lev >= 1; --lev) {
-        int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
-        if (shift32 && (long long)(o1 - o0) * S >= SHIFT_MFMA_MIN_COLS) {
-            ShiftMfmaArgs sa{inodes + o0, icnt + lev, o1 - o0, tr.t.children, tr.t.r, CM, CM, (int64_t)nn * nm, S,
-                             tb.m2m_frag, tb.xyz};
-            launch_shift_mfma(sa, p, false, s);
-        } else {
-            launch_m2m(up, tr.t.level_nodes + o0, o1 - o0, s);
-        }
-    }
-    c->mark("upward");
+    if (!upward_done) upward_pass();
     double* near = c->buf<double>("e_near", (size_t)n * nf * S);
     int* inv = c->buf<int>("s_inv", n);
     // the sorted-site inverse permutation depends on the tree only
@@ -2060,6 +2093,7 @@ int fmmh_context_destroy(fmmh_context* ctx) {
             if (kv.second.fbuf) FMMH_HIP(hipFree(kv.second.fbuf));
         }
         for (hipEvent_t e : ctx->event_pool) FMMH_HIP(hipEventDestroy(e));
+        if (ctx->rb_event) FMMH_HIP(hipEventDestroy(ctx->rb_event));
         if (ctx->graph_exec) FMMH_HIP(hipGraphExecDestroy(ctx->graph_exec));
         FMMH_HIP(hipHostFree(ctx->h_pinned));
         FMMH_HIP(hipStreamDestroy(ctx->stream));
@@ -2466,7 +2500,7 @@ int fmmh_tree_build(fmmh_context* ctx, const double* coordinates, int64_t ld_coo
         TreeOut tr;
         build_tree(c, X, n, n, ncrit, tr);
         ListsOut L;
-        build_lists(c, tr, theta, L, 1, upload_disp(c, Periodic{}));
+        build_lists(c, tr, theta, L, 1, upload_disp(c, Periodic{}), [] {});
         c->sync();
         c->num_nodes = tr.num_nodes; c->num_leaves = tr.num_leaves; c->max_depth = tr.max_depth;
         c->n_m2l = L.n_m2l; c->n_p2p = L.n_p2p; c->n_sites = n;
@@ -2679,7 +2713,7 @@ int fmmh_selector_train(fmmh_context* ctx, const double* coordinates, int64_t ld
         TreeOut tr;
         build_tree(c, X, n, n, ncrit, tr);
         ListsOut L;
-        build_lists(c, tr, theta, L, 1, upload_disp(c, Periodic{}));
+        build_lists(c, tr, theta, L, 1, upload_disp(c, Periodic{}), [] {});
         if (L.n_m2l == 0) throw Error(FMMH_ERR_ARG, "the tree has no M2L pairs to learn from");
         Tables& tb = get_tables(c, cap, m);
         const int nn = tr.num_nodes, nm = ncomp(cap);

@@ -142,9 +142,21 @@ struct TraverseArgs {
     const int* is_leaf; const int* children;
     double theta;
     const double* disp;        // per image id: displacement of the source cell; frontier sources carry the id
+    int* n_clear;              // a frontier counter this step neither reads nor writes, zeroed for the next
 };
 void launch_traverse(const TraverseArgs& a, int grid, hipStream_t s);
+// Traversal set-up: counters (6 of them, stride `stride` ints: 3 frontier counters, M2L, P2P, overflow) =
+// {nimg, 0, 0, 0, 0, 0} and the frontier seeded with one (root, root) pair per periodic image.
+void launch_traverse_init(int* counters, int stride, int nimg, int2* frontier, hipStream_t s);
+// Copies the 6 counters into out[0..5] (contiguous, for one readback).
+void launch_traverse_counts(const int* counters, int stride, int* out, hipStream_t s);
 void launch_csr(const unsigned long long* keys, int n_keys, int num_targets, int* off, int* src, hipStream_t s);
+// CSR (offsets, sources ascending per target, and the keys in that order when keys_out is not null) of an
+// unsorted list of (target << 32 | source) keys.  cnt: num_targets + 1 ints that are zero on entry and on
+// return; tmp: n_keys; scan_temp: exscan_i32_temp(num_targets + 1) bytes.
+void launch_list_csr(const unsigned long long* keys, int n_keys, int num_targets, int* cnt, int* off, unsigned* tmp,
+                     void* scan_temp, size_t scan_temp_bytes, int* src, unsigned long long* keys_out,
+                     hipStream_t s);
 
 // ---------------------------------------------------------------- expansions
 // Shift tables: entries grouped by target component t (CSR by t); each entry
@@ -371,13 +383,15 @@ struct SpotArgs {
     const int* p2p_off; const int* p2p_src;
     const double* xs; const double* ys; const double* zs;
     const double* ms; int64_t ldm;            // Taylor-scaled multipoles [set][component][ldm]
-    double* out;                              // [set][nspots][4] far-field direct sum
+    double* out;                              // [chunk][set][nspots][4] direct sum over all sources, per chunk
+    double* out_near;                         // [set][nspots][4] direct sum over the P2P list's sources
     int* site;                                // [nspots] sorted site index
     int nsets;
 };
 // Source multipole orders the spot checks (and so the learned per-pair orders) cover; calls with higher
 // source orders keep expansion_order for every pair.
 constexpr int SPOT_MMAX = 2;
+int spot_chunks(int n);    // source chunks of launch_spot_far (its output's leading dimension)
 void launch_spot_far(const SpotArgs& a, int m, hipStream_t s);
 void launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 

@@ -622,17 +622,108 @@ void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, int nsets
 // ------------------------------------------------------------------ spot checks
 // Far-field part of the direct sum at spot sites, with the reference's far
 // semantics: every source whose leaf is not in the P2P list of the site's
-// leaf, undamped and without exclusions.  One workgroup per spot site
-// (the first site of an evenly spaced leaf) and block of SPOT_SB sets (each
-// source's tensor is shared by the block); out[set][spot][4] = potential +
-// field of those sources.  The near list of the leaf is sorted by node id,
-// which is also sorted-site order, so membership is a binary search over
-// its site ranges.
-constexpr int SPOT_STAGE = 1024;
+// leaf, undamped and without exclusions.  The spot sites are the first sites
+// of evenly spaced leaves.  It is computed as the sum over all sources minus
+// the sum over the sources of the P2P list (a pair at R = 0 gives 0 in both),
+// so the large sweep needs no membership test:
+//   k_spot_all: a workgroup takes SPOT_NSP spot sites, SPOT_SB multipole sets
+//     and one chunk of SPOT_CHUNK sources, staged in LDS SPOT_TILE at a time
+//     (coordinates and the sets' multipoles, shared by the spot sites);
+//     SPOT_PH threads per spot site take every SPOT_PH-th source of a tile and
+//     each source's tensor is shared by the sets.  out_all[chunk][set][spot][4].
+//   k_spot_near: a workgroup per (spot site, SPOT_SB sets) sums the sources of
+//     the P2P list, 32 threads per list entry.  out_near[set][spot][4].
+// Partial sums are added in thread order (deterministic).
 constexpr int SPOT_SB = 8;
+constexpr int SPOT_NSP = 16;
+constexpr int SPOT_PH = 16;            // SPOT_NSP * SPOT_PH = 256 threads
+constexpr int SPOT_CHUNK = 16384;
+
+int spot_chunks(int n) { return (n + SPOT_CHUNK - 1) / SPOT_CHUNK; }
+
+// undamped tensor of orders <= M + 1 at displacement (dx, dy, dz); 0 at R = 0
+template <int M>
+__device__ __forceinline__ void spot_tensor(double dx, double dy, double dz, double (&T)[ncomp(M + 1)]) {
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    double g[M + 2];
+    radial_inv<M + 1, DAMP_NONE>(R2, R2 > 0.0 ? rsqrt_pos(R2) : 0.0, 0.0, g);
+    md_tensor<M + 1>(dx, dy, dz, g, T);
+}
+
+__device__ __forceinline__ int spot_site(const SpotArgs& a, int spot) {
+    const int leaf = (int)(((long long)spot * a.num_leaves) / a.nspots);
+    return a.start[a.leaf_node[leaf]];
+}
 
 template <int M>
-__global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
+__global__ __launch_bounds__(256) void k_spot_all(SpotArgs a) {
+    constexpr int NM = ncomp(M);
+    constexpr int TILE = M <= 1 ? 128 : 64;
+    __shared__ double sxs[TILE], sys[TILE], szs[TILE];
+    __shared__ double smu[SPOT_SB][NM][TILE];
+    __shared__ double red[4][256];
+    const int t = threadIdx.x;
+    const int sp = t % SPOT_NSP, ph = t / SPOT_NSP;
+    const int spot = blockIdx.x * SPOT_NSP + sp;
+    const bool valid = spot < a.nspots;
+    const int set0 = blockIdx.y * SPOT_SB;
+    const int nsb = min(SPOT_SB, a.nsets - set0);
+    const int j0 = blockIdx.z * SPOT_CHUNK, j1 = min(a.n, j0 + SPOT_CHUNK);
+    const int site = spot_site(a, valid ? spot : 0);
+    const double x = a.xs[site], y = a.ys[site], z = a.zs[site];
+    double acc[SPOT_SB][4];
+#pragma unroll
+    for (int u = 0; u < SPOT_SB; ++u)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[u][c] = 0.0;
+    for (int tb = j0; tb < j1; tb += TILE) {
+        const int cn = min(TILE, j1 - tb);
+        __syncthreads();                       // readers of the previous tile are done
+        for (int e = t; e < TILE; e += 256)
+            if (e < cn) {
+                sxs[e] = a.xs[tb + e];
+                sys[e] = a.ys[tb + e];
+                szs[e] = a.zs[tb + e];
+            }
+        for (int e = t; e < SPOT_SB * NM * TILE; e += 256) {
+            const int jj = e % TILE, us = e / TILE, u = us / NM, sc = us - u * NM;
+            smu[u][sc][jj] = (u < nsb && jj < cn) ? a.ms[((int64_t)(set0 + u) * NM + sc) * a.ldm + tb + jj] : 0.0;
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int jj = ph; jj < cn; jj += SPOT_PH) {
+            double T[ncomp(M + 1)];
+            spot_tensor<M>(x - sxs[jj], y - sys[jj], z - szs[jj], T);
+#pragma unroll
+            for (int u = 0; u < SPOT_SB; ++u) {
+                double m[NM];
+#pragma unroll
+                for (int sc = 0; sc < NM; ++sc) m[sc] = smu[u][sc][jj];
+                contract<M, 1>(m, T, acc[u]);
+            }
+        }
+    }
+    // sum the SPOT_PH partials of each spot site in thread order, set by set
+    for (int u = 0; u < nsb; ++u) {
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[c][t] = acc[u][c];
+        __syncthreads();
+        if (t < SPOT_NSP * 4) {
+            const int s2 = t % SPOT_NSP, c = t / SPOT_NSP;
+            const int spot2 = blockIdx.x * SPOT_NSP + s2;
+            if (spot2 < a.nspots) {
+                double v = 0.0;
+                for (int q = 0; q < SPOT_PH; ++q) v += red[c][q * SPOT_NSP + s2];
+                a.out[(((int64_t)blockIdx.z * a.nsets + set0 + u) * a.nspots + spot2) * 4 + c] = v;
+            }
+        }
+    }
+    if (valid && ph == 0 && blockIdx.y == 0 && blockIdx.z == 0) a.site[spot] = site;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_spot_near(SpotArgs a) {
     constexpr int NM = ncomp(M);
     __shared__ double red[4][256];
     const int spot = blockIdx.x;
@@ -642,64 +733,32 @@ __global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
     const int node = a.leaf_node[leaf];
     const int site = a.start[node];
     const double x = a.xs[site], y = a.ys[site], z = a.zs[site];
-    const int e0 = a.p2p_off[node], e1 = a.p2p_off[node + 1];
-    // the near source leaves' site ranges [first, end), ascending (the list is in pre-order, which is
-    // the sorted-site order), staged in LDS when they fit
-    __shared__ int rfirst[SPOT_STAGE], rend[SPOT_STAGE];
-    const int ne = e1 - e0;
-    const bool staged = ne <= SPOT_STAGE;
-    if (staged)
-        for (int x = threadIdx.x; x < ne; x += blockDim.x) {
-            const int sn = src_node(a.p2p_src[e0 + x]);
-            rfirst[x] = a.start[sn];
-            rend[x] = a.start[sn] + a.count[sn];
-        }
-    __syncthreads();
     double acc[SPOT_SB][4];
 #pragma unroll
     for (int u = 0; u < SPOT_SB; ++u)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[u][c] = 0.0;
-    for (int jj = threadIdx.x; jj < a.n; jj += blockDim.x) {
-        // is jj inside the site range of a near source leaf?  (last range with first <= jj)
-        bool near = false;
-        if (staged) {
-            int lo = 0, hi = ne;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (rfirst[mid] <= jj) lo = mid;
-                else hi = mid;
-            }
-            near = ne > 0 && jj >= rfirst[lo] && jj < rend[lo];
-        } else {
-            int lo = e0, hi = e1;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (a.start[src_node(a.p2p_src[mid])] <= jj) lo = mid;
-                else hi = mid;
-            }
-            if (lo < e1) {
-                const int sn = src_node(a.p2p_src[lo]);
-                near = jj >= a.start[sn] && jj < a.start[sn] + a.count[sn];
-            }
-        }
-        if (near) continue;
-        const double dx = x - a.xs[jj], dy = y - a.ys[jj], dz = z - a.zs[jj];
-        double g[M + 2];
-        radial<M + 1, DAMP_NONE>(dx * dx + dy * dy + dz * dz, 0.0, g);
-        double T[ncomp(M + 1)];
-        md_tensor<M + 1>(dx, dy, dz, g, T);
+    // 8 groups of 32 threads, group g on the list entries g, g + 8, ...; a leaf holds < ncrit sites
+    const int grp = threadIdx.x / 32, ln = threadIdx.x % 32;
+    for (int e = a.p2p_off[node] + grp; e < a.p2p_off[node + 1]; e += 8) {
+        const int sn = src_node(a.p2p_src[e]);
+        const int st = a.start[sn], cnt = a.count[sn];
+        for (int j = st + ln; j < st + cnt; j += 32) {
+            double T[ncomp(M + 1)];
+            spot_tensor<M>(x - a.xs[j], y - a.ys[j], z - a.zs[j], T);
 #pragma unroll
-        for (int u = 0; u < SPOT_SB; ++u) {
-            if (u >= nsb) break;
-            const double* __restrict__ msu = a.ms + (int64_t)(set0 + u) * NM * a.ldm;
-            double m[NM];
+            for (int u = 0; u < SPOT_SB; ++u) {
+                if (u >= nsb) break;
+                double m[NM];
 #pragma unroll
-            for (int s = 0; s < NM; ++s) m[s] = msu[(int64_t)s * a.ldm + jj];
-            contract<M, 1>(m, T, acc[u]);
+                for (int sc = 0; sc < NM; ++sc) m[sc] = a.ms[((int64_t)(set0 + u) * NM + sc) * a.ldm + j];
+                contract<M, 1>(m, T, acc[u]);
+            }
         }
     }
     for (int u = 0; u < nsb; ++u) {
+        __syncthreads();
+#pragma unroll
         for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = acc[u][c];
         __syncthreads();
         for (int w = 128; w > 0; w >>= 1) {
@@ -707,19 +766,28 @@ __global__ __launch_bounds__(256) void k_spot_far(SpotArgs a) {
                 for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x < 4) a.out[((int64_t)(set0 + u) * a.nspots + spot) * 4 + threadIdx.x] = red[threadIdx.x][0];
-        __syncthreads();
+        if (threadIdx.x < 4) a.out_near[((int64_t)(set0 + u) * a.nspots + spot) * 4 + threadIdx.x] = red[threadIdx.x][0];
     }
-    if (threadIdx.x == 0 && blockIdx.y == 0) a.site[spot] = site;
 }
 
 void launch_spot_far(const SpotArgs& a, int m, hipStream_t s) {
     if (a.nspots <= 0) return;
-    const dim3 g(a.nspots, (a.nsets + SPOT_SB - 1) / SPOT_SB);
+    const int sb = (a.nsets + SPOT_SB - 1) / SPOT_SB;
+    const dim3 g((a.nspots + SPOT_NSP - 1) / SPOT_NSP, sb, spot_chunks(a.n));
+    const dim3 gn(a.nspots, sb);
     switch (m) {
-        case 0: hipLaunchKernelGGL(k_spot_far<0>, g, dim3(256), 0, s, a); break;
-        case 1: hipLaunchKernelGGL(k_spot_far<1>, g, dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_spot_far<2>, g, dim3(256), 0, s, a); break;
+        case 0:
+            hipLaunchKernelGGL(k_spot_all<0>, g, dim3(256), 0, s, a);
+            hipLaunchKernelGGL(k_spot_near<0>, gn, dim3(256), 0, s, a);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_spot_all<1>, g, dim3(256), 0, s, a);
+            hipLaunchKernelGGL(k_spot_near<1>, gn, dim3(256), 0, s, a);
+            break;
+        default:
+            hipLaunchKernelGGL(k_spot_all<2>, g, dim3(256), 0, s, a);
+            hipLaunchKernelGGL(k_spot_near<2>, gn, dim3(256), 0, s, a);
+            break;
     }
     FMMH_HIP(hipGetLastError());
 }

@@ -499,6 +499,7 @@ __global__ __launch_bounds__(TRAVERSE_THREADS) void k_traverse(TraverseArgs a) {
     // after an overflowing step the count exceeds what was stored: read only
     // the stored part (the overflow flag is already set and the host retries)
     const int n = min(*a.n_in, a.cap_in);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.n_clear) *a.n_clear = 0;
     const int stride = gridDim.x * TRAVERSE_THREADS;
     // the loop runs over whole workgroups so the workgroup collectives see every thread
     for (int e0 = blockIdx.x * TRAVERSE_THREADS; e0 < n; e0 += stride) {
@@ -571,7 +572,68 @@ __global__ __launch_bounds__(256) void k_csr(const unsigned long long* __restric
         for (int x = t + 1; x <= nt; ++x) off[x] = nk;
 }
 
+// CSR of an interaction list from its unsorted (target << 32 | source) keys without a device-wide sort:
+// per-target counts, their exclusive scan (the offsets), a scatter into the targets' ranges, and a sort of
+// each range by source id in LDS.  The sources of one target are distinct, so each one's position in its
+// range is the number of smaller ones (its rank); the result equals the order of the sorted keys.
+__global__ __launch_bounds__(256) void k_list_count(const unsigned long long* __restrict__ keys, int nk,
+                                                    int* __restrict__ cnt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nk) atomicAdd(&cnt[(int)(keys[e] >> 32)], 1);
+}
+
+// cnt counts back down to 0 (ready for the next list)
+__global__ __launch_bounds__(256) void k_list_scatter(const unsigned long long* __restrict__ keys, int nk,
+                                                      const int* __restrict__ off, int* __restrict__ cnt,
+                                                      unsigned* __restrict__ tmp) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nk) return;
+    const unsigned long long k = keys[e];
+    const int t = (int)(k >> 32);
+    tmp[off[t] + atomicSub(&cnt[t], 1) - 1] = (unsigned)(k & 0xffffffffull);
+}
+
+constexpr int LIST_SORT_LDS = 2048;   // sources of one target sorted in LDS (longer ranges: in global memory)
+
+__global__ __launch_bounds__(256) void k_list_sort(const unsigned* __restrict__ tmp, const int* __restrict__ off,
+                                                   int* __restrict__ src, unsigned long long* __restrict__ keys_out) {
+    __shared__ unsigned v[LIST_SORT_LDS];
+    const int t = blockIdx.x;
+    const int e0 = off[t], k = off[t + 1] - e0;
+    if (k == 0) return;
+    const bool lds = k <= LIST_SORT_LDS;
+    if (lds)
+        for (int x = threadIdx.x; x < k; x += blockDim.x) v[x] = tmp[e0 + x];
+    __syncthreads();
+    for (int x = threadIdx.x; x < k; x += blockDim.x) {
+        const unsigned me = lds ? v[x] : tmp[e0 + x];
+        int rank = 0;
+        if (lds) {
+            for (int y = 0; y < k; ++y) rank += v[y] < me ? 1 : 0;
+        } else {
+            for (int y = 0; y < k; ++y) rank += tmp[e0 + y] < me ? 1 : 0;
+        }
+        src[e0 + rank] = (int)me;
+        if (keys_out) keys_out[e0 + rank] = ((unsigned long long)t << 32) | me;
+    }
+}
+
 static inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
+
+void launch_list_csr(const unsigned long long* keys, int n_keys, int num_targets, int* cnt, int* off, unsigned* tmp,
+                     void* scan_temp, size_t scan_temp_bytes, int* src, unsigned long long* keys_out,
+                     hipStream_t s) {
+    // cnt: num_targets + 1 zeros (left zero); off: num_targets + 1
+    if (n_keys > 0)
+        hipLaunchKernelGGL(k_list_count, dim3(blocks_for(n_keys, 256)), dim3(256), 0, s, keys, n_keys, cnt);
+    exscan_i32(scan_temp, scan_temp_bytes, cnt, off, num_targets + 1, s);
+    if (n_keys > 0) {
+        hipLaunchKernelGGL(k_list_scatter, dim3(blocks_for(n_keys, 256)), dim3(256), 0, s, keys, n_keys, off, cnt,
+                           tmp);
+        hipLaunchKernelGGL(k_list_sort, dim3(num_targets), dim3(256), 0, s, tmp, off, src, keys_out);
+    }
+    FMMH_HIP(hipGetLastError());
+}
 
 void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, hipStream_t s) {
     hipLaunchKernelGGL(k_centroid, dim3(3), dim3(CB_THREADS), 0, s, X, ldx, n, t.root);
@@ -623,6 +685,27 @@ void launch_leaf_sort(int num_leaves, int max_leaf, TreeDev& t, hipStream_t s) {
 
 void launch_gather(const SiteArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+    FMMH_HIP(hipGetLastError());
+}
+
+__global__ void k_traverse_init(int* counters, int stride, int nimg, int2* frontier) {
+    const int t = threadIdx.x;
+    if (t < 6) counters[t * stride] = t == 0 ? nimg : 0;
+    if (t < nimg) frontier[t] = make_int2(0, src_encode(t, 0));
+}
+
+__global__ void k_traverse_counts(const int* counters, int stride, int* out) {
+    const int t = threadIdx.x;
+    if (t < 6) out[t] = counters[t * stride];
+}
+
+void launch_traverse_init(int* counters, int stride, int nimg, int2* frontier, hipStream_t s) {
+    hipLaunchKernelGGL(k_traverse_init, dim3(1), dim3(64), 0, s, counters, stride, nimg, frontier);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_traverse_counts(const int* counters, int stride, int* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_traverse_counts, dim3(1), dim3(64), 0, s, counters, stride, out);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -30,6 +30,9 @@ def have_ref(variant=""):
     return os.path.exists(os.path.join(_HERE, "_ref", f"libfmmref{variant}.so"))
 
 
+_REF_LIBS = {"ref": "_ref/libfmmref.so", "ref_gnu": "_ref/libfmmref_gnu.so", "ref_o3": "_ref/libfmmref_o3.so"}
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
@@ -70,7 +73,7 @@ def field_direct(coordinates, multipoles, exclusions, field_order, damp_type="",
     if impl == "oracle":
         _lib("liboracle.so").orc_field_direct(n, dim, _p(X), _p(M), _p(E), E.shape[1], field_order, d, _p(df), _p(F))
     else:
-        lib = _lib("_ref/libfmmref.so" if impl == "ref" else "_ref/libfmmref_gnu.so")
+        lib = _lib(_REF_LIBS[impl])
         lib.ref_field_direct(n, dim, E.shape[1], _p(X), _p(M), _p(E), field_order, d, _p(df), _p(F))
     return F
 
@@ -89,7 +92,7 @@ def field_fmm(coordinates, multipoles, exclusions, theta, ncrit, expansion_order
             if diag is None:
                 raise ValueError("periodic FMM needs n > ncrit and expansion_order >= 2")
             return F
-        lib = _lib("_ref/libfmmref.so" if impl == "ref" else "_ref/libfmmref_gnu.so")
+        lib = _lib(_REF_LIBS[impl])
         lib.ref_field_fmm_periodic(n, dim, E.shape[1], _p(X), _p(M), _p(E), ctypes.c_double(theta), ncrit,
                                    expansion_order, field_order, d, _p(df), _p(B), _p(F))
         return F
@@ -97,7 +100,7 @@ def field_fmm(coordinates, multipoles, exclusions, theta, ncrit, expansion_order
         _lib("liboracle.so").orc_field_fmm(n, dim, _p(X), _p(M), _p(E), E.shape[1], ctypes.c_double(theta), ncrit,
                                            expansion_order, field_order, d, _p(df), _p(F))
     else:
-        lib = _lib("_ref/libfmmref.so" if impl == "ref" else "_ref/libfmmref_gnu.so")
+        lib = _lib(_REF_LIBS[impl])
         lib.ref_field_fmm(n, dim, E.shape[1], _p(X), _p(M), _p(E), ctypes.c_double(theta), ncrit, expansion_order,
                           field_order, d, _p(df), _p(F))
     return F
