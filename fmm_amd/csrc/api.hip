@@ -83,7 +83,8 @@ struct fmmh_context {
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
     double list_need[3] = {0, 0, 0};  // frontier capacity that sufficed, M2L and P2P entries (x 1.25), per node
-    int m2l_mfma = 1;                 // FMMH_OPT_M2L_MFMA: 0 VALU, 1 auto, 2 per-target MFMA, 3 class-grouped
+    int m2l_mfma = 1;                 // FMMH_OPT_M2L_MFMA: 0 VALU, 1 auto, 2 per-target MFMA, 3 class-grouped,
+                                      // 4 class-grouped with per-pair slots
     int precision = 0;                // FMMH_OPT_PRECISION: 0 FP64, 1 mixed (FP32 MFMA translations)
     // Tree / list cache (FMMH_OPT_REUSE_TREE): the last field_fmm tree, lists
     // and M2L classes, reused while coordinates (bitwise), n, ncrit and theta
@@ -105,6 +106,11 @@ struct fmmh_context {
     int near_nv = 0, near_grid = 0, near_np = 0;   // near-field chunks, workgroup slots, partial sites
     long long near_chunk_cap = 0;     // FMMH_OPT_NEAR_CHUNK (0: automatic)
     int cache_tiles_p = -1;           // class-grouped GEMM tile tables of the cached classes (-1: none)
+    bool cache_block_plan = false;    // target-block M2L plan of the cached classes
+    bool classes_grouped = false;     // pairs sorted by class ("m_idx_sorted", "m_flag", "m_cls") for the classes
+    int class_vbits = 0;              // bits per displacement component of the non-periodic class keys
+    bool dense_marks_ready = false;   // "m_dense_marks" holds zeros
+    int num_cu = 0;                   // compute units of the device
     // Graph replay (FMMH_OPT_GRAPH): a tree-reuse call runs as a captured hipGraph of its device work
     // once a previous call of the same shape ran eagerly (tuning done, buffers allocated).  The shape key
     // holds every pointer, size and option the captured work depends on, the pool generation (bumped by
@@ -302,6 +308,7 @@ fmmh_context* make_context(int device) {
     if (device < 0) FMMH_HIP(hipGetDevice(&device));
     c->device = device;
     FMMH_HIP(hipSetDevice(device));
+    FMMH_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device));
     scratch_setup(device);
     FMMH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     FMMH_HIP(hipHostMalloc((void**)&c->h_pinned, 4096, hipHostMallocDefault));
@@ -532,10 +539,9 @@ __global__ __launch_bounds__(256) void k_pair_count(const long long* __restrict_
 // consecutive list entries with about equal source counts; each chunk is its
 // own workgroup and writes a partial field, which k_near_fix sums in chunk
 // order (deterministic).  Within a range the heaviest chunks come first (a
-// stable sort on the range and the chunk's work in 1/4096 of the cap),
+// stable sort on the range and the chunk's work in 1/64 of the cap),
 // because the hardware starts workgroups in index order and long workgroups
 // should not finish last on an idle GPU.
-constexpr int NUM_XCD = 8;
 constexpr int NEAR_CHUNK_DIV = 16384;
 constexpr long long NEAR_CHUNK_MIN = 256;
 constexpr int NEAR_CHUNK_MAX = 64;   // chunks per leaf
@@ -660,9 +666,11 @@ __global__ void k_leaf_chunks(const long long* __restrict__ work, const long lon
 
 
 // Per leaf (one wavefront): its chunks (consecutive list entries, boundaries at about equal cumulative
-// source counts, at least one entry each) and their sort keys: range, then decreasing work.  The split
-// runs on lane 0 over the entry counts staged in LDS.
+// source counts, at least one entry each) and their order buckets: range, then decreasing work in
+// 1/64 of the chunk cap (VB_WORK_BITS).  The split runs on lane 0 over the entry counts staged in LDS.
 constexpr int VB_STAGE = 512;
+constexpr int VB_WORK_BITS = 6;
+constexpr int VB_BUCKETS = NUM_XCD << VB_WORK_BITS;
 __global__ __launch_bounds__(64) void k_leaf_vblocks(const long long* __restrict__ work,
                                                      const long long* __restrict__ tot,
                                                      const long long* __restrict__ pre, long long cap_set,
@@ -673,7 +681,7 @@ __global__ __launch_bounds__(64) void k_leaf_vblocks(const long long* __restrict
                                                      const int* __restrict__ lvb, const int* __restrict__ lpart,
                                                      int* __restrict__ vb_node, int* __restrict__ vb_e0,
                                                      int* __restrict__ vb_e1, int* __restrict__ vb_part,
-                                                     unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+                                                     int* __restrict__ bucket) {
     __shared__ int sc[VB_STAGE];
     const int l = blockIdx.x;
     if (l >= nl) return;
@@ -686,7 +694,7 @@ __global__ __launch_bounds__(64) void k_leaf_vblocks(const long long* __restrict
     __syncthreads();
     if (threadIdx.x != 0) return;
     const int v0 = lvb[l];
-    const unsigned long long x = (unsigned long long)lx[l];
+    const int x = lx[l];
     const long long cap = cap_set > 0 ? cap_set : max(pre[nl] / NEAR_CHUNK_DIV, NEAR_CHUNK_MIN);
     const long long total = tot[l];
     const long long wl = work[l];
@@ -715,10 +723,8 @@ __global__ __launch_bounds__(64) void k_leaf_vblocks(const long long* __restrict
         vb_e1[v] = e;
         vb_part[v] = K > 1 ? lpart[l] + k * count[node] : -1;
         const long long cw = total > 0 ? (wl * csrc) / total : wl / K;
-        // work in 1/4096 of the cap, 16 bits; ties keep the chunk id order (stable sort)
-        const unsigned long long w = (unsigned long long)min((cw << 12) / cap, 0xffffll);
-        keys[v] = (x << 16) | (0xffffull - w);
-        vals[v] = v;
+        const int wq = (int)min((cw << VB_WORK_BITS) / cap, (long long)((1 << VB_WORK_BITS) - 1));
+        bucket[v] = (x << VB_WORK_BITS) | ((1 << VB_WORK_BITS) - 1 - wq);
     }
 }
 
@@ -726,13 +732,96 @@ struct RangeStarts {
     int first[NUM_XCD + 1];
 };
 
-// sorted position k (range x, rank r within it) -> workgroup slot 8 r + x; unused slots hold -1
-__global__ void k_vb_scatter(const unsigned long long* __restrict__ keys, const int* __restrict__ sorted, int nv,
-                             RangeStarts rs, int* __restrict__ blk) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nv) return;
-    const int x = (int)(keys[k] >> 16);
-    blk[(k - rs.first[x]) * NUM_XCD + x] = sorted[k];
+// Workgroup slots of the chunks, in one workgroup of 1024 threads: a stable counting sort of the chunks
+// by bucket (range, then decreasing work; ties in chunk id order), the chunk at sorted position k of range
+// x going to slot (k - first[x]) NUM_XCD + x, unused slots -1.  Each wave owns a contiguous run of chunks
+// and its own column of bucket counters; the counters' scan in (bucket, wave) order gives every
+// (bucket, wave) its first position, and a wave places 64 chunks at a time, ranking equal buckets among
+// its lanes by ballots.  Chunk loads are issued VB_ORDER_UNROLL rounds at a time.
+constexpr int VB_ORDER_T = 1024;
+constexpr int VB_ORDER_W = VB_ORDER_T / 64;
+constexpr int VB_ORDER_UNROLL = 4;
+constexpr int VB_BUCKET_BITS = 3 + VB_WORK_BITS;
+static_assert(VB_BUCKETS * VB_ORDER_W == 8 * VB_ORDER_T, "the scan gives each thread 8 counters");
+static_assert(NUM_XCD == 8, "3 range bits");
+__global__ __launch_bounds__(VB_ORDER_T) void k_vb_order(const int* __restrict__ bucket, int nv, RangeStarts rs,
+                                                         int* __restrict__ slots, int nslots) {
+    __shared__ int cnt[VB_BUCKETS * VB_ORDER_W];   // [bucket][wave]
+    __shared__ int wsum[VB_ORDER_W];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int i = tid; i < nslots; i += VB_ORDER_T) slots[i] = -1;
+    for (int i = tid; i < VB_BUCKETS * VB_ORDER_W; i += VB_ORDER_T) cnt[i] = 0;
+    const int per = (nv + VB_ORDER_W - 1) / VB_ORDER_W;
+    const int c0 = min(nv, w * per), c1 = min(nv, c0 + per);
+    __syncthreads();
+    for (int r = c0; r < c1; r += 64 * VB_ORDER_UNROLL) {
+        int b[VB_ORDER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < VB_ORDER_UNROLL; ++u) {
+            const int i = r + u * 64 + lane;
+            b[u] = i < c1 ? bucket[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < VB_ORDER_UNROLL; ++u)
+            if (b[u] >= 0) atomicAdd(&cnt[b[u] * VB_ORDER_W + w], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counters in (bucket, wave) order: 8 consecutive counters per thread
+    int loc[8], tot = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        loc[j] = cnt[8 * tid + j];
+        tot += loc[j];
+    }
+    int incl = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int run = incl - tot;
+    for (int v = 0; v < w; ++v) run += wsum[v];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        cnt[8 * tid + j] = run;
+        run += loc[j];
+    }
+    __threadfence();   // the -1 fill is visible before any slot is written below
+    __syncthreads();
+    for (int r = c0; r < c1; r += 64 * VB_ORDER_UNROLL) {
+        int b[VB_ORDER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < VB_ORDER_UNROLL; ++u) {
+            const int i = r + u * 64 + lane;
+            b[u] = i < c1 ? bucket[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < VB_ORDER_UNROLL; ++u) {
+            const int i = r + u * 64 + lane;
+            const bool valid = b[u] >= 0;
+            unsigned long long same = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < VB_BUCKET_BITS; ++bit) {
+                const bool on = ((b[u] >> bit) & 1) != 0;
+                const unsigned long long m = __ballot(valid && on);
+                same &= on ? m : ~m;
+            }
+            // lanes of this round with the same bucket: the lowest one advances the counter
+            const int rank = __popcll(same & ((1ull << lane) - 1ull));
+            int base = 0;
+            if (valid) base = cnt[b[u] * VB_ORDER_W + w];
+            __builtin_amdgcn_wave_barrier();
+            if (valid && rank == 0) cnt[b[u] * VB_ORDER_W + w] = base + __popcll(same);
+            __builtin_amdgcn_wave_barrier();
+            if (valid) {
+                const int k = base + rank;
+                const int x = b[u] >> VB_WORK_BITS;
+                slots[(k - rs.first[x]) * NUM_XCD + x] = i;
+            }
+        }
+    }
 }
 
 __global__ void k_invert_perm(const int* __restrict__ perm, int* __restrict__ inv, int n) {
@@ -743,6 +832,16 @@ __global__ void k_invert_perm(const int* __restrict__ perm, int* __restrict__ in
 __global__ void k_fill_i32(int* a, int v, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
+}
+
+// Exclusive scan: one workgroup for small arrays, the device-wide scan otherwise (temp sized for n).
+void exscan_auto(fmmh_context* c, const char* temp_name, const int* in, int* out, int n, hipStream_t s) {
+    if (n <= EXSCAN_SMALL_MAX) {
+        exscan_i32_small(in, out, n, s);
+        return;
+    }
+    const size_t tb = exscan_i32_temp(n);
+    exscan_i32(c->buf<char>(temp_name, tb), tb, in, out, n, s);
 }
 
 // Device flag of exclusion entries above n (only the device-array entry points can pass them; the host
@@ -759,7 +858,9 @@ void bad_excl_readback(fmmh_context* c) {
                             hipMemcpyDeviceToHost, c->stream));
 }
 void bad_excl_check(fmmh_context* c) {   // after the stream synchronisation that follows the readback
-    if (c->h_pinned[PIN_BAD_EXCL]) throw Error(FMMH_ERR_ARG, "exclusion index outside 0..n");
+    const int f = c->h_pinned[PIN_BAD_EXCL];
+    if (f & 2) throw Error(FMMH_ERR_INTERNAL, "a target with two far pairs of one displacement class");
+    if (f) throw Error(FMMH_ERR_ARG, "exclusion index outside 0..n");
 }
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
@@ -1118,9 +1219,7 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     launch_emit_nodes(n, t, s);
     launch_node_links(nn, t, s);
     c->mark("t_nodes");
-    size_t tb2 = exscan_i32_temp(nn);
-    void* temp2 = c->buf<char>("t_temp2", tb2);
-    exscan_i32(temp2, tb2, t.is_leaf, leaf_scan, nn, s);
+    exscan_auto(c, "t_temp2", t.is_leaf, leaf_scan, nn, s);
     FMMH_HIP(hipMemcpyAsync(hp, t.info, (3 + KEY_LEVELS + 2) * sizeof(int), hipMemcpyDeviceToHost, s));
     FMMH_HIP(hipMemcpyAsync(hp + 40, leaf_scan + (nn - 1), sizeof(int), hipMemcpyDeviceToHost, s));
     FMMH_HIP(hipMemcpyAsync(hp + 41, t.is_leaf + (nn - 1), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1232,6 +1331,8 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int ni
 // for the class count: m2l_classes_begin launches the key, sort, flag, scan and assign kernels and the
 // count's readback (false when there are no pairs); after a stream sync m2l_classes_end sizes and fills
 // the class tensors.  The count is left in cache_classes and stats_classes.
+void m2l_sort_classes(fmmh_context* c, M2LClassArgs& ca, bool assign);
+
 bool m2l_classes_begin(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, const Periodic& P,
                        const double* disp, M2LClassArgs& ca) {
     hipStream_t s = c->stream;
@@ -1260,6 +1361,10 @@ bool m2l_classes_begin(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, in
     ca.class_dl = c->buf<int>("m_class_dl", np);
     ca.periodic = P.on ? 1 : 0;
     ca.disp = disp;
+    // Far-pair displacements are below 2 r_root = 2^{max_depth} u in each component (u the smaller
+    // half-width, root level 1), so max_depth + 2 bits hold v with a bit to spare; the sort then runs over
+    // 3 vbits + 8 key bits instead of 62.
+    ca.vbits = std::min(18, std::max(4, tr.max_depth + 2));
     if (P.on) {
         ca.class_hi_in = c->buf<unsigned long long>("m_hi_in", np);
         ca.class_hi_sorted = c->buf<unsigned long long>("m_hi_sorted", np);
@@ -1267,13 +1372,41 @@ bool m2l_classes_begin(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, in
         ca.class_key_hi = c->buf<unsigned long long>("m_class_key_hi", np);
     }
     c->cache_tiles_p = -1;
+    c->cache_block_plan = false;
+    c->class_vbits = ca.vbits;
     ++c->cache_gen;
-    launch_m2l_keys(ca, s);
+    if (!P.on) {
+        // dense numbering: no sort of the pairs (the grouping by class is made when an engine needs it)
+        const int nwg = nblk(np, 256);
+        ca.wg_range = c->buf<int>("m_wg_range", 4 * (size_t)nwg);
+        launch_m2l_keys(ca, s);
+        int* marks = c->buf<int>("m_dense_marks", CLASS_DENSE_CAP);
+        if (!c->dense_marks_ready) {
+            FMMH_HIP(hipMemsetAsync(marks, 0, sizeof(int) * (size_t)CLASS_DENSE_CAP, s));
+            c->dense_marks_ready = true;
+        }
+        launch_class_dense(ca, nwg, c->buf<int>("m_dense_dims", 8), marks, c->buf<int>("m_dense_ids", CLASS_DENSE_CAP),
+                           c->buf<int>("m_dense_tsum", CLASS_DENSE_CAP / 4096),
+                           c->buf<int>("m_dense_toff", CLASS_DENSE_CAP / 4096), s);
+        c->classes_grouped = false;
+    } else {
+        launch_m2l_keys(ca, s);
+        m2l_sort_classes(c, ca, true);
+    }
+    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 300, dev_scalars, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    return true;
+}
+
+// The pairs grouped by class by sorting the class keys: sorted order (pair_idx_sorted), class-start flags
+// and their scan; with assign, also the class ids, keys and level differences and the count.
+void m2l_sort_classes(fmmh_context* c, M2LClassArgs& ca, bool assign) {
+    hipStream_t s = c->stream;
+    const int np = ca.n_pairs;
     size_t tbytes = std::max(sort_pairs_u64_temp(np), exscan_i32_temp(np));
     void* temp = c->buf<char>("m_temp", tbytes);
-    if (!P.on) {
-        sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, ca.pair_idx_sorted, np, 62,
-                       s);
+    if (!ca.periodic) {
+        sort_pairs_u64(temp, tbytes, ca.class_key_in, ca.class_key_sorted, ca.pair_idx, ca.pair_idx_sorted, np,
+                       3 * ca.vbits + 8, s);
     } else {
         // (hi, lo) lexicographic order by two stable radix sorts: lo first, then hi
         int* idx1 = c->buf<int>("m_idx1", np);
@@ -1285,15 +1418,35 @@ bool m2l_classes_begin(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, in
     }
     launch_m2l_flags(ca, s);
     exscan_i32(temp, tbytes, ca.flag, ca.cls, np, s);
-    launch_m2l_assign(ca, s);
-    FMMH_HIP(hipMemcpyAsync(c->h_pinned + 300, dev_scalars, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    return true;
+    if (assign) launch_m2l_assign(ca, s);
+    c->classes_grouped = true;
+}
+
+// The class-grouped GEMM's grouping of the pairs, for classes numbered without it (the key sort of the
+// packed non-periodic keys, which the dense numbering follows).
+void m2l_group_pairs(fmmh_context* c, int np) {
+    if (c->classes_grouped) return;
+    M2LClassArgs ca{};
+    ca.n_pairs = np;
+    ca.class_key_in = c->buf<unsigned long long>("m_key_in", np);
+    ca.pair_idx = c->buf<int>("m_idx_in", np);
+    ca.class_key_sorted = c->buf<unsigned long long>("m_key_sorted", np);
+    ca.pair_idx_sorted = c->buf<int>("m_idx_sorted", np);
+    ca.flag = c->buf<int>("m_flag", np);
+    ca.cls = c->buf<int>("m_cls", np);
+    ca.vbits = c->class_vbits;
+    m2l_sort_classes(c, ca, false);
 }
 
 void m2l_classes_end(fmmh_context* c, int p, bool pending, M2LClassArgs& ca) {
     int ncls = 0;
     if (pending) {
-        if (c->h_pinned[301]) throw Error(FMMH_ERR_UNSUPPORTED, "M2L displacement outside the class key range");
+        if (c->h_pinned[301] & 1) throw Error(FMMH_ERR_UNSUPPORTED, "M2L displacement outside the class key range");
+        if (c->h_pinned[301] & 4) {   // displacement range too wide for the dense numbering: sort the keys
+            m2l_sort_classes(c, ca, true);
+            FMMH_HIP(hipMemcpyAsync(c->h_pinned + 300, ca.num_classes, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            c->sync();
+        }
         ncls = c->h_pinned[300];
         ca.Tc = c->buf<double>("m_tc", (size_t)ncls * ncomp(2 * p));
         launch_m2l_class_tensors(ca, std::min(ncls, 4096), c->stream);
@@ -1328,9 +1481,49 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     if (order && c->m2l_mfma == 0)
         throw Error(FMMH_ERR_UNSUPPORTED, "learned expansion orders need an MFMA M2L engine (FMMH_OPT_M2L_MFMA != 0)");
     const size_t node_stride = (size_t)nn * nm;
+    // target-block form of the class-grouped GEMM: one FP64 set at a time, every pair at order p
+    const bool block_form = !mfma && (c->m2l_mfma == 1 || c->m2l_mfma == 3) && !order && c->precision != 1 &&
+                            p <= M2LB_PMAX && np > 0 && ncls <= M2LB_MAX_CLASSES;
+    if (block_form) {
+        const int nbm = nblk(nn, M2LB_BT);
+        M2LBlockArgs ba{};
+        ba.m2l_off = L.m2l_off; ba.m2l_src = L.m2l_src; ba.pair_class = pair_class;
+        ba.class_dl = c->buf<int>("m_class_dl", np);
+        ba.level = tr.t.level; ba.octant = tr.t.octant;
+        ba.nn = nn; ba.ncls = ncls; ba.nbm = nbm;
+        ba.tperm = c->buf<int>("b_tperm", nn);
+        ba.tinv = c->buf<int>("b_tinv", nn);
+        ba.tcount = c->buf<int>("b_tcount", nn + 1);
+        ba.pbase = c->buf<int>("b_pbase", nn + 1);
+        ba.ntile = c->buf<int>("b_ntile", nbm + 1);
+        ba.toff = c->buf<int>("b_toff", nbm + 1);
+        ba.rec = c->buf<int>("b_rec", (size_t)np * M2LB_REC);
+        ba.ntg = c->buf<int>("b_ntg", 1);
+        ba.flag = c->buf<int>("x_bad_excl", 1);   // cleared by the near-field launch of this call
+        if (!(classes_cached && c->cache_block_plan)) {
+            launch_m2lb_order(ba, s);
+            exscan_auto(c, "b_scan_temp", ba.tcount, ba.pbase, nn + 1, s);
+            launch_m2lb_tiles(ba, s);
+            exscan_auto(c, "b_scan_temp", ba.ntile, ba.toff, nbm + 1, s);
+            c->cache_block_plan = true;
+            ++c->cache_gen;
+        }
+        // two workgroups per compute unit (LDS: the block's local expansions plus the staged tile)
+        const int grid = std::max(1, 2 * c->num_cu);
+        double* CMs = c->buf<double>("m_cms", node_stride * nsets);
+        launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
+        M2LBRunArgs ra{ba.toff, ba.pbase, ba.rec, nbm, ba.ntg, ba.tperm, ba.tinv, nn, nullptr, Tc, tb.gemm_fidx,
+                       tb.gemm_tdeg, c->buf<double>("b_partial", (size_t)(nbm + grid) * nm * M2LB_BT), grid};
+        for (int set = 0; set < nsets; ++set) {
+            ra.CMs = CMs + set * node_stride;
+            launch_m2lb_run(ra, p, tr.t.r, LE + set * node_stride, s);
+        }
+        return;
+    }
     // class-grouped GEMM: one set at a time, per-pair slots of np x nm doubles
-    const bool gemm = !mfma && (c->m2l_mfma == 1 || c->m2l_mfma == 3) && gemm_fits;
+    const bool gemm = !mfma && (c->m2l_mfma == 1 || c->m2l_mfma == 3 || c->m2l_mfma == 4) && gemm_fits;
     if (gemm) {
+        m2l_group_pairs(c, np);
         const int* flag = c->buf<int>("m_flag", np);
         const int* cls = c->buf<int>("m_cls", np);
         const int* pair_csr = c->buf<int>("m_idx_sorted", np);
@@ -1373,8 +1566,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         // tile tables of the (cached) classes without learned orders are kept with them
         if (!(classes_cached && !order && c->cache_tiles_p == p)) {
             launch_m2l_tiles(flag, cls, np, ngrp, class_off, tile_cnt, s);
-            size_t tbytes = exscan_i32_temp(ngrp + 1);
-            exscan_i32(c->buf<char>("m_temp_tiles", tbytes), tbytes, tile_cnt, tile_off, ngrp + 1, s);
+            exscan_auto(c, "m_temp_tiles", tile_cnt, tile_off, ngrp + 1, s);
             c->cache_tiles_p = order ? -1 : p;
             ++c->cache_gen;
         }
@@ -1634,10 +1826,8 @@ void near_order_begin(fmmh_context* c, TreeOut& tr, ListsOut& L) {
     hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, s, work, pre, nl);
     hipLaunchKernelGGL(k_leaf_chunks, dim3(nblk(nl + 1, 256)), dim3(256), 0, s, work, pre, L.p2p_off,
                        tr.t.leaf_node, tr.t.count, nl, c->near_chunk_cap, lk, lx, lps, range_cnt);
-    size_t eb = exscan_i32_temp(nl + 1);
-    void* etmp = c->buf<char>("s_work_temp", eb);
-    exscan_i32(etmp, eb, lk, lvb, nl + 1, s);
-    exscan_i32(etmp, eb, lps, lpart, nl + 1, s);
+    exscan_auto(c, "s_work_temp", lk, lvb, nl + 1, s);
+    exscan_auto(c, "s_work_temp", lps, lpart, nl + 1, s);
     int* hp = c->h_pinned + 900;
     FMMH_HIP(hipMemcpyAsync(hp, range_cnt, NUM_XCD * sizeof(int), hipMemcpyDeviceToHost, s));
     FMMH_HIP(hipMemcpyAsync(hp + NUM_XCD, lvb + nl, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1670,19 +1860,12 @@ void near_order_end(fmmh_context* c, TreeOut& tr, ListsOut& L) {
     int* vb_e0 = c->buf<int>("s_vb_e0", nv);
     int* vb_e1 = c->buf<int>("s_vb_e1", nv);
     int* vb_part = c->buf<int>("s_vb_part", nv);
-    unsigned long long* vk_in = c->buf<unsigned long long>("s_vb_keys_in", nv);
-    unsigned long long* vk = c->buf<unsigned long long>("s_vb_keys", nv);
-    int* vv_in = c->buf<int>("s_vb_vals_in", nv);
-    int* vv = c->buf<int>("s_vb_sorted", nv);
+    int* vbucket = c->buf<int>("s_vb_bucket", nv);
     hipLaunchKernelGGL(k_leaf_vblocks, dim3(nl), dim3(64), 0, s, work, c->buf<long long>("s_leaf_tot", nl), pre,
                        c->near_chunk_cap, L.p2p_off, c->buf<int>("s_entry_cnt", std::max(L.n_p2p, 1)),
-                       tr.t.leaf_node, tr.t.count, nl, lk, lx, lvb, lpart, vb_node, vb_e0, vb_e1, vb_part, vk_in,
-                       vv_in);
-    size_t wb = sort_pairs_u64_temp(nv);
-    sort_pairs_u64(c->buf<char>("s_vb_temp", wb), wb, vk_in, vk, vv_in, vv, nv, 19, s);   // 3 + 16 key bits
+                       tr.t.leaf_node, tr.t.count, nl, lk, lx, lvb, lpart, vb_node, vb_e0, vb_e1, vb_part, vbucket);
     int* slots = c->buf<int>("s_work_blk", (size_t)c->near_grid);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(c->near_grid, 256)), dim3(256), 0, s, slots, -1, c->near_grid);
-    hipLaunchKernelGGL(k_vb_scatter, dim3(nblk(nv, 256)), dim3(256), 0, s, vk, vv, nv, rs, slots);
+    hipLaunchKernelGGL(k_vb_order, dim3(1), dim3(VB_ORDER_T), 0, s, vbucket, nv, rs, slots, c->near_grid);
 }
 
 void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, const Periodic& P) {
@@ -1758,6 +1941,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         c->cache_valid = false;
         c->cache_work_valid = false;
         c->cache_tiles_p = -1;
+        c->cache_block_plan = false;
         ++c->cache_gen;
         build_tree(c, k.X, k.ldx, n, ncrit, tr);
         c->mark("tree");
@@ -2585,8 +2769,8 @@ int fmmh_set_option(fmmh_context* ctx, int32_t key, double value) {
         } else if (key == FMMH_OPT_ORDER_FORCE_FULL) {
             c->sel_force_full = value != 0.0 ? 1 : 0;
         } else if (key == FMMH_OPT_M2L_MFMA) {
-            if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0)
-                throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1, 2 or 3");
+            if (value != 0.0 && value != 1.0 && value != 2.0 && value != 3.0 && value != 4.0)
+                throw Error(FMMH_ERR_ARG, "M2L MFMA mode must be 0, 1, 2, 3 or 4");
             c->m2l_mfma = (int)value;
         }
         else throw Error(FMMH_ERR_ARG, "unknown option key " + std::to_string(key));

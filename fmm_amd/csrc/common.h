@@ -27,6 +27,7 @@ __host__ __device__ __forceinline__ int src_image(int enc) { return (int)((unsig
 __host__ __device__ __forceinline__ int src_encode(int img, int node) {
     return (int)(((unsigned)img << IMG_SHIFT) | (unsigned)node);
 }
+constexpr int NUM_XCD = 8;       // MI355X: workgroup g runs on XCD g mod 8, each XCD with its own L2
 constexpr int KMAX_NEAR = 6;     // largest near-field tensor order (damped tensors stop at 6)
 
 enum Damp { DAMP_NONE = 0, DAMP_THOLE = 1, DAMP_AMOEBA = 2, DAMP_ERF = 3 };

@@ -201,6 +201,8 @@ struct M2LClassArgs {
     const double* cx; const double* cy; const double* cz; const double* r;
     const int* level;
     unsigned long long* class_key_in;      // per pair: packed (v, level difference)
+    int vbits;                              // bits per component of v in non-periodic keys (<= 18)
+    int* wg_range;                          // per key workgroup: max |v_x|, |v_y|, |v_z|, |dl| (or null)
     int* pair_idx;                          // per pair: its CSR position
     unsigned long long* class_key_sorted;
     int* pair_idx_sorted;
@@ -226,6 +228,13 @@ struct M2LClassArgs {
     const double* disp;                     // per image id: displacement
 };
 void launch_m2l_keys(const M2LClassArgs& a, hipStream_t s);
+// Dense class numbering of non-periodic pairs (after launch_m2l_keys with wg_range over nwg workgroups):
+// pair_class, class_key, class_dl and num_classes without the key sort; error bit 4 when the displacement
+// range exceeds CLASS_DENSE_CAP indices (then nothing is written).  dims: 5 ints; marks: CLASS_DENSE_CAP
+// ints, zero on entry and on return; ids: CLASS_DENSE_CAP ints; tsum, toff: CLASS_DENSE_CAP / 4096 ints.
+constexpr int CLASS_DENSE_CAP = 1 << 23;
+void launch_class_dense(const M2LClassArgs& a, int nwg, int* dims, int* marks, int* ids, int* tsum, int* toff,
+                        hipStream_t s);
 void launch_m2l_flags(const M2LClassArgs& a, hipStream_t s);        // after the key sort
 void launch_m2l_assign(const M2LClassArgs& a, hipStream_t s);       // after the flag scan
 void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s);
@@ -300,6 +309,47 @@ struct M2LGemmArgs {
     const int* grp_order;
     const unsigned char* order;
 };
+// Target-block M2L (one set, FP64, every pair at order p <= M2LB_PMAX): pairs grouped by blocks of
+// M2LB_BT targets in (level, octant, id) order and by class inside a block, in tiles of up to 16 pairs;
+// local expansions accumulate per block in LDS, a partial per (block, workgroup) where a workgroup's
+// share of the tiles leaves the block.
+constexpr int M2LB_PMAX = 6;
+constexpr int M2LB_BT = 64;              // targets per block (one bit each in a class mask)
+constexpr int M2LB_REC = 24;             // ints per tile record: class, dl, count, 0, 16 sources, 16 bytes
+constexpr int M2LB_MAX_CLASSES = 4000;   // class masks and offsets in 64 KB of LDS
+struct M2LBlockArgs {
+    const int* m2l_off;                  // CSR of the far pairs
+    const int* m2l_src;
+    const int* pair_class;               // CSR position -> class
+    const int* class_dl;                 // class -> level_i - level_j
+    const int* level;                    // per node
+    const int* octant;
+    int nn, ncls;
+    int nbm;                             // block capacity, ceil(nn / M2LB_BT)
+    int* tperm;                          // [nn]: block order position -> target node
+    int* tinv;                           // [nn]: node -> position, -1 without far pairs
+    int* tcount;                         // [nn + 1]: pairs per position, 0 past ntg
+    int* pbase;                          // [nn + 1]: exclusive scan of tcount (exscan after the order kernel)
+    int* ntile;                          // [nbm + 1]: tiles per block
+    int* toff;                           // [nbm + 1]: exclusive scan of ntile (exscan after the tiles kernel)
+    int* rec;                            // [n_pairs][M2LB_REC]: tile records at the block's pair offset
+    int* ntg;                            // device scalar: targets with far pairs
+    int* flag;                           // bit 2: a target with two pairs of one class
+};
+void launch_m2lb_order(const M2LBlockArgs& a, hipStream_t s);
+void launch_m2lb_tiles(const M2LBlockArgs& a, hipStream_t s);
+struct M2LBRunArgs {
+    const int* toff; const int* pbase; const int* rec; int nbm;
+    const int* ntg; const int* tperm; const int* tinv; int nn;
+    const double* CMs;                   // scaled source expansions of one set, [node][nm]
+    const double* Tc;                    // class tensors
+    const int* fidx; const int* tdeg;    // the class-grouped GEMM's fragment tables
+    double* partial;                     // [nbm + grid][nm][M2LB_BT]
+    int grid;                            // workgroups of the run kernel
+};
+// Run kernel, per-block sums (scaled by r^{-(|t|+1)}) into LE, and zeros for nodes without far pairs.
+void launch_m2lb_run(const M2LBRunArgs& a, int p, const double* r, double* LE, hipStream_t s);
+
 constexpr int M2L_GEMM_PMAX = 10;
 constexpr int M2L_GEMM_TN = 32;          // pairs per tile
 void launch_m2l_tiles(const int* flag, const int* cls, int n_pairs, int ncls, int* class_off, int* tile_cnt,
@@ -486,5 +536,8 @@ void sort_keys_u64(void* temp, size_t temp_bytes, const unsigned long long* kin,
                    int end_bit, hipStream_t s);
 size_t exscan_i32_temp(int n);
 void exscan_i32(void* temp, size_t temp_bytes, const int* in, int* out, int n, hipStream_t s);
+// The same scan in one workgroup (one launch; for n up to EXSCAN_SMALL_MAX).
+constexpr int EXSCAN_SMALL_MAX = 1 << 16;
+void exscan_i32_small(const int* in, int* out, int n, hipStream_t s);
 
 }  // namespace fmmh

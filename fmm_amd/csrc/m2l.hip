@@ -27,16 +27,36 @@
 
 namespace fmmh {
 
-constexpr int VBITS = 18;
-constexpr long long VOFF = 1ll << (VBITS - 1);
-constexpr int CLASS_KEY_BITS = 3 * VBITS + 8;
+// Non-periodic keys: three a.vbits-bit components and the level difference in the low 8 bits.
 // periodic lo word: three 21-bit components (image pairs can span the box)
 constexpr int PBITS = 21;
 constexpr long long POFF = 1ll << (PBITS - 1);
 
+__device__ void m2l_key(const M2LClassArgs& a, int e, int* range);
+
+// Class keys of the pairs; with a.wg_range, also the workgroup's largest |v_x|, |v_y|, |v_z| and
+// |level difference| (for the dense class numbering).
 __global__ __launch_bounds__(256) void k_m2l_keys(M2LClassArgs a) {
+    __shared__ int red[4][4];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.n_pairs) return;
+    int range[4] = {0, 0, 0, 0};
+    if (e < a.n_pairs) m2l_key(a, e, range);
+    if (!a.wg_range) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int v = range[k];
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
+        if (lane == 0) red[w][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4)
+        a.wg_range[blockIdx.x * 4 + threadIdx.x] =
+            max(max(red[0][threadIdx.x], red[1][threadIdx.x]), max(red[2][threadIdx.x], red[3][threadIdx.x]));
+}
+
+__device__ void m2l_key(const M2LClassArgs& a, int e, int* range) {
     const unsigned long long pk = a.pair_keys[e];
     const int i = (int)(pk >> 32);
     const int jenc = (int)(pk & 0xffffffffull);
@@ -46,9 +66,14 @@ __global__ __launch_bounds__(256) void k_m2l_keys(M2LClassArgs a) {
     const long long vy = llrint((a.cy[i] - a.cy[j]) / u);
     const long long vz = llrint((a.cz[i] - a.cz[j]) / u);
     const int dl = a.level[i] - a.level[j];
-    const long long vlim = a.periodic ? POFF : VOFF;
+    const long long voff = 1ll << (a.vbits - 1);
+    const long long vlim = a.periodic ? POFF : voff;
     if (llabs(vx) >= vlim || llabs(vy) >= vlim || llabs(vz) >= vlim || dl < -100 || dl > 100)
         atomicOr(a.error, 1);
+    range[0] = (int)min(llabs(vx), (long long)vlim);
+    range[1] = (int)min(llabs(vy), (long long)vlim);
+    range[2] = (int)min(llabs(vz), (long long)vlim);
+    range[3] = min(abs(dl), 127);
     if (a.periodic) {
         a.class_key_in[e] = ((unsigned long long)(vx + POFF) << (2 * PBITS)) |
                             ((unsigned long long)(vy + POFF) << PBITS) | (unsigned long long)(vz + POFF);
@@ -57,9 +82,10 @@ __global__ __launch_bounds__(256) void k_m2l_keys(M2LClassArgs a) {
         a.class_hi_in[e] = ((unsigned long long)img << 16) | ((unsigned long long)lev << 8) |
                            (unsigned long long)(dl + 128);
     } else {
-        a.class_key_in[e] = ((unsigned long long)(vx + VOFF) << (2 * VBITS + 8)) |
-                            ((unsigned long long)(vy + VOFF) << (VBITS + 8)) |
-                            ((unsigned long long)(vz + VOFF) << 8) | (unsigned long long)(dl + 128);
+        const int b = a.vbits;
+        a.class_key_in[e] = ((unsigned long long)(vx + voff) << (2 * b + 8)) |
+                            ((unsigned long long)(vy + voff) << (b + 8)) |
+                            ((unsigned long long)(vz + voff) << 8) | (unsigned long long)(dl + 128);
     }
     a.pair_idx[e] = e;
 }
@@ -131,9 +157,12 @@ __global__ __launch_bounds__(64) void k_class_tensor(M2LClassArgs a) {
                 vz -= a.disp[3 * img + 2] / u;
             }
         } else {
-            vx = (double)((long long)((key >> (2 * VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
-            vy = (double)((long long)((key >> (VBITS + 8)) & ((1ull << VBITS) - 1)) - VOFF);
-            vz = (double)((long long)((key >> 8) & ((1ull << VBITS) - 1)) - VOFF);
+            const int b = a.vbits;
+            const unsigned long long mask = (1ull << b) - 1;
+            const long long voff = 1ll << (b - 1);
+            vx = (double)((long long)((key >> (2 * b + 8)) & mask) - voff);
+            vy = (double)((long long)((key >> (b + 8)) & mask) - voff);
+            vz = (double)((long long)((key >> 8) & mask) - voff);
         }
         hermite_aux(vx, vy, vz, K, A, sxyz, slb);
         for (int idx = threadIdx.x; idx < nT; idx += 64) {
@@ -291,6 +320,195 @@ __global__ __launch_bounds__(128, 4) void k_m2l(M2LArgs a) {
 }
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------- dense class numbering
+// Non-periodic classes without sorting the pairs.  With |v_x| <= Rx, |v_y| <= Ry, |v_z| <= Rz and
+// |level difference| <= Rd over all pairs, (v, dl) has the dense index
+//   (((v_x + Rx) (2Ry+1) + v_y + Ry) (2Rz+1) + v_z + Rz) (2Rd+1) + dl + Rd   in [0, N),
+// lexicographic in (v_x, v_y, v_z, dl) like the packed keys.  Every pair marks its index, an exclusive
+// scan of the marks numbers the classes in key order -- the numbering of the key sort -- and every pair
+// reads its class back.  dims = Rx, Ry, Rz, Rd, N (N = 0: beyond CLASS_DENSE_CAP, error bit 4).  The
+// marks are zero before and after (the compaction clears the ones it reads).
+constexpr int DENSE_TILE = 4096;           // scan tile: 256 threads x 16 entries
+constexpr int DENSE_TILES_MAX = CLASS_DENSE_CAP / DENSE_TILE;
+static_assert(DENSE_TILES_MAX <= 2048, "one workgroup scans the tile sums, 2 per thread");
+
+__global__ __launch_bounds__(1024) void k_dense_dims(const int* __restrict__ wg_range, int nwg, int* __restrict__ dims,
+                                                     int* __restrict__ error) {
+    __shared__ int red[16][4];
+    int m[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nwg; b += 1024)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) m[k] = max(m[k], wg_range[b * 4 + k]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) m[k] = max(m[k], __shfl_xor(m[k], d));
+        if (lane == 0) red[w][k] = m[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long R[4], N = 1;
+        for (int k = 0; k < 4; ++k) {
+            R[k] = 0;
+            for (int v = 0; v < 16; ++v) R[k] = max(R[k], (long long)red[v][k]);
+            N *= 2 * R[k] + 1;
+        }
+        for (int k = 0; k < 4; ++k) dims[k] = (int)R[k];
+        dims[4] = N <= CLASS_DENSE_CAP ? (int)N : 0;
+        if (N > CLASS_DENSE_CAP) atomicOr(error, 4);
+    }
+}
+
+struct DenseKey {
+    int vx, vy, vz, dl;
+};
+__device__ __forceinline__ DenseKey dense_decode_key(unsigned long long key, int b) {
+    const unsigned long long mask = (1ull << b) - 1;
+    const int voff = 1 << (b - 1);
+    return DenseKey{(int)((key >> (2 * b + 8)) & mask) - voff, (int)((key >> (b + 8)) & mask) - voff,
+                    (int)((key >> 8) & mask) - voff, (int)(key & 255ull) - 128};
+}
+__device__ __forceinline__ int dense_index(const DenseKey& k, const int* dims) {
+    return (((k.vx + dims[0]) * (2 * dims[1] + 1) + k.vy + dims[1]) * (2 * dims[2] + 1) + k.vz + dims[2]) *
+               (2 * dims[3] + 1) + k.dl + dims[3];
+}
+
+__global__ __launch_bounds__(256) void k_dense_mark(const unsigned long long* __restrict__ key, int np, int vbits,
+                                                    const int* __restrict__ dims, int* __restrict__ marks) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= np || dims[4] == 0) return;
+    marks[dense_index(dense_decode_key(key[e], vbits), dims)] = 1;
+}
+
+__device__ __forceinline__ int block_exscan_256(int v, int* wsum, int& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int run = incl - v;
+    total = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (k < w) run += wsum[k];
+        total += wsum[k];
+    }
+    return run;
+}
+
+__global__ __launch_bounds__(256) void k_dense_tile_sum(const int* __restrict__ marks, const int* __restrict__ dims,
+                                                        int* __restrict__ tsum) {
+    __shared__ int wsum[4];
+    const int N = dims[4];
+    const int t = blockIdx.x;
+    if (t * DENSE_TILE >= N) return;
+    const int i0 = t * DENSE_TILE + threadIdx.x * 16;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v += i0 + k < N ? marks[i0 + k] : 0;
+    int total;
+    block_exscan_256(v, wsum, total);
+    if (threadIdx.x == 0) tsum[t] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_dense_tile_scan(const int* __restrict__ tsum, const int* __restrict__ dims,
+                                                          int* __restrict__ toff, int* __restrict__ num_classes) {
+    __shared__ int wsum[16];
+    const int nt = (dims[4] + DENSE_TILE - 1) / DENSE_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int a0 = 2 * tid < nt ? tsum[2 * tid] : 0, a1 = 2 * tid + 1 < nt ? tsum[2 * tid + 1] : 0;
+    int incl = a0 + a1;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int run = incl - a0 - a1, total = 0;
+    for (int k = 0; k < 16; ++k) {
+        if (k < w) run += wsum[k];
+        total += wsum[k];
+    }
+    if (2 * tid < nt) toff[2 * tid] = run;
+    if (2 * tid + 1 < nt) toff[2 * tid + 1] = run + a0;
+    if (tid == 0) *num_classes = total;
+}
+
+__global__ __launch_bounds__(256) void k_dense_ids(const int* __restrict__ marks, const int* __restrict__ dims,
+                                                   const int* __restrict__ toff, int* __restrict__ ids) {
+    __shared__ int wsum[4];
+    const int N = dims[4];
+    const int t = blockIdx.x;
+    if (t * DENSE_TILE >= N) return;
+    const int i0 = t * DENSE_TILE + threadIdx.x * 16;
+    int m[16], v = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        m[k] = i0 + k < N ? marks[i0 + k] : 0;
+        v += m[k];
+    }
+    int total;
+    int run = toff[t] + block_exscan_256(v, wsum, total);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (i0 + k < N) ids[i0 + k] = run;
+        run += m[k];
+    }
+}
+
+// Pairs' classes (the first nblk(np) workgroups), and the classes' keys and level differences from their
+// dense indices, clearing the marks (the other workgroups, grid-stride over N).
+__global__ __launch_bounds__(256) void k_dense_finish(M2LClassArgs a, const int* __restrict__ dims,
+                                                      int* __restrict__ marks, const int* __restrict__ ids,
+                                                      int pair_blocks) {
+    if ((int)blockIdx.x < pair_blocks) {
+        const int e = blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= a.n_pairs || dims[4] == 0) return;
+        a.pair_class[e] = ids[dense_index(dense_decode_key(a.class_key_in[e], a.vbits), dims)];
+        return;
+    }
+    const int N = dims[4];
+    const int b = a.vbits;
+    const int voff = 1 << (b - 1);
+    const int nd = 2 * dims[3] + 1, nz = 2 * dims[2] + 1, ny = 2 * dims[1] + 1;
+    for (int i = (blockIdx.x - pair_blocks) * blockDim.x + threadIdx.x; i < N;
+         i += (gridDim.x - pair_blocks) * blockDim.x) {
+        if (!marks[i]) continue;
+        marks[i] = 0;
+        int r = i;
+        const int dl = r % nd - dims[3];
+        r /= nd;
+        const int vz = r % nz - dims[2];
+        r /= nz;
+        const int vy = r % ny - dims[1];
+        const int vx = r / ny - dims[0];
+        const int c = ids[i];
+        a.class_key[c] = ((unsigned long long)(vx + voff) << (2 * b + 8)) |
+                         ((unsigned long long)(vy + voff) << (b + 8)) | ((unsigned long long)(vz + voff) << 8) |
+                         (unsigned long long)(dl + 128);
+        a.class_dl[c] = dl;
+    }
+}
+
+void launch_class_dense(const M2LClassArgs& a, int nwg, int* dims, int* marks, int* ids, int* tsum, int* toff,
+                        hipStream_t s) {
+    if (a.n_pairs <= 0) return;
+    hipLaunchKernelGGL(k_dense_dims, dim3(1), dim3(1024), 0, s, a.wg_range, nwg, dims, a.error);
+    hipLaunchKernelGGL(k_dense_mark, dim3(nblk(a.n_pairs, 256)), dim3(256), 0, s, a.class_key_in, a.n_pairs, a.vbits,
+                       dims, marks);
+    hipLaunchKernelGGL(k_dense_tile_sum, dim3(DENSE_TILES_MAX), dim3(256), 0, s, marks, dims, tsum);
+    hipLaunchKernelGGL(k_dense_tile_scan, dim3(1), dim3(1024), 0, s, tsum, dims, toff, a.num_classes);
+    hipLaunchKernelGGL(k_dense_ids, dim3(DENSE_TILES_MAX), dim3(256), 0, s, marks, dims, toff, ids);
+    const int pb = nblk(a.n_pairs, 256);
+    hipLaunchKernelGGL(k_dense_finish, dim3(pb + 256), dim3(256), 0, s, a, dims, marks, ids, pb);
+    FMMH_HIP(hipGetLastError());
+}
 
 void launch_m2l_keys(const M2LClassArgs& a, hipStream_t s) {
     if (a.n_pairs <= 0) return;

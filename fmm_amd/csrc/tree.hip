@@ -618,7 +618,38 @@ __global__ __launch_bounds__(256) void k_list_sort(const unsigned* __restrict__ 
     }
 }
 
+// Exclusive prefix sums of n ints in one workgroup of 1024 threads (each sums a contiguous run, then the
+// runs' totals are scanned in LDS): one launch, deterministic, for the small arrays of the set-up.
+__global__ __launch_bounds__(1024) void k_exscan_small(const int* __restrict__ in, int* __restrict__ out, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+    int sum = 0;
+    for (int i = b0; i < b1; ++i) sum += in[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive Hillis-Steele scan of the run totals
+        const int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int i = b0; i < b1; ++i) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+}
+
 static inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
+
+void exscan_i32_small(const int* in, int* out, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_exscan_small, dim3(1), dim3(1024), 0, s, in, out, n);
+    FMMH_HIP(hipGetLastError());
+}
 
 void launch_list_csr(const unsigned long long* keys, int n_keys, int num_targets, int* cnt, int* off, unsigned* tmp,
                      void* scan_temp, size_t scan_temp_bytes, int* src, unsigned long long* keys_out,
@@ -626,7 +657,8 @@ void launch_list_csr(const unsigned long long* keys, int n_keys, int num_targets
     // cnt: num_targets + 1 zeros (left zero); off: num_targets + 1
     if (n_keys > 0)
         hipLaunchKernelGGL(k_list_count, dim3(blocks_for(n_keys, 256)), dim3(256), 0, s, keys, n_keys, cnt);
-    exscan_i32(scan_temp, scan_temp_bytes, cnt, off, num_targets + 1, s);
+    if (num_targets + 1 <= EXSCAN_SMALL_MAX) exscan_i32_small(cnt, off, num_targets + 1, s);
+    else exscan_i32(scan_temp, scan_temp_bytes, cnt, off, num_targets + 1, s);
     if (n_keys > 0) {
         hipLaunchKernelGGL(k_list_scatter, dim3(blocks_for(n_keys, 256)), dim3(256), 0, s, keys, n_keys, off, cnt,
                            tmp);

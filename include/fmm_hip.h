@@ -269,9 +269,12 @@ enum fmmh_option {
     FMMH_OPT_TIMING = 1,          /* value != 0: record per-phase HIP events */
     FMMH_OPT_LIST_CAPACITY = 2,   /* scale of the initial interaction-list buffers (default 1; they grow on overflow) */
     FMMH_OPT_M2L_MFMA = 3,        /* M2L engine.  0: per-target contraction on the vector ALUs;
-                                     1 (default): per-target FP64 MFMA over sets from 16 sets on (orders
-                                     <= 8), else the class-grouped FP64 MFMA GEMM (orders <= 10);
-                                     2: per-target MFMA for any set count; 3: class-grouped always */
+                                     1 (default): per-target FP64 MFMA over sets from 6 sets on (orders
+                                     <= 8), else the class-grouped FP64 MFMA GEMM (orders <= 10) --
+                                     accumulated per block of 64 targets on the chip for FP64 at
+                                     orders <= 6 without learned orders, else through per-pair slots;
+                                     2: per-target MFMA for any set count; 3: class-grouped always;
+                                     4: class-grouped with per-pair slots always */
     FMMH_OPT_REUSE_TREE = 4,      /* value != 0: keep the last field_fmm octree, interaction lists and M2L
                                      class tensors, and reuse them while the coordinates (compared bitwise
                                      on the device), n, ncrit and theta are unchanged -- e.g. the induced-
