@@ -377,7 +377,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
                     help="default: mixed for config 3 (its workload), fp64 otherwise")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--m2l-mfma", type=int, default=None, choices=[0, 1, 2, 3],
+    ap.add_argument("--m2l-mfma", type=int, default=None, choices=[0, 1, 2, 3, 4],
                     help="M2L engine option (FMMH_OPT_M2L_MFMA); default: the library's choice")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

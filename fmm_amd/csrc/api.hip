@@ -1180,7 +1180,7 @@ void build_tree(fmmh_context* c, const double* X, int64_t ldx, int n, int ncrit,
     t.info = c->buf<int>("t_info", 64);
     t.level_fill = c->buf<int>("t_level_fill", KEY_LEVELS + 3);
     FMMH_HIP(hipMemsetAsync(t.info, 0, 64 * sizeof(int), s));
-    launch_centroid(X, ldx, n, t, s);
+    launch_centroid(X, ldx, n, t, c->buf<char>("t_centroid_work", centroid_work_bytes(n)), s);
     c->mark("t_centroid");
     launch_keys(X, ldx, n, t, keys_in, idx_in, s);
     c->mark("t_keys");
@@ -1448,7 +1448,7 @@ void m2l_classes_end(fmmh_context* c, int p, bool pending, M2LClassArgs& ca) {
             c->sync();
         }
         ncls = c->h_pinned[300];
-        ca.Tc = c->buf<double>("m_tc", (size_t)ncls * ncomp(2 * p));
+        ca.Tc = c->buf<double>("m_tc", (size_t)ncls * ncomp(2 * p) + M2LB_READ_SLACK);
         launch_m2l_class_tensors(ca, std::min(ncls, 4096), c->stream);
     }
     c->stats_classes = ncls;
@@ -1468,7 +1468,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         m2l_classes_end(c, p, pending, ca);
     }
     const int ncls = (int)c->cache_classes;
-    double* Tc = np > 0 ? c->buf<double>("m_tc", (size_t)ncls * nT) : nullptr;
+    double* Tc = np > 0 ? c->buf<double>("m_tc", (size_t)ncls * nT + M2LB_READ_SLACK) : nullptr;
     c->stats_classes = ncls;
     c->mark("m2l_classes");
     // auto: the per-target MFMA form from M2L_MFMA_MIN_SETS sets on (a 16-set tile, padded, costs about
@@ -1510,7 +1510,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         }
         // two workgroups per compute unit (LDS: the block's local expansions plus the staged tile)
         const int grid = std::max(1, 2 * c->num_cu);
-        double* CMs = c->buf<double>("m_cms", node_stride * nsets);
+        double* CMs = c->buf<double>("m_cms", node_stride * nsets + M2LB_READ_SLACK);
         launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
         M2LBRunArgs ra{ba.toff, ba.pbase, ba.rec, nbm, ba.ntg, ba.tperm, ba.tinv, nn, nullptr, Tc, tb.gemm_fidx,
                        tb.gemm_tdeg, c->buf<double>("b_partial", (size_t)(nbm + grid) * nm * M2LB_BT), grid};

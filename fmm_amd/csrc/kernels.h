@@ -104,7 +104,9 @@ struct TreeDev {
     int* level_fill;
 };
 
-void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, hipStream_t s);
+// work: centroid_work_bytes(n) bytes of device memory.
+size_t centroid_work_bytes(int n);
+void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, void* work, hipStream_t s);
 void launch_keys(const double* X, int64_t ldx, int n, TreeDev& t, unsigned long long* keys_out, int* idx_out,
                  hipStream_t s);
 void launch_site_levels(int n, int ncrit, TreeDev& t, hipStream_t s);
@@ -317,6 +319,7 @@ constexpr int M2LB_PMAX = 6;
 constexpr int M2LB_BT = 64;              // targets per block (one bit each in a class mask)
 constexpr int M2LB_REC = 24;             // ints per tile record: class, dl, count, 0, 16 sources, 16 bytes
 constexpr int M2LB_MAX_CLASSES = 4000;   // class masks and offsets in 64 KB of LDS
+constexpr int M2LB_READ_SLACK = 64;      // doubles readable past the scaled sources and class tensors
 struct M2LBlockArgs {
     const int* m2l_off;                  // CSR of the far pairs
     const int* m2l_src;
@@ -341,8 +344,8 @@ void launch_m2lb_tiles(const M2LBlockArgs& a, hipStream_t s);
 struct M2LBRunArgs {
     const int* toff; const int* pbase; const int* rec; int nbm;
     const int* ntg; const int* tperm; const int* tinv; int nn;
-    const double* CMs;                   // scaled source expansions of one set, [node][nm]
-    const double* Tc;                    // class tensors
+    const double* CMs;                   // scaled source expansions of one set, [node][nm] (+ slack)
+    const double* Tc;                    // class tensors (+ M2LB_READ_SLACK doubles)
     const int* fidx; const int* tdeg;    // the class-grouped GEMM's fragment tables
     double* partial;                     // [nbm + grid][nm][M2LB_BT]
     int grid;                            // workgroups of the run kernel

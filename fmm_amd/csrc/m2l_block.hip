@@ -250,11 +250,10 @@ struct M2LBShape {
     static constexpr int NT = ncomp(2 * P);
     static constexpr int MT = (NM + 15) / 16;       // 16-row target tiles
     static constexpr int KS = (NM + 3) / 4;         // 4-deep source steps
-    static constexpr int NW = (MT + RT - 1) / RT;   // wavefronts
+    static constexpr int NW = (MT + RT - 1) / RT;   // wavefronts, RT row tiles each
     static constexpr int NTHR = 64 * NW;
-    static constexpr int TPC = NTHR / 16;           // gather threads per column
-    static constexpr int PB = (NM + TPC - 1) / TPC; // source coefficients per gather thread
-    static constexpr int PT = (NT + NTHR - 1) / NTHR;
+    static constexpr int PB = (NM + 3) / 4;         // source coefficients per lane (4 lanes per column)
+    static constexpr int PT = (NT + 63) / 64;       // class tensor entries per lane
     static constexpr int BLD = (4 * KS) | 1;        // odd LDS row stride of the source tile (zero pad to 4 KS)
 };
 
@@ -288,30 +287,47 @@ __device__ __forceinline__ void cursor_next(const M2LBRunArgs& a, TileCursor& u,
     }
 }
 
+// Each wavefront of a workgroup owns RT row tiles of the block's local expansions and walks the
+// workgroup's share of the tiles on its own: it gathers and stages every tile in its own LDS slot (the
+// wavefronts of a workgroup read the same sources, mostly from the CU's cache) and never waits for the
+// others -- no barriers.  Per tile: the fragments are read from LDS in groups of KG source steps, one
+// group ahead of the MFMAs; the next tile is staged once the last group is read (LDS operations of a
+// wavefront complete in order); the tile after it is gathered into registers; and the previous tile's
+// results are added to the LDS rows while this tile's MFMAs run.  A wavefront issues MFMAs in order
+// and the matrix core takes one 16x16x4 FP64 MFMA per 64 cycles, so the other work is interleaved
+// with the MFMAs (scheduling groups) rather than placed between them; the padded buffers below keep
+// that work free of branches.
 template <int P, int RT>
 __global__ __launch_bounds__((M2LBShape<P, RT>::NTHR)) void k_m2lb_run(M2LBRunArgs a) {
     using S = M2LBShape<P, RT>;
-    constexpr int NM = S::NM, NT = S::NT, MT = S::MT, KS = S::KS, NTHR = S::NTHR;
-    constexpr int TPC = S::TPC, PB = S::PB, PT = S::PT, BLD = S::BLD, BT = M2LB_BT;
-    constexpr int LLD = BT + 1;        // odd row stride: the 4 row groups of a fragment hit other banks
-    __shared__ double Lacc[NM * LLD];  // [row][target in block]
-    __shared__ double Bs[16 * BLD];
-    __shared__ double Ts[NT + 1];      // Ts[NT] = 0: the fragment lanes outside the Hankel matrix
-    __shared__ int tcol[16];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int NM = S::NM, NT = S::NT, MT = S::MT, KS = S::KS, NW = S::NW;
+    constexpr int PB = S::PB, PT = S::PT, BLD = S::BLD, BT = M2LB_BT;
+    constexpr int LLD = BT + 1;        // odd row stride (4 row groups of a fragment hit other banks);
+                                       // column BT takes the results of a tile's unused columns
+    constexpr int NR = 16 * RT * NW;   // rows of every wavefront's row tiles (those past NM are dropped)
+    constexpr int KG = 4;              // source steps per fragment group (RT + 1 LDS reads each)
+    constexpr int NKG = (KS + KG - 1) / KG;
+    constexpr int TZ = 64 * PT;        // zero slot of the class tensor (entries past NT take garbage)
+    __shared__ double Lacc[NR * LLD];  // [row][target in block]
+    __shared__ double Bsh[NW][16 * BLD];
+    __shared__ double Tsh[NW][TZ + 1];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c16 = lane & 15, g4 = lane >> 4;
     const long long T = a.toff[a.nbm];
     const long long G = gridDim.x, g = share_index(blockIdx.x, gridDim.x);
     const int t0 = (int)(g * T / G), t1 = (int)((g + 1) * T / G);
     if (t0 >= t1) return;
-    for (int x = tid; x < NM * LLD; x += NTHR) Lacc[x] = 0.0;
-    for (int x = tid; x < 16 * BLD; x += NTHR) Bs[x] = 0.0;   // the pad past NM stays zero
-    if (tid == 0) Ts[NT] = 0.0;
-    // fragment indices of this wavefront's row tiles (NT outside the matrix, and for a wavefront's row
-    // tiles past the last: their MFMAs then add zeros), and |t| of each lane's A row
+    double* __restrict__ Bs = Bsh[w];
+    double* __restrict__ Ts = Tsh[w];
+    const int row0 = 16 * RT * w, row1 = min(NR, 16 * RT * (w + 1));   // this wavefront's rows
+    for (int x = lane; x < (row1 - row0) * LLD; x += 64) Lacc[row0 * LLD + x] = 0.0;
+    for (int x = lane; x < 16 * BLD; x += 64) Bs[x] = 0.0;
+    if (lane == 0) Ts[TZ] = 0.0;
+    // fragment indices of this wavefront's row tiles (TZ outside the matrix, and for row tiles past the
+    // last: their MFMAs then add zeros), and |t| of the accumulator rows 4 q + l / 16
     int fi[RT][KS];
-    int deg[RT];
+    int dq[RT][4];
 #pragma unroll
     for (int mi = 0; mi < RT; ++mi) {
         const int mt = min(RT * w + mi, MT - 1);
@@ -319,128 +335,146 @@ __global__ __launch_bounds__((M2LBShape<P, RT>::NTHR)) void k_m2lb_run(M2LBRunAr
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int f = a.fidx[(mt * KS + ks) * 64 + lane];
-            fi[mi][ks] = (live && f >= 0) ? f : NT;
+            fi[mi][ks] = (live && f >= 0) ? f : TZ;
         }
-        deg[mi] = a.tdeg[mt * 16 + c16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dq[mi][q] = a.tdeg[mt * 16 + 4 * q + g4];
     }
-    // gather roles: column gcol of the source tile, coefficients gsub + TPC i
-    const int gcol = tid / TPC, gsub = tid - gcol * TPC;
-    TileCursor cur, nx1, nx2;
-    cursor_seek(a, cur, t0);
-    nx1 = cur;
-    cursor_next(a, nx1, t1);
-    nx2 = nx1;
-    cursor_next(a, nx2, t1);
-    // records: class and level difference (uniform), this thread's column source and target
-    auto rec_of = [&](const TileCursor& u) { return a.rec + (int64_t)(u.rbase + (u.t - u.tb)) * M2LB_REC; };
-    int cls0, dl0, src0, tl0;   // tile t
-    int cls1, dl1, src1, tl1;   // tile t + 1
-    {
-        const int* r = rec_of(cur);
-        cls0 = r[0]; dl0 = r[1]; src0 = r[4 + gcol]; tl0 = reinterpret_cast<const unsigned char*>(r + 20)[gcol];
-    }
-    cls1 = dl1 = 0; src1 = -1; tl1 = 0;
-    if (nx1.t < t1) {
-        const int* r = rec_of(nx1);
-        cls1 = r[0]; dl1 = r[1]; src1 = r[4 + gcol]; tl1 = reinterpret_cast<const unsigned char*>(r + 20)[gcol];
-    }
+    // gather roles: column gcol of the source tile, coefficients gsub + 4 i (entries past NM read the
+    // next node's data: finite, and met only by zero fragment entries)
+    const int gcol = lane >> 2, gsub = lane & 3;
+    struct Rec {
+        int cls, dl, src, tl;   // class and level difference; source of column gcol; target of column c16
+    };
+    auto load_rec = [&](const TileCursor& u, Rec& r) {
+        const int* p = a.rec + (int64_t)(u.rbase + (min(u.t, t1 - 1) - u.tb)) * M2LB_REC;
+        r.cls = p[0];
+        r.dl = p[1];
+        r.src = max(p[4 + gcol], 0);   // an unused column reads node 0 (its results go to column BT)
+        r.tl = p[4 + c16] >= 0 ? (int)reinterpret_cast<const unsigned char*>(p + 20)[c16] : BT;
+    };
     double bv[PB], tv[PT];
+    auto gather = [&](const Rec& r) {
 #pragma unroll
-    for (int i = 0; i < PB; ++i) {
-        const int s = gsub + TPC * i;
-        bv[i] = (src0 >= 0 && s < NM) ? a.CMs[(int64_t)src0 * NM + s] : 0.0;
-    }
+        for (int i = 0; i < PB; ++i) bv[i] = a.CMs[(int64_t)r.src * NM + gsub + 4 * i];
 #pragma unroll
-    for (int i = 0; i < PT; ++i) {
-        const int x = tid + NTHR * i;
-        tv[i] = x < NT ? a.Tc[(int64_t)cls0 * NT + x] : 0.0;
-    }
-    for (int t = t0; t < t1; ++t) {
-        // stage tile t in LDS
+        for (int i = 0; i < PT; ++i) tv[i] = a.Tc[(int64_t)r.cls * NT + lane + 64 * i];
+    };
+    auto stage = [&]() {
 #pragma unroll
-        for (int i = 0; i < PB; ++i) {
-            const int s = gsub + TPC * i;
-            if (s < NM) Bs[gcol * BLD + s] = bv[i];
-        }
+        for (int i = 0; i < PB; ++i) Bs[gcol * BLD + gsub + 4 * i] = bv[i];
 #pragma unroll
-        for (int i = 0; i < PT; ++i) {
-            const int x = tid + NTHR * i;
-            if (x < NT) Ts[x] = tv[i];
-        }
-        if (gsub == 0) tcol[gcol] = src0 >= 0 ? tl0 : -1;
-        const int dl = dl0;
-        __syncthreads();
-        // gathers of tile t + 1, record of tile t + 2
-        if (t + 1 < t1) {
-#pragma unroll
-            for (int i = 0; i < PB; ++i) {
-                const int s = gsub + TPC * i;
-                bv[i] = (src1 >= 0 && s < NM) ? a.CMs[(int64_t)src1 * NM + s] : 0.0;
-            }
-#pragma unroll
-            for (int i = 0; i < PT; ++i) {
-                const int x = tid + NTHR * i;
-                tv[i] = x < NT ? a.Tc[(int64_t)cls1 * NT + x] : 0.0;
-            }
-        }
-        int cls2 = 0, dl2 = 0, src2 = -1, tl2 = 0;
-        if (nx2.t < t1) {
-            const int* r = rec_of(nx2);
-            cls2 = r[0]; dl2 = r[1]; src2 = r[4 + gcol];
-            tl2 = reinterpret_cast<const unsigned char*>(r + 20)[gcol];
-        }
-        // tile t on the matrix cores
-        double rs[RT];
+        for (int i = 0; i < PT; ++i) Ts[lane + 64 * i] = tv[i];
+    };
+    // accumulator sets alternate: tile t's results (set X) are added to the block's local expansions
+    // while tile t + 1's MFMAs run (or at once where the share leaves the block)
+    v4d accs[2][RT];
+    int tls[2] = {BT, BT}, dls[2] = {0, 0};
+    auto accumulate = [&](int Y) {
+        // level difference of the class: rows scaled by 2^{-dl |t|} (dl > 0) or 2^{-dl (|t| + 1)}
+        // (exact powers of two), then accumulator register q of lane l -> D[4 q + l / 16][l % 16] added
+        // to the tile's target (all loads first, then all stores)
+        const int tl = tls[Y], dl = dls[Y];
+        double old[RT][4];
 #pragma unroll
         for (int mi = 0; mi < RT; ++mi)
-            rs[mi] = dl > 0 ? ldexp(1.0, -dl * deg[mi]) : (dl < 0 ? ldexp(1.0, -dl * (deg[mi] + 1)) : 1.0);
-        v4d acc[RT];
 #pragma unroll
-        for (int mi = 0; mi < RT; ++mi) acc[mi] = v4d{0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < 4; ++q) old[mi][q] = Lacc[((RT * w + mi) * 16 + 4 * q + g4) * LLD + tl];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const double bf = Bs[c16 * BLD + ks * 4 + g4];
+        for (int mi = 0; mi < RT; ++mi)
 #pragma unroll
-            for (int mi = 0; mi < RT; ++mi) {
-                const double av = Ts[fi[mi][ks]] * rs[mi];
-                acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bf, acc[mi], 0, 0, 0);
+            for (int q = 0; q < 4; ++q) {
+                const int sh = dl > 0 ? -dl * dq[mi][q] : -dl * (dq[mi][q] + 1);   // 0 for dl = 0
+                Lacc[((RT * w + mi) * 16 + 4 * q + g4) * LLD + tl] = old[mi][q] + ldexp(accs[Y][mi][q], sh);
+            }
+        tls[Y] = BT;
+    };
+    // pipeline: tile t in LDS (read as fragments), tile t + 1 gathered in registers (staged once tile t's
+    // fragments are read), records of tiles t + 1 and t + 2 in two alternating sets (the loop is
+    // unrolled by two, so no register copies wait on loads in flight)
+    TileCursor cur, n1, n2, n3;
+    cursor_seek(a, cur, t0);
+    n1 = cur;
+    cursor_next(a, n1, t1);
+    n2 = n1;
+    cursor_next(a, n2, t1);
+    n3 = n2;
+    cursor_next(a, n3, t1);
+    Rec r0, rA, rB;
+    load_rec(cur, r0);
+    load_rec(n1, rA);
+    load_rec(n2, rB);
+    gather(r0);
+    stage();
+    gather(rA);
+    int tl_cur = r0.tl, dl_cur = r0.dl;
+    auto step = [&](int t, int X, Rec& rnx, Rec& rga) {
+        // rnx: record of tile t + 1 (its target and level difference), reloaded with tile t + 3's;
+        // rga: record of tile t + 2 (gathered during this step)
+        const int tl_nx = rnx.tl, dl_nx = rnx.dl;
+#pragma unroll
+        for (int mi = 0; mi < RT; ++mi) accs[X][mi] = v4d{0.0, 0.0, 0.0, 0.0};
+        double fa[2][KG][RT], fb[2][KG];
+        auto load_group = [&](int j, int slot) {
+#pragma unroll
+            for (int k = 0; k < KG; ++k) {
+                const int ks = j * KG + k;
+                if (ks < KS) {
+                    fb[slot][k] = Bs[c16 * BLD + 4 * ks + g4];
+#pragma unroll
+                    for (int mi = 0; mi < RT; ++mi) fa[slot][k][mi] = Ts[fi[mi][ks]];
+                }
+            }
+        };
+        load_group(0, 0);
+#pragma unroll
+        for (int j = 0; j < NKG; ++j) {
+            if (j + 1 < NKG) load_group(j + 1, (j + 1) & 1);
+            if (j == 0) accumulate(1 - X);   // tile t - 1
+            if (j + 2 == NKG || NKG == 1) {  // every fragment of tile t is read: stage t + 1, gather t + 2
+                stage();
+                gather(rga);
+                load_rec(n3, rnx);
+            }
+#pragma unroll
+            for (int k = 0; k < KG; ++k) {
+                if (j * KG + k < KS) {
+#pragma unroll
+                    for (int mi = 0; mi < RT; ++mi)
+                        accs[X][mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[j & 1][k][mi], fb[j & 1][k],
+                                                                           accs[X][mi], 0, 0, 0);
+                }
+            }
+            // one MFMA, then up to 3 LDS and 2 other vector operations, in turn
+#pragma unroll
+            for (int k = 0; k < KG * RT; ++k) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x080, 3, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
             }
         }
-        // accumulator register q of lane l: D[4 q + l / 16][l % 16]
-        // (all loads first, then all stores: the 8 read-modify-writes overlap)
-        const int tl = tcol[c16];
-        if (tl >= 0) {
-            double old[RT][4];
-#pragma unroll
-            for (int mi = 0; mi < RT; ++mi)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int row = (RT * w + mi) * 16 + 4 * q + g4;
-                    old[mi][q] = row < NM ? Lacc[row * LLD + tl] : 0.0;
-                }
-#pragma unroll
-            for (int mi = 0; mi < RT; ++mi)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int row = (RT * w + mi) * 16 + 4 * q + g4;
-                    if (row < NM) Lacc[row * LLD + tl] = old[mi][q] + acc[mi][q];
-                }
-        }
-        __syncthreads();
+        tls[X] = tl_cur;
+        dls[X] = dl_cur;
         if (t + 1 == cur.te || t + 1 == t1) {
-            // the share leaves this block: its partial slot is block + workgroup
+            // the share leaves this block: its partial slot is block + share (this wavefront's rows)
+            accumulate(X);
             double* out = a.partial + (int64_t)(cur.b + g) * NM * BT;
-            for (int x = tid; x < NM * BT; x += NTHR) {
-                const int row = x / BT, k = x - row * BT;
-                out[x] = Lacc[row * LLD + k];
+            for (int x = lane; x < (min(row1, NM) - row0) * BT; x += 64) {
+                const int row = row0 + x / BT, k = x % BT;
+                out[row * BT + k] = Lacc[row * LLD + k];
                 Lacc[row * LLD + k] = 0.0;
             }
         }
-        cur = nx1;
-        nx1 = nx2;
-        cursor_next(a, nx2, t1);
-        cls0 = cls1; dl0 = dl1; src0 = src1; tl0 = tl1;
-        cls1 = cls2; dl1 = dl2; src1 = src2; tl1 = tl2;
+        tl_cur = tl_nx;
+        dl_cur = dl_nx;
+        cur = n1;
+        n1 = n2;
+        n2 = n3;
+        cursor_next(a, n3, t1);
+    };
+    for (int t = t0; t < t1; t += 2) {
+        step(t, 0, rA, rB);
+        if (t + 1 < t1) step(t + 1, 1, rB, rA);
     }
 }
 
@@ -490,7 +524,7 @@ void launch_m2lb_tiles(const M2LBlockArgs& a, hipStream_t s) {
 
 template <int P>
 static void run_p(const M2LBRunArgs& a, const double* r, double* LE, hipStream_t s) {
-    constexpr int RT = 2;
+    constexpr int RT = 3;
     using S = M2LBShape<P, RT>;
     hipLaunchKernelGGL((k_m2lb_run<P, RT>), dim3(a.grid), dim3(S::NTHR), 0, s, a);
     hipLaunchKernelGGL((k_m2lb_finish<S::NM>), dim3(nblk((long long)a.nbm * S::NM * M2LB_BT, FN_T)), dim3(FN_T), 0, s,

@@ -20,6 +20,8 @@
 // All centre and MAC arithmetic is FP64 with contraction disabled, using the
 // same operation order as the reference (compiled without FMA), and the MAC
 // distance reproduces flang's NORM2 (running maximum, scaled sum of squares).
+#include <climits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -64,14 +66,21 @@ constexpr int CB_MIN_PROGRESS = 64;
 constexpr int CB_BURST = 4096;
 static_assert(CB_BURST <= CB_LDS, "a burst is staged in the block buffer");
 
-__global__ __launch_bounds__(CB_THREADS) void k_centroid(const double* __restrict__ X, int64_t ldx, int n,
-                                                         double* __restrict__ root) {
+struct CentroidLds {
+    double xs[CB_LDS];
+    unsigned long long wtot[CB_THREADS / 64];
+    int first_bad;
+    double s_next;
+};
+
+// The left-to-right sum of col[pos, n) continued from s, by the block method (every thread returns the
+// same value; one workgroup of CB_THREADS).
+__device__ double centroid_range(const double* __restrict__ col, int pos, int n, double s, CentroidLds& L) {
 #pragma clang fp contract(off)
-    __shared__ double xs[CB_LDS];
-    __shared__ unsigned long long wtot[CB_THREADS / 64];
-    __shared__ int first_bad;
-    __shared__ double s_next;
-    const double* __restrict__ col = X + (int64_t)blockIdx.x * ldx;
+    double* xs = L.xs;
+    unsigned long long* wtot = L.wtot;
+    int& first_bad = L.first_bad;
+    double& s_next = L.s_next;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     double pre[CB_PER];
     int pre_pos = -1;
@@ -83,8 +92,6 @@ __global__ __launch_bounds__(CB_THREADS) void k_centroid(const double* __restric
         }
         pre_pos = p0;
     };
-    double s = 0.0;   // uniform across the workgroup
-    int pos = 0;
     while (pos < n) {
         if (!(fabs(s) >= 0x1p-1022) || !isfinite(s)) {   // 0, subnormal or non-finite: no binade unit
             s = s + col[pos];
@@ -145,7 +152,10 @@ __global__ __launch_bounds__(CB_THREADS) void k_centroid(const double* __restric
                 if (((badmask >> q) & 1u) || !(a > (1ll << 52) && a < (1ll << 53))) mine = i;
             }
         }
-        if (mine < len) atomicMin(&first_bad, mine);
+        // (a wave minimum first: after a binade change every later thread has a bad addend)
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) mine = min(mine, __shfl_xor(mine, d));
+        if (lane == 0 && mine < len) atomicMin(&first_bad, mine);
         __syncthreads();
         const int f = first_bad;
         // exact partial sum just before f (or at the end of the block)
@@ -184,7 +194,264 @@ __global__ __launch_bounds__(CB_THREADS) void k_centroid(const double* __restric
         }
         __syncthreads();   // xs, s_next and first_bad are rewritten by the next block
     }
-    if (tid == 0) root[blockIdx.x] = s / (double)n;
+    return s;
+}
+
+__global__ __launch_bounds__(CB_THREADS) void k_centroid(const double* __restrict__ X, int64_t ldx, int n,
+                                                         double* __restrict__ root) {
+    __shared__ CentroidLds L;
+    const double s = centroid_range(X + (int64_t)blockIdx.x * ldx, 0, n, 0.0, L);
+    if (threadIdx.x == 0) root[blockIdx.x] = s / (double)n;
+}
+
+// Long inputs: the same sum with most of it in parallel.  The addends are cut into chunks of
+// CS_CHUNK; a first pass sums each chunk in any order (k_centroid_csum), so that the running sum at
+// the start of chunk c is known approximately, P_c.  A second pass (k_centroid_spec) forms the
+// chunk's approximate running sums P_k (a parallel scan) and from them the binades the exact sum will
+// most likely be in: one binade for the whole chunk, or one before and one after a single addend kb
+// where the running sum changes binade.  For each part it forms, exactly in int64, the partial sums
+// J_k of rint(x / u) in the part's unit u = 2^{e-52}, with their least and greatest values; addends
+// that are ties, non-finite or too large, or a second change of binade, mark the chunk.  The chain
+// (k_centroid_chain, one workgroup per axis) walks the chunks with the exact running sum s: when s
+// lies in a part's binade, s = S0 u, and S0 + J_k stays strictly inside (2^52, 2^53) in magnitude for
+// every k, every addition of the part was exact in the sense of the block method above and the part
+// ends at (S0 + J_last) u; the addend kb between the parts is added in FP64 like any other.  Whatever
+// fails a check -- the first chunk, chunks with several changes of binade, a wrong guess -- is summed
+// by the block method from s.  The result is the sequential sum bit for bit.
+constexpr int CS_CHUNK = 8192;
+constexpr int CS_T = 256;
+constexpr int CS_PER = CS_CHUNK / CS_T;   // consecutive addends per thread in the second pass
+constexpr int CS_LDS_CHUNKS = 256;
+struct CentroidPart {
+    long long total, jmin, jmax;   // J over the part (jmin > jmax: empty part)
+    int e;
+};
+struct CentroidChunk {
+    CentroidPart A, B;             // before and after the change of binade (B empty without one)
+    int kb;                        // chunk index of the addend between the parts, -1: none
+    int bad;
+};
+
+__global__ __launch_bounds__(CS_T) void k_centroid_csum(const double* __restrict__ X, int64_t ldx, int n, int nch,
+                                                         double* __restrict__ csum) {
+    __shared__ double red[CS_T / 64];
+    const double* __restrict__ col = X + (int64_t)blockIdx.y * ldx;
+    const int c = blockIdx.x, tid = threadIdx.x;
+    double x[CS_PER];
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {   // all loads in flight together
+        const int k = c * CS_CHUNK + q * CS_T + tid;
+        x[q] = k < n ? col[k] : 0.0;
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) v += x[q];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) csum[blockIdx.y * nch + c] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__device__ __forceinline__ int binade_of(double p) {   // ilogb of a normal finite value, else a sentinel
+    return (fabs(p) >= 0x1p-1022 && isfinite(p)) ? ilogb(p) : -100000;
+}
+
+// Block-wide sums and scans over CS_T threads (4 wavefronts).
+template <class V>
+__device__ __forceinline__ V cs_exscan(V v, V& total, V* wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    V incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const V y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    V off = incl - v;
+    total = V(0);
+    for (int k = 0; k < CS_T / 64; ++k) {
+        if (k < w) off += wsum[k];
+        total += wsum[k];
+    }
+    return off;
+}
+template <bool MAX>
+__device__ __forceinline__ long long cs_minmax(long long v, long long* wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const long long y = __shfl_xor(v, d);
+        v = MAX ? max(v, y) : min(v, y);
+    }
+    __syncthreads();
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    long long r = wsum[0];
+    for (int k = 1; k < CS_T / 64; ++k) r = MAX ? max(r, wsum[k]) : min(r, wsum[k]);
+    return r;
+}
+
+__global__ __launch_bounds__(CS_T) void k_centroid_spec(const double* __restrict__ X, int64_t ldx, int n, int nch,
+                                                         const double* __restrict__ csum,
+                                                         CentroidChunk* __restrict__ spec) {
+    constexpr int LD = CS_PER + 1;   // padded rows: thread t reads its CS_PER addends without conflicts
+    __shared__ double xs[CS_T * LD];
+    __shared__ double wd[CS_T / 64];
+    __shared__ long long wl[CS_T / 64];
+    __shared__ int wi[CS_T / 64];
+    const double* __restrict__ col = X + (int64_t)blockIdx.y * ldx;
+    const int c = blockIdx.x, tid = threadIdx.x;
+    CentroidChunk* out = spec + blockIdx.y * nch + c;
+    const int base = c * CS_CHUNK, len = min(CS_CHUNK, n - base);
+    // the chunk in LDS (coalesced loads, all in flight together)
+    double x[CS_PER];
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {
+        const int k = q * CS_T + tid;
+        x[q] = k < len ? col[base + k] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {
+        const int k = q * CS_T + tid;
+        xs[(k / CS_PER) * LD + k % CS_PER] = x[q];
+    }
+    // approximate running sum at the chunk start
+    double ps = 0.0;
+    for (int k = tid; k < c; k += CS_T) ps += csum[blockIdx.y * nch + k];
+    double pst;
+    cs_exscan(ps, pst, wd);
+    // approximate running sums inside the chunk: this thread's addends k0 .. k0 + CS_PER - 1
+    const int k0 = tid * CS_PER;
+    double loc = 0.0;
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {
+        x[q] = xs[tid * LD + q];
+        loc += x[q];
+    }
+    double dummy;
+    const double pre = pst + cs_exscan(loc, dummy, wd);
+    const int e0 = binade_of(pst);
+    // changes of binade: count and the first position
+    int nchg = 0, kb = CS_CHUNK;
+    {
+        double p = pre;
+        int bprev = k0 == 0 ? e0 : binade_of(p);
+#pragma unroll
+        for (int q = 0; q < CS_PER; ++q) {
+            if (k0 + q >= len) break;
+            p += x[q];
+            const int b = binade_of(p);
+            if (b != bprev) {
+                ++nchg;
+                kb = min(kb, k0 + q);
+            }
+            bprev = b;
+        }
+    }
+    int all_chg;
+    cs_exscan(nchg, all_chg, wi);
+    kb = (int)cs_minmax<false>((long long)kb, wl);
+    // the binade after the change: approximate running sum at kb
+    __shared__ int s_eb;
+    if (tid == 0) s_eb = 0;
+    __syncthreads();
+    if (kb >= k0 && kb < k0 + CS_PER) {
+        double p = pre;
+        for (int q = 0; q <= kb - k0; ++q) p += x[q];
+        s_eb = binade_of(p);
+    }
+    __syncthreads();
+    const int eA = e0, eB = s_eb;
+    int bad = (c == 0 || all_chg > 1 || e0 == -100000 || (all_chg == 1 && eB == -100000)) ? 1 : 0;
+    if (!bad) {
+        // exact increments: part A before kb in unit 2^{eA-52}, part B after it in unit 2^{eB-52}
+        constexpr long long LIM = 1ll << 54;   // |J| beyond this cannot stay inside the binade
+        long long ra = 0, rb = 0;
+        long long amin = LLONG_MAX, amax = LLONG_MIN, bmin = LLONG_MAX, bmax = LLONG_MIN;
+#pragma unroll
+        for (int q = 0; q < CS_PER; ++q) {
+            const int k = k0 + q;
+            if (k >= len || k == kb) continue;
+            const bool partA = k < kb;
+            const double y = ldexp(x[q], 52 - (partA ? eA : eB));
+            const double r = rint(y);
+            if (!(fabs(y) < 0x1p53) || fabs(y - r) == 0.5) {
+                bad = 1;
+                continue;
+            }
+            if (partA) {
+                ra += (long long)r;
+                bad |= ra > LIM || ra < -LIM;
+                amin = min(amin, ra);
+                amax = max(amax, ra);
+            } else {
+                rb += (long long)r;
+                bad |= rb > LIM || rb < -LIM;
+                bmin = min(bmin, rb);
+                bmax = max(bmax, rb);
+            }
+        }
+        long long ta, tb;
+        const long long oa = cs_exscan(ra, ta, wl), ob = cs_exscan(rb, tb, wl);
+        const long long gamin = cs_minmax<false>(amin == LLONG_MAX ? LLONG_MAX : oa + amin, wl);
+        const long long gamax = cs_minmax<true>(amax == LLONG_MIN ? LLONG_MIN : oa + amax, wl);
+        const long long gbmin = cs_minmax<false>(bmin == LLONG_MAX ? LLONG_MAX : ob + bmin, wl);
+        const long long gbmax = cs_minmax<true>(bmax == LLONG_MIN ? LLONG_MIN : ob + bmax, wl);
+        int anybad;
+        cs_exscan(bad, anybad, wi);
+        if (tid == 0) {
+            CentroidChunk r;
+            r.A = CentroidPart{ta, gamin, gamax, eA};
+            r.B = CentroidPart{tb, gbmin, gbmax, eB};
+            r.kb = all_chg == 1 ? kb : -1;
+            r.bad = anybad ? 1 : 0;
+            *out = r;
+        }
+    } else if (tid == 0) {
+        out->bad = 1;
+    }
+}
+
+// One part in its binade, or false: s = S0 u exactly and S0 + J_k strictly inside the binade.
+__device__ __forceinline__ bool centroid_part(double& s, const CentroidPart& p) {
+    if (p.jmin > p.jmax) return true;   // empty part
+    if (!(fabs(s) >= 0x1p-1022) || !isfinite(s) || ilogb(s) != p.e) return false;
+    const long long S0 = (long long)ldexp(s, 52 - p.e);
+    const long long lo = S0 + p.jmin, hi = S0 + p.jmax;
+    const bool ok = s > 0.0 ? (lo > (1ll << 52) && hi < (1ll << 53)) : (hi < -(1ll << 52) && lo > -(1ll << 53));
+    if (ok) s = ldexp((double)(S0 + p.total), p.e - 52);
+    return ok;
+}
+
+__global__ __launch_bounds__(CB_THREADS) void k_centroid_chain(const double* __restrict__ X, int64_t ldx, int n,
+                                                                int nch, const CentroidChunk* __restrict__ spec,
+                                                                double* __restrict__ root) {
+#pragma clang fp contract(off)
+    __shared__ CentroidLds L;
+    __shared__ CentroidChunk sspec[CS_LDS_CHUNKS];   // the first chunks' records, loaded up front
+    const double* __restrict__ col = X + (int64_t)blockIdx.x * ldx;
+    for (int c = threadIdx.x; c < min(nch, CS_LDS_CHUNKS); c += CB_THREADS) sspec[c] = spec[blockIdx.x * nch + c];
+    __syncthreads();
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        const CentroidChunk sp = c < CS_LDS_CHUNKS ? sspec[c] : spec[blockIdx.x * nch + c];
+        const int base = c * CS_CHUNK, end = min(n, base + CS_CHUNK);
+        int from = base;   // where the block method takes over, end: not needed
+        if (!sp.bad) {
+            if (centroid_part(s, sp.A)) {
+                from = end;
+                if (sp.kb >= 0) {
+                    s = s + col[base + sp.kb];
+                    from = centroid_part(s, sp.B) ? end : base + sp.kb + 1;
+                }
+            }
+        }
+        if (from < end) s = centroid_range(col, from, end, s, L);
+    }
+    if (threadIdx.x == 0) root[blockIdx.x] = s / (double)n;
 }
 
 // max |x - c| per axis (exact, so any reduction order is fine).
@@ -667,8 +934,22 @@ void launch_list_csr(const unsigned long long* keys, int n_keys, int num_targets
     FMMH_HIP(hipGetLastError());
 }
 
-void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, hipStream_t s) {
-    hipLaunchKernelGGL(k_centroid, dim3(3), dim3(CB_THREADS), 0, s, X, ldx, n, t.root);
+size_t centroid_work_bytes(int n) {
+    const size_t nch = (size_t)blocks_for(n, CS_CHUNK);
+    return 3 * nch * (sizeof(double) + sizeof(CentroidChunk)) + 64;
+}
+
+void launch_centroid(const double* X, int64_t ldx, int n, TreeDev& t, void* work, hipStream_t s) {
+    const int nch = blocks_for(n, CS_CHUNK);
+    if (nch <= 2) {
+        hipLaunchKernelGGL(k_centroid, dim3(3), dim3(CB_THREADS), 0, s, X, ldx, n, t.root);
+    } else {
+        CentroidChunk* spec = reinterpret_cast<CentroidChunk*>(work);
+        double* csum = reinterpret_cast<double*>(spec + 3 * nch);
+        hipLaunchKernelGGL(k_centroid_csum, dim3(nch, 3), dim3(CS_T), 0, s, X, ldx, n, nch, csum);
+        hipLaunchKernelGGL(k_centroid_spec, dim3(nch, 3), dim3(CS_T), 0, s, X, ldx, n, nch, csum, spec);
+        hipLaunchKernelGGL(k_centroid_chain, dim3(3), dim3(CB_THREADS), 0, s, X, ldx, n, nch, spec, t.root);
+    }
     FMMH_HIP(hipMemsetAsync(t.maxabs, 0, 3 * sizeof(unsigned long long), s));
     int g = std::min(blocks_for(n, 256), 1024);
     hipLaunchKernelGGL(k_maxabs, dim3(g), dim3(256), 0, s, X, ldx, n, t.root, t.maxabs);
