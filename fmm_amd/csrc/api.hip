@@ -1730,55 +1730,54 @@ const unsigned char* select_orders(fmmh_context* c, TreeOut& tr, ListsOut& L, Ta
 
 // Relative RMS error (field block; the potential for field order 0) of the
 // far field at spot sites against the direct sum with the reference's far
-// semantics, per multipole set; out[0] = the largest over the sets (sets
-// without field count 0).  FMM far field at a site = field - field before
-// L2P - near; the direct far field = the sum over all sources (the source
-// chunks' parts dir[chunk][set][spot][4], in chunk order) minus the sum over
-// the P2P list's sources (dnear[set][spot][4]).  One workgroup, the sets in
-// turn.
+// semantics, one workgroup per multipole set: err[set].  FMM far field at a
+// site = field - field before L2P - near; the direct far field is the sum of
+// the source chunks' parts dir[chunk][set][spot][4], in chunk order.
 __global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__ dir, const int* __restrict__ site,
                                                       const int* __restrict__ perm, const double* __restrict__ near,
                                                       int64_t ldn, const double* __restrict__ field, int64_t ldf,
                                                       const double* __restrict__ before, int64_t ldb, int nspots,
-                                                      int nf, int nsets, int nchunks, const double* __restrict__ dnear,
-                                                      double* __restrict__ out) {
+                                                      int nf, int nsets, int nchunks, double* __restrict__ err) {
     __shared__ double red[2][256];
+    const int set = blockIdx.x;
     const int c0 = nf >= 4 ? 1 : 0, c1 = nf >= 4 ? 4 : 1;
-    double worst = 0.0;
-    for (int set = 0; set < nsets; ++set) {
-        const double* __restrict__ nr = near + (int64_t)set * nf * ldn;
-        const double* __restrict__ fl = field + (int64_t)set * nf * ldf;
-        const double* __restrict__ bf = before + (int64_t)set * nf * ldb;
-        double e2 = 0.0, f2 = 0.0;
-        for (int sp = threadIdx.x; sp < nspots; sp += 256) {
-            const int k = site[sp], o = perm[k];
-            for (int c = c0; c < c1; ++c) {
-                const double nv = nr[(int64_t)c * ldn + k];
-                const double ffar = fl[(int64_t)c * ldf + o] - bf[(int64_t)c * ldb + o] - nv;
-                double d = 0.0;
-                for (int ch = 0; ch < nchunks; ++ch) d += dir[(((int64_t)ch * nsets + set) * nspots + sp) * 4 + c];
-                d -= dnear[((int64_t)set * nspots + sp) * 4 + c];
-                e2 += (ffar - d) * (ffar - d);
-                f2 += (nv + d) * (nv + d);
-            }
-        }
-        red[0][threadIdx.x] = e2;
-        red[1][threadIdx.x] = f2;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) {
-                red[0][threadIdx.x] += red[0][threadIdx.x + w];
-                red[1][threadIdx.x] += red[1][threadIdx.x + w];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0 && red[1][0] > 0.0) {
-            const double e = sqrt(red[0][0] / red[1][0]);
-            worst = (e > worst || e != e) ? e : worst;   // a non-finite error wins
+    const double* __restrict__ nr = near + (int64_t)set * nf * ldn;
+    const double* __restrict__ fl = field + (int64_t)set * nf * ldf;
+    const double* __restrict__ bf = before + (int64_t)set * nf * ldb;
+    double e2 = 0.0, f2 = 0.0;
+    for (int x = threadIdx.x; x < 4 * nspots; x += 256) {   // x = 4 spot + c: the chunk loads are coalesced
+        const int sp = x >> 2, c = x & 3;
+        if (c < c0 || c >= c1) continue;
+        const int k = site[sp], o = perm[k];
+        const double nv = nr[(int64_t)c * ldn + k];
+        const double ffar = fl[(int64_t)c * ldf + o] - bf[(int64_t)c * ldb + o] - nv;
+        double d = 0.0;
+        for (int ch = 0; ch < nchunks; ++ch) d += dir[((int64_t)ch * nsets + set) * nspots * 4 + x];
+        e2 += (ffar - d) * (ffar - d);
+        f2 += (nv + d) * (nv + d);
+    }
+    red[0][threadIdx.x] = e2;
+    red[1][threadIdx.x] = f2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = worst;
+    if (threadIdx.x == 0) err[set] = red[1][0] > 0.0 ? sqrt(red[0][0] / red[1][0]) : 0.0;   // no field: 0
+}
+
+// out[0] = the largest error over the sets (a non-finite one wins).
+__global__ void k_spot_worst(const double* __restrict__ err, int nsets, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double worst = 0.0;
+    for (int set = 0; set < nsets; ++set) {
+        const double e = err[set];
+        worst = (e > worst || e != e) ? e : worst;
+    }
+    out[0] = worst;
 }
 
 double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, const double* ys, const double* zs,
@@ -1786,16 +1785,22 @@ double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, c
                   int nspots) {
     hipStream_t s = c->stream;
     nspots = std::min(nspots, tr.num_leaves);
-    const int nchunks = spot_chunks(n);
+    // source chunks: about four workgroups per compute unit in all
+    const int tile = spot_tile(m);
+    const int sets_blocks = (k.nsets + 31) / 32;
+    const long long others = (long long)spot_row_blocks(nspots) * sets_blocks;
+    const int want = (int)std::max(1ll, (4ll * c->num_cu + others - 1) / others);
+    const int chunk = std::max(tile, (int)(((long long)n + (long long)want * tile - 1) / ((long long)want * tile)) * tile);
+    const int nchunks = (n + chunk - 1) / chunk;
     double* out = c->buf<double>("spot_out", (size_t)nchunks * nspots * 4 * k.nsets);
     int* site = c->buf<int>("spot_site", nspots);
-    double* res = c->buf<double>("spot_res", 2);
-    double* out_near = c->buf<double>("spot_out_near", (size_t)nspots * 4 * k.nsets);
+    double* res = c->buf<double>("spot_res", 1 + (size_t)k.nsets);
     SpotArgs sa{nspots, tr.num_leaves, n, tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, xs, ys, zs,
-                ms, n, out, out_near, site, k.nsets};
+                ms, n, out, site, k.nsets, chunk, nchunks};
     launch_spot_far(sa, m, s);
-    hipLaunchKernelGGL(k_spot_compare, dim3(1), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n, k.field,
-                       k.ldf, before, (int64_t)n, nspots, nf, k.nsets, nchunks, out_near, res);
+    hipLaunchKernelGGL(k_spot_compare, dim3(k.nsets), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n,
+                       k.field, k.ldf, before, (int64_t)n, nspots, nf, k.nsets, nchunks, res + 1);
+    hipLaunchKernelGGL(k_spot_worst, dim3(1), dim3(64), 0, s, res + 1, k.nsets, res);
     FMMH_HIP(hipGetLastError());
     double* h = reinterpret_cast<double*>(c->h_pinned + 800);
     FMMH_HIP(hipMemcpyAsync(h, res, sizeof(double), hipMemcpyDeviceToHost, s));

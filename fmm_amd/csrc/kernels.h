@@ -436,15 +436,16 @@ struct SpotArgs {
     const int* p2p_off; const int* p2p_src;
     const double* xs; const double* ys; const double* zs;
     const double* ms; int64_t ldm;            // Taylor-scaled multipoles [set][component][ldm]
-    double* out;                              // [chunk][set][nspots][4] direct sum over all sources, per chunk
-    double* out_near;                         // [set][nspots][4] direct sum over the P2P list's sources
+    double* out;                              // [chunk][set][nspots][4] far-field direct sum, per source chunk
     int* site;                                // [nspots] sorted site index
     int nsets;
+    int chunk; int nchunks;                   // sources per chunk (a multiple of spot_tile(m)), chunk count
 };
 // Source multipole orders the spot checks (and so the learned per-pair orders) cover; calls with higher
 // source orders keep expansion_order for every pair.
 constexpr int SPOT_MMAX = 2;
-int spot_chunks(int n);    // source chunks of launch_spot_far (its output's leading dimension)
+int spot_tile(int m);                  // sources per staged tile of launch_spot_far
+int spot_row_blocks(int nspots);       // its workgroups along the spot sites
 void launch_spot_far(const SpotArgs& a, int m, hipStream_t s);
 void launch_to_f32(const double* in, float* out, int64_t n, hipStream_t s);
 

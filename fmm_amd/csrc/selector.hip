@@ -18,6 +18,8 @@
 // per sampled pair, M2L at every order q <= cap plus exact evaluation at up
 // to SEL_NS sites of box i, against the direct sum over the sites of box j),
 // by the forward / backward / Adam kernels below.
+#include <climits>
+
 #include "common.h"
 #include "kernels.h"
 #include "hermite.h"
@@ -623,171 +625,153 @@ void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, int nsets
 // Far-field part of the direct sum at spot sites, with the reference's far
 // semantics: every source whose leaf is not in the P2P list of the site's
 // leaf, undamped and without exclusions.  The spot sites are the first sites
-// of evenly spaced leaves.  It is computed as the sum over all sources minus
-// the sum over the sources of the P2P list (a pair at R = 0 gives 0 in both),
-// so the large sweep needs no membership test:
-//   k_spot_all: a workgroup takes SPOT_NSP spot sites, SPOT_SB multipole sets
-//     and one chunk of SPOT_CHUNK sources, staged in LDS SPOT_TILE at a time
-//     (coordinates and the sets' multipoles, shared by the spot sites);
-//     SPOT_PH threads per spot site take every SPOT_PH-th source of a tile and
-//     each source's tensor is shared by the sets.  out_all[chunk][set][spot][4].
-//   k_spot_near: a workgroup per (spot site, SPOT_SB sets) sums the sources of
-//     the P2P list, 32 threads per list entry.  out_near[set][spot][4].
-// Partial sums are added in thread order (deterministic).
-constexpr int SPOT_SB = 8;
-constexpr int SPOT_NSP = 16;
-constexpr int SPOT_PH = 16;            // SPOT_NSP * SPOT_PH = 256 threads
-constexpr int SPOT_CHUNK = 16384;
-
-int spot_chunks(int n) { return (n + SPOT_CHUNK - 1) / SPOT_CHUNK; }
-
-// undamped tensor of orders <= M + 1 at displacement (dx, dy, dz); 0 at R = 0
-template <int M>
-__device__ __forceinline__ void spot_tensor(double dx, double dy, double dz, double (&T)[ncomp(M + 1)]) {
-    const double R2 = dx * dx + dy * dy + dz * dz;
-    double g[M + 2];
-    radial_inv<M + 1, DAMP_NONE>(R2, R2 > 0.0 ? rsqrt_pos(R2) : 0.0, 0.0, g);
-    md_tensor<M + 1>(dx, dy, dz, g, T);
-}
-
-__device__ __forceinline__ int spot_site(const SpotArgs& a, int spot) {
-    const int leaf = (int)(((long long)spot * a.num_leaves) / a.nspots);
-    return a.start[a.leaf_node[leaf]];
-}
+// of evenly spaced leaves.
+// The sum over the sources is a GEMM on the FP64 matrix cores: for field
+// component c (potential and field) and source component a,
+//   D_c[spot][set] += sum_j T_{a+c}(x_spot - x_j) m~_j(a)[set]
+// (the contraction of contract<M, 1>).  A workgroup takes SP_W 16-row tiles
+// of spot sites (one per wavefront), 16 NTL multipole sets and one chunk of
+// the sources, staged in LDS SP_TJ<M> at a time; in each 4-source step lane
+// (row, k) computes the tensor of pair (spot row, source k) and feeds it to
+// the 4 ncomp(M) NTL MFMAs of the step.  The tensor is zero at R = 0 and for
+// sources of the spot leaf's P2P list: the list's leaves are in site order, so
+// each lane keeps a cursor on the first list range not yet passed.  Partial
+// sums per chunk, out[chunk][set][spot][4], added in chunk order by the
+// comparison (deterministic).
+using v4d = double __attribute__((ext_vector_type(4)));
+constexpr int SP_W = 4;   // wavefronts per workgroup
 
 template <int M>
-__global__ __launch_bounds__(256) void k_spot_all(SpotArgs a) {
-    constexpr int NM = ncomp(M);
-    constexpr int TILE = M <= 1 ? 128 : 64;
-    __shared__ double sxs[TILE], sys[TILE], szs[TILE];
-    __shared__ double smu[SPOT_SB][NM][TILE];
-    __shared__ double red[4][256];
-    const int t = threadIdx.x;
-    const int sp = t % SPOT_NSP, ph = t / SPOT_NSP;
-    const int spot = blockIdx.x * SPOT_NSP + sp;
+constexpr int sp_tj() { return M >= 2 ? 16 : 32; }   // sources per staged tile
+
+int spot_tile(int m) { return m >= 2 ? sp_tj<2>() : sp_tj<1>(); }
+int spot_row_blocks(int nspots) { return (nspots + 16 * SP_W - 1) / (16 * SP_W); }
+
+template <int M, int NTL>
+__global__ __launch_bounds__(64 * SP_W) void k_spot_far(SpotArgs a) {
+    constexpr int NM = ncomp(M), NC = ncomp(M + 1);
+    constexpr int TJ = sp_tj<M>();
+    constexpr int BW = 16 * NTL, BWP = BW + 1;   // odd row stride: the staging stores hit distinct banks
+    __shared__ double sx[TJ], sy[TJ], sz[TJ];
+    __shared__ double sb[NM * TJ * BWP];        // [component][source][set]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    const int spot = (blockIdx.x * SP_W + w) * 16 + row;
     const bool valid = spot < a.nspots;
-    const int set0 = blockIdx.y * SPOT_SB;
-    const int nsb = min(SPOT_SB, a.nsets - set0);
-    const int j0 = blockIdx.z * SPOT_CHUNK, j1 = min(a.n, j0 + SPOT_CHUNK);
-    const int site = spot_site(a, valid ? spot : 0);
-    const double x = a.xs[site], y = a.ys[site], z = a.zs[site];
-    double acc[SPOT_SB][4];
-#pragma unroll
-    for (int u = 0; u < SPOT_SB; ++u)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[u][c] = 0.0;
-    for (int tb = j0; tb < j1; tb += TILE) {
-        const int cn = min(TILE, j1 - tb);
-        __syncthreads();                       // readers of the previous tile are done
-        for (int e = t; e < TILE; e += 256)
-            if (e < cn) {
-                sxs[e] = a.xs[tb + e];
-                sys[e] = a.ys[tb + e];
-                szs[e] = a.zs[tb + e];
-            }
-        for (int e = t; e < SPOT_SB * NM * TILE; e += 256) {
-            const int jj = e % TILE, us = e / TILE, u = us / NM, sc = us - u * NM;
-            smu[u][sc][jj] = (u < nsb && jj < cn) ? a.ms[((int64_t)(set0 + u) * NM + sc) * a.ldm + tb + jj] : 0.0;
-        }
-        __syncthreads();
-        if (!valid) continue;
-        for (int jj = ph; jj < cn; jj += SPOT_PH) {
-            double T[ncomp(M + 1)];
-            spot_tensor<M>(x - sxs[jj], y - sys[jj], z - szs[jj], T);
-#pragma unroll
-            for (int u = 0; u < SPOT_SB; ++u) {
-                double m[NM];
-#pragma unroll
-                for (int sc = 0; sc < NM; ++sc) m[sc] = smu[u][sc][jj];
-                contract<M, 1>(m, T, acc[u]);
-            }
-        }
-    }
-    // sum the SPOT_PH partials of each spot site in thread order, set by set
-    for (int u = 0; u < nsb; ++u) {
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 4; ++c) red[c][t] = acc[u][c];
-        __syncthreads();
-        if (t < SPOT_NSP * 4) {
-            const int s2 = t % SPOT_NSP, c = t / SPOT_NSP;
-            const int spot2 = blockIdx.x * SPOT_NSP + s2;
-            if (spot2 < a.nspots) {
-                double v = 0.0;
-                for (int q = 0; q < SPOT_PH; ++q) v += red[c][q * SPOT_NSP + s2];
-                a.out[(((int64_t)blockIdx.z * a.nsets + set0 + u) * a.nspots + spot2) * 4 + c] = v;
-            }
-        }
-    }
-    if (valid && ph == 0 && blockIdx.y == 0 && blockIdx.z == 0) a.site[spot] = site;
-}
-
-template <int M>
-__global__ __launch_bounds__(256) void k_spot_near(SpotArgs a) {
-    constexpr int NM = ncomp(M);
-    __shared__ double red[4][256];
-    const int spot = blockIdx.x;
-    const int set0 = blockIdx.y * SPOT_SB;
-    const int nsb = min(SPOT_SB, a.nsets - set0);
-    const int leaf = (int)(((long long)spot * a.num_leaves) / a.nspots);
+    const int leaf = (int)(((long long)(valid ? spot : 0) * a.num_leaves) / a.nspots);
     const int node = a.leaf_node[leaf];
     const int site = a.start[node];
     const double x = a.xs[site], y = a.ys[site], z = a.zs[site];
-    double acc[SPOT_SB][4];
+    const int set0 = blockIdx.y * BW;
+    const int j0 = blockIdx.z * a.chunk, j1 = min(a.n, j0 + a.chunk);
+    // cursor: the first P2P-list entry whose site range ends after this lane's first source
+    const int eend = a.p2p_off[node + 1];
+    int e = a.p2p_off[node];
+    {
+        int hi = eend;
+        const int jf = j0 + kq;
+        while (e < hi) {
+            const int mid = (e + hi) >> 1;
+            const int sn = src_node(a.p2p_src[mid]);
+            if (a.start[sn] + a.count[sn] <= jf) e = mid + 1;
+            else hi = mid;
+        }
+    }
+    int rlo = INT_MAX, rhi = INT_MAX;   // the entry's site range [rlo, rhi)
+    auto load_range = [&]() {
+        if (e < eend) {
+            const int sn = src_node(a.p2p_src[e]);
+            rlo = a.start[sn];
+            rhi = rlo + a.count[sn];
+        } else {
+            rlo = rhi = INT_MAX;
+        }
+    };
+    load_range();
+    v4d acc[4][NTL];
 #pragma unroll
-    for (int u = 0; u < SPOT_SB; ++u)
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[u][c] = 0.0;
-    // 8 groups of 32 threads, group g on the list entries g, g + 8, ...; a leaf holds < ncrit sites
-    const int grp = threadIdx.x / 32, ln = threadIdx.x % 32;
-    for (int e = a.p2p_off[node] + grp; e < a.p2p_off[node + 1]; e += 8) {
-        const int sn = src_node(a.p2p_src[e]);
-        const int st = a.start[sn], cnt = a.count[sn];
-        for (int j = st + ln; j < st + cnt; j += 32) {
-            double T[ncomp(M + 1)];
-            spot_tensor<M>(x - a.xs[j], y - a.ys[j], z - a.zs[j], T);
+        for (int nt = 0; nt < NTL; ++nt) acc[c][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int tb = j0; tb < j1; tb += TJ) {
+        const int cn = min(TJ, j1 - tb);
+        __syncthreads();   // readers of the previous tile are done
+        for (int q = tid; q < TJ; q += 64 * SP_W) {
+            const bool ok = q < cn;
+            sx[q] = ok ? a.xs[tb + q] : 0.0;
+            sy[q] = ok ? a.ys[tb + q] : 0.0;
+            sz[q] = ok ? a.zs[tb + q] : 0.0;
+        }
+        for (int q = tid; q < NM * BW * TJ; q += 64 * SP_W) {
+            const int jl = q % TJ, r = q / TJ, comp = r % NM, u = r / NM;   // source fastest: coalesced loads
+            const int set = set0 + u;
+            sb[(comp * TJ + jl) * BWP + u] =
+                (jl < cn && set < a.nsets) ? a.ms[((int64_t)set * NM + comp) * a.ldm + tb + jl] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int k4 = 0; k4 < TJ; k4 += 4) {
+            const int jl = k4 + kq, j = tb + jl;
+            while (j >= rhi) {
+                ++e;
+                load_range();
+            }
+            const bool use = valid && jl < cn && j < rlo;
+            const double dx = x - sx[jl], dy = y - sy[jl], dz = z - sz[jl];
+            const double R2 = dx * dx + dy * dy + dz * dz;
+            double g[M + 2];
+            radial_inv<M + 1, DAMP_NONE>(R2, (use && R2 > 0.0) ? rsqrt_pos(R2) : 0.0, 0.0, g);
+            double T[NC];
+            md_tensor<M + 1>(dx, dy, dz, g, T);
+            double bv[NM][NTL];
 #pragma unroll
-            for (int u = 0; u < SPOT_SB; ++u) {
-                if (u >= nsb) break;
-                double m[NM];
+            for (int comp = 0; comp < NM; ++comp)
 #pragma unroll
-                for (int sc = 0; sc < NM; ++sc) m[sc] = a.ms[((int64_t)(set0 + u) * NM + sc) * a.ldm + j];
-                contract<M, 1>(m, T, acc[u]);
+                for (int nt = 0; nt < NTL; ++nt) bv[comp][nt] = sb[(comp * TJ + jl) * BWP + 16 * nt + row];
+#pragma unroll
+            for (int comp = 0; comp < NM; ++comp) {
+                const Exps es = idx2xyz(comp);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double av = T[xyz2idx(es.x + (c == 1), es.y + (c == 2), es.z + (c == 3))];
+#pragma unroll
+                    for (int nt = 0; nt < NTL; ++nt)
+                        acc[c][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[comp][nt], acc[c][nt], 0, 0, 0);
+                }
             }
         }
     }
-    for (int u = 0; u < nsb; ++u) {
-        __syncthreads();
+    // accumulator register q of lane l holds D[spot row 4 q + l / 16][set l % 16]
 #pragma unroll
-        for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = acc[u][c];
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w)
-                for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
-            __syncthreads();
+    for (int q = 0; q < 4; ++q) {
+        const int sp = (blockIdx.x * SP_W + w) * 16 + 4 * q + kq;
+        if (sp >= a.nspots) continue;
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) {
+            const int set = set0 + 16 * nt + row;
+            if (set >= a.nsets) continue;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                a.out[(((int64_t)blockIdx.z * a.nsets + set) * a.nspots + sp) * 4 + c] = acc[c][nt][q];
         }
-        if (threadIdx.x < 4) a.out_near[((int64_t)(set0 + u) * a.nspots + spot) * 4 + threadIdx.x] = red[threadIdx.x][0];
     }
+    if (valid && blockIdx.y == 0 && blockIdx.z == 0 && kq == 0) a.site[spot] = site;
+}
+
+template <int M>
+static void spot_far_m(const SpotArgs& a, hipStream_t s) {
+    const int ntl = a.nsets > 16 ? 2 : 1;
+    const dim3 g(spot_row_blocks(a.nspots), (a.nsets + 16 * ntl - 1) / (16 * ntl), a.nchunks);
+    if (ntl == 2) hipLaunchKernelGGL((k_spot_far<M, 2>), g, dim3(64 * SP_W), 0, s, a);
+    else hipLaunchKernelGGL((k_spot_far<M, 1>), g, dim3(64 * SP_W), 0, s, a);
 }
 
 void launch_spot_far(const SpotArgs& a, int m, hipStream_t s) {
     if (a.nspots <= 0) return;
-    const int sb = (a.nsets + SPOT_SB - 1) / SPOT_SB;
-    const dim3 g((a.nspots + SPOT_NSP - 1) / SPOT_NSP, sb, spot_chunks(a.n));
-    const dim3 gn(a.nspots, sb);
     switch (m) {
-        case 0:
-            hipLaunchKernelGGL(k_spot_all<0>, g, dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_spot_near<0>, gn, dim3(256), 0, s, a);
-            break;
-        case 1:
-            hipLaunchKernelGGL(k_spot_all<1>, g, dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_spot_near<1>, gn, dim3(256), 0, s, a);
-            break;
-        default:
-            hipLaunchKernelGGL(k_spot_all<2>, g, dim3(256), 0, s, a);
-            hipLaunchKernelGGL(k_spot_near<2>, gn, dim3(256), 0, s, a);
-            break;
+        case 0: spot_far_m<0>(a, s); break;
+        case 1: spot_far_m<1>(a, s); break;
+        default: spot_far_m<2>(a, s); break;
     }
     FMMH_HIP(hipGetLastError());
 }
