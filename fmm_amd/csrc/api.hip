@@ -1603,8 +1603,9 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     for (int c0 = 0; c0 < nsets; c0 += chunk) {
         const int ns = std::min(chunk, nsets - c0);
         const int s_ld = mfma ? pad_sets(ns) : 0;
-        launch_scale_sources(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, s_ld, s);
-        if (f32) launch_to_f32(CMs, CMs32, (int64_t)node_stride * s_ld, s);
+        if (mfma) launch_scale_sources_red(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, s_ld, s);
+        else launch_scale_sources(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, 0, s);
+        if (f32) launch_to_f32(CMs, CMs32, (int64_t)nn * (p + 1) * (p + 1) * s_ld, s);
         M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,
                    nm, p, ns, s_ld, nn, order, tb.gemm_fidx, CMs32, Tcp};
         if (!mfma) {
@@ -1616,6 +1617,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         auto it = c->tuned.find(key);
         if (it != c->tuned.end() || !c->autotune || c->capturing) {
             launch_m2l_mfma(ma, nn, it != c->tuned.end() ? it->second : m2l_mfma_default_variant(ma), s);
+            launch_le_unfold(LE + c0 * node_stride, (long long)ns * nn, p, s);
             continue;
         }
         // first call of this shape: time every valid variant on this call's pairs; each one writes
@@ -1639,6 +1641,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
             if (ms < best_ms) { best_ms = ms; best = v; }
         }
         c->tuned[key] = best;
+        launch_le_unfold(LE + c0 * node_stride, (long long)ns * nn, p, s);
     }
 }
 

@@ -549,6 +549,102 @@ void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s) {
     FMMH_HIP(hipGetLastError());
 }
 
+// Harmonic (traceless) reduction of the many-sets M2L.  Away from R = 0 the
+// Laplace tensor obeys T_{k+2x} + T_{k+2y} + T_{k+2z} = 0, so the contraction
+// sum_s T_{s+t} M~(s) over |s| <= p equals the one over the 2n + 1 components
+// of each degree n with s_z <= 1, of the folded sources
+//   M'(x + 2i, y + 2(k - i), rho) = sum_{k, i} (-1)^k C(k, i) M~(x, y, rho + 2k),
+// and the local-expansion components with t_z >= 2 follow from those with
+// t_z <= 1 by the same identity,
+//   L(tx, ty, 2k + rho) = (-1)^k sum_i C(k, i) L(tx + 2i, ty + 2(k - i), rho).
+// The Hankel matrix shrinks from ncomp(p)^2 to (p + 1)^4 entries (220^2 ->
+// 100^2 at p = 9); the class tensors stay whole.  Reduced components are
+// ordered by degree, then w = y + z, then z, as the packed order runs.
+__host__ __device__ constexpr int rdim(int q) { return (q + 1) * (q + 1); }
+
+// Reduced component r -> degree | w << 8 | z << 16 (the packing of decode_target).
+__device__ __forceinline__ int decode_reduced(int r) {
+    int k = 0;
+    while (rdim(k) <= r) ++k;
+    const int pos = r - k * k;   // k^2 = rdim(k - 1) components of lower degree
+    const int w = (pos + 1) >> 1;
+    const int z = (pos > 0 && (pos & 1) == 0) ? 1 : 0;
+    return k | (w << 8) | (z << 16);
+}
+
+// Packed (full) index of a decoded component.
+__device__ __forceinline__ int packed_of(int d) {
+    const int k = d & 255, w = (d >> 8) & 255, z = d >> 16;
+    return ncomp(k - 1) + tri(w) + z;
+}
+
+// Folded, box-scaled sources M'_j(s') in the MFMA layout [node][s' < rdim(p)][s_ld sets] (padding sets
+// zero): the sum over the components s that fold onto s' of (-1)^k C(k, i) c_s M_j(s), times r_j^{-|s'|}.
+__global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restrict__ CM, const double* __restrict__ cs,
+                                                           const double* __restrict__ r, double* __restrict__ CMs,
+                                                           int num_nodes, int p, int nsets, int s_ld) {
+    const int nr = rdim(p), nm = ncomp(p);
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (int64_t)num_nodes * nr * s_ld) return;
+    const int64_t js = x / s_ld;
+    const int set = (int)(x - js * s_ld);
+    if (set >= nsets) { CMs[x] = 0.0; return; }
+    const int j = (int)(js / nr), sr = (int)(js - (int64_t)j * nr);
+    const int d = decode_reduced(sr);
+    const int deg = d & 255, w = (d >> 8) & 255, rho = d >> 16;
+    const int X = deg - w, Y = w - rho;
+    const double* __restrict__ cm = CM + (int64_t)set * num_nodes * nm + (int64_t)j * nm;
+    double acc = 0.0;
+    for (int k = 0; 2 * k <= X + Y; ++k) {
+        const double sg = (k & 1) ? -1.0 : 1.0;
+        double binom = 1.0;
+        for (int i = 0; i <= k; ++i) {
+            if (i > 0) binom = binom * (k - i + 1) / i;
+            const int sx = X - 2 * i, sy = Y - 2 * (k - i);
+            if (sx < 0 || sy < 0) continue;
+            const int s = xyz2idx(sx, sy, rho + 2 * k);
+            acc += (sg * binom) * (cs[s] * cm[s]);
+        }
+    }
+    const double rinv = 1.0 / r[j];
+    double f = 1.0;
+    for (int q = 0; q < deg; ++q) f *= rinv;
+    CMs[x] = acc * f;
+}
+
+// Local-expansion components with t_z >= 2 from the reduced ones the many-sets M2L wrote
+// (LE[set][node][ncomp(p)], in place: only components with t_z <= 1 are read).
+__global__ __launch_bounds__(256) void k_le_unfold(double* __restrict__ LE, int64_t rows, int p) {
+    const int nm = ncomp(p);
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= rows * nm) return;
+    const int t = (int)(x % nm);
+    const int d = decode_target(t);
+    const int deg = d & 255, w = (d >> 8) & 255, tz = d >> 16;
+    if (tz < 2) return;
+    const int tx = deg - w, ty = w - tz, k = tz >> 1, rho = tz & 1;
+    const double* __restrict__ row = LE + (x - t);
+    double acc = 0.0, binom = 1.0;
+    for (int i = 0; i <= k; ++i) {
+        if (i > 0) binom = binom * (k - i + 1) / i;
+        acc += binom * row[xyz2idx(tx + 2 * i, ty + 2 * (k - i), rho)];
+    }
+    LE[x] = (k & 1) ? -acc : acc;
+}
+
+void launch_scale_sources_red(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
+                              int nsets, int s_ld, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_sources_red, dim3(nblk((long long)num_nodes * rdim(p) * s_ld, 256)), dim3(256), 0, s,
+                       CM, cs, r, CMs, num_nodes, p, nsets, s_ld);
+    FMMH_HIP(hipGetLastError());
+}
+
+void launch_le_unfold(double* LE, long long rows, int p, hipStream_t s) {
+    if (rows <= 0 || p < 2) return;
+    hipLaunchKernelGGL(k_le_unfold, dim3(nblk(rows * ncomp(p), 256)), dim3(256), 0, s, LE, (int64_t)rows, p);
+    FMMH_HIP(hipGetLastError());
+}
+
 void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
                           int nsets, int s_ld_out, hipStream_t s) {
     const int nm = ncomp(p);
@@ -582,7 +678,7 @@ constexpr int M2L_MFMA_PMAX = 10;
 constexpr int M2L_SETS_TILE = 16;
 
 template <int P, bool F32>
-constexpr int mfma_waves() { return ((ncomp(P) + 15) / 16) < 4 ? ((ncomp(P) + 15) / 16) : 4; }
+constexpr int mfma_waves() { return ((rdim(P) + 15) / 16) < 4 ? ((rdim(P) + 15) / 16) : 4; }
 
 // NTL = 16-set tiles per workgroup: an A fragment, gathered once, feeds NTL MFMAs.
 // FP64: the row scale is applied to each A element and the pair accumulates
@@ -602,7 +698,7 @@ __device__ __forceinline__ void dma16(const void* g, void* lds) {
 }
 
 template <int P, bool F32>
-constexpr int mfma_kpad() { return ((ncomp(P) + 3) / 4 + 1) / 2 * 2; }   // k-steps, even
+constexpr int mfma_kpad() { return ((rdim(P) + 3) / 4 + 1) / 2 * 2; }   // k-steps over reduced components, even
 
 // Source blocks are staged in LDS when the two buffers take at most 80 KB (with the class tensors
 // and tables a workgroup then stays under 96 KB: p 10 FP32 with 16 sets runs one workgroup per CU).
@@ -620,9 +716,10 @@ constexpr bool mfma_stage_b() {
 // selected to zero).  Two k-steps per iteration.
 template <int P, bool F32, int NTL, bool STG>
 __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
-    constexpr int NM = ncomp(P);
+    constexpr int NM = ncomp(P);            // local-expansion row length
+    constexpr int NK = rdim(P);             // reduced (harmonic) components: the GEMM's rows and k
     constexpr int NT = ncomp(2 * P);
-    constexpr int MT = (NM + 15) / 16;      // 16-row tiles of target coefficients
+    constexpr int MT = (NK + 15) / 16;      // 16-row tiles of reduced target coefficients
     constexpr int KSP = mfma_kpad<P, F32>();  // 4-deep steps over source coefficients, padded
     constexpr int NW = mfma_waves<P, F32>();  // wavefronts per workgroup, tile mt to wave mt % NW
     constexpr int MW = (MT + NW - 1) / NW;  // row tiles per wavefront
@@ -630,7 +727,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int NL = (NT + NTH - 1) / NTH;
     constexpr bool STAGE = STG && mfma_stage_b<P, F32, NTL>();
     constexpr int BW = 16 * NTL;            // sets per workgroup
-    constexpr int NBL = STAGE ? (NM * BW + NTH - 1) / NTH : 1;
+    constexpr int NBL = STAGE ? (NK * BW + NTH - 1) / NTH : 1;
     constexpr int NGD = 2 * P + 1;
     using TE = typename std::conditional<F32, float, double>::type;
     using Acc = typename std::conditional<F32, v4f, v4d>::type;
@@ -652,20 +749,20 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     const int e0 = a.off[i], e1 = a.off[i + 1];
     const TE* __restrict__ cms = F32 ? (const TE*)a.CMs32 : (const TE*)a.CMs;
     for (int x = tid; x < 4 * (KSP + 8); x += NTH) {
-        const int d = x < NM ? decode_target(x) : 0;
+        const int d = x < NK ? decode_reduced(x) : 0;
         sdec[x] = 4 * ((d & 255) * NGD + ((d >> 8) & 255)) | (ES * (d >> 16)) << 16;
     }
-    for (int x = tid; x < 16 * MT; x += NTH) degl[x] = x < NM ? (decode_target(x) & 255) : 99;
+    for (int x = tid; x < 16 * MT; x += NTH) degl[x] = x < NK ? (decode_reduced(x) & 255) : 99;
     for (int x = tid; x < NGD * NGD; x += NTH) gtab[x] = ES * (tet(x / NGD) + tri(x % NGD));
     // A fragment of row tile mt holds H[t = 16 mt + row][s = 4 ks + kq]
     int trow[MW], tz[MW], tdeg[MW];
 #pragma unroll
     for (int mi = 0; mi < MW; ++mi) {
         const int t = (w + NW * mi) * 16 + row;
-        const int d = t < NM ? decode_target(t) : 0;
+        const int d = t < NK ? decode_reduced(t) : 0;
         trow[mi] = 4 * ((d & 255) * NGD + ((d >> 8) & 255));
         tz[mi] = ES * (d >> 16);
-        tdeg[mi] = t < NM ? (d & 255) : 99;
+        tdeg[mi] = t < NK ? (d & 255) : 99;
     }
     v4d acc[MW][NTL];
 #pragma unroll
@@ -701,14 +798,14 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     const int boff_l = brow * a.s_ld + (lane % CPR) * CH;
     auto fetch = [&](int buf) {
         if constexpr (STAGE) {
-            const int nqn = ncomp(q_n);
+            const int nqn = rdim(q_n);
             const int kend = 4 * (((nqn + 3) / 4 + 1) & ~1);
             for (int x = nqn * BW + tid; x < kend * BW; x += NTH) Bl[buf][x] = TE(0);
             const TE* __restrict__ Tn = tcp + (int64_t)cls_n * NTP;
             for (int k = w; k * 64 * CH < NTP; k += NW)
                 if (64 * k * CH + lane * CH < NTP) dma16(Tn + 64 * k * CH + lane * CH, &Tl[buf][64 * k * CH]);
             // instruction k copies rows 64 k / CPR + lane / CPR, chunk lane % CPR of each
-            const TE* __restrict__ B = cms + (int64_t)j_n * NM * a.s_ld + sg;
+            const TE* __restrict__ B = cms + (int64_t)j_n * NK * a.s_ld + sg;
             for (int k = w; k * 64 < nqn * CPR; k += NW)
                 if (k * (64 / CPR) + brow < nqn) dma16(B + (int64_t)k * (64 / CPR) * a.s_ld + boff_l, &Bl[buf][64 * k * CH]);
         } else {
@@ -740,7 +837,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     for (int e = e0; e < e1; ++e) {
         const int cur = (e - e0) & 1;
         const bool more = e + 1 < e1;
-        // this pair: source node, level offset, learned order q (only the leading ncomp(q)
+        // this pair: source node, level offset, learned order q (only the leading rdim(q)
         // rows and columns)
         const int j = j_n;
         const int dl = li - __builtin_amdgcn_readfirstlane(lv_n);
@@ -751,7 +848,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
             lv_n = a.level[j_n];
             if (e + 2 < e1) load_raw(e + 2);
         }
-        const int nq = ncomp(qo);
+        const int nq = rdim(qo);
         const int KSq = ((nq + 3) / 4 + 1) & ~1;
         // level-offset scale of row t as a power of two, 2^-(dl |t|) or 2^(dl (|t| + 1)), i.e.
         // 2^(sa |t| + sb); rows past the pair's order get 2^-2000, which flushes them to zero
@@ -766,7 +863,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
         for (int mi = 0; mi < MW; ++mi) tzc[mi] = tz[mi] + cur * NTP * ES;
         const char* __restrict__ T = (const char*)&Tl[0][0];
         const TE* __restrict__ Bc = Bl[cur];
-        const int64_t boff = (int64_t)j * NM * a.s_ld + sg + row;
+        const int64_t boff = (int64_t)j * NK * a.s_ld + sg + row;
         Acc part[MW][NTL];
         // Software pipeline over step pairs (k0, k0 + 1): while the MFMAs of pair k0 run, the
         // tensor elements and source rows of pair k0 + 2, the G lookups of pair k0 + 4 and the
@@ -913,14 +1010,14 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int t = mt * 16 + (F32 ? 4 * kq + q : 4 * q + kq);
-            if (mt >= MT || t >= NM) continue;
-            const int deg = decode_target(t) & 255;
+            if (mt >= MT || t >= NK) continue;
+            const int dt = decode_reduced(t), deg = dt & 255, tf = packed_of(dt);
             double f = rinv;
             for (int k = 0; k < deg; ++k) f *= rinv;
 #pragma unroll
             for (int nt = 0; nt < NTL; ++nt) {
                 const int set = sg + 16 * nt + row;
-                if (set < a.nsets) a.LE[((int64_t)set * a.num_nodes + i) * NM + t] = acc[mi][nt][q] * f;
+                if (set < a.nsets) a.LE[((int64_t)set * a.num_nodes + i) * NM + tf] = acc[mi][nt][q] * f;
             }
         }
     }
