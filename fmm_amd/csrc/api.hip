@@ -49,8 +49,9 @@ struct Tables {
     const double* cs = nullptr;       // Taylor factors for the p expansion
     const double* scale = nullptr;    // Taylor factors for the dim source components
     int nsrc = 0;
-    const int* gemm_fidx = nullptr;   // class-grouped M2L: A-fragment index table (p <= M2L_GEMM_PMAX)
-    const int* gemm_tdeg = nullptr;   //   and |t| per target row
+    const int* gemm_fidx = nullptr;   // class-grouped / block M2L: A-fragment index table over the reduced
+    const int* gemm_tdeg = nullptr;   //   (harmonic) components, |t| per reduced target row,
+    const int* gemm_tfull = nullptr;  //   and the packed index of each reduced component
 };
 
 struct TreeOut {
@@ -408,22 +409,36 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     std::vector<double> cs(nm), scale(ncomp(m));
     for (int s = 0; s < nm; ++s) cs[s] = taylor_factor(comp[s]);
     for (int s = 0; s < ncomp(m); ++s) scale[s] = taylor_factor(comp[s]);
-    // Class-grouped M2L fragment tables: lane l of the A fragment of row tile
-    // mt and source step ks holds H[t][s], t = 16 mt + l % 16, s = 4 ks + l / 16.
-    std::vector<int> gemm_fidx, gemm_tdeg;
+    // Class-grouped / block M2L fragment tables over the reduced (harmonic) components -- those with
+    // z <= 1, by degree, then w = y + z, then z (m2l.hip): lane l of the A fragment of row tile mt and
+    // source step ks holds H[t][s], t = 16 mt + l % 16, s = 4 ks + l / 16 (reduced indices).
+    std::vector<int> gemm_fidx, gemm_tdeg, gemm_tfull;
     if (p <= M2L_GEMM_PMAX) {
-        const int MT = (nm + 15) / 16, KS = (nm + 3) / 4;
+        const int nk = (p + 1) * (p + 1);
+        std::vector<XYZ> red;
+        for (int n = 0; n <= p; ++n) {
+            red.push_back(XYZ{n, 0, 0});
+            for (int w = 1; w <= n; ++w) {
+                red.push_back(XYZ{n - w, w, 0});
+                red.push_back(XYZ{n - w, w - 1, 1});
+            }
+        }
+        const int MT = (nk + 15) / 16, KS = (nk + 3) / 4;
         gemm_fidx.assign((size_t)MT * KS * 64, -1);
         gemm_tdeg.assign((size_t)MT * 16, 0);
+        gemm_tfull.assign((size_t)nk, 0);
         for (int mt = 0; mt < MT; ++mt)
             for (int ks = 0; ks < KS; ++ks)
                 for (int l = 0; l < 64; ++l) {
                     const int t = 16 * mt + l % 16, s = 4 * ks + l / 16;
-                    if (t < nm && s < nm)
+                    if (t < nk && s < nk)
                         gemm_fidx[((size_t)mt * KS + ks) * 64 + l] =
-                            idx(comp[t].x + comp[s].x, comp[t].y + comp[s].y, comp[t].z + comp[s].z);
+                            idx(red[t].x + red[s].x, red[t].y + red[s].y, red[t].z + red[s].z);
                 }
-        for (int t = 0; t < nm; ++t) gemm_tdeg[t] = comp[t].x + comp[t].y + comp[t].z;
+        for (int t = 0; t < nk; ++t) {
+            gemm_tdeg[t] = red[t].x + red[t].y + red[t].z;
+            gemm_tfull[t] = idx(red[t].x, red[t].y, red[t].z);
+        }
     }
 
     // pack into one int and one double buffer
@@ -432,7 +447,7 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     t.p = p; t.m = m; t.nsrc = ncomp(msrc);
     std::vector<std::vector<int>*> ilist = {&p2m_off, &p2m_s, &p2m_e, &m2m_off, &m2m_s, &m2m_e, &l2l_off, &l2l_s,
                                             &l2l_e, &mono_parent, &mono_axis, &xyz, &level_base, &gemm_fidx,
-                                            &gemm_tdeg};
+                                            &gemm_tdeg, &gemm_tfull};
     std::vector<std::vector<double>*> dlist = {&p2m_c, &m2m_c, &l2l_c, &cs, &scale};
     std::vector<size_t> ioff, doff;
     size_t ni = 0, nd = 0;
@@ -458,6 +473,7 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     if (p <= M2L_GEMM_PMAX) {
         t.gemm_fidx = I + ioff[13];
         t.gemm_tdeg = I + ioff[14];
+        t.gemm_tfull = I + ioff[15];
     }
     t.cs = D + doff[3];
     t.scale = D + doff[4];
@@ -1473,7 +1489,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     c->mark("m2l_classes");
     // auto: the per-target MFMA form from M2L_MFMA_MIN_SETS sets on (a 16-set tile, padded, costs about
     // as much as 6 passes of the class-grouped GEMM at p 10), or whenever the GEMM's slot buffer does not fit
-    const bool gemm_fits = p <= M2L_GEMM_PMAX && np > 0 && (double)np * nm * sizeof(double) <= M2L_GEMM_MAX_BYTES;
+    const bool gemm_fits = p <= M2L_GEMM_PMAX && np > 0 && (double)np * rdim(p) * sizeof(double) <= M2L_GEMM_MAX_BYTES;
     const bool mfma = c->m2l_mfma == 2 ? m2l_mfma_supported(p, 16)
                                        : (c->m2l_mfma == 1 && (m2l_mfma_supported(p, nsets) ||
                                                                (m2l_mfma_supported(p, 16) &&
@@ -1510,12 +1526,15 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         }
         // two workgroups per compute unit (LDS: the block's local expansions plus the staged tile)
         const int grid = std::max(1, 2 * c->num_cu);
-        double* CMs = c->buf<double>("m_cms", node_stride * nsets + M2LB_READ_SLACK);
-        launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
+        const size_t red_stride = (size_t)nn * rdim(p);
+        double* CMs = c->buf<double>("m_cms", red_stride * nsets + M2LB_READ_SLACK);
+        for (int set = 0; set < nsets; ++set)
+            launch_scale_sources_red(CM + set * node_stride, tb.cs, tr.t.r, CMs + set * red_stride, nn, p, 1, 1, s);
         M2LBRunArgs ra{ba.toff, ba.pbase, ba.rec, nbm, ba.ntg, ba.tperm, ba.tinv, nn, nullptr, Tc, tb.gemm_fidx,
-                       tb.gemm_tdeg, c->buf<double>("b_partial", (size_t)(nbm + grid) * nm * M2LB_BT), grid};
+                       tb.gemm_tdeg, tb.gemm_tfull,
+                       c->buf<double>("b_partial", (size_t)(nbm + grid) * rdim(p) * M2LB_BT), grid};
         for (int set = 0; set < nsets; ++set) {
-            ra.CMs = CMs + set * node_stride;
+            ra.CMs = CMs + set * red_stride;
             launch_m2lb_run(ra, p, tr.t.r, LE + set * node_stride, s);
         }
         return;
@@ -1572,18 +1591,22 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         }
         const int grid = ngrp + (np + M2L_GEMM_TN - 1) / M2L_GEMM_TN;   // >= tiles
         const bool f32 = c->precision == 1;
-        double* pbuf = f32 ? nullptr : c->buf<double>("m_pbuf", (size_t)np * nm);
-        float* pbuf32 = f32 ? c->buf<float>("m_pbuf32", (size_t)np * nm) : nullptr;
-        float* cms32 = f32 ? c->buf<float>("m_cms32", node_stride) : nullptr;
-        double* CMs = c->buf<double>("m_cms", node_stride * nsets);
-        launch_scale_sources(CM, tb.cs, tr.t.r, CMs, nn, p, nsets, 0, s);
+        const int nk = rdim(p);
+        const size_t red_stride = (size_t)nn * nk;
+        double* pbuf = f32 ? nullptr : c->buf<double>("m_pbuf", (size_t)np * nk);
+        float* pbuf32 = f32 ? c->buf<float>("m_pbuf32", (size_t)np * nk) : nullptr;
+        float* cms32 = f32 ? c->buf<float>("m_cms32", red_stride) : nullptr;
+        double* CMs = c->buf<double>("m_cms", red_stride * nsets);
+        for (int set = 0; set < nsets; ++set)
+            launch_scale_sources_red(CM + set * node_stride, tb.cs, tr.t.r, CMs + set * red_stride, nn, p, 1, 1, s);
         for (int set = 0; set < nsets; ++set) {
-            if (f32) launch_to_f32(CMs + set * node_stride, cms32, (int64_t)node_stride, s);
-            M2LGemmArgs ga{class_off, tile_off, ngrp, pair_csr, L.m2l_src, class_dl, Tc, CMs + set * node_stride,
+            if (f32) launch_to_f32(CMs + set * red_stride, cms32, (int64_t)red_stride, s);
+            M2LGemmArgs ga{class_off, tile_off, ngrp, pair_csr, L.m2l_src, class_dl, Tc, CMs + set * red_stride,
                            tb.gemm_fidx, tb.gemm_tdeg, pbuf, p, f32 ? 1 : 0, cms32, pbuf32, grp_class, grp_order,
-                           order};
+                           order, tb.gemm_tfull};
             launch_m2l_gemm(ga, grid, s);
             launch_m2l_reduce(ga, L.m2l_off, tr.t.r, LE + set * node_stride, nn, s);
+            launch_le_unfold(LE + set * node_stride, nn, p, s);
         }
         return;
     }
@@ -2096,20 +2119,20 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         double fl = 0.0, cnt = 0.0, qs = 0.0;
         for (int q = 0; q <= PMAX; ++q) {
             c->sel_hist[q] = hh[q];
-            fl += (double)hh[q] * 2.0 * ncomp(q) * ncomp(q) * S;
+            fl += (double)hh[q] * 2.0 * rdim(q) * rdim(q) * S;
             cnt += (double)hh[q];
             qs += (double)hh[q] * q;
         }
         c->sel_stats[2] = ht[2];
-        c->sel_stats[3] = c->sel_stats[1] > 0.0 ? (double)L.n_m2l * 2.0 * nm * nm * S : fl;
-        c->sel_stats[4] = (double)L.n_m2l * 2.0 * nm * nm * S;
+        c->sel_stats[3] = c->sel_stats[1] > 0.0 ? (double)L.n_m2l * 2.0 * rdim(p) * rdim(p) * S : fl;
+        c->sel_stats[4] = (double)L.n_m2l * 2.0 * rdim(p) * rdim(p) * S;
         c->sel_stats[5] = cnt > 0 ? qs / cnt : 0.0;
         c->sel_stats[6] = 1.0;
         c->sel_stats[7] = std::sqrt(ht[0] / n);
     } else {
         for (int q = 0; q <= PMAX; ++q) c->sel_hist[q] = 0;
         c->sel_stats[2] = 0.0;
-        c->sel_stats[3] = c->sel_stats[4] = (double)L.n_m2l * 2.0 * nm * nm * S;
+        c->sel_stats[3] = c->sel_stats[4] = (double)L.n_m2l * 2.0 * rdim(p) * rdim(p) * S;
         c->sel_stats[5] = p;
         c->sel_stats[6] = 0.0;
         c->sel_stats[7] = 0.0;

@@ -33,6 +33,8 @@ constexpr int KMAX_NEAR = 6;     // largest near-field tensor order (damped tens
 enum Damp { DAMP_NONE = 0, DAMP_THOLE = 1, DAMP_AMOEBA = 2, DAMP_ERF = 3 };
 
 __host__ __device__ constexpr int ncomp(int k) { return (k + 1) * (k + 2) * (k + 3) / 6; }
+// Harmonic (traceless) components of degree <= q: those with z <= 1, 2n + 1 of degree n.
+__host__ __device__ constexpr int rdim(int q) { return (q + 1) * (q + 1); }
 __host__ __device__ constexpr int ndeg(int k) { return (k + 1) * (k + 2) / 2; }
 __host__ __device__ constexpr int xyz2idx(int x, int y, int z) {
     return (x + y + z) * (x + y + z + 1) * (x + y + z + 2) / 6 + (y * y + 2 * y * z + y + z * z + 3 * z) / 2;

@@ -299,22 +299,24 @@ struct M2LGemmArgs {
     const int* src;                      // CSR position -> source node
     const int* class_dl;                 // class -> level_i - level_j
     const double* Tc;                    // class tensors, ncomp(2p) each
-    const double* CMs;                   // scaled source expansions of one set, [node][nm]
-    const int* fidx;                     // [MT][KS][64]: packed index of s + t per A-fragment lane, -1 outside
-    const int* tdeg;                     // [16 MT]: |t| of each target row, 0 past nm
-    double* pbuf;                        // [CSR position][nm] per-pair partial local coefficients
+    const double* CMs;                   // folded scaled source expansions of one set, [node][(p + 1)^2]
+    const int* fidx;                     // [MT][KS][64]: packed index of s + t per A-fragment lane (s, t reduced
+                                         //   components), -1 outside
+    const int* tdeg;                     // [16 MT]: |t| of each reduced target row, 0 past (p + 1)^2
+    double* pbuf;                        // [CSR position][(p + 1)^2] per-pair partial local coefficients
     int p;
     int f32;                             // mixed precision: FP32 MFMA on CMs32, slots in pbuf32
     const float* CMs32;
     float* pbuf32;
     // Per-pair expansion orders (learned order selection): tiles are then
     // runs of one (class, order) group; grp_class / grp_order give each
-    // group's class and order, and only the leading ncomp(order) rows and
+    // group's class and order, and only the leading rdim(order) rows and
     // columns of the Hankel matrix are applied.  order[CSR position] tells the
     // reduction how many coefficients a slot holds.  All null: every pair at p.
     const int* grp_class;
     const int* grp_order;
     const unsigned char* order;
+    const int* tfull;                    // [(p + 1)^2]: packed index of each reduced component
 };
 // Target-block M2L (one set, FP64, every pair at order p <= M2LB_PMAX): pairs grouped by blocks of
 // M2LB_BT targets in (level, octant, id) order and by class inside a block, in tiles of up to 16 pairs;
@@ -349,10 +351,12 @@ void launch_m2lb_tiles(const M2LBlockArgs& a, hipStream_t s);
 struct M2LBRunArgs {
     const int* toff; const int* pbase; const int* rec; int nbm;
     const int* ntg; const int* tperm; const int* tinv; int nn;
-    const double* CMs;                   // scaled source expansions of one set, [node][nm] (+ slack)
+    const double* CMs;                   // folded scaled source expansions of one set, [node][(p + 1)^2]
+                                         //   (+ slack)
     const double* Tc;                    // class tensors (+ M2LB_READ_SLACK doubles)
-    const int* fidx; const int* tdeg;    // the class-grouped GEMM's fragment tables
-    double* partial;                     // [nbm + grid][nm][M2LB_BT]
+    const int* fidx; const int* tdeg;    // the class-grouped GEMM's fragment tables (reduced components)
+    const int* tfull;                    //   and the packed index of each reduced component
+    double* partial;                     // [nbm + grid][(p + 1)^2][M2LB_BT]
     int grid;                            // workgroups of the run kernel
 };
 // Run kernel, per-block sums (scaled by r^{-(|t|+1)}) into LE, and zeros for nodes without far pairs.

@@ -560,8 +560,6 @@ void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s) {
 // The Hankel matrix shrinks from ncomp(p)^2 to (p + 1)^4 entries (220^2 ->
 // 100^2 at p = 9); the class tensors stay whole.  Reduced components are
 // ordered by degree, then w = y + z, then z, as the packed order runs.
-__host__ __device__ constexpr int rdim(int q) { return (q + 1) * (q + 1); }
-
 // Reduced component r -> degree | w << 8 | z << 16 (the packing of decode_target).
 __device__ __forceinline__ int decode_reduced(int r) {
     int k = 0;
@@ -612,24 +610,34 @@ __global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restr
     CMs[x] = acc * f;
 }
 
-// Local-expansion components with t_z >= 2 from the reduced ones the many-sets M2L wrote
-// (LE[set][node][ncomp(p)], in place: only components with t_z <= 1 are read).
+// Local-expansion components with t_z >= 2 from the reduced ones the M2L wrote
+// (LE[row][ncomp(p)], rows = sets x nodes, in place: only components with
+// t_z <= 1 are read).  One thread per (row, component with t_z >= 2).
 __global__ __launch_bounds__(256) void k_le_unfold(double* __restrict__ LE, int64_t rows, int p) {
-    const int nm = ncomp(p);
+    const int nm = ncomp(p), nz = nm - rdim(p);   // components with t_z >= 2
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= rows * nm) return;
-    const int t = (int)(x % nm);
-    const int d = decode_target(t);
-    const int deg = d & 255, w = (d >> 8) & 255, tz = d >> 16;
-    if (tz < 2) return;
-    const int tx = deg - w, ty = w - tz, k = tz >> 1, rho = tz & 1;
-    const double* __restrict__ row = LE + (x - t);
+    if (x >= rows * nz) return;
+    const int64_t rw = x / nz;
+    int l = (int)(x - rw * nz);
+    // degree n holds (n - 1) n / 2 of them: (w, z) for 2 <= z <= w <= n, in packed order
+    int n = 2;
+    while (l >= (n - 1) * n / 2) {
+        l -= (n - 1) * n / 2;
+        ++n;
+    }
+    int w = 2;
+    while (l >= w - 1) {
+        l -= w - 1;
+        ++w;
+    }
+    const int tz = 2 + l, tx = n - w, ty = w - tz, k = tz >> 1, rho = tz & 1;
+    double* __restrict__ row = LE + rw * nm;
     double acc = 0.0, binom = 1.0;
     for (int i = 0; i <= k; ++i) {
         if (i > 0) binom = binom * (k - i + 1) / i;
         acc += binom * row[xyz2idx(tx + 2 * i, ty + 2 * (k - i), rho)];
     }
-    LE[x] = (k & 1) ? -acc : acc;
+    row[xyz2idx(tx, ty, tz)] = (k & 1) ? -acc : acc;
 }
 
 void launch_scale_sources_red(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
@@ -641,7 +649,7 @@ void launch_scale_sources_red(const double* CM, const double* cs, const double* 
 
 void launch_le_unfold(double* LE, long long rows, int p, hipStream_t s) {
     if (rows <= 0 || p < 2) return;
-    hipLaunchKernelGGL(k_le_unfold, dim3(nblk(rows * ncomp(p), 256)), dim3(256), 0, s, LE, (int64_t)rows, p);
+    hipLaunchKernelGGL(k_le_unfold, dim3(nblk(rows * (ncomp(p) - rdim(p)), 256)), dim3(256), 0, s, LE, (int64_t)rows, p);
     FMMH_HIP(hipGetLastError());
 }
 
@@ -728,7 +736,6 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr bool STAGE = STG && mfma_stage_b<P, F32, NTL>();
     constexpr int BW = 16 * NTL;            // sets per workgroup
     constexpr int NBL = STAGE ? (NK * BW + NTH - 1) / NTH : 1;
-    constexpr int NGD = 2 * P + 1;
     using TE = typename std::conditional<F32, float, double>::type;
     using Acc = typename std::conditional<F32, v4f, v4d>::type;
     constexpr int ES = sizeof(TE);
@@ -738,9 +745,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     constexpr int CPR = BW / CH;            // chunks per staged source row
     __shared__ __align__(16) TE Tl[2][NTP];
     __shared__ __align__(16) TE Bl[2][BROWS];   // staged source block [s][set]
-    __shared__ int sdec[4 * (KSP + 8)];     // source s: 4 (|s| NGD + w_s) | ES z_s << 16 (+ read-ahead)
-    __shared__ int gtab[NGD * NGD];         // ES (tet(d) + tri(w)) at d NGD + w
-    __shared__ unsigned char degl[16 * MT]; // degree of target row t (99 past n_m)
+    __shared__ unsigned char degl[16 * MT]; // degree of target row t (99 past the reduced components)
     const int i = blockIdx.x;
     const int sg = blockIdx.y * BW;         // first set of this workgroup
     const int tid = threadIdx.x;
@@ -748,21 +753,24 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     const int row = lane & 15, kq = lane >> 4;
     const int e0 = a.off[i], e1 = a.off[i + 1];
     const TE* __restrict__ cms = F32 ? (const TE*)a.CMs32 : (const TE*)a.CMs;
-    for (int x = tid; x < 4 * (KSP + 8); x += NTH) {
-        const int d = x < NK ? decode_reduced(x) : 0;
-        sdec[x] = 4 * ((d & 255) * NGD + ((d >> 8) & 255)) | (ES * (d >> 16)) << 16;
-    }
     for (int x = tid; x < 16 * MT; x += NTH) degl[x] = x < NK ? (decode_reduced(x) & 255) : 99;
-    for (int x = tid; x < NGD * NGD; x += NTH) gtab[x] = ES * (tet(x / NGD) + tri(x % NGD));
-    // A fragment of row tile mt holds H[t = 16 mt + row][s = 4 ks + kq]
-    int trow[MW], tz[MW], tdeg[MW];
+    // A fragment of row tile mt holds H[t = 16 mt + row][s = 4 ks + kq] = T~_c[s + t]: its byte offset in
+    // a class-tensor buffer, per lane and k-step, from the host-built fragment table (kept in registers
+    // for the whole target; outside the matrix an offset of 0 reads a finite element that is either
+    // never stored (rows) or multiplies zero-filled source rows (columns))
+    constexpr int KS = (NK + 3) / 4;        // k-steps of the fragment table
+    int fo[MW][KSP], tdeg[MW];
 #pragma unroll
     for (int mi = 0; mi < MW; ++mi) {
-        const int t = (w + NW * mi) * 16 + row;
-        const int d = t < NK ? decode_reduced(t) : 0;
-        trow[mi] = 4 * ((d & 255) * NGD + ((d >> 8) & 255));
-        tz[mi] = ES * (d >> 16);
-        tdeg[mi] = t < NK ? (d & 255) : 99;
+        const int mt = w + NW * mi;
+        const int t = mt * 16 + row;
+        tdeg[mi] = t < NK ? (decode_reduced(t) & 255) : 99;
+#pragma unroll
+        for (int ks = 0; ks < KSP; ++ks) {
+            const int f = (mt < MT && ks < KS) ? a.fidx[(mt * KS + ks) * 64 + lane] : -1;
+            fo[mi][ks] = ES * (f >= 0 ? f : 0);
+            asm volatile("" : "+v"(fo[mi][ks]));   // opaque: kept in a register, not re-loaded per pair
+        }
     }
     v4d acc[MW][NTL];
 #pragma unroll
@@ -833,7 +841,6 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
         publish(0);
     }
     __syncthreads();
-    const char* __restrict__ gt = (const char*)gtab;
     for (int e = e0; e < e1; ++e) {
         const int cur = (e - e0) & 1;
         const bool more = e + 1 < e1;
@@ -858,60 +865,44 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi) sh[mi] = tdeg[mi] <= qo ? __mul24(sa, tdeg[mi]) + sb : -2000;
         }
-        int tzc[MW];
-#pragma unroll
-        for (int mi = 0; mi < MW; ++mi) tzc[mi] = tz[mi] + cur * NTP * ES;
-        const char* __restrict__ T = (const char*)&Tl[0][0];
+        const char* __restrict__ T = (const char*)&Tl[cur][0];
         const TE* __restrict__ Bc = Bl[cur];
         const int64_t boff = (int64_t)j * NK * a.s_ld + sg + row;
         Acc part[MW][NTL];
-        // Software pipeline over step pairs (k0, k0 + 1): while the MFMAs of pair k0 run, the
-        // tensor elements and source rows of pair k0 + 2, the G lookups of pair k0 + 4 and the
-        // source decodes of pair k0 + 6 are in flight (LDS returns in order, so each wait only
-        // covers loads issued an iteration earlier).  Offsets are byte offsets into Tl.
-        auto decode = [&](int k, int (&d)[2]) {
-#pragma unroll
-            for (int u = 0; u < 2; ++u) d[u] = sdec[(k + u) * 4 + kq];
-        };
-        auto lookup = [&](const int (&d)[2], int (&o)[2][MW]) {
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int mi = 0; mi < MW; ++mi) o[u][mi] = *(const int*)(gt + trow[mi] + (d[u] & 0xffff));
-        };
-        auto finish = [&](const int (&d)[2], int (&o)[2][MW]) {
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int mi = 0; mi < MW; ++mi) o[u][mi] += tzc[mi] + (d[u] >> 16);
-        };
-        auto loadA = [&](const int (&o)[2][MW], TE (&x)[2][MW]) {
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int mi = 0; mi < MW; ++mi) x[u][mi] = *(const TE*)(T + o[u][mi]);
-        };
-        auto loadB = [&](int k, TE (&x)[2][NTL]) {
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int s = (k + u) * 4 + kq;
-#pragma unroll
-                for (int nt = 0; nt < NTL; ++nt) {
-                    if constexpr (STAGE) {
-                        x[u][nt] = Bc[s * BW + 16 * nt + row];
-                    } else {
-                        const bool sv = s < nq;
-                        const TE v = cms[boff + (int64_t)(sv ? s : 0) * a.s_ld + 16 * nt];
-                        x[u][nt] = sv ? v : TE(0);
-                    }
-                }
-            }
-        };
         // row tiles wholly past the pair's order are skipped (a wave-uniform test)
         bool tile_on[MW];
 #pragma unroll
         for (int mi = 0; mi < MW; ++mi) tile_on[mi] = (w + NW * mi) * 16 < nq;
-        auto multiply = [&](const TE (&x)[2][MW], const TE (&y)[2][NTL], auto first) {
+        // Software pipeline over step pairs (k0, k0 + 1), unrolled over the padded k-steps: the
+        // tensor elements and source rows of step pair k0 + 2 are loaded while the MFMAs of k0 run
+        // (two register sets in turn).  Loads past the pair's k-steps stay inside the buffers and
+        // are never multiplied.
+        TE av[2][2][MW], bv[2][2][NTL];
+        auto load = [&](int k0, int sl) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+#pragma unroll
+                for (int mi = 0; mi < MW; ++mi) av[sl][u][mi] = *(const TE*)(T + fo[mi][k0 + u]);
+                const int s = (k0 + u) * 4 + kq;
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt) {
+                    if constexpr (STAGE) {
+                        bv[sl][u][nt] = Bc[s * BW + 16 * nt + row];
+                    } else {
+                        const bool sv = s < nq;
+                        const TE v = cms[boff + (int64_t)(sv ? s : 0) * a.s_ld + 16 * nt];
+                        bv[sl][u][nt] = sv ? v : TE(0);
+                    }
+                }
+            }
+        };
+        load(0, 0);
+#pragma unroll
+        for (int k0 = 0; k0 < KSP; k0 += 2) {
+            if (k0 >= KSq) continue;   // (a wave-uniform skip of the rest: no break, so the loop unrolls)
+            const int sl = (k0 >> 1) & 1;
+            if (k0 + 2 < KSP) load(k0 + 2, 1 - sl);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -919,71 +910,18 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
                     if (!tile_on[mi]) continue;
                     if constexpr (F32) {
 #pragma unroll
-                        for (int nt = 0; nt < NTL; ++nt) {
-                            if (decltype(first)::value && u == 0)
-                                part[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u][mi], y[u][nt], Acc{0, 0, 0, 0}, 0, 0, 0);
-                            else
-                                part[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u][mi], y[u][nt], part[mi][nt], 0, 0, 0);
-                        }
+                        for (int nt = 0; nt < NTL; ++nt)
+                            part[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                av[sl][u][mi], bv[sl][u][nt], (k0 == 0 && u == 0) ? Acc{0, 0, 0, 0} : part[mi][nt], 0,
+                                0, 0);
                     } else {
-                        const double av = ldexp(x[u][mi], sh[mi]);
+                        const double x = ldexp(av[sl][u][mi], sh[mi]);
 #pragma unroll
                         for (int nt = 0; nt < NTL; ++nt)
-                            acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, y[u][nt], acc[mi][nt], 0, 0, 0);
+                            acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, bv[sl][u][nt], acc[mi][nt], 0, 0, 0);
                     }
                 }
-        };
-        // two register sets of pipeline state, used in turn (no copies between iterations)
-        TE a1[2][MW], b1[2][NTL], a2[2][MW], b2[2][NTL];
-        int g1[2][MW], d1[2], g2[2][MW], d2[2];
-        {
-            int d[2];
-            decode(0, d);
-            lookup(d, g1);
-            finish(d, g1);
-            loadA(g1, a1);
-            loadB(0, b1);
-            decode(2, d);
-            lookup(d, g1);
-            finish(d, g1);
-            decode(4, d1);
-        }
-        // state (A, Bv: pair k0; G: offsets of k0 + 2; D: decodes of k0 + 4) -> the same for k0 + 2
-        auto iteration = [&](int k0, auto first, const TE (&A)[2][MW], const TE (&Bv)[2][NTL], const int (&G)[2][MW],
-                             const int (&D)[2], TE (&An)[2][MW], TE (&Bn)[2][NTL], int (&Gn)[2][MW], int (&Dn)[2]) {
-            loadA(G, An);
-            loadB(k0 + 2, Bn);
-            lookup(D, Gn);
-            decode(k0 + 6, Dn);
             __builtin_amdgcn_sched_barrier(0);
-            multiply(A, Bv, first);
-            __builtin_amdgcn_sched_barrier(0);
-            finish(D, Gn);
-        };
-        iteration(0, std::true_type{}, a1, b1, g1, d1, a2, b2, g2, d2);
-        if constexpr (F32) {
-            for (int k0 = 2; k0 < KSq; k0 += 4) {
-                iteration(k0, std::false_type{}, a2, b2, g2, d2, a1, b1, g1, d1);
-                if (k0 + 2 >= KSq) break;
-                iteration(k0 + 2, std::false_type{}, a1, b1, g1, d1, a2, b2, g2, d2);
-            }
-        } else {
-            // FP64: the accumulators stay in place across pairs; one set of state, copied
-            // (unrolling by two doubles the accumulator registers here)
-            for (int k0 = 2; k0 < KSq; k0 += 2) {
-                iteration(k0, std::false_type{}, a2, b2, g2, d2, a1, b1, g1, d1);
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    d2[u] = d1[u];
-#pragma unroll
-                    for (int mi = 0; mi < MW; ++mi) {
-                        a2[u][mi] = a1[u][mi];
-                        g2[u][mi] = g1[u][mi];
-                    }
-#pragma unroll
-                    for (int nt = 0; nt < NTL; ++nt) b2[u][nt] = b1[u][nt];
-                }
-            }
         }
         if constexpr (F32) {
             // this pair's FP32 tiles, row-scaled, into the FP64 accumulators (the f32 layout keeps

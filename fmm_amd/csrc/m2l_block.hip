@@ -13,8 +13,12 @@
 // tiles and accumulates the block's local expansions in LDS: per tile, the
 // class tensor and the 16 scaled source expansions are gathered (one tile
 // ahead, into registers), each wavefront runs RT 16-row target tiles of
-// v_mfma_f64_16x16x4f64 over the 84 (p = 6) source coefficients and adds its
-// 16 x 16 result to the LDS rows of the tile's targets.  Where a workgroup's
+// v_mfma_f64_16x16x4f64 over the 49 (p = 6) reduced source coefficients and
+// adds its 16 x 16 result to the LDS rows of the tile's targets.  Rows and
+// source coefficients are the reduced (harmonic) components -- the (p + 1)^2
+// with z <= 1, sources folded onto them by the Laplace identity (m2l.hip) --
+// instead of the ncomp(p) = 84 packed ones; the local components with
+// t_z >= 2 are rebuilt after the finish kernel (launch_le_unfold).  Where a workgroup's
 // share leaves a block, the LDS block is written to the block's partial slot
 // (block + workgroup), so a block split over several workgroups has one
 // partial per workgroup; k_m2lb_finish adds them in workgroup order, scales
@@ -246,7 +250,7 @@ __device__ __forceinline__ int share_index(int g, int G) { return (g % NUM_XCD) 
 
 template <int P, int RT>
 struct M2LBShape {
-    static constexpr int NM = ncomp(P);
+    static constexpr int NM = rdim(P);              // reduced components (rows and source coefficients)
     static constexpr int NT = ncomp(2 * P);
     static constexpr int MT = (NM + 15) / 16;       // 16-row target tiles
     static constexpr int KS = (NM + 3) / 4;         // 4-deep source steps
@@ -478,11 +482,12 @@ __global__ __launch_bounds__((M2LBShape<P, RT>::NTHR)) void k_m2lb_run(M2LBRunAr
     }
 }
 
-// L_i(t) = r_i^{-(|t|+1)} x (the block's partials, in share order); one thread per (block, row, target).
+// L_i(t) = r_i^{-(|t|+1)} x (the block's partials, in share order); one thread per (block, reduced row,
+// target), written at the row's packed index of the nmf-long local expansion.
 constexpr int FN_T = 256;
 template <int NM>
 __global__ __launch_bounds__(FN_T) void k_m2lb_finish(M2LBRunArgs a, const double* __restrict__ r,
-                                                      double* __restrict__ LE) {
+                                                      double* __restrict__ LE, int nmf) {
     constexpr int BT = M2LB_BT;
     const int64_t x = (int64_t)blockIdx.x * FN_T + threadIdx.x;
     const int b = (int)(x / (NM * BT));
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(FN_T) void k_m2lb_finish(M2LBRunArgs a, const doubl
     const double rinv = 1.0 / r[i];
     double f = rinv;
     for (int d = 0; d < a.tdeg[t]; ++d) f *= rinv;
-    LE[(int64_t)i * NM + t] = s * f;
+    LE[(int64_t)i * nmf + a.tfull[t]] = s * f;
 }
 
 // Nodes without far pairs: zero local expansions (as the per-pair reduction writes them).
@@ -524,13 +529,15 @@ void launch_m2lb_tiles(const M2LBlockArgs& a, hipStream_t s) {
 
 template <int P>
 static void run_p(const M2LBRunArgs& a, const double* r, double* LE, hipStream_t s) {
-    constexpr int RT = 3;
+    constexpr int MT = (rdim(P) + 15) / 16;
+    constexpr int RT = MT >= 3 ? 2 : 1;
     using S = M2LBShape<P, RT>;
     hipLaunchKernelGGL((k_m2lb_run<P, RT>), dim3(a.grid), dim3(S::NTHR), 0, s, a);
     hipLaunchKernelGGL((k_m2lb_finish<S::NM>), dim3(nblk((long long)a.nbm * S::NM * M2LB_BT, FN_T)), dim3(FN_T), 0, s,
-                       a, r, LE);
-    hipLaunchKernelGGL(k_m2lb_zero, dim3(nblk((long long)a.nn * S::NM, 256)), dim3(256), 0, s, a.tinv, a.nn, S::NM,
-                       LE);
+                       a, r, LE, ncomp(P));
+    hipLaunchKernelGGL(k_m2lb_zero, dim3(nblk((long long)a.nn * ncomp(P), 256)), dim3(256), 0, s, a.tinv, a.nn,
+                       ncomp(P), LE);
+    launch_le_unfold(LE, a.nn, P, s);
 }
 
 void launch_m2lb_run(const M2LBRunArgs& a, int p, const double* r, double* LE, hipStream_t s) {

@@ -12,9 +12,13 @@
 // host-built index table (packed index of s + t per fragment lane); the
 // level-difference factor of the class is a per-row scale.
 //
-// Results are written per pair to its CSR slot, and k_m2l_reduce sums each
-// target's slots in CSR order, so the sum order is fixed (deterministic) and
-// independent of the tiling.
+// The Hankel matrix runs over the reduced (harmonic) components only -- the
+// (p + 1)^2 with z <= 1, sources folded onto them (m2l.hip, the Laplace
+// identity T_{k+2x} + T_{k+2y} + T_{k+2z} = 0): 100 x 100 instead of 220 x
+// 220 at p = 9.  Results are written per pair to its CSR slot, and
+// k_m2l_reduce sums each target's slots in CSR order, so the sum order is
+// fixed (deterministic) and independent of the tiling; the local components
+// with t_z >= 2 are rebuilt after it (launch_le_unfold).
 //
 // Mixed precision (F32): the same tiles on v_mfma_f32_16x16x4f32 with FP32
 // sources and FP32 accumulation over s; the class tensor stays FP64 in LDS
@@ -75,14 +79,14 @@ template <int P, bool F32>
 __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
     using G = GemmT<F32>;
     using E = typename G::E;
-    constexpr int NM = ncomp(P);
+    constexpr int NK = rdim(P);                 // reduced components: rows and k of the GEMM
     constexpr int NT = ncomp(2 * P);
-    constexpr int MT = (NM + 15) / 16;          // 16-row target tiles
-    constexpr int KS = (NM + 3) / 4;            // 4-deep source steps
+    constexpr int MT = (NK + 15) / 16;          // 16-row target tiles
+    constexpr int KS = (NK + 3) / 4;            // 4-deep source steps
     constexpr int MW = (MT + 3) / 4;            // target tiles per wavefront
     constexpr int TN = M2L_GEMM_TN;
     constexpr int NTL = TN / 16;
-    constexpr int BLD = NM | 1;                 // odd LDS row stride of the source tile
+    constexpr int BLD = NK | 1;                 // odd LDS row stride of the source tile
     __shared__ double Ts[NT];
     __shared__ E Bs[TN * BLD];
     __shared__ int ecsr[TN];
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
     const int c = lo;                           // class, or (class, order) group
     const int cls = a.grp_class ? a.grp_class[c] : c;
     const int q = a.grp_order ? a.grp_order[c] : P;
-    const int nq = ncomp(q);                    // coefficients applied for this tile's pairs
+    const int nq = rdim(q);                     // coefficients applied for this tile's pairs
     const int MTq = (nq + 15) / 16, KSq = (nq + 3) / 4;
     const int k0 = a.class_off[c] + (b - a.tile_off[c]) * TN;
     const int cnt = min(TN, a.class_off[c + 1] - k0);
@@ -114,10 +118,10 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
     }
     __syncthreads();
     const E* __restrict__ cms = F32 ? (const E*)a.CMs32 : (const E*)a.CMs;
-    for (int x = tid; x < TN * NM; x += 256) {
-        const int n = x / NM, s = x - n * NM;
+    for (int x = tid; x < TN * NK; x += 256) {
+        const int n = x / NK, s = x - n * NK;
         const int j = jsrc[n];
-        Bs[n * BLD + s] = (j >= 0 && s < nq) ? cms[(int64_t)j * NM + s] : E(0);
+        Bs[n * BLD + s] = (j >= 0 && s < nq) ? cms[(int64_t)j * NK + s] : E(0);
     }
     __syncthreads();
     const int w = tid >> 6, lane = tid & 63;
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
             const int s = ks * 4 + g4;
             E bv[NTL];
 #pragma unroll
-            for (int nt = 0; nt < NTL; ++nt) bv[nt] = s < NM ? Bs[(nt * 16 + c16) * BLD + s] : E(0);
+            for (int nt = 0; nt < NTL; ++nt) bv[nt] = s < NK ? Bs[(nt * 16 + c16) * BLD + s] : E(0);
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi) {
                 const int mt = w + 4 * mi;
@@ -175,7 +179,7 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
         for (int nt = 0; nt < NTL; ++nt) {
             const int n = nt * 16 + c16;
             if (n >= cnt) continue;
-            E* __restrict__ out = (F32 ? (E*)a.pbuf32 : (E*)a.pbuf) + (int64_t)ecsr[n] * NM;
+            E* __restrict__ out = (F32 ? (E*)a.pbuf32 : (E*)a.pbuf) + (int64_t)ecsr[n] * NK;
 #pragma unroll
             for (int r4 = 0; r4 < 4; ++r4) {
                 const int t = mt * 16 + G::row(r4, g4);
@@ -186,38 +190,41 @@ __global__ __launch_bounds__(256) void k_m2l_gemm(M2LGemmArgs a) {
 }
 
 // L_i(t) = r_i^{-(|t|+1)} sum over the CSR slots of target i, in CSR order
-// (FP64 accumulation for either slot type).
-// A slot of a pair with a learned order q holds only its first ncomp(q) coefficients.
+// (FP64 accumulation for either slot type), for the reduced components t
+// (slots of nk = (p + 1)^2 coefficients; written at their packed index tfull[t]
+// of the ncomp(p)-long local expansion).
+// A slot of a pair with a learned order q holds only its first rdim(q) coefficients.
 template <class E>
 __global__ __launch_bounds__(128) void k_m2l_reduce(const E* __restrict__ pbuf, const int* __restrict__ off,
                                                     const double* __restrict__ r, const int* __restrict__ tdeg,
+                                                    const int* __restrict__ tfull,
                                                     const unsigned char* __restrict__ order,
-                                                    double* __restrict__ LE, int nm) {
+                                                    double* __restrict__ LE, int nk, int nm) {
     const int i = blockIdx.x;
     const int e0 = off[i], e1 = off[i + 1];
     const double rinv = 1.0 / r[i];
-    for (int t = threadIdx.x; t < nm; t += 128) {
+    for (int t = threadIdx.x; t < nk; t += 128) {
         double sum = 0.0;
         if (order) {
             const int dt = tdeg[t];
 #pragma unroll 8
             for (int e = e0; e < e1; ++e)
-                if (dt <= order[e]) sum += (double)pbuf[(int64_t)e * nm + t];
+                if (dt <= order[e]) sum += (double)pbuf[(int64_t)e * nk + t];
         } else {
             // 16 slots loaded per batch (the adds stay in CSR order)
             int e = e0;
             for (; e + 16 <= e1; e += 16) {
                 E v[16];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) v[u] = pbuf[(int64_t)(e + u) * nm + t];
+                for (int u = 0; u < 16; ++u) v[u] = pbuf[(int64_t)(e + u) * nk + t];
 #pragma unroll
                 for (int u = 0; u < 16; ++u) sum += (double)v[u];
             }
-            for (; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nm + t];
+            for (; e < e1; ++e) sum += (double)pbuf[(int64_t)e * nk + t];
         }
         double f = rinv;
         for (int k = 0; k < tdeg[t]; ++k) f *= rinv;
-        LE[(int64_t)i * nm + t] = sum * f;
+        LE[(int64_t)i * nm + tfull[t]] = sum * f;
     }
 }
 
@@ -258,13 +265,13 @@ void launch_m2l_gemm(const M2LGemmArgs& a, int grid, hipStream_t s) {
 void launch_m2l_reduce(const M2LGemmArgs& a, const int* off, const double* r, double* LE, int num_nodes,
                        hipStream_t s) {
     if (num_nodes <= 0) return;
-    const int nm = ncomp(a.p);
+    const int nm = ncomp(a.p), nk = rdim(a.p);
     if (a.f32)
-        hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(num_nodes), dim3(128), 0, s, a.pbuf32, off, r, a.tdeg, a.order,
-                           LE, nm);
+        hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(num_nodes), dim3(128), 0, s, a.pbuf32, off, r, a.tdeg, a.tfull,
+                           a.order, LE, nk, nm);
     else
-        hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(num_nodes), dim3(128), 0, s, a.pbuf, off, r, a.tdeg, a.order,
-                           LE, nm);
+        hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(num_nodes), dim3(128), 0, s, a.pbuf, off, r, a.tdeg, a.tfull,
+                           a.order, LE, nk, nm);
     FMMH_HIP(hipGetLastError());
 }
 

@@ -380,7 +380,8 @@ int fmmh_selector_predict(fmmh_context* ctx, const float* weights, const float* 
 /* Selection figures of the last field_fmm call: out[0] spot-check relative
  * error, the largest over the multipole sets (-1: not checked), [1] 1 if it fell
  * back to expansion_order, [2] per-pair tolerance tau, [3] M2L flops at the chosen
- * orders (2 n_m(q)^2 per pair and set), [4] M2L flops at expansion_order, [5] mean
+ * orders (2 (q + 1)^4 per pair and set: the harmonic form's Hankel matrix), [4] M2L
+ * flops at expansion_order, [5] mean
  * order, [6] 1 if selection ran, [7] RMS near field used for tau (the smallest
  * non-zero one over the sets), [8 + q] pairs at order q (q = 0..10), [19] the
  * context's tolerance scale for later calls (1; after a fallback whose full-order
