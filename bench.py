@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_near_water200k_thole_p6.md, valid for the near-field sources it
 # was measured on: K_NEAR_TRAFFIC_SRC is the sha256 prefix of those sources
 # (near_src_sha()); with other sources the bench line reports null.
-K_NEAR_TRAFFIC_BYTES = None
-K_NEAR_TRAFFIC_SRC = None
+K_NEAR_TRAFFIC_BYTES = 146452919     # 59.2 MB read + 87.3 MB written
+K_NEAR_TRAFFIC_SRC = "a16195073b28ddff"
 NEAR_SOURCES = ("fmm_amd/csrc/near_kernels.h", "fmm_amd/csrc/near.hip", "fmm_amd/csrc/tensors.h")
 
 

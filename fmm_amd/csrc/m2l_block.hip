@@ -530,7 +530,7 @@ void launch_m2lb_tiles(const M2LBlockArgs& a, hipStream_t s) {
 template <int P>
 static void run_p(const M2LBRunArgs& a, const double* r, double* LE, hipStream_t s) {
     constexpr int MT = (rdim(P) + 15) / 16;
-    constexpr int RT = MT >= 3 ? 2 : 1;
+    constexpr int RT = 1;
     using S = M2LBShape<P, RT>;
     hipLaunchKernelGGL((k_m2lb_run<P, RT>), dim3(a.grid), dim3(S::NTHR), 0, s, a);
     hipLaunchKernelGGL((k_m2lb_finish<S::NM>), dim3(nblk((long long)a.nbm * S::NM * M2LB_BT, FN_T)), dim3(FN_T), 0, s,
