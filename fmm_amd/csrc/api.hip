@@ -80,7 +80,8 @@ struct fmmh_context {
     std::vector<hipEvent_t> event_pool;
     size_t event_used = 0;
     std::vector<std::pair<std::string, double>> phase_ms;
-    int64_t stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t stats[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t host_waits = 0;           // host waits on the device during the current call
     int64_t stats_classes = 0;
     double list_cap_scale = 1.0;      // initial interaction-list capacity per node (tests force retries)
     double list_need[3] = {0, 0, 0};  // frontier capacity that sufficed, M2L and P2P entries (x 1.25), per node
@@ -205,7 +206,10 @@ struct fmmh_context {
             phase_ms.emplace_back(marks[i].first, (double)ms);
         }
     }
-    void sync() { FMMH_HIP(hipStreamSynchronize(stream)); }
+    void sync() {
+        ++host_waits;
+        FMMH_HIP(hipStreamSynchronize(stream));
+    }
     // An event the host waits on for a readback while later work stays queued behind it.
     hipEvent_t rb_event = nullptr;
     hipEvent_t readback_event() {
@@ -1302,6 +1306,7 @@ void build_lists(fmmh_context* c, TreeOut& tr, double theta, ListsOut& L, int ni
             c->mark("l_traverse");
             FMMH_HIP(hipEventRecord(c->readback_event(), s));
             overlap();
+            ++c->host_waits;
             FMMH_HIP(hipEventSynchronize(c->readback_event()));
         } else {
             c->sync();
@@ -1900,6 +1905,7 @@ void near_order_end(fmmh_context* c, TreeOut& tr, ListsOut& L) {
 }
 
 void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, const Periodic& P) {
+    c->host_waits = 0;
     hipStream_t s = c->stream;
     const double* disp = upload_disp(c, P);
     const int n = k.n;
@@ -2147,16 +2153,17 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
         hipLaunchKernelGGL(k_pair_count, dim3(nblk(nl, 256)), dim3(256), 0, s,
                            c->buf<long long>("s_leaf_tot", nl), tr.t.leaf_node, tr.t.count, nl, pc);
         FMMH_HIP(hipMemcpyAsync(c->h_pinned + 200, pc, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        c->sync();
-        std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
-    }
-    if (c->reuse_tree) {
-        if (!reuse) {
+        if (c->reuse_tree) {
             // keep a copy of the coordinates the cached tree was built from
             double* cx = c->buf<double>("c_X", (size_t)n * 3);
             FMMH_HIP(hipMemcpy2DAsync(cx, n * sizeof(double), k.X, k.ldx * sizeof(double), n * sizeof(double), 3,
                                       hipMemcpyDeviceToDevice, s));
-            c->sync();
+        }
+        c->sync();   // the call's completion; the pair count and the coordinate copy ride on it
+        std::memcpy(&pairs, c->h_pinned + 200, sizeof(pairs));
+    }
+    if (c->reuse_tree) {
+        if (!reuse) {
             c->cache_tr = tr;
             c->cache_L = L;
             c->cache_n = n; c->cache_ncrit = ncrit; c->cache_theta = theta;
@@ -2176,6 +2183,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     c->stats[8] = replay ? 2 : (capture ? 1 : 0);
     c->stats[9] = c->graph_replays;
     c->stats[10] = c->near_nv;
+    c->stats[11] = c->host_waits;   // up to here; the field readback of a host-array call adds one
 }
 
 // ---------------------------------------------------------------- host <-> device staging
@@ -2841,7 +2849,7 @@ int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t b
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap) {
     return guarded([&] {
         fmmh_context* c = resolve(ctx);
-        for (int i = 0; i < std::min(cap, 11); ++i) out[i] = c->stats[i];
+        for (int i = 0; i < std::min(cap, 12); ++i) out[i] = c->stats[i];
     });
 }
 

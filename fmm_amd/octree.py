@@ -446,10 +446,10 @@ class Context:
                     full_order_spot_error=float(out[20]))
 
     def stats(self):
-        out = np.zeros(11, dtype=np.int64)
-        check(self._lib.fmmh_stats(self._h, _p(out), 11))
+        out = np.zeros(12, dtype=np.int64)
+        check(self._lib.fmmh_stats(self._h, _p(out), 12))
         keys = ("num_nodes", "num_leaves", "max_depth", "n_m2l", "n_p2p", "p2p_site_pairs", "m2l_classes",
-                "list_retries", "graph", "graph_replays", "near_workgroups")
+                "list_retries", "graph", "graph_replays", "near_workgroups", "host_waits")
         return dict(zip(keys, (int(v) for v in out)))
 
     def debug_buffer(self, name, count, dtype=np.float64):

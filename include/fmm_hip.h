@@ -332,7 +332,9 @@ int fmmh_debug_buffer(fmmh_context* ctx, const char* name, void* host, int64_t b
 /* Counters of the last field_fmm call: {num_nodes, num_leaves, max_depth,
  * n_m2l, n_p2p, p2p_site_pairs, m2l_classes, list_retries, graph (0: eager,
  * 1: captured and launched, 2: replayed; FMMH_OPT_GRAPH), graph replays so far,
- * near-field workgroups (leaves plus extra chunks; FMMH_OPT_NEAR_CHUNK)}. */
+ * near-field workgroups (leaves plus extra chunks; FMMH_OPT_NEAR_CHUNK),
+ * host waits on the device during the call (stream or event synchronisations
+ * before the result is complete; the readback of a host-array call is one more)}. */
 int fmmh_stats(fmmh_context* ctx, int64_t* out, int32_t cap);
 
 /* ---- learned per-pair expansion orders (no counterpart in the reference) ------

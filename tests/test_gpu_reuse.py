@@ -98,3 +98,21 @@ def test_reuse_with_sets_and_device_calls(ctx):
         outs.append(out.cpu().numpy())
     assert "tree_reused" in ctx.phase_times()
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_host_waits_per_call(ctx):
+    """Host waits on the device inside one field_fmm call (stats host_waits): a call that builds its tree
+    waits five times -- the node count (to size the node arrays), the tree's depth and level sizes, the
+    interaction-list sizes, the near-field workgroup order and M2L class counts (one round trip), and the
+    call's completion; a call that reuses the tree waits for the coordinate comparison and its completion."""
+    g = load_golden("water999_q.npz")
+    x, e, df = g["coordinates"], g["exclusions"], g["thole_df"]
+    m = np.asfortranarray(synth.random_multipoles(np.random.default_rng(9), x.shape[0], 4))
+    ctx.set_tree_reuse(True)
+    ctx.field_fmm(0, x, m, e, 0.5, 64, 6, 2, "THOLE", df)
+    built = ctx.stats()["host_waits"]
+    ctx.field_fmm(0, x, m, e, 0.5, 64, 6, 2, "THOLE", df)
+    reused = ctx.stats()["host_waits"]
+    print(f"host waits: tree built {built}, tree reused {reused}")
+    assert built == 5
+    assert reused == 2
