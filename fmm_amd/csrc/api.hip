@@ -1815,7 +1815,7 @@ double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, c
                   const double* ms, int n, int m, int nf, const double* near, const Call& k, const double* before,
                   int nspots) {
     hipStream_t s = c->stream;
-    nspots = std::min(nspots, tr.num_leaves);
+    nspots = std::min(nspots, n);
     // source chunks: about four workgroups per compute unit in all
     const int tile = spot_tile(m);
     const int sets_blocks = (k.nsets + 31) / 32;

@@ -624,8 +624,10 @@ void launch_sel_budget(const double* near, int64_t ldn, int n, int nf, int nsets
 // ------------------------------------------------------------------ spot checks
 // Far-field part of the direct sum at spot sites, with the reference's far
 // semantics: every source whose leaf is not in the P2P list of the site's
-// leaf, undamped and without exclusions.  The spot sites are the first sites
-// of evenly spaced leaves.
+// leaf, undamped and without exclusions.  The spot sites are evenly spaced in
+// the sorted (octree) site order, spot k at sorted site (2k + 1) n / (2 nspots):
+// a systematic sample over all sites, spread over the box, so that the
+// relative RMS over the spots estimates the one over all sites.
 // The sum over the sources is a GEMM on the FP64 matrix cores: for field
 // component c (potential and field) and source component a,
 //   D_c[spot][set] += sum_j T_{a+c}(x_spot - x_j) m~_j(a)[set]
@@ -658,9 +660,17 @@ __global__ __launch_bounds__(64 * SP_W) void k_spot_far(SpotArgs a) {
     const int row = lane & 15, kq = lane >> 4;
     const int spot = (blockIdx.x * SP_W + w) * 16 + row;
     const bool valid = spot < a.nspots;
-    const int leaf = (int)(((long long)(valid ? spot : 0) * a.num_leaves) / a.nspots);
+    const int site = (int)(((2ll * (valid ? spot : 0) + 1) * a.n) / (2ll * a.nspots));
+    int leaf = 0;   // the leaf holding the site: the last one starting at or before it (leaves are in site order)
+    {
+        int hi = a.num_leaves;
+        while (hi - leaf > 1) {
+            const int mid = (leaf + hi) >> 1;
+            if (a.start[a.leaf_node[mid]] <= site) leaf = mid;
+            else hi = mid;
+        }
+    }
     const int node = a.leaf_node[leaf];
-    const int site = a.start[node];
     const double x = a.xs[site], y = a.ys[site], z = a.zs[site];
     const int set0 = blockIdx.y * BW;
     const int j0 = blockIdx.z * a.chunk, j1 = min(a.n, j0 + a.chunk);

@@ -294,8 +294,8 @@ enum fmmh_option {
      * for source multipole orders above 2, every pair keeps expansion_order. */
     FMMH_OPT_ORDER_TOL = 6,       /* requested relative RMS field error of the far field (0 = off, default) */
     FMMH_OPT_ORDER_MARGIN = 7,    /* added to the predicted order before rounding up (default 0.25) */
-    FMMH_OPT_SPOT_CHECKS = 8,     /* > 0: after the call, compare the far field at this many sites (the
-                                     first site of evenly spaced leaves) against direct sums with the
+    FMMH_OPT_SPOT_CHECKS = 8,     /* > 0: after the call, compare the far field at this many sites (evenly
+                                     spaced in the octree's site order) against direct sums with the
                                      reference's far semantics (undamped, no exclusions), set by set; if
                                      the relative RMS error of any set exceeds ORDER_TOL, redo M2L..L2P
                                      at expansion_order (fallback).  0 (default): no check. */
