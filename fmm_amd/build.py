@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfmm_hip.so")
 # slowest first: the wide-order near-field units take minutes each (one kernel per order pair and damping)
 SOURCES = ["near_wide_k7a.hip", "near_wide_k7b.hip", "near_wide_k6a.hip", "near_wide_k6b.hip", "near_wide_k5.hip",
-           "near_wide_k34.hip", "expansions.hip",
+           "near_wide_k34.hip", "expansions.hip", "expansions_mfma.hip",
            "near.hip", "m2l.hip", "selector.hip", "shift_mfma.hip", "m2l_gemm.hip", "m2l_block.hip", "tree.hip", "periodic.hip",
            "qmmm.hip", "prims.hip", "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

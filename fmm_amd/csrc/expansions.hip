@@ -579,6 +579,7 @@ __global__ __launch_bounds__(256) void k_l2p_rt(DownArgs a) {
 
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
+    if (launch_p2m_mfma(a, num_leaves, s)) return;
     const size_t chunk = up_chunk(a.nsrc);
     if (a.nsets == 1)
         hipLaunchKernelGGL(k_p2m, dim3(num_leaves, 1), dim3(128), chunk * (a.nm + a.nsrc) * sizeof(double), s, a);
@@ -605,6 +606,7 @@ void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s) {
 
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s) {
     if (num_leaves <= 0) return;
+    if (launch_l2p_mfma(a, num_leaves, s)) return;
     if (a.nf > ncomp(2)) {
         hipLaunchKernelGGL(k_l2p_rt, dim3(num_leaves, a.nsets), dim3(256), 0, s, a);
     } else if (a.nsets == 1) {

@@ -188,6 +188,10 @@ struct UpArgs {
     const int* xyz;           // packed index -> ex | ey << 8 | ez << 16
 };
 void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s);
+// P2M and L2P (field order <= 1) of EXP_MFMA_MIN_SETS or more sets on the FP64 matrix cores
+// (expansions_mfma.hip); false (nothing launched) where they do not apply.
+constexpr int EXP_MFMA_MIN_SETS = 16;
+bool launch_p2m_mfma(const UpArgs& a, int num_leaves, hipStream_t s);
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s);
 
 // M2L by displacement class.  For a far pair (i, j) let u = min(r_i, r_j):
@@ -498,6 +502,7 @@ struct DownArgs {
 };
 void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s);
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s);
+bool launch_l2p_mfma(const DownArgs& a, int num_leaves, hipStream_t s);
 
 // ---------------------------------------------------------------- periodic boundaries
 // Lattice sum beyond the 27 cells (src/fmm.F90:884-944): at shell level l

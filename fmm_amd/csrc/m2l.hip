@@ -560,6 +560,8 @@ void launch_m2l_class_tensors(const M2LClassArgs& a, int grid, hipStream_t s) {
 // The Hankel matrix shrinks from ncomp(p)^2 to (p + 1)^4 entries (220^2 ->
 // 100^2 at p = 9); the class tensors stay whole.  Reduced components are
 // ordered by degree, then w = y + z, then z, as the packed order runs.
+constexpr int MAX_NM_FOLD = ncomp(10);   // expansion coefficients at the largest order of the M2L engines
+
 // Reduced component r -> degree | w << 8 | z << 16 (the packing of decode_target).
 __device__ __forceinline__ int decode_reduced(int r) {
     int k = 0;
@@ -578,78 +580,90 @@ __device__ __forceinline__ int packed_of(int d) {
 
 // Folded, box-scaled sources M'_j(s') in the MFMA layout [node][s' < rdim(p)][s_ld sets] (padding sets
 // zero): the sum over the components s that fold onto s' of (-1)^k C(k, i) c_s M_j(s), times r_j^{-|s'|}.
+// One workgroup per (node, 16 sets): the sets' expansions are loaded into LDS along the coefficients
+// (coalesced) and written along the sets.
+constexpr int FOLD_SETS = 16;
 __global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restrict__ CM, const double* __restrict__ cs,
                                                            const double* __restrict__ r, double* __restrict__ CMs,
                                                            int num_nodes, int p, int nsets, int s_ld) {
+    __shared__ double cm[FOLD_SETS][MAX_NM_FOLD];
     const int nr = rdim(p), nm = ncomp(p);
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= (int64_t)num_nodes * nr * s_ld) return;
-    const int64_t js = x / s_ld;
-    const int set = (int)(x - js * s_ld);
-    if (set >= nsets) { CMs[x] = 0.0; return; }
-    const int j = (int)(js / nr), sr = (int)(js - (int64_t)j * nr);
-    const int d = decode_reduced(sr);
-    const int deg = d & 255, w = (d >> 8) & 255, rho = d >> 16;
-    const int X = deg - w, Y = w - rho;
-    const double* __restrict__ cm = CM + (int64_t)set * num_nodes * nm + (int64_t)j * nm;
-    double acc = 0.0;
-    for (int k = 0; 2 * k <= X + Y; ++k) {
-        const double sg = (k & 1) ? -1.0 : 1.0;
-        double binom = 1.0;
-        for (int i = 0; i <= k; ++i) {
-            if (i > 0) binom = binom * (k - i + 1) / i;
-            const int sx = X - 2 * i, sy = Y - 2 * (k - i);
-            if (sx < 0 || sy < 0) continue;
-            const int s = xyz2idx(sx, sy, rho + 2 * k);
-            acc += (sg * binom) * (cs[s] * cm[s]);
-        }
+    const int j = blockIdx.x, set0 = blockIdx.y * FOLD_SETS;
+    const int nsl = min(FOLD_SETS, s_ld - set0);   // output columns of this workgroup
+    for (int x = threadIdx.x; x < FOLD_SETS * nm; x += blockDim.x) {
+        const int u = x / nm, sc = x - u * nm;
+        cm[u][sc] = set0 + u < nsets ? CM[((int64_t)(set0 + u) * num_nodes + j) * nm + sc] : 0.0;
     }
+    __syncthreads();
     const double rinv = 1.0 / r[j];
-    double f = 1.0;
-    for (int q = 0; q < deg; ++q) f *= rinv;
-    CMs[x] = acc * f;
+    for (int x = threadIdx.x; x < nr * nsl; x += blockDim.x) {
+        const int sr = x / nsl, u = x - sr * nsl;
+        double out = 0.0;
+        if (set0 + u < nsets) {
+            const int d = decode_reduced(sr);
+            const int deg = d & 255, w = (d >> 8) & 255, rho = d >> 16;
+            const int X = deg - w, Y = w - rho;
+            double acc = 0.0;
+            for (int k = 0; 2 * k <= X + Y; ++k) {
+                const double sg = (k & 1) ? -1.0 : 1.0;
+                double binom = 1.0;
+                for (int i = 0; i <= k; ++i) {
+                    if (i > 0) binom = binom * (k - i + 1) / i;
+                    const int sx = X - 2 * i, sy = Y - 2 * (k - i);
+                    if (sx < 0 || sy < 0) continue;
+                    const int sc = xyz2idx(sx, sy, rho + 2 * k);
+                    acc += (sg * binom) * (cs[sc] * cm[u][sc]);
+                }
+            }
+            double f = 1.0;
+            for (int q = 0; q < deg; ++q) f *= rinv;
+            out = acc * f;
+        }
+        CMs[((int64_t)j * nr + sr) * s_ld + set0 + u] = out;
+    }
 }
 
 // Local-expansion components with t_z >= 2 from the reduced ones the M2L wrote
-// (LE[row][ncomp(p)], rows = sets x nodes, in place: only components with
-// t_z <= 1 are read).  One thread per (row, component with t_z >= 2).
+// (LE[row][ncomp(p)], rows = sets x nodes): UNFOLD_ROWS rows per workgroup are
+// loaded into LDS, completed there and written back whole (coalesced).
+constexpr int UNFOLD_ROWS = 8;
 __global__ __launch_bounds__(256) void k_le_unfold(double* __restrict__ LE, int64_t rows, int p) {
-    const int nm = ncomp(p), nz = nm - rdim(p);   // components with t_z >= 2
-    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= rows * nz) return;
-    const int64_t rw = x / nz;
-    int l = (int)(x - rw * nz);
-    // degree n holds (n - 1) n / 2 of them: (w, z) for 2 <= z <= w <= n, in packed order
-    int n = 2;
-    while (l >= (n - 1) * n / 2) {
-        l -= (n - 1) * n / 2;
-        ++n;
+    __shared__ double lr[UNFOLD_ROWS][MAX_NM_FOLD];
+    const int nm = ncomp(p);
+    const int64_t r0 = (int64_t)blockIdx.x * UNFOLD_ROWS;
+    const int nrw = (int)min((int64_t)UNFOLD_ROWS, rows - r0);
+    double* __restrict__ base = LE + r0 * nm;
+    for (int x = threadIdx.x; x < nrw * nm; x += blockDim.x) lr[x / nm][x % nm] = base[x];
+    __syncthreads();
+    for (int x = threadIdx.x; x < nrw * nm; x += blockDim.x) {
+        const int rw = x / nm, t = x - rw * nm;
+        const int d = decode_target(t);
+        const int deg = d & 255, w = (d >> 8) & 255, tz = d >> 16;
+        if (tz < 2) {   // unchanged; stored again so that every cache line is written whole
+            base[x] = lr[rw][t];
+            continue;
+        }
+        const int tx = deg - w, ty = w - tz, k = tz >> 1, rho = tz & 1;
+        double acc = 0.0, binom = 1.0;
+        for (int i = 0; i <= k; ++i) {
+            if (i > 0) binom = binom * (k - i + 1) / i;
+            acc += binom * lr[rw][xyz2idx(tx + 2 * i, ty + 2 * (k - i), rho)];
+        }
+        base[x] = (k & 1) ? -acc : acc;
     }
-    int w = 2;
-    while (l >= w - 1) {
-        l -= w - 1;
-        ++w;
-    }
-    const int tz = 2 + l, tx = n - w, ty = w - tz, k = tz >> 1, rho = tz & 1;
-    double* __restrict__ row = LE + rw * nm;
-    double acc = 0.0, binom = 1.0;
-    for (int i = 0; i <= k; ++i) {
-        if (i > 0) binom = binom * (k - i + 1) / i;
-        acc += binom * row[xyz2idx(tx + 2 * i, ty + 2 * (k - i), rho)];
-    }
-    row[xyz2idx(tx, ty, tz)] = (k & 1) ? -acc : acc;
 }
 
 void launch_scale_sources_red(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
                               int nsets, int s_ld, hipStream_t s) {
-    hipLaunchKernelGGL(k_scale_sources_red, dim3(nblk((long long)num_nodes * rdim(p) * s_ld, 256)), dim3(256), 0, s,
+    if (num_nodes <= 0) return;
+    hipLaunchKernelGGL(k_scale_sources_red, dim3(num_nodes, (s_ld + FOLD_SETS - 1) / FOLD_SETS), dim3(256), 0, s,
                        CM, cs, r, CMs, num_nodes, p, nsets, s_ld);
     FMMH_HIP(hipGetLastError());
 }
 
 void launch_le_unfold(double* LE, long long rows, int p, hipStream_t s) {
     if (rows <= 0 || p < 2) return;
-    hipLaunchKernelGGL(k_le_unfold, dim3(nblk(rows * (ncomp(p) - rdim(p)), 256)), dim3(256), 0, s, LE, (int64_t)rows, p);
+    hipLaunchKernelGGL(k_le_unfold, dim3(nblk(rows, UNFOLD_ROWS)), dim3(256), 0, s, LE, (int64_t)rows, p);
     FMMH_HIP(hipGetLastError());
 }
 
