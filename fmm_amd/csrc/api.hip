@@ -1761,14 +1761,19 @@ const unsigned char* select_orders(fmmh_context* c, TreeOut& tr, ListsOut& L, Ta
 
 // Relative RMS error (field block; the potential for field order 0) of the
 // far field at spot sites against the direct sum with the reference's far
-// semantics, one workgroup per multipole set: err[set].  FMM far field at a
-// site = field - field before L2P - near; the direct far field is the sum of
-// the source chunks' parts dir[chunk][set][spot][4], in chunk order.
+// semantics.  FMM far field at a site = field - field before L2P - near; the
+// direct far field is the sum of the source chunks' parts
+// dir[chunk][set][spot][4], in chunk order.  k_spot_compare: workgroup
+// (set, b) sums the squared error and field over entries x = 4 spot + c in
+// [SPOT_CMP_X b, SPOT_CMP_X (b + 1)) into part[set][b][2]; k_spot_worst adds
+// each set's parts in block order and returns the largest relative error
+// (a non-finite one wins).
+constexpr int SPOT_CMP_X = 1024;
 __global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__ dir, const int* __restrict__ site,
                                                       const int* __restrict__ perm, const double* __restrict__ near,
                                                       int64_t ldn, const double* __restrict__ field, int64_t ldf,
                                                       const double* __restrict__ before, int64_t ldb, int nspots,
-                                                      int nf, int nsets, int nchunks, double* __restrict__ err) {
+                                                      int nf, int nsets, int nchunks, double* __restrict__ part) {
     __shared__ double red[2][256];
     const int set = blockIdx.x;
     const int c0 = nf >= 4 ? 1 : 0, c1 = nf >= 4 ? 4 : 1;
@@ -1776,7 +1781,8 @@ __global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__
     const double* __restrict__ fl = field + (int64_t)set * nf * ldf;
     const double* __restrict__ bf = before + (int64_t)set * nf * ldb;
     double e2 = 0.0, f2 = 0.0;
-    for (int x = threadIdx.x; x < 4 * nspots; x += 256) {   // x = 4 spot + c: the chunk loads are coalesced
+    const int x1 = min(4 * nspots, SPOT_CMP_X * ((int)blockIdx.y + 1));
+    for (int x = SPOT_CMP_X * blockIdx.y + threadIdx.x; x < x1; x += 256) {   // chunk loads are coalesced
         const int sp = x >> 2, c = x & 3;
         if (c < c0 || c >= c1) continue;
         const int k = site[sp], o = perm[k];
@@ -1797,15 +1803,19 @@ __global__ __launch_bounds__(256) void k_spot_compare(const double* __restrict__
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) err[set] = red[1][0] > 0.0 ? sqrt(red[0][0] / red[1][0]) : 0.0;   // no field: 0
+    if (threadIdx.x < 2) part[((int64_t)set * gridDim.y + blockIdx.y) * 2 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out[0] = the largest error over the sets (a non-finite one wins).
-__global__ void k_spot_worst(const double* __restrict__ err, int nsets, double* __restrict__ out) {
+__global__ void k_spot_worst(const double* __restrict__ part, int nsets, int nb, double* __restrict__ out) {
     if (threadIdx.x != 0) return;
     double worst = 0.0;
     for (int set = 0; set < nsets; ++set) {
-        const double e = err[set];
+        double e2 = 0.0, f2 = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            e2 += part[((int64_t)set * nb + b) * 2];
+            f2 += part[((int64_t)set * nb + b) * 2 + 1];
+        }
+        const double e = f2 > 0.0 ? sqrt(e2 / f2) : 0.0;   // a set without field counts 0
         worst = (e > worst || e != e) ? e : worst;
     }
     out[0] = worst;
@@ -1825,13 +1835,14 @@ double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, c
     const int nchunks = (n + chunk - 1) / chunk;
     double* out = c->buf<double>("spot_out", (size_t)nchunks * nspots * 4 * k.nsets);
     int* site = c->buf<int>("spot_site", nspots);
-    double* res = c->buf<double>("spot_res", 1 + (size_t)k.nsets);
+    const int ncb = (4 * nspots + SPOT_CMP_X - 1) / SPOT_CMP_X;
+    double* res = c->buf<double>("spot_res", 1 + (size_t)k.nsets * ncb * 2);
     SpotArgs sa{nspots, tr.num_leaves, n, tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, xs, ys, zs,
                 ms, n, out, site, k.nsets, chunk, nchunks};
     launch_spot_far(sa, m, s);
-    hipLaunchKernelGGL(k_spot_compare, dim3(k.nsets), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n,
+    hipLaunchKernelGGL(k_spot_compare, dim3(k.nsets, ncb), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n,
                        k.field, k.ldf, before, (int64_t)n, nspots, nf, k.nsets, nchunks, res + 1);
-    hipLaunchKernelGGL(k_spot_worst, dim3(1), dim3(64), 0, s, res + 1, k.nsets, res);
+    hipLaunchKernelGGL(k_spot_worst, dim3(1), dim3(64), 0, s, res + 1, k.nsets, ncb, res);
     FMMH_HIP(hipGetLastError());
     double* h = reinterpret_cast<double*>(c->h_pinned + 800);
     FMMH_HIP(hipMemcpyAsync(h, res, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2059,6 +2070,10 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     if (P.on) run_periodic(c, tr, tb, P, k, p, CM, LE);
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
                 tr.t.leaf_node, tr.t.start, tr.t.count, xs, ys, zs, tr.t.perm, near, n, k.field, k.ldf, nf, nn, S, tb.xyz};
+    if (S >= EXP_MFMA_MIN_SETS && p <= EXP_MFMA_PMAX && (nf == 1 || nf == 4)) {
+        da.basis = c->buf<double>("l2p_basis", (size_t)rdim(p) * n);   // the expansions here are harmonic
+        da.ld_basis = n;
+    }
     auto downward = [&]() {
         for (int lev = 2; lev <= tr.max_depth; ++lev) {
             if (shift32) {   // the parents of this level's nodes, on FP32 MFMA (one octant per workgroup)

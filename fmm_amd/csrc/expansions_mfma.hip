@@ -6,12 +6,14 @@
 //        component s; d_j = x_j - leaf centre; coef and the monomial index
 //        from the P2M shift table)
 //   L2P  F_j(c)[set] = sum_t d_j^(t - c) / (t - c)! L(t)[set]
-//        (rows: site j and field component c, field order <= 1; k: t)
-// One workgroup of four wavefronts per (leaf, 16 sets): monomials of a chunk
-// of sites and the B operand (multipoles, or the leaf's local expansion) are
-// staged in LDS, each wavefront runs its 16-row tiles of
-// v_mfma_f64_16x16x4f64, A gathered from the monomials through a table built
-// in LDS per workgroup.  FP64 throughout, like the one-set kernels
+//        (rows: site j; k: the (p + 1)^2 harmonic components of t, see
+//        k_l2p_basis; one accumulator per field component c, order <= 1)
+// P2M: one workgroup of four wavefronts per (leaf, 16 sets): monomials of a
+// chunk of sites and the multipoles are staged in LDS, each wavefront runs its
+// 16-row tiles of v_mfma_f64_16x16x4f64, A gathered from the monomials
+// through a table built in LDS per workgroup.  L2P: a per-site basis pass,
+// then one workgroup of two wavefronts per (leaf, 16 sets) with the leaf's
+// local expansion staged in LDS.  FP64 throughout, like the one-set kernels
 // (expansions.hip); the sums run in another order, so a set of a many-set
 // call agrees with its one-set call to rounding (the vector-ALU many-set
 // kernels, used below EXP_MFMA_MIN_SETS sets, agree bit for bit).
@@ -26,7 +28,7 @@ namespace {
 
 constexpr int XM_W = 4;        // wavefronts per workgroup
 constexpr int XM_CH = 16;      // sites per P2M chunk
-constexpr int XM_PMAX = 10;
+constexpr int XM_PMAX = EXP_MFMA_PMAX;
 
 __device__ __forceinline__ void powers(double d, int p, double* w) {
     double v = 1.0;
@@ -121,87 +123,162 @@ __global__ __launch_bounds__(64 * XM_W) void k_p2m_mfma(UpArgs a) {
         }
 }
 
-// LDS: the leaf's local expansion as the B operand, the Taylor monomials of a chunk of 16 sites and the
-// (t, c) table; the chunk's NF 16-row tiles go to the wavefronts in turn.
+// Harmonic form.  The local expansion of the far field is the Taylor expansion of a harmonic function,
+// L(t) = d^t phi, so L(t + 2x) + L(t + 2y) + L(t + 2z) = 0 (the M2L builds its components with t_z >= 2
+// from those with t_z <= 1 by this identity, see m2l.hip, and the L2L shifts preserve it).  Then
+//   F_j(0) = sum_t d_j^t / t! L(t) = sum_{s : s_z <= 1} H_j(s) L(s)
+// over the (p + 1)^2 reduced components s, with H_j(s) = H(s)(d_j) the fold of the monomials,
+//   H(X, Y, rho)(d) = sum_{k, i} (-1)^k C(k, i) d^(X - 2i, Y - 2(k - i), rho + 2k) / (...)!,
+// a homogeneous harmonic polynomial of degree n = X + Y + rho.  Its derivatives are basis polynomials
+// again: d/dx H(X, Y, rho) = H(X - 1, Y, rho), d/dy H(X, Y, rho) = H(X, Y - 1, rho),
+// d/dz H(X, Y, 1) = H(X, Y, 0), d/dz H(X, Y, 0) = -(H(X, Y - 2, 1) + H(X - 2, Y, 1)), which gives
+//  (a) the basis by Euler's relation for homogeneous polynomials, n H = x H_x + y H_y + z H_z, from the
+//      degree below (k_l2p_basis: about four FMAs per component, once per site);
+//  (b) the field components as the same contraction against shifted components of L:
+//      F_j(x) = sum_s H_j(s) L(s + x), F_j(y) = sum_s H_j(s) L(s + y),
+//      F_j(z) = sum_{s_z = 0} H_j(s) L(s + z) - sum_{s_z = 1} H_j(s) [L(s + 2y - z) + L(s + 2x - z)]
+//      (components past order p are zero).
+// The L2P is then one GEMM per (leaf, 16 sets): rows the leaf's sites, k the reduced components,
+// columns the sets, one accumulator per field component.  Only for local expansions this library built:
+// those a caller passes in (interpolate_electron_fields) need not be harmonic and take the vector-ALU
+// kernels.
+
+// reduced component (X, Y, rho), rho = t_z <= 1: degree n = X + Y + rho, then w = Y + rho, then rho
+__host__ __device__ constexpr int red_index(int X, int Y, int rho) {
+    return (X + Y + rho) * (X + Y + rho) + (Y + rho == 0 ? 0 : 2 * (Y + rho) - 1 + rho);
+}
+
+// H_j(s) for every site j of a leaf, stored [s][sorted site] (a.ld_basis per component row).
+template <int P>
+__global__ __launch_bounds__(64) void k_l2p_basis(DownArgs a) {
+    const int node = a.leaf_node[blockIdx.x];
+    const int s0 = a.start[node], c = a.count[node];
+    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
+    double* out = a.basis;
+    for (int jj = threadIdx.x; jj < c; jj += 64) {
+        const int j = s0 + jj;
+        const double x = a.xs[j] - cx, y = a.ys[j] - cy, z = a.zs[j] - cz;
+        double h[rdim(P)];
+        h[0] = 1.0;
+        out[j] = 1.0;
+#pragma unroll
+        for (int n = 1; n <= P; ++n) {
+            const double inv = 1.0 / (double)n;
+#pragma unroll
+            for (int w = 0; w <= n; ++w) {
+#pragma unroll
+                for (int rho = 0; rho <= 1; ++rho) {
+                    if (rho > w) continue;
+                    const int X = n - w, Y = w - rho;
+                    double v = X >= 1 ? x * h[red_index(X - 1, Y, rho)] : 0.0;
+                    if (Y >= 1) v = fma(y, h[red_index(X, Y - 1, rho)], v);
+                    if (rho == 1) {
+                        v = fma(z, h[red_index(X, Y, 0)], v);
+                    } else if (X >= 2 || Y >= 2) {
+                        double u = 0.0;
+                        if (Y >= 2) u = h[red_index(X, Y - 2, 1)];
+                        if (X >= 2) u += h[red_index(X - 2, Y, 1)];
+                        v = fma(-z, u, v);
+                    }
+                    const int r = red_index(X, Y, rho);
+                    h[r] = v * inv;
+                    out[(int64_t)r * a.ld_basis + j] = h[r];
+                }
+            }
+        }
+    }
+}
+
+// LDS: the leaf's reduced local expansion for 16 sets (rows past the last component and the extra last row
+// are zero) and, per field component and k, the two rows of it that make the B operand (and the sign).
+constexpr int XL_W = 2;
 template <int P, int F>
-__global__ __launch_bounds__(64 * XM_W) void k_l2p_mfma(DownArgs a) {
+__global__ __launch_bounds__(64 * XL_W) void k_l2p_mfma(DownArgs a) {
     constexpr int NM = ncomp(P);
+    constexpr int NK = rdim(P);
     constexpr int NF = ncomp(F);
-    constexpr int KS = (NM + 3) / 4;
-    constexpr int SC = 16;                 // sites per chunk
-    constexpr int ST = 16 / NF;            // sites per row tile
-    __shared__ double L[4 * KS * 16];
-    __shared__ double monoF[SC * NM];
-    __shared__ int tabD[NM * NF];
-    __shared__ double pw[SC][3][XM_PMAX + 1];
-    __shared__ double ifact[XM_PMAX + 1];
+    constexpr int KS = (NK + 3) / 4;
+    constexpr int ZR = 4 * KS;             // the zero row
+    __shared__ double L[(ZR + 1) * 16];
+    __shared__ int gi[NF][4 * KS];         // row1 | row2 << 8 | negate << 16
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int row = lane & 15, kq = lane >> 4;
     const int node = a.leaf_node[blockIdx.x];
     const int set0 = blockIdx.y * 16;
     const int s0 = a.start[node], c = a.count[node];
-    const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
-    if (tid == 0) {
-        double f = 1.0;
-        ifact[0] = 1.0;
-        for (int n = 1; n <= XM_PMAX; ++n) {
-            f *= (double)n;
-            ifact[n] = 1.0 / f;
-        }
-    }
-    for (int x = tid; x < 4 * KS * 16; x += 64 * XM_W) {
+    for (int x = tid; x < (ZR + 1) * 16; x += 64 * XL_W) {
         const int t = x >> 4, u = x & 15;
-        L[x] = (t < NM && set0 + u < a.nsets) ? a.LE[((int64_t)(set0 + u) * a.num_nodes + node) * NM + t] : 0.0;
-    }
-    for (int x = tid; x < NM * NF; x += 64 * XM_W) {
-        const int t = x / NF, cf = x - t * NF;
-        const int pk = a.xyz[t];
-        const int ex = (pk & 255) - (cf == 1), ey = ((pk >> 8) & 255) - (cf == 2), ez = (pk >> 16) - (cf == 3);
-        tabD[x] = (ex < 0 || ey < 0 || ez < 0) ? -1 : xyz2idx(ex, ey, ez);
-    }
-    for (int j0 = 0; j0 < c; j0 += SC) {
-        const int cc = min(SC, c - j0);
-        __syncthreads();   // readers of the previous chunk are done (and L, tabD, ifact are in place)
-        if (tid < 3 * cc) {
-            const int jj = tid / 3, ax = tid - 3 * jj;
-            const int j = s0 + j0 + jj;
-            const double d = ax == 0 ? a.xs[j] - cx : (ax == 1 ? a.ys[j] - cy : a.zs[j] - cz);
-            powers(d, P, pw[jj][ax]);
+        double v = 0.0;
+        if (t < NK && set0 + u < a.nsets) {
+            int n = 0;
+            while ((n + 1) * (n + 1) <= t) ++n;
+            const int pos = t - n * n, wv = (pos + 1) >> 1, rho = (pos > 0 && (pos & 1) == 0) ? 1 : 0;
+            v = a.LE[((int64_t)(set0 + u) * a.num_nodes + node) * NM + xyz2idx(n - wv, wv - rho, rho)];
         }
-        __syncthreads();
-        for (int x = tid; x < cc * NM; x += 64 * XM_W) {
-            const int jj = x / NM, e = x - jj * NM;
-            const int pk = a.xyz[e];
-            const int ex = pk & 255, ey = (pk >> 8) & 255, ez = pk >> 16;
-            monoF[jj * NM + e] = ((pw[jj][0][ex] * ifact[ex]) * (pw[jj][1][ey] * ifact[ey])) * (pw[jj][2][ez] * ifact[ez]);
-        }
-        __syncthreads();
-        for (int tile = w; tile * ST < cc; tile += XM_W) {
-            // row r of the tile: site tile ST + r / NF of the chunk, field component r % NF
-            const int jj = tile * ST + row / NF, cf = row - (row / NF) * NF;
-            const bool rv = jj < cc;
-            v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
-            for (int ks = 0; ks < KS; ++ks) {
-                const int t = 4 * ks + kq;
-                double av = 0.0;
-                if (rv && t < NM) {
-                    const int e = tabD[t * NF + cf];
-                    if (e >= 0) av = monoF[jj * NM + e];
+        L[x] = v;
+    }
+    for (int x = tid; x < NF * 4 * KS; x += 64 * XL_W) {
+        const int cf = x / (4 * KS), t = x - cf * (4 * KS);
+        int r1 = ZR, r2 = ZR, neg = 0;
+        if (t < NK) {
+            int n = 0;
+            while ((n + 1) * (n + 1) <= t) ++n;
+            const int pos = t - n * n, wv = (pos + 1) >> 1, rho = (pos > 0 && (pos & 1) == 0) ? 1 : 0;
+            const int X = n - wv, Y = wv - rho;
+            if (cf == 0) r1 = t;
+            else if (n < P) {
+                if (cf == 1) r1 = red_index(X + 1, Y, rho);
+                else if (cf == 2) r1 = red_index(X, Y + 1, rho);
+                else if (rho == 0) r1 = red_index(X, Y, 1);
+                else {
+                    r1 = red_index(X, Y + 2, 0);
+                    r2 = red_index(X + 2, Y, 0);
+                    neg = 1;
                 }
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, L[t * 16 + row], acc, 0, 0, 0);
             }
-            // accumulator register q of lane l holds D[row 4 q + l / 16][set l % 16]
-            const int set = set0 + row;
-            if (set < a.nsets) {
+        }
+        gi[cf][t] = r1 | (r2 << 8) | (neg << 16);
+    }
+    __syncthreads();
+    const int set = set0 + row;
+    for (int tile = w; tile * 16 < c; tile += XL_W) {
+        const int jj = tile * 16 + row;            // A row: site of the leaf
+        const bool okj = jj < c;
+        const double* hb = a.basis + s0 + jj;
+        v4d acc[NF];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = 4 * q + kq;
-                    const int sj = tile * ST + r / NF, c2 = r - (r / NF) * NF;
-                    if (sj >= cc) continue;
-                    const int k = s0 + j0 + sj;
-                    const int o = a.perm[k];
-                    double* fp = a.field + ((int64_t)set * NF + c2) * a.ld_field + o;
-                    *fp = (*fp + a.near[((int64_t)set * NF + c2) * a.ld_near + k]) + acc[q];
+        for (int cf = 0; cf < NF; ++cf) acc[cf] = v4d{0.0, 0.0, 0.0, 0.0};
+        // rolled loop with the next A element loaded ahead (unrolled, the compiler keeps the B gather for
+        // every k live: 256 VGPRs and one wavefront per SIMD at order 10)
+        double an = (okj && kq < NK) ? hb[kq * a.ld_basis] : 0.0;
+#pragma unroll 1
+        for (int ks = 0; ks < KS; ++ks) {
+            const int t = 4 * ks + kq;
+            const double av = an;
+            an = (okj && t + 4 < NK) ? hb[(int64_t)(t + 4) * a.ld_basis] : 0.0;
+#pragma unroll
+            for (int cf = 0; cf < NF; ++cf) {
+                const int g = gi[cf][t];
+                double bv = L[(g & 255) * 16 + row];
+                if (cf == 3) {
+                    bv += L[((g >> 8) & 255) * 16 + row];
+                    if (g >> 16) bv = -bv;
+                }
+                acc[cf] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cf], 0, 0, 0);
+            }
+        }
+        // accumulator register q of lane l holds D[site 4 q + l / 16][set l % 16]
+        if (set < a.nsets) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int sj = tile * 16 + 4 * q + kq;
+                if (sj >= c) continue;
+                const int k = s0 + sj;
+                const int o = a.perm[k];
+#pragma unroll
+                for (int cf = 0; cf < NF; ++cf) {
+                    double* fp = a.field + ((int64_t)set * NF + cf) * a.ld_field + o;
+                    *fp = (*fp + a.near[((int64_t)set * NF + cf) * a.ld_near + k]) + acc[cf][q];
                 }
             }
         }
@@ -220,7 +297,8 @@ void p2m_p(const UpArgs& a, int num_leaves, hipStream_t s) {
 
 template <int P, int F>
 void l2p_pf(const DownArgs& a, int num_leaves, hipStream_t s) {
-    hipLaunchKernelGGL((k_l2p_mfma<P, F>), dim3(num_leaves, (a.nsets + 15) / 16), dim3(64 * XM_W), 0, s, a);
+    hipLaunchKernelGGL((k_l2p_basis<P>), dim3(num_leaves), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((k_l2p_mfma<P, F>), dim3(num_leaves, (a.nsets + 15) / 16), dim3(64 * XL_W), 0, s, a);
 }
 
 template <int F>
@@ -262,9 +340,7 @@ bool launch_p2m_mfma(const UpArgs& a, int num_leaves, hipStream_t s) {
 }
 
 bool launch_l2p_mfma(const DownArgs& a, int num_leaves, hipStream_t s) {
-    // to order 9 (at 10 the staged expansion and monomials take 82 KB of LDS, one workgroup per CU, and
-    // the vector-ALU kernel is faster: 2.7 against 2.9 ms for 32 sets on the 200k-site water box)
-    if (a.nsets < EXP_MFMA_MIN_SETS || a.p > 9 || (a.nf != 1 && a.nf != 4)) return false;
+    if (!a.basis || a.nsets < EXP_MFMA_MIN_SETS || a.p > XM_PMAX || (a.nf != 1 && a.nf != 4)) return false;
     if (a.nf == 1) l2p_f<0>(a, num_leaves, s);
     else l2p_f<1>(a, num_leaves, s);
     FMMH_HIP(hipGetLastError());

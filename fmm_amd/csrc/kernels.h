@@ -191,6 +191,7 @@ void launch_p2m(const UpArgs& a, int num_leaves, hipStream_t s);
 // P2M and L2P (field order <= 1) of EXP_MFMA_MIN_SETS or more sets on the FP64 matrix cores
 // (expansions_mfma.hip); false (nothing launched) where they do not apply.
 constexpr int EXP_MFMA_MIN_SETS = 16;
+constexpr int EXP_MFMA_PMAX = 10;
 bool launch_p2m_mfma(const UpArgs& a, int num_leaves, hipStream_t s);
 void launch_m2m(const UpArgs& a, const int* nodes, int count, hipStream_t s);
 
@@ -499,6 +500,11 @@ struct DownArgs {
     int num_nodes;                         // LE is [set][node][nm]; near and field are [set][component][ld]
     int nsets;                             // one grid row per set
     const int* xyz;                        // packed index -> ex | ey << 8 | ez << 16
+    // many-set L2P on the matrix cores: (p + 1)^2 x ld_basis doubles for the per-site harmonic basis.  Set
+    // only where LE holds harmonic local expansions (built by this library's M2L and L2L); null selects the
+    // vector-ALU kernels
+    double* basis;
+    int64_t ld_basis;
 };
 void launch_l2l(const DownArgs& a, const int* nodes, int count, hipStream_t s);
 void launch_l2p(const DownArgs& a, int num_leaves, hipStream_t s);
