@@ -176,8 +176,9 @@ def plain_sweep(dev, steps, thetas=(0.25, 0.3, 0.35, 0.4), orders=(7, 8, 9, 10))
     return {"workload": "water-200k-undamped-dipole", "rows": rows, "cheapest_within_1e-6": best}
 
 
-def target_comparison(dev, steps, plain=((0.3, 9), (0.35, 10), (0.4, 10)), batched=((0.3, 9), (0.35, 10)), tol=1e-6,
-                      sets=32, spots=1024, margins=(0.25, 0.1)):
+def target_comparison(dev, steps, plain=((0.3, 8), (0.3, 9), (0.35, 9), (0.35, 10), (0.4, 10)),
+                      batched=((0.3, 9), (0.35, 10)), tol=1e-6,
+                      sets=32, spots=1024, margins=(0.25, 0.1), safeties=(1.0, 2.0), ncrits=(64,)):
     """Throughput at <= 1e-6 relative RMS field error against the FP64 direct
     sum: the plain FP64 FMM (one multipole set, every pair at expansion_order)
     against the batched (sets on one tree), mixed-precision (FP32 MFMA M2L),
@@ -187,10 +188,14 @@ def target_comparison(dev, steps, plain=((0.3, 9), (0.35, 10), (0.4, 10)), batch
     4096 seeded target sites for the first and last set.  The plain side is the
     cheapest of the `plain` (theta, order) settings within tol (the full sweep
     is --plain-sweep); the batched side runs each `batched` (theta, order cap)
-    setting at each order margin, and the speedup is the fastest batched run
-    within tol (both sets' errors, no fallback) over that plain side.  The
-    selector is trained on the GPU on a 1/20-scale box of the same recipe per
-    batched setting (the training time is reported, not counted in the step)."""
+    setting at each order margin and order safety factor (the per-pair
+    tolerance's factor, FMMH_OPT_ORDER_SAFETY; the spot checks still hold the
+    call to tol), and the speedup is the fastest batched run within tol (both
+    sets' errors, no fallback) over that plain side.  Both sides run at each
+    leaf size of `ncrits` (default: 64, the leaf size of every benchmarked
+    configuration).  The selector is trained on the GPU on a 1/20-scale box of
+    the same recipe per batched setting and leaf size (the training time is
+    reported, not counted in the step)."""
     import torch
     from fmm_amd import synth
     from fmm_amd.octree import Context
@@ -226,15 +231,17 @@ def target_comparison(dev, steps, plain=((0.3, 9), (0.35, 10), (0.4, 10)), batch
     M1 = col(np.asfortranarray(mh[:, :, 0]), dev)
     o1 = col(np.zeros((n, 4)), dev)
     prow = []
-    for th_p, p_p in plain:
-        def f_plain():
-            o1.zero_()
-            ctx.field_fmm_device(X, M1, E, th_p, 64, p_p, 1, o1, "", None)
-        ms_p = timed(f_plain)
-        err_p = rel(o1.cpu().numpy()[tg, 1:4], ref[0][:, 1:4])
-        prow.append({"theta": th_p, "expansion_order": p_p, "sets": 1, "ms_per_call": ms_p, "value": n / (ms_p / 1e3),
-                     "error": err_p, "within_1e-6": bool(err_p <= tol)})
-        print(f"plain theta {th_p} p {p_p}: {ms_p:.2f} ms, error {err_p:.3e}", file=sys.stderr, flush=True)
+    for nc in ncrits:
+        for th_p, p_p in plain:
+            def f_plain():
+                o1.zero_()
+                ctx.field_fmm_device(X, M1, E, th_p, nc, p_p, 1, o1, "", None)
+            ms_p = timed(f_plain)
+            err_p = rel(o1.cpu().numpy()[tg, 1:4], ref[0][:, 1:4])
+            prow.append({"theta": th_p, "expansion_order": p_p, "ncrit": nc, "sets": 1, "ms_per_call": ms_p,
+                         "value": n / (ms_p / 1e3), "error": err_p, "within_1e-6": bool(err_p <= tol)})
+            print(f"plain theta {th_p} p {p_p} ncrit {nc}: {ms_p:.2f} ms, error {err_p:.3e}", file=sys.stderr,
+                  flush=True)
     pok = [r for r in prow if r["within_1e-6"]]
     pbest = min(pok, key=lambda r: r["ms_per_call"]) if pok else None
     MS = col3(mh, dev)
@@ -244,24 +251,24 @@ def target_comparison(dev, steps, plain=((0.3, 9), (0.35, 10), (0.4, 10)), batch
     mt[:, 0] = ct["multipoles"][:, 0]
     mt[:, 1:4] = np.random.default_rng(7).normal(0.0, 0.05, (mt.shape[0], 3))
     brow = []
-    for th_b, p_b in batched:
-        info = ctx.train_selector(ct["coordinates"], np.asfortranarray(mt), th_b, 64, p_b, nsamples=4096, epochs=60,
+    for nc, (th_b, p_b) in [(nc, b) for nc in ncrits for b in batched]:
+        info = ctx.train_selector(ct["coordinates"], np.asfortranarray(mt), th_b, nc, p_b, nsamples=4096, epochs=60,
                                   seed=5)
         ctx.set_precision(True)
-        for margin in margins:
-            ctx.set_order_tolerance(tol, spot_checks=spots, margin=margin)
+        for margin, kap in [(mg, kp) for mg in margins for kp in safeties]:
+            ctx.set_order_tolerance(tol, spot_checks=spots, margin=margin, safety=kap)
 
             def f_batched():
                 os_.zero_()
-                ctx.field_fmm_sets_device(X, MS, E, th_b, 64, p_b, 1, os_, "", None)
+                ctx.field_fmm_sets_device(X, MS, E, th_b, nc, p_b, 1, os_, "", None)
             ms_b = timed(f_batched)
             st = ctx.selection_stats()
             phases = ctx.phase_times()
             ob = os_.cpu().numpy()
             err_b = [rel(ob[tg, 1:4, k], ref[k][:, 1:4]) for k in (0, sets - 1)]
-            r = {"theta": th_b, "expansion_order_cap": p_b, "sets": sets, "ms_per_call": ms_b,
+            r = {"theta": th_b, "expansion_order_cap": p_b, "ncrit": nc, "sets": sets, "ms_per_call": ms_b,
                  "value": n * sets / (ms_b / 1e3), "error_set0": err_b[0], "error_last_set": err_b[1],
-                 "order_tolerance": tol, "order_margin": margin, "spot_checks": spots,
+                 "order_tolerance": tol, "order_margin": margin, "order_safety": kap, "spot_checks": spots,
                  "spot_error": st["spot_error"], "fell_back": st["fallback"], "tolerance_scale": st["tolerance_scale"],
                  "mean_order": st["mean_order"], "m2l_flops_vs_full": st["m2l_flops"] / st["m2l_flops_full"],
                  "phases_ms": {k: round(v, 3) for k, v in phases.items()},
@@ -270,10 +277,10 @@ def target_comparison(dev, steps, plain=((0.3, 9), (0.35, 10), (0.4, 10)), batch
             if pbest is not None:
                 r["speedup"] = r["value"] / pbest["value"]
             brow.append(r)
-            print(f"batched theta {th_b} cap {p_b} margin {margin}: {ms_b:.2f} ms, errors {err_b[0]:.3e} "
-                  f"{err_b[1]:.3e}, spot {st['spot_error']:.3e}, tolerance scale {st['tolerance_scale']}",
-                  file=sys.stderr, flush=True)
-        ctx.set_order_tolerance(0.0)
+            print(f"batched theta {th_b} cap {p_b} ncrit {nc} margin {margin} safety {kap}: {ms_b:.2f} ms, errors "
+                  f"{err_b[0]:.3e} {err_b[1]:.3e}, spot {st['spot_error']:.3e}, tolerance scale "
+                  f"{st['tolerance_scale']}", file=sys.stderr, flush=True)
+        ctx.set_order_tolerance(0.0, safety=1.0)
         ctx.set_precision(False)
     ctx.close()
     bok = [r for r in brow if r["within_1e-6"]]
@@ -494,10 +501,12 @@ def main():
     ap.add_argument("--no-target", action="store_true",
                     help="skip the comparison at <= 1e-6 error (plain FP64 vs batched mixed learned-order)")
     ap.add_argument("--target-only", action="store_true", help="only run that comparison and print it")
-    ap.add_argument("--target-plain", default="0.3,9;0.35,10;0.4,10",
+    ap.add_argument("--target-plain", default="0.3,8;0.3,9;0.35,9;0.35,10;0.4,10",
                     help="theta,p settings of the plain FP64 side (';'-separated; the cheapest within 1e-6 is used)")
     ap.add_argument("--target-batched", default="0.3,9;0.35,10",
                     help="theta,cap settings of the batched side (';'-separated)")
+    ap.add_argument("--target-ncrit", default="64",
+                    help="leaf sizes of both sides of that comparison (','-separated)")
     ap.add_argument("--damped-target-only", action="store_true",
                     help="only run the <= 1e-6 comparison on the damped quadrupole workload and print it")
     ap.add_argument("--plain-sweep", action="store_true",
@@ -506,9 +515,10 @@ def main():
     def settings(arg):
         return tuple((float(a.split(",")[0]), int(a.split(",")[1])) for a in arg.split(";") if a)
     tp, tb_ = settings(args.target_plain), settings(args.target_batched)
+    tn = tuple(int(v) for v in args.target_ncrit.split(",") if v)
     if args.target_only:
         import torch
-        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_)))
+        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_, ncrits=tn)))
         return
     if args.damped_target_only:
         import torch
@@ -728,7 +738,8 @@ def main():
                     "of far pairs)",
         }
     cpu = None if args.no_cpu else cpu_baseline()
-    target = None if (args.no_target or args.config != 1) else target_comparison(dev, min(args.steps, 3), tp, tb_)
+    target = (None if (args.no_target or args.config != 1)
+              else target_comparison(dev, min(args.steps, 3), tp, tb_, ncrits=tn))
     damped = None if (args.no_target or args.config != 1) else damped_comparison(dev, min(args.steps, 3))
     line = {
         "metric": "site-field evals/s at <=1e-6 rel err vs FP64 direct; % of FP64/FP32 peak",
@@ -739,10 +750,13 @@ def main():
         "value_at_1e-6": (None if target is None else {
             "batched_mixed_learned": target["batched_mixed_learned"] and target["batched_mixed_learned"]["value"],
             "batched_setting": target["batched_mixed_learned"] and {
-                k: target["batched_mixed_learned"][k] for k in ("theta", "expansion_order_cap", "order_margin")},
+                k: target["batched_mixed_learned"][k]
+                for k in ("theta", "expansion_order_cap", "ncrit", "order_margin", "order_safety")},
             "plain_fp64": target["plain_fp64"] and target["plain_fp64"]["value"],
             "plain_setting": target["plain_fp64"] and {
-                k: target["plain_fp64"][k] for k in ("theta", "expansion_order")},
+                k: target["plain_fp64"][k] for k in ("theta", "expansion_order", "ncrit")},
+            "leaf_size_note": ("both sides at the leaf size given by --target-ncrit (default 64, that of every "
+                               "benchmarked configuration); --target-ncrit 64,80,96 sweeps it on both sides"),
             "speedup": target["speedup"], "workload": target["workload"],
             "damped_speedup": damped and damped.get("speedup"), "damped_workload": damped and damped["workload"]}),
         "unit": "site-field evals/s",

@@ -1821,31 +1821,47 @@ __global__ void k_spot_worst(const double* __restrict__ part, int nsets, int nb,
     out[0] = worst;
 }
 
-double spot_check(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, const double* ys, const double* zs,
-                  const double* ms, int n, int m, int nf, const double* near, const Call& k, const double* before,
-                  int nspots) {
-    hipStream_t s = c->stream;
-    nspots = std::min(nspots, n);
+// The direct far field at the spot sites depends on the sources and lists only: spot_direct launches
+// it once, spot_compare compares it with the FMM's far field as often as the far field is redone (the
+// full-order fallback reuses it).
+struct SpotPlan {
+    int nspots = 0, nchunks = 0, ncb = 0;
+    double* out = nullptr;
+    int* site = nullptr;
+    double* res = nullptr;
+};
+
+SpotPlan spot_direct(fmmh_context* c, TreeOut& tr, ListsOut& L, const double* xs, const double* ys,
+                     const double* zs, const double* ms, int n, int m, int nsets, int nspots) {
+    SpotPlan sp;
+    sp.nspots = nspots = std::min(nspots, n);
     // source chunks: about four workgroups per compute unit in all
     const int tile = spot_tile(m);
-    const int sets_blocks = (k.nsets + 31) / 32;
+    const int sets_blocks = (nsets + 31) / 32;
     const long long others = (long long)spot_row_blocks(nspots) * sets_blocks;
     const int want = (int)std::max(1ll, (4ll * c->num_cu + others - 1) / others);
     const int chunk = std::max(tile, (int)(((long long)n + (long long)want * tile - 1) / ((long long)want * tile)) * tile);
-    const int nchunks = (n + chunk - 1) / chunk;
-    double* out = c->buf<double>("spot_out", (size_t)nchunks * nspots * 4 * k.nsets);
-    int* site = c->buf<int>("spot_site", nspots);
-    const int ncb = (4 * nspots + SPOT_CMP_X - 1) / SPOT_CMP_X;
-    double* res = c->buf<double>("spot_res", 1 + (size_t)k.nsets * ncb * 2);
+    sp.nchunks = (n + chunk - 1) / chunk;
+    sp.out = c->buf<double>("spot_out", (size_t)sp.nchunks * nspots * 4 * nsets);
+    sp.site = c->buf<int>("spot_site", nspots);
+    sp.ncb = (4 * nspots + SPOT_CMP_X - 1) / SPOT_CMP_X;
+    sp.res = c->buf<double>("spot_res", 1 + (size_t)nsets * sp.ncb * 2);
     SpotArgs sa{nspots, tr.num_leaves, n, tr.t.leaf_node, tr.t.start, tr.t.count, L.p2p_off, L.p2p_src, xs, ys, zs,
-                ms, n, out, site, k.nsets, chunk, nchunks};
-    launch_spot_far(sa, m, s);
-    hipLaunchKernelGGL(k_spot_compare, dim3(k.nsets, ncb), dim3(256), 0, s, out, site, tr.t.perm, near, (int64_t)n,
-                       k.field, k.ldf, before, (int64_t)n, nspots, nf, k.nsets, nchunks, res + 1);
-    hipLaunchKernelGGL(k_spot_worst, dim3(1), dim3(64), 0, s, res + 1, k.nsets, ncb, res);
+                ms, n, sp.out, sp.site, nsets, chunk, sp.nchunks};
+    launch_spot_far(sa, m, c->stream);
+    return sp;
+}
+
+double spot_compare(fmmh_context* c, TreeOut& tr, const SpotPlan& sp, int n, int nf, const double* near,
+                    const Call& k, const double* before) {
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_spot_compare, dim3(k.nsets, sp.ncb), dim3(256), 0, s, sp.out, sp.site, tr.t.perm, near,
+                       (int64_t)n, k.field, k.ldf, before, (int64_t)n, sp.nspots, nf, k.nsets, sp.nchunks,
+                       sp.res + 1);
+    hipLaunchKernelGGL(k_spot_worst, dim3(1), dim3(64), 0, s, sp.res + 1, k.nsets, sp.ncb, sp.res);
     FMMH_HIP(hipGetLastError());
     double* h = reinterpret_cast<double*>(c->h_pinned + 800);
-    FMMH_HIP(hipMemcpyAsync(h, res, sizeof(double), hipMemcpyDeviceToHost, s));
+    FMMH_HIP(hipMemcpyAsync(h, sp.res, sizeof(double), hipMemcpyDeviceToHost, s));
     c->sync();
     c->mark("spot_check");
     return h[0];
@@ -2100,7 +2116,8 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     }
     downward();
     if (check) {
-        const double err = spot_check(c, tr, L, xs, ys, zs, ms, n, m, nf, near, k, before, c->spot_checks);
+        const SpotPlan spot = spot_direct(c, tr, L, xs, ys, zs, ms, n, m, S, c->spot_checks);
+        const double err = spot_compare(c, tr, spot, n, nf, near, k, before);
         c->sel_stats[0] = err;
         if (!(err <= c->order_tol)) {
             // the learned orders missed the tolerance: redo the far field at expansion_order
@@ -2114,7 +2131,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
             // the full-order result at the same sites; where it meets the tolerance, later calls get a
             // per-pair tolerance tighter by about the factor the learned orders missed (where it does
             // not, the tolerance is out of reach at this expansion_order and tightening cannot help)
-            const double err_full = spot_check(c, tr, L, xs, ys, zs, ms, n, m, nf, near, k, before, c->spot_checks);
+            const double err_full = spot_compare(c, tr, spot, n, nf, near, k, before);
             c->sel_full_err = err_full;
             if (err_full <= c->order_tol) {
                 const double f = err > 0.0 && std::isfinite(err) ? 0.8 * c->order_tol / err : 0.5;
