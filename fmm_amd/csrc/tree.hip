@@ -64,7 +64,7 @@ constexpr int CB_CHUNK = CB_THREADS * CB_PER;
 constexpr int CB_LDS = CB_CHUNK + CB_CHUNK / CB_PER;
 constexpr int CB_MIN_PROGRESS = 64;
 constexpr int CB_BURST = 4096;
-static_assert(CB_BURST <= CB_LDS, "a burst is staged in the block buffer");
+static_assert(CB_BURST + 32 <= CB_LDS, "a burst and its read-ahead fit in the block buffer");
 
 struct CentroidLds {
     double xs[CB_LDS];
@@ -176,13 +176,27 @@ __device__ double centroid_range(const double* __restrict__ col, int pos, int n,
                 for (int k = tid; k < blen; k += CB_THREADS) xs[k] = col[pos + k];
                 __syncthreads();
                 if (tid == 0) {
+                    // the dependent adds in order, each batch of 16 addends read from LDS while the previous
+                    // one is added (reads past blen stay inside xs, CB_LDS >= CB_BURST + 32, and are not added)
                     double t = s;
+                    double a[16], b[16];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) a[q] = xs[q];
                     int k = 0;
-                    for (; k + 8 <= blen; k += 8) {
-                        const double a0 = xs[k], a1 = xs[k + 1], a2 = xs[k + 2], a3 = xs[k + 3];
-                        const double a4 = xs[k + 4], a5 = xs[k + 5], a6 = xs[k + 6], a7 = xs[k + 7];
-                        t = t + a0; t = t + a1; t = t + a2; t = t + a3;
-                        t = t + a4; t = t + a5; t = t + a6; t = t + a7;
+                    for (; k + 32 <= blen; k += 32) {
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) b[q] = xs[k + 16 + q];
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) t = t + a[q];
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) a[q] = xs[k + 32 + q];
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) t = t + b[q];
+                    }
+                    if (k + 16 <= blen) {
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) t = t + a[q];
+                        k += 16;
                     }
                     for (; k < blen; ++k) t = t + xs[k];
                     s_next = t;
@@ -207,28 +221,32 @@ __global__ __launch_bounds__(CB_THREADS) void k_centroid(const double* __restric
 // Long inputs: the same sum with most of it in parallel.  The addends are cut into chunks of
 // CS_CHUNK; a first pass sums each chunk in any order (k_centroid_csum), so that the running sum at
 // the start of chunk c is known approximately, P_c.  A second pass (k_centroid_spec) forms the
-// chunk's approximate running sums P_k (a parallel scan) and from them the binades the exact sum will
-// most likely be in: one binade for the whole chunk, or one before and one after a single addend kb
-// where the running sum changes binade.  For each part it forms, exactly in int64, the partial sums
-// J_k of rint(x / u) in the part's unit u = 2^{e-52}, with their least and greatest values; addends
-// that are ties, non-finite or too large, or a second change of binade, mark the chunk.  The chain
-// (k_centroid_chain, one workgroup per axis) walks the chunks with the exact running sum s: when s
-// lies in a part's binade, s = S0 u, and S0 + J_k stays strictly inside (2^52, 2^53) in magnitude for
-// every k, every addition of the part was exact in the sense of the block method above and the part
-// ends at (S0 + J_last) u; the addend kb between the parts is added in FP64 like any other.  Whatever
-// fails a check -- the first chunk, chunks with several changes of binade, a wrong guess -- is summed
-// by the block method from s.  The result is the sequential sum bit for bit.
+// chunk's approximate running sums P_k and marks its special addends: those after which P_k is in
+// another binade than before, and those that would be a tie or too large in the unit u = 2^{e-52} of
+// their binade e.  Between the special addends (at most CS_MAXSP) lie runs of addends that keep one
+// binade; for each run it forms, exactly in int64, the partial sums J_k of rint(x / u), with their
+// least and greatest values.  The chain (k_centroid_chain, one workgroup per axis) walks the chunks
+// with the exact running sum s: when s lies in a run's binade, s = S0 u, and S0 + J_k stays strictly
+// inside (2^52, 2^53) in magnitude for every k, every addition of the run was exact in the sense of
+// the block method above and the run ends at (S0 + J_last) u; each special addend is added in FP64 as
+// the reference does.  Whatever fails a check -- the first chunk, chunks with more special addends, a
+// wrong guess -- is summed by the block method from s.  The result is the sequential sum bit for bit.
 constexpr int CS_CHUNK = 8192;
 constexpr int CS_T = 256;
 constexpr int CS_PER = CS_CHUNK / CS_T;   // consecutive addends per thread in the second pass
 constexpr int CS_LDS_CHUNKS = 256;
+constexpr int CS_MAXSP = 4;               // special addends per chunk (see k_centroid_spec)
+constexpr int CS_PARTS = CS_MAXSP + 1;
+constexpr long long CS_LIM = 1ll << 54;   // |J| beyond this cannot stay inside a binade
 struct CentroidPart {
     long long total, jmin, jmax;   // J over the part (jmin > jmax: empty part)
     int e;
 };
 struct CentroidChunk {
-    CentroidPart A, B;             // before and after the change of binade (B empty without one)
-    int kb;                        // chunk index of the addend between the parts, -1: none
+    CentroidPart part[CS_PARTS];   // the runs between the special addends
+    double xk[CS_MAXSP];           // the special addends
+    int kb[CS_MAXSP];              // their positions in the chunk, ascending
+    int nsp;
     int bad;
 };
 
@@ -294,14 +312,81 @@ __device__ __forceinline__ long long cs_minmax(long long v, long long* wsum) {
     return r;
 }
 
+// Block-wide exclusive scans and min / max of CS_PARTS values per thread (one pair of barriers each).
+__device__ __forceinline__ void cs_exscan_parts(const long long (&v)[CS_PARTS], long long (&off)[CS_PARTS],
+                                                long long (&tot)[CS_PARTS], long long* wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long long incl[CS_PARTS];
+#pragma unroll
+    for (int p = 0; p < CS_PARTS; ++p) {
+        incl[p] = v[p];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long y = __shfl_up(incl[p], d);
+            if (lane >= d) incl[p] += y;
+        }
+    }
+    __syncthreads();
+    if (lane == 63)
+#pragma unroll
+        for (int p = 0; p < CS_PARTS; ++p) wsum[w * CS_PARTS + p] = incl[p];
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < CS_PARTS; ++p) {
+        off[p] = incl[p] - v[p];
+        tot[p] = 0;
+        for (int k = 0; k < CS_T / 64; ++k) {
+            if (k < w) off[p] += wsum[k * CS_PARTS + p];
+            tot[p] += wsum[k * CS_PARTS + p];
+        }
+    }
+}
+__device__ __forceinline__ void cs_minmax_parts(long long (&mn)[CS_PARTS], long long (&mx)[CS_PARTS],
+                                                long long* wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int p = 0; p < CS_PARTS; ++p) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            mn[p] = min(mn[p], (long long)__shfl_xor(mn[p], d));
+            mx[p] = max(mx[p], (long long)__shfl_xor(mx[p], d));
+        }
+    }
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int p = 0; p < CS_PARTS; ++p) {
+            wsum[w * 2 * CS_PARTS + p] = mn[p];
+            wsum[w * 2 * CS_PARTS + CS_PARTS + p] = mx[p];
+        }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < CS_PARTS; ++p) {
+        mn[p] = wsum[p];
+        mx[p] = wsum[CS_PARTS + p];
+        for (int k = 1; k < CS_T / 64; ++k) {
+            mn[p] = min(mn[p], wsum[k * 2 * CS_PARTS + p]);
+            mx[p] = max(mx[p], wsum[k * 2 * CS_PARTS + CS_PARTS + p]);
+        }
+    }
+}
+
+__device__ __forceinline__ bool cs_unit_ok(double x, int e) {   // neither too large nor a tie in unit 2^{e-52}
+    const double y = ldexp(x, 52 - e);
+    return fabs(y) < 0x1p53 && fabs(y - rint(y)) != 0.5;
+}
+
 __global__ __launch_bounds__(CS_T) void k_centroid_spec(const double* __restrict__ X, int64_t ldx, int n, int nch,
                                                          const double* __restrict__ csum,
                                                          CentroidChunk* __restrict__ spec) {
     constexpr int LD = CS_PER + 1;   // padded rows: thread t reads its CS_PER addends without conflicts
     __shared__ double xs[CS_T * LD];
     __shared__ double wd[CS_T / 64];
-    __shared__ long long wl[CS_T / 64];
     __shared__ int wi[CS_T / 64];
+    __shared__ long long wl[CS_T / 64 * 2 * CS_PARTS];
+    __shared__ int lastb[CS_T];
+    __shared__ int s_kb[CS_MAXSP + 1], s_eb[CS_MAXSP + 1];
+    __shared__ double s_xk[CS_MAXSP + 1];
     const double* __restrict__ col = X + (int64_t)blockIdx.y * ldx;
     const int c = blockIdx.x, tid = threadIdx.x;
     CentroidChunk* out = spec + blockIdx.y * nch + c;
@@ -334,84 +419,113 @@ __global__ __launch_bounds__(CS_T) void k_centroid_spec(const double* __restrict
     double dummy;
     const double pre = pst + cs_exscan(loc, dummy, wd);
     const int e0 = binade_of(pst);
-    // changes of binade: count and the first position
-    int nchg = 0, kb = CS_CHUNK;
+    // binades after each addend; a thread's first addend compares with the previous thread's last
+    int bq[CS_PER];
     {
         double p = pre;
-        int bprev = k0 == 0 ? e0 : binade_of(p);
 #pragma unroll
         for (int q = 0; q < CS_PER; ++q) {
-            if (k0 + q >= len) break;
             p += x[q];
-            const int b = binade_of(p);
-            if (b != bprev) {
-                ++nchg;
-                kb = min(kb, k0 + q);
-            }
-            bprev = b;
+            bq[q] = binade_of(p);
         }
     }
-    int all_chg;
-    cs_exscan(nchg, all_chg, wi);
-    kb = (int)cs_minmax<false>((long long)kb, wl);
-    // the binade after the change: approximate running sum at kb
-    __shared__ int s_eb;
-    if (tid == 0) s_eb = 0;
+    lastb[tid] = bq[CS_PER - 1];
     __syncthreads();
-    if (kb >= k0 && kb < k0 + CS_PER) {
-        double p = pre;
-        for (int q = 0; q <= kb - k0; ++q) p += x[q];
-        s_eb = binade_of(p);
-    }
-    __syncthreads();
-    const int eA = e0, eB = s_eb;
-    int bad = (c == 0 || all_chg > 1 || e0 == -100000 || (all_chg == 1 && eB == -100000)) ? 1 : 0;
-    if (!bad) {
-        // exact increments: part A before kb in unit 2^{eA-52}, part B after it in unit 2^{eB-52}
-        constexpr long long LIM = 1ll << 54;   // |J| beyond this cannot stay inside the binade
-        long long ra = 0, rb = 0;
-        long long amin = LLONG_MAX, amax = LLONG_MIN, bmin = LLONG_MAX, bmax = LLONG_MIN;
+    int nsp = 0, sentinel = 0;
+    {
+        int bprev = tid == 0 ? e0 : lastb[tid - 1];
 #pragma unroll
         for (int q = 0; q < CS_PER; ++q) {
-            const int k = k0 + q;
-            if (k >= len || k == kb) continue;
-            const bool partA = k < kb;
-            const double y = ldexp(x[q], 52 - (partA ? eA : eB));
-            const double r = rint(y);
-            if (!(fabs(y) < 0x1p53) || fabs(y - r) == 0.5) {
-                bad = 1;
-                continue;
+            if (k0 + q < len) {
+                sentinel |= bq[q] == -100000;
+                nsp += (bq[q] != bprev || !cs_unit_ok(x[q], bq[q])) ? 1 : 0;
             }
-            if (partA) {
-                ra += (long long)r;
-                bad |= ra > LIM || ra < -LIM;
-                amin = min(amin, ra);
-                amax = max(amax, ra);
-            } else {
-                rb += (long long)r;
-                bad |= rb > LIM || rb < -LIM;
-                bmin = min(bmin, rb);
-                bmax = max(bmax, rb);
+            bprev = bq[q];
+        }
+    }
+    int all_sp;
+    const int sp_off = cs_exscan(nsp, all_sp, wi);
+    int anysent;
+    cs_exscan(sentinel, anysent, wi);
+    const bool bad0 = c == 0 || e0 == -100000 || anysent != 0 || all_sp > CS_MAXSP;
+    if (bad0) {
+        if (tid == 0) out->bad = 1;
+        return;   // uniform across the workgroup
+    }
+    // the special addends' positions, values and the binades after them
+    if (tid == 0) {
+        s_eb[0] = e0;
+        for (int i = 0; i <= CS_MAXSP; ++i) s_kb[i] = INT_MAX;
+    }
+    __syncthreads();
+    {
+        int bprev = tid == 0 ? e0 : lastb[tid - 1], i = sp_off;
+#pragma unroll
+        for (int q = 0; q < CS_PER; ++q) {
+            if (k0 + q < len && (bq[q] != bprev || !cs_unit_ok(x[q], bq[q]))) {
+                s_kb[i] = k0 + q;
+                s_xk[i] = x[q];
+                s_eb[i + 1] = bq[q];
+                ++i;
+            }
+            bprev = bq[q];
+        }
+    }
+    __syncthreads();
+    // exact increments per run (the run of an addend: the number of special addends before it)
+    long long T[CS_PARTS], mn[CS_PARTS], mx[CS_PARTS];
+#pragma unroll
+    for (int p = 0; p < CS_PARTS; ++p) {
+        T[p] = 0;
+        mn[p] = LLONG_MAX;
+        mx[p] = LLONG_MIN;
+    }
+    int bad = 0;
+#pragma unroll
+    for (int q = 0; q < CS_PER; ++q) {
+        const int k = k0 + q;
+        if (k >= len) continue;
+        int part = 0;
+        bool special = false;
+#pragma unroll
+        for (int i = 0; i < CS_MAXSP; ++i) {
+            part += k > s_kb[i] ? 1 : 0;
+            special |= k == s_kb[i];
+        }
+        if (special) continue;
+        const long long r = (long long)rint(ldexp(x[q], 52 - s_eb[part]));
+#pragma unroll
+        for (int p = 0; p < CS_PARTS; ++p) {
+            if (p == part) {
+                T[p] += r;
+                bad |= T[p] > CS_LIM || T[p] < -CS_LIM;
+                mn[p] = min(mn[p], T[p]);
+                mx[p] = max(mx[p], T[p]);
             }
         }
-        long long ta, tb;
-        const long long oa = cs_exscan(ra, ta, wl), ob = cs_exscan(rb, tb, wl);
-        const long long gamin = cs_minmax<false>(amin == LLONG_MAX ? LLONG_MAX : oa + amin, wl);
-        const long long gamax = cs_minmax<true>(amax == LLONG_MIN ? LLONG_MIN : oa + amax, wl);
-        const long long gbmin = cs_minmax<false>(bmin == LLONG_MAX ? LLONG_MAX : ob + bmin, wl);
-        const long long gbmax = cs_minmax<true>(bmax == LLONG_MIN ? LLONG_MIN : ob + bmax, wl);
-        int anybad;
-        cs_exscan(bad, anybad, wi);
-        if (tid == 0) {
-            CentroidChunk r;
-            r.A = CentroidPart{ta, gamin, gamax, eA};
-            r.B = CentroidPart{tb, gbmin, gbmax, eB};
-            r.kb = all_chg == 1 ? kb : -1;
-            r.bad = anybad ? 1 : 0;
-            *out = r;
+    }
+    long long off[CS_PARTS], tot[CS_PARTS];
+    cs_exscan_parts(T, off, tot, wl);
+#pragma unroll
+    for (int p = 0; p < CS_PARTS; ++p) {
+        if (mn[p] != LLONG_MAX) mn[p] += off[p];
+        if (mx[p] != LLONG_MIN) mx[p] += off[p];
+    }
+    cs_minmax_parts(mn, mx, wl);
+    int anybad;
+    cs_exscan(bad, anybad, wi);
+    if (tid == 0) {
+        CentroidChunk r;
+#pragma unroll
+        for (int p = 0; p < CS_PARTS; ++p) r.part[p] = CentroidPart{tot[p], mn[p], mx[p], p <= all_sp ? s_eb[p] : 0};
+#pragma unroll
+        for (int i = 0; i < CS_MAXSP; ++i) {
+            r.kb[i] = i < all_sp ? s_kb[i] : -1;
+            r.xk[i] = i < all_sp ? s_xk[i] : 0.0;
         }
-    } else if (tid == 0) {
-        out->bad = 1;
+        r.nsp = all_sp;
+        r.bad = anybad ? 1 : 0;
+        *out = r;
     }
 }
 
@@ -437,16 +551,18 @@ __global__ __launch_bounds__(CB_THREADS) void k_centroid_chain(const double* __r
     __syncthreads();
     double s = 0.0;
     for (int c = 0; c < nch; ++c) {
-        const CentroidChunk sp = c < CS_LDS_CHUNKS ? sspec[c] : spec[blockIdx.x * nch + c];
+        const CentroidChunk* sp = c < CS_LDS_CHUNKS ? &sspec[c] : &spec[blockIdx.x * nch + c];
         const int base = c * CS_CHUNK, end = min(n, base + CS_CHUNK);
         int from = base;   // where the block method takes over, end: not needed
-        if (!sp.bad) {
-            if (centroid_part(s, sp.A)) {
-                from = end;
-                if (sp.kb >= 0) {
-                    s = s + col[base + sp.kb];
-                    from = centroid_part(s, sp.B) ? end : base + sp.kb + 1;
+        if (!sp->bad) {
+            from = end;
+            const int nsp = sp->nsp;
+            for (int p = 0; p <= nsp; ++p) {
+                if (!centroid_part(s, sp->part[p])) {
+                    from = base + (p == 0 ? 0 : sp->kb[p - 1] + 1);
+                    break;
                 }
+                if (p < nsp) s = s + sp->xk[p];
             }
         }
         if (from < end) s = centroid_range(col, from, end, s, L);
@@ -887,26 +1003,47 @@ __global__ __launch_bounds__(256) void k_list_sort(const unsigned* __restrict__ 
 
 // Exclusive prefix sums of n ints in one workgroup of 1024 threads (each sums a contiguous run, then the
 // runs' totals are scanned in LDS): one launch, deterministic, for the small arrays of the set-up.
+// Thread t takes the contiguous run [t per, (t + 1) per): up to 16 values are loaded at once into
+// registers (all loads in flight together), run totals are scanned with wavefront shuffles and one LDS
+// step across the 16 wavefronts, and the run is written from the registers.
+constexpr int EXSCAN_REG = 16;
 __global__ __launch_bounds__(1024) void k_exscan_small(const int* __restrict__ in, int* __restrict__ out, int n) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n + 1023) / 1024;
     const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+    int v[EXSCAN_REG];
     int sum = 0;
-    for (int i = b0; i < b1; ++i) sum += in[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {   // inclusive Hillis-Steele scan of the run totals
-        const int v = t >= d ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    if (per <= EXSCAN_REG) {
+#pragma unroll
+        for (int q = 0; q < EXSCAN_REG; ++q) v[q] = b0 + q < b1 ? in[b0 + q] : 0;
+#pragma unroll
+        for (int q = 0; q < EXSCAN_REG; ++q) sum += v[q];
+    } else {
+        for (int i = b0; i < b1; ++i) sum += in[i];
     }
-    int run = part[t] - sum;
-    for (int i = b0; i < b1; ++i) {
-        const int v = in[i];
-        out[i] = run;
-        run += v;
+    int x = sum;   // inclusive scan of the run totals in the wavefront
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int run = x - sum;
+    for (int k = 0; k < w; ++k) run += wsum[k];
+    if (per <= EXSCAN_REG) {
+#pragma unroll
+        for (int q = 0; q < EXSCAN_REG; ++q) {
+            if (b0 + q < b1) out[b0 + q] = run;
+            run += v[q];
+        }
+    } else {
+        for (int i = b0; i < b1; ++i) {
+            const int u = in[i];
+            out[i] = run;
+            run += u;
+        }
     }
 }
 
