@@ -583,6 +583,44 @@ __device__ __forceinline__ int packed_of(int d) {
 // One workgroup per (node, 16 sets): the sets' expansions are loaded into LDS along the coefficients
 // (coalesced) and written along the sets.
 constexpr int FOLD_SETS = 16;
+// C(k, i) for the folds: k <= 5 at orders <= 10 (2k <= the degree)
+__constant__ double kFoldBinom[6][6] = {{1, 0, 0, 0, 0, 0},   {1, 1, 0, 0, 0, 0},    {1, 2, 1, 0, 0, 0},
+                                        {1, 3, 3, 1, 0, 0},   {1, 4, 6, 4, 1, 0},    {1, 5, 10, 10, 5, 1}};
+
+// The fold of one reduced source component s = (X, Y, rho): sum_{k, i} (-1)^k C(k, i) c M(X - 2i, Y - 2(k - i),
+// rho + 2k), with c M read through cms(packed index).
+template <class F>
+__device__ __forceinline__ double fold_component(int X, int Y, int rho, F cms) {
+    double acc = 0.0;
+    for (int k = 0; 2 * k <= X + Y; ++k) {
+        const double sg = (k & 1) ? -1.0 : 1.0;
+        for (int i = 0; i <= k; ++i) {
+            const int sx = X - 2 * i, sy = Y - 2 * (k - i);
+            if (sx < 0 || sy < 0) continue;
+            acc += (sg * kFoldBinom[k][i]) * cms(xyz2idx(sx, sy, rho + 2 * k));
+        }
+    }
+    return acc;
+}
+
+// One set: one thread per (node, reduced component), output CMs[node][rdim(p)].
+__global__ __launch_bounds__(256) void k_scale_sources_red1(const double* __restrict__ CM, const double* __restrict__ cs,
+                                                            const double* __restrict__ r, double* __restrict__ CMs,
+                                                            int num_nodes, int p) {
+    const int nr = rdim(p), nm = ncomp(p);
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (int64_t)num_nodes * nr) return;
+    const int j = (int)(x / nr), sr = (int)(x - (int64_t)j * nr);
+    const double* __restrict__ m = CM + (int64_t)j * nm;
+    const int d = decode_reduced(sr);
+    const int deg = d & 255, w = (d >> 8) & 255, rho = d >> 16;
+    const double acc = fold_component(deg - w, w - rho, rho, [&](int sc) { return cs[sc] * m[sc]; });
+    const double rinv = 1.0 / r[j];
+    double f = 1.0;
+    for (int q = 0; q < deg; ++q) f *= rinv;
+    CMs[x] = acc * f;
+}
+
 __global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restrict__ CM, const double* __restrict__ cs,
                                                            const double* __restrict__ r, double* __restrict__ CMs,
                                                            int num_nodes, int p, int nsets, int s_ld) {
@@ -602,19 +640,7 @@ __global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restr
         if (set0 + u < nsets) {
             const int d = decode_reduced(sr);
             const int deg = d & 255, w = (d >> 8) & 255, rho = d >> 16;
-            const int X = deg - w, Y = w - rho;
-            double acc = 0.0;
-            for (int k = 0; 2 * k <= X + Y; ++k) {
-                const double sg = (k & 1) ? -1.0 : 1.0;
-                double binom = 1.0;
-                for (int i = 0; i <= k; ++i) {
-                    if (i > 0) binom = binom * (k - i + 1) / i;
-                    const int sx = X - 2 * i, sy = Y - 2 * (k - i);
-                    if (sx < 0 || sy < 0) continue;
-                    const int sc = xyz2idx(sx, sy, rho + 2 * k);
-                    acc += (sg * binom) * (cs[sc] * cm[u][sc]);
-                }
-            }
+            const double acc = fold_component(deg - w, w - rho, rho, [&](int sc) { return cs[sc] * cm[u][sc]; });
             double f = 1.0;
             for (int q = 0; q < deg; ++q) f *= rinv;
             out = acc * f;
@@ -644,11 +670,8 @@ __global__ __launch_bounds__(256) void k_le_unfold(double* __restrict__ LE, int6
             continue;
         }
         const int tx = deg - w, ty = w - tz, k = tz >> 1, rho = tz & 1;
-        double acc = 0.0, binom = 1.0;
-        for (int i = 0; i <= k; ++i) {
-            if (i > 0) binom = binom * (k - i + 1) / i;
-            acc += binom * lr[rw][xyz2idx(tx + 2 * i, ty + 2 * (k - i), rho)];
-        }
+        double acc = 0.0;
+        for (int i = 0; i <= k; ++i) acc += kFoldBinom[k][i] * lr[rw][xyz2idx(tx + 2 * i, ty + 2 * (k - i), rho)];
         base[x] = (k & 1) ? -acc : acc;
     }
 }
@@ -656,8 +679,14 @@ __global__ __launch_bounds__(256) void k_le_unfold(double* __restrict__ LE, int6
 void launch_scale_sources_red(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
                               int nsets, int s_ld, hipStream_t s) {
     if (num_nodes <= 0) return;
-    hipLaunchKernelGGL(k_scale_sources_red, dim3(num_nodes, (s_ld + FOLD_SETS - 1) / FOLD_SETS), dim3(256), 0, s,
-                       CM, cs, r, CMs, num_nodes, p, nsets, s_ld);
+    if (nsets == 1 && s_ld == 1) {
+        const int64_t tot = (int64_t)num_nodes * rdim(p);
+        hipLaunchKernelGGL(k_scale_sources_red1, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, CM, cs, r, CMs,
+                           num_nodes, p);
+    } else {
+        hipLaunchKernelGGL(k_scale_sources_red, dim3(num_nodes, (s_ld + FOLD_SETS - 1) / FOLD_SETS), dim3(256), 0, s,
+                           CM, cs, r, CMs, num_nodes, p, nsets, s_ld);
+    }
     FMMH_HIP(hipGetLastError());
 }
 
