@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_near_water200k_thole_p6.md, valid for the near-field sources it
 # was measured on: K_NEAR_TRAFFIC_SRC is the sha256 prefix of those sources
 # (near_src_sha()); with other sources the bench line reports null.
-K_NEAR_TRAFFIC_BYTES = 146452919     # 59.2 MB read + 87.3 MB written
+K_NEAR_TRAFFIC_BYTES = 148291317     # 61.6 MB read + 86.7 MB written
 K_NEAR_TRAFFIC_SRC = "a16195073b28ddff"
 NEAR_SOURCES = ("fmm_amd/csrc/near_kernels.h", "fmm_amd/csrc/near.hip", "fmm_amd/csrc/tensors.h")
 
@@ -507,6 +507,10 @@ def main():
                     help="theta,cap settings of the batched side (';'-separated)")
     ap.add_argument("--target-ncrit", default="64",
                     help="leaf sizes of both sides of that comparison (','-separated)")
+    ap.add_argument("--target-margin", default="0.25,0.1",
+                    help="order margins of the batched side (','-separated)")
+    ap.add_argument("--target-safety", default="1,2",
+                    help="order safety factors of the batched side (','-separated)")
     ap.add_argument("--damped-target-only", action="store_true",
                     help="only run the <= 1e-6 comparison on the damped quadrupole workload and print it")
     ap.add_argument("--plain-sweep", action="store_true",
@@ -516,9 +520,12 @@ def main():
         return tuple((float(a.split(",")[0]), int(a.split(",")[1])) for a in arg.split(";") if a)
     tp, tb_ = settings(args.target_plain), settings(args.target_batched)
     tn = tuple(int(v) for v in args.target_ncrit.split(",") if v)
+    tmg = tuple(float(v) for v in args.target_margin.split(",") if v)
+    tsf = tuple(float(v) for v in args.target_safety.split(",") if v)
     if args.target_only:
         import torch
-        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_, ncrits=tn)))
+        print(json.dumps(target_comparison(torch.device("cuda:0"), args.steps, tp, tb_, ncrits=tn, margins=tmg,
+                                           safeties=tsf)))
         return
     if args.damped_target_only:
         import torch
@@ -739,7 +746,7 @@ def main():
         }
     cpu = None if args.no_cpu else cpu_baseline()
     target = (None if (args.no_target or args.config != 1)
-              else target_comparison(dev, min(args.steps, 3), tp, tb_, ncrits=tn))
+              else target_comparison(dev, min(args.steps, 3), tp, tb_, ncrits=tn, margins=tmg, safeties=tsf))
     damped = None if (args.no_target or args.config != 1) else damped_comparison(dev, min(args.steps, 3))
     line = {
         "metric": "site-field evals/s at <=1e-6 rel err vs FP64 direct; % of FP64/FP32 peak",
