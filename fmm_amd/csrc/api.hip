@@ -41,6 +41,7 @@ struct Tables {
     float* fbuf = nullptr;
     const float* m2m_frag = nullptr;  // mixed precision: M2M / L2L matrices per octant, MFMA fragment order
     const float* l2l_frag = nullptr;
+    const float* l2l_frag_red = nullptr;   // L2L of harmonic expansions on the (p + 1)^2 reduced components
     ShiftTab p2m{}, m2m{}, l2l{};
     const int* mono_parent = nullptr;
     const int* mono_axis = nullptr;
@@ -486,7 +487,7 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     // octant bits; lane l of the fragment (mt, ks) holds S[16 mt + l % 16][4 ks + l / 16].
     if (p >= 1) {
         const int KS = (nm + 3) / 4;
-        const size_t nfrag = (size_t)shift_mfma_frag_floats(p);
+        const size_t nfrag = (size_t)shift_mfma_frag_floats(p, false);
         std::vector<float> hf(2 * 8 * nfrag, 0.f);
         auto fill = [&](float* dst, const std::vector<int>& off, const std::vector<int>& ss,
                         const std::vector<int>& ee, const std::vector<double>& cc, bool l2l, int o) {
@@ -506,10 +507,58 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
             fill(hf.data() + (size_t)o * nfrag, m2m_off, m2m_s, m2m_e, m2m_c, false, o);
             fill(hf.data() + (size_t)(8 + o) * nfrag, l2l_off, l2l_s, l2l_e, l2l_c, true, o);
         }
+        // Reduced L2L for harmonic local expansions: R S U per octant, with U the unfold (component
+        // (x, y, 2k + rho) = (-1)^k sum_i C(k, i) reduced (x + 2i, y + 2(k - i), rho)) and R the
+        // restriction to the reduced rows; reduced order: by degree n, then w = y + z, then z.
+        const bool red_ok = p <= M2L_GEMM_PMAX;
+        const size_t nfr = red_ok ? (size_t)shift_mfma_frag_floats(p, true) : 0;
+        if (red_ok) {
+            const int nk = (p + 1) * (p + 1), KSr = (nk + 3) / 4;
+            auto ridx = [](int X, int Y, int rho) {
+                const int n = X + Y + rho, w = Y + rho;
+                return n * n + (w == 0 ? 0 : 2 * w - 1 + rho);
+            };
+            std::vector<int> rfull(nk);
+            for (int u = 0; u < nm; ++u)
+                if (comp[u].z <= 1) rfull[ridx(comp[u].x, comp[u].y, comp[u].z)] = u;
+            std::vector<double> U((size_t)nm * nk, 0.0);
+            for (int u = 0; u < nm; ++u) {
+                const int k = comp[u].z / 2, rho = comp[u].z % 2;
+                double binom = 1.0;
+                for (int i = 0; i <= k; ++i) {
+                    if (i > 0) binom = binom * (k - i + 1) / i;
+                    U[(size_t)u * nk + ridx(comp[u].x + 2 * i, comp[u].y + 2 * (k - i), rho)] += ((k & 1) ? -1.0 : 1.0) * binom;
+                }
+            }
+            hf.resize(hf.size() + 8 * nfr, 0.f);
+            std::vector<double> S((size_t)nm * nm);
+            for (int o = 0; o < 8; ++o) {
+                const int ux = (o & 1) ? 1 : -1, uy = (o & 2) ? 1 : -1, uz = (o & 4) ? 1 : -1;
+                std::fill(S.begin(), S.end(), 0.0);
+                for (int tt = 0; tt < nm; ++tt)
+                    for (int b = l2l_off[tt]; b < l2l_off[tt + 1]; ++b) {
+                        const int sv = l2l_s[b];
+                        const XYZ& E = comp[l2l_e[b]];
+                        const XYZ& T = comp[sv];
+                        const double sign = ((E.x & 1) ? ux : 1) * ((E.y & 1) ? uy : 1) * ((E.z & 1) ? uz : 1);
+                        S[(size_t)tt * nm + sv] = l2l_c[b] * sign * std::ldexp(1.0, -(T.x + T.y + T.z));
+                    }
+                float* dst = hf.data() + 16 * nfrag + (size_t)o * nfr;
+                for (int tr = 0; tr < nk; ++tr) {
+                    const double* Srow = S.data() + (size_t)rfull[tr] * nm;
+                    for (int sr = 0; sr < nk; ++sr) {
+                        double a = 0.0;
+                        for (int u = 0; u < nm; ++u) a += Srow[u] * U[(size_t)u * nk + sr];
+                        dst[((size_t)(tr / 16) * KSr + sr / 4) * 64 + tr % 16 + 16 * (sr % 4)] = (float)a;
+                    }
+                }
+            }
+        }
         FMMH_HIP(hipMalloc((void**)&t.fbuf, hf.size() * sizeof(float)));
         FMMH_HIP(hipMemcpy(t.fbuf, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice));
         t.m2m_frag = t.fbuf;
         t.l2l_frag = t.fbuf + 8 * nfrag;
+        if (red_ok) t.l2l_frag_red = t.fbuf + 16 * nfrag;
     }
     return c->tables.emplace(key, t).first->second;
 }
@@ -1476,8 +1525,12 @@ void m2l_classes_end(fmmh_context* c, int p, bool pending, M2LClassArgs& ca) {
     c->cache_classes = ncls;
 }
 
+// unfold = false: the harmonic engines leave the local-expansion components with t_z >= 2 unset (for a
+// downward pass that reads only the reduced components: the reduced L2L and the many-set L2P); the
+// target-block form (one set) always completes them.
 void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int nsets, const double* CM, double* LE,
-             bool classes_cached, const Periodic& P, const double* disp, const unsigned char* order = nullptr) {
+             bool classes_cached, const Periodic& P, const double* disp, const unsigned char* order = nullptr,
+             bool unfold = true) {
     hipStream_t s = c->stream;
     const int nn = tr.num_nodes, np = L.n_m2l;
     const int nm = ncomp(p), nT = ncomp(2 * p);
@@ -1611,7 +1664,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
                            order, tb.gemm_tfull};
             launch_m2l_gemm(ga, grid, s);
             launch_m2l_reduce(ga, L.m2l_off, tr.t.r, LE + set * node_stride, nn, s);
-            launch_le_unfold(LE + set * node_stride, nn, p, s);
+            if (unfold) launch_le_unfold(LE + set * node_stride, nn, p, s);
         }
         return;
     }
@@ -1645,7 +1698,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         auto it = c->tuned.find(key);
         if (it != c->tuned.end() || !c->autotune || c->capturing) {
             launch_m2l_mfma(ma, nn, it != c->tuned.end() ? it->second : m2l_mfma_default_variant(ma), s);
-            launch_le_unfold(LE + c0 * node_stride, (long long)ns * nn, p, s);
+            if (unfold) launch_le_unfold(LE + c0 * node_stride, (long long)ns * nn, p, s);
             continue;
         }
         // first call of this shape: time every valid variant on this call's pairs; each one writes
@@ -1669,7 +1722,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
             if (ms < best_ms) { best_ms = ms; best = v; }
         }
         c->tuned[key] = best;
-        launch_le_unfold(LE + c0 * node_stride, (long long)ns * nn, p, s);
+        if (unfold) launch_le_unfold(LE + c0 * node_stride, (long long)ns * nn, p, s);
     }
 }
 
@@ -2081,7 +2134,12 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     launch_near_part(na, ngrid, m, k.F, k.damp, 2, s);
     c->mark("p2p_excl");
     const unsigned char* order = select ? select_orders(c, tr, L, tb, p, S, CM, near, n, nf) : nullptr;
-    run_m2l(c, tr, L, tb, p, S, CM, LE, true, P, disp, order);
+    // many sets in mixed precision: the downward pass runs on the reduced (harmonic) components -- the FP32
+    // L2L shifts with the reduced matrices and the matrix-core L2P read only those -- so the M2L leaves the
+    // components with t_z >= 2 unset
+    const bool reduced_down = shift32 && S >= EXP_MFMA_MIN_SETS && p <= EXP_MFMA_PMAX && (nf == 1 || nf == 4) &&
+                              !P.on && tb.l2l_frag_red && tb.gemm_tfull;
+    run_m2l(c, tr, L, tb, p, S, CM, LE, true, P, disp, order, !reduced_down);
     c->mark("m2l");
     if (P.on) run_periodic(c, tr, tb, P, k, p, CM, LE);
     DownArgs da{tr.t.cx, tr.t.cy, tr.t.cz, tr.t.parent, LE, nm, p, tb.l2l, tb.l2l, tb.mono_parent, tb.mono_axis,
@@ -2095,7 +2153,8 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
             if (shift32) {   // the parents of this level's nodes, on FP32 MFMA (one octant per workgroup)
                 int o0 = tr.level_off[lev - 1], o1 = tr.level_off[lev];
                 ShiftMfmaArgs sa{inodes + o0, icnt + lev - 1, o1 - o0, tr.t.children, tr.t.r, LE, LE,
-                                 (int64_t)nn * nm, S, tb.l2l_frag, tb.xyz};
+                                 (int64_t)nn * nm, S, reduced_down ? tb.l2l_frag_red : tb.l2l_frag, tb.xyz,
+                                 reduced_down ? tb.gemm_tfull : nullptr};
                 launch_shift_mfma(sa, p, true, s);
             } else {
                 int o0 = tr.level_off[lev], o1 = tr.level_off[lev + 1];
@@ -2125,7 +2184,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
             FMMH_HIP(hipMemcpy2DAsync(k.field, k.ldf * sizeof(double), before, n * sizeof(double),
                                       n * sizeof(double), (size_t)nf * S, hipMemcpyDeviceToDevice, s));
             c->cache_classes = c->stats_classes;   // the class tensors of this call are in place
-            run_m2l(c, tr, L, tb, p, S, CM, LE, true, P, disp, nullptr);
+            run_m2l(c, tr, L, tb, p, S, CM, LE, true, P, disp, nullptr, !reduced_down);
             c->mark("m2l_fallback");
             downward();
             // the full-order result at the same sites; where it meets the tolerance, later calls get a

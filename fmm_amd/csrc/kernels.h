@@ -478,8 +478,11 @@ struct ShiftMfmaArgs {
     int nsets;
     const float* A;                // 8 shift matrices (one per octant) in MFMA fragment order
     const int* xyz;                // packed index -> ex | ey << 8 | ez << 16
+    // L2L of harmonic local expansions on their (p + 1)^2 components with t_z <= 1 (A: the reduced
+    // matrices): reduced index -> packed index; null: the full shift
+    const int* tfull;
 };
-int shift_mfma_frag_floats(int p);
+int shift_mfma_frag_floats(int p, bool reduced);
 void launch_shift_mfma(const ShiftMfmaArgs& a, int p, bool l2l, hipStream_t s);
 void launch_internal_nodes(int num_nodes, const int* level, const int* children, const LevelStarts& ls, int* out,
                            int* cnt, hipStream_t s);

@@ -38,15 +38,22 @@ constexpr int KC = 8;         // source steps whose A fragments are loaded toget
 // that the top levels, with a handful of nodes, do not serialise all tiles of all octants on four waves;
 // L2L also takes one octant per workgroup.  Every output element is still computed by one wavefront in
 // the same order.
-constexpr int shift_row_groups(int p) { return (((ncomp(p) + 15) / 16) + 7) / 8; }
+constexpr int shift_row_groups(int nrows) { return (((nrows + 15) / 16) + 7) / 8; }
 
-template <int P, bool L2L>
+// RED (L2L only): the local expansions are harmonic and the shift runs on their (p + 1)^2 components with
+// t_z <= 1 (read and written at their packed positions a.tfull[k]; the other components are neither read
+// nor kept up to date) with the matrices R S U -- the full shift S between the unfold U (components with
+// t_z >= 2 from the reduced ones, exact for a harmonic expansion) and the restriction R to the reduced
+// rows; a harmonic parent gives a harmonic child, so the reduced child is exact.
+template <int P, bool L2L, bool RED>
 __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
-    constexpr int NM = ncomp(P);
+    static_assert(!RED || L2L, "the reduced form is for local expansions");
+    constexpr int NMS = ncomp(P);               // row length of the expansions in memory
+    constexpr int NM = RED ? rdim(P) : NMS;     // rows and k of the shift
     constexpr int MT = (NM + 15) / 16;          // 16-row target tiles
     constexpr int KS = (NM + 3) / 4;            // 4-deep source steps
     constexpr int KP = KS * 4;
-    constexpr int G = shift_row_groups(P);      // workgroups sharing the row tiles of one column block
+    constexpr int G = shift_row_groups(NM);     // workgroups sharing the row tiles of one column block
     constexpr int MW = (MT + 4 * G - 1) / (4 * G);   // target tiles per wavefront
     constexpr int NT = SHIFT_NT;
     constexpr int NC = 16 * NT;                 // columns per workgroup
@@ -55,6 +62,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
     __shared__ double rin[NC][P + 1];           // r_child^-k = (2 / r_parent)^k
     __shared__ int cnode[NC], cset[NC];
     __shared__ int deg[KP];
+    __shared__ int pos[RED ? KP : 1];           // packed position of reduced component k
     const int count = *a.count;                 // internal nodes of the level (device count)
     const int ncol = count * a.nsets;
     const int col0 = blockIdx.x * NC;
@@ -78,8 +86,15 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
         }
     }
     for (int k = tid; k < KP; k += 256) {
-        const int x = k < NM ? a.xyz[k] : 0;
-        deg[k] = (x & 255) + ((x >> 8) & 255) + (x >> 16);
+        if constexpr (RED) {
+            const int f = k < NM ? a.tfull[k] : 0;
+            const int x = a.xyz[f];
+            deg[k] = k < NM ? (x & 255) + ((x >> 8) & 255) + (x >> 16) : 0;
+            pos[k] = f;
+        } else {
+            const int x = k < NM ? a.xyz[k] : 0;
+            deg[k] = (x & 255) + ((x >> 8) & 255) + (x >> 16);
+        }
     }
     __syncthreads();
     // the columns' source expansions, box-scaled, FP32: M2M M_child(s) r_child^-|s|, L2L L_parent(s) r_parent^|s|
@@ -91,7 +106,8 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
             if (node >= 0 && k < NM) {
                 const int sn = L2L ? node : a.children[node * 8 + o];
                 if (sn >= 0) {
-                    const double e = a.src[(int64_t)cset[c] * a.set_stride + (int64_t)sn * NM + k];
+                    const int kk = RED ? pos[k] : k;
+                    const double e = a.src[(int64_t)cset[c] * a.set_stride + (int64_t)sn * NMS + kk];
                     v = (float)(e * (L2L ? rpw[c][deg[k]] : rin[c][deg[k]]));
                 }
             }
@@ -172,7 +188,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
                 const int node = cnode[c];
                 const int ch = node >= 0 ? a.children[node * 8 + o] : -1;
                 if (ch < 0) continue;
-                double* __restrict__ dst = a.dst + (int64_t)cset[c] * a.set_stride + (int64_t)ch * NM;
+                double* __restrict__ dst = a.dst + (int64_t)cset[c] * a.set_stride + (int64_t)ch * NMS;
 #pragma unroll
                 for (int mi = 0; mi < MW; ++mi) {
                     const int mt = mrow + 4 * G * mi;
@@ -180,7 +196,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int t = mt * 16 + 4 * g4 + q;
-                        if (t < NM) dst[t] += acc64[mi][nt][q] * rin[c][deg[t]];
+                        if (t < NM) dst[RED ? pos[t] : t] += acc64[mi][nt][q] * rin[c][deg[t]];
                     }
                 }
             }
@@ -192,7 +208,7 @@ __global__ __launch_bounds__(256) void k_shift_mfma(ShiftMfmaArgs a) {
             const int c = nt * 16 + c16;
             const int node = cnode[c];
             if (node < 0) continue;
-            double* __restrict__ dst = a.dst + (int64_t)cset[c] * a.set_stride + (int64_t)node * NM;
+            double* __restrict__ dst = a.dst + (int64_t)cset[c] * a.set_stride + (int64_t)node * NMS;
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi) {
                 const int mt = mrow + 4 * G * mi;
@@ -242,33 +258,34 @@ void launch_internal_nodes(int num_nodes, const int* level, const int* children,
     FMMH_HIP(hipGetLastError());
 }
 
-int shift_mfma_frag_floats(int p) {
-    const int nm = ncomp(p);
+int shift_mfma_frag_floats(int p, bool reduced) {
+    const int nm = reduced ? rdim(p) : ncomp(p);
     return ((nm + 15) / 16) * ((nm + 3) / 4) * 64;
 }
 
-template <bool L2L>
+template <bool L2L, bool RED>
 static void launch_p(const ShiftMfmaArgs& a, int p, hipStream_t s) {
     const dim3 g((unsigned)(((long long)a.max_count * a.nsets + 16 * SHIFT_NT - 1) / (16 * SHIFT_NT)),
-                 shift_row_groups(p), L2L ? 8 : 1);
+                 shift_row_groups(RED ? rdim(p) : ncomp(p)), L2L ? 8 : 1);
     switch (p) {
-        case 1: hipLaunchKernelGGL((k_shift_mfma<1, L2L>), g, dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_shift_mfma<2, L2L>), g, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_shift_mfma<3, L2L>), g, dim3(256), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_shift_mfma<4, L2L>), g, dim3(256), 0, s, a); break;
-        case 5: hipLaunchKernelGGL((k_shift_mfma<5, L2L>), g, dim3(256), 0, s, a); break;
-        case 6: hipLaunchKernelGGL((k_shift_mfma<6, L2L>), g, dim3(256), 0, s, a); break;
-        case 7: hipLaunchKernelGGL((k_shift_mfma<7, L2L>), g, dim3(256), 0, s, a); break;
-        case 8: hipLaunchKernelGGL((k_shift_mfma<8, L2L>), g, dim3(256), 0, s, a); break;
-        case 9: hipLaunchKernelGGL((k_shift_mfma<9, L2L>), g, dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_shift_mfma<10, L2L>), g, dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_shift_mfma<1, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_shift_mfma<2, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_shift_mfma<3, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_shift_mfma<4, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 5: hipLaunchKernelGGL((k_shift_mfma<5, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 6: hipLaunchKernelGGL((k_shift_mfma<6, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_shift_mfma<7, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_shift_mfma<8, L2L, RED>), g, dim3(256), 0, s, a); break;
+        case 9: hipLaunchKernelGGL((k_shift_mfma<9, L2L, RED>), g, dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_shift_mfma<10, L2L, RED>), g, dim3(256), 0, s, a); break;
     }
 }
 
 void launch_shift_mfma(const ShiftMfmaArgs& a, int p, bool l2l, hipStream_t s) {
     if (a.max_count <= 0 || p < 1 || p > PMAX) return;
-    if (l2l) launch_p<true>(a, p, s);
-    else launch_p<false>(a, p, s);
+    if (l2l && a.tfull) launch_p<true, true>(a, p, s);
+    else if (l2l) launch_p<true, false>(a, p, s);
+    else launch_p<false, false>(a, p, s);
     FMMH_HIP(hipGetLastError());
 }
 
