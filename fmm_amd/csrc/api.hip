@@ -105,6 +105,8 @@ struct fmmh_context {
     TreeOut cache_tr;
     ListsOut cache_L;
     int cache_class_p = -1;           // expansion order of the cached class tensors (-1: none)
+    int class_zmax = 0;               // their components with z > class_zmax are not computed
+    M2LClassArgs class_args{};        // the arguments they were computed with
     bool cache_work_valid = false;    // near-field block order of the cached tree ("s_work_blk", "s_inv")
     int near_nv = 0, near_grid = 0, near_np = 0;   // near-field chunks, workgroup slots, partial sites
     long long near_chunk_cap = 0;     // FMMH_OPT_NEAR_CHUNK (0: automatic)
@@ -1519,7 +1521,12 @@ void m2l_classes_end(fmmh_context* c, int p, bool pending, M2LClassArgs& ca) {
         }
         ncls = c->h_pinned[300];
         ca.Tc = c->buf<double>("m_tc", (size_t)ncls * ncomp(2 * p) + M2LB_READ_SLACK);
+        // the harmonic engines read the components with z <= 2 only; the VALU form (FMMH_OPT_M2L_MFMA = 0)
+        // reads them all (run_m2l completes reduced tensors before it runs)
+        ca.zmax = c->m2l_mfma != 0 ? 2 : 2 * p;
         launch_m2l_class_tensors(ca, std::min(ncls, 4096), c->stream);
+        c->class_zmax = ca.zmax;
+        c->class_args = ca;
     }
     c->stats_classes = ncls;
     c->cache_classes = ncls;
@@ -1690,6 +1697,13 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,
                    nm, p, ns, s_ld, nn, order, tb.gemm_fidx, CMs32, Tcp};
         if (!mfma) {
+            if (c->class_zmax < 2 * p && ncls > 0) {   // the VALU form reads every tensor component
+                M2LClassArgs fa = c->class_args;
+                fa.zmax = 2 * p;
+                fa.Tc = Tc;
+                launch_m2l_class_tensors(fa, std::min(ncls, 4096), s);
+                c->class_zmax = 2 * p;
+            }
             launch_m2l(ma, nn, s);
             continue;
         }

@@ -15,8 +15,10 @@ constexpr int HERMITE_KMAX = 2 * PMAX;
 
 __host__ __device__ constexpr int hermite_aux_size(int K) { return (K + 1) * (K + 2) * (K + 3) * (K + 4) / 24; }
 
+// zmax < K: only the entries with v <= zmax (the recursion for (t, u, v) reads entries with v' <= v, so
+// they are complete; the others are left unset).
 __device__ inline void hermite_aux(double vx, double vy, double vz, int K, double* A, const int* __restrict__ xyz,
-                                   const int* __restrict__ level_base) {
+                                   const int* __restrict__ level_base, int zmax = HERMITE_KMAX) {
     if ((int)threadIdx.x <= K) {
         const double R2 = vx * vx + vy * vy + vz * vz;
         const double inv = 1.0 / sqrt(R2);
@@ -28,13 +30,27 @@ __device__ inline void hermite_aux(double vx, double vy, double vz, int K, doubl
     __syncthreads();
     for (int k = 1; k <= K; ++k) {
         const int nd = (k + 1) * (k + 2) / 2, ndm1 = k * (k + 1) / 2, ndm2 = (k - 1) * k / 2;
-        const int nent = nd * (K - k + 1);
+        const int vmax = k < zmax ? k : zmax;
+        const int ndc = (vmax + 1) * (k + 1) - vmax * (vmax + 1) / 2;   // components with v <= vmax
+        const int nent = ndc * (K - k + 1);
         const int base = level_base[k], bm1 = level_base[k - 1], bm2 = k >= 2 ? level_base[k - 2] : 0;
         const int c0 = ncomp(k - 1), c1 = ncomp(k - 2), c2 = ncomp(k - 3);
         for (int x = threadIdx.x; x < nent; x += 64) {
-            const int n = x / nd, l = x - n * nd;
-            const int pk = xyz[c0 + l];
-            const int t = pk & 255, u = (pk >> 8) & 255, v = pk >> 16;
+            const int n = x / ndc;
+            int l = x - n * ndc, t, u, v;
+            if (vmax == k) {   // every component, in packed order
+                const int pk = xyz[c0 + l];
+                t = pk & 255; u = (pk >> 8) & 255; v = pk >> 16;
+            } else {           // v = 0, 1, .., vmax, each with its k - v + 1 components
+                v = 0;
+                while (l > k - v) {
+                    l -= k - v + 1;
+                    ++v;
+                }
+                u = l;
+                t = k - v - u;
+                l = xyz2idx(t, u, v) - c0;
+            }
             double val;
             if (t > 0) {
                 val = vx * A[bm1 + (n + 1) * ndm1 + (xyz2idx(t - 1, u, v) - c1)];

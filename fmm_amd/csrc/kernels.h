@@ -221,6 +221,8 @@ struct M2LClassArgs {
     int* error;                             // device scalar: displacement out of range
     double* Tc;                             // class tensors, ncomp(2p) each
     int p;
+    int zmax;                               // components with z > zmax are set to 0 (2: all the harmonic
+                                            // M2L engines read; 2p: every component)
     const int* xyz;                         // packed index -> t | u << 8 | v << 16
     const int* level_base;                  // aux slot base per degree (2p + 2 entries)
     int* class_dl;                          // class id -> level_i - level_j

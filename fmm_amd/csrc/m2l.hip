@@ -164,11 +164,12 @@ __global__ __launch_bounds__(64) void k_class_tensor(M2LClassArgs a) {
             vy = (double)((long long)((key >> (b + 8)) & mask) - voff);
             vz = (double)((long long)((key >> 8) & mask) - voff);
         }
-        hermite_aux(vx, vy, vz, K, A, sxyz, slb);
+        const int zmax = a.zmax;
+        hermite_aux(vx, vy, vz, K, A, sxyz, slb, zmax);
         for (int idx = threadIdx.x; idx < nT; idx += 64) {
             const int pk = sxyz[idx];
             const int k = (pk & 255) + ((pk >> 8) & 255) + (pk >> 16);
-            double val = hermite_T(A, idx, sxyz, slb);
+            double val = (pk >> 16) <= zmax ? hermite_T(A, idx, sxyz, slb) : 0.0;
             if (dl > 0) val = ldexp(val, dl * k);
             a.Tc[(int64_t)c * nT + idx] = val;
         }
