@@ -2139,6 +2139,7 @@ void run_fmm(fmmh_context* c, const Call& k, double theta, int ncrit, int p, con
     na.nsites = n;
     na.bad = bad_excl_flag(c);
     na.coinc_list = 0;   // sites at one position share a leaf
+    na.pairs2 = S >= EXP_MFMA_MIN_SETS ? 1 : 0;   // many sets: two targets per lane (16 sets and more)
     c->mark("p2p_order");
     launch_near_part(na, ngrid, m, k.F, k.damp, 1, s);
     if (np_sites > 0)

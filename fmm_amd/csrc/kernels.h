@@ -49,6 +49,7 @@ struct NearArgs {
     int* bad = nullptr;        // set to 1 when an exclusion entry is above nsites (null: not reported)
     int coinc_list = 0;        // sites at one position: 0 in the target block (octree leaves), 1 in its list
     int m_rt = 0, f_rt = 0;    // source and field orders (the runtime-order exclusion kernel)
+    int pairs2 = 0;            // many sets: two targets per lane where the orders allow (another summation order)
 };
 
 bool near_supported(int m, int F);

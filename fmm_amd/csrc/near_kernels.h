@@ -58,12 +58,20 @@ constexpr int near_tile(int snm) {
     return t;
 }
 
-template <int M, int F, int DAMP, int S>
+// T2 (many sets, closed-form contractions, field order <= 1, used for source order <= 1 at 3 wavefronts per
+// SIMD: 32 undamped dipole sets 32.8 -> 30.7 ms): each lane takes two targets of one group, ti and
+// ti + nh, so that every source's coordinates and S sets of multipoles read from LDS serve two pairs (the
+// 4-set form is bound by those LDS reads as much as by its arithmetic); groups then have nh = ceil(nt / 2)
+// lanes, and the sums run in another order than the one-target form's (a call selects it with
+// NearArgs::pairs2, so that results do not depend on it otherwise).
+template <int M, int F, int DAMP, int S, bool T2 = false>
 __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 : 1)
-                                        : ((M + F <= 3 && DAMP != DAMP_ERF) ? 2 : 1)) void k_near(NearArgs a) {
+                                        : (T2 ? 3 : ((M + F <= 3 && DAMP != DAMP_ERF) ? 2 : 1))) void k_near(NearArgs a) {
     constexpr int KT = pair_tensor_order<M, F, DAMP>();
     constexpr int NM = ncomp(M);
     constexpr int NF = ncomp(F);
+    static_assert(!T2 || (S > 1 && F <= 1 && has_closed_form<M, F>()), "two targets per lane: closed forms, F <= 1");
+    constexpr int TPL = T2 ? 2 : 1;             // targets per lane
     // Source tile: TILE sites, sized so that the staged multipoles take at most 8 KB (S NM TILE
     // doubles), which keeps several workgroups per CU; all 64 lanes prefetch the tile's multipoles
     // cooperatively (NPM registers each, element e = lane + 64 i of the [S][NM][TILE] block).
@@ -104,12 +112,12 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
     // plus about two iterations' worth of re-staging the tile (measured: weights 1 and 2 run
     // alike, 0 is 17 % slower) -- so that a block of, say, 40 targets runs as 2 passes of 20
     // (r = 3) instead of one pass with 24 idle lanes.
-    int ct = min(tc, 64);
+    int ct = min(tc, 64 * TPL);
     {
         int best = 1 << 30;
-        for (int c = (tc + 63) / 64; c <= 16; ++c) {
+        for (int c = (tc + 64 * TPL - 1) / (64 * TPL); c <= 16; ++c) {
             const int t = (tc + c - 1) / c;
-            const int rr = 64 / t;
+            const int rr = 64 / ((t + TPL - 1) / TPL);
             const int cost = c * ((64 + rr - 1) / rr + 2);
             if (cost < best) {
                 best = cost;
@@ -119,18 +127,29 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
     }
     for (int base = 0; base < tc; base += ct) {
         const int nt = min(ct, tc - base);
-        const int r = 64 / nt;                   // lane groups
-        const int g = lane / nt;                 // this lane's group (== r for idle lanes)
-        const int ti = lane - g * nt;            // this lane's target within the block
+        const int nh = (nt + TPL - 1) / TPL;     // lanes per group
+        const int r = 64 / nh;                   // lane groups
+        const int g = lane / nh;                 // this lane's group (== r for idle lanes)
+        const int ti = lane - g * nh;            // this lane's target within the block (and ti + nh with T2)
         const bool active = g < r;
         const int k = t0 + base + (active ? ti : 0);
         const double xi = a.xs[k], yi = a.ys[k], zi = a.zs[k];
         const double dfi = DAMP != DAMP_NONE ? a.dfs[k] : 0.0;
-        double acc[S][NF];
+        // the second target (T2): a lane without one repeats the first and its sums are not stored
+        const int k2 = t0 + base + ((active && ti + nh < nt) ? ti + nh : (active ? ti : 0));
+        const double xi2 = T2 ? a.xs[k2] : 0.0, yi2 = T2 ? a.ys[k2] : 0.0, zi2 = T2 ? a.zs[k2] : 0.0;
+        const double dfi2 = (T2 && DAMP != DAMP_NONE) ? a.dfs[k2] : 0.0;
+        double acc[S][NF], acc2[T2 ? S : 1][NF];
 #pragma unroll
         for (int u = 0; u < S; ++u)
 #pragma unroll
             for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
+        if constexpr (T2) {
+#pragma unroll
+            for (int u = 0; u < S; ++u)
+#pragma unroll
+                for (int t = 0; t < NF; ++t) acc2[u][t] = 0.0;
+        }
         for (int w0 = e0; w0 < e1; w0 += NEAR_WIN) {
             const int wn = min(NEAR_WIN, e1 - w0);
             int cnt = 0, st = 0, im = 0;
@@ -218,7 +237,26 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
                         if constexpr (DAMP != DAMP_NONE) aij = dfi * sdf[jj];
                         double gr[KT + 1];
                         radial_inv<KT, DAMP, DAMP != DAMP_NONE>(R2, inv, aij, gr, etab);
-                        if constexpr (has_closed_form<M, F>()) {
+                        if constexpr (T2) {
+                            const double ex = xi2 - sx[jj];
+                            const double ey = yi2 - sy[jj];
+                            const double ez = zi2 - sz[jj];
+                            const double Q2 = ex * ex + ey * ey + ez * ez;
+                            const double qinv = rsqrt_pos(Q2);
+                            const double inv2 = Q2 > 0.0 ? qinv : 0.0;
+                            double aij2 = 0.0;
+                            if constexpr (DAMP != DAMP_NONE) aij2 = dfi2 * sdf[jj];
+                            double gr2[KT + 1];
+                            radial_inv<KT, DAMP, DAMP != DAMP_NONE>(Q2, inv2, aij2, gr2, etab);
+#pragma unroll
+                            for (int u = 0; u < S; ++u) {
+                                double m[NM];
+#pragma unroll
+                                for (int s = 0; s < NM; ++s) m[s] = sm[u][s][jj];
+                                contract_closed<M, F>(dx, dy, dz, gr, m, acc[u]);
+                                contract_closed<M, F>(ex, ey, ez, gr2, m, acc2[u]);
+                            }
+                        } else if constexpr (has_closed_form<M, F>()) {
 #pragma unroll
                             for (int u = 0; u < S; ++u) {
                                 double m[NM];
@@ -242,28 +280,33 @@ __global__ __launch_bounds__(64, S == 1 ? ((M + F <= 3 && DAMP != DAMP_ERF) ? 4 
                 __syncthreads();
             }
         }
-        // sum the r partial fields of each target in group order, set by set
+        // sum the r partial fields of each target in group order, set by set (T2: the first targets, then
+        // the second ones)
 #pragma unroll
-        for (int u = 0; u < S; ++u) {
+        for (int h = 0; h < TPL; ++h) {
 #pragma unroll
-            for (int t = 0; t < NF; ++t) red[t][lane] = acc[u][t];
-            __syncthreads();
-            if (lane < nt && u < ns) {
-                const int kk = t0 + base + lane;
+            for (int u = 0; u < S; ++u) {
 #pragma unroll
-                for (int t = 0; t < NF; ++t) {
-                    double v = red[t][lane];
-                    for (int q = 1; q < r; ++q) v += red[t][lane + q * nt];
-                    if (pbase >= 0) {
-                        a.part[((int64_t)(set0 + u) * NF + t) * a.ldp + pbase + base + lane] = v;
-                    } else {
-                        double* o = out + ((int64_t)u * NF + t) * a.ldo + kk;
-                        if (a.accumulate) *o += v;
-                        else *o = v;
+                for (int t = 0; t < NF; ++t) red[t][lane] = (T2 && h == 1) ? acc2[T2 ? u : 0][t] : acc[u][t];
+                __syncthreads();
+                const int tt = lane + h * nh;        // target of this sum within the pass
+                if (lane < nh && tt < nt && u < ns) {
+                    const int kk = t0 + base + tt;
+#pragma unroll
+                    for (int t = 0; t < NF; ++t) {
+                        double v = red[t][lane];
+                        for (int q = 1; q < r; ++q) v += red[t][lane + q * nh];
+                        if (pbase >= 0) {
+                            a.part[((int64_t)(set0 + u) * NF + t) * a.ldp + pbase + base + tt] = v;
+                        } else {
+                            double* o = out + ((int64_t)u * NF + t) * a.ldo + kk;
+                            if (a.accumulate) *o += v;
+                            else *o = v;
+                        }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
 }
@@ -546,9 +589,16 @@ static void launch_sets(const NearArgs& a, int nblocks, int part, hipStream_t s)
     if constexpr (!ROWS_OF_ONE) {
         if (a.nsets > 1) {
             constexpr int S = near_sets<M, F, DAMP>();
+            // two targets per lane: charges and dipoles (quadrupole sources spill at 3 waves per SIMD and ran
+            // slower, 38.0 -> 41.4 ms on 32 AMOEBA sets)
+            constexpr bool T2_OK = S == 4 && M <= 1 && F <= 1 && has_closed_form<M, F>();
             const dim3 g(nblocks, (a.nsets + S - 1) / S);
-            if (part == 1) hipLaunchKernelGGL((k_near<M, F, DAMP, S>), g, dim3(64), 0, s, a);
-            else hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S>), g, dim3(64), 0, s, a);
+            if (part == 1) {
+                if (T2_OK && a.pairs2) hipLaunchKernelGGL((k_near<M, F, DAMP, S, T2_OK>), g, dim3(64), 0, s, a);
+                else hipLaunchKernelGGL((k_near<M, F, DAMP, S>), g, dim3(64), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S>), g, dim3(64), 0, s, a);
+            }
         }
     }
 }
