@@ -749,6 +749,22 @@ __device__ __forceinline__ void dma16(const void* g, void* lds) {
     __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
 }
 
+// Target node of a workgroup: workgroup b runs on XCD b mod 8 (each XCD with its own L2), so XCD x
+// takes chunks x, x + 8, x + 16, ... of M2L_XCD_CHUNK consecutive target nodes.  Nodes close in node
+// order (depth-first over the Morton-sorted sites) share most of their M2L sources, which the
+// workgroups an XCD runs at once then read from its L2; with plain order each XCD's L2 would see
+// every eighth target and hold far fewer reused source blocks.  The grid is padded to whole rounds of
+// chunks (m2l_xcd_grid); workgroups past the last node return at once.
+constexpr int M2L_XCD_CHUNK = 16;
+__device__ __forceinline__ int m2l_xcd_target(int b) {
+    const int x = b % NUM_XCD, k = b / NUM_XCD;
+    return ((k / M2L_XCD_CHUNK) * NUM_XCD + x) * M2L_XCD_CHUNK + k % M2L_XCD_CHUNK;
+}
+static int m2l_xcd_grid(int n) {
+    const int r = NUM_XCD * M2L_XCD_CHUNK;
+    return (n + r - 1) / r * r;
+}
+
 template <int P, bool F32>
 constexpr int mfma_kpad() { return ((rdim(P) + 3) / 4 + 1) / 2 * 2; }   // k-steps over reduced components, even
 
@@ -790,7 +806,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(M2LArgs a) {
     __shared__ __align__(16) TE Tl[2][NTP];
     __shared__ __align__(16) TE Bl[2][BROWS];   // staged source block [s][set]
     __shared__ unsigned char degl[16 * MT]; // degree of target row t (99 past the reduced components)
-    const int i = blockIdx.x;
+    const int i = m2l_xcd_target(blockIdx.x);
+    if (i >= a.num_nodes) return;           // grid padding (the whole workgroup)
     const int sg = blockIdx.y * BW;         // first set of this workgroup
     const int tid = threadIdx.x;
     const int w = tid >> 6, lane = tid & 63;
@@ -1047,7 +1064,7 @@ template <int P, bool F32>
 static void launch_mfma_one(const M2LArgs& a, int num_nodes, int v, hipStream_t s) {
     const dim3 b(64 * mfma_waves<P, F32>());
     const int sets_per = (v & 1) ? 2 * M2L_SETS_TILE : M2L_SETS_TILE;
-    const dim3 g(num_nodes, (a.nsets + sets_per - 1) / sets_per);
+    const dim3 g(m2l_xcd_grid(num_nodes), (a.nsets + sets_per - 1) / sets_per);
     switch (v) {
         case 0: hipLaunchKernelGGL((k_m2l_mfma<P, F32, 1, true>), g, b, 0, s, a); break;
         case 1: hipLaunchKernelGGL((k_m2l_mfma<P, F32, 2, true>), g, b, 0, s, a); break;

@@ -177,7 +177,9 @@ __device__ double centroid_range(const double* __restrict__ col, int pos, int n,
                 __syncthreads();
                 if (tid == 0) {
                     // the dependent adds in order, each batch of 16 addends read from LDS while the previous
-                    // one is added (reads past blen stay inside xs, CB_LDS >= CB_BURST + 32, and are not added)
+                    // one is added (reads past blen stay inside xs, CB_LDS >= CB_BURST + 32, and are not added);
+                    // the scheduling barriers keep a batch's reads ahead of the other batch's adds, so that no
+                    // add waits for LDS (left to itself the compiler moves each read next to its add)
                     double t = s;
                     double a[16], b[16];
 #pragma unroll
@@ -186,12 +188,16 @@ __device__ double centroid_range(const double* __restrict__ col, int pos, int n,
                     for (; k + 32 <= blen; k += 32) {
 #pragma unroll
                         for (int q = 0; q < 16; ++q) b[q] = xs[k + 16 + q];
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int q = 0; q < 16; ++q) t = t + a[q];
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int q = 0; q < 16; ++q) a[q] = xs[k + 32 + q];
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int q = 0; q < 16; ++q) t = t + b[q];
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                     if (k + 16 <= blen) {
 #pragma unroll
