@@ -419,8 +419,14 @@ __device__ void pair_contract_rt(int M, int F, int KT, double x, double y, doubl
 // only; the direct sum looks through the block's whole source list.
 // M < 0: runtime orders a.m_rt, a.f_rt (S = 1).
 constexpr int NEAR_EXCL_STAGE = 512;
+// ALLROWS (S > 1): one workgroup takes every set row of its block.  The coincidence count and the list
+// search of each exclusion entry (chains of dependent loads) run once; the entries' additions are made
+// row by row by the same statements as with one row per workgroup, so each set's sums stay bit for bit
+// those of its single-set call.  Calls with more than EXCL_ROWS_MAX exclusion entries per site run one
+// workgroup per set row.
+constexpr int EXCL_ROWS_MAX = 64;
 
-template <int M, int F, int DAMP, int S>
+template <int M, int F, int DAMP, int S, bool ALLROWS = false>
 __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
     constexpr bool RT = M < 0;
     static_assert(!RT || S == 1, "runtime orders take one set per row");
@@ -440,8 +446,9 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
         if (threadIdx.x < 32) etab[threadIdx.x] = kExp2By32[threadIdx.x];
         __syncthreads();
     }
-    const int set0 = blockIdx.y * S;
-    const int ns = min(S, a.nsets - set0);
+    static_assert(!ALLROWS || (S > 1 && !RT), "all rows in one workgroup: sets per row, fixed orders");
+    int set0 = ALLROWS ? 0 : (int)blockIdx.y * S;
+    int ns = min(S, a.nsets - set0);
     const int t0 = a.tgt_start[b], tc = a.tgt_count[b];
     const int e0 = a.list_off[b], e1 = a.list_off[b + 1];
     const double* __restrict__ ms = a.ms + (int64_t)set0 * nm * a.ldm;
@@ -500,75 +507,92 @@ __global__ __launch_bounds__(64) void k_near_excl(NearArgs a) {
         }
         if (!act) continue;                        // after the barriers of this pass
         const double dfi = DAMP != DAMP_NONE ? a.dfs[k] : 0.0;
-        double acc[S][NF];
+        const int coinc_site = coinc;
+        unsigned long long nearmask = 0ull;        // ALLROWS: entries found in the list by row 0
+        const int nrow = ALLROWS ? (a.nsets + S - 1) / S : 1;
+        for (int row = 0; row < nrow; ++row) {
+            if constexpr (ALLROWS) {
+                set0 = row * S;
+                ns = min(S, a.nsets - set0);
+                ms = a.ms + (int64_t)set0 * nm * a.ldm;
+                out = a.out + (int64_t)set0 * nf * a.ldo;
+            }
+            coinc = coinc_site;
+            double acc[S][NF];
 #pragma unroll
-        for (int u = 0; u < S; ++u)
+            for (int u = 0; u < S; ++u)
 #pragma unroll
-            for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
-        auto add = [&](int j, double dx, double dy, double dz, double sign) {
-            double aij = 0.0;
-            if constexpr (DAMP != DAMP_NONE) aij = dfi * a.dfs[j];
-            const double R2 = dx * dx + dy * dy + dz * dz;
-            if constexpr (RT) {
-                if (ns < 1) return;
-                const int KT = (DAMP != DAMP_NONE && a.m_rt + a.f_rt > KMAX_NEAR) ? KMAX_NEAR : a.m_rt + a.f_rt;
-                double m[NM], g[NEAR_KMAX + 1];
-                for (int s = 0; s < nm; ++s) m[s] = sign * ms[(int64_t)s * a.ldm + j];
-                radial_rt<DAMP>(KT, R2, rsqrt_pos(R2), aij, g, etab);
-                pair_contract_rt(a.m_rt, a.f_rt, KT, dx, dy, dz, g, m, acc[0]);
-            } else {
+                for (int t = 0; t < NF; ++t) acc[u][t] = 0.0;
+            auto add = [&](int j, double dx, double dy, double dz, double sign) {
+                double aij = 0.0;
+                if constexpr (DAMP != DAMP_NONE) aij = dfi * a.dfs[j];
+                const double R2 = dx * dx + dy * dy + dz * dz;
+                if constexpr (RT) {
+                    if (ns < 1) return;
+                    const int KT = (DAMP != DAMP_NONE && a.m_rt + a.f_rt > KMAX_NEAR) ? KMAX_NEAR : a.m_rt + a.f_rt;
+                    double m[NM], g[NEAR_KMAX + 1];
+                    for (int s = 0; s < nm; ++s) m[s] = sign * ms[(int64_t)s * a.ldm + j];
+                    radial_rt<DAMP>(KT, R2, rsqrt_pos(R2), aij, g, etab);
+                    pair_contract_rt(a.m_rt, a.f_rt, KT, dx, dy, dz, g, m, acc[0]);
+                } else {
 #pragma unroll
-                for (int u = 0; u < S; ++u) {
-                    if (u >= ns) break;
-                    double m[NM];
+                    for (int u = 0; u < S; ++u) {
+                        if (u >= ns) break;
+                        double m[NM];
 #pragma unroll
-                    for (int s = 0; s < NM; ++s) m[s] = sign * ms[((int64_t)u * NM + s) * a.ldm + j];
-                    pair_field<M, F, DAMP>(dx, dy, dz, R2, aij, m, acc[u], etab);
+                        for (int s = 0; s < NM; ++s) m[s] = sign * ms[((int64_t)u * NM + s) * a.ldm + j];
+                        pair_field<M, F, DAMP>(dx, dy, dz, R2, aij, m, acc[u], etab);
+                    }
                 }
-            }
-        };
-        bool self = false;
-        for (int q = 0; q < a.nexcl; ++q) {
-            const int e = a.excl[(int64_t)q * a.lde + k];
-            if (e <= 0) continue;
-            if (e > a.nsites) {                    // not a site: reported to the caller, entry skipped
-                if (a.bad) *a.bad = 1;
-                continue;
-            }
-            bool dup = false;
-            for (int q2 = 0; q2 < q; ++q2) dup |= a.excl[(int64_t)q2 * a.lde + k] == e;
-            if (dup) continue;
-            if (e == myo) {
-                self = true;
-                add(k, 1e10, 1e10, 1e10, 1.0);
-                continue;
-            }
-            const int j = a.inv ? a.inv[e - 1] : e - 1;   // sorted position of the excluded site
-            bool near = false;
-            if (staged) {
-                for (int x = 0; x < e1 - e0 && !near; ++x) near = j >= rfirst[x] && j < rend[x];
-            } else {
-                for (int x = e0; x < e1 && !near; ++x) {
-                    const int enc = a.list_src[x];
-                    if (src_image(enc) != 0) continue;
-                    const int sb = src_node(enc);
-                    const int st = a.src_start[sb];
-                    near = j >= st && j < st + a.src_count[sb];
+            };
+            bool self = false;
+            for (int q = 0; q < a.nexcl; ++q) {
+                const int e = a.excl[(int64_t)q * a.lde + k];
+                if (e <= 0) continue;
+                if (e > a.nsites) {                    // not a site: reported to the caller, entry skipped
+                    if (a.bad) *a.bad = 1;
+                    continue;
                 }
+                bool dup = false;
+                for (int q2 = 0; q2 < q; ++q2) dup |= a.excl[(int64_t)q2 * a.lde + k] == e;
+                if (dup) continue;
+                if (e == myo) {
+                    self = true;
+                    add(k, 1e10, 1e10, 1e10, 1.0);
+                    continue;
+                }
+                const int j = a.inv ? a.inv[e - 1] : e - 1;   // sorted position of the excluded site
+                bool near = false;
+                if (ALLROWS && row > 0) {
+                    near = (nearmask >> q) & 1ull;
+                } else {
+                    if (staged) {
+                        for (int x = 0; x < e1 - e0 && !near; ++x) near = j >= rfirst[x] && j < rend[x];
+                    } else {
+                        for (int x = e0; x < e1 && !near; ++x) {
+                            const int enc = a.list_src[x];
+                            if (src_image(enc) != 0) continue;
+                            const int sb = src_node(enc);
+                            const int st = a.src_start[sb];
+                            near = j >= st && j < st + a.src_count[sb];
+                        }
+                    }
+                    if (ALLROWS && near) nearmask |= 1ull << q;
+                }
+                if (!near) continue;
+                add(j, 1e10, 1e10, 1e10, 1.0);
+                const double dx = xi - a.xs[j], dy = yi - a.ys[j], dz = zi - a.zs[j];
+                if (dx == 0.0 && dy == 0.0 && dz == 0.0) --coinc;   // excluded: no R = 0 pair to remove
+                else add(j, dx, dy, dz, -1.0);
             }
-            if (!near) continue;
-            add(j, 1e10, 1e10, 1e10, 1.0);
-            const double dx = xi - a.xs[j], dy = yi - a.ys[j], dz = zi - a.zs[j];
-            if (dx == 0.0 && dy == 0.0 && dz == 0.0) --coinc;   // excluded: no R = 0 pair to remove
-            else add(j, dx, dy, dz, -1.0);
-        }
-        // the reference's pair at R = 0: non-finite
-        if (!self || coinc > 0) add(k, 0.0, 0.0, 0.0, 1.0);
+            // the reference's pair at R = 0: non-finite
+            if (!self || coinc > 0) add(k, 0.0, 0.0, 0.0, 1.0);
 #pragma unroll
-        for (int u = 0; u < S; ++u) {
-            if (u >= ns) break;
-            for (int t = 0; t < nf; ++t) out[((int64_t)u * nf + t) * a.ldo + k] += acc[u][t];
-        }
+            for (int u = 0; u < S; ++u) {
+                if (u >= ns) break;
+                for (int t = 0; t < nf; ++t) out[((int64_t)u * nf + t) * a.ldo + k] += acc[u][t];
+            }
+        }   // rows
     }
 }
 
@@ -596,6 +620,8 @@ static void launch_sets(const NearArgs& a, int nblocks, int part, hipStream_t s)
             if (part == 1) {
                 if (T2_OK && a.pairs2) hipLaunchKernelGGL((k_near<M, F, DAMP, S, T2_OK>), g, dim3(64), 0, s, a);
                 else hipLaunchKernelGGL((k_near<M, F, DAMP, S>), g, dim3(64), 0, s, a);
+            } else if (a.nexcl <= EXCL_ROWS_MAX) {
+                hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S, true>), dim3(nblocks), dim3(64), 0, s, a);
             } else {
                 hipLaunchKernelGGL((k_near_excl<M, F, DAMP, S>), g, dim3(64), 0, s, a);
             }
