@@ -53,6 +53,7 @@ struct Tables {
     const int* gemm_fidx = nullptr;   // class-grouped / block M2L: A-fragment index table over the reduced
     const int* gemm_tdeg = nullptr;   //   (harmonic) components, |t| per reduced target row,
     const int* gemm_tfull = nullptr;  //   and the packed index of each reduced component
+    FoldTab fold{};                   // many-sets source fold, term by term (null off: p beyond the table)
 };
 
 struct TreeOut {
@@ -419,7 +420,8 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     // Class-grouped / block M2L fragment tables over the reduced (harmonic) components -- those with
     // z <= 1, by degree, then w = y + z, then z (m2l.hip): lane l of the A fragment of row tile mt and
     // source step ks holds H[t][s], t = 16 mt + l % 16, s = 4 ks + l / 16 (reduced indices).
-    std::vector<int> gemm_fidx, gemm_tdeg, gemm_tfull;
+    std::vector<int> gemm_fidx, gemm_tdeg, gemm_tfull, fold_off, fold_sc, fold_deg;
+    std::vector<double> fold_w, fold_c;
     if (p <= M2L_GEMM_PMAX) {
         const int nk = (p + 1) * (p + 1);
         std::vector<XYZ> red;
@@ -446,6 +448,26 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
             gemm_tdeg[t] = red[t].x + red[t].y + red[t].z;
             gemm_tfull[t] = idx(red[t].x, red[t].y, red[t].z);
         }
+        // the fold of c M onto reduced component (X, Y, rho): terms (-1)^k C(k, i) c M(X - 2i, Y - 2(k - i),
+        // rho + 2k), k from 0, i from 0, those with both indices >= 0 (the order of the device fold)
+        fold_off.assign(1, 0);
+        for (int t = 0; t < nk; ++t) {
+            const int X = red[t].x, Y = red[t].y, rho = red[t].z;
+            for (int k = 0; 2 * k <= X + Y; ++k) {
+                double binom = 1.0;
+                for (int i = 0; i <= k; ++i) {
+                    if (i > 0) binom = binom * (k - i + 1) / i;
+                    const int sx = X - 2 * i, sy = Y - 2 * (k - i);
+                    if (sx < 0 || sy < 0) continue;
+                    const int sc = idx(sx, sy, rho + 2 * k);
+                    fold_sc.push_back(sc);
+                    fold_w.push_back(((k & 1) ? -1.0 : 1.0) * binom);
+                    fold_c.push_back(cs[sc]);
+                }
+            }
+            fold_off.push_back((int)fold_sc.size());
+            fold_deg.push_back(X + Y + rho);
+        }
     }
 
     // pack into one int and one double buffer
@@ -454,8 +476,8 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
     t.p = p; t.m = m; t.nsrc = ncomp(msrc);
     std::vector<std::vector<int>*> ilist = {&p2m_off, &p2m_s, &p2m_e, &m2m_off, &m2m_s, &m2m_e, &l2l_off, &l2l_s,
                                             &l2l_e, &mono_parent, &mono_axis, &xyz, &level_base, &gemm_fidx,
-                                            &gemm_tdeg, &gemm_tfull};
-    std::vector<std::vector<double>*> dlist = {&p2m_c, &m2m_c, &l2l_c, &cs, &scale};
+                                            &gemm_tdeg, &gemm_tfull, &fold_off, &fold_sc, &fold_deg};
+    std::vector<std::vector<double>*> dlist = {&p2m_c, &m2m_c, &l2l_c, &cs, &scale, &fold_w, &fold_c};
     std::vector<size_t> ioff, doff;
     size_t ni = 0, nd = 0;
     for (auto* v : ilist) { ioff.push_back(ni); ni += v->size(); }
@@ -481,6 +503,7 @@ Tables& get_tables(fmmh_context* c, int p, int m) {
         t.gemm_fidx = I + ioff[13];
         t.gemm_tdeg = I + ioff[14];
         t.gemm_tfull = I + ioff[15];
+        t.fold = FoldTab{I + ioff[16], I + ioff[17], I + ioff[18], D + doff[5], D + doff[6]};
     }
     t.cs = D + doff[3];
     t.scale = D + doff[4];
@@ -1594,7 +1617,8 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         const size_t red_stride = (size_t)nn * rdim(p);
         double* CMs = c->buf<double>("m_cms", red_stride * nsets + M2LB_READ_SLACK);
         for (int set = 0; set < nsets; ++set)
-            launch_scale_sources_red(CM + set * node_stride, tb.cs, tr.t.r, CMs + set * red_stride, nn, p, 1, 1, s);
+            launch_scale_sources_red(CM + set * node_stride, tb.cs, tb.fold, tr.t.r, CMs + set * red_stride, nn, p, 1, 1,
+                                     s);
         M2LBRunArgs ra{ba.toff, ba.pbase, ba.rec, nbm, ba.ntg, ba.tperm, ba.tinv, nn, nullptr, Tc, tb.gemm_fidx,
                        tb.gemm_tdeg, tb.gemm_tfull,
                        c->buf<double>("b_partial", (size_t)(nbm + grid) * rdim(p) * M2LB_BT), grid};
@@ -1663,7 +1687,8 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
         float* cms32 = f32 ? c->buf<float>("m_cms32", red_stride) : nullptr;
         double* CMs = c->buf<double>("m_cms", red_stride * nsets);
         for (int set = 0; set < nsets; ++set)
-            launch_scale_sources_red(CM + set * node_stride, tb.cs, tr.t.r, CMs + set * red_stride, nn, p, 1, 1, s);
+            launch_scale_sources_red(CM + set * node_stride, tb.cs, tb.fold, tr.t.r, CMs + set * red_stride, nn, p, 1, 1,
+                                     s);
         for (int set = 0; set < nsets; ++set) {
             if (f32) launch_to_f32(CMs + set * red_stride, cms32, (int64_t)red_stride, s);
             M2LGemmArgs ga{class_off, tile_off, ngrp, pair_csr, L.m2l_src, class_dl, Tc, CMs + set * red_stride,
@@ -1691,7 +1716,7 @@ void run_m2l(fmmh_context* c, TreeOut& tr, ListsOut& L, Tables& tb, int p, int n
     for (int c0 = 0; c0 < nsets; c0 += chunk) {
         const int ns = std::min(chunk, nsets - c0);
         const int s_ld = mfma ? pad_sets(ns) : 0;
-        if (mfma) launch_scale_sources_red(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, s_ld, s);
+        if (mfma) launch_scale_sources_red(CM + c0 * node_stride, tb.cs, tb.fold, tr.t.r, CMs, nn, p, ns, s_ld, s);
         else launch_scale_sources(CM + c0 * node_stride, tb.cs, tr.t.r, CMs, nn, p, ns, 0, s);
         if (f32) launch_to_f32(CMs, CMs32, (int64_t)nn * (p + 1) * (p + 1) * s_ld, s);
         M2LArgs ma{L.m2l_off, L.m2l_src, pair_class, Tc, CMs, LE + c0 * node_stride, tr.t.level, tr.t.r,

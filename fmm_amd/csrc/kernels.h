@@ -276,10 +276,18 @@ int m2l_mfma_class_stride(int p, bool f32);
 void launch_m2l_class_pad(const double* Tc, void* Tcp, int ncls, int p, bool f32, hipStream_t s);
 void launch_scale_sources(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
                           int nsets, int s_ld_out, hipStream_t s);
+// Fold of c M onto the reduced components, term by term (built on the host in the order the device fold
+// visits them): reduced component sr takes terms [off[sr], off[sr + 1]), each w (c M)(sc) with w the signed
+// binomial and c the Taylor factor of sc; deg[sr] its degree.
+struct FoldTab {
+    const int* off; const int* sc; const int* deg;
+    const double* w; const double* c;
+};
 // Many-sets M2L in harmonic (traceless) form: folded sources [node][(p + 1)^2][s_ld] and the
 // local-expansion components with t_z >= 2 rebuilt from the others after it (rows = sets x nodes).
-void launch_scale_sources_red(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
-                              int nsets, int s_ld, hipStream_t s);
+// One set (nsets = s_ld = 1) folds with cs; several sets with the term table ft.
+void launch_scale_sources_red(const double* CM, const double* cs, const FoldTab& ft, const double* r, double* CMs,
+                              int num_nodes, int p, int nsets, int s_ld, hipStream_t s);
 void launch_le_unfold(double* LE, long long rows, int p, hipStream_t s);
 void launch_m2l(const M2LArgs& a, int num_nodes, hipStream_t s);
 // MFMA form for many sets: one wavefront per (target, 16 sets) accumulates

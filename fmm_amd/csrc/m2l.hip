@@ -622,10 +622,13 @@ __global__ __launch_bounds__(256) void k_scale_sources_red1(const double* __rest
     CMs[x] = acc * f;
 }
 
-__global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restrict__ CM, const double* __restrict__ cs,
+// Several sets: the terms come from the host-built table (those fold_component visits, in its order, with
+// the same products), the powers of 1 / r_j from LDS (the same sequence of products as one set's).
+__global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restrict__ CM, FoldTab ft,
                                                            const double* __restrict__ r, double* __restrict__ CMs,
                                                            int num_nodes, int p, int nsets, int s_ld) {
     __shared__ double cm[FOLD_SETS][MAX_NM_FOLD];
+    __shared__ double pw[PMAX + 1];
     const int nr = rdim(p), nm = ncomp(p);
     const int j = blockIdx.x, set0 = blockIdx.y * FOLD_SETS;
     const int nsl = min(FOLD_SETS, s_ld - set0);   // output columns of this workgroup
@@ -633,18 +636,22 @@ __global__ __launch_bounds__(256) void k_scale_sources_red(const double* __restr
         const int u = x / nm, sc = x - u * nm;
         cm[u][sc] = set0 + u < nsets ? CM[((int64_t)(set0 + u) * num_nodes + j) * nm + sc] : 0.0;
     }
+    if (threadIdx.x == 0) {
+        const double rinv = 1.0 / r[j];
+        double f = 1.0;
+        for (int q = 0; q <= p; ++q) {
+            pw[q] = f;
+            f *= rinv;
+        }
+    }
     __syncthreads();
-    const double rinv = 1.0 / r[j];
     for (int x = threadIdx.x; x < nr * nsl; x += blockDim.x) {
         const int sr = x / nsl, u = x - sr * nsl;
         double out = 0.0;
         if (set0 + u < nsets) {
-            const int d = decode_reduced(sr);
-            const int deg = d & 255, w = (d >> 8) & 255, rho = d >> 16;
-            const double acc = fold_component(deg - w, w - rho, rho, [&](int sc) { return cs[sc] * cm[u][sc]; });
-            double f = 1.0;
-            for (int q = 0; q < deg; ++q) f *= rinv;
-            out = acc * f;
+            double acc = 0.0;
+            for (int t = ft.off[sr]; t < ft.off[sr + 1]; ++t) acc += ft.w[t] * (ft.c[t] * cm[u][ft.sc[t]]);
+            out = acc * pw[ft.deg[sr]];
         }
         CMs[((int64_t)j * nr + sr) * s_ld + set0 + u] = out;
     }
@@ -677,16 +684,17 @@ __global__ __launch_bounds__(256) void k_le_unfold(double* __restrict__ LE, int6
     }
 }
 
-void launch_scale_sources_red(const double* CM, const double* cs, const double* r, double* CMs, int num_nodes, int p,
-                              int nsets, int s_ld, hipStream_t s) {
+void launch_scale_sources_red(const double* CM, const double* cs, const FoldTab& ft, const double* r, double* CMs,
+                              int num_nodes, int p, int nsets, int s_ld, hipStream_t s) {
     if (num_nodes <= 0) return;
     if (nsets == 1 && s_ld == 1) {
         const int64_t tot = (int64_t)num_nodes * rdim(p);
         hipLaunchKernelGGL(k_scale_sources_red1, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, CM, cs, r, CMs,
                            num_nodes, p);
     } else {
+        if (!ft.off) throw Error(FMMH_ERR_INTERNAL, "fold table missing");
         hipLaunchKernelGGL(k_scale_sources_red, dim3(num_nodes, (s_ld + FOLD_SETS - 1) / FOLD_SETS), dim3(256), 0, s,
-                           CM, cs, r, CMs, num_nodes, p, nsets, s_ld);
+                           CM, ft, r, CMs, num_nodes, p, nsets, s_ld);
     }
     FMMH_HIP(hipGetLastError());
 }
