@@ -38,8 +38,8 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_near_water200k_thole_p6.md, valid for the near-field sources it
 # was measured on: K_NEAR_TRAFFIC_SRC is the sha256 prefix of those sources
 # (near_src_sha()); with other sources the bench line reports null.
-K_NEAR_TRAFFIC_BYTES = 148331520     # 61.7 MB read + 86.7 MB written
-K_NEAR_TRAFFIC_SRC = "b2c7a644cc4a5c64"
+K_NEAR_TRAFFIC_BYTES = 161808384     # 76.9 MB read + 84.9 MB written
+K_NEAR_TRAFFIC_SRC = "fcf67a6ee15079f2"
 NEAR_SOURCES = ("fmm_amd/csrc/near_kernels.h", "fmm_amd/csrc/near.hip", "fmm_amd/csrc/tensors.h")
 
 
