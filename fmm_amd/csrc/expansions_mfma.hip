@@ -39,22 +39,25 @@ __device__ __forceinline__ void powers(double d, int p, double* w) {
 }
 
 // NS source components (a.nsrc); LDS: monomials of a chunk of sites, the B operand and the (t, s) table.
-template <int P, int NS>
+// NTL 16-set tiles per workgroup: each gathered A element feeds NTL MFMAs (each tile's sums are those of a
+// workgroup of one tile).
+template <int P, int NS, int NTL>
 __global__ __launch_bounds__(64 * XM_W) void k_p2m_mfma(UpArgs a) {
     constexpr int NM = ncomp(P);
     constexpr int MT = (NM + 15) / 16;
     constexpr int MW = (MT + XM_W - 1) / XM_W;
     constexpr int nsrc = NS;
     constexpr int kmax = (XM_CH * NS + 3) / 4 * 4;
+    constexpr int BW = 16 * NTL;            // sets per workgroup
     __shared__ double mono[XM_CH * NM];
-    __shared__ double B[kmax * 16];
+    __shared__ double B[kmax * BW];
     __shared__ double tabC[NM * NS];
     __shared__ int tabE[NM * NS];
     __shared__ double pw[XM_CH][3][XM_PMAX + 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int row = lane & 15, kq = lane >> 4;
     const int node = a.leaf_node[blockIdx.x];
-    const int set0 = blockIdx.y * 16;
+    const int set0 = blockIdx.y * BW;
     const int s0 = a.start[node], c = a.count[node];
     const double cx = a.cx[node], cy = a.cy[node], cz = a.cz[node];
     for (int x = tid; x < NM * nsrc; x += 64 * XM_W) {
@@ -68,9 +71,11 @@ __global__ __launch_bounds__(64 * XM_W) void k_p2m_mfma(UpArgs a) {
             tabE[t * nsrc + sc] = a.p2m.e[b];
             tabC[t * nsrc + sc] = a.p2m.coef[b];
         }
-    v4d acc[MW];
+    v4d acc[MW][NTL];
 #pragma unroll
-    for (int mi = 0; mi < MW; ++mi) acc[mi] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = v4d{0.0, 0.0, 0.0, 0.0};
     for (int j0 = 0; j0 < c; j0 += XM_CH) {
         const int cc = min(XM_CH, c - j0);
         __syncthreads();   // readers of the previous chunk are done (and the tables are in place)
@@ -80,8 +85,8 @@ __global__ __launch_bounds__(64 * XM_W) void k_p2m_mfma(UpArgs a) {
             const double d = ax == 0 ? a.xs[j] - cx : (ax == 1 ? a.ys[j] - cy : a.zs[j] - cz);
             powers(d, P, pw[jj][ax]);
         }
-        for (int x = tid; x < kmax * 16; x += 64 * XM_W) {
-            const int k = x >> 4, u = x & 15;
+        for (int x = tid; x < kmax * BW; x += 64 * XM_W) {
+            const int k = x / BW, u = x - k * BW;
             const int jj = k / nsrc, sc = k - jj * nsrc;
             B[x] = (jj < cc && set0 + u < a.nsets)
                        ? a.ms_raw[((int64_t)(set0 + u) * a.dim + sc) * a.ldm + s0 + j0 + jj] : 0.0;
@@ -98,7 +103,9 @@ __global__ __launch_bounds__(64 * XM_W) void k_p2m_mfma(UpArgs a) {
             const int k = 4 * ks + kq;
             const int jj = k / nsrc, sc = k - jj * nsrc;
             const bool kv = jj < cc;
-            const double bv = B[k * 16 + row];
+            double bv[NTL];
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) bv[nt] = B[k * BW + 16 * nt + row];
 #pragma unroll
             for (int mi = 0; mi < MW; ++mi) {
                 const int t = (w + XM_W * mi) * 16 + row;
@@ -107,20 +114,25 @@ __global__ __launch_bounds__(64 * XM_W) void k_p2m_mfma(UpArgs a) {
                     const int e = tabE[t * nsrc + sc];
                     if (e >= 0) av = tabC[t * nsrc + sc] * mono[jj * NM + e];
                 }
-                acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[mi], 0, 0, 0);
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt)
+                    acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[nt], acc[mi][nt], 0, 0, 0);
             }
         }
     }
     // accumulator register q of lane l holds D[coefficient 4 q + l / 16][set l % 16]
-    const int set = set0 + row;
-    if (set >= a.nsets) return;
 #pragma unroll
-    for (int mi = 0; mi < MW; ++mi)
+    for (int nt = 0; nt < NTL; ++nt) {
+        const int set = set0 + 16 * nt + row;
+        if (set >= a.nsets) continue;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int t = (w + XM_W * mi) * 16 + 4 * q + kq;
-            if (t < NM) a.CM[((int64_t)set * a.num_nodes + node) * NM + t] = acc[mi][q];
-        }
+        for (int mi = 0; mi < MW; ++mi)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = (w + XM_W * mi) * 16 + 4 * q + kq;
+                if (t < NM) a.CM[((int64_t)set * a.num_nodes + node) * NM + t] = acc[mi][nt][q];
+            }
+    }
 }
 
 // Harmonic form.  The local expansion of the far field is the Taylor expansion of a harmonic function,
@@ -285,14 +297,21 @@ __global__ __launch_bounds__(64 * XL_W) void k_l2p_mfma(DownArgs a) {
     }
 }
 
+template <int P, int NTL>
+void p2m_pt(const UpArgs& a, int num_leaves, hipStream_t s) {
+    const dim3 g(num_leaves, (a.nsets + 16 * NTL - 1) / (16 * NTL)), b(64 * XM_W);
+    switch (a.nsrc) {
+        case 1: hipLaunchKernelGGL((k_p2m_mfma<P, 1, NTL>), g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_p2m_mfma<P, 4, NTL>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_p2m_mfma<P, 10, NTL>), g, b, 0, s, a); break;
+    }
+}
+
+// two 16-set tiles per workgroup from 17 sets on (the gathered A element feeds both)
 template <int P>
 void p2m_p(const UpArgs& a, int num_leaves, hipStream_t s) {
-    const dim3 g(num_leaves, (a.nsets + 15) / 16), b(64 * XM_W);
-    switch (a.nsrc) {
-        case 1: hipLaunchKernelGGL((k_p2m_mfma<P, 1>), g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_p2m_mfma<P, 4>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_p2m_mfma<P, 10>), g, b, 0, s, a); break;
-    }
+    if (a.nsets > 16) p2m_pt<P, 2>(a, num_leaves, s);
+    else p2m_pt<P, 1>(a, num_leaves, s);
 }
 
 template <int P, int F>
